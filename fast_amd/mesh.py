@@ -1,0 +1,303 @@
+"""Mesh data model and seeded generators for the element/assembly/solve path.
+
+The data model mirrors what FAST's element plugins read from a neutral file:
+
+* nodes (control points) with coordinates and supports
+  (node/node.h:295-369, ``%NODE.COORD`` / ``%NODE.SUPPORT``);
+* one element family per mesh: 8-node hexahedra (``%ELEMENT.BRICK8[TL]``,
+  elc/element.cpp:1985-2041) or B-spline solids over one patch
+  (``%ELEMENT.BSPLINE.SOLID[TL]``, elc/elmparam.cpp:152-190);
+* per-element isotropic material (E, nu) -- or (mu, lambda) for the
+  neo-Hookean total-Lagrangian form, which the reference does not have;
+* the equation map ``dofmap[node, dir]`` with the semantics of
+  ``cControl::InitDofs`` / ``CompEqs`` (ctrl/ctrl.cpp:2004-2104): a direction is
+  active when an element uses the node, supports deactivate it, and active
+  directions are numbered 1.. in node order; 0 marks a fixed direction.
+
+Node order of a hex8 follows shp/shpsolid.cpp:173-184 (t = +1 face first).
+Control points of a B-spline element follow shp/shpbsp.cpp:873-902
+(k(t) slowest, i(r) fastest) and patch indexing geom/patch.cpp:1221-1231.
+"""
+from __future__ import annotations
+
+import dataclasses
+from typing import Optional, Sequence
+
+import numpy as np
+
+FORMS = {"linear": 0, "tl_svk": 1, "tl_neohooke": 2}
+SHAPES = {"hex8": 0, "bsp": 1}
+
+
+@dataclasses.dataclass
+class Mesh:
+    coords: np.ndarray                 # (nnode, 3) float64
+    conn: np.ndarray                   # (nelem, nn) int32, 0-based
+    shape: str                         # "hex8" | "bsp"
+    nip: tuple                         # Gauss points per direction
+    form: str                          # "linear" | "tl_svk" | "tl_neohooke"
+    mat: np.ndarray                    # (nelem, 2) float64
+    fixed: np.ndarray                  # (nnode, 3) bool
+    deg: tuple = (1, 1, 1)
+    ext: Optional[np.ndarray] = None   # (nelem, 3, p+1, p+1) float64
+    cpw: Optional[np.ndarray] = None   # (nnode,) float64
+    span: Optional[np.ndarray] = None  # (nelem, 3) int32 knot-span ids
+    knots: Optional[list] = None       # per-direction knot vectors
+    nbas: Optional[tuple] = None       # basis functions per direction
+    fext: Optional[np.ndarray] = None  # (nnode, 3) reference load vector
+    grid: Optional[tuple] = None       # structured element counts
+    dofmap: Optional[np.ndarray] = None
+
+    def __post_init__(self):
+        self.coords = np.ascontiguousarray(self.coords, dtype=np.float64)
+        self.conn = np.ascontiguousarray(self.conn, dtype=np.int32)
+        self.mat = np.ascontiguousarray(self.mat, dtype=np.float64)
+        self.fixed = np.ascontiguousarray(self.fixed, dtype=bool)
+        if self.dofmap is None:
+            self.dofmap = number_dofs(self.conn, self.fixed, len(self.coords))
+
+    @property
+    def nnode(self) -> int:
+        return int(self.coords.shape[0])
+
+    @property
+    def nelem(self) -> int:
+        return int(self.conn.shape[0])
+
+    @property
+    def nn(self) -> int:
+        return int(self.conn.shape[1])
+
+    @property
+    def neq(self) -> int:
+        return int(self.dofmap.max(initial=0))
+
+    def load_vector(self, scale: float = 1.0) -> np.ndarray:
+        """Global external load vector (equation order) from nodal loads."""
+        f = np.zeros(self.neq)
+        if self.fext is None:
+            return f
+        d = self.dofmap.ravel()
+        m = d > 0
+        np.add.at(f, d[m] - 1, scale * self.fext.ravel()[m])
+        return f
+
+    def scatter(self, u_eq: np.ndarray) -> np.ndarray:
+        """Equation vector -> nodal (nnode, 3) field (ctrl.cpp:1094-1113)."""
+        out = np.zeros((self.nnode, 3))
+        d = self.dofmap
+        m = d > 0
+        out[m] = np.asarray(u_eq)[d[m] - 1]
+        return out
+
+    def gather(self, u_nodal: np.ndarray) -> np.ndarray:
+        u = np.zeros(self.neq)
+        d = self.dofmap
+        m = d > 0
+        u[d[m] - 1] = np.asarray(u_nodal)[m]
+        return u
+
+
+def number_dofs(conn: np.ndarray, fixed: np.ndarray, nnode: int) -> np.ndarray:
+    """Equation numbering of ctrl/ctrl.cpp:2004-2104 (InitDofs + CompEqs)."""
+    active = np.zeros((nnode, 3), dtype=bool)
+    active[np.unique(conn.ravel())] = True
+    active &= ~fixed
+    dofmap = np.zeros((nnode, 3), dtype=np.int32)
+    flat = active.ravel()
+    dofmap.ravel()[flat] = np.arange(1, int(flat.sum()) + 1, dtype=np.int32)
+    return dofmap
+
+
+# ---------------------------------------------------------------- hex8 ----
+
+def hex8_block(nx: int, ny: int, nz: int, size=(1.0, 1.0, 1.0), E=1.0,
+               nu=0.3, form="linear", nip=2, perturb=0.0, seed=0,
+               clamp_x0=True, traction=None, mat=None) -> Mesh:
+    """Structured block of nx*ny*nz 8-node bricks, nodes numbered x fastest.
+
+    ``perturb`` moves interior nodes by U(-perturb, perturb) * h per
+    direction.  ``traction`` (3-vector) is a uniform surface traction on
+    the face x = size[0], integrated exactly for flat faces (q*A/4 per
+    corner, what a 2x2 Gauss face rule gives on a parallelogram face).
+    """
+    lx, ly, lz = size
+    xs = np.linspace(0.0, lx, nx + 1)
+    ys = np.linspace(0.0, ly, ny + 1)
+    zs = np.linspace(0.0, lz, nz + 1)
+    Z, Y, X = np.meshgrid(zs, ys, xs, indexing="ij")
+    coords = np.stack([X.ravel(), Y.ravel(), Z.ravel()], axis=1)
+    if perturb:
+        rng = np.random.default_rng(seed)
+        h = np.array([lx / nx, ly / ny, lz / nz])
+        i = np.tile(np.arange(nx + 1), (nz + 1) * (ny + 1))
+        j = np.tile(np.repeat(np.arange(ny + 1), nx + 1), nz + 1)
+        k = np.repeat(np.arange(nz + 1), (nx + 1) * (ny + 1))
+        inner = ((i > 0) & (i < nx) & (j > 0) & (j < ny) & (k > 0) & (k < nz))
+        d = rng.uniform(-perturb, perturb, size=coords.shape) * h
+        coords[inner] += d[inner]
+
+    def nid(i, j, k):
+        return i + (nx + 1) * (j + (ny + 1) * k)
+
+    ei, ej, ek = np.meshgrid(np.arange(nx), np.arange(ny), np.arange(nz),
+                             indexing="ij")
+    ei = ei.transpose(2, 1, 0).ravel()
+    ej = ej.transpose(2, 1, 0).ravel()
+    ek = ek.transpose(2, 1, 0).ravel()
+    conn = np.stack([
+        nid(ei, ej, ek + 1), nid(ei + 1, ej, ek + 1),
+        nid(ei + 1, ej + 1, ek + 1), nid(ei, ej + 1, ek + 1),
+        nid(ei, ej, ek), nid(ei + 1, ej, ek),
+        nid(ei + 1, ej + 1, ek), nid(ei, ej + 1, ek)], axis=1).astype(np.int32)
+    nnode = coords.shape[0]
+    fixed = np.zeros((nnode, 3), dtype=bool)
+    if clamp_x0:
+        fixed[np.isclose(coords[:, 0], 0.0)] = True
+    if mat is None:
+        mat = np.tile([E, nu], (conn.shape[0], 1))
+    fext = None
+    if traction is not None:
+        fext = np.zeros((nnode, 3))
+        t = np.asarray(traction, dtype=np.float64)
+        area = (ly / ny) * (lz / nz)
+        for k in range(nz):
+            for j in range(ny):
+                for (jj, kk) in ((j, k), (j + 1, k), (j + 1, k + 1), (j, k + 1)):
+                    fext[nid(nx, jj, kk)] += t * area / 4.0
+    return Mesh(coords=coords, conn=conn, shape="hex8", nip=(nip, nip, nip),
+                form=form, mat=mat, fixed=fixed, fext=fext, grid=(nx, ny, nz))
+
+
+# ---------------------------------------------------------------- B-spline --
+
+def open_uniform_knots(p: int, nspan: int) -> np.ndarray:
+    inner = np.arange(1, nspan) / nspan
+    return np.concatenate([np.zeros(p + 1), inner, np.ones(p + 1)])
+
+
+def bernstein(p: int, u: float) -> np.ndarray:
+    B = np.zeros(p + 1)
+    B[0] = 1.0
+    for j in range(1, p + 1):
+        saved = 0.0
+        for k in range(j):
+            temp = B[k]
+            B[k] = saved + (1.0 - u) * temp
+            saved = u * temp
+        B[j] = saved
+    return B
+
+
+def bezier_extraction(p: int, U: Sequence[float]) -> np.ndarray:
+    """Bezier extraction operators, one (p+1)x(p+1) matrix per span.
+
+    Algorithm of Borden, Scott, Evans & Hughes (2011), the one used by the
+    reference at geom/knotvec.cpp:415-493: successive knot insertion, rows =
+    local B-spline functions, columns = Bernstein polynomials of the span.
+    """
+    U = np.asarray(U, dtype=np.float64)
+    m = len(U)
+    ns = int(np.sum(np.abs(np.diff(U)) > 1e-6))
+    q = p + 1
+    C = np.zeros((ns, q, q))
+    C[0] = np.eye(q)
+    a, b, nb = p + 1, p + 2, 1
+    while b < m:
+        if nb < ns:
+            C[nb] = np.eye(q)
+        i = b
+        while b < m and U[b] == U[b - 1]:
+            b += 1
+        mult = b - i + 1
+        if mult < p:
+            numer = U[b - 1] - U[a - 1]
+            alphas = np.zeros(p)
+            for j in range(p, mult, -1):
+                alphas[j - mult - 1] = numer / (U[a + j - 1] - U[a - 1])
+            r = p - mult
+            for j in range(1, r + 1):
+                save = r - j + 1
+                s = mult + j
+                for k in range(p + 1, s, -1):
+                    al = alphas[k - s - 1]
+                    C[nb - 1][:, k - 1] = (al * C[nb - 1][:, k - 1]
+                                           + (1.0 - al) * C[nb - 1][:, k - 2])
+                if b < m and nb < ns:
+                    for w in range(save, j + save + 1):
+                        v = p - j + 1 + (w - save)
+                        C[nb][w - 1, save - 1] = C[nb - 1][v - 1, p]
+        nb += 1
+        if b < m:
+            a = b
+            b += 1
+    return C
+
+
+def greville(p: int, U: np.ndarray) -> np.ndarray:
+    n = len(U) - p - 1
+    return np.array([U[i + 1:i + p + 1].mean() for i in range(n)])
+
+
+def bsp_block(ne=(32, 32, 8), p=3, size=(1.0, 1.0, 1.0), E=1.0, nu=0.3,
+              form="linear", nip=None, perturb=0.05, seed=0,
+              clamp_x0=True, mat=None) -> Mesh:
+    """Single B-spline solid patch over a box, open uniform knots.
+
+    Control points sit at the Greville abscissae of the box, interior ones
+    moved by U(-perturb, perturb) * h_dir (h_dir = size/ne).  Elements are
+    ordered with the r-span fastest; each element's control points follow the
+    reference order (k slowest, i fastest).  All weights are 1 and the patch
+    is rational, the case the reference evaluates correctly.
+    """
+    if nip is None:
+        nip = p + 1
+    deg = (p, p, p)
+    knots = [open_uniform_knots(p, n) for n in ne]
+    nb = tuple(len(U) - p - 1 for U in knots)
+    gx = greville(p, knots[0]) * size[0]
+    gy = greville(p, knots[1]) * size[1]
+    gz = greville(p, knots[2]) * size[2]
+    Z, Y, X = np.meshgrid(gz, gy, gx, indexing="ij")
+    coords = np.stack([X.ravel(), Y.ravel(), Z.ravel()], axis=1)
+    if perturb:
+        rng = np.random.default_rng(seed)
+        h = np.array([size[0] / ne[0], size[1] / ne[1], size[2] / ne[2]])
+        i = np.tile(np.arange(nb[0]), nb[1] * nb[2])
+        j = np.tile(np.repeat(np.arange(nb[1]), nb[0]), nb[2])
+        k = np.repeat(np.arange(nb[2]), nb[0] * nb[1])
+        inner = ((i > 0) & (i < nb[0] - 1) & (j > 0) & (j < nb[1] - 1)
+                 & (k > 0) & (k < nb[2] - 1))
+        d = rng.uniform(-perturb, perturb, size=coords.shape) * h
+        coords[inner] += d[inner]
+    exts = [bezier_extraction(p, U) for U in knots]
+    nel = ne[0] * ne[1] * ne[2]
+    span = np.zeros((nel, 3), dtype=np.int32)
+    q = p + 1
+    conn = np.zeros((nel, q ** 3), dtype=np.int32)
+    ext = np.zeros((nel, 3, q, q))
+    e = 0
+    li = np.arange(q)
+    for st in range(ne[2]):
+        for ss in range(ne[1]):
+            for sr in range(ne[0]):
+                span[e] = (sr, ss, st)
+                # basis index of span s for open knots with simple interior
+                # knots is s (bspbasis.cpp:59-62: FindSpan(s) - p)
+                kk, jj, ii = np.meshgrid(st + li, ss + li, sr + li,
+                                         indexing="ij")
+                conn[e] = (ii + nb[0] * (jj + nb[1] * kk)).ravel()
+                ext[e, 0] = exts[0][sr]
+                ext[e, 1] = exts[1][ss]
+                ext[e, 2] = exts[2][st]
+                e += 1
+    nnode = coords.shape[0]
+    fixed = np.zeros((nnode, 3), dtype=bool)
+    if clamp_x0:
+        fixed[np.arange(nnode) % nb[0] == 0] = True
+    if mat is None:
+        mat = np.tile([E, nu], (nel, 1))
+    return Mesh(coords=coords, conn=conn, shape="bsp", nip=(nip, nip, nip),
+                form=form, mat=mat, fixed=fixed, deg=deg, ext=ext,
+                cpw=np.ones(nnode), span=span, knots=knots, nbas=nb,
+                grid=tuple(ne))

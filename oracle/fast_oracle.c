@@ -1,0 +1,921 @@
+/* fast_oracle.c - CPU FP64 restatement of FAST's element-integration,
+ * assembly and linear-solve path.
+ *
+ * TEST INFRASTRUCTURE ONLY: the checker of the HIP path (tests, smoke check,
+ * CPU-baseline leg of the benchmark).  The product library never links it.
+ *
+ * Operation order follows the reference routines cited at each function so
+ * that, built with -ffp-contract=off, element matrices and vectors agree bit
+ * for bit with the reference build.  Neo-Hookean is not in the reference; its
+ * routines are this project's own and are pinned by finite differences and by
+ * their small-strain limit (which equals the reference's linear model).
+ */
+#define _GNU_SOURCE
+#include "fast_oracle.h"
+
+#include <math.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+#define MAXNN 512   /* nodes per element upper bound (p <= 7 B-splines) */
+#define MAXDOF (3 * MAXNN)
+
+/* ------------------------------------------------------------------------
+ * Quadrature.  Reference: elc/intpoint.cpp:741-920 tabulates Gauss-Legendre
+ * abscissae and weights to 15 decimals.  We compute the rule by Newton
+ * iteration on P_n in long double and round each value to 15 decimals through
+ * its decimal text, which yields the same doubles as the reference's literals.
+ * ---------------------------------------------------------------------- */
+static double round15(long double x)
+{
+  char buf[64];
+  snprintf(buf, sizeof buf, "%.15Lf", x);
+  if (strcmp(buf, "-0.000000000000000") == 0) return 0.0;
+  return strtod(buf, NULL);
+}
+
+int orc_gauss(int n, double *r, double *w)
+{
+  if (n < 1 || n > 32) return 0;
+  const long double PI = 3.141592653589793238462643383279502884L;
+  for (int i = 0; i < n; i++) {
+    /* root i in ascending order */
+    long double x = -cosl(PI * (i + 0.75L) / (n + 0.5L));
+    long double dp = 1.0L;
+    for (int it = 0; it < 100; it++) {
+      long double p0 = 1.0L, p1 = x;
+      if (n == 1) { p1 = x; p0 = 1.0L; }
+      for (int k = 2; k <= n; k++) {
+        long double p2 = ((2 * k - 1) * x * p1 - (k - 1) * p0) / k;
+        p0 = p1; p1 = p2;
+      }
+      /* p1 = P_n(x), p0 = P_{n-1}(x) */
+      dp = n * (x * p1 - p0) / (x * x - 1.0L);
+      long double dx = p1 / dp;
+      x -= dx;
+      if (fabsl(dx) < 1e-30L) break;
+    }
+    if (n == 1) { x = 0.0L; dp = 1.0L; }
+    r[i] = round15(x);
+    w[i] = round15(2.0L / ((1.0L - x * x) * dp * dp));
+  }
+  return 1;
+}
+
+/* Reference: elc/intpoint.cpp:579-617 (CreateBrickPoints): r fastest,
+ * weight = w_r * w_s * w_t in that multiplication order. */
+int orc_brick_points(const int nip[3], double *rst, double *w)
+{
+  double gr[3][32], gw[3][32];
+  for (int d = 0; d < 3; d++)
+    if (!orc_gauss(nip[d], gr[d], gw[d])) return 0;
+  int k = 0;
+  for (int i = 0; i < nip[2]; i++)
+    for (int j = 0; j < nip[1]; j++)
+      for (int l = 0; l < nip[0]; l++) {
+        rst[3 * k + 0] = gr[0][l];
+        rst[3 * k + 1] = gr[1][j];
+        rst[3 * k + 2] = gr[2][i];
+        w[k] = gw[0][l] * gw[1][j] * gw[2][i];
+        k++;
+      }
+  return k;
+}
+
+/* ------------------------------------------------------------------------
+ * Hex8 shape functions.  Reference: shp/shpsolid.cpp:173-238 (node order:
+ * t = +1 face first, counter-clockwise in (r,s); then t = -1 face).
+ * dN is [8][3] = (d/dr, d/ds, d/dt).
+ * ---------------------------------------------------------------------- */
+void orc_hex8_shape(double r, double s, double t, double *N, double *dN)
+{
+  static const double nr[8] = {-1, 1, 1, -1, -1, 1, 1, -1};
+  static const double ns[8] = {-1, -1, 1, 1, -1, -1, 1, 1};
+  static const double nt[8] = {1, 1, 1, 1, -1, -1, -1, -1};
+  for (int a = 0; a < 8; a++) {
+    double fr = 1.0 + nr[a] * r, fs = 1.0 + ns[a] * s, ft = 1.0 + nt[a] * t;
+    /* (1 +- r) written as (1.0 - r) or (1.0 + r): same double either way */
+    if (N) N[a] = 0.125 * fr * fs * ft;
+    if (dN) {
+      dN[3 * a + 0] = (nr[a] > 0 ? 0.125 : -0.125) * fs * ft;
+      dN[3 * a + 1] = (ns[a] > 0 ? 0.125 : -0.125) * fr * ft;
+      dN[3 * a + 2] = (nt[a] > 0 ? 0.125 : -0.125) * fr * fs;
+    }
+  }
+}
+
+/* ------------------------------------------------------------------------
+ * Bernstein basis on [0,1].  Reference: geom/bernbasis.cpp:91-106
+ * (CompPols, de Casteljau-style triangle) and 110-124 (CompDervPols:
+ * n * (B_{i-1,n-1} - B_{i,n-1})).
+ * ---------------------------------------------------------------------- */
+void orc_bernstein(int n, double u, double *B)
+{
+  B[0] = 1.0;
+  double u1 = 1.0 - u;
+  for (int j = 1; j <= n; j++) {
+    double saved = 0.0;
+    for (int k = 0; k < j; k++) {
+      double temp = B[k];
+      B[k] = saved + u1 * temp;
+      saved = u * temp;
+    }
+    B[j] = saved;
+  }
+}
+
+void orc_bernstein_deriv(int n, double u, double *dB)
+{
+  orc_bernstein(n - 1, u, dB);
+  dB[n] = 0.0;
+  double prev = 0.0;
+  for (int i = 0; i <= n; i++) {
+    double curr = dB[i];
+    dB[i] = n * (prev - curr);
+    prev = curr;
+  }
+}
+
+/* ------------------------------------------------------------------------
+ * Bezier extraction (Borden, Scott, Evans, Hughes 2011, Algorithm 1).
+ * Reference: geom/knotvec.cpp:415-493.  C is [nspan][p+1][p+1] row-major:
+ * row = local B-spline basis, column = Bernstein polynomial.
+ * Returns the number of non-zero spans.
+ * ---------------------------------------------------------------------- */
+int orc_bezier_extraction(int p, int m, const double *U, double *C)
+{
+  int ns = 0;
+  for (int i = 1; i < m; i++)
+    if (fabs(U[i] - U[i - 1]) > 1e-6) ns++;
+  int q = p + 1;
+  memset(C, 0, sizeof(double) * (size_t)ns * q * q);
+#define CE(e, i, j) C[((size_t)(e) * q + (i)) * q + (j)]
+  for (int i = 0; i < q; i++) CE(0, i, i) = 1.0;
+  int a = p + 1, b = a + 1, nb = 1;
+  double alphas[64];
+  while (b < m) {
+    if (nb < ns)
+      for (int i = 0; i < q; i++) CE(nb, i, i) = 1.0;
+    int i = b;
+    while (b < m && U[b] == U[b - 1]) b++;
+    int mult = b - i + 1;
+    if (mult < p) {
+      double numer = U[b - 1] - U[a - 1];
+      for (int j = p; j >= mult + 1; j--)
+        alphas[j - mult - 1] = numer / (U[a + j - 1] - U[a - 1]);
+      int r = p - mult;
+      for (int j = 1; j <= r; j++) {
+        int save = r - j + 1;
+        int s = mult + j;
+        for (int k = p + 1; k >= s + 1; k--) {
+          double alpha = alphas[k - s - 1];
+          for (int w = 0; w < q; w++)
+            CE(nb - 1, w, k - 1) =
+                alpha * CE(nb - 1, w, k - 1) + (1.0 - alpha) * CE(nb - 1, w, k - 2);
+        }
+        if (b < m && nb < ns)
+          for (int w = save, v = p - j + 1; w <= j + save; w++, v++)
+            CE(nb, w - 1, save - 1) = CE(nb - 1, v - 1, p);
+      }
+    }
+    nb++;
+    if (b < m) a = b++;
+  }
+#undef CE
+  return ns;
+}
+
+/* ------------------------------------------------------------------------
+ * Natural derivatives dN/d(r,s,t) for one element.
+ * Hex8: shp/shpsolid.cpp:211-238.
+ * B-spline solid (rational branch; the non-rational branch of the reference
+ * never advances its basis counter, shp/shpbsp.cpp:1015-1031, so decks always
+ * declare rational patches and unit weights reduce to plain B-splines):
+ * shp/shpbsp.cpp:254-300 (EvalBspBas/Derv: t' = (t+1)/2, derivative / 2,
+ * N_i += C[i][j] B_j) and 985-1067 (tensor product k(r) fastest, quotient).
+ * ---------------------------------------------------------------------- */
+static void bsp_univariate(int p, const double *C, double t, double *N,
+                           double *dN)
+{
+  double B[16], dB[16];
+  double u = (t + 1.0) / 2.0;
+  orc_bernstein(p, u, B);
+  orc_bernstein_deriv(p, u, dB);
+  for (int i = 0; i <= p; i++) dB[i] /= 2.0;
+  for (int i = 0; i <= p; i++) {
+    N[i] = 0.0;
+    dN[i] = 0.0;
+  }
+  for (int i = 0; i <= p; i++)
+    for (int j = 0; j <= p; j++) N[i] += C[i * (p + 1) + j] * B[j];
+  for (int i = 0; i <= p; i++)
+    for (int j = 0; j <= p; j++) dN[i] += C[i * (p + 1) + j] * dB[j];
+}
+
+static void natural_derivs(const orc_etype *et, const double *ext,
+                           const double *w, const double rst[3], double *dN)
+{
+  if (et->shape == ORC_HEX8) {
+    orc_hex8_shape(rst[0], rst[1], rst[2], NULL, dN);
+    return;
+  }
+  const int *p = et->deg;
+  double Nr[16], Ns[16], Nt[16], dNr[16], dNs[16], dNt[16];
+  size_t o0 = 0, o1 = (size_t)(p[0] + 1) * (p[0] + 1);
+  size_t o2 = o1 + (size_t)(p[1] + 1) * (p[1] + 1);
+  bsp_univariate(p[0], ext + o0, rst[0], Nr, dNr);
+  bsp_univariate(p[1], ext + o1, rst[1], Ns, dNs);
+  bsp_univariate(p[2], ext + o2, rst[2], Nt, dNt);
+  double N[MAXNN];
+  int a = 0;
+  if (!et->rational) {
+    /* plain tensor product (what the reference intends for this branch) */
+    for (int i = 0; i <= p[2]; i++)
+      for (int j = 0; j <= p[1]; j++)
+        for (int k = 0; k <= p[0]; k++, a++) {
+          dN[3 * a + 0] = dNr[k] * Ns[j] * Nt[i];
+          dN[3 * a + 1] = Nr[k] * dNs[j] * Nt[i];
+          dN[3 * a + 2] = Nr[k] * Ns[j] * dNt[i];
+        }
+    return;
+  }
+  double W = 0, dwr = 0, dws = 0, dwt = 0;
+  for (int i = 0; i <= p[2]; i++)
+    for (int j = 0; j <= p[1]; j++)
+      for (int k = 0; k <= p[0]; k++) {
+        double wa = w ? w[a] : 1.0;
+        N[a] = Nr[k] * Ns[j] * Nt[i] * wa;
+        W += N[a];
+        dN[3 * a + 0] = dNr[k] * Ns[j] * Nt[i] * wa;
+        dN[3 * a + 1] = Nr[k] * dNs[j] * Nt[i] * wa;
+        dN[3 * a + 2] = Nr[k] * Ns[j] * dNt[i] * wa;
+        dwr += dN[3 * a + 0];
+        dws += dN[3 * a + 1];
+        dwt += dN[3 * a + 2];
+        a++;
+      }
+  for (int i = 0; i < a; i++) {
+    N[i] /= W;
+    dN[3 * i + 0] = (dN[3 * i + 0] - N[i] * dwr) / W;
+    dN[3 * i + 1] = (dN[3 * i + 1] - N[i] * dws) / W;
+    dN[3 * i + 2] = (dN[3 * i + 2] - N[i] * dwt) / W;
+  }
+}
+
+/* Reference: shp/shpsolid.cpp:58-126 (J = dN^T X with rows r,s,t) and
+ * shp/shape.cpp:360-391 (cofactor inverse; fails on det == 0). */
+int orc_shape_dxyz(const orc_etype *et, const double *X, const double *ext,
+                   const double *w, const double rst[3], double *detJ,
+                   double *dNxyz)
+{
+  double dN[3 * MAXNN];
+  natural_derivs(et, ext, w, rst, dN);
+  double A[3][3] = {{0}};
+  int nn = et->nn;
+  for (int i = 0; i < nn; i++) {
+    A[0][0] += dN[3 * i + 0] * X[3 * i + 0];
+    A[0][1] += dN[3 * i + 0] * X[3 * i + 1];
+    A[0][2] += dN[3 * i + 0] * X[3 * i + 2];
+    A[1][0] += dN[3 * i + 1] * X[3 * i + 0];
+    A[1][1] += dN[3 * i + 1] * X[3 * i + 1];
+    A[1][2] += dN[3 * i + 1] * X[3 * i + 2];
+    A[2][0] += dN[3 * i + 2] * X[3 * i + 0];
+    A[2][1] += dN[3 * i + 2] * X[3 * i + 1];
+    A[2][2] += dN[3 * i + 2] * X[3 * i + 2];
+  }
+  double det = A[0][0] * (A[1][1] * A[2][2] - A[1][2] * A[2][1]) -
+               A[0][1] * (A[1][0] * A[2][2] - A[1][2] * A[2][0]) +
+               A[0][2] * (A[1][0] * A[2][1] - A[1][1] * A[2][0]);
+  *detJ = det;
+  if (fabs(det) == 0.0) return 0;
+  double B[3][3];
+  B[0][0] = (A[1][1] * A[2][2] - A[1][2] * A[2][1]) / det;
+  B[0][1] = -(A[0][1] * A[2][2] - A[0][2] * A[2][1]) / det;
+  B[0][2] = (A[0][1] * A[1][2] - A[0][2] * A[1][1]) / det;
+  B[1][0] = -(A[1][0] * A[2][2] - A[1][2] * A[2][0]) / det;
+  B[1][1] = (A[0][0] * A[2][2] - A[0][2] * A[2][0]) / det;
+  B[1][2] = -(A[0][0] * A[1][2] - A[0][2] * A[1][0]) / det;
+  B[2][0] = (A[1][0] * A[2][1] - A[1][1] * A[2][0]) / det;
+  B[2][1] = -(A[0][0] * A[2][1] - A[0][1] * A[2][0]) / det;
+  B[2][2] = (A[0][0] * A[1][1] - A[0][1] * A[1][0]) / det;
+  for (int i = 0; i < nn; i++) {
+    double r = dN[3 * i + 0], s = dN[3 * i + 1], t = dN[3 * i + 2];
+    dNxyz[3 * i + 0] = B[0][0] * r + B[0][1] * s + B[0][2] * t;
+    dNxyz[3 * i + 1] = B[1][0] * r + B[1][1] * s + B[1][2] * t;
+    dNxyz[3 * i + 2] = B[2][0] * r + B[2][1] * s + B[2][2] * t;
+  }
+  return 1;
+}
+
+/* ------------------------------------------------------------------------
+ * Kinematic matrices of the 3D solid model.  Reference: elc/anmodel.cpp
+ * 1418-1447 (B: Voigt xx,yy,zz,xy,xz,yz with engineering shear),
+ * 1451-1510 (Bl), 1514-1532 (Bnl, 9 rows), 1536-1568 (S block diagonal),
+ * 1286-1298 (isotropic Q).
+ * ---------------------------------------------------------------------- */
+static void bmatrix(int nn, const double *d, double *B /* [6][3nn] */)
+{
+  int nd = 3 * nn;
+  memset(B, 0, sizeof(double) * 6 * nd);
+  for (int i = 0; i < nn; i++) {
+    double x = d[3 * i], y = d[3 * i + 1], z = d[3 * i + 2];
+    B[0 * nd + 3 * i] = x;
+    B[1 * nd + 3 * i + 1] = y;
+    B[2 * nd + 3 * i + 2] = z;
+    B[3 * nd + 3 * i] = y;
+    B[3 * nd + 3 * i + 1] = x;
+    B[4 * nd + 3 * i] = z;
+    B[4 * nd + 3 * i + 2] = x;
+    B[5 * nd + 3 * i + 1] = z;
+    B[5 * nd + 3 * i + 2] = y;
+  }
+}
+
+static void blmatrix(int nn, const double *d, const double *u, double *Bl)
+{
+  int nd = 3 * nn;
+  double l11 = 0, l12 = 0, l13 = 0, l21 = 0, l22 = 0, l23 = 0, l31 = 0,
+         l32 = 0, l33 = 0;
+  for (int i = 0; i < nn; i++) {
+    double x = d[3 * i], y = d[3 * i + 1], z = d[3 * i + 2];
+    l11 += u[3 * i] * x;     l12 += u[3 * i] * y;     l13 += u[3 * i] * z;
+    l21 += u[3 * i + 1] * x; l22 += u[3 * i + 1] * y; l23 += u[3 * i + 1] * z;
+    l31 += u[3 * i + 2] * x; l32 += u[3 * i + 2] * y; l33 += u[3 * i + 2] * z;
+  }
+  for (int i = 0; i < nn; i++) {
+    double x = d[3 * i], y = d[3 * i + 1], z = d[3 * i + 2];
+    double *r0 = Bl, *r1 = Bl + nd, *r2 = Bl + 2 * nd, *r3 = Bl + 3 * nd,
+           *r4 = Bl + 4 * nd, *r5 = Bl + 5 * nd;
+    r0[3 * i] = l11 * x; r0[3 * i + 1] = l21 * x; r0[3 * i + 2] = l31 * x;
+    r1[3 * i] = l12 * y; r1[3 * i + 1] = l22 * y; r1[3 * i + 2] = l32 * y;
+    r2[3 * i] = l13 * z; r2[3 * i + 1] = l23 * z; r2[3 * i + 2] = l33 * z;
+    r3[3 * i] = l11 * y + l12 * x;
+    r3[3 * i + 1] = l21 * y + l22 * x;
+    r3[3 * i + 2] = l31 * y + l32 * x;
+    r4[3 * i] = l11 * z + l13 * x;
+    r4[3 * i + 1] = l21 * z + l23 * x;
+    r4[3 * i + 2] = l31 * z + l33 * x;
+    r5[3 * i] = l12 * z + l13 * y;
+    r5[3 * i + 1] = l22 * z + l23 * y;
+    r5[3 * i + 2] = l32 * z + l33 * y;
+  }
+}
+
+static void bnlmatrix(int nn, const double *d, double *Bnl /* [9][3nn] */)
+{
+  int nd = 3 * nn;
+  memset(Bnl, 0, sizeof(double) * 9 * nd);
+  for (int i = 0; i < nn; i++)
+    for (int c = 0; c < 3; c++)
+      for (int k = 0; k < 3; k++) Bnl[(3 * c + k) * nd + 3 * i + c] = d[3 * i + k];
+}
+
+static void smatrix(const double *s, double *S /* [9][9] */)
+{
+  memset(S, 0, sizeof(double) * 81);
+  for (int b = 0; b < 3; b++) {
+    double *M = S + (3 * b) * 9 + 3 * b;
+    M[0 * 9 + 0] = s[0]; M[1 * 9 + 0] = s[3]; M[2 * 9 + 0] = s[4];
+    M[0 * 9 + 1] = s[3]; M[1 * 9 + 1] = s[1]; M[2 * 9 + 1] = s[5];
+    M[0 * 9 + 2] = s[4]; M[1 * 9 + 2] = s[5]; M[2 * 9 + 2] = s[2];
+  }
+}
+
+static void qmatrix(double E, double nu, double *Q /* [6][6] */)
+{
+  double d = (1.0 + nu) * (1.0 - 2.0 * nu);
+  memset(Q, 0, sizeof(double) * 36);
+  Q[0] = Q[7] = Q[14] = E * (1.0 - nu) / d;
+  Q[1] = Q[6] = Q[2] = Q[12] = Q[8] = Q[13] = E * nu / d;
+  Q[21] = Q[28] = Q[35] = 0.5 * E / (1.0 + nu);
+}
+
+/* Compressible neo-Hookean (this project's model; absent from the reference):
+ * W = mu/2 (I1 - 3) - mu ln J + lambda/2 (ln J)^2.  S = mu (I - C^-1) +
+ * lambda ln J C^-1; D_IJKL = lambda c_IJ c_KL + (mu - lambda ln J)
+ * (c_IK c_JL + c_IL c_JK), c = C^-1, in Voigt form for engineering shear.
+ * Returns 0 when det C <= 0. */
+static const int VI[6] = {0, 1, 2, 0, 0, 1}, VJ[6] = {0, 1, 2, 1, 2, 2};
+
+static int neohooke(double mu, double lam, const double *e, double *s,
+                    double *D)
+{
+  double C[3][3];
+  C[0][0] = 1.0 + 2.0 * e[0];
+  C[1][1] = 1.0 + 2.0 * e[1];
+  C[2][2] = 1.0 + 2.0 * e[2];
+  C[0][1] = C[1][0] = e[3];
+  C[0][2] = C[2][0] = e[4];
+  C[1][2] = C[2][1] = e[5];
+  double det = C[0][0] * (C[1][1] * C[2][2] - C[1][2] * C[2][1]) -
+               C[0][1] * (C[1][0] * C[2][2] - C[1][2] * C[2][0]) +
+               C[0][2] * (C[1][0] * C[2][1] - C[1][1] * C[2][0]);
+  if (!(det > 0.0)) return 0;
+  double c[3][3];
+  c[0][0] = (C[1][1] * C[2][2] - C[1][2] * C[2][1]) / det;
+  c[0][1] = -(C[0][1] * C[2][2] - C[0][2] * C[2][1]) / det;
+  c[0][2] = (C[0][1] * C[1][2] - C[0][2] * C[1][1]) / det;
+  c[1][0] = c[0][1];
+  c[1][1] = (C[0][0] * C[2][2] - C[0][2] * C[2][0]) / det;
+  c[1][2] = -(C[0][0] * C[1][2] - C[0][2] * C[1][0]) / det;
+  c[2][0] = c[0][2];
+  c[2][1] = c[1][2];
+  c[2][2] = (C[0][0] * C[1][1] - C[0][1] * C[1][0]) / det;
+  double lnJ = 0.5 * log(det);
+  for (int a = 0; a < 6; a++) {
+    int I = VI[a], J = VJ[a];
+    s[a] = mu * ((I == J ? 1.0 : 0.0) - c[I][J]) + lam * lnJ * c[I][J];
+  }
+  if (D) {
+    double m2 = mu - lam * lnJ;
+    for (int a = 0; a < 6; a++)
+      for (int b = 0; b < 6; b++) {
+        int I = VI[a], J = VJ[a], K = VI[b], L = VJ[b];
+        D[a * 6 + b] = lam * c[I][J] * c[K][L] +
+                       m2 * (c[I][K] * c[J][L] + c[I][L] * c[J][K]);
+      }
+  }
+  return 1;
+}
+
+/* Reference: mlib/matvec.cpp:372-390 (K += w B^T C B, skipping zero B and C
+ * entries; operation order aux1 = w*B, aux2 = aux1*C, K += aux2*B). */
+static void trip_btcb(int m, int n, const double *B, const double *C,
+                      double w, double *K)
+{
+  for (int i = 0; i < n; i++)
+    for (int k = 0; k < m; k++) {
+      double bki = B[k * n + i];
+      if (bki != 0.0) {
+        double aux1 = w * bki;
+        for (int l = 0; l < m; l++) {
+          double ckl = C[k * m + l];
+          if (ckl != 0.0) {
+            double aux2 = aux1 * ckl;
+            const double *Bl = B + (size_t)l * n;
+            double *Ki = K + (size_t)i * n;
+            for (int j = 0; j < n; j++) Ki[j] += aux2 * Bl[j];
+          }
+        }
+      }
+    }
+}
+
+/* Reference: mlib/matvec.cpp:324-335 (row dot products from 0.0). */
+static void matvec(int m, int n, const double *A, const double *x, double *y)
+{
+  for (int i = 0; i < m; i++) {
+    double d = 0.0;
+    for (int j = 0; j < n; j++) d += A[(size_t)i * n + j] * x[j];
+    y[i] = d;
+  }
+}
+
+/* Reference: mlib/matvec.cpp:359-368 (g += a B^T s, column by column). */
+static void multtacc(double a, int m, int n, const double *B, const double *s,
+                     double *g)
+{
+  for (int j = 0; j < m; j++) {
+    double auj = a * s[j];
+    const double *Bj = B + (size_t)j * n;
+    for (int i = 0; i < n; i++) g[i] += Bj[i] * auj;
+  }
+}
+
+/* Material response at one point.  Linear / SVK: mat/cmodel.cpp:203-216 and
+ * 273-288 (sigma = Q eps, Q rebuilt per call). */
+static int material(int form, const double *mat, const double *e, double *s,
+                    double *D)
+{
+  if (form == ORC_TL_NEOHOOKE) return neohooke(mat[0], mat[1], e, s, D);
+  double Q[36];
+  qmatrix(mat[0], mat[1], Q);
+  if (s) matvec(6, 6, Q, e, s);
+  if (D) memcpy(D, Q, sizeof Q);
+  return 1;
+}
+
+/* Green-Lagrange strain: elc/elmparam.cpp:1899-1919 (B u + 0.5 (Bl u)). */
+static void green_strain(int nd, const double *B, const double *Bl,
+                         const double *u, double *e)
+{
+  double s1[6], s2[6];
+  matvec(6, nd, B, u, s1);
+  matvec(6, nd, Bl, u, s2);
+  for (int i = 0; i < 6; i++) s2[i] *= 0.5;
+  for (int i = 0; i < 6; i++) e[i] = s1[i] + s2[i];
+}
+
+/* Linear: elc/elmparam.cpp:539-597 (StiffMat).  TL: elc/elmparam.cpp
+ * 1711-1807 (K_t = (B+Bl)^T C (B+Bl) + Bnl^T S Bnl per point; a failed
+ * stress update ends the point loop, as in the reference). */
+int orc_elem_stiff(const orc_etype *et, const double *X, const double *ext,
+                   const double *w, const double *mat, const double *u,
+                   double *K)
+{
+  int nn = et->nn, nd = 3 * nn;
+  double rst[3 * 512], wq[512];
+  int nq = orc_brick_points(et->nip, rst, wq);
+  double *B = malloc(sizeof(double) * 6 * nd);
+  double *Bl = malloc(sizeof(double) * 6 * nd);
+  double *Bnl = malloc(sizeof(double) * 9 * nd);
+  double *d = malloc(sizeof(double) * 3 * nn);
+  double *uz = calloc(nd, sizeof(double));
+  const double *ue = u ? u : uz;
+  memset(K, 0, sizeof(double) * nd * nd);
+  int ok = 1;
+  for (int q = 0; q < nq; q++) {
+    double detJ;
+    if (!orc_shape_dxyz(et, X, ext, w, rst + 3 * q, &detJ, d)) { ok = 0; }
+    bmatrix(nn, d, B);
+    double coeff = wq[q] * detJ * 1.0;
+    if (et->form == ORC_LINEAR) {
+      double C[36];
+      material(ORC_LINEAR, mat, NULL, NULL, C);
+      trip_btcb(6, nd, B, C, coeff, K);
+    } else {
+      double C[36], e[6], s[6], S[81];
+      blmatrix(nn, d, ue, Bl);
+      bnlmatrix(nn, d, Bnl);
+      green_strain(nd, B, Bl, ue, e);
+      if (et->form == ORC_TL_SVK) {
+        material(ORC_LINEAR, mat, NULL, NULL, C);
+        material(ORC_LINEAR, mat, e, s, NULL);
+      } else if (!material(et->form, mat, e, s, C)) {
+        ok = 0;
+        break;
+      }
+      smatrix(s, S);
+      for (int i = 0; i < 6 * nd; i++) B[i] += Bl[i];
+      trip_btcb(6, nd, B, C, coeff, K);
+      trip_btcb(9, nd, Bnl, S, coeff, K);
+    }
+  }
+  free(B); free(Bl); free(Bnl); free(d); free(uz);
+  return ok;
+}
+
+/* Linear: elc/elmparam.cpp:449-535.  TL: elc/elmparam.cpp:1615-1707. */
+int orc_elem_intforce(const orc_etype *et, const double *X,
+                      const double *ext, const double *w, const double *mat,
+                      const double *u, double *g)
+{
+  int nn = et->nn, nd = 3 * nn;
+  double rst[3 * 512], wq[512];
+  int nq = orc_brick_points(et->nip, rst, wq);
+  double *B = malloc(sizeof(double) * 6 * nd);
+  double *Bl = malloc(sizeof(double) * 6 * nd);
+  double *d = malloc(sizeof(double) * 3 * nn);
+  memset(g, 0, sizeof(double) * nd);
+  int ok = 1;
+  for (int q = 0; q < nq; q++) {
+    double detJ, e[6], s[6];
+    if (!orc_shape_dxyz(et, X, ext, w, rst + 3 * q, &detJ, d)) ok = 0;
+    bmatrix(nn, d, B);
+    if (et->form == ORC_LINEAR) {
+      matvec(6, nd, B, u, e);
+    } else {
+      blmatrix(nn, d, u, Bl);
+      green_strain(nd, B, Bl, u, e);
+    }
+    if (!material(et->form == ORC_TL_SVK ? ORC_LINEAR : et->form, mat, e, s,
+                  NULL)) {
+      ok = 0;
+      break;
+    }
+    if (et->form != ORC_LINEAR)
+      for (int i = 0; i < 6 * nd; i++) B[i] += Bl[i];
+    double coeff = wq[q] * detJ * 1.0;
+    multtacc(coeff, 6, nd, B, s, g);
+  }
+  free(B); free(Bl); free(d);
+  return ok;
+}
+
+/* ------------------------------------------------------------------------
+ * Mesh-level loops.
+ * ---------------------------------------------------------------------- */
+static void gather_elem(const orc_mesh *m, int e, const double *u_nodal,
+                        double *X, double *ue, double *we, int *edof)
+{
+  int nn = m->et.nn;
+  const int *cn = m->conn + (size_t)e * nn;
+  for (int a = 0; a < nn; a++) {
+    int n = cn[a];
+    for (int c = 0; c < 3; c++) {
+      X[3 * a + c] = m->coords[3 * (size_t)n + c];
+      if (ue) ue[3 * a + c] = u_nodal ? u_nodal[3 * (size_t)n + c] : 0.0;
+      if (edof) edof[3 * a + c] = m->dofmap[3 * (size_t)n + c];
+    }
+    if (we) we[a] = m->cpw ? m->cpw[n] : 1.0;
+  }
+}
+
+static size_t ext_stride(const orc_etype *et)
+{
+  if (et->shape != ORC_BSP) return 0;
+  size_t s = 0;
+  for (int d = 0; d < 3; d++) s += (size_t)(et->deg[d] + 1) * (et->deg[d] + 1);
+  return s;
+}
+
+static long long csr_find(const long long *rowptr, const int *col, int r,
+                          int c)
+{
+  long long lo = rowptr[r], hi = rowptr[r + 1] - 1;
+  while (lo <= hi) {
+    long long mid = (lo + hi) >> 1;
+    if (col[mid] == c) return mid;
+    if (col[mid] < c) lo = mid + 1; else hi = mid - 1;
+  }
+  return -1;
+}
+
+/* Reference: ctrl/ctrl.cpp:886-975 (element loop, element order) and
+ * elc/element.cpp:2975-3002 (AddGlobMat: only dof[i] >= dof[j] > 0 for the
+ * symmetric matrix).  The lower entry is mirrored to the upper position, so
+ * the full CSR's lower triangle equals the reference's stored values.
+ * With nthreads > 1 the element loop is split into element ranges whose
+ * element matrices are computed in parallel and added in element order. */
+int orc_assemble_K(const orc_mesh *m, const double *u_nodal, int neq,
+                   const long long *rowptr, const int *col, double *val,
+                   int nthreads, int eb, int ee)
+{
+  (void)neq;
+  int nn = m->et.nn, nd = 3 * nn;
+  size_t es = ext_stride(&m->et);
+  if (ee < 0 || ee > m->nelem) ee = m->nelem;
+  int ok = 1;
+  const int CH = 256;
+  double *Kbuf = malloc(sizeof(double) * (size_t)CH * nd * nd);
+  int *dbuf = malloc(sizeof(int) * (size_t)CH * nd);
+  int *okb = malloc(sizeof(int) * CH);
+  if (nthreads < 1) nthreads = 1;
+  for (int c0 = eb; c0 < ee; c0 += CH) {
+    int c1 = c0 + CH < ee ? c0 + CH : ee;
+#pragma omp parallel for num_threads(nthreads) schedule(dynamic, 4)
+    for (int e = c0; e < c1; e++) {
+      double X[3 * MAXNN], ue[3 * MAXNN], we[MAXNN];
+      int *edof = dbuf + (size_t)(e - c0) * nd;
+      gather_elem(m, e, u_nodal, X, ue, we, edof);
+      okb[e - c0] = orc_elem_stiff(&m->et, X, m->ext ? m->ext + es * e : NULL,
+                                   we, m->mat + 2 * (size_t)e,
+                                   u_nodal ? ue : NULL,
+                                   Kbuf + (size_t)(e - c0) * nd * nd);
+    }
+    for (int e = c0; e < c1; e++) {
+      const double *Ke = Kbuf + (size_t)(e - c0) * nd * nd;
+      const int *edof = dbuf + (size_t)(e - c0) * nd;
+      if (!okb[e - c0]) ok = 0;
+      for (int i = 0; i < nd; i++) {
+        int di = edof[i];
+        if (!di) continue;
+        for (int j = 0; j < nd; j++) {
+          int dj = edof[j];
+          if (!dj || di < dj) continue;
+          long long p = csr_find(rowptr, col, di - 1, dj - 1);
+          if (p < 0) { ok = 0; continue; }
+          val[p] += Ke[(size_t)i * nd + j];
+          if (di != dj) {
+            long long p2 = csr_find(rowptr, col, dj - 1, di - 1);
+            if (p2 < 0) { ok = 0; continue; }
+            val[p2] += Ke[(size_t)i * nd + j];
+          }
+        }
+      }
+    }
+  }
+  free(Kbuf); free(dbuf); free(okb);
+  return ok;
+}
+
+/* Reference: ctrl/ctrl.cpp:719-813 and elc/element.cpp:2953-2971. */
+int orc_assemble_fint(const orc_mesh *m, const double *u_nodal, int neq,
+                      double *g, int nthreads, int eb, int ee)
+{
+  (void)neq;
+  int nn = m->et.nn, nd = 3 * nn;
+  size_t es = ext_stride(&m->et);
+  if (ee < 0 || ee > m->nelem) ee = m->nelem;
+  int ok = 1;
+  const int CH = 1024;
+  double *gbuf = malloc(sizeof(double) * (size_t)CH * nd);
+  int *dbuf = malloc(sizeof(int) * (size_t)CH * nd);
+  int *okb = malloc(sizeof(int) * CH);
+  if (nthreads < 1) nthreads = 1;
+  for (int c0 = eb; c0 < ee && ok; c0 += CH) {
+    int c1 = c0 + CH < ee ? c0 + CH : ee;
+#pragma omp parallel for num_threads(nthreads) schedule(static)
+    for (int e = c0; e < c1; e++) {
+      double X[3 * MAXNN], ue[3 * MAXNN], we[MAXNN];
+      int *edof = dbuf + (size_t)(e - c0) * nd;
+      gather_elem(m, e, u_nodal, X, ue, we, edof);
+      okb[e - c0] = orc_elem_intforce(&m->et, X,
+                                      m->ext ? m->ext + es * e : NULL, we,
+                                      m->mat + 2 * (size_t)e, ue,
+                                      gbuf + (size_t)(e - c0) * nd);
+    }
+    for (int e = c0; e < c1; e++) {
+      if (!okb[e - c0]) { ok = 0; break; }
+      const double *ge = gbuf + (size_t)(e - c0) * nd;
+      const int *edof = dbuf + (size_t)(e - c0) * nd;
+      for (int i = 0; i < nd; i++)
+        if (edof[i]) g[edof[i] - 1] += ge[i];
+    }
+  }
+  free(gbuf); free(dbuf); free(okb);
+  return ok;
+}
+
+/* Brute-force pattern: per row, the set of equations of all elements that
+ * touch the row's equation (sorted, unique). */
+static int cmp_int(const void *a, const void *b)
+{
+  int x = *(const int *)a, y = *(const int *)b;
+  return (x > y) - (x < y);
+}
+
+static void pattern_rows(const orc_mesh *m, int neq, long long *rowptr,
+                         int *col)
+{
+  int nn = m->et.nn;
+  /* node -> element adjacency */
+  int *cnt = calloc((size_t)m->nnode + 1, sizeof(int));
+  for (size_t i = 0; i < (size_t)m->nelem * nn; i++) cnt[m->conn[i] + 1]++;
+  for (int i = 0; i < m->nnode; i++) cnt[i + 1] += cnt[i];
+  int *adj = malloc(sizeof(int) * ((size_t)m->nelem * nn + 1));
+  int *pos = malloc(sizeof(int) * ((size_t)m->nnode + 1));
+  memcpy(pos, cnt, sizeof(int) * m->nnode);
+  int maxadj = 0;
+  for (int i = 0; i < m->nnode; i++)
+    if (cnt[i + 1] - cnt[i] > maxadj) maxadj = cnt[i + 1] - cnt[i];
+  for (int e = 0; e < m->nelem; e++)
+    for (int a = 0; a < nn; a++) adj[pos[m->conn[(size_t)e * nn + a]]++] = e;
+  int *buf = malloc(sizeof(int) * ((size_t)maxadj * nn * 3 + 1));
+  int *eqnode = malloc(sizeof(int) * (size_t)(neq > 0 ? neq : 1));
+  for (int n = 0; n < m->nnode; n++)
+    for (int c = 0; c < 3; c++) {
+      int d = m->dofmap[3 * n + c];
+      if (d) eqnode[d - 1] = n;
+    }
+  long long total = 0;
+  rowptr[0] = 0;
+  for (int r = 0; r < neq; r++) {
+    int n = eqnode[r];
+    int k = 0;
+    for (int t = cnt[n]; t < cnt[n + 1]; t++) {
+      int e = adj[t];
+      for (int a = 0; a < nn; a++)
+        for (int c = 0; c < 3; c++) {
+          int d = m->dofmap[3 * m->conn[(size_t)e * nn + a] + c];
+          if (d) buf[k++] = d - 1;
+        }
+    }
+    qsort(buf, k, sizeof(int), cmp_int);
+    int u = 0;
+    for (int i = 0; i < k; i++)
+      if (i == 0 || buf[i] != buf[i - 1]) {
+        if (col) col[total + u] = buf[i];
+        u++;
+      }
+    total += u;
+    rowptr[r + 1] = total;
+  }
+  free(cnt); free(adj); free(pos); free(buf); free(eqnode);
+}
+
+long long orc_csr_count(const orc_mesh *m, int neq, long long *rowptr)
+{
+  pattern_rows(m, neq, rowptr, NULL);
+  return rowptr[neq];
+}
+
+void orc_csr_fill(const orc_mesh *m, int neq, const long long *rowptr,
+                  int *col)
+{
+  long long *rp = malloc(sizeof(long long) * (size_t)(neq + 1));
+  pattern_rows(m, neq, rp, col);
+  (void)rowptr;
+  free(rp);
+}
+
+/* ------------------------------------------------------------------------
+ * Solvers.
+ * ---------------------------------------------------------------------- */
+void orc_spmv(int n, const long long *rowptr, const int *col,
+              const double *val, const double *x, double *y)
+{
+  for (int i = 0; i < n; i++) {
+    double s = 0.0;
+    for (long long p = rowptr[i]; p < rowptr[i + 1]; p++) s += val[p] * x[col[p]];
+    y[i] = s;
+  }
+}
+
+static double vlen(int n, const double *u)
+{
+  double d = 0.0;
+  for (int i = 0; i < n; i++) d += u[i] * u[i];
+  return sqrt(d);
+}
+
+/* Reference: mlib/sysmat.cpp:1061-1125 (PCGSolver): Jacobi preconditioner,
+ * u0 = 0, convergence on |r| / |f| < tol with |f| > 1e11 replaced by 1,
+ * immediate return when |f| < tol.  Returns 1 when converged. */
+int orc_pcg(int n, const long long *rowptr, const int *col, const double *val,
+            const double *f, double *u, int maxit, double tol, int *iters,
+            double *relres)
+{
+  double lenf = vlen(n, f);
+  for (int i = 0; i < n; i++) u[i] = 0.0;
+  if (iters) *iters = 0;
+  if (relres) *relres = 0.0;
+  if (lenf < tol) return 1;
+  if (lenf > 10e+10) lenf = 1.0;
+  double *r = malloc(sizeof(double) * n), *p = malloc(sizeof(double) * n);
+  double *d = malloc(sizeof(double) * n), *w = malloc(sizeof(double) * n);
+  double *z = malloc(sizeof(double) * n);
+  for (int i = 0; i < n; i++) {
+    d[i] = 0.0;
+    for (long long k = rowptr[i]; k < rowptr[i + 1]; k++)
+      if (col[k] == i) d[i] = val[k];
+  }
+  for (int i = 0; i < n; i++) { r[i] = f[i]; z[i] = r[i] / d[i]; p[i] = z[i]; }
+  double dot0 = 0.0;
+  for (int i = 0; i < n; i++) dot0 += z[i] * r[i];
+  int conv = 0, k;
+  double err = 1.0;
+  for (k = 1; k <= maxit; k++) {
+    orc_spmv(n, rowptr, col, val, p, w);
+    double pw = 0.0;
+    for (int i = 0; i < n; i++) pw += p[i] * w[i];
+    double alpha = dot0 / pw;
+    for (int i = 0; i < n; i++) u[i] = u[i] + alpha * p[i];
+    for (int i = 0; i < n; i++) r[i] = r[i] + (-alpha) * w[i];
+    for (int i = 0; i < n; i++) z[i] = r[i] / d[i];
+    err = vlen(n, r) / lenf;
+    if (err < tol) { conv = 1; break; }
+    double dot1 = 0.0;
+    for (int i = 0; i < n; i++) dot1 += z[i] * r[i];
+    double beta = dot1 / dot0;
+    dot0 = dot1;
+    for (int i = 0; i < n; i++) p[i] = z[i] + beta * p[i];
+  }
+  if (iters) *iters = conv ? k : maxit;
+  if (relres) *relres = err;
+  free(r); free(p); free(d); free(w); free(z);
+  return conv;
+}
+
+/* Reference: ctrl/ctrl.cpp:2112-2157 (profile = lowest equation coupled to
+ * each row) and mlib/matvec.cpp:609-664 (CroutSolver: L D L^T on the skyline,
+ * forward reduction, diagonal scaling, back substitution; fails on a pivot
+ * below 1e-12).  Skyline rows are built from the CSR lower part. */
+int orc_skyline_solve(int n, const long long *rowptr, const int *col,
+                      const double *val, const double *f, double *u)
+{
+  int *p = malloc(sizeof(int) * n);
+  size_t total = 0;
+  for (int i = 0; i < n; i++) {
+    int mn = i;
+    for (long long k = rowptr[i]; k < rowptr[i + 1]; k++)
+      if (col[k] < mn) mn = col[k];
+    p[i] = mn;
+    total += (size_t)(i - mn + 1);
+  }
+  double **A = malloc(sizeof(double *) * n);
+  double *store = calloc(total, sizeof(double));
+  size_t off = 0;
+  for (int i = 0; i < n; i++) {
+    A[i] = store + off - p[i]; /* A[i][j] valid for p[i] <= j <= i */
+    off += (size_t)(i - p[i] + 1);
+    for (long long k = rowptr[i]; k < rowptr[i + 1]; k++)
+      if (col[k] <= i) A[i][col[k]] = val[k];
+  }
+  int ok = 1;
+  for (int j = 1; j < n && ok; j++) {
+    for (int i = p[j] + 1; i < j; i++) {
+      int k0 = p[i] > p[j] ? p[i] : p[j];
+      for (int k = k0; k < i; k++) A[j][i] -= A[i][k] * A[j][k];
+    }
+    for (int k = p[j]; k < j; k++) {
+      if (fabs(A[k][k]) < 1e-12) { ok = 0; break; }
+      A[j][j] -= A[j][k] / A[k][k] * A[j][k];
+      A[j][k] /= A[k][k];
+    }
+  }
+  if (ok) {
+    for (int i = 0; i < n; i++) u[i] = f[i];
+    for (int i = 1; i < n; i++)
+      for (int k = p[i]; k < i; k++) u[i] -= A[i][k] * u[k];
+    for (int i = 0; i < n; i++) u[i] /= A[i][i];
+    for (int i = n - 1; i > 0; i--)
+      for (int k = p[i]; k < i; k++) u[k] -= A[i][k] * u[i];
+  }
+  free(p); free(A); free(store);
+  return ok;
+}

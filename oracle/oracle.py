@@ -1,0 +1,293 @@
+"""ctypes bindings of the CPU FP64 oracle (liboracle.so).
+
+TEST INFRASTRUCTURE ONLY: imported by tests, by ``__graft_entry__.smoke`` and
+by the CPU-baseline leg of ``bench.py`` as the checker of the HIP path.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB = None
+
+FORMS = {"linear": 0, "tl_svk": 1, "tl_neohooke": 2}
+SHAPES = {"hex8": 0, "bsp": 1}
+
+
+class EType(C.Structure):
+    _fields_ = [("shape", C.c_int), ("nn", C.c_int), ("deg", C.c_int * 3),
+                ("nip", C.c_int * 3), ("rational", C.c_int), ("form", C.c_int)]
+
+
+class OMesh(C.Structure):
+    _fields_ = [("nnode", C.c_int), ("nelem", C.c_int),
+                ("coords", C.c_void_p), ("conn", C.c_void_p),
+                ("dofmap", C.c_void_p), ("ext", C.c_void_p),
+                ("cpw", C.c_void_p), ("mat", C.c_void_p), ("et", EType)]
+
+
+def build():
+    subprocess.check_call(["make", "-s", "-C", _HERE, "liboracle.so"])
+
+
+def lib():
+    global _LIB
+    if _LIB is None:
+        path = os.path.join(_HERE, "liboracle.so")
+        if not os.path.exists(path):
+            build()
+        L = C.CDLL(path)
+        vp, i, d = C.c_void_p, C.c_int, C.c_double
+        L.orc_gauss.argtypes = [i, vp, vp]
+        L.orc_brick_points.argtypes = [vp, vp, vp]
+        L.orc_hex8_shape.argtypes = [d, d, d, vp, vp]
+        L.orc_bernstein.argtypes = [i, d, vp]
+        L.orc_bernstein_deriv.argtypes = [i, d, vp]
+        L.orc_bezier_extraction.argtypes = [i, i, vp, vp]
+        L.orc_shape_dxyz.argtypes = [vp, vp, vp, vp, vp, vp, vp]
+        L.orc_elem_stiff.argtypes = [vp, vp, vp, vp, vp, vp, vp]
+        L.orc_elem_intforce.argtypes = [vp, vp, vp, vp, vp, vp, vp]
+        L.orc_assemble_K.argtypes = [vp, vp, i, vp, vp, vp, i, i, i]
+        L.orc_assemble_fint.argtypes = [vp, vp, i, vp, i, i, i]
+        L.orc_csr_count.argtypes = [vp, i, vp]
+        L.orc_csr_count.restype = C.c_longlong
+        L.orc_csr_fill.argtypes = [vp, i, vp, vp]
+        L.orc_pcg.argtypes = [i, vp, vp, vp, vp, vp, i, d, vp, vp]
+        L.orc_spmv.argtypes = [i, vp, vp, vp, vp, vp]
+        L.orc_skyline_solve.argtypes = [i, vp, vp, vp, vp, vp]
+        _LIB = L
+    return _LIB
+
+
+def _p(a):
+    return a.ctypes.data if a is not None else None
+
+
+def gauss(n):
+    r, w = np.zeros(n), np.zeros(n)
+    assert lib().orc_gauss(n, _p(r), _p(w))
+    return r, w
+
+
+def brick_points(nip):
+    nip = np.asarray(nip, dtype=np.int32)
+    n = int(np.prod(nip))
+    rst, w = np.zeros((n, 3)), np.zeros(n)
+    lib().orc_brick_points(_p(nip), _p(rst), _p(w))
+    return rst, w
+
+
+def bezier_extraction(p, knots):
+    U = np.ascontiguousarray(knots, dtype=np.float64)
+    ns = int(np.sum(np.abs(np.diff(U)) > 1e-6))
+    Cm = np.zeros((ns, p + 1, p + 1))
+    lib().orc_bezier_extraction(p, len(U), _p(U), _p(Cm))
+    return Cm
+
+
+def bernstein(p, u, deriv=False):
+    B = np.zeros(p + 1)
+    (lib().orc_bernstein_deriv if deriv else lib().orc_bernstein)(p, u, _p(B))
+    return B
+
+
+def hex8_shape(r, s, t):
+    N, dN = np.zeros(8), np.zeros((8, 3))
+    lib().orc_hex8_shape(r, s, t, _p(N), _p(dN))
+    return N, dN
+
+
+class OracleMesh:
+    """Holds the C views of a ``fast_amd.mesh.Mesh`` for the oracle."""
+
+    def __init__(self, mesh):
+        self.mesh = mesh
+        self._keep = []
+
+        def c(a, dt):
+            a = np.ascontiguousarray(a, dtype=dt)
+            self._keep.append(a)
+            return a
+
+        self.coords = c(mesh.coords, np.float64)
+        self.conn = c(mesh.conn, np.int32)
+        self.dofmap = c(mesh.dofmap, np.int32)
+        self.mat = c(mesh.mat, np.float64)
+        self.ext = c(mesh.ext, np.float64) if mesh.ext is not None else None
+        self.cpw = c(mesh.cpw, np.float64) if mesh.cpw is not None else None
+        et = EType()
+        et.shape = SHAPES[mesh.shape]
+        et.nn = mesh.nn
+        for d in range(3):
+            et.deg[d] = mesh.deg[d]
+            et.nip[d] = mesh.nip[d]
+        et.rational = 1
+        et.form = FORMS[mesh.form]
+        self.et = et
+        om = OMesh()
+        om.nnode, om.nelem = mesh.nnode, mesh.nelem
+        om.coords, om.conn, om.dofmap = _p(self.coords), _p(self.conn), _p(self.dofmap)
+        om.ext, om.cpw, om.mat = _p(self.ext), _p(self.cpw), _p(self.mat)
+        om.et = et
+        self.om = om
+        self._pattern = None
+
+    # ---- element level
+    def elem_data(self, e):
+        m = self.mesh
+        X = np.ascontiguousarray(m.coords[m.conn[e]])
+        ext = np.ascontiguousarray(m.ext[e]) if m.ext is not None else None
+        w = (np.ascontiguousarray(m.cpw[m.conn[e]]) if m.cpw is not None
+             else np.ones(m.nn))
+        return X, ext, w, np.ascontiguousarray(m.mat[e])
+
+    def elem_stiff(self, e, u_nodal=None):
+        X, ext, w, mat = self.elem_data(e)
+        nd = 3 * self.mesh.nn
+        K = np.zeros((nd, nd))
+        ue = None
+        if u_nodal is not None:
+            ue = np.ascontiguousarray(np.asarray(u_nodal)[self.mesh.conn[e]].ravel())
+        ok = lib().orc_elem_stiff(C.byref(self.et), _p(X), _p(ext), _p(w),
+                                  _p(mat), _p(ue), _p(K))
+        return K, ok
+
+    def elem_intforce(self, e, u_nodal):
+        X, ext, w, mat = self.elem_data(e)
+        nd = 3 * self.mesh.nn
+        g = np.zeros(nd)
+        ue = np.ascontiguousarray(np.asarray(u_nodal)[self.mesh.conn[e]].ravel())
+        ok = lib().orc_elem_intforce(C.byref(self.et), _p(X), _p(ext), _p(w),
+                                     _p(mat), _p(ue), _p(g))
+        return g, ok
+
+    def shape_dxyz(self, e, rst):
+        X, ext, w, _ = self.elem_data(e)
+        d = np.zeros((self.mesh.nn, 3))
+        det = np.zeros(1)
+        rst = np.ascontiguousarray(rst, dtype=np.float64)
+        lib().orc_shape_dxyz(C.byref(self.et), _p(X), _p(ext), _p(w), _p(rst),
+                             _p(det), _p(d))
+        return det[0], d
+
+    # ---- mesh level
+    def pattern(self):
+        if self._pattern is None:
+            neq = self.mesh.neq
+            rowptr = np.zeros(neq + 1, dtype=np.int64)
+            nnz = lib().orc_csr_count(C.byref(self.om), neq, _p(rowptr))
+            col = np.zeros(max(nnz, 1), dtype=np.int32)
+            lib().orc_csr_fill(C.byref(self.om), neq, _p(rowptr), _p(col))
+            self._pattern = (rowptr, col[:nnz])
+        return self._pattern
+
+    def assemble_K(self, u_nodal=None, nthreads=1, elem_range=(0, -1)):
+        rowptr, col = self.pattern()
+        val = np.zeros(len(col))
+        u = (np.ascontiguousarray(u_nodal, dtype=np.float64)
+             if u_nodal is not None else None)
+        ok = lib().orc_assemble_K(C.byref(self.om), _p(u), self.mesh.neq,
+                                  _p(rowptr), _p(col), _p(val), nthreads,
+                                  elem_range[0], elem_range[1])
+        return val, ok
+
+    def assemble_fint(self, u_nodal, nthreads=1, elem_range=(0, -1)):
+        g = np.zeros(self.mesh.neq)
+        u = np.ascontiguousarray(u_nodal, dtype=np.float64)
+        ok = lib().orc_assemble_fint(C.byref(self.om), _p(u), self.mesh.neq,
+                                     _p(g), nthreads, elem_range[0],
+                                     elem_range[1])
+        return g, ok
+
+    def to_scipy(self, val):
+        import scipy.sparse as sp
+        rowptr, col = self.pattern()
+        n = self.mesh.neq
+        return sp.csr_matrix((val, col, rowptr), shape=(n, n))
+
+
+def pcg(rowptr, col, val, f, maxit, tol):
+    n = len(f)
+    u = np.zeros(n)
+    it = np.zeros(1, dtype=np.int32)
+    rr = np.zeros(1)
+    f = np.ascontiguousarray(f, dtype=np.float64)
+    conv = lib().orc_pcg(n, _p(rowptr), _p(col), _p(val), _p(f), _p(u), maxit,
+                         tol, _p(it), _p(rr))
+    return u, int(it[0]), float(rr[0]), conv
+
+
+def skyline_solve(rowptr, col, val, f):
+    n = len(f)
+    u = np.zeros(n)
+    f = np.ascontiguousarray(f, dtype=np.float64)
+    ok = lib().orc_skyline_solve(n, _p(rowptr), _p(col), _p(val), _p(f), _p(u))
+    return u, ok
+
+
+def spmv(rowptr, col, val, x):
+    y = np.zeros(len(rowptr) - 1)
+    x = np.ascontiguousarray(x, dtype=np.float64)
+    lib().orc_spmv(len(y), _p(rowptr), _p(col), _p(val), _p(x), _p(y))
+    return y
+
+
+def linear_static(om, f, solver="skyline", maxit=100000, tol=1e-10):
+    """Reference: ctrl/ctrllins.cpp:46-99 (assemble K, solve, no MPCs)."""
+    val, ok = om.assemble_K()
+    rowptr, col = om.pattern()
+    if solver == "skyline":
+        u, ok2 = skyline_solve(rowptr, col, val, f)
+        return u
+    u, it, rr, conv = pcg(rowptr, col, val, f, maxit, tol)
+    return u
+
+
+def newton_raphson(om, q, steps, factor, maxit, tol, solver="skyline",
+                   pcg_tol=1e-5, on_iter=None):
+    """Load-control Newton-Raphson of ctrl/ctrlnr.cpp:47-267.
+
+    f = lambda*q per step; full Newton with K rebuilt every iteration;
+    convergence |r| / max(1, |f|) < tol (ctrl/ctrl.cpp:1366-1380).  Returns the
+    per-step equation vectors and the per-step iteration counts.
+    """
+    mesh = om.mesh
+    n = mesh.neq
+    u = np.zeros(n)
+    g, _ = om.assemble_fint(mesh.scatter(u))
+    lam = 0.0
+    us, its = [], []
+    rowptr, col = om.pattern()
+    for step in range(1, steps + 1):
+        lam += factor
+        f = lam * q
+        flen = np.linalg.norm(f)
+        r = f - g
+        conv = False
+        for it in range(1, maxit + 1):
+            val, _ = om.assemble_K(mesh.scatter(u))
+            if solver == "skyline":
+                du, _ = skyline_solve(rowptr, col, val, r)
+            else:
+                du, _, _, _ = pcg(rowptr, col, val, r, n, pcg_tol)
+            u = u + du
+            g, ok = om.assemble_fint(mesh.scatter(u))
+            if not ok:
+                break
+            r = f - g
+            ref = flen if flen > tol else np.linalg.norm(g)
+            err = np.linalg.norm(r) / max(1.0, ref)
+            if on_iter:
+                on_iter(step, it, err)
+            if err < tol:
+                conv = True
+                break
+        us.append(u.copy())
+        its.append(it)
+        if not conv:
+            break
+    return us, its
