@@ -1,0 +1,190 @@
+"""Benchmark: element integration + assembly throughput on one MI355X.
+
+Metric: elements/s over one step = global stiffness (element K_e on MFMA +
+deterministic assembly into the FP64 CSR) + internal force f_int (FP64), with
+inputs resident in HBM.  Default workload: cubic B-spline solid patch
+32x32x8 elements (open uniform knots, control points on the unit box moved by
+seeded U(-0.05, 0.05) h), 4x4x4 Gauss, E = 1, nu = 0.3.
+``--workload hex8`` runs the 64^3 neo-Hookean hex8 block instead.
+
+Prints ONE JSON line (rank 0).  Multi-GPU runs are not part of this build.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+PEAK_FP32_MFMA = 157.3  # TFLOP/s, MI355X dense FP32 matrix
+PEAK_HBM = 8000.0       # GB/s
+
+
+def make_workload(name):
+    from fast_amd import mesh as M
+    if name == "iga":
+        m = M.bsp_block(ne=(32, 32, 8), p=3, perturb=0.05, seed=0, E=1.0, nu=0.3)
+        cfg = {"workload": "iga_cubic_bspline_32x32x8", "elements": m.nelem, "nqp": 64,
+               "ndof_elem": 192, "neq": int(m.neq), "E": 1.0, "nu": 0.3, "gauss": "4x4x4",
+               "perturb": "U(-0.05,0.05)*h seed 0"}
+        X = m.coords
+        u = 1e-3 * np.stack([np.sin(3 * X[:, 0]) * X[:, 2], np.cos(2 * X[:, 1]) * X[:, 0],
+                             X[:, 0] * X[:, 1]], 1)
+    else:
+        n = 64
+        m = M.hex8_block(n, n, n, form="tl_neohooke", mat=np.tile((1.0, 1.5), (n ** 3, 1)),
+                         traction=(0.0, 0.0, -0.2))
+        cfg = {"workload": "hex8_neohooke_64x64x64", "elements": m.nelem, "nqp": 8,
+               "ndof_elem": 24, "neq": int(m.neq), "mu": 1.0, "lambda": 1.5, "gauss": "2x2x2"}
+        X = m.coords
+        u = np.stack([0.02 * X[:, 0] * X[:, 2], 0.01 * X[:, 1] ** 2, -0.03 * X[:, 0] ** 2], 1)
+    u[m.fixed] = 0.0
+    return m, cfg, u
+
+
+# algorithmic flops of the element-stiffness kernel per element
+def kernel_flops(m):
+    if m.shape == "bsp":
+        # 21 MFMA output tiles of 32x32 with K = 64 points: P_ij = G_i^T W G_j
+        return 21 * 32 * 32 * 64 * 2
+    return None
+
+
+def cpu_baseline(m, u, budget_s=12.0):
+    """The oracle (C restatement of the reference loops) on a bounded sample of
+    the same workload: K_e + assembly + f_int for the first elements."""
+    from oracle import oracle as O
+    om = O.OracleMesh(m)
+    om.pattern()
+    ncores = os.cpu_count() or 1
+    res = {}
+    for nt in (1, ncores):
+        ne = 8
+        while True:
+            t0 = time.perf_counter()
+            om.assemble_K(u, nthreads=nt, elem_range=(0, ne))
+            om.assemble_fint(u, nthreads=nt, elem_range=(0, ne))
+            dt = time.perf_counter() - t0
+            if dt > budget_s / 4 or ne >= m.nelem:
+                break
+            ne = min(m.nelem, ne * 4)
+        res[nt] = (ne / dt, ne, dt)
+    return res, ncores
+
+
+def accuracy(model, m, u, nsample=32):
+    """Relative error of K_e u_e (FP32 path) against the FP64 oracle on a
+    sample of elements, normwise and forward."""
+    from oracle import oracle as O
+    om = O.OracleMesh(m)
+    idx = np.linspace(0, m.nelem - 1, nsample).astype(int)
+    Kg = model.element_matrices(0, m.nelem) if m.nelem * (3 * m.nn) ** 2 < 2e8 else None
+    num = den = 0.0
+    worst_fw = 0.0
+    for e in idx:
+        K0, _ = om.elem_stiff(int(e), u if m.form != "linear" else None)
+        Ke = (Kg[e] if Kg is not None else model.element_matrices(int(e), 1)[0]).astype(np.float64)
+        ue = u[m.conn[e]].ravel()
+        d = (Ke - K0) @ ue
+        num += d @ d
+        den += (np.linalg.norm(K0) * np.linalg.norm(ue)) ** 2
+        worst_fw = max(worst_fw, np.linalg.norm(d) / max(np.linalg.norm(K0 @ ue), 1e-300))
+    return float(np.sqrt(num / den)), float(worst_fw)
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--workload", default="iga", choices=["iga", "hex8"])
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py: this build measures one GPU only (multi-GPU runs are not part of "
+              "it); run with --gpus 1", file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+    assert torch.cuda.is_available(), "bench.py needs a HIP device"
+    torch.cuda.set_device(0)
+    from fast_amd import fgpu
+
+    m, cfg, u = make_workload(args.workload)
+    print("bench: workload %s nelem %d neq %d" % (cfg["workload"], m.nelem, m.neq),
+          file=sys.stderr, flush=True)
+    model = fgpu.GpuModel(m)
+    L = fgpu.lib()
+    model.stiffness(u)                     # uploads u into the context
+    d_u = model.device_buffer(2)
+    g = torch.zeros(model.neq, dtype=torch.float64, device="cuda")
+
+    def step():
+        if not L.fgpu_assemble_K_dev(model.h, d_u):
+            raise fgpu.FgpuError(L.fgpu_last_error(model.h).decode())
+        tk, ta = L.fgpu_last_time_ms(model.h, 0), L.fgpu_last_time_ms(model.h, 1)
+        if not L.fgpu_int_force_dev(model.h, d_u, g.data_ptr()):
+            raise fgpu.FgpuError(L.fgpu_last_error(model.h).decode())
+        tf = L.fgpu_last_time_ms(model.h, 2)
+        return tk, ta, tf
+
+    for _ in range(args.warmup):
+        step()
+    torch.cuda.synchronize()
+    times = []
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        times.append(step())
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    times = np.array(times)
+    tk, ta, tf = times.mean(axis=0)
+    value = m.nelem * args.steps / wall
+    fl = kernel_flops(m)
+    roof = None
+    if fl:
+        achieved = fl * m.nelem / (tk * 1e-3) / 1e12
+        roof = {"bound": "mfma", "achieved": round(achieved, 2), "peak": PEAK_FP32_MFMA,
+                "unit": "TFLOP/s", "frac": round(achieved / PEAK_FP32_MFMA, 4), "traffic": None,
+                "kernel": "bsp_K_kernel", "flops_per_elem": fl,
+                "kernel_ms": round(float(tk), 4)}
+    else:
+        # hex8: element kernel bound by its FP32 element-matrix stores
+        bytes_per = 36 * 9 * 4
+        achieved = bytes_per * m.nelem / (tk * 1e-3) / 1e9
+        roof = {"bound": "hbm", "achieved": round(achieved, 1), "peak": PEAK_HBM,
+                "unit": "GB/s", "frac": round(achieved / PEAK_HBM, 4), "traffic": None,
+                "kernel": "hex8_K_kernel", "bytes_per_elem": bytes_per,
+                "kernel_ms": round(float(tk), 4)}
+    nrm, fwd = accuracy(model, m, u)
+    cpu = None
+    if not args.no_cpu:
+        res, ncores = cpu_baseline(m, u)
+        v, ne, dt = res[ncores]
+        v1 = res[1]
+        cpu = {"value": round(v, 2), "unit": "elements/s", "cores": ncores, "kind": "port",
+               "sample": "oracle K_e+assembly+f_int over the first %d elements (%.1f s) of the "
+                         "same workload" % (ne, dt),
+               "single_thread": round(v1[0], 2)}
+    out = {
+        "metric": "elements/s (K+f_int+assembly)",
+        "value": round(value, 1), "unit": "elements/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": round(wall / args.steps * 1e3, 4),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "fp32-mfma/f64",
+        "data": "synthetic (seeded generated mesh)", "config": cfg,
+        "phase_ms": {"element_K": round(float(tk), 4), "assembly": round(float(ta), 4),
+                     "f_int": round(float(tf), 4)},
+        "accuracy": {"Ku_normwise_rel": nrm, "Ku_forward_rel_max": fwd},
+        "roofline": roof, "cpu_baseline": cpu,
+    }
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

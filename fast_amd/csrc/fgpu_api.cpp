@@ -1,0 +1,579 @@
+// fgpu_api.cpp - C-ABI entry points and host-side set-up of libfgpu.
+//
+// Set-up replaces the reference's equation numbering and matrix creation
+// (ctrl/ctrl.cpp:2004-2157 InitDofs/CompEqs/CompProf, mlib/sysmat.cpp:42-70
+// CreateMatrix): it builds, once per mesh,
+//   * the full symmetric CSR pattern over equations (rows of one node share
+//     their column list: the free directions of every neighbour node),
+//   * the deterministic assembly map: for every upper node block (A <= B) the
+//     segment of element blocks that add into it, sorted by element id
+//     (the order in which ctrl.cpp:945-956 adds element matrices), and
+//   * the node -> (element, local node) lists of the internal-force gather.
+// All integer maps are exact; no floating-point atomics are used anywhere.
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <numeric>
+#include <thread>
+
+#include "fgpu_internal.h"
+
+using namespace fgpu;
+
+namespace {
+
+template <class T>
+T *dalloc(size_t n) {
+  T *p = nullptr;
+  if (n == 0) n = 1;
+  FGPU_CHECK(hipMalloc(&p, n * sizeof(T)));
+  return p;
+}
+
+template <class T>
+T *dupload(const T *h, size_t n, hipStream_t s) {
+  T *d = dalloc<T>(n);
+  if (n) FGPU_CHECK(hipMemcpyAsync(d, h, n * sizeof(T), hipMemcpyHostToDevice, s));
+  return d;
+}
+
+template <class F>
+void parallel_for(long n, F f) {
+  unsigned nt = std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
+  if (n < 4096) nt = 1;
+  std::vector<std::thread> th;
+  long chunk = (n + nt - 1) / nt;
+  for (unsigned t = 0; t < nt; t++) {
+    long b = t * chunk, e = std::min(n, b + chunk);
+    if (b >= e) break;
+    th.emplace_back([=, &f] {
+      for (long i = b; i < e; i++) f(i);
+    });
+  }
+  for (auto &x : th) x.join();
+}
+
+void gauss_rule(int n, double *r, double *w) {
+  // Gauss-Legendre by Newton iteration on P_n, rounded to 15 decimals as in
+  // the reference's quadrature table (elc/intpoint.cpp:741-920).
+  const long double PI = 3.141592653589793238462643383279502884L;
+  for (int i = 0; i < n; i++) {
+    long double x = -cosl(PI * (i + 0.75L) / (n + 0.5L)), dp = 1.0L;
+    for (int it = 0; it < 100 && n > 1; it++) {
+      long double p0 = 1.0L, p1 = x;
+      for (int k = 2; k <= n; k++) {
+        long double p2 = ((2 * k - 1) * x * p1 - (k - 1) * p0) / k;
+        p0 = p1;
+        p1 = p2;
+      }
+      dp = n * (x * p1 - p0) / (x * x - 1.0L);
+      long double dx = p1 / dp;
+      x -= dx;
+      if (fabsl(dx) < 1e-30L) break;
+    }
+    if (n == 1) { x = 0.0L; dp = 1.0L; }
+    char buf[64];
+    snprintf(buf, sizeof buf, "%.15Lf", x);
+    r[i] = strtod(buf, nullptr);
+    if (r[i] == 0.0) r[i] = 0.0;
+    snprintf(buf, sizeof buf, "%.15Lf", 2.0L / ((1.0L - x * x) * dp * dp));
+    w[i] = strtod(buf, nullptr);
+  }
+}
+
+void build_maps(Ctx &c, const int *conn) {
+  const int nn = c.nn, nnode = c.nnode, nelem = c.nelem;
+  const int nsym = nsym_blocks(nn);
+  // node -> (element, local) lists, element order
+  std::vector<int> nptr(nnode + 1, 0);
+  for (long i = 0; i < (long)nelem * nn; i++) nptr[conn[i] + 1]++;
+  for (int i = 0; i < nnode; i++) nptr[i + 1] += nptr[i];
+  std::vector<int> nadj((size_t)nelem * nn);
+  {
+    std::vector<int> pos(nptr.begin(), nptr.end() - 1);
+    for (int e = 0; e < nelem; e++)
+      for (int a = 0; a < nn; a++) nadj[pos[conn[(size_t)e * nn + a]]++] = e * nn + a;
+  }
+  std::vector<int> nfree(nnode, 0);
+  for (int n = 0; n < nnode; n++)
+    for (int d = 0; d < 3; d++) nfree[n] += c.eqmap[3 * n + d] >= 0;
+
+  // per node: sorted unique neighbour list (kept, CSR-like), the column
+  // offset of each neighbour inside the node's rows, upper-block and
+  // contribution counts.
+  auto neighbours = [&](int A, std::vector<int> &nb) {
+    nb.clear();
+    for (int t = nptr[A]; t < nptr[A + 1]; t++) {
+      int e = nadj[t] / nn;
+      for (int b = 0; b < nn; b++) nb.push_back(conn[(size_t)e * nn + b]);
+    }
+    std::sort(nb.begin(), nb.end());
+    nb.erase(std::unique(nb.begin(), nb.end()), nb.end());
+  };
+  std::vector<long long> nbcnt(nnode + 1, 0), nc_up(nnode);
+  parallel_for(nnode, [&](long A) {
+    thread_local std::vector<int> nb;
+    neighbours((int)A, nb);
+    nbcnt[A + 1] = (long long)nb.size();
+    long long ncon = 0;
+    for (int t = nptr[A]; t < nptr[A + 1]; t++) {
+      int e = nadj[t] / nn;
+      for (int b = 0; b < nn; b++) ncon += conn[(size_t)e * nn + b] >= A;
+    }
+    nc_up[A] = ncon;
+  });
+  for (int n = 0; n < nnode; n++) nbcnt[n + 1] += nbcnt[n];
+  std::vector<int> nbidx(nbcnt[nnode]), nbcol(nbcnt[nnode]);
+  std::vector<long long> rowlen(nnode), nb_up(nnode);
+  parallel_for(nnode, [&](long A) {
+    thread_local std::vector<int> nb;
+    neighbours((int)A, nb);
+    long long base = nbcnt[A];
+    int off = 0;
+    long long up = 0;
+    for (size_t k = 0; k < nb.size(); k++) {
+      nbidx[base + k] = nb[k];
+      nbcol[base + k] = off;
+      off += nfree[nb[k]];
+      up += nb[k] >= A;
+    }
+    rowlen[A] = off;
+    nb_up[A] = up;
+  });
+  auto col_offset = [&](int A, int B) {  // column offset of B in the rows of A
+    const int *b0 = nbidx.data() + nbcnt[A], *b1 = nbidx.data() + nbcnt[A + 1];
+    const int *it = std::lower_bound(b0, b1, B);
+    return nbcol[nbcnt[A] + (it - b0)];
+  };
+  // CSR row pointers in equation order (equations are node-major)
+  std::vector<long long> rowptr(c.neq + 1, 0);
+  for (int n = 0; n < nnode; n++)
+    for (int d = 0; d < 3; d++) {
+      int q = c.eqmap[3 * n + d];
+      if (q >= 0) rowptr[q + 1] = rowlen[n];
+    }
+  for (int q = 0; q < c.neq; q++) rowptr[q + 1] += rowptr[q];
+  c.nnz = rowptr[c.neq];
+  std::vector<long long> bptr(nnode + 1, 0), cptr0(nnode + 1, 0);
+  for (int n = 0; n < nnode; n++) {
+    bptr[n + 1] = bptr[n] + nb_up[n];
+    cptr0[n + 1] = cptr0[n] + nc_up[n];
+  }
+  c.nblk = bptr[nnode];
+  c.ncontrib = cptr0[nnode];
+  if (c.ncontrib >= (1LL << 31) || (long long)nelem * nsym * 2 >= (1LL << 31))
+    throw Error("mesh too large for 32-bit assembly codes");
+  std::vector<int> col(c.nnz);
+  std::vector<long long> diag(c.neq);
+  std::vector<int> blkA(c.nblk), blkB(c.nblk), offAB(c.nblk), offBA(c.nblk);
+  std::vector<long long> cptr(c.nblk + 1);
+  std::vector<int> contrib(c.ncontrib);
+  cptr[c.nblk] = c.ncontrib;
+
+  parallel_for(nnode, [&](long Al) {
+    int A = (int)Al;
+    thread_local std::vector<std::pair<int, int>> items;
+    // column list of the rows of A (identical for its free directions)
+    long long nb0 = nbcnt[A], nb1 = nbcnt[A + 1];
+    int self = 0;
+    for (int d = 0; d < 3; d++) {
+      int q = c.eqmap[3 * A + d];
+      if (q < 0) continue;
+      long long p = rowptr[q];
+      for (long long k = nb0; k < nb1; k++) {
+        int B = nbidx[k];
+        for (int dd = 0; dd < 3; dd++) {
+          int qq = c.eqmap[3 * B + dd];
+          if (qq >= 0) col[p++] = qq;
+        }
+      }
+      diag[q] = rowptr[q] + col_offset(A, A) + self;
+      self++;
+    }
+    // contributions of upper blocks, sorted by (B, element, local a, local b)
+    items.clear();
+    for (int t = nptr[A]; t < nptr[A + 1]; t++) {
+      int e = nadj[t] / nn, a = nadj[t] % nn;
+      for (int b = 0; b < nn; b++) {
+        int B = conn[(size_t)e * nn + b];
+        if (B < A) continue;
+        int lo = std::min(a, b), hi = std::max(a, b);
+        int code = ((e * nsym + sym_index(lo, hi, nn)) << 1) | (a > b ? 1 : 0);
+        items.emplace_back(B, code);
+      }
+    }
+    std::stable_sort(items.begin(), items.end(),
+                     [](const std::pair<int, int> &x, const std::pair<int, int> &y) {
+                       return x.first < y.first;
+                     });
+    long long bk = bptr[A], cc = cptr0[A];
+    size_t i = 0;
+    while (i < items.size()) {
+      int B = items[i].first;
+      blkA[bk] = A;
+      blkB[bk] = B;
+      offAB[bk] = col_offset(A, B);
+      offBA[bk] = col_offset(B, A);
+      cptr[bk] = cc;
+      while (i < items.size() && items[i].first == B) contrib[cc++] = items[i++].second;
+      bk++;
+    }
+  });
+
+  hipStream_t s = c.stream;
+  c.d_rowptr = dupload(rowptr.data(), rowptr.size(), s);
+  c.d_col = dupload(col.data(), col.size(), s);
+  c.d_diag = dupload(diag.data(), diag.size(), s);
+  c.d_val = dalloc<double>(c.nnz);
+  c.d_blkA = dupload(blkA.data(), blkA.size(), s);
+  c.d_blkB = dupload(blkB.data(), blkB.size(), s);
+  c.d_offAB = dupload(offAB.data(), offAB.size(), s);
+  c.d_offBA = dupload(offBA.data(), offBA.size(), s);
+  c.d_cptr = dupload(cptr.data(), cptr.size(), s);
+  c.d_contrib = dupload(contrib.data(), contrib.size(), s);
+  c.d_nptr = dupload(nptr.data(), nptr.size(), s);
+  c.d_nadj = dupload(nadj.data(), nadj.size(), s);
+  FGPU_CHECK(hipStreamSynchronize(s));
+}
+
+void destroy(Ctx *c) {
+  if (!c) return;
+  void *bufs[] = {c->d_coords, c->d_conn, c->d_eqmap, c->d_mat, c->d_ext,
+                  c->d_cpw, c->d_gauss, c->d_gauss64, c->d_Ke, c->d_ge,
+                  c->d_status, c->d_rowptr, c->d_col, c->d_val, c->d_diag,
+                  c->d_blkA, c->d_blkB, c->d_offAB, c->d_offBA, c->d_cptr,
+                  c->d_contrib, c->d_nptr, c->d_nadj, c->d_u, c->d_b, c->d_x,
+                  c->d_r, c->d_p, c->d_w, c->d_z, c->d_dinv, c->d_part,
+                  c->d_scal, c->d_done};
+  for (void *p : bufs)
+    if (p) (void)hipFree(p);
+  if (c->h_done) (void)hipHostFree(c->h_done);
+  if (c->h_scal) (void)hipHostFree(c->h_scal);
+  for (auto &e : c->ev)
+    if (e) (void)hipEventDestroy(e);
+  if (c->stream) (void)hipStreamDestroy(c->stream);
+  delete c;
+}
+
+template <class F>
+int guarded(fgpu_ctx *ctx, F f) {
+  Ctx *c = reinterpret_cast<Ctx *>(ctx);
+  if (!c) return 0;
+  try {
+    return f(*c);
+  } catch (const Error &e) {
+    c->err = e.msg;
+  } catch (const std::exception &e) {
+    c->err = e.what();
+  }
+  return 0;
+}
+
+float elapsed(Ctx &c, int a, int b) {
+  float ms = 0;
+  FGPU_CHECK(hipEventElapsedTime(&ms, c.ev[a], c.ev[b]));
+  return ms;
+}
+
+int check_status(Ctx &c, const char *what) {
+  int st[2];
+  FGPU_CHECK(hipMemcpyAsync(st, c.d_status, sizeof st, hipMemcpyDeviceToHost, c.stream));
+  FGPU_CHECK(hipStreamSynchronize(c.stream));
+  if (st[0]) {
+    c.err = std::string(what) + ": " + std::to_string(st[0]) +
+            " integration points with non-positive Jacobian";
+    return 0;
+  }
+  if (st[1]) {
+    c.err = std::string(what) + ": stress update failed at " +
+            std::to_string(st[1]) + " integration points (det C <= 0)";
+    return 0;
+  }
+  return 1;
+}
+
+}  // namespace
+
+extern "C" {
+
+int fgpu_create(fgpu_ctx **out, const fgpu_mesh *m) {
+  if (!out || !m) return 0;
+  *out = nullptr;
+  Ctx *c = new Ctx();
+  try {
+    if (m->shape != FGPU_HEX8 && m->shape != FGPU_BSP) throw Error("unknown shape");
+    if (m->form < FGPU_LINEAR || m->form > FGPU_TL_NEOHOOKE) throw Error("unknown form");
+    if (m->shape == FGPU_HEX8 && m->nn != 8) throw Error("hex8 needs nn = 8");
+    if (m->shape == FGPU_BSP) {
+      if (m->nn != (m->deg[0] + 1) * (m->deg[1] + 1) * (m->deg[2] + 1))
+        throw Error("B-spline solid: nn != prod(p+1)");
+      if (m->deg[0] != 3 || m->deg[1] != 3 || m->deg[2] != 3 || m->nip[0] != 4 ||
+          m->nip[1] != 4 || m->nip[2] != 4)
+        throw Error("B-spline solid kernels support degree 3 with 4x4x4 Gauss");
+      if (m->form != FGPU_LINEAR) throw Error("B-spline solid kernels support the linear form");
+      if (!m->ext) throw Error("B-spline solid needs extraction operators");
+    } else {
+      for (int d = 0; d < 3; d++)
+        if (m->nip[d] != m->nip[0] || m->nip[0] < 1 || m->nip[0] > 3)
+          throw Error("hex8 kernels support isotropic Gauss orders 1..3");
+    }
+    if (m->nnode <= 0 || m->nelem <= 0) throw Error("empty mesh");
+    c->nnode = m->nnode;
+    c->nelem = m->nelem;
+    c->shape = m->shape;
+    c->nn = m->nn;
+    c->form = m->form;
+    for (int d = 0; d < 3; d++) {
+      c->deg[d] = m->deg[d];
+      c->nip[d] = m->nip[d];
+    }
+    for (long i = 0; i < (long)m->nelem * m->nn; i++)
+      if (m->conn[i] < 0 || m->conn[i] >= m->nnode) throw Error("connectivity out of range");
+    c->eqmap.resize((size_t)m->nnode * 3);
+    int neq = 0;
+    for (long i = 0; i < (long)m->nnode * 3; i++) {
+      c->eqmap[i] = m->dofmap[i] - 1;
+      neq = std::max(neq, m->dofmap[i]);
+    }
+    c->neq = neq;
+    FGPU_CHECK(hipGetDevice(&c->device));
+    FGPU_CHECK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+    for (auto &e : c->ev) FGPU_CHECK(hipEventCreate(&e));
+    hipStream_t s = c->stream;
+    c->d_coords = dupload(m->coords, (size_t)m->nnode * 3, s);
+    c->d_conn = dupload(m->conn, (size_t)m->nelem * m->nn, s);
+    c->d_eqmap = dupload(c->eqmap.data(), c->eqmap.size(), s);
+    c->d_mat = dupload(m->mat, (size_t)m->nelem * 2, s);
+    double gp[3 * 2 * 16];
+    float gpf[3 * 2 * 16];
+    for (int d = 0; d < 3; d++) {
+      double r[16], w[16];
+      gauss_rule(m->nip[d], r, w);
+      for (int k = 0; k < m->nip[d]; k++) {
+        gp[(d * 16 + k) * 2 + 0] = r[k];
+        gp[(d * 16 + k) * 2 + 1] = w[k];
+        gpf[(d * 16 + k) * 2 + 0] = (float)r[k];
+        gpf[(d * 16 + k) * 2 + 1] = (float)w[k];
+      }
+    }
+    c->d_gauss64 = dupload(gp, 3 * 2 * 16, s);
+    c->d_gauss = dupload(gpf, 3 * 2 * 16, s);
+    c->d_status = dalloc<int>(2);
+    build_maps(*c, m->conn);
+    int nsym = nsym_blocks(c->nn);
+    c->kel = KeLayout{c->shape == FGPU_HEX8 ? 64 : 1, nsym};
+    size_t ntile = (size_t)(c->nelem + c->kel.T - 1) / c->kel.T;
+    c->d_Ke = dalloc<float>(ntile * c->kel.T * (size_t)nsym * 9);
+    c->d_ge = dalloc<double>((size_t)c->nelem * c->nn * 3);
+    if (c->shape == FGPU_BSP) bsp_prepare_tables(*c, m->ext, m->cpw);
+    c->d_u = dalloc<double>((size_t)c->nnode * 3);
+    size_t n = (size_t)std::max(c->neq, 1);
+    c->d_b = dalloc<double>(n);
+    c->d_x = dalloc<double>(n);
+    c->d_r = dalloc<double>(n);
+    c->d_p = dalloc<double>(n);
+    c->d_w = dalloc<double>(n);
+    c->d_z = dalloc<double>(n);
+    c->d_dinv = dalloc<double>(n);
+    c->nblocks_red = 1024;
+    c->d_part = dalloc<double>(4 * c->nblocks_red);
+    c->d_scal = dalloc<double>(16);
+    c->d_done = dalloc<int>(4);
+    FGPU_CHECK(hipHostMalloc(&c->h_done, 4 * sizeof(int)));
+    FGPU_CHECK(hipHostMalloc(&c->h_scal, 16 * sizeof(double)));
+    FGPU_CHECK(hipStreamSynchronize(s));
+  } catch (const Error &e) {
+    fprintf(stderr, "fgpu_create: %s\n", e.msg.c_str());
+    destroy(c);
+    return 0;
+  }
+  *out = reinterpret_cast<fgpu_ctx *>(c);
+  return 1;
+}
+
+void fgpu_destroy(fgpu_ctx *ctx) { destroy(reinterpret_cast<Ctx *>(ctx)); }
+
+const char *fgpu_last_error(fgpu_ctx *ctx) {
+  Ctx *c = reinterpret_cast<Ctx *>(ctx);
+  return c ? c->err.c_str() : "null context";
+}
+
+int fgpu_neq(fgpu_ctx *ctx) { return ctx ? reinterpret_cast<Ctx *>(ctx)->neq : 0; }
+
+long long fgpu_nnz(fgpu_ctx *ctx) { return ctx ? reinterpret_cast<Ctx *>(ctx)->nnz : 0; }
+
+int fgpu_assemble_K_dev(fgpu_ctx *ctx, const double *d_u) {
+  return guarded(ctx, [&](Ctx &c) {
+    FGPU_CHECK(hipMemsetAsync(c.d_status, 0, 2 * sizeof(int), c.stream));
+    FGPU_CHECK(hipEventRecord(c.ev[0], c.stream));
+    if (c.shape == FGPU_HEX8)
+      launch_hex8_K(c, d_u);
+    else
+      launch_bsp_K(c, d_u);
+    FGPU_CHECK(hipEventRecord(c.ev[1], c.stream));
+    launch_assemble_K(c);
+    FGPU_CHECK(hipEventRecord(c.ev[2], c.stream));
+    FGPU_CHECK(hipEventSynchronize(c.ev[2]));
+    c.last_ms[0] = elapsed(c, 0, 1);
+    c.last_ms[1] = elapsed(c, 1, 2);
+    return check_status(c, "stiffness");
+  });
+}
+
+int fgpu_assemble_K(fgpu_ctx *ctx, const double *u) {
+  return guarded(ctx, [&](Ctx &c) {
+    const double *du = nullptr;
+    if (u) {
+      FGPU_CHECK(hipMemcpyAsync(c.d_u, u, sizeof(double) * 3 * c.nnode,
+                                hipMemcpyHostToDevice, c.stream));
+      du = c.d_u;
+    }
+    return fgpu_assemble_K_dev(ctx, du);
+  });
+}
+
+int fgpu_int_force_dev(fgpu_ctx *ctx, const double *d_u, double *d_g) {
+  return guarded(ctx, [&](Ctx &c) {
+    FGPU_CHECK(hipMemsetAsync(c.d_status, 0, 2 * sizeof(int), c.stream));
+    FGPU_CHECK(hipEventRecord(c.ev[3], c.stream));
+    if (c.shape == FGPU_HEX8)
+      launch_hex8_fint(c, d_u);
+    else
+      launch_bsp_fint(c, d_u);
+    launch_gather_fint(c, d_g);
+    FGPU_CHECK(hipEventRecord(c.ev[4], c.stream));
+    FGPU_CHECK(hipEventSynchronize(c.ev[4]));
+    c.last_ms[2] = elapsed(c, 3, 4);
+    return check_status(c, "internal force");
+  });
+}
+
+int fgpu_int_force(fgpu_ctx *ctx, const double *u, double *g) {
+  return guarded(ctx, [&](Ctx &c) {
+    if (!u || !g) throw Error("null argument");
+    FGPU_CHECK(hipMemcpyAsync(c.d_u, u, sizeof(double) * 3 * c.nnode,
+                              hipMemcpyHostToDevice, c.stream));
+    int ok = fgpu_int_force_dev(ctx, c.d_u, c.d_b);
+    FGPU_CHECK(hipMemcpyAsync(g, c.d_b, sizeof(double) * c.neq, hipMemcpyDeviceToHost,
+                              c.stream));
+    FGPU_CHECK(hipStreamSynchronize(c.stream));
+    return ok;
+  });
+}
+
+int fgpu_add_diag(fgpu_ctx *ctx, int n, const int *eq, const double *v) {
+  return guarded(ctx, [&](Ctx &c) {
+    if (n <= 0) return 1;
+    for (int k = 0; k < n; k++)
+      if (eq[k] < 1 || eq[k] > c.neq) throw Error("fgpu_add_diag: equation out of range");
+    int *d_eq = dupload(eq, n, c.stream);
+    double *d_v = dupload(v, n, c.stream);
+    launch_add_diag(c, n, d_eq, d_v);
+    FGPU_CHECK(hipStreamSynchronize(c.stream));
+    (void)hipFree(d_eq);
+    (void)hipFree(d_v);
+    return 1;
+  });
+}
+
+int fgpu_spmv_dev(fgpu_ctx *ctx, const double *x, double *y) {
+  return guarded(ctx, [&](Ctx &c) {
+    launch_spmv(c, x, y);
+    FGPU_CHECK(hipStreamSynchronize(c.stream));
+    return 1;
+  });
+}
+
+int fgpu_solve_dev(fgpu_ctx *ctx, const double *b, double *x, const double *x0,
+                   int maxit, double rtol, int *iters, double *relres) {
+  return guarded(ctx, [&](Ctx &c) {
+    FGPU_CHECK(hipEventRecord(c.ev[5], c.stream));
+    int conv = run_pcg(c, b, x, x0, maxit, rtol, iters, relres);
+    FGPU_CHECK(hipEventRecord(c.ev[6], c.stream));
+    FGPU_CHECK(hipEventSynchronize(c.ev[6]));
+    c.last_ms[3] = elapsed(c, 5, 6);
+    if (!conv) c.err = "PCG did not reach the tolerance";
+    return conv;
+  });
+}
+
+int fgpu_solve(fgpu_ctx *ctx, const double *b, double *x, const double *x0,
+               int maxit, double rtol, int *iters, double *relres) {
+  return guarded(ctx, [&](Ctx &c) {
+    if (!b || !x) throw Error("null argument");
+    size_t nb = sizeof(double) * c.neq;
+    double *db = dalloc<double>(c.neq), *dx = dalloc<double>(c.neq), *dx0 = nullptr;
+    FGPU_CHECK(hipMemcpyAsync(db, b, nb, hipMemcpyHostToDevice, c.stream));
+    if (x0) {
+      dx0 = dalloc<double>(c.neq);
+      FGPU_CHECK(hipMemcpyAsync(dx0, x0, nb, hipMemcpyHostToDevice, c.stream));
+    }
+    int conv = fgpu_solve_dev(ctx, db, dx, dx0, maxit, rtol, iters, relres);
+    FGPU_CHECK(hipMemcpyAsync(x, dx, nb, hipMemcpyDeviceToHost, c.stream));
+    FGPU_CHECK(hipStreamSynchronize(c.stream));
+    (void)hipFree(db);
+    (void)hipFree(dx);
+    if (dx0) (void)hipFree(dx0);
+    return conv;
+  });
+}
+
+int fgpu_get_csr(fgpu_ctx *ctx, long long *rowptr, int *col, double *val) {
+  return guarded(ctx, [&](Ctx &c) {
+    if (rowptr)
+      FGPU_CHECK(hipMemcpyAsync(rowptr, c.d_rowptr, sizeof(long long) * (c.neq + 1),
+                                hipMemcpyDeviceToHost, c.stream));
+    if (col)
+      FGPU_CHECK(hipMemcpyAsync(col, c.d_col, sizeof(int) * c.nnz, hipMemcpyDeviceToHost,
+                                c.stream));
+    if (val)
+      FGPU_CHECK(hipMemcpyAsync(val, c.d_val, sizeof(double) * c.nnz,
+                                hipMemcpyDeviceToHost, c.stream));
+    FGPU_CHECK(hipStreamSynchronize(c.stream));
+    return 1;
+  });
+}
+
+int fgpu_set_csr_values(fgpu_ctx *ctx, const double *val) {
+  return guarded(ctx, [&](Ctx &c) {
+    if (!val) throw Error("null argument");
+    FGPU_CHECK(hipMemcpyAsync(c.d_val, val, sizeof(double) * c.nnz, hipMemcpyHostToDevice,
+                              c.stream));
+    FGPU_CHECK(hipStreamSynchronize(c.stream));
+    return 1;
+  });
+}
+
+int fgpu_get_element_matrices(fgpu_ctx *ctx, int e0, int ne, float *Ke) {
+  return guarded(ctx, [&](Ctx &c) {
+    if (e0 < 0 || ne < 0 || e0 + ne > c.nelem) throw Error("element range out of bounds");
+    size_t nd = 3 * (size_t)c.nn;
+    float *d = dalloc<float>((size_t)ne * nd * nd);
+    launch_expand_Ke(c, e0, ne, d);
+    FGPU_CHECK(hipMemcpyAsync(Ke, d, sizeof(float) * ne * nd * nd, hipMemcpyDeviceToHost,
+                              c.stream));
+    FGPU_CHECK(hipStreamSynchronize(c.stream));
+    (void)hipFree(d);
+    return 1;
+  });
+}
+
+double fgpu_last_time_ms(fgpu_ctx *ctx, int phase) {
+  Ctx *c = reinterpret_cast<Ctx *>(ctx);
+  if (!c || phase < 0 || phase > 3) return -1.0;
+  return c->last_ms[phase];
+}
+
+void *fgpu_device_buffer(fgpu_ctx *ctx, int which) {
+  Ctx *c = reinterpret_cast<Ctx *>(ctx);
+  if (!c) return nullptr;
+  switch (which) {
+    case 0: return c->d_val;
+    case 1: return c->d_Ke;
+    case 2: return c->d_u;
+    default: return nullptr;
+  }
+}
+
+}  // extern "C"

@@ -1,0 +1,124 @@
+// fgpu_internal.h - context layout and kernel launchers shared by the
+// translation units of libfgpu.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "../../include/fgpu.h"
+
+#define FGPU_CHECK(call)                                                     \
+  do {                                                                       \
+    hipError_t e_ = (call);                                                  \
+    if (e_ != hipSuccess) {                                                  \
+      throw fgpu::Error(std::string(#call) + ": " + hipGetErrorString(e_));  \
+    }                                                                        \
+  } while (0)
+
+namespace fgpu {
+
+struct Error {
+  std::string msg;
+  explicit Error(std::string m) : msg(std::move(m)) {}
+};
+
+// Number of stored symmetric 3x3 node blocks of an element (a <= b).
+inline int nsym_blocks(int nn) { return nn * (nn + 1) / 2; }
+// Packed index of the upper block (a <= b), row-major upper triangle.
+__host__ __device__ inline int sym_index(int a, int b, int nn) {
+  return a * nn - (a * (a - 1)) / 2 + (b - a);
+}
+
+// Element-matrix storage: tiles of T elements; inside a tile, block-major:
+//   Ke[((e / T) * nsym + blk) * T * 9 + (e % T) * 9 + (3 i + j)]
+// so that one element's 3x3 block is 9 contiguous floats (the assembly
+// gather reads it as a unit) and a wave of hex8 elements stores coalesced.
+struct KeLayout {
+  int T;     // elements per tile (64 for hex8, 1 for B-spline solids)
+  int nsym;  // stored blocks per element
+  __host__ __device__ size_t offset(long e, int blk) const {
+    return ((size_t)(e / T) * nsym + blk) * (size_t)T * 9 + (size_t)(e % T) * 9;
+  }
+};
+
+struct Ctx {
+  // ---- mesh (host copies)
+  int nnode = 0, nelem = 0, shape = 0, nn = 0, form = 0;
+  int deg[3] = {1, 1, 1}, nip[3] = {2, 2, 2};
+  int neq = 0;
+  std::vector<int> eqmap;  // [nnode*3] 0-based equation or -1
+  std::string err;
+  int device = 0;
+  hipStream_t stream = nullptr;
+
+  // ---- device mesh
+  double *d_coords = nullptr;   // [nnode*3]
+  int *d_conn = nullptr;        // [nelem*nn]
+  int *d_eqmap = nullptr;       // [nnode*3]
+  double *d_mat = nullptr;      // [nelem*2]
+  double *d_ext = nullptr;      // B-spline extraction rows [nelem*3*16]
+  double *d_cpw = nullptr;      // [nnode] control-point weights
+  float *d_gauss = nullptr;     // [3][nip][2] (point, weight) fp32
+  double *d_gauss64 = nullptr;  // same, fp64
+
+  // ---- element outputs
+  KeLayout kel{64, 36};
+  float *d_Ke = nullptr;        // element matrices (upper node blocks)
+  double *d_ge = nullptr;       // element internal forces [nelem*nn*3]
+  int *d_status = nullptr;      // [2]: bad-Jacobian count, stress failure
+
+  // ---- CSR (full symmetric pattern over equations)
+  long long nnz = 0;
+  long long *d_rowptr = nullptr;  // [neq+1]
+  int *d_col = nullptr;           // [nnz]
+  double *d_val = nullptr;        // [nnz]
+  long long *d_diag = nullptr;    // [neq] position of the diagonal entry
+
+  // ---- assembly map: upper node blocks (A <= B)
+  long long nblk = 0;
+  int *d_blkA = nullptr, *d_blkB = nullptr;       // [nblk]
+  int *d_offAB = nullptr;  // [nblk] column offset of node B inside the rows of A
+  int *d_offBA = nullptr;  // [nblk] column offset of node A inside the rows of B
+  long long *d_cptr = nullptr;   // [nblk+1] contribution segments
+  int *d_contrib = nullptr;      // [ncontrib] (e * nsym + sym) << 1 | transpose
+  long long ncontrib = 0;
+  // node -> (element, local node) adjacency for the f_int gather
+  int *d_nptr = nullptr;         // [nnode+1]
+  int *d_nadj = nullptr;         // [nelem*nn] e * nn + a
+
+  // ---- solver workspace
+  double *d_u = nullptr;      // nodal vector scratch [nnode*3]
+  double *d_b = nullptr, *d_x = nullptr, *d_r = nullptr, *d_p = nullptr,
+         *d_w = nullptr, *d_z = nullptr, *d_dinv = nullptr;  // [neq]
+  double *d_part = nullptr;   // partial sums
+  double *d_scal = nullptr;   // device scalars
+  int *d_done = nullptr;      // convergence iteration (0 = running)
+  int *h_done = nullptr;      // pinned mirror
+  double *h_scal = nullptr;   // pinned mirror
+  int nblocks_red = 0;        // reduction blocks
+
+  // ---- timing
+  hipEvent_t ev[8] = {};
+  double last_ms[4] = {0, 0, 0, 0};
+};
+
+// kernel launchers (defined in the .hip files)
+void launch_hex8_K(Ctx &c, const double *d_unodal);
+void launch_hex8_fint(Ctx &c, const double *d_unodal);
+void launch_bsp_K(Ctx &c, const double *d_unodal);
+void launch_bsp_fint(Ctx &c, const double *d_unodal);
+void launch_assemble_K(Ctx &c);
+void launch_gather_fint(Ctx &c, double *d_g);
+void launch_spmv(Ctx &c, const double *x, double *y);
+int run_pcg(Ctx &c, const double *b, double *x, const double *x0, int maxit,
+            double rtol, int *iters, double *relres);
+void launch_nodal_to_eq(Ctx &c, const double *nodal, double *eq);
+void launch_eq_to_nodal(Ctx &c, const double *eq, double *nodal);
+void launch_add_diag(Ctx &c, int n, const int *d_eq, const double *d_v);
+void launch_expand_Ke(Ctx &c, int e0, int ne, float *d_out);
+void bsp_prepare_tables(Ctx &c, const double *h_ext, const double *h_cpw);
+
+}  // namespace fgpu

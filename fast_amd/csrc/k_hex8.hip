@@ -1,0 +1,451 @@
+// k_hex8.hip - 8-node brick element kernels (stiffness in FP32, internal
+// force in FP64) for gfx950.
+//
+// Reference path: cElmParam::StiffMat / IntForce (elc/elmparam.cpp:449-597)
+// and cElmParamTL::StiffMat / IntForce (elc/elmparam.cpp:1615-1807) over
+// cShapeBrick8 (shp/shpsolid.cpp:173-238), cShapeSolid::DrvShpXYZ
+// (shp/shpsolid.cpp:58-126) and cSolid B / Bl / Bnl / S matrices
+// (elc/anmodel.cpp:1418-1568) with the isotropic Q of anmodel.cpp:1286-1298.
+//
+// The dense contraction B^T C B is evaluated in closed form for isotropic
+// materials.  With g = grad N_a, h = grad N_b (per integration point):
+//   linear:  K_ab = lambda g h^T + mu h g^T + mu (g.h) I
+//   TL SVK:  K_ab = lambda (Fg)(Fh)^T + mu (Fh)(Fg)^T + mu (g.h) F F^T
+//                   + (g^T S h) I            [= (B+Bl)^T C (B+Bl) + Bnl^T S Bnl]
+//   TL neo-Hooke: with spatial gradients g' = F^-T g, lambda' = lambda,
+//                 mu' = mu - lambda ln J:
+//            K_ab = lambda g' h'^T + mu' h' g'^T + mu (g.h) I
+// so only three-vectors per node and point are needed.  A workgroup stages
+// those vectors for TE elements x all points in LDS (phase 1: one thread per
+// element-point), then each thread forms whole 3x3 node blocks (phase 2:
+// one thread per element-block, consecutive threads = consecutive elements,
+// so the block stores of a wave are contiguous).
+#include "fgpu_internal.h"
+
+namespace fgpu {
+namespace {
+
+__constant__ signed char kNodeR[8] = {-1, 1, 1, -1, -1, 1, 1, -1};
+__constant__ signed char kNodeS[8] = {-1, -1, 1, 1, -1, -1, 1, 1};
+__constant__ signed char kNodeT[8] = {1, 1, 1, 1, -1, -1, -1, -1};
+// upper node blocks (a <= b) of a hex8, packed row-major
+__constant__ unsigned char kBlkA[36] = {0, 0, 0, 0, 0, 0, 0, 0, 1, 1, 1, 1, 1, 1, 1, 2, 2, 2,
+                                        2, 2, 2, 3, 3, 3, 3, 3, 4, 4, 4, 4, 5, 5, 5, 6, 6, 7};
+__constant__ unsigned char kBlkB[36] = {0, 1, 2, 3, 4, 5, 6, 7, 1, 2, 3, 4, 5, 6, 7, 2, 3, 4,
+                                        5, 6, 7, 3, 4, 5, 6, 7, 4, 5, 6, 7, 5, 6, 7, 6, 7, 7};
+
+template <class T>
+__device__ inline void hex8_dN(T r, T s, T t, int a, T &dr, T &ds, T &dt) {
+  const T q = T(0.125);
+  T fr = T(1) + T(kNodeR[a]) * r, fs = T(1) + T(kNodeS[a]) * s, ft = T(1) + T(kNodeT[a]) * t;
+  dr = (kNodeR[a] > 0 ? q : -q) * fs * ft;
+  ds = (kNodeS[a] > 0 ? q : -q) * fr * ft;
+  dt = (kNodeT[a] > 0 ? q : -q) * fr * fs;
+}
+
+template <class T>
+__device__ inline T inv3(const T J[3][3], T I[3][3]) {
+  T det = J[0][0] * (J[1][1] * J[2][2] - J[1][2] * J[2][1]) -
+          J[0][1] * (J[1][0] * J[2][2] - J[1][2] * J[2][0]) +
+          J[0][2] * (J[1][0] * J[2][1] - J[1][1] * J[2][0]);
+  T id = T(1) / det;
+  I[0][0] = (J[1][1] * J[2][2] - J[1][2] * J[2][1]) * id;
+  I[0][1] = -(J[0][1] * J[2][2] - J[0][2] * J[2][1]) * id;
+  I[0][2] = (J[0][1] * J[1][2] - J[0][2] * J[1][1]) * id;
+  I[1][0] = -(J[1][0] * J[2][2] - J[1][2] * J[2][0]) * id;
+  I[1][1] = (J[0][0] * J[2][2] - J[0][2] * J[2][0]) * id;
+  I[1][2] = -(J[0][0] * J[1][2] - J[0][2] * J[1][0]) * id;
+  I[2][0] = (J[1][0] * J[2][1] - J[1][1] * J[2][0]) * id;
+  I[2][1] = -(J[0][0] * J[2][1] - J[0][1] * J[2][0]) * id;
+  I[2][2] = (J[0][0] * J[1][1] - J[0][1] * J[1][0]) * id;
+  return det;
+}
+
+// Per element-point vectors staged in LDS (floats per point):
+//   LINEAR:   g (24), c = w detJ
+//   SVK:      Fg (24), g (24), FF^T (6), S (6), c
+//   NEOHOOKE: g' (24), FF^T (6), c*lambda, c*mu', c*mu
+template <int FORM> struct Stage;
+template <> struct Stage<FGPU_LINEAR> { static constexpr int NF = 25; };
+template <> struct Stage<FGPU_TL_SVK> { static constexpr int NF = 61; };
+template <> struct Stage<FGPU_TL_NEOHOOKE> { static constexpr int NF = 33; };
+
+__device__ inline void lame(double E, double nu, float &lam, float &mu) {
+  lam = (float)(E * nu / ((1.0 + nu) * (1.0 - 2.0 * nu)));
+  mu = (float)(0.5 * E / (1.0 + nu));
+}
+
+template <int FORM, int NIP, int TE>
+__global__ __launch_bounds__(256) void hex8_K_kernel(
+    int nelem, const double *__restrict__ coords, const int *__restrict__ conn,
+    const double *__restrict__ mat, const double *__restrict__ unodal,
+    const float *__restrict__ gp, float *__restrict__ Ke, KeLayout L,
+    int *__restrict__ status) {
+  constexpr int NQ = NIP * NIP * NIP;
+  constexpr int NF = Stage<FORM>::NF;
+  extern __shared__ float sh[];  // [q][f][TE]
+  const int e0 = blockIdx.x * TE;
+  const int tid = threadIdx.x;
+
+  // ---------------- phase 1: one thread per (element, point)
+  for (int task = tid; task < TE * NQ; task += blockDim.x) {
+    const int el = task % TE, q = task / TE;
+    const int e = e0 + el;
+    float *S = sh + (size_t)q * NF * TE + el;
+    if (e >= nelem) {
+      for (int f = 0; f < NF; f++) S[f * TE] = 0.f;
+      continue;
+    }
+    const int *cn = conn + (size_t)e * 8;
+    int nd[8];
+    for (int a = 0; a < 8; a++) nd[a] = cn[a];
+    // coordinates relative to node 0 (FP64 subtraction, then FP32)
+    double x0 = coords[3 * nd[0]], y0 = coords[3 * nd[0] + 1], z0 = coords[3 * nd[0] + 2];
+    float X[8][3];
+    for (int a = 0; a < 8; a++) {
+      X[a][0] = (float)(coords[3 * nd[a]] - x0);
+      X[a][1] = (float)(coords[3 * nd[a] + 1] - y0);
+      X[a][2] = (float)(coords[3 * nd[a] + 2] - z0);
+    }
+    const int ir = q % NIP, is = (q / NIP) % NIP, it = q / (NIP * NIP);
+    const float r = gp[(0 * 16 + ir) * 2], s = gp[(1 * 16 + is) * 2], t = gp[(2 * 16 + it) * 2];
+    const float w = gp[(0 * 16 + ir) * 2 + 1] * gp[(1 * 16 + is) * 2 + 1] * gp[(2 * 16 + it) * 2 + 1];
+    float dN[8][3];
+    float J[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
+    for (int a = 0; a < 8; a++) {
+      hex8_dN(r, s, t, a, dN[a][0], dN[a][1], dN[a][2]);
+      for (int i = 0; i < 3; i++)
+        for (int k = 0; k < 3; k++) J[i][k] += dN[a][i] * X[a][k];
+    }
+    float Ji[3][3];
+    float det = inv3(J, Ji);
+    if (!(det > 0.f)) atomicAdd(&status[0], 1);
+    float g[8][3];
+    for (int a = 0; a < 8; a++)
+      for (int i = 0; i < 3; i++)
+        g[a][i] = Ji[i][0] * dN[a][0] + Ji[i][1] * dN[a][1] + Ji[i][2] * dN[a][2];
+    const float c = w * det;
+    if constexpr (FORM == FGPU_LINEAR) {
+      for (int a = 0; a < 8; a++)
+        for (int i = 0; i < 3; i++) S[(3 * a + i) * TE] = g[a][i];
+      S[24 * TE] = c;
+    } else {
+      float F[3][3] = {{1, 0, 0}, {0, 1, 0}, {0, 0, 1}};
+      for (int a = 0; a < 8; a++) {
+        const double *ua = unodal + 3 * (size_t)nd[a];
+        float u0 = (float)ua[0], u1 = (float)ua[1], u2 = (float)ua[2];
+        for (int j = 0; j < 3; j++) {
+          F[0][j] += u0 * g[a][j];
+          F[1][j] += u1 * g[a][j];
+          F[2][j] += u2 * g[a][j];
+        }
+      }
+      float B[6];  // F F^T (xx, yy, zz, xy, xz, yz)
+      B[0] = F[0][0] * F[0][0] + F[0][1] * F[0][1] + F[0][2] * F[0][2];
+      B[1] = F[1][0] * F[1][0] + F[1][1] * F[1][1] + F[1][2] * F[1][2];
+      B[2] = F[2][0] * F[2][0] + F[2][1] * F[2][1] + F[2][2] * F[2][2];
+      B[3] = F[0][0] * F[1][0] + F[0][1] * F[1][1] + F[0][2] * F[1][2];
+      B[4] = F[0][0] * F[2][0] + F[0][1] * F[2][1] + F[0][2] * F[2][2];
+      B[5] = F[1][0] * F[2][0] + F[1][1] * F[2][1] + F[1][2] * F[2][2];
+      float lam, mu;
+      if constexpr (FORM == FGPU_TL_SVK) {
+        lame(mat[2 * (size_t)e], mat[2 * (size_t)e + 1], lam, mu);
+        // Green-Lagrange E = (F^T F - I) / 2 ; S = lambda tr(E) I + 2 mu E
+        float C00 = F[0][0] * F[0][0] + F[1][0] * F[1][0] + F[2][0] * F[2][0];
+        float C11 = F[0][1] * F[0][1] + F[1][1] * F[1][1] + F[2][1] * F[2][1];
+        float C22 = F[0][2] * F[0][2] + F[1][2] * F[1][2] + F[2][2] * F[2][2];
+        float C01 = F[0][0] * F[0][1] + F[1][0] * F[1][1] + F[2][0] * F[2][1];
+        float C02 = F[0][0] * F[0][2] + F[1][0] * F[1][2] + F[2][0] * F[2][2];
+        float C12 = F[0][1] * F[0][2] + F[1][1] * F[1][2] + F[2][1] * F[2][2];
+        float E00 = 0.5f * (C00 - 1.f), E11 = 0.5f * (C11 - 1.f), E22 = 0.5f * (C22 - 1.f);
+        float tr = E00 + E11 + E22;
+        float Sv[6] = {lam * tr + 2.f * mu * E00, lam * tr + 2.f * mu * E11,
+                       lam * tr + 2.f * mu * E22, mu * C01, mu * C02, mu * C12};
+        for (int a = 0; a < 8; a++)
+          for (int i = 0; i < 3; i++) {
+            S[(3 * a + i) * TE] = F[i][0] * g[a][0] + F[i][1] * g[a][1] + F[i][2] * g[a][2];
+            S[(24 + 3 * a + i) * TE] = g[a][i];
+          }
+        for (int k = 0; k < 6; k++) {
+          S[(48 + k) * TE] = B[k];
+          S[(54 + k) * TE] = Sv[k];
+        }
+        S[60 * TE] = c;
+      } else {  // neo-Hooke: mat = (mu, lambda)
+        mu = (float)mat[2 * (size_t)e];
+        lam = (float)mat[2 * (size_t)e + 1];
+        float Fi[3][3];
+        float Jd = inv3(F, Fi);
+        if (!(Jd > 0.f)) {
+          atomicAdd(&status[1], 1);
+          Jd = 1.f;
+        }
+        float lnJ = logf(Jd);
+        for (int a = 0; a < 8; a++)
+          for (int i = 0; i < 3; i++)  // g' = F^-T g
+            S[(3 * a + i) * TE] = Fi[0][i] * g[a][0] + Fi[1][i] * g[a][1] + Fi[2][i] * g[a][2];
+        for (int k = 0; k < 6; k++) S[(24 + k) * TE] = B[k];
+        S[30 * TE] = c * lam;
+        S[31 * TE] = c * (mu - lam * lnJ);
+        S[32 * TE] = c * mu;
+      }
+    }
+  }
+  __syncthreads();
+
+  // ---------------- phase 2: one thread per (element, upper node block)
+  for (int task = tid; task < TE * 36; task += blockDim.x) {
+    const int el = task % TE, blk = task / TE;
+    const int e = e0 + el;
+    if (e >= nelem) continue;
+    const int a = kBlkA[blk], b = kBlkB[blk];
+    float K[9];
+    if constexpr (FORM == FGPU_LINEAR) {
+      float M[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+      for (int q = 0; q < NQ; q++) {
+        const float *S = sh + (size_t)q * NF * TE + el;
+        float c = S[24 * TE];
+        float ga0 = c * S[(3 * a) * TE], ga1 = c * S[(3 * a + 1) * TE], ga2 = c * S[(3 * a + 2) * TE];
+        float hb0 = S[(3 * b) * TE], hb1 = S[(3 * b + 1) * TE], hb2 = S[(3 * b + 2) * TE];
+        M[0] += ga0 * hb0; M[1] += ga0 * hb1; M[2] += ga0 * hb2;
+        M[3] += ga1 * hb0; M[4] += ga1 * hb1; M[5] += ga1 * hb2;
+        M[6] += ga2 * hb0; M[7] += ga2 * hb1; M[8] += ga2 * hb2;
+      }
+      float lam, mu;
+      lame(mat[2 * (size_t)e], mat[2 * (size_t)e + 1], lam, mu);
+      float tr = mu * (M[0] + M[4] + M[8]);
+      for (int i = 0; i < 3; i++)
+        for (int j = 0; j < 3; j++)
+          K[3 * i + j] = lam * M[3 * i + j] + mu * M[3 * j + i] + (i == j ? tr : 0.f);
+    } else if constexpr (FORM == FGPU_TL_SVK) {
+      float M[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+      float Bs[6] = {0, 0, 0, 0, 0, 0};
+      float s = 0.f;
+      for (int q = 0; q < NQ; q++) {
+        const float *S = sh + (size_t)q * NF * TE + el;
+        float c = S[60 * TE];
+        float fa[3], fb[3], ga[3], gb[3];
+        for (int i = 0; i < 3; i++) {
+          fa[i] = c * S[(3 * a + i) * TE];
+          fb[i] = S[(3 * b + i) * TE];
+          ga[i] = c * S[(24 + 3 * a + i) * TE];
+          gb[i] = S[(24 + 3 * b + i) * TE];
+        }
+        for (int i = 0; i < 3; i++)
+          for (int j = 0; j < 3; j++) M[3 * i + j] += fa[i] * fb[j];
+        float gg = ga[0] * gb[0] + ga[1] * gb[1] + ga[2] * gb[2];
+        for (int k = 0; k < 6; k++) Bs[k] += gg * S[(48 + k) * TE];
+        float s0 = S[54 * TE], s1 = S[55 * TE], s2 = S[56 * TE], s3 = S[57 * TE],
+              s4 = S[58 * TE], s5 = S[59 * TE];
+        s += ga[0] * (s0 * gb[0] + s3 * gb[1] + s4 * gb[2]) +
+             ga[1] * (s3 * gb[0] + s1 * gb[1] + s5 * gb[2]) +
+             ga[2] * (s4 * gb[0] + s5 * gb[1] + s2 * gb[2]);
+      }
+      float lam, mu;
+      lame(mat[2 * (size_t)e], mat[2 * (size_t)e + 1], lam, mu);
+      const float Bm[9] = {Bs[0], Bs[3], Bs[4], Bs[3], Bs[1], Bs[5], Bs[4], Bs[5], Bs[2]};
+      for (int i = 0; i < 3; i++)
+        for (int j = 0; j < 3; j++)
+          K[3 * i + j] = lam * M[3 * i + j] + mu * M[3 * j + i] + mu * Bm[3 * i + j] +
+                         (i == j ? s : 0.f);
+    } else {
+      float M1[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0}, M2[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+      float s = 0.f;
+      for (int q = 0; q < NQ; q++) {
+        const float *S = sh + (size_t)q * NF * TE + el;
+        float ga[3], gb[3];
+        for (int i = 0; i < 3; i++) {
+          ga[i] = S[(3 * a + i) * TE];
+          gb[i] = S[(3 * b + i) * TE];
+        }
+        float cl = S[30 * TE], cm2 = S[31 * TE], cm = S[32 * TE];
+        for (int i = 0; i < 3; i++)
+          for (int j = 0; j < 3; j++) {
+            float p = ga[i] * gb[j];
+            M1[3 * i + j] += cl * p;
+            M2[3 * i + j] += cm2 * p;
+          }
+        // g.h = g'^T (F F^T) h'
+        float b0 = S[24 * TE], b1 = S[25 * TE], b2 = S[26 * TE], b3 = S[27 * TE],
+              b4 = S[28 * TE], b5 = S[29 * TE];
+        s += cm * (ga[0] * (b0 * gb[0] + b3 * gb[1] + b4 * gb[2]) +
+                   ga[1] * (b3 * gb[0] + b1 * gb[1] + b5 * gb[2]) +
+                   ga[2] * (b4 * gb[0] + b5 * gb[1] + b2 * gb[2]));
+      }
+      for (int i = 0; i < 3; i++)
+        for (int j = 0; j < 3; j++)
+          K[3 * i + j] = M1[3 * i + j] + M2[3 * j + i] + (i == j ? s : 0.f);
+    }
+    float *out = Ke + L.offset(e, blk);
+    for (int k = 0; k < 9; k++) out[k] = K[k];
+  }
+}
+
+// ---------------------------------------------------------------- f_int
+// One thread per element, FP64 throughout.  Linear: g = sum c B^T (C B u);
+// TL: g_a = sum c P grad N_a with P = F S (the B-bar^T S of the reference).
+template <int FORM, int NIP>
+__global__ __launch_bounds__(128) void hex8_fint_kernel(
+    int nelem, const double *__restrict__ coords, const int *__restrict__ conn,
+    const double *__restrict__ mat, const double *__restrict__ unodal,
+    const double *__restrict__ gp, double *__restrict__ ge, int *__restrict__ status) {
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= nelem) return;
+  const int *cn = conn + (size_t)e * 8;
+  double X[8][3], U[8][3], g[8][3];
+  const int n0 = cn[0];
+  for (int a = 0; a < 8; a++) {
+    int n = cn[a];
+    for (int k = 0; k < 3; k++) {
+      X[a][k] = coords[3 * (size_t)n + k] - coords[3 * (size_t)n0 + k];
+      U[a][k] = unodal[3 * (size_t)n + k];
+      g[a][k] = 0.0;
+    }
+  }
+  double p0 = mat[2 * (size_t)e], p1 = mat[2 * (size_t)e + 1];
+  int bad = 0, fail = 0;
+  for (int q = 0; q < NIP * NIP * NIP; q++) {
+    const int ir = q % NIP, is = (q / NIP) % NIP, it = q / (NIP * NIP);
+    const double r = gp[(0 * 16 + ir) * 2], s = gp[(1 * 16 + is) * 2], t = gp[(2 * 16 + it) * 2];
+    const double w = gp[(0 * 16 + ir) * 2 + 1] * gp[(1 * 16 + is) * 2 + 1] * gp[(2 * 16 + it) * 2 + 1];
+    double J[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
+    for (int a = 0; a < 8; a++) {
+      double d[3];
+      hex8_dN(r, s, t, a, d[0], d[1], d[2]);
+      for (int i = 0; i < 3; i++)
+        for (int k = 0; k < 3; k++) J[i][k] += d[i] * X[a][k];
+    }
+    double Ji[3][3];
+    double det = inv3(J, Ji);
+    if (!(det > 0.0)) bad++;
+    const double c = w * det;
+    // L = grad u (L_ij = d u_i / d X_j)
+    double Lg[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
+    for (int a = 0; a < 8; a++) {
+      double d[3], G[3];
+      hex8_dN(r, s, t, a, d[0], d[1], d[2]);
+      for (int i = 0; i < 3; i++) G[i] = Ji[i][0] * d[0] + Ji[i][1] * d[1] + Ji[i][2] * d[2];
+      for (int i = 0; i < 3; i++)
+        for (int j = 0; j < 3; j++) Lg[i][j] += U[a][i] * G[j];
+    }
+    double P[3][3];  // stress acting on material gradients: g_a += c P grad N_a
+    if constexpr (FORM == FGPU_LINEAR) {
+      double nu = p1, E = p0;
+      double lam = E * nu / ((1.0 + nu) * (1.0 - 2.0 * nu)), mu = 0.5 * E / (1.0 + nu);
+      double tr = Lg[0][0] + Lg[1][1] + Lg[2][2];
+      for (int i = 0; i < 3; i++)
+        for (int j = 0; j < 3; j++)
+          P[i][j] = mu * (Lg[i][j] + Lg[j][i]) + (i == j ? lam * tr : 0.0);
+    } else {
+      double F[3][3];
+      for (int i = 0; i < 3; i++)
+        for (int j = 0; j < 3; j++) F[i][j] = Lg[i][j] + (i == j ? 1.0 : 0.0);
+      double C[3][3];
+      for (int i = 0; i < 3; i++)
+        for (int j = 0; j < 3; j++)
+          C[i][j] = F[0][i] * F[0][j] + F[1][i] * F[1][j] + F[2][i] * F[2][j];
+      double Sm[3][3];
+      if constexpr (FORM == FGPU_TL_SVK) {
+        double nu = p1, E = p0;
+        double lam = E * nu / ((1.0 + nu) * (1.0 - 2.0 * nu)), mu = 0.5 * E / (1.0 + nu);
+        double tr = 0.5 * (C[0][0] + C[1][1] + C[2][2] - 3.0);
+        for (int i = 0; i < 3; i++)
+          for (int j = 0; j < 3; j++)
+            Sm[i][j] = mu * (C[i][j] - (i == j ? 1.0 : 0.0)) + (i == j ? lam * tr : 0.0);
+      } else {
+        double mu = p0, lam = p1;
+        double Ci[3][3];
+        double detC = inv3(C, Ci);
+        if (!(detC > 0.0)) {
+          fail++;
+          detC = 1.0;
+        }
+        double lnJ = 0.5 * log(detC);
+        for (int i = 0; i < 3; i++)
+          for (int j = 0; j < 3; j++)
+            Sm[i][j] = mu * ((i == j ? 1.0 : 0.0) - Ci[i][j]) + lam * lnJ * Ci[i][j];
+      }
+      for (int i = 0; i < 3; i++)
+        for (int j = 0; j < 3; j++)
+          P[i][j] = F[i][0] * Sm[0][j] + F[i][1] * Sm[1][j] + F[i][2] * Sm[2][j];
+    }
+    for (int a = 0; a < 8; a++) {
+      double d[3], G[3];
+      hex8_dN(r, s, t, a, d[0], d[1], d[2]);
+      for (int i = 0; i < 3; i++) G[i] = Ji[i][0] * d[0] + Ji[i][1] * d[1] + Ji[i][2] * d[2];
+      for (int i = 0; i < 3; i++) g[a][i] += c * (P[i][0] * G[0] + P[i][1] * G[1] + P[i][2] * G[2]);
+    }
+  }
+  if (bad) atomicAdd(&status[0], bad);
+  if (fail) atomicAdd(&status[1], fail);
+  double *out = ge + (size_t)e * 24;
+  for (int a = 0; a < 8; a++)
+    for (int i = 0; i < 3; i++) out[3 * a + i] = g[a][i];
+}
+
+template <int FORM, int NIP>
+void launch_K_t(Ctx &c, const double *u) {
+  constexpr int NQ = NIP * NIP * NIP;
+  constexpr int NF = Stage<FORM>::NF;
+  // elements per workgroup: largest divisor of 64 keeping LDS <= 64 KB
+  constexpr int TE0 = (65536 / (NQ * NF * 4));
+  constexpr int TE = TE0 >= 64 ? 64 : TE0 >= 32 ? 32 : TE0 >= 16 ? 16 : TE0 >= 8 ? 8 : 4;
+  size_t lds = (size_t)NQ * NF * TE * sizeof(float);
+  int grid = (c.nelem + TE - 1) / TE;
+  hipLaunchKernelGGL((hex8_K_kernel<FORM, NIP, TE>), dim3(grid), dim3(256), lds, c.stream,
+                     c.nelem, c.d_coords, c.d_conn, c.d_mat, u, c.d_gauss, c.d_Ke, c.kel,
+                     c.d_status);
+  FGPU_CHECK(hipGetLastError());
+}
+
+template <int FORM>
+void launch_K_f(Ctx &c, const double *u) {
+  switch (c.nip[0]) {
+    case 1: launch_K_t<FORM, 1>(c, u); break;
+    case 2: launch_K_t<FORM, 2>(c, u); break;
+    default: launch_K_t<FORM, 3>(c, u); break;
+  }
+}
+
+template <int FORM, int NIP>
+void launch_f_t(Ctx &c, const double *u) {
+  int grid = (c.nelem + 127) / 128;
+  hipLaunchKernelGGL((hex8_fint_kernel<FORM, NIP>), dim3(grid), dim3(128), 0, c.stream,
+                     c.nelem, c.d_coords, c.d_conn, c.d_mat, u, c.d_gauss64, c.d_ge,
+                     c.d_status);
+  FGPU_CHECK(hipGetLastError());
+}
+
+template <int FORM>
+void launch_f_f(Ctx &c, const double *u) {
+  switch (c.nip[0]) {
+    case 1: launch_f_t<FORM, 1>(c, u); break;
+    case 2: launch_f_t<FORM, 2>(c, u); break;
+    default: launch_f_t<FORM, 3>(c, u); break;
+  }
+}
+
+}  // namespace
+
+void launch_hex8_K(Ctx &c, const double *u) {
+  if (c.form != FGPU_LINEAR && !u) {
+    // zero displacement: stage a zero nodal field
+    FGPU_CHECK(hipMemsetAsync(c.d_u, 0, sizeof(double) * 3 * c.nnode, c.stream));
+    u = c.d_u;
+  }
+  switch (c.form) {
+    case FGPU_LINEAR: launch_K_f<FGPU_LINEAR>(c, u); break;
+    case FGPU_TL_SVK: launch_K_f<FGPU_TL_SVK>(c, u); break;
+    default: launch_K_f<FGPU_TL_NEOHOOKE>(c, u); break;
+  }
+}
+
+void launch_hex8_fint(Ctx &c, const double *u) {
+  switch (c.form) {
+    case FGPU_LINEAR: launch_f_f<FGPU_LINEAR>(c, u); break;
+    case FGPU_TL_SVK: launch_f_f<FGPU_TL_SVK>(c, u); break;
+    default: launch_f_f<FGPU_TL_NEOHOOKE>(c, u); break;
+  }
+}
+
+}  // namespace fgpu

@@ -1,0 +1,182 @@
+"""Python host mirror of the C-ABI in include/fgpu.h (ctypes).
+
+The class ``GpuModel`` exposes the reference's operator seam for this path:
+``stiffness`` (cControl::StiffnessMatrix, ctrl/ctrl.cpp:886-975),
+``int_force`` (cControl::IntForceVector, ctrl/ctrl.cpp:719-813) and ``solve``
+(cSysMatrix::Solve with PCGSolver, mlib/sysmat.cpp:1061-1125), with the same
+1 = ok / 0 = failure convention surfaced as ``FgpuError``.
+
+The native library must be built (``python -m fast_amd.build``); there is no
+fallback: importing without it raises.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+from .mesh import FORMS, SHAPES, Mesh
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIBPATH = os.path.join(_HERE, "libfgpu.so")
+_LIB = None
+
+
+class FgpuError(RuntimeError):
+    pass
+
+
+class FgpuMesh(C.Structure):
+    _fields_ = [("nnode", C.c_int), ("nelem", C.c_int), ("shape", C.c_int),
+                ("nn", C.c_int), ("deg", C.c_int * 3), ("nip", C.c_int * 3),
+                ("form", C.c_int), ("coords", C.c_void_p), ("conn", C.c_void_p),
+                ("dofmap", C.c_void_p), ("ext", C.c_void_p), ("cpw", C.c_void_p),
+                ("mat", C.c_void_p)]
+
+
+EXPORTS = {
+    "fgpu_create": (C.c_int, [C.c_void_p, C.c_void_p]),
+    "fgpu_destroy": (None, [C.c_void_p]),
+    "fgpu_last_error": (C.c_char_p, [C.c_void_p]),
+    "fgpu_neq": (C.c_int, [C.c_void_p]),
+    "fgpu_nnz": (C.c_longlong, [C.c_void_p]),
+    "fgpu_assemble_K": (C.c_int, [C.c_void_p, C.c_void_p]),
+    "fgpu_assemble_K_dev": (C.c_int, [C.c_void_p, C.c_void_p]),
+    "fgpu_int_force": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p]),
+    "fgpu_int_force_dev": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p]),
+    "fgpu_add_diag": (C.c_int, [C.c_void_p, C.c_int, C.c_void_p, C.c_void_p]),
+    "fgpu_solve": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int,
+                             C.c_double, C.c_void_p, C.c_void_p]),
+    "fgpu_solve_dev": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int,
+                                 C.c_double, C.c_void_p, C.c_void_p]),
+    "fgpu_spmv_dev": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p]),
+    "fgpu_get_csr": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),
+    "fgpu_set_csr_values": (C.c_int, [C.c_void_p, C.c_void_p]),
+    "fgpu_get_element_matrices": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_void_p]),
+    "fgpu_last_time_ms": (C.c_double, [C.c_void_p, C.c_int]),
+    "fgpu_device_buffer": (C.c_void_p, [C.c_void_p, C.c_int]),
+}
+
+
+def lib():
+    global _LIB
+    if _LIB is None:
+        if not os.path.exists(LIBPATH):
+            raise FgpuError("libfgpu.so is not built (run: python -m fast_amd.build)")
+        L = C.CDLL(LIBPATH)
+        for name, (res, args) in EXPORTS.items():
+            f = getattr(L, name)
+            f.restype = res
+            f.argtypes = args
+        _LIB = L
+    return _LIB
+
+
+def _p(a):
+    return None if a is None else a.ctypes.data
+
+
+class GpuModel:
+    """One fgpu context: a mesh resident on the current HIP device."""
+
+    def __init__(self, mesh: Mesh):
+        self.mesh = mesh
+        L = lib()
+        self._keep = []
+
+        def c(a, dt):
+            a = np.ascontiguousarray(a, dtype=dt)
+            self._keep.append(a)
+            return a
+
+        m = FgpuMesh()
+        m.nnode, m.nelem = mesh.nnode, mesh.nelem
+        m.shape = SHAPES[mesh.shape]
+        m.nn = mesh.nn
+        for d in range(3):
+            m.deg[d] = int(mesh.deg[d])
+            m.nip[d] = int(mesh.nip[d])
+        m.form = FORMS[mesh.form]
+        m.coords = _p(c(mesh.coords, np.float64))
+        m.conn = _p(c(mesh.conn, np.int32))
+        m.dofmap = _p(c(mesh.dofmap, np.int32))
+        m.mat = _p(c(mesh.mat, np.float64))
+        m.ext = _p(c(mesh.ext, np.float64)) if mesh.ext is not None else None
+        m.cpw = _p(c(mesh.cpw, np.float64)) if mesh.cpw is not None else None
+        self._m = m
+        h = C.c_void_p()
+        if not L.fgpu_create(C.byref(h), C.byref(m)):
+            raise FgpuError("fgpu_create failed (see stderr)")
+        self.h = h
+        self.neq = L.fgpu_neq(h)
+        self.nnz = L.fgpu_nnz(h)
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().fgpu_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _check(self, ok, what):
+        if not ok:
+            raise FgpuError("%s: %s" % (what, lib().fgpu_last_error(self.h).decode()))
+
+    # ---- the operator seam
+    def stiffness(self, u_nodal=None):
+        u = None if u_nodal is None else np.ascontiguousarray(u_nodal, dtype=np.float64)
+        self._check(lib().fgpu_assemble_K(self.h, _p(u)), "fgpu_assemble_K")
+
+    def int_force(self, u_nodal):
+        u = np.ascontiguousarray(u_nodal, dtype=np.float64)
+        g = np.zeros(self.neq)
+        self._check(lib().fgpu_int_force(self.h, _p(u), _p(g)), "fgpu_int_force")
+        return g
+
+    def add_diag(self, eq, v):
+        eq = np.ascontiguousarray(eq, dtype=np.int32)
+        v = np.ascontiguousarray(v, dtype=np.float64)
+        self._check(lib().fgpu_add_diag(self.h, len(eq), _p(eq), _p(v)), "fgpu_add_diag")
+
+    def solve(self, b, x0=None, maxit=None, tol=1e-8, raise_on_fail=False):
+        b = np.ascontiguousarray(b, dtype=np.float64)
+        x = np.zeros(self.neq)
+        x0 = None if x0 is None else np.ascontiguousarray(x0, dtype=np.float64)
+        it = C.c_int(0)
+        rr = C.c_double(0.0)
+        conv = lib().fgpu_solve(self.h, _p(b), _p(x), _p(x0),
+                                self.neq if maxit is None else int(maxit), tol,
+                                C.byref(it), C.byref(rr))
+        if raise_on_fail:
+            self._check(conv, "fgpu_solve")
+        return x, it.value, rr.value, bool(conv)
+
+    def csr(self):
+        rowptr = np.zeros(self.neq + 1, dtype=np.int64)
+        col = np.zeros(self.nnz, dtype=np.int32)
+        val = np.zeros(self.nnz)
+        self._check(lib().fgpu_get_csr(self.h, _p(rowptr), _p(col), _p(val)), "fgpu_get_csr")
+        return rowptr, col, val
+
+    def set_csr_values(self, val):
+        val = np.ascontiguousarray(val, dtype=np.float64)
+        assert val.shape == (self.nnz,)
+        self._check(lib().fgpu_set_csr_values(self.h, _p(val)), "fgpu_set_csr_values")
+
+    def element_matrices(self, e0, ne):
+        nd = 3 * self.mesh.nn
+        K = np.zeros((ne, nd, nd), dtype=np.float32)
+        self._check(lib().fgpu_get_element_matrices(self.h, e0, ne, _p(K)),
+                    "fgpu_get_element_matrices")
+        return K
+
+    def last_times(self):
+        return [lib().fgpu_last_time_ms(self.h, k) for k in range(4)]
+
+    def device_buffer(self, which):
+        return lib().fgpu_device_buffer(self.h, which)

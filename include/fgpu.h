@@ -1,0 +1,120 @@
+/* fgpu.h - C-ABI of the MI355X element-integration / assembly / solve path.
+ *
+ * The seam is FAST's global element loops and system-matrix interface: a C++
+ * analysis driver (linear static, load-control Newton-Raphson) calls these
+ * entry points where the reference calls its per-element virtuals and its
+ * cSysMatrix.  Plain pointers and sizes only; no exceptions cross the ABI.
+ *
+ * Conventions (mirroring the reference):
+ *   - return 1 = ok, 0 = failure (elc/element.h:813 IntForce, mlib/sysmat.h:90
+ *     Solve); fgpu_last_error() describes the failure.  Nothing calls exit().
+ *   - equations are the reference's VecDof numbering (node/node.h:319-320):
+ *     dofmap[node*3 + dir] = 1-based equation, 0 = fixed (ctrl/ctrl.cpp
+ *     2004-2104); vectors passed as "equation vectors" have neq entries.
+ *   - nodal vectors are [nnode*3], node-major (cNode::VecDispl, 0..2).
+ *   - host pointers unless the name ends in _dev (device pointers on the
+ *     context's HIP device, already resident in HBM).
+ * One context = one mesh (or one batch of independent meshes) = one HIP
+ * stream on the current device.  A context is not re-entrant; independent
+ * contexts may be used from different host threads.
+ */
+#ifndef FGPU_H
+#define FGPU_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct fgpu_ctx fgpu_ctx;
+
+/* Element shape: cShapeBrick8 (shp/shpsolid.cpp:173-238) or cShapeBspSolid
+ * (shp/shpbsp.cpp:873-1067, Bezier extraction per element). */
+enum { FGPU_HEX8 = 0, FGPU_BSP = 1 };
+/* Formulation: cElmParam (elc/elmparam.cpp:449-597, linear), cElmParamTL
+ * (elc/elmparam.cpp:1615-1807, Green-Lagrange + St.Venant-Kirchhoff), or
+ * total-Lagrangian compressible neo-Hookean (no reference counterpart). */
+enum { FGPU_LINEAR = 0, FGPU_TL_SVK = 1, FGPU_TL_NEOHOOKE = 2 };
+
+/* Mesh description, cf. the reference's model tables read by InpReadData
+ * (main/input.cpp:355-401): nodes, one element family, per-element material
+ * (mat/material.cpp:846-851 cMatElastIso: E, nu; neo-Hookean: mu, lambda). */
+typedef struct {
+  int nnode, nelem;
+  int shape;            /* FGPU_HEX8 | FGPU_BSP */
+  int nn;               /* nodes per element: 8, or (p+1)^3 */
+  int deg[3];           /* FGPU_BSP: degree per direction */
+  int nip[3];           /* Gauss points per direction (intpoint.cpp:579-617) */
+  int form;             /* FGPU_LINEAR | FGPU_TL_SVK | FGPU_TL_NEOHOOKE */
+  const double *coords; /* [nnode*3] */
+  const int *conn;      /* [nelem*nn] 0-based, reference local node order */
+  const int *dofmap;    /* [nnode*3] 1-based equation or 0 */
+  const double *ext;    /* FGPU_BSP: [nelem*3*(p+1)^2] extraction rows */
+  const double *cpw;    /* FGPU_BSP: [nnode] weights (NULL = all 1) */
+  const double *mat;    /* [nelem*2] */
+} fgpu_mesh;
+
+/* Build a context: uploads the mesh, numbers the sparsity pattern and the
+ * deterministic assembly maps (replaces cControl::CompProf + CreateMatrix,
+ * ctrl/ctrl.cpp:2112-2157, mlib/sysmat.cpp:42-70). */
+int fgpu_create(fgpu_ctx **ctx, const fgpu_mesh *mesh);
+void fgpu_destroy(fgpu_ctx *ctx);
+const char *fgpu_last_error(fgpu_ctx *ctx);
+int fgpu_neq(fgpu_ctx *ctx);
+long long fgpu_nnz(fgpu_ctx *ctx);
+
+/* Global stiffness: element K_e (FP32) + deterministic assembly into the FP64
+ * CSR.  Replaces cControl::StiffnessMatrix (ctrl/ctrl.cpp:886-975) and the
+ * per-element cElement::StiffMat / AddGlobMat (elc/element.cpp:2975-3002).
+ * u_nodal: current nodal displacements for TL forms, NULL = zero. */
+int fgpu_assemble_K(fgpu_ctx *ctx, const double *u_nodal);
+int fgpu_assemble_K_dev(fgpu_ctx *ctx, const double *u_nodal_dev);
+
+/* Internal force (FP64): replaces cControl::IntForceVector
+ * (ctrl/ctrl.cpp:719-813) with cElement::IntForce + AddGlobVec
+ * (elc/element.cpp:2953-2971).  g has neq entries.  Returns 0 when a stress
+ * update fails (non-positive det C) or an element Jacobian is not positive. */
+int fgpu_int_force(fgpu_ctx *ctx, const double *u_nodal, double *g);
+int fgpu_int_force_dev(fgpu_ctx *ctx, const double *u_nodal_dev, double *g_dev);
+
+/* Add v[k] to the diagonal of equation eq[k] (1-based), as the prescribed-DOF
+ * penalty of ctrl/ctrl.cpp:960-974 does with cSysMatrix::Add. */
+int fgpu_add_diag(fgpu_ctx *ctx, int n, const int *eq, const double *v);
+
+/* Jacobi-PCG in FP64 on the assembled CSR: replaces cSysMatrix::Solve with
+ * PCGSolver (mlib/sysmat.cpp:1061-1125): relres = |r| / |b| (|b| > 1e11 taken
+ * as 1), x0 = 0 unless x0 is given (warm start).  Returns 1 when converged. */
+int fgpu_solve(fgpu_ctx *ctx, const double *b, double *x, const double *x0,
+               int maxit, double rtol, int *iters, double *relres);
+int fgpu_solve_dev(fgpu_ctx *ctx, const double *b_dev, double *x_dev,
+                   const double *x0_dev, int maxit, double rtol, int *iters,
+                   double *relres);
+
+/* y = K x on the assembled CSR (cSymSprsMatrix::MultVect, sysmat.cpp:942). */
+int fgpu_spmv_dev(fgpu_ctx *ctx, const double *x_dev, double *y_dev);
+
+/* Overwrite the CSR values (host array of fgpu_nnz entries, the pattern of
+ * fgpu_get_csr): the counterpart of filling a cSysMatrix through Add
+ * (mlib/sysmat.h:60-99) for callers that assemble on their own. */
+int fgpu_set_csr_values(fgpu_ctx *ctx, const double *val);
+/* Read back the CSR (full symmetric pattern, 0-based, rows in equation
+ * order).  Any pointer may be NULL to skip it. */
+int fgpu_get_csr(fgpu_ctx *ctx, long long *rowptr, int *col, double *val);
+/* Element matrices of elements [e0, e0+ne) as dense row-major
+ * (3nn x 3nn) FP32, element-local DOF order (node-major). */
+int fgpu_get_element_matrices(fgpu_ctx *ctx, int e0, int ne, float *Ke);
+
+/* Time (ms, HIP events on the context stream) of the last call of each
+ * phase: 0 = element K, 1 = assembly, 2 = element f_int (+ gather),
+ * 3 = solve. */
+double fgpu_last_time_ms(fgpu_ctx *ctx, int phase);
+/* Raw device pointers of internal buffers, for benchmarks and tests:
+ * 0 = CSR values (f64), 1 = element matrices (f32), 2 = scratch nodal
+ * vector (f64 [nnode*3]). */
+void *fgpu_device_buffer(fgpu_ctx *ctx, int which);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -762,26 +762,37 @@ static void pattern_rows(const orc_mesh *m, int neq, long long *rowptr,
       int d = m->dofmap[3 * n + c];
       if (d) eqnode[d - 1] = n;
     }
-  long long total = 0;
+  long long total = 0, prev_start = 0;
+  int prev_node = -1, prev_len = 0;
   rowptr[0] = 0;
   for (int r = 0; r < neq; r++) {
     int n = eqnode[r];
+    if (n == prev_node) { /* rows of one node share their columns */
+      if (col) memcpy(col + total, col + prev_start, sizeof(int) * prev_len);
+      total += prev_len;
+      rowptr[r + 1] = total;
+      continue;
+    }
+    /* neighbour nodes (sorted, unique), then their active equations:
+     * equations are node-major, so node order gives equation order */
     int k = 0;
     for (int t = cnt[n]; t < cnt[n + 1]; t++) {
       int e = adj[t];
-      for (int a = 0; a < nn; a++)
-        for (int c = 0; c < 3; c++) {
-          int d = m->dofmap[3 * m->conn[(size_t)e * nn + a] + c];
-          if (d) buf[k++] = d - 1;
-        }
+      for (int a = 0; a < nn; a++) buf[k++] = m->conn[(size_t)e * nn + a];
     }
     qsort(buf, k, sizeof(int), cmp_int);
     int u = 0;
     for (int i = 0; i < k; i++)
-      if (i == 0 || buf[i] != buf[i - 1]) {
-        if (col) col[total + u] = buf[i];
-        u++;
-      }
+      if (i == 0 || buf[i] != buf[i - 1])
+        for (int c = 0; c < 3; c++) {
+          int d = m->dofmap[3 * buf[i] + c];
+          if (!d) continue;
+          if (col) col[total + u] = d - 1;
+          u++;
+        }
+    prev_node = n;
+    prev_start = total;
+    prev_len = u;
     total += u;
     rowptr[r + 1] = total;
   }

@@ -1,0 +1,211 @@
+"""Parity of the HIP path (through the C-ABI) with the CPU oracle, the
+reference's golden fixtures, and size-independent properties at full size.
+
+Tolerances (written per test):
+* element / global stiffness: FP32 element matrices vs the FP64 oracle,
+  normwise relative error |K_gpu - K_64|_F / |K_64|_F <= 1e-6;
+* internal force: FP64 throughout, relative error <= 1e-11;
+* assembly pattern and DOF maps: bit-exact (integer);
+* PCG on identical FP64 matrices: same iteration count as the oracle's
+  restatement of the reference PCG, solution to 1e-10.
+"""
+import numpy as np
+import pytest
+
+from conftest import load_golden, mesh_from_golden
+from fast_amd import mesh as M
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def fg():
+    import torch
+    assert torch.cuda.is_available(), "GPU tests need a HIP device"
+    from fast_amd import fgpu
+    fgpu.lib()
+    return fgpu
+
+
+def nrel(a, b):
+    return np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300)
+
+
+def elem_check(fg, oracle_mod, mesh, u=None, tol=1e-6, elems=None):
+    g = fg.GpuModel(mesh)
+    g.stiffness(u)
+    om = oracle_mod.OracleMesh(mesh)
+    elems = range(mesh.nelem) if elems is None else elems
+    Kg = g.element_matrices(0, mesh.nelem)
+    worst = 0.0
+    for e in elems:
+        Ko, ok = om.elem_stiff(e, u)
+        err = nrel(Kg[e].astype(np.float64), Ko)
+        worst = max(worst, err)
+        assert err <= tol, (e, err)
+    return worst, g, om
+
+
+@pytest.mark.parametrize("form", ["linear", "tl_svk"])
+def test_hex8_element_matrices_vs_reference_fixture(fg, oracle_mod, form):
+    d = load_golden("elem_hex8_%s.npz" % form)
+    m = mesh_from_golden(d)
+    g = fg.GpuModel(m)
+    u = d["u"] if form != "linear" else None
+    g.stiffness(u)
+    Kg = g.element_matrices(0, m.nelem).astype(np.float64)
+    for e in range(m.nelem):
+        assert nrel(Kg[e], d["K"][e]) <= 1e-6
+
+
+@pytest.mark.parametrize("form,matp", [("linear", (200e3, 0.3)), ("tl_svk", (2.6, 0.3)),
+                                        ("tl_neohooke", (1.0, 1.5))])
+@pytest.mark.parametrize("nip", [1, 2, 3])
+def test_hex8_element_matrices_vs_oracle(fg, oracle_mod, form, matp, nip):
+    m = M.hex8_block(3, 2, 2, form=form, nip=nip, perturb=0.3, seed=nip,
+                     mat=np.tile(matp, (12, 1)))
+    u = None
+    if form != "linear":
+        u = np.random.default_rng(3).uniform(-0.05, 0.05, (m.nnode, 3))
+    elem_check(fg, oracle_mod, m, u, tol=1e-6 if nip > 1 else 2e-6)
+
+
+def test_bsp_element_matrices_vs_reference_fixture(fg, oracle_mod):
+    d = load_golden("elem_bsp3_linear.npz")
+    m = mesh_from_golden(d)
+    g = fg.GpuModel(m)
+    g.stiffness()
+    Kg = g.element_matrices(0, m.nelem).astype(np.float64)
+    assert nrel(Kg[0], d["K0"]) <= 1e-6
+    for e in range(m.nelem):
+        assert nrel(Kg[e] @ d["v"], d["Kv"][e]) <= 2e-6
+
+
+def test_bsp_element_matrices_vs_oracle_perturbed_patch(fg, oracle_mod):
+    m = M.bsp_block(ne=(4, 3, 2), p=3, size=(1.0, 0.7, 0.4), E=2.6, nu=0.3, perturb=0.08, seed=21)
+    worst, _, _ = elem_check(fg, oracle_mod, m, tol=1e-6)
+    print("bsp worst element rel err", worst)
+
+
+@pytest.mark.parametrize("kind", ["hex8_linear", "hex8_tl", "hex8_nh", "bsp"])
+def test_assembly_pattern_bitexact_and_values(fg, oracle_mod, kind):
+    rng = np.random.default_rng(8)
+    if kind == "bsp":
+        m = M.bsp_block(ne=(3, 3, 2), p=3, perturb=0.05, seed=2)
+        u = None
+    else:
+        form = {"hex8_linear": "linear", "hex8_tl": "tl_svk", "hex8_nh": "tl_neohooke"}[kind]
+        mat = (1.0, 1.5) if form == "tl_neohooke" else (2.6, 0.3)
+        m = M.hex8_block(5, 4, 3, form=form, perturb=0.25, seed=4, mat=np.tile(mat, (60, 1)))
+        u = None if form == "linear" else rng.uniform(-0.05, 0.05, (m.nnode, 3))
+    g = fg.GpuModel(m)
+    g.stiffness(u)
+    rowptr, col, val = g.csr()
+    om = oracle_mod.OracleMesh(m)
+    r0, c0 = om.pattern()
+    assert np.array_equal(rowptr, r0)
+    assert np.array_equal(col, c0)
+    v0, ok = om.assemble_K(u)
+    assert nrel(val, v0) <= 1e-6
+    # the assembled matrix is exactly symmetric (mirrored blocks)
+    A = om.to_scipy(val)
+    assert (A - A.T).count_nonzero() == 0
+
+
+@pytest.mark.parametrize("form,matp", [("linear", (3.0, 0.25)), ("tl_svk", (2.6, 0.3)),
+                                        ("tl_neohooke", (1.0, 1.5))])
+def test_internal_force_vs_oracle_fp64(fg, oracle_mod, form, matp):
+    m = M.hex8_block(4, 3, 3, form=form, perturb=0.3, seed=5, mat=np.tile(matp, (36, 1)))
+    u = np.random.default_rng(6).uniform(-0.06, 0.06, (m.nnode, 3))
+    g = fg.GpuModel(m).int_force(u)
+    g0, ok = oracle_mod.OracleMesh(m).assemble_fint(u)
+    assert ok
+    assert nrel(g, g0) <= 1e-11
+
+
+def test_internal_force_bsp_vs_reference_fixture(fg, oracle_mod):
+    d = load_golden("elem_bsp3_linear.npz")
+    m = mesh_from_golden(d)
+    gg = fg.GpuModel(m).int_force(d["u"])
+    g0 = np.zeros(m.neq)
+    for e in range(m.nelem):
+        dofs = d["edof"][e]
+        np.add.at(g0, dofs[dofs > 0] - 1, d["g"][e][dofs > 0])
+    assert nrel(gg, g0) <= 1e-11
+
+
+def test_pcg_same_iterations_as_reference_restatement(fg, oracle_mod):
+    m = M.hex8_block(6, 4, 4, perturb=0.2, seed=3, traction=(0, 0, -1.0))
+    om = oracle_mod.OracleMesh(m)
+    v0, _ = om.assemble_K()
+    rowptr, col = om.pattern()
+    f = m.load_vector()
+    u0, it0, rr0, conv0 = oracle_mod.pcg(rowptr, col, v0, f, 10000, 1e-10)
+    g = fg.GpuModel(m)
+    g.set_csr_values(v0)
+    x, it, rr, conv = g.solve(f, tol=1e-10, maxit=10000)
+    assert conv and conv0
+    assert abs(it - it0) <= 1, (it, it0)
+    assert nrel(x, u0) <= 1e-9
+    # warm start from the solution converges at once
+    x2, it2, rr2, conv2 = g.solve(f, x0=u0, tol=1e-8)
+    assert conv2 and it2 <= 1
+
+
+def test_linear_static_cfg1_vs_reference_golden(fg, oracle_mod):
+    d = load_golden("cfg1_linear_u.npz")
+    m = mesh_from_golden(d)
+    g = fg.GpuModel(m)
+    g.stiffness()
+    x, it, rr, conv = g.solve(m.load_vector(), tol=1e-10)
+    assert conv
+    err = nrel(m.scatter(x), d["u"])
+    print("cfg1 u rel err vs reference", err, "iters", it)
+    assert err <= 1e-5
+
+
+def test_newton_raphson_tl_vs_reference_golden(fg, oracle_mod):
+    d = load_golden("nr_tl_hex8_4x4x4.npz")
+    m = mesh_from_golden(d)
+    g = fg.GpuModel(m)
+    q = m.load_vector()
+    u = np.zeros(m.neq)
+    fint = g.int_force(m.scatter(u))
+    lam = 0.0
+    for step in range(5):
+        lam += 0.2
+        f = lam * q
+        r = f - fint
+        for it in range(30):
+            g.stiffness(m.scatter(u))
+            du, _, _, conv = g.solve(r, tol=1e-12)
+            u = u + du
+            fint = g.int_force(m.scatter(u))
+            r = f - fint
+            if np.linalg.norm(r) / max(1.0, np.linalg.norm(f)) < 1e-9:
+                break
+        err = nrel(m.scatter(u), d["u_steps"][step])
+        assert err <= 1e-8, (step, err)
+
+
+def test_full_size_cfg3_properties(fg):
+    """64^3 hex8 (neo-Hookean) at a deformed state: K_t symmetric, and
+    f_int(u + h v) - f_int(u - h v) = 2 h K_t v to FP32 accuracy."""
+    m = M.hex8_block(64, 64, 64, form="tl_neohooke", mat=np.tile((1.0, 1.5), (64 ** 3, 1)))
+    rng = np.random.default_rng(0)
+    X = m.coords
+    u = np.stack([0.02 * X[:, 0] * X[:, 2], 0.01 * X[:, 1] ** 2, -0.03 * X[:, 0] ** 2], 1)
+    u[m.dofmap[:, 0] == 0] = 0.0
+    g = fg.GpuModel(m)
+    g.stiffness(u)
+    rowptr, col, val = g.csr()
+    assert len(val) == 63_695_790
+    v = rng.standard_normal(m.neq)
+    h = 1e-6
+    vp = m.scatter(v)
+    d = (g.int_force(u + h * vp) - g.int_force(u - h * vp)) / (2 * h)
+    import scipy.sparse as sp
+    K = sp.csr_matrix((val, col, rowptr), shape=(m.neq, m.neq))
+    Kv = K @ v
+    assert nrel(Kv, d) <= 1e-5
+    assert abs(K - K.T).max() == 0
