@@ -295,6 +295,40 @@ int check_status(Ctx &c, const char *what) {
 
 }  // namespace
 
+namespace fgpu {
+
+int stiffness_dev(Ctx &c, const double *d_u) {
+  FGPU_CHECK(hipMemsetAsync(c.d_status, 0, 2 * sizeof(int), c.stream));
+  FGPU_CHECK(hipEventRecord(c.ev[0], c.stream));
+  if (c.shape == FGPU_HEX8)
+    launch_hex8_K(c, d_u);
+  else
+    launch_bsp_K(c, d_u);
+  FGPU_CHECK(hipEventRecord(c.ev[1], c.stream));
+  launch_assemble_K(c);
+  FGPU_CHECK(hipEventRecord(c.ev[2], c.stream));
+  FGPU_CHECK(hipEventSynchronize(c.ev[2]));
+  c.last_ms[0] = elapsed(c, 0, 1);
+  c.last_ms[1] = elapsed(c, 1, 2);
+  return check_status(c, "stiffness");
+}
+
+int int_force_dev(Ctx &c, const double *d_u, double *d_g) {
+  FGPU_CHECK(hipMemsetAsync(c.d_status, 0, 2 * sizeof(int), c.stream));
+  FGPU_CHECK(hipEventRecord(c.ev[3], c.stream));
+  if (c.shape == FGPU_HEX8)
+    launch_hex8_fint(c, d_u);
+  else
+    launch_bsp_fint(c, d_u);
+  launch_gather_fint(c, d_g);
+  FGPU_CHECK(hipEventRecord(c.ev[4], c.stream));
+  FGPU_CHECK(hipEventSynchronize(c.ev[4]));
+  c.last_ms[2] = elapsed(c, 3, 4);
+  return check_status(c, "internal force");
+}
+
+}  // namespace fgpu
+
 extern "C" {
 
 int fgpu_create(fgpu_ctx **out, const fgpu_mesh *m) {
@@ -404,21 +438,7 @@ int fgpu_neq(fgpu_ctx *ctx) { return ctx ? reinterpret_cast<Ctx *>(ctx)->neq : 0
 long long fgpu_nnz(fgpu_ctx *ctx) { return ctx ? reinterpret_cast<Ctx *>(ctx)->nnz : 0; }
 
 int fgpu_assemble_K_dev(fgpu_ctx *ctx, const double *d_u) {
-  return guarded(ctx, [&](Ctx &c) {
-    FGPU_CHECK(hipMemsetAsync(c.d_status, 0, 2 * sizeof(int), c.stream));
-    FGPU_CHECK(hipEventRecord(c.ev[0], c.stream));
-    if (c.shape == FGPU_HEX8)
-      launch_hex8_K(c, d_u);
-    else
-      launch_bsp_K(c, d_u);
-    FGPU_CHECK(hipEventRecord(c.ev[1], c.stream));
-    launch_assemble_K(c);
-    FGPU_CHECK(hipEventRecord(c.ev[2], c.stream));
-    FGPU_CHECK(hipEventSynchronize(c.ev[2]));
-    c.last_ms[0] = elapsed(c, 0, 1);
-    c.last_ms[1] = elapsed(c, 1, 2);
-    return check_status(c, "stiffness");
-  });
+  return guarded(ctx, [&](Ctx &c) { return stiffness_dev(c, d_u); });
 }
 
 int fgpu_assemble_K(fgpu_ctx *ctx, const double *u) {
@@ -434,19 +454,7 @@ int fgpu_assemble_K(fgpu_ctx *ctx, const double *u) {
 }
 
 int fgpu_int_force_dev(fgpu_ctx *ctx, const double *d_u, double *d_g) {
-  return guarded(ctx, [&](Ctx &c) {
-    FGPU_CHECK(hipMemsetAsync(c.d_status, 0, 2 * sizeof(int), c.stream));
-    FGPU_CHECK(hipEventRecord(c.ev[3], c.stream));
-    if (c.shape == FGPU_HEX8)
-      launch_hex8_fint(c, d_u);
-    else
-      launch_bsp_fint(c, d_u);
-    launch_gather_fint(c, d_g);
-    FGPU_CHECK(hipEventRecord(c.ev[4], c.stream));
-    FGPU_CHECK(hipEventSynchronize(c.ev[4]));
-    c.last_ms[2] = elapsed(c, 3, 4);
-    return check_status(c, "internal force");
-  });
+  return guarded(ctx, [&](Ctx &c) { return int_force_dev(c, d_u, d_g); });
 }
 
 int fgpu_int_force(fgpu_ctx *ctx, const double *u, double *g) {

@@ -120,5 +120,12 @@ void launch_eq_to_nodal(Ctx &c, const double *eq, double *nodal);
 void launch_add_diag(Ctx &c, int n, const int *d_eq, const double *d_v);
 void launch_expand_Ke(Ctx &c, int e0, int ne, float *d_out);
 void bsp_prepare_tables(Ctx &c, const double *h_ext, const double *h_cpw);
+void launch_axpby(Ctx &c, int n, double a, const double *x, double b, const double *y,
+                  double *z);
+double dev_dot(Ctx &c, const double *x, const double *y, int n);
+// element K + assembly / element f_int + gather on the context stream, with
+// the status check (returns 0 and sets c.err on failure)
+int stiffness_dev(Ctx &c, const double *d_unodal);
+int int_force_dev(Ctx &c, const double *d_unodal, double *d_g);
 
 }  // namespace fgpu

@@ -302,6 +302,32 @@ __global__ void residual_kernel(int n, const double *__restrict__ b, const doubl
   if (i < n) r[i] = b[i] - Ax[i];
 }
 
+__global__ __launch_bounds__(256) void axpby_kernel(int n, double a, const double *__restrict__ x,
+                                                    double b, const double *__restrict__ y,
+                                                    double *__restrict__ z) {
+  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (long long)gridDim.x * blockDim.x)
+    z[i] = a * x[i] + b * (y ? y[i] : 0.0);
+}
+
+__global__ __launch_bounds__(256) void dot_partial_kernel(int n, const double *__restrict__ x,
+                                                          const double *__restrict__ y,
+                                                          double *__restrict__ part) {
+  __shared__ double sh[16];
+  double s = 0.0;
+  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (long long)gridDim.x * blockDim.x)
+    s += x[i] * y[i];
+  double t = block_sum(s, sh);
+  if (threadIdx.x == 0) part[blockIdx.x] = t;
+}
+
+__global__ void dot_finish_kernel(const double *__restrict__ part, double *__restrict__ out) {
+  __shared__ double sh[16];
+  double s = reduce_partials(part, sh);
+  if (threadIdx.x == 0) *out = s;
+}
+
 int spmv_width(const Ctx &c) {
   double avg = c.neq ? (double)c.nnz / c.neq : 1.0;
   if (avg > 256) return 64;
@@ -379,6 +405,26 @@ void launch_nodal_to_eq(Ctx &c, const double *u, double *x) {
 }
 
 void launch_spmv(Ctx &c, const double *x, double *y) { spmv_impl(c, x, y, nullptr, nullptr); }
+
+void launch_axpby(Ctx &c, int n, double a, const double *x, double b, const double *y,
+                  double *z) {
+  int grid = std::min(RED, (n + 255) / 256);
+  if (grid < 1) grid = 1;
+  hipLaunchKernelGGL(axpby_kernel, dim3(grid), dim3(256), 0, c.stream, n, a, x, b, y, z);
+  FGPU_CHECK(hipGetLastError());
+}
+
+double dev_dot(Ctx &c, const double *x, const double *y, int n) {
+  hipLaunchKernelGGL(dot_partial_kernel, dim3(RED), dim3(256), 0, c.stream, n, x, y,
+                     c.d_part + 3 * RED);
+  hipLaunchKernelGGL(dot_finish_kernel, dim3(1), dim3(256), 0, c.stream, c.d_part + 3 * RED,
+                     c.d_scal + 8);
+  FGPU_CHECK(hipGetLastError());
+  FGPU_CHECK(hipMemcpyAsync(c.h_scal + 8, c.d_scal + 8, sizeof(double), hipMemcpyDeviceToHost,
+                            c.stream));
+  FGPU_CHECK(hipStreamSynchronize(c.stream));
+  return c.h_scal[8];
+}
 
 int run_pcg(Ctx &c, const double *b, double *x, const double *x0, int maxit, double rtol,
             int *iters, double *relres) {

@@ -35,7 +35,27 @@ class FgpuMesh(C.Structure):
                 ("mat", C.c_void_p)]
 
 
+PREDICT_FN = C.CFUNCTYPE(C.c_int, C.c_void_p, C.c_int, C.c_void_p, C.c_double, C.c_double,
+                         C.c_void_p)
+
+
+class NrParams(C.Structure):
+    _fields_ = [("steps", C.c_int), ("factor", C.c_double), ("maxit", C.c_int),
+                ("tol", C.c_double), ("pcg_maxit", C.c_int), ("pcg_tol", C.c_double),
+                ("adaptive_pcg", C.c_int), ("predict", PREDICT_FN), ("predict_user", C.c_void_p)]
+
+
+class NrStat(C.Structure):
+    _fields_ = [("step", C.c_int), ("iters", C.c_int), ("pcg_iters", C.c_int),
+                ("converged", C.c_int), ("lam", C.c_double), ("err", C.c_double),
+                ("ms_stiffness", C.c_double), ("ms_fint", C.c_double),
+                ("ms_solve", C.c_double), ("ms_total", C.c_double)]
+
+
 EXPORTS = {
+    "fgpu_linear_static": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int, C.c_double,
+                                     C.c_void_p, C.c_void_p]),
+    "fgpu_newton": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),
     "fgpu_create": (C.c_int, [C.c_void_p, C.c_void_p]),
     "fgpu_destroy": (None, [C.c_void_p]),
     "fgpu_last_error": (C.c_char_p, [C.c_void_p]),
@@ -162,6 +182,44 @@ class GpuModel:
         val = np.zeros(self.nnz)
         self._check(lib().fgpu_get_csr(self.h, _p(rowptr), _p(col), _p(val)), "fgpu_get_csr")
         return rowptr, col, val
+
+    def linear_static(self, f, maxit=None, tol=1e-8):
+        """cLinStat::Solver on the device (ctrl/ctrllins.cpp:46-99)."""
+        f = np.ascontiguousarray(f, dtype=np.float64)
+        u = np.zeros(self.neq)
+        it = C.c_int(0)
+        rr = C.c_double(0.0)
+        ok = lib().fgpu_linear_static(self.h, _p(f), _p(u), 0 if maxit is None else int(maxit),
+                                      tol, C.byref(it), C.byref(rr))
+        self._check(ok, "fgpu_linear_static")
+        return u, it.value, rr.value
+
+    def newton(self, q, steps, factor, maxit=20, tol=1e-5, pcg_tol=1e-5, pcg_maxit=0,
+               adaptive_pcg=False, predict=None):
+        """cCtrlNR::Solver on the device (ctrl/ctrlnr.cpp:47-267).
+
+        ``predict(step, u_dev_ptr, lam_prev, lam, du_dev_ptr) -> bool`` may
+        write a displacement-increment prediction into du (device, neq)."""
+        q = np.ascontiguousarray(q, dtype=np.float64)
+        prm = NrParams()
+        prm.steps, prm.factor, prm.maxit, prm.tol = steps, factor, maxit, tol
+        prm.pcg_maxit, prm.pcg_tol, prm.adaptive_pcg = pcg_maxit, pcg_tol, int(adaptive_pcg)
+        cb = None
+        if predict is not None:
+            def _cb(user, step, u_dev, lam_prev, lam, du_dev):
+                try:
+                    return 1 if predict(step, u_dev, lam_prev, lam, du_dev) else 0
+                except Exception as exc:  # never let an exception cross the ABI
+                    print("predictor failed:", exc)
+                    return 0
+            cb = PREDICT_FN(_cb)
+            prm.predict = cb
+        us = np.zeros((steps, self.neq))
+        stats = (NrStat * steps)()
+        done = lib().fgpu_newton(self.h, _p(q), C.byref(prm), _p(us), stats)
+        st = [{k: getattr(stats[i], k) for k, _ in NrStat._fields_} for i in range(steps)]
+        err = lib().fgpu_last_error(self.h).decode() if done < steps else ""
+        return us[:max(done, 0)], st[:max(done, 0) + (1 if done < steps else 0)], err
 
     def set_csr_values(self, val):
         val = np.ascontiguousarray(val, dtype=np.float64)

@@ -105,6 +105,50 @@ int fgpu_get_csr(fgpu_ctx *ctx, long long *rowptr, int *col, double *val);
  * (3nn x 3nn) FP32, element-local DOF order (node-major). */
 int fgpu_get_element_matrices(fgpu_ctx *ctx, int e0, int ne, float *Ke);
 
+/* ---------------------------------------------------------------- drivers
+ * Analysis drivers running the whole loop on the device (the caller side of
+ * the seam, cf. cLinStat::Solver ctrl/ctrllins.cpp:46-99 and
+ * cCtrlNR::Solver ctrl/ctrlnr.cpp:47-267). */
+
+/* Linear static: assemble K, solve K u = f with PCG to rtol (u: neq). */
+int fgpu_linear_static(fgpu_ctx *ctx, const double *f, double *u, int maxit,
+                       double rtol, int *iters, double *relres);
+
+/* Predictor hook for the Newton driver: given the converged equation vector
+ * u_dev of the previous step and the load factors, write a predicted
+ * displacement increment into du_dev (neq, device).  Return 1 to use it,
+ * 0 to start the step from du = 0. */
+typedef int (*fgpu_predict_fn)(void *user, int step, const double *u_dev,
+                               double lam_prev, double lam, double *du_dev);
+
+typedef struct {
+  int steps;          /* MaxStep */
+  double factor;      /* StepFactor: lambda += factor per step */
+  int maxit;          /* MaxIter: Newton iterations per step */
+  double tol;         /* Tol: |r| / max(1, |f|) */
+  int pcg_maxit;      /* <= 0: neq (as the reference's NR) */
+  double pcg_tol;     /* reference NR: 1e-5 (sysmat.h:77-78 default) */
+  int adaptive_pcg;   /* 1: PCG tolerance min(pcg_tol, 0.1 * err) (inexact
+                         Newton; 0 = fixed pcg_tol as in the reference) */
+  fgpu_predict_fn predict;  /* optional */
+  void *predict_user;
+} fgpu_nr_params;
+
+typedef struct {
+  int step, iters, pcg_iters, converged;
+  double lam, err;
+  double ms_stiffness, ms_fint, ms_solve, ms_total;
+} fgpu_nr_stat;
+
+/* Load-control Newton-Raphson: f = lambda q; full Newton with the tangent
+ * rebuilt every iteration; convergence |r| / max(1, |f|) < tol (ctrl/ctrl.cpp
+ * 1366-1380).  q: neq host vector.  u_steps (optional, host): neq * steps,
+ * the converged u of each step.  stats (optional): one entry per step.
+ * Returns the number of converged steps (the reference stops at the first
+ * non-converged step); 0 with fgpu_last_error on a failure. */
+int fgpu_newton(fgpu_ctx *ctx, const double *q, const fgpu_nr_params *prm,
+                double *u_steps, fgpu_nr_stat *stats);
+
 /* Time (ms, HIP events on the context stream) of the last call of each
  * phase: 0 = element K, 1 = assembly, 2 = element f_int (+ gather),
  * 3 = solve. */

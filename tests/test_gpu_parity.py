@@ -209,3 +209,24 @@ def test_full_size_cfg3_properties(fg):
     Kv = K @ v
     assert nrel(Kv, d) <= 1e-5
     assert abs(K - K.T).max() == 0
+
+
+def test_newton_driver_on_device_vs_reference_golden(fg, oracle_mod):
+    """fgpu_newton (the ctrlnr.cpp loop run natively on the device)."""
+    d = load_golden("nr_tl_hex8_4x4x4.npz")
+    m = mesh_from_golden(d)
+    g = fg.GpuModel(m)
+    us, st, err = g.newton(m.load_vector(), steps=5, factor=0.2, maxit=30, tol=1e-9,
+                           pcg_tol=1e-12)
+    assert len(us) == 5, err
+    for s in range(5):
+        assert nrel(m.scatter(us[s]), d["u_steps"][s]) <= 1e-8
+        assert st[s]["converged"]
+
+
+def test_linear_static_driver(fg, oracle_mod):
+    d = load_golden("cfg1_linear_u.npz")
+    m = mesh_from_golden(d)
+    g = fg.GpuModel(m)
+    u, it, rr = g.linear_static(m.load_vector(), tol=1e-10)
+    assert nrel(m.scatter(u), d["u"]) <= 1e-5
