@@ -204,12 +204,13 @@ __global__ __launch_bounds__(256) void hex8_K_kernel(
       float M[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
       for (int q = 0; q < NQ; q++) {
         const float *S = sh + (size_t)q * NF * TE + el;
+        // c * (g_i h_j): the product is formed before the weight so that a
+        // diagonal block (a == b) comes out exactly symmetric
         float c = S[24 * TE];
-        float ga0 = c * S[(3 * a) * TE], ga1 = c * S[(3 * a + 1) * TE], ga2 = c * S[(3 * a + 2) * TE];
-        float hb0 = S[(3 * b) * TE], hb1 = S[(3 * b + 1) * TE], hb2 = S[(3 * b + 2) * TE];
-        M[0] += ga0 * hb0; M[1] += ga0 * hb1; M[2] += ga0 * hb2;
-        M[3] += ga1 * hb0; M[4] += ga1 * hb1; M[5] += ga1 * hb2;
-        M[6] += ga2 * hb0; M[7] += ga2 * hb1; M[8] += ga2 * hb2;
+        float ga[3] = {S[(3 * a) * TE], S[(3 * a + 1) * TE], S[(3 * a + 2) * TE]};
+        float hb[3] = {S[(3 * b) * TE], S[(3 * b + 1) * TE], S[(3 * b + 2) * TE]};
+        for (int i = 0; i < 3; i++)
+          for (int j = 0; j < 3; j++) M[3 * i + j] += c * (ga[i] * hb[j]);
       }
       float lam, mu;
       lame(mat[2 * (size_t)e], mat[2 * (size_t)e + 1], lam, mu);
@@ -226,13 +227,13 @@ __global__ __launch_bounds__(256) void hex8_K_kernel(
         float c = S[60 * TE];
         float fa[3], fb[3], ga[3], gb[3];
         for (int i = 0; i < 3; i++) {
-          fa[i] = c * S[(3 * a + i) * TE];
+          fa[i] = S[(3 * a + i) * TE];
           fb[i] = S[(3 * b + i) * TE];
           ga[i] = c * S[(24 + 3 * a + i) * TE];
           gb[i] = S[(24 + 3 * b + i) * TE];
         }
         for (int i = 0; i < 3; i++)
-          for (int j = 0; j < 3; j++) M[3 * i + j] += fa[i] * fb[j];
+          for (int j = 0; j < 3; j++) M[3 * i + j] += c * (fa[i] * fb[j]);
         float gg = ga[0] * gb[0] + ga[1] * gb[1] + ga[2] * gb[2];
         for (int k = 0; k < 6; k++) Bs[k] += gg * S[(48 + k) * TE];
         float s0 = S[54 * TE], s1 = S[55 * TE], s2 = S[56 * TE], s3 = S[57 * TE],
