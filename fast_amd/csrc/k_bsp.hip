@@ -12,22 +12,25 @@
 // Stiffness of an isotropic element (64 control points, 64 Gauss points):
 //   P_ij = G_i^T diag(w detJ) G_j      (G_i[q][a] = dN_a/dx_i at point q)
 //   K_ab[i][j] = lambda P_ij[a][b] + mu P_ji[a][b] + mu delta_ij sum_k P_kk[a][b]
-// The six 64x64x64 products P_ij (i <= j; the diagonal ones only on their
-// upper 32x32 tiles) run on v_mfma_f32_32x32x2f32 with FP32 accumulation.
-// One workgroup (4 waves) per element: phase 1 stages G (FP32) in LDS from an
-// FP64 geometry pass, phase 2 runs the 21 output tiles on MFMA into LDS,
-// phase 3 forms the 2080 upper 3x3 node blocks and stores them contiguously.
+// One workgroup (4 waves) per element.  Phase 1 evaluates the geometry in
+// FP64 and stages G (FP32, 48 KB) in LDS.  Phase 2: the 64 x 64 node-block
+// matrix is cut into 16 x 16 tiles; each wave takes upper tile pairs
+// (ta <= tb) and, for its pair, runs all nine products P_ij (i, j = x, y, z)
+// on v_mfma_f32_16x16x4f32 (K = 64 points), sharing the three A and three B
+// operands of every k-step between the nine chains.  All nine accumulators
+// of a tile sit in the same lanes, so K_ab is formed in registers and each
+// lane stores whole 3x3 blocks; no P matrices go through LDS, which keeps the
+// workgroup small enough for three of them to share a CU.
 #include "fgpu_internal.h"
 
 namespace fgpu {
 namespace {
 
-typedef float floatx16 __attribute__((ext_vector_type(16)));
+typedef float floatx4 __attribute__((ext_vector_type(4)));
 
-constexpr int P3 = 3, NB = 4, NQ1 = 4;   // degree, basis and points per direction
+constexpr int NB = 4, NQ1 = 4;   // basis and points per direction
 constexpr int NBAS = 64, NQP = 64;
 constexpr int NSYM = NBAS * (NBAS + 1) / 2;  // 2080
-constexpr int PLD = 65;                      // padded row of a P tile in LDS
 
 __device__ inline void bernstein3(double u, double *B, double *dB) {
   double v = 1.0 - u;
@@ -60,10 +63,8 @@ __device__ inline double inv3d(const double J[3][3], double I[3][3]) {
   return det;
 }
 
-// Shared geometry pass for one element: univariate tables, then for each
-// point q the rational basis derivatives, J, J^-1 and w detJ.  Threads are
-// (q, part) with part owning 16 control points; J partial sums are reduced
-// across the 4 lanes of a point with shuffles.
+// Element geometry staged in LDS: univariate tables at the Gauss points,
+// control points (relative to the first) and weights.
 struct Geo {
   double N[3][NQ1][NB], dN[3][NQ1][NB];  // [dir][point][basis]
   double X[NBAS][3];
@@ -96,8 +97,8 @@ __device__ inline void geo_load(Geo &g, int e, const double *__restrict__ coords
   }
 }
 
-// For point q and control point a: numerator values (N w, dN w) of the
-// tensor-product basis; a = i + 4 j + 16 k with i along r.
+// Numerators (N w, dN w) of the tensor-product basis a at point q;
+// a = i + 4 j + 16 k with i along r, q likewise.
 __device__ inline void tensor(const Geo &g, int q, int a, double &n, double d[3]) {
   const int qr = q % 4, qs = (q / 4) % 4, qt = q / 16;
   const int i = a % 4, j = (a / 4) % 4, k = a / 16;
@@ -109,15 +110,30 @@ __device__ inline void tensor(const Geo &g, int q, int a, double &n, double d[3]
   d[2] = Nr * Ns * g.dN[2][qt][k] * w;
 }
 
-// Computes, for the calling thread's (q, part): J^-1 (all lanes of the point
-// agree) and w detJ.  Returns the rational sums through Wsum, dW.
+// Rational basis derivatives of the 16 control points [16 part, 16 part + 16)
+// at point q (quotient rule of shpbsp.cpp:1049-1066), their Jacobian and its
+// inverse.  The four lanes of a point (consecutive lanes) reduce their
+// partial sums with shuffles, so all of them hold the same J^-1.
+struct PointGeo {
+  double Ji[3][3];
+  double detJ, c, inv_s, dW[3];
+};
+
+// d R_a / d(r,s,t) at point q (rational quotient with the point's sums)
+__device__ inline void rational_dR(const Geo &g, int q, int a, const PointGeo &P, double dR[3]) {
+  double n, d[3];
+  tensor(g, q, a, n, d);
+  const double R = n * P.inv_s;
+  for (int i = 0; i < 3; i++) dR[i] = (d[i] - R * P.dW[i]) * P.inv_s;
+}
+
 __device__ inline void point_geometry(const Geo &g, int q, int part, const double *gp,
-                                      double Ji[3][3], double &c, double &Wsum, double dW[3],
-                                      double &detJ) {
+                                      PointGeo &P) {
   double s = 0.0, ds[3] = {0, 0, 0};
-  for (int a = part * 16; a < part * 16 + 16; a++) {
+  #pragma unroll 2
+  for (int k = 0; k < 16; k++) {
     double n, d[3];
-    tensor(g, q, a, n, d);
+    tensor(g, q, part * 16 + k, n, d);
     s += n;
     ds[0] += d[0];
     ds[1] += d[1];
@@ -127,44 +143,40 @@ __device__ inline void point_geometry(const Geo &g, int q, int part, const doubl
     s += __shfl_xor(s, m);
     for (int k = 0; k < 3; k++) ds[k] += __shfl_xor(ds[k], m);
   }
-  Wsum = s;
-  for (int k = 0; k < 3; k++) dW[k] = ds[k];
+  P.inv_s = 1.0 / s;
+  for (int k = 0; k < 3; k++) P.dW[k] = ds[k];
   double J[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
-  for (int a = part * 16; a < part * 16 + 16; a++) {
-    double n, d[3];
-    tensor(g, q, a, n, d);
-    double R = n / s;
-    for (int i = 0; i < 3; i++) {
-      double dR = (d[i] - R * dW[i]) / s;
-      for (int k = 0; k < 3; k++) J[i][k] += dR * g.X[a][k];
-    }
+  #pragma unroll 2
+  for (int k = 0; k < 16; k++) {
+    const int a = part * 16 + k;
+    double dR[3];
+    rational_dR(g, q, a, P, dR);
+    for (int i = 0; i < 3; i++)
+      for (int l = 0; l < 3; l++) J[i][l] += dR[i] * g.X[a][l];
   }
   for (int m = 1; m < 4; m <<= 1)
     for (int i = 0; i < 3; i++)
-      for (int k = 0; k < 3; k++) J[i][k] += __shfl_xor(J[i][k], m);
-  detJ = inv3d(J, Ji);
+      for (int l = 0; l < 3; l++) J[i][l] += __shfl_xor(J[i][l], m);
+  P.detJ = inv3d(J, P.Ji);
   const int qr = q % 4, qs = (q / 4) % 4, qt = q / 16;
-  c = gp[(0 * 16 + qr) * 2 + 1] * gp[(1 * 16 + qs) * 2 + 1] * gp[(2 * 16 + qt) * 2 + 1] * detJ;
+  P.c = gp[(0 * 16 + qr) * 2 + 1] * gp[(1 * 16 + qs) * 2 + 1] * gp[(2 * 16 + qt) * 2 + 1] * P.detJ;
 }
 
 struct KShared {
   Geo g;
-  float G[3][NQP][NBAS];   // dN_a/dx_i at point q
-  float c[NQP];            // w detJ
-  float P[6][NBAS][PLD];   // xx yy zz xy xz yz
+  float G[3][NQP][NBAS];  // dN_a/dx_i at point q
+  float c[NQP];           // w detJ
 };
 
-__constant__ unsigned char kTileI[21] = {0, 0, 0, 1, 1, 1, 2, 2, 2, 0, 0, 0, 0, 0, 0, 0, 0, 1, 1, 1, 1};
-__constant__ unsigned char kTileJ[21] = {0, 0, 0, 1, 1, 1, 2, 2, 2, 1, 1, 1, 1, 2, 2, 2, 2, 2, 2, 2, 2};
-__constant__ unsigned char kTileP[21] = {0, 0, 0, 1, 1, 1, 2, 2, 2, 3, 3, 3, 3, 4, 4, 4, 4, 5, 5, 5, 5};
-__constant__ unsigned char kTileA[21] = {0, 0, 1, 0, 0, 1, 0, 0, 1, 0, 0, 1, 1, 0, 0, 1, 1, 0, 0, 1, 1};
-__constant__ unsigned char kTileB[21] = {0, 1, 1, 0, 1, 1, 0, 1, 1, 0, 1, 0, 1, 0, 1, 0, 1, 0, 1, 0, 1};
+// upper 16x16 tile pairs (ta <= tb) of the 64 x 64 node-block matrix
+__constant__ unsigned char kPairA[10] = {0, 0, 0, 0, 1, 1, 1, 2, 2, 3};
+__constant__ unsigned char kPairB[10] = {0, 1, 2, 3, 1, 2, 3, 2, 3, 3};
 
-__global__ __launch_bounds__(256) void bsp_K_kernel(
+__global__ __launch_bounds__(256, 3) void bsp_K_kernel(
     int nelem, const double *__restrict__ coords, const int *__restrict__ conn,
     const double *__restrict__ cpw, const double *__restrict__ ext,
-    const double *__restrict__ mat, const double *__restrict__ gp,
-    const unsigned short *__restrict__ blkab, float *__restrict__ Ke, int *__restrict__ status) {
+    const double *__restrict__ mat, const double *__restrict__ gp, float *__restrict__ Ke,
+    int *__restrict__ status) {
   extern __shared__ __align__(16) unsigned char smem[];
   KShared &S = *reinterpret_cast<KShared *>(smem);
   const int e = blockIdx.x;
@@ -173,76 +185,80 @@ __global__ __launch_bounds__(256) void bsp_K_kernel(
   __syncthreads();
   {  // phase 1: G and c
     const int q = tid / 4, part = tid % 4;
-    double Ji[3][3], c, Ws, dW[3], det;
-    point_geometry(S.g, q, part, gp, Ji, c, Ws, dW, det);
+    PointGeo P;
+    point_geometry(S.g, q, part, gp, P);
     if (part == 0) {
-      S.c[q] = (float)c;
-      if (!(det > 0.0)) atomicAdd(&status[0], 1);
+      S.c[q] = (float)P.c;
+      if (!(P.detJ > 0.0)) atomicAdd(&status[0], 1);
     }
-    for (int a = part * 16; a < part * 16 + 16; a++) {
-      double n, d[3];
-      tensor(S.g, q, a, n, d);
-      double R = n / Ws, dR[3];
-      for (int i = 0; i < 3; i++) dR[i] = (d[i] - R * dW[i]) / Ws;
+    #pragma unroll 2
+    for (int k = 0; k < 16; k++) {
+      double dR[3];
+      rational_dR(S.g, q, part * 16 + k, P, dR);
       for (int i = 0; i < 3; i++)
-        S.G[i][q][a] = (float)(Ji[i][0] * dR[0] + Ji[i][1] * dR[1] + Ji[i][2] * dR[2]);
+        S.G[i][q][part * 16 + k] =
+            (float)(P.Ji[i][0] * dR[0] + P.Ji[i][1] * dR[1] + P.Ji[i][2] * dR[2]);
     }
   }
   __syncthreads();
-  {  // phase 2: 21 tiles of 32x32 (K = 64 points) on MFMA
-    const int wave = tid / 64, lane = tid % 64;
-    const int lm = lane % 32, lk = lane / 32;
-    for (int t = wave; t < 21; t += 4) {
-      const int i = kTileI[t], j = kTileJ[t], p = kTileP[t];
-      const int a0 = 32 * kTileA[t], b0 = 32 * kTileB[t];
-      floatx16 acc;
-      for (int r = 0; r < 16; r++) acc[r] = 0.f;
-#pragma unroll 8
-      for (int k = 0; k < NQP; k += 2) {
-        const int q = k + lk;
-        float av = S.c[q] * S.G[i][q][a0 + lm];
-        float bv = S.G[j][q][b0 + lm];
-        acc = __builtin_amdgcn_mfma_f32_32x32x2f32(av, bv, acc, 0, 0, 0);
+  // phase 2: nine 16x16x64 products per tile pair, combined in registers
+  const double E = mat[2 * (size_t)e], nu = mat[2 * (size_t)e + 1];
+  const float lam = (float)(E * nu / ((1.0 + nu) * (1.0 - 2.0 * nu)));
+  const float mu = (float)(0.5 * E / (1.0 + nu));
+  const int wave = tid / 64, lane = tid % 64;
+  const int ln = lane % 16, lk = lane / 16;
+  float *out = Ke + (size_t)e * NSYM * 9;
+  for (int tp = wave; tp < 10; tp += 4) {
+    const int a0 = 16 * kPairA[tp], b0 = 16 * kPairB[tp];
+    floatx4 acc[9];
+    for (int t = 0; t < 9; t++)
+      for (int r = 0; r < 4; r++) acc[t][r] = 0.f;
+#pragma unroll 4
+    for (int k0 = 0; k0 < NQP; k0 += 4) {
+      const int q = k0 + lk;
+      const float cq = S.c[q];
+      float A[3], B[3];
+      for (int i = 0; i < 3; i++) {
+        A[i] = cq * S.G[i][q][a0 + ln];
+        B[i] = S.G[i][q][b0 + ln];
       }
-      // D[m][n]: n = lane % 32, m = 8 (r / 4) + 4 (lane / 32) + r % 4
-      for (int r = 0; r < 16; r++) {
-        const int m = 8 * (r / 4) + 4 * lk + (r % 4);
-        S.P[p][a0 + m][b0 + lm] = acc[r];
-      }
-    }
-  }
-  __syncthreads();
-  {  // phase 3: K_ab for a <= b
-    const double E = mat[2 * (size_t)e], nu = mat[2 * (size_t)e + 1];
-    const float lam = (float)(E * nu / ((1.0 + nu) * (1.0 - 2.0 * nu)));
-    const float mu = (float)(0.5 * E / (1.0 + nu));
-    float *out = Ke + (size_t)e * NSYM * 9;
-    for (int blk = tid; blk < NSYM; blk += blockDim.x) {
-      const unsigned short ab = blkab[blk];
-      const int a = ab >> 8, b = ab & 255;
-      float Pab[3][3];
-      Pab[0][0] = S.P[0][a][b];
-      Pab[1][1] = S.P[1][a][b];
-      Pab[2][2] = S.P[2][a][b];
-      Pab[0][1] = S.P[3][a][b];
-      Pab[1][0] = S.P[3][b][a];
-      Pab[0][2] = S.P[4][a][b];
-      Pab[2][0] = S.P[4][b][a];
-      Pab[1][2] = S.P[5][a][b];
-      Pab[2][1] = S.P[5][b][a];
-      const float tr = mu * (Pab[0][0] + Pab[1][1] + Pab[2][2]);
-      float *o = out + (size_t)blk * 9;
       for (int i = 0; i < 3; i++)
         for (int j = 0; j < 3; j++)
-          o[3 * i + j] = lam * Pab[i][j] + mu * Pab[j][i] + (i == j ? tr : 0.f);
+          acc[3 * i + j] = __builtin_amdgcn_mfma_f32_16x16x4f32(A[i], B[j], acc[3 * i + j], 0, 0, 0);
+    }
+    // lane holds D[m][n] with n = lane % 16, m = 4 (lane / 16) + r
+    for (int r = 0; r < 4; r++) {
+      const int a = a0 + 4 * lk + r, b = b0 + ln;
+      if (a > b) continue;
+      float P[3][3];
+      for (int i = 0; i < 3; i++)
+        for (int j = 0; j < 3; j++) P[i][j] = acc[3 * i + j][r];
+      if (a == b) {  // exact symmetry of diagonal blocks
+        for (int i = 0; i < 3; i++)
+          for (int j = i + 1; j < 3; j++) {
+            float s = 0.5f * (P[i][j] + P[j][i]);
+            P[i][j] = s;
+            P[j][i] = s;
+          }
+      }
+      const float tr = mu * (P[0][0] + P[1][1] + P[2][2]);
+      float *o = out + (size_t)sym_index(a, b, NBAS) * 9;
+      for (int i = 0; i < 3; i++)
+        for (int j = 0; j < 3; j++)
+          o[3 * i + j] = lam * P[i][j] + mu * P[j][i] + (i == j ? tr : 0.f);
     }
   }
 }
 
 // ---------------------------------------------------------------- f_int
+// Phase 1, thread (q, part): geometry, then grad u at q from its 16 control
+// points (shuffle-reduced), then c * sigma.  Phase 2, thread (a, quarter):
+// g_a = sum_q c sigma_q . grad N_a(q) over 16 points, recomputing grad N_a
+// from the tensor tables and the stored J^-1, reduced over the 4 quarters.
 struct FShared {
   Geo g;
-  double G[3][NQP][NBAS];
+  double Ji[NQP][9];
+  double rs[NQP][4];   // 1 / W, dW/dr, dW/ds, dW/dt
   double sig[NQP][6];  // c * sigma (Voigt xx yy zz xy xz yz)
 };
 
@@ -260,20 +276,16 @@ __global__ __launch_bounds__(256) void bsp_fint_kernel(
   __syncthreads();
   {
     const int q = tid / 4, part = tid % 4;
-    double Ji[3][3], c, Ws, dW[3], det;
-    point_geometry(S.g, q, part, gp, Ji, c, Ws, dW, det);
-    if (part == 0 && !(det > 0.0)) atomicAdd(&status[0], 1);
+    PointGeo P;
+    point_geometry(S.g, q, part, gp, P);
     double L[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
-    for (int a = part * 16; a < part * 16 + 16; a++) {
-      double n, d[3];
-      tensor(S.g, q, a, n, d);
-      double R = n / Ws, dR[3], G[3];
-      for (int i = 0; i < 3; i++) dR[i] = (d[i] - R * dW[i]) / Ws;
-      for (int i = 0; i < 3; i++) {
-        G[i] = Ji[i][0] * dR[0] + Ji[i][1] * dR[1] + Ji[i][2] * dR[2];
-        S.G[i][q][a] = G[i];
-      }
-      const double *ua = unodal + 3 * (size_t)cn[a];
+    #pragma unroll 2
+    for (int k = 0; k < 16; k++) {
+      const double *ua = unodal + 3 * (size_t)cn[part * 16 + k];
+      double dR[3], G[3];
+      rational_dR(S.g, q, part * 16 + k, P, dR);
+      for (int i = 0; i < 3; i++)
+        G[i] = P.Ji[i][0] * dR[0] + P.Ji[i][1] * dR[1] + P.Ji[i][2] * dR[2];
       for (int i = 0; i < 3; i++)
         for (int j = 0; j < 3; j++) L[i][j] += ua[i] * G[j];
     }
@@ -281,9 +293,14 @@ __global__ __launch_bounds__(256) void bsp_fint_kernel(
       for (int i = 0; i < 3; i++)
         for (int j = 0; j < 3; j++) L[i][j] += __shfl_xor(L[i][j], m);
     if (part == 0) {
+      if (!(P.detJ > 0.0)) atomicAdd(&status[0], 1);
+      for (int i = 0; i < 3; i++)
+        for (int j = 0; j < 3; j++) S.Ji[q][3 * i + j] = P.Ji[i][j];
+      S.rs[q][0] = P.inv_s;
+      for (int k = 0; k < 3; k++) S.rs[q][1 + k] = P.dW[k];
       const double E = mat[2 * (size_t)e], nu = mat[2 * (size_t)e + 1];
       const double lam = E * nu / ((1.0 + nu) * (1.0 - 2.0 * nu)), mu = 0.5 * E / (1.0 + nu);
-      double tr = L[0][0] + L[1][1] + L[2][2];
+      const double tr = L[0][0] + L[1][1] + L[2][2], c = P.c;
       S.sig[q][0] = c * (lam * tr + 2.0 * mu * L[0][0]);
       S.sig[q][1] = c * (lam * tr + 2.0 * mu * L[1][1]);
       S.sig[q][2] = c * (lam * tr + 2.0 * mu * L[2][2]);
@@ -293,32 +310,44 @@ __global__ __launch_bounds__(256) void bsp_fint_kernel(
     }
   }
   __syncthreads();
-  if (tid < NBAS * 3) {
-    const int a = tid / 3, i = tid % 3;
-    // sigma row i in Voigt positions
-    const int v0 = i == 0 ? 0 : (i == 1 ? 3 : 4);
-    const int v1 = i == 0 ? 3 : (i == 1 ? 1 : 5);
-    const int v2 = i == 0 ? 4 : (i == 1 ? 5 : 2);
-    double s = 0.0;
-    for (int q = 0; q < NQP; q++)
-      s += S.sig[q][v0] * S.G[0][q][a] + S.sig[q][v1] * S.G[1][q][a] + S.sig[q][v2] * S.G[2][q][a];
-    ge[((size_t)e * NBAS + a) * 3 + i] = s;
+  {
+    const int a = tid / 4, quarter = tid % 4;
+    double g0 = 0.0, g1 = 0.0, g2 = 0.0;
+    for (int q = quarter * 16; q < quarter * 16 + 16; q++) {
+      double n, d[3];
+      tensor(S.g, q, a, n, d);
+      const double is = S.rs[q][0];
+      const double R = n * is;
+      double dR[3], G[3];
+      for (int i = 0; i < 3; i++) dR[i] = (d[i] - R * S.rs[q][1 + i]) * is;
+      for (int i = 0; i < 3; i++)
+        G[i] = S.Ji[q][3 * i] * dR[0] + S.Ji[q][3 * i + 1] * dR[1] + S.Ji[q][3 * i + 2] * dR[2];
+      const double *s = S.sig[q];
+      g0 += s[0] * G[0] + s[3] * G[1] + s[4] * G[2];
+      g1 += s[3] * G[0] + s[1] * G[1] + s[5] * G[2];
+      g2 += s[4] * G[0] + s[5] * G[1] + s[2] * G[2];
+    }
+    for (int m = 1; m < 4; m <<= 1) {
+      g0 += __shfl_xor(g0, m);
+      g1 += __shfl_xor(g1, m);
+      g2 += __shfl_xor(g2, m);
+    }
+    if (quarter == 0) {
+      double *o = ge + ((size_t)e * NBAS + a) * 3;
+      o[0] = g0;
+      o[1] = g1;
+      o[2] = g2;
+    }
   }
 }
 
-unsigned short *g_blkab = nullptr;  // shared (a << 8 | b) table, per process
-int g_blkab_dev = -1;
-
-unsigned short *blkab_table(Ctx &c) {
-  if (g_blkab && g_blkab_dev == c.device) return g_blkab;
-  std::vector<unsigned short> t;
-  for (int a = 0; a < NBAS; a++)
-    for (int b = a; b < NBAS; b++) t.push_back((unsigned short)(a << 8 | b));
-  FGPU_CHECK(hipMalloc(&g_blkab, t.size() * sizeof(unsigned short)));
-  FGPU_CHECK(hipMemcpy(g_blkab, t.data(), t.size() * sizeof(unsigned short),
-                       hipMemcpyHostToDevice));
-  g_blkab_dev = c.device;
-  return g_blkab;
+template <class K>
+void set_lds(K kernel, size_t lds, int dev) {
+  static int done_dev = -1;
+  if (done_dev == dev) return;
+  FGPU_CHECK(hipFuncSetAttribute((const void *)kernel,
+                                 hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+  done_dev = dev;
 }
 
 }  // namespace
@@ -326,47 +355,29 @@ unsigned short *blkab_table(Ctx &c) {
 void bsp_prepare_tables(Ctx &c, const double *h_ext, const double *h_cpw) {
   // extraction rows and weights stay FP64 on the device
   size_t n = (size_t)c.nelem * 3 * 16;
-  double *d_ext = nullptr, *d_w = nullptr;
-  FGPU_CHECK(hipMalloc(&d_ext, n * sizeof(double)));
-  FGPU_CHECK(hipMemcpyAsync(d_ext, h_ext, n * sizeof(double), hipMemcpyHostToDevice, c.stream));
+  FGPU_CHECK(hipMalloc(&c.d_ext, n * sizeof(double)));
+  FGPU_CHECK(hipMemcpyAsync(c.d_ext, h_ext, n * sizeof(double), hipMemcpyHostToDevice, c.stream));
   if (h_cpw) {
-    FGPU_CHECK(hipMalloc(&d_w, c.nnode * sizeof(double)));
-    FGPU_CHECK(hipMemcpyAsync(d_w, h_cpw, c.nnode * sizeof(double), hipMemcpyHostToDevice,
+    FGPU_CHECK(hipMalloc(&c.d_cpw, c.nnode * sizeof(double)));
+    FGPU_CHECK(hipMemcpyAsync(c.d_cpw, h_cpw, c.nnode * sizeof(double), hipMemcpyHostToDevice,
                               c.stream));
   }
-  c.d_ext = d_ext;
-  c.d_cpw = d_w;
-  blkab_table(c);
 }
 
 void launch_bsp_K(Ctx &c, const double *u) {
   (void)u;
-  size_t lds = sizeof(KShared);
-  static bool attr = false;
-  if (!attr) {
-    FGPU_CHECK(hipFuncSetAttribute((const void *)bsp_K_kernel,
-                                   hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-    attr = true;
-  }
-  hipLaunchKernelGGL(bsp_K_kernel, dim3(c.nelem), dim3(256), lds, c.stream, c.nelem,
-                     c.d_coords, c.d_conn, c.d_cpw,
-                     c.d_ext, c.d_mat, c.d_gauss64,
-                     blkab_table(c), c.d_Ke, c.d_status);
+  const size_t lds = sizeof(KShared);
+  set_lds(bsp_K_kernel, lds, c.device);
+  hipLaunchKernelGGL(bsp_K_kernel, dim3(c.nelem), dim3(256), lds, c.stream, c.nelem, c.d_coords,
+                     c.d_conn, c.d_cpw, c.d_ext, c.d_mat, c.d_gauss64, c.d_Ke, c.d_status);
   FGPU_CHECK(hipGetLastError());
 }
 
 void launch_bsp_fint(Ctx &c, const double *u) {
-  size_t lds = sizeof(FShared);
-  static bool attr = false;
-  if (!attr) {
-    FGPU_CHECK(hipFuncSetAttribute((const void *)bsp_fint_kernel,
-                                   hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-    attr = true;
-  }
+  const size_t lds = sizeof(FShared);
   hipLaunchKernelGGL(bsp_fint_kernel, dim3(c.nelem), dim3(256), lds, c.stream, c.nelem,
-                     c.d_coords, c.d_conn, c.d_cpw,
-                     c.d_ext, c.d_mat, u, c.d_gauss64,
-                     c.d_ge, c.d_status);
+                     c.d_coords, c.d_conn, c.d_cpw, c.d_ext, c.d_mat, u, c.d_gauss64, c.d_ge,
+                     c.d_status);
   FGPU_CHECK(hipGetLastError());
 }
 

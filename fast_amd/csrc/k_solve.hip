@@ -35,6 +35,7 @@ __global__ __launch_bounds__(256) void assemble_kernel(
   if (k >= nblk) return;
   double s[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
   const long long c0 = cptr[k], c1 = cptr[k + 1];
+#pragma unroll 4
   for (long long t = c0; t < c1; t++) {
     const int code = contrib[t];
     const int x = code >> 1;
