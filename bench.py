@@ -84,12 +84,11 @@ def accuracy(model, m, u, nsample=32):
     from oracle import oracle as O
     om = O.OracleMesh(m)
     idx = np.linspace(0, m.nelem - 1, nsample).astype(int)
-    Kg = model.element_matrices(0, m.nelem) if m.nelem * (3 * m.nn) ** 2 < 2e8 else None
     num = den = 0.0
     worst_fw = 0.0
     for e in idx:
         K0, _ = om.elem_stiff(int(e), u if m.form != "linear" else None)
-        Ke = (Kg[e] if Kg is not None else model.element_matrices(int(e), 1)[0]).astype(np.float64)
+        Ke = model.element_matrices(int(e), 1)[0].astype(np.float64)
         ue = u[m.conn[e]].ravel()
         d = (Ke - K0) @ ue
         num += d @ d
@@ -105,6 +104,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--workload", default="iga", choices=["iga", "hex8"])
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--solve", action="store_true", help="also time a plain PCG solve to 1e-8")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py: this build measures one GPU only (multi-GPU runs are not part of "
@@ -163,6 +163,20 @@ def main():
                 "kernel": "hex8_K_kernel", "bytes_per_elem": bytes_per,
                 "kernel_ms": round(float(tk), 4)}
     nrm, fwd = accuracy(model, m, u)
+    solve = None
+    if args.solve:
+        # plain Jacobi-PCG on the tangent at u, right-hand side = reference load
+        f = m.load_vector() if m.fext is not None else model.int_force(u)
+        b = torch.from_numpy(f).cuda()
+        x = torch.zeros_like(b)
+        it = fgpu.C.c_int(0)
+        rr = fgpu.C.c_double(0.0)
+        model.stiffness(u)
+        conv = L.fgpu_solve_dev(model.h, b.data_ptr(), x.data_ptr(), None, 100000, 1e-8,
+                                fgpu.C.byref(it), fgpu.C.byref(rr))
+        ms = L.fgpu_last_time_ms(model.h, 3)
+        solve = {"rtol": 1e-8, "iters": it.value, "relres": rr.value, "converged": bool(conv),
+                 "ms": round(ms, 3), "ms_per_iter": round(ms / max(it.value, 1), 4)}
     cpu = None
     if not args.no_cpu:
         res, ncores = cpu_baseline(m, u)
@@ -181,7 +195,7 @@ def main():
         "phase_ms": {"element_K": round(float(tk), 4), "assembly": round(float(ta), 4),
                      "f_int": round(float(tf), 4)},
         "accuracy": {"Ku_normwise_rel": nrm, "Ku_forward_rel_max": fwd},
-        "roofline": roof, "cpu_baseline": cpu,
+        "roofline": roof, "cpu_baseline": cpu, "solve_plain_pcg": solve,
     }
     print(json.dumps(out))
 

@@ -141,11 +141,15 @@ void build_maps(Ctx &c, const int *conn) {
     rowlen[A] = off;
     nb_up[A] = up;
   });
-  auto col_offset = [&](int A, int B) {  // column offset of B in the rows of A
+  auto blk_pos = [&](int A, int B) {  // position of block (A, B) in the node-block rows
     const int *b0 = nbidx.data() + nbcnt[A], *b1 = nbidx.data() + nbcnt[A + 1];
     const int *it = std::lower_bound(b0, b1, B);
-    return nbcol[nbcnt[A] + (it - b0)];
+    return nbcnt[A] + (long long)(it - b0);
   };
+  auto col_offset = [&](int A, int B) {  // column offset of B in the rows of A
+    return nbcol[blk_pos(A, B)];
+  };
+  if (nbcnt[nnode] >= (1LL << 31)) throw Error("mesh too large for 32-bit block positions");
   // CSR row pointers in equation order (equations are node-major)
   std::vector<long long> rowptr(c.neq + 1, 0);
   for (int n = 0; n < nnode; n++)
@@ -213,19 +217,34 @@ void build_maps(Ctx &c, const int *conn) {
       int B = items[i].first;
       blkA[bk] = A;
       blkB[bk] = B;
-      offAB[bk] = col_offset(A, B);
-      offBA[bk] = col_offset(B, A);
+      offAB[bk] = (int)blk_pos(A, B);
+      offBA[bk] = (int)blk_pos(B, A);
       cptr[bk] = cc;
       while (i < items.size() && items[i].first == B) contrib[cc++] = items[i++].second;
       bk++;
     }
   });
 
+  // node-block (3x3) matrix: rows = nodes, blocks = sorted neighbours
+  c.nbf = nbcnt[nnode];
+  std::vector<long long> bdiag(nnode);
+  std::vector<unsigned char> fmask(nnode);
+  for (int n = 0; n < nnode; n++) {
+    bdiag[n] = blk_pos(n, n);
+    unsigned char mk = 0;
+    for (int d = 0; d < 3; d++)
+      if (c.eqmap[3 * n + d] >= 0) mk |= (unsigned char)(1 << d);
+    fmask[n] = mk;
+  }
   hipStream_t s = c.stream;
   c.d_rowptr = dupload(rowptr.data(), rowptr.size(), s);
   c.d_col = dupload(col.data(), col.size(), s);
-  c.d_diag = dupload(diag.data(), diag.size(), s);
-  c.d_val = dalloc<double>(c.nnz);
+  c.d_bptr = dupload(nbcnt.data(), nbcnt.size(), s);
+  c.d_bcol = dupload(nbidx.data(), nbidx.size(), s);
+  c.d_bcoloff = dupload(nbcol.data(), nbcol.size(), s);
+  c.d_bdiag = dupload(bdiag.data(), bdiag.size(), s);
+  c.d_fmask = dupload(fmask.data(), fmask.size(), s);
+  c.d_bval = dalloc<double>((size_t)c.nbf * 9);
   c.d_blkA = dupload(blkA.data(), blkA.size(), s);
   c.d_blkB = dupload(blkB.data(), blkB.size(), s);
   c.d_offAB = dupload(offAB.data(), offAB.size(), s);
@@ -241,11 +260,12 @@ void destroy(Ctx *c) {
   if (!c) return;
   void *bufs[] = {c->d_coords, c->d_conn, c->d_eqmap, c->d_mat, c->d_ext,
                   c->d_cpw, c->d_gauss, c->d_gauss64, c->d_Ke, c->d_ge,
-                  c->d_status, c->d_rowptr, c->d_col, c->d_val, c->d_diag,
+                  c->d_status, c->d_rowptr, c->d_col, c->d_bptr, c->d_bcol,
+                  c->d_bcoloff, c->d_bdiag, c->d_fmask, c->d_bval, c->d_bn,
                   c->d_blkA, c->d_blkB, c->d_offAB, c->d_offBA, c->d_cptr,
                   c->d_contrib, c->d_nptr, c->d_nadj, c->d_u, c->d_b, c->d_x,
                   c->d_r, c->d_p, c->d_w, c->d_z, c->d_dinv, c->d_part,
-                  c->d_scal, c->d_done};
+                  c->d_scal, c->d_done, c->d_eqnode};
   for (void *p : bufs)
     if (p) (void)hipFree(p);
   if (c->h_done) (void)hipHostFree(c->h_done);
@@ -378,6 +398,13 @@ int fgpu_create(fgpu_ctx **out, const fgpu_mesh *m) {
     c->d_coords = dupload(m->coords, (size_t)m->nnode * 3, s);
     c->d_conn = dupload(m->conn, (size_t)m->nelem * m->nn, s);
     c->d_eqmap = dupload(c->eqmap.data(), c->eqmap.size(), s);
+    {
+      std::vector<int> eqnode(std::max(neq, 1), 0);
+      for (long i = 0; i < (long)m->nnode * 3; i++)
+        if (c->eqmap[i] >= 0) eqnode[c->eqmap[i]] = (int)i;
+      c->d_eqnode = dupload(eqnode.data(), eqnode.size(), s);
+      FGPU_CHECK(hipStreamSynchronize(s));
+    }
     c->d_mat = dupload(m->mat, (size_t)m->nelem * 2, s);
     double gp[3 * 2 * 16];
     float gpf[3 * 2 * 16];
@@ -402,7 +429,8 @@ int fgpu_create(fgpu_ctx **out, const fgpu_mesh *m) {
     c->d_ge = dalloc<double>((size_t)c->nelem * c->nn * 3);
     if (c->shape == FGPU_BSP) bsp_prepare_tables(*c, m->ext, m->cpw);
     c->d_u = dalloc<double>((size_t)c->nnode * 3);
-    size_t n = (size_t)std::max(c->neq, 1);
+    size_t n = (size_t)std::max(3 * c->nnode, std::max(c->neq, 1));
+    c->d_bn = dalloc<double>(n);
     c->d_b = dalloc<double>(n);
     c->d_x = dalloc<double>(n);
     c->d_r = dalloc<double>(n);
@@ -487,7 +515,9 @@ int fgpu_add_diag(fgpu_ctx *ctx, int n, const int *eq, const double *v) {
 
 int fgpu_spmv_dev(fgpu_ctx *ctx, const double *x, double *y) {
   return guarded(ctx, [&](Ctx &c) {
-    launch_spmv(c, x, y);
+    launch_eq_to_nodal(c, x, c.d_bn);
+    launch_spmv_nodal(c, c.d_bn, c.d_w);
+    launch_nodal_to_eq(c, c.d_w, y);
     FGPU_CHECK(hipStreamSynchronize(c.stream));
     return 1;
   });
@@ -535,9 +565,14 @@ int fgpu_get_csr(fgpu_ctx *ctx, long long *rowptr, int *col, double *val) {
     if (col)
       FGPU_CHECK(hipMemcpyAsync(col, c.d_col, sizeof(int) * c.nnz, hipMemcpyDeviceToHost,
                                 c.stream));
-    if (val)
-      FGPU_CHECK(hipMemcpyAsync(val, c.d_val, sizeof(double) * c.nnz,
-                                hipMemcpyDeviceToHost, c.stream));
+    if (val) {
+      double *d = dalloc<double>(c.nnz);
+      launch_csr_convert(c, d, 1);
+      FGPU_CHECK(hipMemcpyAsync(val, d, sizeof(double) * c.nnz, hipMemcpyDeviceToHost,
+                                c.stream));
+      FGPU_CHECK(hipStreamSynchronize(c.stream));
+      (void)hipFree(d);
+    }
     FGPU_CHECK(hipStreamSynchronize(c.stream));
     return 1;
   });
@@ -546,9 +581,10 @@ int fgpu_get_csr(fgpu_ctx *ctx, long long *rowptr, int *col, double *val) {
 int fgpu_set_csr_values(fgpu_ctx *ctx, const double *val) {
   return guarded(ctx, [&](Ctx &c) {
     if (!val) throw Error("null argument");
-    FGPU_CHECK(hipMemcpyAsync(c.d_val, val, sizeof(double) * c.nnz, hipMemcpyHostToDevice,
-                              c.stream));
+    double *d = dupload(val, c.nnz, c.stream);
+    launch_csr_convert(c, d, 0);
     FGPU_CHECK(hipStreamSynchronize(c.stream));
+    (void)hipFree(d);
     return 1;
   });
 }
@@ -577,7 +613,7 @@ void *fgpu_device_buffer(fgpu_ctx *ctx, int which) {
   Ctx *c = reinterpret_cast<Ctx *>(ctx);
   if (!c) return nullptr;
   switch (which) {
-    case 0: return c->d_val;
+    case 0: return c->d_bval;
     case 1: return c->d_Ke;
     case 2: return c->d_u;
     default: return nullptr;

@@ -58,6 +58,7 @@ struct Ctx {
   double *d_coords = nullptr;   // [nnode*3]
   int *d_conn = nullptr;        // [nelem*nn]
   int *d_eqmap = nullptr;       // [nnode*3]
+  int *d_eqnode = nullptr;      // [neq] 3 node + direction of each equation
   double *d_mat = nullptr;      // [nelem*2]
   double *d_ext = nullptr;      // B-spline extraction rows [nelem*3*16]
   double *d_cpw = nullptr;      // [nnode] control-point weights
@@ -70,18 +71,26 @@ struct Ctx {
   double *d_ge = nullptr;       // element internal forces [nelem*nn*3]
   int *d_status = nullptr;      // [2]: bad-Jacobian count, stress failure
 
-  // ---- CSR (full symmetric pattern over equations)
+  // ---- global matrix: node-block rows of 3x3 FP64 blocks (all nodes; a
+  // fixed direction has a unit diagonal and no coupling), the solver's
+  // storage.  The equation CSR of the reference (rows/columns = equations,
+  // full symmetric pattern) is kept as a pattern for export.
+  long long nbf = 0;               // stored blocks
+  long long *d_bptr = nullptr;     // [nnode+1]
+  int *d_bcol = nullptr;           // [nbf] neighbour node
+  int *d_bcoloff = nullptr;        // [nbf] column offset of the block in the equation rows
+  long long *d_bdiag = nullptr;    // [nnode] position of the diagonal block
+  unsigned char *d_fmask = nullptr;  // [nnode] bit d set = direction d free
+  double *d_bval = nullptr;        // [nbf*9]
   long long nnz = 0;
-  long long *d_rowptr = nullptr;  // [neq+1]
+  long long *d_rowptr = nullptr;  // [neq+1] equation CSR pattern
   int *d_col = nullptr;           // [nnz]
-  double *d_val = nullptr;        // [nnz]
-  long long *d_diag = nullptr;    // [neq] position of the diagonal entry
 
   // ---- assembly map: upper node blocks (A <= B)
   long long nblk = 0;
   int *d_blkA = nullptr, *d_blkB = nullptr;       // [nblk]
-  int *d_offAB = nullptr;  // [nblk] column offset of node B inside the rows of A
-  int *d_offBA = nullptr;  // [nblk] column offset of node A inside the rows of B
+  int *d_offAB = nullptr;  // [nblk] position of block (A, B) among the node blocks
+  int *d_offBA = nullptr;  // [nblk] position of block (B, A)
   long long *d_cptr = nullptr;   // [nblk+1] contribution segments
   int *d_contrib = nullptr;      // [ncontrib] (e * nsym + sym) << 1 | transpose
   long long ncontrib = 0;
@@ -91,8 +100,9 @@ struct Ctx {
 
   // ---- solver workspace
   double *d_u = nullptr;      // nodal vector scratch [nnode*3]
+  // PCG vectors on nodes [nnode*3]; d_b is an equation-vector scratch
   double *d_b = nullptr, *d_x = nullptr, *d_r = nullptr, *d_p = nullptr,
-         *d_w = nullptr, *d_z = nullptr, *d_dinv = nullptr;  // [neq]
+         *d_w = nullptr, *d_z = nullptr, *d_dinv = nullptr, *d_bn = nullptr;
   double *d_part = nullptr;   // partial sums
   double *d_scal = nullptr;   // device scalars
   int *d_done = nullptr;      // convergence iteration (0 = running)
@@ -112,7 +122,8 @@ void launch_bsp_K(Ctx &c, const double *d_unodal);
 void launch_bsp_fint(Ctx &c, const double *d_unodal);
 void launch_assemble_K(Ctx &c);
 void launch_gather_fint(Ctx &c, double *d_g);
-void launch_spmv(Ctx &c, const double *x, double *y);
+void launch_spmv_nodal(Ctx &c, const double *x, double *y);  // node vectors
+void launch_csr_convert(Ctx &c, double *d_csrval, int to_csr);
 int run_pcg(Ctx &c, const double *b, double *x, const double *x0, int maxit,
             double rtol, int *iters, double *relres);
 void launch_nodal_to_eq(Ctx &c, const double *nodal, double *eq);

@@ -1,12 +1,21 @@
-// k_solve.hip - deterministic assembly, internal-force gather, CSR SpMV and
+// k_solve.hip - deterministic assembly, internal-force gather, block SpMV and
 // the FP64 Jacobi-PCG of libfgpu (gfx950).
+//
+// Global matrix storage: node-block rows of 3x3 FP64 blocks ("block CSR"),
+// one row per node, blocks in increasing neighbour order.  A fixed direction
+// keeps a unit diagonal and no coupling, so the node-space system restricted
+// to the free directions is exactly the reference's equation system; the
+// equation CSR (rows = equations) is produced on export.  Per stored block
+// the SpMV reads 72 B of values and one 4 B column index (12 B per nonzero
+// in scalar CSR).
 //
 // Assembly replaces the element loop + cElement::AddGlobMat of
 // ctrl/ctrl.cpp:886-975 / elc/element.cpp:2975-3002: one thread per upper
 // node block (A <= B) sums, in FP64 and in element order, the FP32 element
 // blocks listed for it (a segmented reduction over contributions pre-sorted
-// by row block), and writes the 3x3 result into the rows of A and, mirrored,
-// of B.  Every CSR value is written exactly once; no atomics.
+// by row block), then stores the 3x3 result at (A, B) and its transpose at
+// (B, A), each as 9 contiguous doubles.  Every value is written exactly once;
+// no atomics.
 //
 // PCG restates PCGSolver (mlib/sysmat.cpp:1061-1125): Jacobi preconditioner,
 // x0 = 0 (or a caller warm start), stop when |r| / |b| < tol with |b| > 1e11
@@ -27,10 +36,10 @@ constexpr int TPB = 256;
 
 __global__ __launch_bounds__(256) void assemble_kernel(
     long long nblk, const int *__restrict__ blkA, const int *__restrict__ blkB,
-    const int *__restrict__ offAB, const int *__restrict__ offBA,
+    const int *__restrict__ posAB, const int *__restrict__ posBA,
     const long long *__restrict__ cptr, const int *__restrict__ contrib,
-    const float *__restrict__ Ke, KeLayout L, const long long *__restrict__ rowptr,
-    const int *__restrict__ eqmap, double *__restrict__ val) {
+    const float *__restrict__ Ke, KeLayout L, const unsigned char *__restrict__ fmask,
+    double *__restrict__ bval) {
   long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   if (k >= nblk) return;
   double s[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
@@ -52,28 +61,18 @@ __global__ __launch_bounds__(256) void assemble_kernel(
     }
   }
   const int A = blkA[k], B = blkB[k];
-  int eA[3], eB[3];
-  for (int d = 0; d < 3; d++) {
-    eA[d] = eqmap[3 * A + d];
-    eB[d] = eqmap[3 * B + d];
-  }
-  const int oAB = offAB[k];
-  for (int i = 0; i < 3; i++) {
-    if (eA[i] < 0) continue;
-    double *row = val + rowptr[eA[i]] + oAB;
-    int jj = 0;
-    for (int j = 0; j < 3; j++)
-      if (eB[j] >= 0) row[jj++] = s[3 * i + j];
-  }
-  if (A != B) {
-    const int oBA = offBA[k];
+  const unsigned mA = fmask[A], mB = fmask[B];
+  for (int i = 0; i < 3; i++)
     for (int j = 0; j < 3; j++) {
-      if (eB[j] < 0) continue;
-      double *row = val + rowptr[eB[j]] + oBA;
-      int ii = 0;
-      for (int i = 0; i < 3; i++)
-        if (eA[i] >= 0) row[ii++] = s[3 * i + j];
+      const bool live = ((mA >> i) & 1u) && ((mB >> j) & 1u);
+      s[3 * i + j] = live ? s[3 * i + j] : ((A == B && i == j) ? 1.0 : 0.0);
     }
+  double *o = bval + 9 * (size_t)posAB[k];
+  for (int m = 0; m < 9; m++) o[m] = s[m];
+  if (A != B) {
+    double *t = bval + 9 * (size_t)posBA[k];
+    for (int i = 0; i < 3; i++)
+      for (int j = 0; j < 3; j++) t[3 * j + i] = s[3 * i + j];
   }
 }
 
@@ -96,10 +95,14 @@ __global__ __launch_bounds__(256) void gather_fint_kernel(
 }
 
 __global__ void add_diag_kernel(int n, const int *__restrict__ eq, const double *__restrict__ v,
-                                const long long *__restrict__ diag, double *__restrict__ val) {
+                                const int *__restrict__ eqnode,
+                                const long long *__restrict__ bdiag, double *__restrict__ bval) {
   // sequential in k so repeated equations add in order
   if (blockIdx.x == 0 && threadIdx.x == 0)
-    for (int k = 0; k < n; k++) val[diag[eq[k] - 1]] += v[k];
+    for (int k = 0; k < n; k++) {
+      const int nd = eqnode[eq[k] - 1];
+      bval[9 * bdiag[nd / 3] + 4 * (nd % 3)] += v[k];
+    }
 }
 
 __global__ void expand_Ke_kernel(int e0, int ne, int nn, KeLayout L, const float *__restrict__ Ke,
@@ -132,6 +135,37 @@ __global__ void nodal_to_eq_kernel(int nnode, const int *__restrict__ eqmap,
   if (q >= 0) x[q] = u[i];
 }
 
+// equation CSR values <-> node-block values (export / import)
+__global__ void bsr_to_csr_kernel(int nnode, const long long *__restrict__ bptr,
+                                  const int *__restrict__ bcol, const int *__restrict__ bcoloff,
+                                  const double *__restrict__ bval, const int *__restrict__ eqmap,
+                                  const long long *__restrict__ rowptr, double *__restrict__ val,
+                                  int to_csr) {
+  const int A = blockIdx.x * blockDim.x + threadIdx.x;
+  if (A >= nnode) return;
+  for (long long k = bptr[A]; k < bptr[A + 1]; k++) {
+    const int B = bcol[k];
+    for (int i = 0; i < 3; i++) {
+      const int qi = eqmap[3 * A + i];
+      if (qi < 0) continue;
+      long long p = rowptr[qi] + bcoloff[k];
+      for (int j = 0; j < 3; j++) {
+        if (eqmap[3 * B + j] < 0) continue;
+        if (to_csr)
+          val[p] = bval[9 * k + 3 * i + j];
+        else
+          const_cast<double *>(bval)[9 * k + 3 * i + j] = val[p];
+        p++;
+      }
+    }
+    if (!to_csr)  // fixed directions: unit diagonal, no coupling
+      for (int i = 0; i < 3; i++)
+        for (int j = 0; j < 3; j++)
+          if (eqmap[3 * A + i] < 0 || eqmap[3 * B + j] < 0)
+            const_cast<double *>(bval)[9 * k + 3 * i + j] = (A == B && i == j) ? 1.0 : 0.0;
+  }
+}
+
 // ---------------------------------------------------------------- reductions
 __device__ inline double block_sum(double v, double *sh) {
   // fixed-shape tree: deterministic for a given block size
@@ -157,26 +191,40 @@ __device__ inline double reduce_partials(const double *__restrict__ part, double
   return sh[8];
 }
 
-// y = A x over rows, W lanes per row; optional fused partial dot(x, y).
-template <int W>
-__global__ __launch_bounds__(256) void spmv_kernel(
-    int n, const long long *__restrict__ rowptr, const int *__restrict__ col,
-    const double *__restrict__ val, const double *__restrict__ x, double *__restrict__ y,
+// y = K x on node-block rows, G lanes per row (lane s takes blocks s, s+G..);
+// optional fused partial dot(x, y).
+template <int G>
+__global__ __launch_bounds__(256) void bspmv_kernel(
+    int nnode, const long long *__restrict__ bptr, const int *__restrict__ bcol,
+    const double *__restrict__ bval, const double *__restrict__ x, double *__restrict__ y,
     double *__restrict__ part, const int *__restrict__ done) {
   __shared__ double sh[16];
   if (done && *done) return;
-  const int sub = threadIdx.x % W;
-  const int rows_per_block = blockDim.x / W;
+  const int sub = threadIdx.x % G;
+  const int rows_per_block = blockDim.x / G;
   double dot = 0.0;
-  for (long long r = (long long)blockIdx.x * rows_per_block + threadIdx.x / W; r < n;
+  for (long long r = (long long)blockIdx.x * rows_per_block + threadIdx.x / G; r < nnode;
        r += (long long)gridDim.x * rows_per_block) {
-    double s = 0.0;
-    const long long p0 = rowptr[r], p1 = rowptr[r + 1];
-    for (long long p = p0 + sub; p < p1; p += W) s += val[p] * x[col[p]];
-    for (int o = W / 2; o > 0; o >>= 1) s += __shfl_down(s, o, W);
+    double y0 = 0.0, y1 = 0.0, y2 = 0.0;
+    const long long k1 = bptr[r + 1];
+    for (long long k = bptr[r] + sub; k < k1; k += G) {
+      const double *v = bval + 9 * k;
+      const int B = bcol[k];
+      const double x0 = x[3 * B], x1 = x[3 * B + 1], x2 = x[3 * B + 2];
+      y0 += v[0] * x0 + v[1] * x1 + v[2] * x2;
+      y1 += v[3] * x0 + v[4] * x1 + v[5] * x2;
+      y2 += v[6] * x0 + v[7] * x1 + v[8] * x2;
+    }
+    for (int o = G / 2; o > 0; o >>= 1) {
+      y0 += __shfl_down(y0, o, G);
+      y1 += __shfl_down(y1, o, G);
+      y2 += __shfl_down(y2, o, G);
+    }
     if (sub == 0) {
-      y[r] = s;
-      dot += s * x[r];
+      y[3 * r] = y0;
+      y[3 * r + 1] = y1;
+      y[3 * r + 2] = y2;
+      dot += y0 * x[3 * r] + y1 * x[3 * r + 1] + y2 * x[3 * r + 2];
     }
   }
   if (part) {
@@ -187,7 +235,7 @@ __global__ __launch_bounds__(256) void spmv_kernel(
 
 // PCG state on the device (scal):
 //   [0] lenf   [1],[2] z.r (double-buffered by iteration parity)   [3] |r|/lenf
-//   done[0] = iteration at convergence (0 = running), done[1] = breakdown
+//   [4] |b|    done[0] = iteration at convergence (0 = running), done[1] = breakdown
 __global__ __launch_bounds__(256) void pcg_init_kernel(
     int n, const double *__restrict__ b, double *__restrict__ r, const double *__restrict__ dinv,
     double *__restrict__ z, double *__restrict__ p, double *__restrict__ part) {
@@ -214,7 +262,7 @@ __global__ __launch_bounds__(256) void pcg_init_kernel(
 }
 
 __global__ void pcg_start_kernel(const double *__restrict__ part, double *__restrict__ scal,
-                                 int *__restrict__ done, double tol) {
+                                 int *__restrict__ done) {
   __shared__ double sh[16];
   double zr = reduce_partials(part, sh);
   double bb = reduce_partials(part + RED, sh);
@@ -291,10 +339,10 @@ __global__ __launch_bounds__(256) void pcg_dir_kernel(
   }
 }
 
-__global__ void diag_inv_kernel(int n, const long long *__restrict__ diag,
-                                const double *__restrict__ val, double *__restrict__ dinv) {
+__global__ void diag_inv_kernel(int nnode, const long long *__restrict__ bdiag,
+                                const double *__restrict__ bval, double *__restrict__ dinv) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n) dinv[i] = 1.0 / val[diag[i]];
+  if (i < 3 * nnode) dinv[i] = 1.0 / bval[9 * bdiag[i / 3] + 4 * (i % 3)];
 }
 
 __global__ void residual_kernel(int n, const double *__restrict__ b, const double *__restrict__ Ax,
@@ -329,37 +377,32 @@ __global__ void dot_finish_kernel(const double *__restrict__ part, double *__res
   if (threadIdx.x == 0) *out = s;
 }
 
-int spmv_width(const Ctx &c) {
-  double avg = c.neq ? (double)c.nnz / c.neq : 1.0;
-  if (avg > 256) return 64;
-  if (avg > 96) return 32;
-  if (avg > 24) return 16;
-  return 8;
+int spmv_group(const Ctx &c) {
+  double avg = c.nnode ? (double)c.nbf / c.nnode : 1.0;
+  if (avg > 96) return 64;
+  if (avg > 40) return 32;
+  if (avg > 20) return 16;
+  if (avg > 6) return 8;
+  return 4;
+}
+
+template <int G>
+void bspmv_launch(Ctx &c, const double *x, double *y, double *part, const int *done) {
+  const int rows_per_block = TPB / G;
+  long long want = (c.nnode + rows_per_block - 1) / rows_per_block;
+  int grid = (int)std::min<long long>(want, 4 * RED);
+  if (part) grid = RED;  // partial slots are all written
+  hipLaunchKernelGGL(bspmv_kernel<G>, dim3(grid), dim3(TPB), 0, c.stream, c.nnode, c.d_bptr,
+                     c.d_bcol, c.d_bval, x, y, part, done);
 }
 
 void spmv_impl(Ctx &c, const double *x, double *y, double *part, const int *done) {
-  const int W = spmv_width(c);
-  const int rows_per_block = TPB / W;
-  long long want = (c.neq + rows_per_block - 1) / rows_per_block;
-  int grid = (int)std::min<long long>(want, RED);
-  if (part) grid = RED;  // partial slots are all written
-  switch (W) {
-    case 64:
-      hipLaunchKernelGGL(spmv_kernel<64>, dim3(grid), dim3(TPB), 0, c.stream, c.neq,
-                         c.d_rowptr, c.d_col, c.d_val, x, y, part, done);
-      break;
-    case 32:
-      hipLaunchKernelGGL(spmv_kernel<32>, dim3(grid), dim3(TPB), 0, c.stream, c.neq,
-                         c.d_rowptr, c.d_col, c.d_val, x, y, part, done);
-      break;
-    case 16:
-      hipLaunchKernelGGL(spmv_kernel<16>, dim3(grid), dim3(TPB), 0, c.stream, c.neq,
-                         c.d_rowptr, c.d_col, c.d_val, x, y, part, done);
-      break;
-    default:
-      hipLaunchKernelGGL(spmv_kernel<8>, dim3(grid), dim3(TPB), 0, c.stream, c.neq,
-                         c.d_rowptr, c.d_col, c.d_val, x, y, part, done);
-      break;
+  switch (spmv_group(c)) {
+    case 64: bspmv_launch<64>(c, x, y, part, done); break;
+    case 32: bspmv_launch<32>(c, x, y, part, done); break;
+    case 16: bspmv_launch<16>(c, x, y, part, done); break;
+    case 8: bspmv_launch<8>(c, x, y, part, done); break;
+    default: bspmv_launch<4>(c, x, y, part, done); break;
   }
   FGPU_CHECK(hipGetLastError());
 }
@@ -370,7 +413,7 @@ void launch_assemble_K(Ctx &c) {
   int grid = (int)((c.nblk + 255) / 256);
   hipLaunchKernelGGL(assemble_kernel, dim3(grid), dim3(256), 0, c.stream, c.nblk, c.d_blkA,
                      c.d_blkB, c.d_offAB, c.d_offBA, c.d_cptr, c.d_contrib, c.d_Ke, c.kel,
-                     c.d_rowptr, c.d_eqmap, c.d_val);
+                     c.d_fmask, c.d_bval);
   FGPU_CHECK(hipGetLastError());
 }
 
@@ -381,8 +424,8 @@ void launch_gather_fint(Ctx &c, double *g) {
 }
 
 void launch_add_diag(Ctx &c, int n, const int *d_eq, const double *d_v) {
-  hipLaunchKernelGGL(add_diag_kernel, dim3(1), dim3(64), 0, c.stream, n, d_eq, d_v, c.d_diag,
-                     c.d_val);
+  hipLaunchKernelGGL(add_diag_kernel, dim3(1), dim3(64), 0, c.stream, n, d_eq, d_v, c.d_eqnode,
+                     c.d_bdiag, c.d_bval);
   FGPU_CHECK(hipGetLastError());
 }
 
@@ -390,6 +433,13 @@ void launch_expand_Ke(Ctx &c, int e0, int ne, float *out) {
   long long tot = (long long)ne * 9 * c.nn * c.nn;
   hipLaunchKernelGGL(expand_Ke_kernel, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0,
                      c.stream, e0, ne, c.nn, c.kel, c.d_Ke, out);
+  FGPU_CHECK(hipGetLastError());
+}
+
+void launch_csr_convert(Ctx &c, double *d_csrval, int to_csr) {
+  hipLaunchKernelGGL(bsr_to_csr_kernel, dim3((c.nnode + 255) / 256), dim3(256), 0, c.stream,
+                     c.nnode, c.d_bptr, c.d_bcol, c.d_bcoloff, c.d_bval, c.d_eqmap, c.d_rowptr,
+                     d_csrval, to_csr);
   FGPU_CHECK(hipGetLastError());
 }
 
@@ -405,7 +455,9 @@ void launch_nodal_to_eq(Ctx &c, const double *u, double *x) {
   FGPU_CHECK(hipGetLastError());
 }
 
-void launch_spmv(Ctx &c, const double *x, double *y) { spmv_impl(c, x, y, nullptr, nullptr); }
+void launch_spmv_nodal(Ctx &c, const double *x, double *y) {
+  spmv_impl(c, x, y, nullptr, nullptr);
+}
 
 void launch_axpby(Ctx &c, int n, double a, const double *x, double b, const double *y,
                   double *z) {
@@ -427,58 +479,69 @@ double dev_dot(Ctx &c, const double *x, const double *y, int n) {
   return c.h_scal[8];
 }
 
+// b, x, x0: equation vectors on the device.  The iteration runs on node
+// vectors (3 nnode entries, fixed directions identically zero).
 int run_pcg(Ctx &c, const double *b, double *x, const double *x0, int maxit, double rtol,
             int *iters, double *relres) {
-  const int n = c.neq;
+  const int n = 3 * c.nnode;
   hipStream_t s = c.stream;
   if (iters) *iters = 0;
   if (relres) *relres = 0.0;
-  if (n == 0) return 1;
+  if (c.neq == 0) return 1;
   const int eb = (n + 255) / 256;
-  hipLaunchKernelGGL(diag_inv_kernel, dim3(eb), dim3(256), 0, s, n, c.d_diag, c.d_val, c.d_dinv);
+  hipLaunchKernelGGL(diag_inv_kernel, dim3(eb), dim3(256), 0, s, c.nnode, c.d_bdiag, c.d_bval,
+                     c.d_dinv);
+  launch_eq_to_nodal(c, b, c.d_bn);
   if (x0) {
-    FGPU_CHECK(hipMemcpyAsync(x, x0, sizeof(double) * n, hipMemcpyDeviceToDevice, s));
-    spmv_impl(c, x, c.d_w, nullptr, nullptr);
-    hipLaunchKernelGGL(residual_kernel, dim3(eb), dim3(256), 0, s, n, b, c.d_w, c.d_r);
+    launch_eq_to_nodal(c, x0, c.d_x);
+    spmv_impl(c, c.d_x, c.d_w, nullptr, nullptr);
+    hipLaunchKernelGGL(residual_kernel, dim3(eb), dim3(256), 0, s, n, c.d_bn, c.d_w, c.d_r);
   } else {
-    FGPU_CHECK(hipMemsetAsync(x, 0, sizeof(double) * n, s));
-    FGPU_CHECK(hipMemcpyAsync(c.d_r, b, sizeof(double) * n, hipMemcpyDeviceToDevice, s));
+    FGPU_CHECK(hipMemsetAsync(c.d_x, 0, sizeof(double) * n, s));
+    FGPU_CHECK(hipMemcpyAsync(c.d_r, c.d_bn, sizeof(double) * n, hipMemcpyDeviceToDevice, s));
   }
-  hipLaunchKernelGGL(pcg_init_kernel, dim3(RED), dim3(TPB), 0, s, n, b, c.d_r, c.d_dinv, c.d_z,
-                     c.d_p, c.d_part);
-  hipLaunchKernelGGL(pcg_start_kernel, dim3(1), dim3(TPB), 0, s, c.d_part, c.d_scal, c.d_done,
-                     rtol);
+  hipLaunchKernelGGL(pcg_init_kernel, dim3(RED), dim3(TPB), 0, s, n, c.d_bn, c.d_r, c.d_dinv,
+                     c.d_z, c.d_p, c.d_part);
+  hipLaunchKernelGGL(pcg_start_kernel, dim3(1), dim3(TPB), 0, s, c.d_part, c.d_scal, c.d_done);
   FGPU_CHECK(hipGetLastError());
   FGPU_CHECK(hipMemcpyAsync(c.h_scal, c.d_scal, 5 * sizeof(double), hipMemcpyDeviceToHost, s));
   FGPU_CHECK(hipStreamSynchronize(s));
-  // reference: |b| < tol returns at once (x stays at its start value)
-  if (c.h_scal[4] < rtol) return 1;
-  if (c.h_scal[3] < rtol) {  // warm start already converged
-    if (relres) *relres = c.h_scal[3];
-    return 1;
-  }
-  const int CHUNK = 16;
-  int launched = 0;
   int conv_it = 0, broke = 0;
-  while (launched < maxit) {
-    int upto = std::min(maxit, launched + CHUNK);
-    for (int it = launched + 1; it <= upto; it++) {
-      spmv_impl(c, c.d_p, c.d_w, c.d_part, c.d_done);
-      hipLaunchKernelGGL(pcg_update_kernel, dim3(RED), dim3(TPB), 0, s, n, it, x, c.d_r, c.d_z,
-                         c.d_p, c.d_w, c.d_dinv, c.d_part, c.d_scal, c.d_done);
-      hipLaunchKernelGGL(pcg_dir_kernel, dim3(RED), dim3(TPB), 0, s, n, it, c.d_p, c.d_z,
-                         c.d_part, c.d_scal, c.d_done, rtol);
+  bool early = false;
+  // reference: |b| < tol returns at once (x stays at its start value)
+  if (c.h_scal[4] < rtol) early = true;
+  if (!early && c.h_scal[3] < rtol) {  // warm start already converged
+    early = true;
+  }
+  if (!early) {
+    const int CHUNK = 16;
+    int launched = 0;
+    while (launched < maxit) {
+      int upto = std::min(maxit, launched + CHUNK);
+      for (int it = launched + 1; it <= upto; it++) {
+        spmv_impl(c, c.d_p, c.d_w, c.d_part, c.d_done);
+        hipLaunchKernelGGL(pcg_update_kernel, dim3(RED), dim3(TPB), 0, s, n, it, c.d_x, c.d_r,
+                           c.d_z, c.d_p, c.d_w, c.d_dinv, c.d_part, c.d_scal, c.d_done);
+        hipLaunchKernelGGL(pcg_dir_kernel, dim3(RED), dim3(TPB), 0, s, n, it, c.d_p, c.d_z,
+                           c.d_part, c.d_scal, c.d_done, rtol);
+      }
+      FGPU_CHECK(hipGetLastError());
+      launched = upto;
+      FGPU_CHECK(hipMemcpyAsync(c.h_done, c.d_done, 2 * sizeof(int), hipMemcpyDeviceToHost, s));
+      FGPU_CHECK(hipMemcpyAsync(c.h_scal, c.d_scal, 4 * sizeof(double), hipMemcpyDeviceToHost, s));
+      FGPU_CHECK(hipStreamSynchronize(s));
+      if (c.h_done[0]) {
+        conv_it = c.h_done[0];
+        broke = c.h_done[1];
+        break;
+      }
     }
-    FGPU_CHECK(hipGetLastError());
-    launched = upto;
-    FGPU_CHECK(hipMemcpyAsync(c.h_done, c.d_done, 2 * sizeof(int), hipMemcpyDeviceToHost, s));
-    FGPU_CHECK(hipMemcpyAsync(c.h_scal, c.d_scal, 4 * sizeof(double), hipMemcpyDeviceToHost, s));
-    FGPU_CHECK(hipStreamSynchronize(s));
-    if (c.h_done[0]) {
-      conv_it = c.h_done[0];
-      broke = c.h_done[1];
-      break;
-    }
+  }
+  launch_nodal_to_eq(c, c.d_x, x);
+  FGPU_CHECK(hipStreamSynchronize(s));
+  if (early) {
+    if (relres) *relres = c.h_scal[4] < rtol ? 0.0 : c.h_scal[3];
+    return 1;
   }
   if (iters) *iters = conv_it ? conv_it : maxit;
   if (relres) *relres = c.h_scal[3];
