@@ -36,6 +36,14 @@ def make_workload(name):
         X = m.coords
         u = 1e-3 * np.stack([np.sin(3 * X[:, 0]) * X[:, 2], np.cos(2 * X[:, 1]) * X[:, 0],
                              X[:, 0] * X[:, 1]], 1)
+    elif name == "batch":
+        m, prm = M.cube_batch(1024, nmin=4, nmax=32, seed=0)
+        cfg = {"workload": "hex8_batch_1024_cubes", "meshes": 1024, "elements": m.nelem,
+               "nqp": 8, "ndof_elem": 24, "neq": int(m.neq), "edge_n": "U{4..32} seed 0",
+               "nu": "U(0.2,0.45)", "E": 1.0, "gauss": "2x2x2"}
+        X = m.coords
+        u = 1e-3 * np.stack([np.sin(3 * X[:, 0]) * X[:, 2], np.cos(2 * X[:, 1]) * X[:, 0],
+                             X[:, 0] * X[:, 1]], 1)
     else:
         n = 64
         m = M.hex8_block(n, n, n, form="tl_neohooke", mat=np.tile((1.0, 1.5), (n ** 3, 1)),
@@ -108,7 +116,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--workload", default="iga", choices=["iga", "hex8"])
+    ap.add_argument("--workload", default="iga", choices=["iga", "hex8", "batch"])
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--solve", action="store_true", help="also time a plain PCG solve to 1e-8")
     args = ap.parse_args()
@@ -180,11 +188,23 @@ def main():
         it = fgpu.C.c_int(0)
         rr = fgpu.C.c_double(0.0)
         model.stiffness(u)
-        conv = L.fgpu_solve_dev(model.h, b.data_ptr(), x.data_ptr(), None, 100000, 1e-8,
-                                fgpu.C.byref(it), fgpu.C.byref(rr))
-        ms = L.fgpu_last_time_ms(model.h, 3)
-        solve = {"rtol": 1e-8, "iters": it.value, "relres": rr.value, "converged": bool(conv),
-                 "ms": round(ms, 3), "ms_per_iter": round(ms / max(it.value, 1), 4)}
+        if m.batch_nodeptr is not None:
+            nb = len(m.batch_nodeptr) - 1
+            its = np.zeros(nb, dtype=np.int32)
+            rrs = np.zeros(nb)
+            nconv = L.fgpu_solve_batch_dev(model.h, b.data_ptr(), x.data_ptr(), 100000, 1e-8,
+                                           its.ctypes.data, rrs.ctypes.data)
+            ms = L.fgpu_last_time_ms(model.h, 3)
+            solve = {"rtol": 1e-8, "meshes": nb, "converged": int(nconv),
+                     "iters_max": int(its.max()), "iters_mean": float(its.mean()),
+                     "relres_max": float(rrs.max()), "ms": round(ms, 3)}
+        else:
+            conv = L.fgpu_solve_dev(model.h, b.data_ptr(), x.data_ptr(), None, 100000, 1e-8,
+                                    fgpu.C.byref(it), fgpu.C.byref(rr))
+            ms = L.fgpu_last_time_ms(model.h, 3)
+            solve = {"rtol": 1e-8, "iters": it.value, "relres": rr.value,
+                     "converged": bool(conv), "ms": round(ms, 3),
+                     "ms_per_iter": round(ms / max(it.value, 1), 4)}
     cpu = None
     if not args.no_cpu:
         res, ncores = cpu_baseline(m, u)

@@ -17,7 +17,8 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 OUT = os.path.join(HERE, "libfgpu.so")
 BUILD = os.path.join(HERE, "_build")
-SOURCES = ["fgpu_api.cpp", "fgpu_drivers.cpp", "k_hex8.hip", "k_bsp.hip", "k_solve.hip"]
+SOURCES = ["fgpu_api.cpp", "fgpu_drivers.cpp", "k_hex8.hip", "k_bsp.hip", "k_solve.hip",
+           "k_pcg.hip"]
 ARCH = os.environ.get("FGPU_ARCH", "gfx950")
 
 

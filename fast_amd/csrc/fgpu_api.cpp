@@ -168,8 +168,7 @@ void build_maps(Ctx &c, const int *conn) {
   c.ncontrib = cptr0[nnode];
   if (c.ncontrib >= (1LL << 31) || (long long)nelem * nsym * 2 >= (1LL << 31))
     throw Error("mesh too large for 32-bit assembly codes");
-  std::vector<int> col(c.nnz);
-  std::vector<long long> diag(c.neq);
+  (void)col_offset;
   std::vector<int> blkA(c.nblk), blkB(c.nblk), offAB(c.nblk), offBA(c.nblk);
   std::vector<long long> cptr(c.nblk + 1);
   std::vector<int> contrib(c.ncontrib);
@@ -178,23 +177,6 @@ void build_maps(Ctx &c, const int *conn) {
   parallel_for(nnode, [&](long Al) {
     int A = (int)Al;
     thread_local std::vector<std::pair<int, int>> items;
-    // column list of the rows of A (identical for its free directions)
-    long long nb0 = nbcnt[A], nb1 = nbcnt[A + 1];
-    int self = 0;
-    for (int d = 0; d < 3; d++) {
-      int q = c.eqmap[3 * A + d];
-      if (q < 0) continue;
-      long long p = rowptr[q];
-      for (long long k = nb0; k < nb1; k++) {
-        int B = nbidx[k];
-        for (int dd = 0; dd < 3; dd++) {
-          int qq = c.eqmap[3 * B + dd];
-          if (qq >= 0) col[p++] = qq;
-        }
-      }
-      diag[q] = rowptr[q] + col_offset(A, A) + self;
-      self++;
-    }
     // contributions of upper blocks, sorted by (B, element, local a, local b)
     items.clear();
     for (int t = nptr[A]; t < nptr[A + 1]; t++) {
@@ -237,8 +219,7 @@ void build_maps(Ctx &c, const int *conn) {
     fmask[n] = mk;
   }
   hipStream_t s = c.stream;
-  c.d_rowptr = dupload(rowptr.data(), rowptr.size(), s);
-  c.d_col = dupload(col.data(), col.size(), s);
+  c.d_rowptr = dupload(rowptr.data(), rowptr.size(), s);  // columns: built on first export
   c.d_bptr = dupload(nbcnt.data(), nbcnt.size(), s);
   c.d_bcol = dupload(nbidx.data(), nbidx.size(), s);
   c.d_bcoloff = dupload(nbcol.data(), nbcol.size(), s);
@@ -253,6 +234,49 @@ void build_maps(Ctx &c, const int *conn) {
   c.d_contrib = dupload(contrib.data(), contrib.size(), s);
   c.d_nptr = dupload(nptr.data(), nptr.size(), s);
   c.d_nadj = dupload(nadj.data(), nadj.size(), s);
+  FGPU_CHECK(hipStreamSynchronize(s));
+}
+
+// Batch set-up: meshes are contiguous node ranges; the solver works on
+// chunks of at most CHN nodes inside one mesh (one workgroup per chunk).
+void setup_batch(Ctx &c, const fgpu_mesh &m) {
+  const int nb = m.nbatch;
+  if (!m.batch_nodeptr) throw Error("batch_nodeptr missing");
+  c.h_bnodeptr.assign(m.batch_nodeptr, m.batch_nodeptr + nb + 1);
+  if (c.h_bnodeptr[0] != 0 || c.h_bnodeptr[nb] != c.nnode) throw Error("batch_nodeptr range");
+  for (int k = 0; k < nb; k++)
+    if (c.h_bnodeptr[k + 1] <= c.h_bnodeptr[k]) throw Error("empty mesh in batch");
+  // no element may couple two meshes
+  std::vector<int> mesh_of(c.nnode);
+  for (int k = 0; k < nb; k++)
+    for (long long n = c.h_bnodeptr[k]; n < c.h_bnodeptr[k + 1]; n++) mesh_of[n] = k;
+  for (long e = 0; e < c.nelem; e++) {
+    int k0 = mesh_of[m.conn[(size_t)e * c.nn]];
+    for (int a = 1; a < c.nn; a++)
+      if (mesh_of[m.conn[(size_t)e * c.nn + a]] != k0) throw Error("element couples two meshes");
+  }
+  const long long CHN = 1024;
+  std::vector<long long> n0, n1;
+  std::vector<int> chm, meshch(nb + 1, 0);
+  for (int k = 0; k < nb; k++) {
+    meshch[k] = (int)n0.size();
+    for (long long a = c.h_bnodeptr[k]; a < c.h_bnodeptr[k + 1]; a += CHN) {
+      n0.push_back(a);
+      n1.push_back(std::min(a + CHN, c.h_bnodeptr[k + 1]));
+      chm.push_back(k);
+    }
+  }
+  meshch[nb] = (int)n0.size();
+  c.nbatch = nb;
+  c.nchunk = (int)n0.size();
+  hipStream_t s = c.stream;
+  c.d_chn0 = dupload(n0.data(), n0.size(), s);
+  c.d_chn1 = dupload(n1.data(), n1.size(), s);
+  c.d_chmesh = dupload(chm.data(), chm.size(), s);
+  c.d_meshch = dupload(meshch.data(), meshch.size(), s);
+  c.d_mscal = dalloc<double>(8 * (size_t)nb);
+  c.d_mdone = dalloc<int>(4 * (size_t)nb);
+  c.d_ndone = dalloc<int>(1);
   FGPU_CHECK(hipStreamSynchronize(s));
 }
 
@@ -272,6 +296,12 @@ void destroy(Ctx *c) {
   if (c->h_scal) (void)hipHostFree(c->h_scal);
   for (auto &e : c->ev)
     if (e) (void)hipEventDestroy(e);
+  void *bbufs[] = {c->d_chn0, c->d_chn1, c->d_chmesh, c->d_meshch, c->d_mscal, c->d_mdone,
+                   c->d_ndone};
+  for (void *p : bbufs)
+    if (p) (void)hipFree(p);
+  if (c->g_single) (void)hipGraphExecDestroy(c->g_single);
+  if (c->g_batch) (void)hipGraphExecDestroy(c->g_batch);
   if (c->stream) (void)hipStreamDestroy(c->stream);
   delete c;
 }
@@ -439,7 +469,10 @@ int fgpu_create(fgpu_ctx **out, const fgpu_mesh *m) {
     c->d_z = dalloc<double>(n);
     c->d_dinv = dalloc<double>(n);
     c->nblocks_red = 1024;
-    c->d_part = dalloc<double>(4 * c->nblocks_red);
+    const char *ng = getenv("FGPU_NO_GRAPH");
+    c->use_graphs = !(ng && ng[0] == '1');
+    if (m->nbatch > 0) setup_batch(*c, *m);
+    c->d_part = dalloc<double>(std::max<size_t>(4 * c->nblocks_red, 3 * (size_t)c->nchunk + 8));
     c->d_scal = dalloc<double>(16);
     c->d_done = dalloc<int>(4);
     FGPU_CHECK(hipHostMalloc(&c->h_done, 4 * sizeof(int)));
@@ -462,6 +495,38 @@ const char *fgpu_last_error(fgpu_ctx *ctx) {
 }
 
 int fgpu_neq(fgpu_ctx *ctx) { return ctx ? reinterpret_cast<Ctx *>(ctx)->neq : 0; }
+
+int fgpu_nbatch(fgpu_ctx *ctx) { return ctx ? reinterpret_cast<Ctx *>(ctx)->nbatch : 0; }
+
+int fgpu_solve_batch_dev(fgpu_ctx *ctx, const double *b, double *x, int maxit, double rtol,
+                         int *iters, double *relres) {
+  return guarded(ctx, [&](Ctx &c) {
+    FGPU_CHECK(hipEventRecord(c.ev[5], c.stream));
+    int nconv = run_pcg_batch(c, b, x, maxit, rtol, iters, relres);
+    FGPU_CHECK(hipEventRecord(c.ev[6], c.stream));
+    FGPU_CHECK(hipEventSynchronize(c.ev[6]));
+    c.last_ms[3] = elapsed(c, 5, 6);
+    if (nconv < c.nbatch)
+      c.err = std::to_string(c.nbatch - nconv) + " meshes did not reach the tolerance";
+    return nconv;
+  });
+}
+
+int fgpu_solve_batch(fgpu_ctx *ctx, const double *b, double *x, int maxit, double rtol,
+                     int *iters, double *relres) {
+  return guarded(ctx, [&](Ctx &c) {
+    if (!b || !x) throw Error("null argument");
+    size_t nb = sizeof(double) * c.neq;
+    double *db = dalloc<double>(c.neq), *dx = dalloc<double>(c.neq);
+    FGPU_CHECK(hipMemcpyAsync(db, b, nb, hipMemcpyHostToDevice, c.stream));
+    int nconv = fgpu_solve_batch_dev(ctx, db, dx, maxit, rtol, iters, relres);
+    FGPU_CHECK(hipMemcpyAsync(x, dx, nb, hipMemcpyDeviceToHost, c.stream));
+    FGPU_CHECK(hipStreamSynchronize(c.stream));
+    (void)hipFree(db);
+    (void)hipFree(dx);
+    return nconv;
+  });
+}
 
 long long fgpu_nnz(fgpu_ctx *ctx) { return ctx ? reinterpret_cast<Ctx *>(ctx)->nnz : 0; }
 
@@ -562,9 +627,11 @@ int fgpu_get_csr(fgpu_ctx *ctx, long long *rowptr, int *col, double *val) {
     if (rowptr)
       FGPU_CHECK(hipMemcpyAsync(rowptr, c.d_rowptr, sizeof(long long) * (c.neq + 1),
                                 hipMemcpyDeviceToHost, c.stream));
-    if (col)
+    if (col) {
+      ensure_eq_csr(c);
       FGPU_CHECK(hipMemcpyAsync(col, c.d_col, sizeof(int) * c.nnz, hipMemcpyDeviceToHost,
                                 c.stream));
+    }
     if (val) {
       double *d = dalloc<double>(c.nnz);
       launch_csr_convert(c, d, 1);

@@ -109,6 +109,19 @@ struct Ctx {
   int *h_done = nullptr;      // pinned mirror
   double *h_scal = nullptr;   // pinned mirror
   int nblocks_red = 0;        // reduction blocks
+  bool use_graphs = true;     // capture PCG iteration chunks into hipGraphs
+  hipGraphExec_t g_single = nullptr, g_batch = nullptr;
+
+  // ---- batch of independent meshes (contiguous node ranges)
+  int nbatch = 0;
+  std::vector<long long> h_bnodeptr;  // [nbatch+1]
+  int nchunk = 0;                     // solver chunks (never cross a mesh)
+  long long *d_chn0 = nullptr, *d_chn1 = nullptr;  // [nchunk] node ranges
+  int *d_chmesh = nullptr;            // [nchunk]
+  int *d_meshch = nullptr;            // [nbatch+1] chunk range of each mesh
+  double *d_mscal = nullptr;          // [nbatch*8]
+  int *d_mdone = nullptr;             // [nbatch*4]
+  int *d_ndone = nullptr;             // finished meshes
 
   // ---- timing
   hipEvent_t ev[8] = {};
@@ -126,6 +139,9 @@ void launch_spmv_nodal(Ctx &c, const double *x, double *y);  // node vectors
 void launch_csr_convert(Ctx &c, double *d_csrval, int to_csr);
 int run_pcg(Ctx &c, const double *b, double *x, const double *x0, int maxit,
             double rtol, int *iters, double *relres);
+int run_pcg_batch(Ctx &c, const double *b, double *x, int maxit, double rtol, int *iters,
+                  double *relres);
+void ensure_eq_csr(Ctx &c);
 void launch_nodal_to_eq(Ctx &c, const double *nodal, double *eq);
 void launch_eq_to_nodal(Ctx &c, const double *eq, double *nodal);
 void launch_add_diag(Ctx &c, int n, const int *d_eq, const double *d_v);

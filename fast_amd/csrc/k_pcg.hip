@@ -1,0 +1,491 @@
+// k_pcg.hip - FP64 Jacobi-PCG of libfgpu on the node-block matrix: one
+// system, or a batch of independent meshes with per-mesh convergence.
+//
+// Restates PCGSolver (mlib/sysmat.cpp:1061-1125): Jacobi preconditioner,
+// x0 = 0 (or a caller warm start), stop when |r| / |b| < tol with |b| > 1e11
+// replaced by 1, immediate return when |b| < tol.  Dot products are block
+// partial sums reduced in one fixed order by every consumer block, so results
+// do not depend on scheduling.  All scalars (alpha, beta, z.r, the iteration
+// counter, the tolerance) live on the device; every kernel is a no-op once
+// its system has converged, so the iterate returned is the first one meeting
+// the tolerance.  A chunk of iterations is captured once into a hipGraph and
+// replayed; the host polls one flag per chunk.
+//
+// Batch mode: the meshes of a batch own contiguous node ranges.  The node
+// space is cut into chunks of at most CHN nodes that never cross a mesh; each
+// chunk is one workgroup, and the per-mesh sums are the chunk partials of
+// that mesh added in chunk order by every chunk of the mesh.
+#include <cmath>
+
+#include "fgpu_dev.h"
+
+namespace fgpu {
+namespace {
+
+constexpr int CHUNK_IT = 16;  // iterations per captured graph
+
+// ---------------------------------------------------------------- single
+// scal: [0] lenf  [1],[2] z.r by iteration parity  [3] |r|/lenf  [4] |b|
+//       [6] tol ; done[0] = iteration at convergence, done[1] = breakdown,
+//       done[2] = current iteration
+template <int G>
+__global__ __launch_bounds__(256) void pcg_spmv_kernel(
+    int nnode, const long long *__restrict__ bptr, const int *__restrict__ bcol,
+    const double *__restrict__ bval, const double *__restrict__ p, double *__restrict__ w,
+    double *__restrict__ part, const int *__restrict__ done) {
+  __shared__ double sh[16];
+  if (done[0]) return;
+  const int rows_per_block = blockDim.x / G;
+  double dot = bspmv_rows<G>((long long)blockIdx.x * rows_per_block, nnode,
+                             (long long)gridDim.x * rows_per_block, bptr, bcol, bval, p, w);
+  double t = block_sum(dot, sh);
+  if (threadIdx.x == 0) part[blockIdx.x] = t;
+}
+
+__global__ __launch_bounds__(256) void pcg_init_kernel(
+    int n, const double *__restrict__ b, double *__restrict__ r, const double *__restrict__ dinv,
+    double *__restrict__ z, double *__restrict__ p, double *__restrict__ part) {
+  __shared__ double sh[16];
+  double zr = 0.0, bb = 0.0, rr = 0.0;
+  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (long long)gridDim.x * blockDim.x) {
+    double ri = r[i], zi = ri * dinv[i];  // r holds b - K x0 (or b)
+    z[i] = zi;
+    p[i] = zi;
+    zr += zi * ri;
+    bb += b[i] * b[i];
+    rr += ri * ri;
+  }
+  double t0 = block_sum(zr, sh);
+  double t1 = block_sum(bb, sh);
+  double t2 = block_sum(rr, sh);
+  if (threadIdx.x == 0) {
+    part[blockIdx.x] = t0;
+    part[RED + blockIdx.x] = t1;
+    part[2 * RED + blockIdx.x] = t2;
+  }
+}
+
+__global__ void pcg_start_kernel(const double *__restrict__ part, double *__restrict__ scal,
+                                 int *__restrict__ done, double tol) {
+  __shared__ double sh[16];
+  double zr = reduce_partials(part, RED, sh);
+  double bb = reduce_partials(part + RED, RED, sh);
+  double rr = reduce_partials(part + 2 * RED, RED, sh);
+  if (threadIdx.x == 0) {
+    double lenf = sqrt(bb);
+    scal[4] = lenf;
+    if (lenf > 10e+10) lenf = 1.0;
+    scal[0] = lenf;
+    scal[1] = zr;
+    scal[3] = sqrt(rr) / lenf;
+    scal[6] = tol;
+    done[0] = 0;
+    done[1] = 0;
+    done[2] = 1;
+  }
+}
+
+__global__ __launch_bounds__(256) void pcg_update_kernel(
+    int n, double *__restrict__ x, double *__restrict__ r, double *__restrict__ z,
+    const double *__restrict__ p, const double *__restrict__ w, const double *__restrict__ dinv,
+    double *__restrict__ part, const double *__restrict__ scal, const int *__restrict__ done) {
+  __shared__ double sh[16];
+  if (done[0]) return;
+  const int it = done[2];
+  const double pw = reduce_partials(part, RED, sh);
+  const double alpha = scal[1 + ((it - 1) & 1)] / pw;
+  double zr = 0.0, rr = 0.0;
+  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (long long)gridDim.x * blockDim.x) {
+    x[i] = x[i] + alpha * p[i];
+    double ri = r[i] + (-alpha) * w[i];
+    r[i] = ri;
+    double zi = ri * dinv[i];
+    z[i] = zi;
+    zr += zi * ri;
+    rr += ri * ri;
+  }
+  double t0 = block_sum(zr, sh);
+  double t1 = block_sum(rr, sh);
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    part[RED + blockIdx.x] = t0;
+    part[2 * RED + blockIdx.x] = t1;
+  }
+}
+
+__global__ __launch_bounds__(256) void pcg_dir_kernel(
+    int n, double *__restrict__ p, const double *__restrict__ z, const double *__restrict__ part,
+    double *__restrict__ scal, int *__restrict__ done) {
+  __shared__ double sh[16];
+  if (done[0]) return;
+  const int it = done[2];
+  const double zr = reduce_partials(part + RED, RED, sh);
+  const double rr = reduce_partials(part + 2 * RED, RED, sh);
+  const double err = sqrt(rr) / scal[0];
+  if (err < scal[6] || !(err == err)) {
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+      scal[3] = err;
+      done[1] = !(err == err);
+    }
+    return;  // done[0] is set by the step kernel, after every block has read it
+  }
+  const double beta = zr / scal[1 + ((it - 1) & 1)];
+  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (long long)gridDim.x * blockDim.x)
+    p[i] = z[i] + beta * p[i];
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    scal[1 + (it & 1)] = zr;
+    scal[3] = err;
+  }
+}
+
+// closes iteration `it`: converged -> done[0] = it, else it + 1
+__global__ void pcg_step_kernel(const double *__restrict__ scal, int *__restrict__ done) {
+  if (threadIdx.x != 0 || done[0]) return;
+  const double err = scal[3];
+  if (err < scal[6] || done[1])
+    done[0] = done[2];
+  else
+    done[2] = done[2] + 1;
+}
+
+__global__ void diag_inv_kernel(int nnode, const long long *__restrict__ bdiag,
+                                const double *__restrict__ bval, double *__restrict__ dinv) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < 3 * nnode) dinv[i] = 1.0 / bval[9 * bdiag[i / 3] + 4 * (i % 3)];
+}
+
+__global__ void residual_kernel(int n, const double *__restrict__ b, const double *__restrict__ Ax,
+                                double *__restrict__ r) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) r[i] = b[i] - Ax[i];
+}
+
+template <int G>
+void pcg_spmv_launch(Ctx &c) {
+  hipLaunchKernelGGL(pcg_spmv_kernel<G>, dim3(RED), dim3(TPB), 0, c.stream, c.nnode, c.d_bptr,
+                     c.d_bcol, c.d_bval, c.d_p, c.d_w, c.d_part, c.d_done);
+}
+
+void enqueue_iteration(Ctx &c) {
+  const int n = 3 * c.nnode;
+  switch (spmv_group(c)) {
+    case 64: pcg_spmv_launch<64>(c); break;
+    case 32: pcg_spmv_launch<32>(c); break;
+    case 16: pcg_spmv_launch<16>(c); break;
+    case 8: pcg_spmv_launch<8>(c); break;
+    default: pcg_spmv_launch<4>(c); break;
+  }
+  hipLaunchKernelGGL(pcg_update_kernel, dim3(RED), dim3(TPB), 0, c.stream, n, c.d_x, c.d_r,
+                     c.d_z, c.d_p, c.d_w, c.d_dinv, c.d_part, c.d_scal, c.d_done);
+  hipLaunchKernelGGL(pcg_dir_kernel, dim3(RED), dim3(TPB), 0, c.stream, n, c.d_p, c.d_z,
+                     c.d_part, c.d_scal, c.d_done);
+  hipLaunchKernelGGL(pcg_step_kernel, dim3(1), dim3(64), 0, c.stream, c.d_scal, c.d_done);
+}
+
+// ---------------------------------------------------------------- batch
+// per-mesh scalars mscal[8 m + k]: [0] lenf [1],[2] z.r by parity [3] err
+// [4] |b| [6] tol; mdone[4 m + k]: [0] converged at, [1] breakdown,
+// [2] current iteration (per mesh), [3] started
+__device__ inline double mesh_sum(const double *__restrict__ part, int c0, int c1) {
+  double s = 0.0;
+  for (int c = c0; c < c1; c++) s += part[c];
+  return s;
+}
+
+template <int G>
+__global__ __launch_bounds__(256) void bpcg_spmv_kernel(
+    const long long *__restrict__ chn0, const long long *__restrict__ chn1,
+    const int *__restrict__ chmesh, const long long *__restrict__ bptr,
+    const int *__restrict__ bcol, const double *__restrict__ bval,
+    const double *__restrict__ p, double *__restrict__ w, double *__restrict__ part,
+    const int *__restrict__ mdone) {
+  __shared__ double sh[16];
+  const int ch = blockIdx.x;
+  if (mdone[4 * chmesh[ch]]) return;
+  const int rows_per_block = blockDim.x / G;
+  double dot = bspmv_rows<G>(chn0[ch], chn1[ch], rows_per_block, bptr, bcol, bval, p, w);
+  double t = block_sum(dot, sh);
+  if (threadIdx.x == 0) part[ch] = t;
+}
+
+__global__ __launch_bounds__(256) void bpcg_init_kernel(
+    const long long *__restrict__ chn0, const long long *__restrict__ chn1,
+    const double *__restrict__ b, double *__restrict__ r, const double *__restrict__ dinv,
+    double *__restrict__ z, double *__restrict__ p, double *__restrict__ part, int nch) {
+  __shared__ double sh[16];
+  const int ch = blockIdx.x;
+  double zr = 0.0, bb = 0.0, rr = 0.0;
+  for (long long i = 3 * chn0[ch] + threadIdx.x; i < 3 * chn1[ch]; i += blockDim.x) {
+    double ri = r[i], zi = ri * dinv[i];
+    z[i] = zi;
+    p[i] = zi;
+    zr += zi * ri;
+    bb += b[i] * b[i];
+    rr += ri * ri;
+  }
+  double t0 = block_sum(zr, sh);
+  double t1 = block_sum(bb, sh);
+  double t2 = block_sum(rr, sh);
+  if (threadIdx.x == 0) {
+    part[ch] = t0;
+    part[nch + ch] = t1;
+    part[2 * nch + ch] = t2;
+  }
+}
+
+__global__ void bpcg_start_kernel(int nmesh, const int *__restrict__ meshch,
+                                  const double *__restrict__ part, int nch,
+                                  double *__restrict__ mscal, int *__restrict__ mdone,
+                                  int *__restrict__ ndone, double tol) {
+  const int m = blockIdx.x * blockDim.x + threadIdx.x;
+  if (m == 0) *ndone = 0;
+  if (m >= nmesh) return;
+  const int c0 = meshch[m], c1 = meshch[m + 1];
+  const double zr = mesh_sum(part, c0, c1), bb = mesh_sum(part + nch, c0, c1),
+               rr = mesh_sum(part + 2 * nch, c0, c1);
+  double lenf = sqrt(bb);
+  double *s = mscal + 8 * m;
+  s[4] = lenf;
+  if (lenf > 10e+10) lenf = 1.0;
+  s[0] = lenf;
+  s[1] = zr;
+  s[3] = sqrt(rr) / lenf;
+  s[6] = tol;
+  int *d = mdone + 4 * m;
+  d[1] = 0;
+  d[2] = 1;
+  // |b| < tol or a warm start already converged: finished before iterating
+  d[0] = (s[4] < tol || s[3] < tol) ? -1 : 0;
+}
+
+__global__ void bpcg_count_kernel(int nmesh, const int *__restrict__ mdone, int *__restrict__ ndone) {
+  if (blockIdx.x != 0 || threadIdx.x != 0) return;
+  int k = 0;
+  for (int m = 0; m < nmesh; m++) k += mdone[4 * m] != 0;
+  *ndone = k;
+}
+
+__global__ __launch_bounds__(256) void bpcg_update_kernel(
+    const long long *__restrict__ chn0, const long long *__restrict__ chn1,
+    const int *__restrict__ chmesh, const int *__restrict__ meshch, double *__restrict__ x,
+    double *__restrict__ r, double *__restrict__ z, const double *__restrict__ p,
+    const double *__restrict__ w, const double *__restrict__ dinv, double *__restrict__ part,
+    int nch, const double *__restrict__ mscal, const int *__restrict__ mdone) {
+  __shared__ double sh[16];
+  const int ch = blockIdx.x, m = chmesh[ch];
+  const int *d = mdone + 4 * m;
+  if (d[0]) return;
+  const int it = d[2];
+  const double pw = mesh_sum(part, meshch[m], meshch[m + 1]);
+  const double alpha = mscal[8 * m + 1 + ((it - 1) & 1)] / pw;
+  double zr = 0.0, rr = 0.0;
+  for (long long i = 3 * chn0[ch] + threadIdx.x; i < 3 * chn1[ch]; i += blockDim.x) {
+    x[i] = x[i] + alpha * p[i];
+    double ri = r[i] + (-alpha) * w[i];
+    r[i] = ri;
+    double zi = ri * dinv[i];
+    z[i] = zi;
+    zr += zi * ri;
+    rr += ri * ri;
+  }
+  double t0 = block_sum(zr, sh);
+  double t1 = block_sum(rr, sh);
+  if (threadIdx.x == 0) {
+    part[nch + ch] = t0;
+    part[2 * nch + ch] = t1;
+  }
+}
+
+__global__ __launch_bounds__(256) void bpcg_dir_kernel(
+    const long long *__restrict__ chn0, const long long *__restrict__ chn1,
+    const int *__restrict__ chmesh, const int *__restrict__ meshch, double *__restrict__ p,
+    const double *__restrict__ z, const double *__restrict__ part, int nch,
+    const double *__restrict__ mscal, const int *__restrict__ mdone) {
+  const int ch = blockIdx.x, m = chmesh[ch];
+  const int *d = mdone + 4 * m;
+  if (d[0]) return;
+  const int it = d[2];
+  const double *s = mscal + 8 * m;
+  const double zr = mesh_sum(part + nch, meshch[m], meshch[m + 1]);
+  const double rr = mesh_sum(part + 2 * nch, meshch[m], meshch[m + 1]);
+  const double err = sqrt(rr) / s[0];
+  if (err < s[6] || !(err == err)) return;
+  const double beta = zr / s[1 + ((it - 1) & 1)];
+  for (long long i = 3 * chn0[ch] + threadIdx.x; i < 3 * chn1[ch]; i += blockDim.x)
+    p[i] = z[i] + beta * p[i];
+}
+
+// per mesh: close the iteration (store z.r, err; converged -> done)
+__global__ void bpcg_step_kernel(int nmesh, const int *__restrict__ meshch,
+                                 const double *__restrict__ part, int nch,
+                                 double *__restrict__ mscal, int *__restrict__ mdone,
+                                 int *__restrict__ ndone) {
+  const int m = blockIdx.x * blockDim.x + threadIdx.x;
+  if (m >= nmesh) return;
+  int *d = mdone + 4 * m;
+  if (d[0]) return;
+  double *s = mscal + 8 * m;
+  const int it = d[2];
+  const double zr = mesh_sum(part + nch, meshch[m], meshch[m + 1]);
+  const double rr = mesh_sum(part + 2 * nch, meshch[m], meshch[m + 1]);
+  const double err = sqrt(rr) / s[0];
+  s[3] = err;
+  if (err < s[6] || !(err == err)) {
+    d[1] = !(err == err);
+    d[0] = it;
+    atomicAdd(ndone, 1);  // integer count of finished meshes
+  } else {
+    s[1 + (it & 1)] = zr;
+    d[2] = it + 1;
+  }
+}
+
+template <int G>
+void bpcg_spmv_launch(Ctx &c) {
+  hipLaunchKernelGGL(bpcg_spmv_kernel<G>, dim3(c.nchunk), dim3(TPB), 0, c.stream, c.d_chn0,
+                     c.d_chn1, c.d_chmesh, c.d_bptr, c.d_bcol, c.d_bval, c.d_p, c.d_w, c.d_part,
+                     c.d_mdone);
+}
+
+void enqueue_batch_iteration(Ctx &c) {
+  switch (spmv_group(c)) {
+    case 64: bpcg_spmv_launch<64>(c); break;
+    case 32: bpcg_spmv_launch<32>(c); break;
+    case 16: bpcg_spmv_launch<16>(c); break;
+    case 8: bpcg_spmv_launch<8>(c); break;
+    default: bpcg_spmv_launch<4>(c); break;
+  }
+  hipLaunchKernelGGL(bpcg_update_kernel, dim3(c.nchunk), dim3(TPB), 0, c.stream, c.d_chn0,
+                     c.d_chn1, c.d_chmesh, c.d_meshch, c.d_x, c.d_r, c.d_z, c.d_p, c.d_w,
+                     c.d_dinv, c.d_part, c.nchunk, c.d_mscal, c.d_mdone);
+  hipLaunchKernelGGL(bpcg_dir_kernel, dim3(c.nchunk), dim3(TPB), 0, c.stream, c.d_chn0,
+                     c.d_chn1, c.d_chmesh, c.d_meshch, c.d_p, c.d_z, c.d_part, c.nchunk,
+                     c.d_mscal, c.d_mdone);
+  hipLaunchKernelGGL(bpcg_step_kernel, dim3((c.nbatch + 255) / 256), dim3(256), 0, c.stream,
+                     c.nbatch, c.d_meshch, c.d_part, c.nchunk, c.d_mscal, c.d_mdone, c.d_ndone);
+}
+
+// Capture CHUNK_IT iterations once per context into a graph; replay it.
+void run_chunk(Ctx &c, bool batch) {
+  hipGraphExec_t &ge = batch ? c.g_batch : c.g_single;
+  if (!c.use_graphs) {
+    for (int k = 0; k < CHUNK_IT; k++) batch ? enqueue_batch_iteration(c) : enqueue_iteration(c);
+    FGPU_CHECK(hipGetLastError());
+    return;
+  }
+  if (!ge) {
+    hipGraph_t g;
+    FGPU_CHECK(hipStreamBeginCapture(c.stream, hipStreamCaptureModeThreadLocal));
+    for (int k = 0; k < CHUNK_IT; k++) batch ? enqueue_batch_iteration(c) : enqueue_iteration(c);
+    FGPU_CHECK(hipStreamEndCapture(c.stream, &g));
+    FGPU_CHECK(hipGraphInstantiate(&ge, g, nullptr, nullptr, 0));
+    FGPU_CHECK(hipGraphDestroy(g));
+  }
+  FGPU_CHECK(hipGraphLaunch(ge, c.stream));
+}
+
+void prepare(Ctx &c, const double *b, const double *x0) {
+  const int n = 3 * c.nnode;
+  const int eb = (n + 255) / 256;
+  hipStream_t s = c.stream;
+  hipLaunchKernelGGL(diag_inv_kernel, dim3(eb), dim3(256), 0, s, c.nnode, c.d_bdiag, c.d_bval,
+                     c.d_dinv);
+  launch_eq_to_nodal(c, b, c.d_bn);
+  if (x0) {
+    launch_eq_to_nodal(c, x0, c.d_x);
+    launch_spmv_nodal(c, c.d_x, c.d_w);
+    hipLaunchKernelGGL(residual_kernel, dim3(eb), dim3(256), 0, s, n, c.d_bn, c.d_w, c.d_r);
+  } else {
+    FGPU_CHECK(hipMemsetAsync(c.d_x, 0, sizeof(double) * n, s));
+    FGPU_CHECK(hipMemcpyAsync(c.d_r, c.d_bn, sizeof(double) * n, hipMemcpyDeviceToDevice, s));
+  }
+  FGPU_CHECK(hipGetLastError());
+}
+
+}  // namespace
+
+// b, x, x0: equation vectors on the device.  The iteration runs on node
+// vectors (3 nnode entries, fixed directions identically zero).
+int run_pcg(Ctx &c, const double *b, double *x, const double *x0, int maxit, double rtol,
+            int *iters, double *relres) {
+  hipStream_t s = c.stream;
+  if (iters) *iters = 0;
+  if (relres) *relres = 0.0;
+  if (c.neq == 0) return 1;
+  prepare(c, b, x0);
+  hipLaunchKernelGGL(pcg_init_kernel, dim3(RED), dim3(TPB), 0, s, 3 * c.nnode, c.d_bn, c.d_r,
+                     c.d_dinv, c.d_z, c.d_p, c.d_part);
+  hipLaunchKernelGGL(pcg_start_kernel, dim3(1), dim3(TPB), 0, s, c.d_part, c.d_scal, c.d_done,
+                     rtol);
+  FGPU_CHECK(hipGetLastError());
+  FGPU_CHECK(hipMemcpyAsync(c.h_scal, c.d_scal, 5 * sizeof(double), hipMemcpyDeviceToHost, s));
+  FGPU_CHECK(hipStreamSynchronize(s));
+  // reference: |b| < tol returns at once (x stays at its start value)
+  const bool early = c.h_scal[4] < rtol || c.h_scal[3] < rtol;
+  int conv_it = 0, broke = 0, launched = 0;
+  while (!early && launched < maxit) {
+    run_chunk(c, false);
+    launched += CHUNK_IT;
+    FGPU_CHECK(hipMemcpyAsync(c.h_done, c.d_done, 3 * sizeof(int), hipMemcpyDeviceToHost, s));
+    FGPU_CHECK(hipMemcpyAsync(c.h_scal, c.d_scal, 4 * sizeof(double), hipMemcpyDeviceToHost, s));
+    FGPU_CHECK(hipStreamSynchronize(s));
+    if (c.h_done[0]) {
+      conv_it = c.h_done[0];
+      broke = c.h_done[1];
+      break;
+    }
+    if (c.h_done[2] > maxit) break;
+  }
+  // a chunk may run past maxit: the iteration count is what the device did
+  launch_nodal_to_eq(c, c.d_x, x);
+  FGPU_CHECK(hipStreamSynchronize(s));
+  if (early) {
+    if (relres) *relres = c.h_scal[4] < rtol ? 0.0 : c.h_scal[3];
+    return 1;
+  }
+  if (iters) *iters = conv_it ? conv_it : c.h_done[2] - 1;
+  if (relres) *relres = c.h_scal[3];
+  return conv_it > 0 && conv_it <= maxit && !broke;
+}
+
+int run_pcg_batch(Ctx &c, const double *b, double *x, int maxit, double rtol, int *iters,
+                  double *relres) {
+  hipStream_t s = c.stream;
+  if (c.nbatch <= 0) throw Error("not a batch context");
+  prepare(c, b, nullptr);
+  hipLaunchKernelGGL(bpcg_init_kernel, dim3(c.nchunk), dim3(TPB), 0, s, c.d_chn0, c.d_chn1,
+                     c.d_bn, c.d_r, c.d_dinv, c.d_z, c.d_p, c.d_part, c.nchunk);
+  hipLaunchKernelGGL(bpcg_start_kernel, dim3((c.nbatch + 255) / 256), dim3(256), 0, s, c.nbatch,
+                     c.d_meshch, c.d_part, c.nchunk, c.d_mscal, c.d_mdone, c.d_ndone, rtol);
+  hipLaunchKernelGGL(bpcg_count_kernel, dim3(1), dim3(64), 0, s, c.nbatch, c.d_mdone, c.d_ndone);
+  FGPU_CHECK(hipGetLastError());
+  int launched = 0;
+  while (true) {
+    FGPU_CHECK(hipMemcpyAsync(c.h_done, c.d_ndone, sizeof(int), hipMemcpyDeviceToHost, s));
+    FGPU_CHECK(hipStreamSynchronize(s));
+    if (c.h_done[0] >= c.nbatch || launched >= maxit) break;
+    run_chunk(c, true);
+    launched += CHUNK_IT;
+  }
+  launch_nodal_to_eq(c, c.d_x, x);
+  std::vector<int> md(4 * (size_t)c.nbatch);
+  std::vector<double> ms(8 * (size_t)c.nbatch);
+  FGPU_CHECK(hipMemcpyAsync(md.data(), c.d_mdone, md.size() * sizeof(int), hipMemcpyDeviceToHost, s));
+  FGPU_CHECK(hipMemcpyAsync(ms.data(), c.d_mscal, ms.size() * sizeof(double),
+                            hipMemcpyDeviceToHost, s));
+  FGPU_CHECK(hipStreamSynchronize(s));
+  int nconv = 0;
+  for (int m = 0; m < c.nbatch; m++) {
+    const int d0 = md[4 * m], it = d0 > 0 ? d0 : (d0 < 0 ? 0 : md[4 * m + 2] - 1);
+    const bool ok = d0 < 0 || (d0 > 0 && d0 <= maxit && !md[4 * m + 1]);
+    nconv += ok;
+    if (iters) iters[m] = it;
+    if (relres) relres[m] = ms[8 * m + 4] < rtol ? 0.0 : ms[8 * m + 3];
+  }
+  return nconv;
+}
+
+}  // namespace fgpu

@@ -1,5 +1,5 @@
-// k_solve.hip - deterministic assembly, internal-force gather, block SpMV and
-// the FP64 Jacobi-PCG of libfgpu (gfx950).
+// k_solve.hip - deterministic assembly, internal-force gather, block SpMV,
+// vector helpers and format conversions of libfgpu (gfx950).
 //
 // Global matrix storage: node-block rows of 3x3 FP64 blocks ("block CSR"),
 // one row per node, blocks in increasing neighbour order.  A fixed direction
@@ -17,22 +17,13 @@
 // (B, A), each as 9 contiguous doubles.  Every value is written exactly once;
 // no atomics.
 //
-// PCG restates PCGSolver (mlib/sysmat.cpp:1061-1125): Jacobi preconditioner,
-// x0 = 0 (or a caller warm start), stop when |r| / |b| < tol with |b| > 1e11
-// replaced by 1.  Dot products are block partial sums reduced in a fixed
-// order by every consumer block, so results do not depend on scheduling.
-// Scalars stay on the device; the host polls a convergence flag once per
-// chunk of iterations, and every kernel becomes a no-op after convergence,
-// so the iterate returned is the first one meeting the tolerance.
+// The PCG itself lives in k_pcg.hip.
 #include <cmath>
 
-#include "fgpu_internal.h"
+#include "fgpu_dev.h"
 
 namespace fgpu {
 namespace {
-
-constexpr int RED = 1024;  // partial-sum slots (== blocks of reduction kernels)
-constexpr int TPB = 256;
 
 __global__ __launch_bounds__(256) void assemble_kernel(
     long long nblk, const int *__restrict__ blkA, const int *__restrict__ blkB,
@@ -166,189 +157,25 @@ __global__ void bsr_to_csr_kernel(int nnode, const long long *__restrict__ bptr,
   }
 }
 
-// ---------------------------------------------------------------- reductions
-__device__ inline double block_sum(double v, double *sh) {
-  // fixed-shape tree: deterministic for a given block size
-  for (int o = 32; o > 0; o >>= 1) v += __shfl_down(v, o);
-  const int lane = threadIdx.x % 64, w = threadIdx.x / 64;
-  __syncthreads();
-  if (lane == 0) sh[w] = v;
-  __syncthreads();
-  double r = 0.0;
-  if (threadIdx.x == 0)
-    for (int k = 0; k < (int)(blockDim.x / 64); k++) r += sh[k];
-  return r;  // valid in thread 0
-}
-
-// sum of RED partials (each block reduces them identically)
-__device__ inline double reduce_partials(const double *__restrict__ part, double *sh) {
-  double v = 0.0;
-  for (int k = threadIdx.x; k < RED; k += blockDim.x) v += part[k];
-  double r = block_sum(v, sh);
-  __syncthreads();
-  if (threadIdx.x == 0) sh[8] = r;
-  __syncthreads();
-  return sh[8];
-}
-
-// y = K x on node-block rows, G lanes per row (lane s takes blocks s, s+G..);
-// optional fused partial dot(x, y).
-template <int G>
-__global__ __launch_bounds__(256) void bspmv_kernel(
-    int nnode, const long long *__restrict__ bptr, const int *__restrict__ bcol,
-    const double *__restrict__ bval, const double *__restrict__ x, double *__restrict__ y,
-    double *__restrict__ part, const int *__restrict__ done) {
-  __shared__ double sh[16];
-  if (done && *done) return;
-  const int sub = threadIdx.x % G;
-  const int rows_per_block = blockDim.x / G;
-  double dot = 0.0;
-  for (long long r = (long long)blockIdx.x * rows_per_block + threadIdx.x / G; r < nnode;
-       r += (long long)gridDim.x * rows_per_block) {
-    double y0 = 0.0, y1 = 0.0, y2 = 0.0;
-    const long long k1 = bptr[r + 1];
-    for (long long k = bptr[r] + sub; k < k1; k += G) {
-      const double *v = bval + 9 * k;
+// equation CSR columns: rows of node A list the free directions of its
+// neighbours in node order (the same list for each free direction of A)
+__global__ void fill_eq_cols_kernel(int nnode, const long long *__restrict__ bptr,
+                                    const int *__restrict__ bcol, const int *__restrict__ eqmap,
+                                    const long long *__restrict__ rowptr, int *__restrict__ col) {
+  const int A = blockIdx.x * blockDim.x + threadIdx.x;
+  if (A >= nnode) return;
+  for (int i = 0; i < 3; i++) {
+    const int qi = eqmap[3 * A + i];
+    if (qi < 0) continue;
+    long long p = rowptr[qi];
+    for (long long k = bptr[A]; k < bptr[A + 1]; k++) {
       const int B = bcol[k];
-      const double x0 = x[3 * B], x1 = x[3 * B + 1], x2 = x[3 * B + 2];
-      y0 += v[0] * x0 + v[1] * x1 + v[2] * x2;
-      y1 += v[3] * x0 + v[4] * x1 + v[5] * x2;
-      y2 += v[6] * x0 + v[7] * x1 + v[8] * x2;
-    }
-    for (int o = G / 2; o > 0; o >>= 1) {
-      y0 += __shfl_down(y0, o, G);
-      y1 += __shfl_down(y1, o, G);
-      y2 += __shfl_down(y2, o, G);
-    }
-    if (sub == 0) {
-      y[3 * r] = y0;
-      y[3 * r + 1] = y1;
-      y[3 * r + 2] = y2;
-      dot += y0 * x[3 * r] + y1 * x[3 * r + 1] + y2 * x[3 * r + 2];
+      for (int j = 0; j < 3; j++) {
+        const int qj = eqmap[3 * B + j];
+        if (qj >= 0) col[p++] = qj;
+      }
     }
   }
-  if (part) {
-    double t = block_sum(dot, sh);
-    if (threadIdx.x == 0) part[blockIdx.x] = t;
-  }
-}
-
-// PCG state on the device (scal):
-//   [0] lenf   [1],[2] z.r (double-buffered by iteration parity)   [3] |r|/lenf
-//   [4] |b|    done[0] = iteration at convergence (0 = running), done[1] = breakdown
-__global__ __launch_bounds__(256) void pcg_init_kernel(
-    int n, const double *__restrict__ b, double *__restrict__ r, const double *__restrict__ dinv,
-    double *__restrict__ z, double *__restrict__ p, double *__restrict__ part) {
-  __shared__ double sh[16];
-  double zr = 0.0, bb = 0.0, rr = 0.0;
-  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n;
-       i += (long long)gridDim.x * blockDim.x) {
-    // r already holds b - A x0 (or b); z = r / d; p = z
-    double ri = r[i], zi = ri * dinv[i];
-    z[i] = zi;
-    p[i] = zi;
-    zr += zi * ri;
-    bb += b[i] * b[i];
-    rr += ri * ri;
-  }
-  double t0 = block_sum(zr, sh);
-  double t1 = block_sum(bb, sh);
-  double t2 = block_sum(rr, sh);
-  if (threadIdx.x == 0) {
-    part[blockIdx.x] = t0;
-    part[RED + blockIdx.x] = t1;
-    part[2 * RED + blockIdx.x] = t2;
-  }
-}
-
-__global__ void pcg_start_kernel(const double *__restrict__ part, double *__restrict__ scal,
-                                 int *__restrict__ done) {
-  __shared__ double sh[16];
-  double zr = reduce_partials(part, sh);
-  double bb = reduce_partials(part + RED, sh);
-  double rr = reduce_partials(part + 2 * RED, sh);
-  if (threadIdx.x == 0) {
-    double lenf = sqrt(bb);
-    scal[4] = lenf;
-    if (lenf > 10e+10) lenf = 1.0;
-    scal[0] = lenf;
-    scal[1] = zr;
-    scal[3] = sqrt(rr) / lenf;
-    done[0] = 0;
-    done[1] = 0;
-  }
-}
-
-// x += alpha p; r -= alpha w; z = r / d; partials of z.r and r.r
-__global__ __launch_bounds__(256) void pcg_update_kernel(
-    int n, int it, double *__restrict__ x, double *__restrict__ r, double *__restrict__ z,
-    const double *__restrict__ p, const double *__restrict__ w, const double *__restrict__ dinv,
-    double *__restrict__ part, const double *__restrict__ scal, int *__restrict__ done) {
-  __shared__ double sh[16];
-  if (*done) return;
-  const double pw = reduce_partials(part, sh);
-  const double dot0 = scal[1 + ((it - 1) & 1)];
-  const double alpha = dot0 / pw;
-  double zr = 0.0, rr = 0.0;
-  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n;
-       i += (long long)gridDim.x * blockDim.x) {
-    x[i] = x[i] + alpha * p[i];
-    double ri = r[i] + (-alpha) * w[i];
-    r[i] = ri;
-    double zi = ri * dinv[i];
-    z[i] = zi;
-    zr += zi * ri;
-    rr += ri * ri;
-  }
-  double t0 = block_sum(zr, sh);
-  double t1 = block_sum(rr, sh);
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    part[RED + blockIdx.x] = t0;
-    part[2 * RED + blockIdx.x] = t1;
-  }
-}
-
-// convergence test; beta; p = z + beta p
-__global__ __launch_bounds__(256) void pcg_dir_kernel(
-    int n, int it, double *__restrict__ p, const double *__restrict__ z,
-    const double *__restrict__ part, double *__restrict__ scal, int *__restrict__ done,
-    double tol) {
-  __shared__ double sh[16];
-  if (*done) return;
-  const double zr = reduce_partials(part + RED, sh);
-  const double rr = reduce_partials(part + 2 * RED, sh);
-  const double err = sqrt(rr) / scal[0];
-  __syncthreads();
-  if (err < tol || !(err == err)) {
-    if (blockIdx.x == 0 && threadIdx.x == 0) {
-      scal[3] = err;
-      done[1] = !(err == err);
-      done[0] = it;
-    }
-    return;
-  }
-  const double dot0 = scal[1 + ((it - 1) & 1)];
-  const double beta = zr / dot0;
-  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n;
-       i += (long long)gridDim.x * blockDim.x)
-    p[i] = z[i] + beta * p[i];
-  if (blockIdx.x == 0 && threadIdx.x == 0) {
-    scal[1 + (it & 1)] = zr;
-    scal[3] = err;
-  }
-}
-
-__global__ void diag_inv_kernel(int nnode, const long long *__restrict__ bdiag,
-                                const double *__restrict__ bval, double *__restrict__ dinv) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < 3 * nnode) dinv[i] = 1.0 / bval[9 * bdiag[i / 3] + 4 * (i % 3)];
-}
-
-__global__ void residual_kernel(int n, const double *__restrict__ b, const double *__restrict__ Ax,
-                                double *__restrict__ r) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n) r[i] = b[i] - Ax[i];
 }
 
 __global__ __launch_bounds__(256) void axpby_kernel(int n, double a, const double *__restrict__ x,
@@ -373,38 +200,29 @@ __global__ __launch_bounds__(256) void dot_partial_kernel(int n, const double *_
 
 __global__ void dot_finish_kernel(const double *__restrict__ part, double *__restrict__ out) {
   __shared__ double sh[16];
-  double s = reduce_partials(part, sh);
+  double s = reduce_partials(part, RED, sh);
   if (threadIdx.x == 0) *out = s;
 }
 
-int spmv_group(const Ctx &c) {
-  double avg = c.nnode ? (double)c.nbf / c.nnode : 1.0;
-  if (avg > 96) return 64;
-  if (avg > 40) return 32;
-  if (avg > 20) return 16;
-  if (avg > 6) return 8;
-  return 4;
+
+template <int G>
+__global__ __launch_bounds__(256) void bspmv_kernel(int nnode, const long long *__restrict__ bptr,
+                                                    const int *__restrict__ bcol,
+                                                    const double *__restrict__ bval,
+                                                    const double *__restrict__ x,
+                                                    double *__restrict__ y) {
+  const int rows_per_block = blockDim.x / G;
+  bspmv_rows<G>((long long)blockIdx.x * rows_per_block, nnode,
+                (long long)gridDim.x * rows_per_block, bptr, bcol, bval, x, y);
 }
 
 template <int G>
-void bspmv_launch(Ctx &c, const double *x, double *y, double *part, const int *done) {
+void bspmv_launch(Ctx &c, const double *x, double *y) {
   const int rows_per_block = TPB / G;
   long long want = (c.nnode + rows_per_block - 1) / rows_per_block;
   int grid = (int)std::min<long long>(want, 4 * RED);
-  if (part) grid = RED;  // partial slots are all written
   hipLaunchKernelGGL(bspmv_kernel<G>, dim3(grid), dim3(TPB), 0, c.stream, c.nnode, c.d_bptr,
-                     c.d_bcol, c.d_bval, x, y, part, done);
-}
-
-void spmv_impl(Ctx &c, const double *x, double *y, double *part, const int *done) {
-  switch (spmv_group(c)) {
-    case 64: bspmv_launch<64>(c, x, y, part, done); break;
-    case 32: bspmv_launch<32>(c, x, y, part, done); break;
-    case 16: bspmv_launch<16>(c, x, y, part, done); break;
-    case 8: bspmv_launch<8>(c, x, y, part, done); break;
-    default: bspmv_launch<4>(c, x, y, part, done); break;
-  }
-  FGPU_CHECK(hipGetLastError());
+                     c.d_bcol, c.d_bval, x, y);
 }
 
 }  // namespace
@@ -436,6 +254,14 @@ void launch_expand_Ke(Ctx &c, int e0, int ne, float *out) {
   FGPU_CHECK(hipGetLastError());
 }
 
+void ensure_eq_csr(Ctx &c) {
+  if (c.d_col) return;
+  FGPU_CHECK(hipMalloc(&c.d_col, std::max<long long>(c.nnz, 1) * sizeof(int)));
+  hipLaunchKernelGGL(fill_eq_cols_kernel, dim3((c.nnode + 255) / 256), dim3(256), 0, c.stream,
+                     c.nnode, c.d_bptr, c.d_bcol, c.d_eqmap, c.d_rowptr, c.d_col);
+  FGPU_CHECK(hipGetLastError());
+}
+
 void launch_csr_convert(Ctx &c, double *d_csrval, int to_csr) {
   hipLaunchKernelGGL(bsr_to_csr_kernel, dim3((c.nnode + 255) / 256), dim3(256), 0, c.stream,
                      c.nnode, c.d_bptr, c.d_bcol, c.d_bcoloff, c.d_bval, c.d_eqmap, c.d_rowptr,
@@ -456,7 +282,14 @@ void launch_nodal_to_eq(Ctx &c, const double *u, double *x) {
 }
 
 void launch_spmv_nodal(Ctx &c, const double *x, double *y) {
-  spmv_impl(c, x, y, nullptr, nullptr);
+  switch (spmv_group(c)) {
+    case 64: bspmv_launch<64>(c, x, y); break;
+    case 32: bspmv_launch<32>(c, x, y); break;
+    case 16: bspmv_launch<16>(c, x, y); break;
+    case 8: bspmv_launch<8>(c, x, y); break;
+    default: bspmv_launch<4>(c, x, y); break;
+  }
+  FGPU_CHECK(hipGetLastError());
 }
 
 void launch_axpby(Ctx &c, int n, double a, const double *x, double b, const double *y,
@@ -479,73 +312,5 @@ double dev_dot(Ctx &c, const double *x, const double *y, int n) {
   return c.h_scal[8];
 }
 
-// b, x, x0: equation vectors on the device.  The iteration runs on node
-// vectors (3 nnode entries, fixed directions identically zero).
-int run_pcg(Ctx &c, const double *b, double *x, const double *x0, int maxit, double rtol,
-            int *iters, double *relres) {
-  const int n = 3 * c.nnode;
-  hipStream_t s = c.stream;
-  if (iters) *iters = 0;
-  if (relres) *relres = 0.0;
-  if (c.neq == 0) return 1;
-  const int eb = (n + 255) / 256;
-  hipLaunchKernelGGL(diag_inv_kernel, dim3(eb), dim3(256), 0, s, c.nnode, c.d_bdiag, c.d_bval,
-                     c.d_dinv);
-  launch_eq_to_nodal(c, b, c.d_bn);
-  if (x0) {
-    launch_eq_to_nodal(c, x0, c.d_x);
-    spmv_impl(c, c.d_x, c.d_w, nullptr, nullptr);
-    hipLaunchKernelGGL(residual_kernel, dim3(eb), dim3(256), 0, s, n, c.d_bn, c.d_w, c.d_r);
-  } else {
-    FGPU_CHECK(hipMemsetAsync(c.d_x, 0, sizeof(double) * n, s));
-    FGPU_CHECK(hipMemcpyAsync(c.d_r, c.d_bn, sizeof(double) * n, hipMemcpyDeviceToDevice, s));
-  }
-  hipLaunchKernelGGL(pcg_init_kernel, dim3(RED), dim3(TPB), 0, s, n, c.d_bn, c.d_r, c.d_dinv,
-                     c.d_z, c.d_p, c.d_part);
-  hipLaunchKernelGGL(pcg_start_kernel, dim3(1), dim3(TPB), 0, s, c.d_part, c.d_scal, c.d_done);
-  FGPU_CHECK(hipGetLastError());
-  FGPU_CHECK(hipMemcpyAsync(c.h_scal, c.d_scal, 5 * sizeof(double), hipMemcpyDeviceToHost, s));
-  FGPU_CHECK(hipStreamSynchronize(s));
-  int conv_it = 0, broke = 0;
-  bool early = false;
-  // reference: |b| < tol returns at once (x stays at its start value)
-  if (c.h_scal[4] < rtol) early = true;
-  if (!early && c.h_scal[3] < rtol) {  // warm start already converged
-    early = true;
-  }
-  if (!early) {
-    const int CHUNK = 16;
-    int launched = 0;
-    while (launched < maxit) {
-      int upto = std::min(maxit, launched + CHUNK);
-      for (int it = launched + 1; it <= upto; it++) {
-        spmv_impl(c, c.d_p, c.d_w, c.d_part, c.d_done);
-        hipLaunchKernelGGL(pcg_update_kernel, dim3(RED), dim3(TPB), 0, s, n, it, c.d_x, c.d_r,
-                           c.d_z, c.d_p, c.d_w, c.d_dinv, c.d_part, c.d_scal, c.d_done);
-        hipLaunchKernelGGL(pcg_dir_kernel, dim3(RED), dim3(TPB), 0, s, n, it, c.d_p, c.d_z,
-                           c.d_part, c.d_scal, c.d_done, rtol);
-      }
-      FGPU_CHECK(hipGetLastError());
-      launched = upto;
-      FGPU_CHECK(hipMemcpyAsync(c.h_done, c.d_done, 2 * sizeof(int), hipMemcpyDeviceToHost, s));
-      FGPU_CHECK(hipMemcpyAsync(c.h_scal, c.d_scal, 4 * sizeof(double), hipMemcpyDeviceToHost, s));
-      FGPU_CHECK(hipStreamSynchronize(s));
-      if (c.h_done[0]) {
-        conv_it = c.h_done[0];
-        broke = c.h_done[1];
-        break;
-      }
-    }
-  }
-  launch_nodal_to_eq(c, c.d_x, x);
-  FGPU_CHECK(hipStreamSynchronize(s));
-  if (early) {
-    if (relres) *relres = c.h_scal[4] < rtol ? 0.0 : c.h_scal[3];
-    return 1;
-  }
-  if (iters) *iters = conv_it ? conv_it : maxit;
-  if (relres) *relres = c.h_scal[3];
-  return conv_it > 0 && !broke;
-}
 
 }  // namespace fgpu

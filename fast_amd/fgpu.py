@@ -32,7 +32,7 @@ class FgpuMesh(C.Structure):
                 ("nn", C.c_int), ("deg", C.c_int * 3), ("nip", C.c_int * 3),
                 ("form", C.c_int), ("coords", C.c_void_p), ("conn", C.c_void_p),
                 ("dofmap", C.c_void_p), ("ext", C.c_void_p), ("cpw", C.c_void_p),
-                ("mat", C.c_void_p)]
+                ("mat", C.c_void_p), ("nbatch", C.c_int), ("batch_nodeptr", C.c_void_p)]
 
 
 PREDICT_FN = C.CFUNCTYPE(C.c_int, C.c_void_p, C.c_int, C.c_void_p, C.c_double, C.c_double,
@@ -71,6 +71,11 @@ EXPORTS = {
     "fgpu_solve_dev": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int,
                                  C.c_double, C.c_void_p, C.c_void_p]),
     "fgpu_spmv_dev": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p]),
+    "fgpu_solve_batch": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int, C.c_double,
+                                   C.c_void_p, C.c_void_p]),
+    "fgpu_solve_batch_dev": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int,
+                                       C.c_double, C.c_void_p, C.c_void_p]),
+    "fgpu_nbatch": (C.c_int, [C.c_void_p]),
     "fgpu_get_csr": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),
     "fgpu_set_csr_values": (C.c_int, [C.c_void_p, C.c_void_p]),
     "fgpu_get_element_matrices": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_void_p]),
@@ -124,6 +129,9 @@ class GpuModel:
         m.mat = _p(c(mesh.mat, np.float64))
         m.ext = _p(c(mesh.ext, np.float64)) if mesh.ext is not None else None
         m.cpw = _p(c(mesh.cpw, np.float64)) if mesh.cpw is not None else None
+        if mesh.batch_nodeptr is not None:
+            m.nbatch = len(mesh.batch_nodeptr) - 1
+            m.batch_nodeptr = _p(c(mesh.batch_nodeptr, np.int64))
         self._m = m
         h = C.c_void_p()
         if not L.fgpu_create(C.byref(h), C.byref(m)):
@@ -175,6 +183,18 @@ class GpuModel:
         if raise_on_fail:
             self._check(conv, "fgpu_solve")
         return x, it.value, rr.value, bool(conv)
+
+    def solve_batch(self, b, maxit=None, tol=1e-8):
+        """Per-mesh Jacobi-PCG of a batch context.  Returns x, per-mesh
+        iteration counts, per-mesh relative residuals, number converged."""
+        nb = lib().fgpu_nbatch(self.h)
+        b = np.ascontiguousarray(b, dtype=np.float64)
+        x = np.zeros(self.neq)
+        it = np.zeros(nb, dtype=np.int32)
+        rr = np.zeros(nb)
+        n = lib().fgpu_solve_batch(self.h, _p(b), _p(x), self.neq if maxit is None else int(maxit),
+                                   tol, _p(it), _p(rr))
+        return x, it, rr, n
 
     def csr(self):
         rowptr = np.zeros(self.neq + 1, dtype=np.int64)

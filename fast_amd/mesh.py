@@ -47,6 +47,7 @@ class Mesh:
     fext: Optional[np.ndarray] = None  # (nnode, 3) reference load vector
     grid: Optional[tuple] = None       # structured element counts
     dofmap: Optional[np.ndarray] = None
+    batch_nodeptr: Optional[np.ndarray] = None  # independent meshes: node ranges
 
     def __post_init__(self):
         self.coords = np.ascontiguousarray(self.coords, dtype=np.float64)
@@ -301,3 +302,67 @@ def bsp_block(ne=(32, 32, 8), p=3, size=(1.0, 1.0, 1.0), E=1.0, nu=0.3,
                 form=form, mat=mat, fixed=fixed, deg=deg, ext=ext,
                 cpw=np.ones(nnode), span=span, knots=knots, nbas=nb,
                 grid=tuple(ne))
+
+
+# ---------------------------------------------------------------- batches --
+
+def hex8_cube(n: int, E=1.0, nu=0.3, clamp_face=0, load_face=1, pressure=1.0) -> Mesh:
+    """Unit cube of n^3 bricks; one face clamped, uniform pressure on another.
+
+    Faces are numbered 2*axis + side (side 0: coordinate 0, side 1:
+    coordinate 1).  The pressure acts against the outward normal of the
+    loaded face; nodal forces are the exact q*A/4 corner shares of each face.
+    """
+    m = hex8_block(n, n, n, E=E, nu=nu, clamp_x0=False)
+    X = m.coords
+    ax, side = divmod(int(clamp_face), 2)
+    m.fixed[np.isclose(X[:, ax], float(side))] = True
+    lax, lside = divmod(int(load_face), 2)
+    on = np.isclose(X[:, lax], float(lside))
+    # corner shares: interior face nodes 4 quarters, edges 2, corners 1
+    others = [d for d in range(3) if d != lax]
+    cnt = np.ones(len(X))
+    for d in others:
+        edge = np.isclose(X[:, d], 0.0) | np.isclose(X[:, d], 1.0)
+        cnt *= np.where(edge, 1.0, 2.0)
+    f = np.zeros((len(X), 3))
+    normal = 1.0 if lside == 1 else -1.0
+    f[on, lax] = -normal * pressure * cnt[on] * (1.0 / n) ** 2 / 4.0
+    m.fext = f
+    m.dofmap = number_dofs(m.conn, m.fixed, m.nnode)
+    return m
+
+
+def union(meshes) -> Mesh:
+    """Independent meshes of one element family as one block-diagonal model.
+
+    Nodes, elements and equations of mesh k follow those of mesh k-1, so each
+    mesh owns a contiguous node range (``batch_nodeptr``) and a contiguous
+    equation range."""
+    m0 = meshes[0]
+    nodeptr = np.zeros(len(meshes) + 1, dtype=np.int64)
+    nodeptr[1:] = np.cumsum([m.nnode for m in meshes])
+    conn = np.concatenate([m.conn + nodeptr[k] for k, m in enumerate(meshes)]).astype(np.int32)
+    fext = None
+    if all(m.fext is not None for m in meshes):
+        fext = np.concatenate([m.fext for m in meshes])
+    out = Mesh(coords=np.concatenate([m.coords for m in meshes]), conn=conn,
+               shape=m0.shape, nip=m0.nip, form=m0.form,
+               mat=np.concatenate([m.mat for m in meshes]),
+               fixed=np.concatenate([m.fixed for m in meshes]), deg=m0.deg, fext=fext)
+    out.batch_nodeptr = nodeptr
+    return out
+
+
+def cube_batch(nmesh=1024, nmin=4, nmax=32, seed=0, E=1.0):
+    """Batch of independent elastic cubes: edge n ~ U{nmin..nmax},
+    nu ~ U(0.2, 0.45), a random clamped face and pressure 1 on a random other
+    face, all from ``seed``.  Returns (union mesh, per-mesh parameters)."""
+    rng = np.random.default_rng(seed)
+    ns = rng.integers(nmin, nmax + 1, size=nmesh)
+    nus = rng.uniform(0.2, 0.45, size=nmesh)
+    cf = rng.integers(0, 6, size=nmesh)
+    lf = (cf + rng.integers(1, 6, size=nmesh)) % 6
+    meshes = [hex8_cube(int(n), E=E, nu=float(nu), clamp_face=int(c), load_face=int(l))
+              for n, nu, c, l in zip(ns, nus, cf, lf)]
+    return union(meshes), {"n": ns, "nu": nus, "clamp_face": cf, "load_face": lf}

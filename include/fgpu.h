@@ -53,6 +53,11 @@ typedef struct {
   const double *ext;    /* FGPU_BSP: [nelem*3*(p+1)^2] extraction rows */
   const double *cpw;    /* FGPU_BSP: [nnode] weights (NULL = all 1) */
   const double *mat;    /* [nelem*2] */
+  /* Batch of independent meshes (0 / NULL for one mesh): mesh k owns nodes
+   * [batch_nodeptr[k], batch_nodeptr[k+1]), no element couples two meshes,
+   * and its equations are contiguous (node-major numbering). */
+  int nbatch;
+  const long long *batch_nodeptr;  /* [nbatch+1] */
 } fgpu_mesh;
 
 /* Build a context: uploads the mesh, numbers the sparsity pattern and the
@@ -90,6 +95,16 @@ int fgpu_solve(fgpu_ctx *ctx, const double *b, double *x, const double *x0,
 int fgpu_solve_dev(fgpu_ctx *ctx, const double *b_dev, double *x_dev,
                    const double *x0_dev, int maxit, double rtol, int *iters,
                    double *relres);
+
+/* Batched solve: one Jacobi-PCG per mesh of a batch context, each with its
+ * own |b|, scalars and stopping test (the reference solves each model on its
+ * own), all meshes advancing in the same launches.  iters / relres (optional)
+ * have nbatch entries.  Returns the number of meshes that converged. */
+int fgpu_solve_batch(fgpu_ctx *ctx, const double *b, double *x, int maxit, double rtol,
+                     int *iters, double *relres);
+int fgpu_solve_batch_dev(fgpu_ctx *ctx, const double *b_dev, double *x_dev, int maxit,
+                         double rtol, int *iters, double *relres);
+int fgpu_nbatch(fgpu_ctx *ctx);
 
 /* y = K x on the assembled CSR (cSymSprsMatrix::MultVect, sysmat.cpp:942). */
 int fgpu_spmv_dev(fgpu_ctx *ctx, const double *x_dev, double *y_dev);
