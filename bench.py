@@ -138,6 +138,7 @@ def main():
     model.stiffness(u)                     # uploads u into the context
     d_u = model.device_buffer(2)
     g = torch.zeros(model.neq, dtype=torch.float64, device="cuda")
+    torch.cuda.synchronize()  # the library runs on its own non-blocking stream
 
     def step():
         if not L.fgpu_assemble_K_dev(model.h, d_u):
@@ -185,6 +186,7 @@ def main():
         f = m.load_vector() if m.fext is not None else model.int_force(u)
         b = torch.from_numpy(f).cuda()
         x = torch.zeros_like(b)
+        torch.cuda.synchronize()
         it = fgpu.C.c_int(0)
         rr = fgpu.C.c_double(0.0)
         model.stiffness(u)

@@ -74,46 +74,87 @@ int fgpu_newton(fgpu_ctx *ctx, const double *q, const fgpu_nr_params *prm, doubl
     if (!q || !prm) throw Error("null argument");
     const int n = c.neq;
     const size_t nb = sizeof(double) * n;
-    DevVec dq(n), df(n), dg(n), dr(n), du(n), ddu(n), dpred(n);
+    DevVec dq(n), df(n), dg(n), dr(n), du(n), ddu(n), dpred(n), dlast(n), dinc(n), dgsave(n),
+        dr0(n);
     FGPU_CHECK(hipMemcpyAsync(dq.p, q, nb, hipMemcpyHostToDevice, c.stream));
-    FGPU_CHECK(hipMemsetAsync(du.p, 0, nb, c.stream));
+    if (prm->u0_dev)
+      FGPU_CHECK(hipMemcpyAsync(du.p, prm->u0_dev, nb, hipMemcpyDeviceToDevice, c.stream));
+    else
+      FGPU_CHECK(hipMemsetAsync(du.p, 0, nb, c.stream));
     double *unod = c.d_u;  // nodal displacement field
     launch_eq_to_nodal(c, du.p, unod);
     if (!int_force_dev(c, unod, dg.p)) return 0;  // initial stresses
     const int pcg_maxit = prm->pcg_maxit > 0 ? prm->pcg_maxit : n;
-    double lam = 0.0;
+    double lam = prm->u0_dev ? prm->lam0 : 0.0, lam_last_step = lam;
+    bool have_last = false;
     int done_steps = 0;
     for (int step = 1; step <= prm->steps; step++) {
       const double t_step = now_ms();
       fgpu_nr_stat st{};
       st.step = step;
       const double lam_prev = lam;
-      lam += prm->factor;
+      lam = prm->lambdas ? prm->lambdas[step - 1] : lam + prm->factor;
       st.lam = lam;
       launch_axpby(c, n, lam, dq.p, 0.0, nullptr, df.p);  // f = lambda q
       const double flen = std::sqrt(dev_dot(c, df.p, df.p, n));
+      launch_axpby(c, n, 1.0, df.p, -1.0, dg.p, dr.p);  // r = f - g(u_prev)
+      const double ref0 = flen > prm->tol ? flen : std::sqrt(dev_dot(c, dg.p, dg.p, n));
+      double err = std::sqrt(dev_dot(c, dr.p, dr.p, n)) / std::max(1.0, ref0);
+      FGPU_CHECK(hipMemcpyAsync(dlast.p, du.p, nb, hipMemcpyDeviceToDevice, c.stream));
+      // predictor: u += du_pred; kept only when it lowers the residual
+      bool predicted = false;
       if (prm->predict) {
-        // predictor: u += du_pred, then refresh g
-        if (prm->predict(prm->predict_user, step, du.p, lam_prev, lam, dpred.p)) {
-          launch_axpby(c, n, 1.0, du.p, 1.0, dpred.p, du.p);
+        predicted = prm->predict(prm->predict_user, step, du.p, lam_prev, lam, dpred.p) != 0;
+      } else if (prm->secant && have_last && lam_prev != lam_last_step) {
+        launch_axpby(c, n, (lam - lam_prev) / (lam_prev - lam_last_step), dinc.p, 0.0, nullptr,
+                     dpred.p);
+        predicted = true;
+      }
+      if (predicted) {
+        // accept the prediction d when it lowers the potential energy:
+        // Pi(u + d) - Pi(u) = -int_0^1 d . r(u + s d) ds ~ -d . (r0 + r1) / 2
+        // (trapezoidal rule on the residuals before and after).  The l2 norm
+        // of the residual is not the test: it weights the stiff, fast-to-solve
+        // modes most and rejects good extrapolations.
+        FGPU_CHECK(hipMemcpyAsync(dgsave.p, dg.p, nb, hipMemcpyDeviceToDevice, c.stream));
+        FGPU_CHECK(hipMemcpyAsync(dr0.p, dr.p, nb, hipMemcpyDeviceToDevice, c.stream));
+        launch_axpby(c, n, 1.0, du.p, 1.0, dpred.p, du.p);
+        launch_eq_to_nodal(c, du.p, unod);
+        bool ok = int_force_dev(c, unod, dg.p);
+        st.ms_fint += c.last_ms[2];
+        double errp = 0, work = 0;
+        if (ok) {
+          launch_axpby(c, n, 1.0, df.p, -1.0, dg.p, dr.p);
+          const double refp = flen > prm->tol ? flen : std::sqrt(dev_dot(c, dg.p, dg.p, n));
+          errp = std::sqrt(dev_dot(c, dr.p, dr.p, n)) / std::max(1.0, refp);
+          work = dev_dot(c, dpred.p, dr0.p, n) + dev_dot(c, dpred.p, dr.p, n);
+        }
+        if (ok && std::isfinite(errp) && work > 0) {
+          err = errp;
+          st.predicted = 1;
+        } else {  // back to the converged state of the previous step
+          c.err.clear();
+          st.predicted = -1;
+          FGPU_CHECK(hipMemcpyAsync(du.p, dlast.p, nb, hipMemcpyDeviceToDevice, c.stream));
+          FGPU_CHECK(hipMemcpyAsync(dg.p, dgsave.p, nb, hipMemcpyDeviceToDevice, c.stream));
           launch_eq_to_nodal(c, du.p, unod);
-          if (!int_force_dev(c, unod, dg.p)) return 0;
-          st.ms_fint += c.last_ms[2];
+          launch_axpby(c, n, 1.0, df.p, -1.0, dg.p, dr.p);
         }
       }
-      launch_axpby(c, n, 1.0, df.p, -1.0, dg.p, dr.p);  // r = f - g
-      double err = 1.0;
-      bool conv = false;
-      int it;
-      for (it = 1; it <= prm->maxit; it++) {
+      st.err_start = err;
+      bool conv = err < prm->tol;
+      int it = 0;
+      for (it = 1; it <= prm->maxit && !conv; it++) {
         if (!stiffness_dev(c, unod)) return 0;
         st.ms_stiffness += c.last_ms[0] + c.last_ms[1];
         double ptol = prm->pcg_tol;
-        if (prm->adaptive_pcg && it > 1) ptol = std::min(ptol, std::max(1e-12, 0.1 * err));
+        if (prm->adaptive_pcg == 1 && it > 1) ptol = std::min(ptol, std::max(1e-12, 0.1 * err));
+        if (prm->adaptive_pcg == 2)
+          ptol = std::min(prm->pcg_tol, std::max(1e-12, 0.5 * prm->tol / std::max(err, 1e-300)));
         int pit = 0;
         double prr = 0;
         FGPU_CHECK(hipEventRecord(c.ev[5], c.stream));
-        run_pcg(c, dr.p, ddu.p, nullptr, pcg_maxit, ptol, &pit, &prr);
+        run_pcg(c, dr.p, ddu.p, nullptr, pcg_maxit, ptol, &pit, &prr, prm->adaptive_pcg == 0);
         FGPU_CHECK(hipEventRecord(c.ev[6], c.stream));
         FGPU_CHECK(hipEventSynchronize(c.ev[6]));
         float ms = 0;
@@ -133,20 +174,24 @@ int fgpu_newton(fgpu_ctx *ctx, const double *q, const fgpu_nr_params *prm, doubl
         double ref = flen;
         if (!(flen > prm->tol)) ref = std::sqrt(dev_dot(c, dg.p, dg.p, n));
         err = rlen / std::max(1.0, ref);
-        if (err < prm->tol) {
-          conv = true;
-          break;
-        }
+        if (err < prm->tol) conv = true;
       }
-      st.iters = std::min(it, prm->maxit);
+      st.iters = conv ? it - 1 : prm->maxit;
       st.err = err;
       st.converged = conv;
-      st.ms_total = now_ms() - t_step;
-      if (stats) stats[step - 1] = st;
+      // converged increment of this step (for the secant predictor)
+      launch_axpby(c, n, 1.0, du.p, -1.0, dlast.p, dinc.p);
+      have_last = true;
+      lam_last_step = lam_prev;
+      if (prm->u_steps_dev)
+        FGPU_CHECK(hipMemcpyAsync(prm->u_steps_dev + (size_t)(step - 1) * n, du.p, nb,
+                                  hipMemcpyDeviceToDevice, c.stream));
       if (u_steps)
         FGPU_CHECK(hipMemcpyAsync(u_steps + (size_t)(step - 1) * n, du.p, nb,
                                   hipMemcpyDeviceToHost, c.stream));
       FGPU_CHECK(hipStreamSynchronize(c.stream));
+      st.ms_total = now_ms() - t_step;
+      if (stats) stats[step - 1] = st;
       if (!conv) {
         c.err = "Newton iterations did not converge at step " + std::to_string(step);
         break;

@@ -137,8 +137,11 @@ void launch_assemble_K(Ctx &c);
 void launch_gather_fint(Ctx &c, double *d_g);
 void launch_spmv_nodal(Ctx &c, const double *x, double *y);  // node vectors
 void launch_csr_convert(Ctx &c, double *d_csrval, int to_csr);
+// abs_exit: the reference's early return when |b| < rtol (an absolute
+// test, mlib/sysmat.cpp PCGSolver); callers that pass a relative tolerance
+// computed from the residual (inexact Newton) turn it off.
 int run_pcg(Ctx &c, const double *b, double *x, const double *x0, int maxit,
-            double rtol, int *iters, double *relres);
+            double rtol, int *iters, double *relres, bool abs_exit = true);
 int run_pcg_batch(Ctx &c, const double *b, double *x, int maxit, double rtol, int *iters,
                   double *relres);
 void ensure_eq_csr(Ctx &c);

@@ -410,7 +410,7 @@ void prepare(Ctx &c, const double *b, const double *x0) {
 // b, x, x0: equation vectors on the device.  The iteration runs on node
 // vectors (3 nnode entries, fixed directions identically zero).
 int run_pcg(Ctx &c, const double *b, double *x, const double *x0, int maxit, double rtol,
-            int *iters, double *relres) {
+            int *iters, double *relres, bool abs_exit) {
   hipStream_t s = c.stream;
   if (iters) *iters = 0;
   if (relres) *relres = 0.0;
@@ -424,7 +424,7 @@ int run_pcg(Ctx &c, const double *b, double *x, const double *x0, int maxit, dou
   FGPU_CHECK(hipMemcpyAsync(c.h_scal, c.d_scal, 5 * sizeof(double), hipMemcpyDeviceToHost, s));
   FGPU_CHECK(hipStreamSynchronize(s));
   // reference: |b| < tol returns at once (x stays at its start value)
-  const bool early = c.h_scal[4] < rtol || c.h_scal[3] < rtol;
+  const bool early = (abs_exit && c.h_scal[4] < rtol) || c.h_scal[4] == 0.0 || c.h_scal[3] < rtol;
   int conv_it = 0, broke = 0, launched = 0;
   while (!early && launched < maxit) {
     run_chunk(c, false);
@@ -443,7 +443,7 @@ int run_pcg(Ctx &c, const double *b, double *x, const double *x0, int maxit, dou
   launch_nodal_to_eq(c, c.d_x, x);
   FGPU_CHECK(hipStreamSynchronize(s));
   if (early) {
-    if (relres) *relres = c.h_scal[4] < rtol ? 0.0 : c.h_scal[3];
+    if (relres) *relres = (abs_exit && c.h_scal[4] < rtol) || c.h_scal[4] == 0.0 ? 0.0 : c.h_scal[3];
     return 1;
   }
   if (iters) *iters = conv_it ? conv_it : c.h_done[2] - 1;

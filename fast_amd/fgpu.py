@@ -42,12 +42,15 @@ PREDICT_FN = C.CFUNCTYPE(C.c_int, C.c_void_p, C.c_int, C.c_void_p, C.c_double, C
 class NrParams(C.Structure):
     _fields_ = [("steps", C.c_int), ("factor", C.c_double), ("maxit", C.c_int),
                 ("tol", C.c_double), ("pcg_maxit", C.c_int), ("pcg_tol", C.c_double),
-                ("adaptive_pcg", C.c_int), ("predict", PREDICT_FN), ("predict_user", C.c_void_p)]
+                ("adaptive_pcg", C.c_int), ("predict", PREDICT_FN), ("predict_user", C.c_void_p),
+                ("lambdas", C.c_void_p), ("u0_dev", C.c_void_p), ("lam0", C.c_double),
+                ("u_steps_dev", C.c_void_p), ("secant", C.c_int)]
 
 
 class NrStat(C.Structure):
     _fields_ = [("step", C.c_int), ("iters", C.c_int), ("pcg_iters", C.c_int),
-                ("converged", C.c_int), ("lam", C.c_double), ("err", C.c_double),
+                ("converged", C.c_int), ("predicted", C.c_int), ("lam", C.c_double),
+                ("err", C.c_double), ("err_start", C.c_double),
                 ("ms_stiffness", C.c_double), ("ms_fint", C.c_double),
                 ("ms_solve", C.c_double), ("ms_total", C.c_double)]
 
@@ -81,6 +84,20 @@ EXPORTS = {
     "fgpu_get_element_matrices": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_void_p]),
     "fgpu_last_time_ms": (C.c_double, [C.c_void_p, C.c_int]),
     "fgpu_device_buffer": (C.c_void_p, [C.c_void_p, C.c_int]),
+    "fgpu_gnn_create": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint]),
+    "fgpu_gnn_destroy": (None, [C.c_void_p]),
+    "fgpu_gnn_num_params": (C.c_int, [C.c_void_p]),
+    "fgpu_gnn_set_scales": (C.c_int, [C.c_void_p, C.c_double, C.c_double, C.c_double]),
+    "fgpu_gnn_get_params": (C.c_int, [C.c_void_p, C.c_void_p]),
+    "fgpu_gnn_set_params": (C.c_int, [C.c_void_p, C.c_void_p]),
+    "fgpu_gnn_loss_grad": (C.c_int, [C.c_void_p, C.c_void_p, C.c_double, C.c_double,
+                                     C.c_void_p, C.c_void_p, C.c_void_p]),
+    "fgpu_gnn_predict": (C.c_int, [C.c_void_p, C.c_void_p, C.c_double, C.c_double,
+                                   C.c_void_p]),
+    "fgpu_gnn_train": (C.c_int, [C.c_void_p, C.c_int, C.c_void_p, C.c_void_p, C.c_int, C.c_int,
+                                 C.c_float, C.c_uint, C.c_void_p]),
+    "fgpu_gnn_newton_predictor": (C.c_int, [C.c_void_p, C.c_int, C.c_void_p, C.c_double,
+                                            C.c_double, C.c_void_p]),
 }
 
 
@@ -214,18 +231,27 @@ class GpuModel:
         self._check(ok, "fgpu_linear_static")
         return u, it.value, rr.value
 
-    def newton(self, q, steps, factor, maxit=20, tol=1e-5, pcg_tol=1e-5, pcg_maxit=0,
-               adaptive_pcg=False, predict=None):
+    def newton(self, q, steps, factor=0.0, maxit=20, tol=1e-5, pcg_tol=1e-5, pcg_maxit=0,
+               adaptive_pcg=0, predict=None, lambdas=None, u0_dev=None, lam0=0.0,
+               u_steps_dev=None, secant=False, keep_host=True):
         """cCtrlNR::Solver on the device (ctrl/ctrlnr.cpp:47-267).
 
-        ``predict(step, u_dev_ptr, lam_prev, lam, du_dev_ptr) -> bool`` may
-        write a displacement-increment prediction into du (device, neq)."""
+        ``predict`` is a Python callable ``(step, u_dev_ptr, lam_prev, lam,
+        du_dev_ptr) -> bool`` or a ``Gnn`` (its native predictor is passed
+        without a Python hop).  ``lambdas`` replaces the fixed increment
+        ``factor`` by a list of load factors; ``u0_dev``/``lam0`` start from
+        a converged state; ``u_steps_dev`` (device, steps x neq) receives the
+        converged displacement of every step."""
         q = np.ascontiguousarray(q, dtype=np.float64)
         prm = NrParams()
         prm.steps, prm.factor, prm.maxit, prm.tol = steps, factor, maxit, tol
         prm.pcg_maxit, prm.pcg_tol, prm.adaptive_pcg = pcg_maxit, pcg_tol, int(adaptive_pcg)
+        prm.secant = int(secant)
         cb = None
-        if predict is not None:
+        if isinstance(predict, Gnn):
+            prm.predict = C.cast(lib().fgpu_gnn_newton_predictor, PREDICT_FN)
+            prm.predict_user = predict.h
+        elif predict is not None:
             def _cb(user, step, u_dev, lam_prev, lam, du_dev):
                 try:
                     return 1 if predict(step, u_dev, lam_prev, lam, du_dev) else 0
@@ -234,12 +260,22 @@ class GpuModel:
                     return 0
             cb = PREDICT_FN(_cb)
             prm.predict = cb
-        us = np.zeros((steps, self.neq))
+        lam_arr = None
+        if lambdas is not None:
+            lam_arr = np.ascontiguousarray(lambdas, dtype=np.float64)
+            assert lam_arr.shape == (steps,)
+            prm.lambdas = lam_arr.ctypes.data
+        prm.u0_dev = u0_dev
+        prm.lam0 = lam0
+        prm.u_steps_dev = u_steps_dev
+        us = np.zeros((steps, self.neq)) if keep_host else None
         stats = (NrStat * steps)()
         done = lib().fgpu_newton(self.h, _p(q), C.byref(prm), _p(us), stats)
         st = [{k: getattr(stats[i], k) for k, _ in NrStat._fields_} for i in range(steps)]
         err = lib().fgpu_last_error(self.h).decode() if done < steps else ""
-        return us[:max(done, 0)], st[:max(done, 0) + (1 if done < steps else 0)], err
+        if us is not None:
+            us = us[:max(done, 0)]
+        return us, st[:max(done, 0) + (1 if done < steps else 0)], err
 
     def set_csr_values(self, val):
         val = np.ascontiguousarray(val, dtype=np.float64)
@@ -258,3 +294,67 @@ class GpuModel:
 
     def device_buffer(self, which):
         return lib().fgpu_device_buffer(self.h, which)
+
+
+class Gnn:
+    """Learned warm start of a load step (fgpu_gnn_*): a 4-layer residual
+    message-passing network, hidden width 64, trained on the device with
+    Adam on converged states of a continuation run of the same mesh."""
+
+    def __init__(self, model: GpuModel, load_mask=None, seed=0):
+        self.model = model
+        lm = None
+        if load_mask is not None:
+            lm = np.ascontiguousarray(load_mask, dtype=np.uint8)
+            assert lm.shape == (model.mesh.nnode,)
+        h = C.c_void_p()
+        if not lib().fgpu_gnn_create(C.byref(h), model.h, _p(lm), seed):
+            raise FgpuError("fgpu_gnn_create: %s" % lib().fgpu_last_error(model.h).decode())
+        self.h = h
+        self.nparam = lib().fgpu_gnn_num_params(h)
+
+    def _check(self, ok, what):
+        if not ok:
+            raise FgpuError("%s: %s" % (what, lib().fgpu_last_error(self.model.h).decode()))
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().fgpu_gnn_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_scales(self, u_scale, lam_scale, out_scale):
+        self._check(lib().fgpu_gnn_set_scales(self.h, u_scale, lam_scale, out_scale),
+                    "fgpu_gnn_set_scales")
+
+    def params(self):
+        p = np.zeros(self.nparam, dtype=np.float32)
+        self._check(lib().fgpu_gnn_get_params(self.h, _p(p)), "fgpu_gnn_get_params")
+        return p
+
+    def set_params(self, p):
+        p = np.ascontiguousarray(p, dtype=np.float32)
+        assert p.shape == (self.nparam,)
+        self._check(lib().fgpu_gnn_set_params(self.h, _p(p)), "fgpu_gnn_set_params")
+
+    def loss_grad(self, u_dev, lam, dlam, ut_dev, grad=True):
+        loss = C.c_double(0.0)
+        g = np.zeros(self.nparam, dtype=np.float32) if grad else None
+        self._check(lib().fgpu_gnn_loss_grad(self.h, u_dev, lam, dlam, ut_dev, C.byref(loss),
+                                             _p(g)), "fgpu_gnn_loss_grad")
+        return loss.value, g
+
+    def predict(self, u_dev, lam, dlam, du_dev):
+        self._check(lib().fgpu_gnn_predict(self.h, u_dev, lam, dlam, du_dev), "fgpu_gnn_predict")
+
+    def train(self, states_dev, lams, nsteps, maxgap=4, lr=1e-3, seed=0):
+        lams = np.ascontiguousarray(lams, dtype=np.float64)
+        hist = np.zeros(nsteps, dtype=np.float32)
+        self._check(lib().fgpu_gnn_train(self.h, len(lams), states_dev, _p(lams), nsteps, maxgap,
+                                         lr, seed, _p(hist)), "fgpu_gnn_train")
+        return hist

@@ -15,8 +15,11 @@
  *   - host pointers unless the name ends in _dev (device pointers on the
  *     context's HIP device, already resident in HBM).
  * One context = one mesh (or one batch of independent meshes) = one HIP
- * stream on the current device.  A context is not re-entrant; independent
- * contexts may be used from different host threads.
+ * stream on the current device, created non-blocking.  Device inputs must be
+ * complete when a call starts (synchronise the stream that wrote them); every
+ * call has finished its device work, outputs included, when it returns.  A
+ * context is not re-entrant; independent contexts may be used from different
+ * host threads.
  */
 #ifndef FGPU_H
 #define FGPU_H
@@ -132,7 +135,9 @@ int fgpu_linear_static(fgpu_ctx *ctx, const double *f, double *u, int maxit,
 /* Predictor hook for the Newton driver: given the converged equation vector
  * u_dev of the previous step and the load factors, write a predicted
  * displacement increment into du_dev (neq, device).  Return 1 to use it,
- * 0 to start the step from du = 0. */
+ * 0 to start the step from du = 0.  A prediction d is discarded (the step
+ * then starts from du = 0) unless it lowers the potential energy by the
+ * trapezoidal estimate d . (r(u) + r(u + d)) > 0, r = f - f_int. */
 typedef int (*fgpu_predict_fn)(void *user, int step, const double *u_dev,
                                double lam_prev, double lam, double *du_dev);
 
@@ -143,15 +148,28 @@ typedef struct {
   double tol;         /* Tol: |r| / max(1, |f|) */
   int pcg_maxit;      /* <= 0: neq (as the reference's NR) */
   double pcg_tol;     /* reference NR: 1e-5 (sysmat.h:77-78 default) */
-  int adaptive_pcg;   /* 1: PCG tolerance min(pcg_tol, 0.1 * err) (inexact
-                         Newton; 0 = fixed pcg_tol as in the reference) */
+  int adaptive_pcg;   /* 0: fixed pcg_tol, as the reference;
+                         1: min(pcg_tol, 0.1 err) after the first iteration;
+                         2: the PCG reduction each Newton iteration needs to
+                            reach tol: clamp(0.5 tol max(1,|f|) / |r|,
+                            1e-12, pcg_tol) */
   fgpu_predict_fn predict;  /* optional */
   void *predict_user;
+  const double *lambdas;    /* optional [steps]: load factor of each step
+                               (else lambda += factor) */
+  const double *u0_dev;     /* optional start state (equation vector, device) */
+  double lam0;              /* load factor of the start state */
+  double *u_steps_dev;      /* optional [steps * neq] converged states (device) */
+  int secant;               /* 1: without a predictor, extrapolate the last
+                               converged increment linearly in lambda */
 } fgpu_nr_params;
 
 typedef struct {
   int step, iters, pcg_iters, converged;
+  int predicted;      /* 1 prediction used; 0 none; -1 rejected (energy
+                         test failed or f_int failed at the prediction) */
   double lam, err;
+  double err_start;   /* |r| / max(1, |f|) at the start of the step */
   double ms_stiffness, ms_fint, ms_solve, ms_total;
 } fgpu_nr_stat;
 
@@ -163,6 +181,41 @@ typedef struct {
  * non-converged step); 0 with fgpu_last_error on a failure. */
 int fgpu_newton(fgpu_ctx *ctx, const double *q, const fgpu_nr_params *prm,
                 double *u_steps, fgpu_nr_stat *stats);
+
+/* ---------------------------------------------------------------- learned
+ * warm start (no counterpart in the reference; for callers that run many
+ * related Newton solves on one mesh).  A 4-layer message-passing network
+ * (hidden 64, FP16 MFMA with FP32 accumulation) over the mesh graph predicts
+ * the displacement increment of a load step from the current state; the
+ * unchanged Newton/PCG then drives the residual to tolerance.  Vectors are
+ * equation vectors on the device. */
+typedef struct fgpu_gnn fgpu_gnn;
+/* load_mask: [nnode] bit d set where the load acts in direction d (or NULL) */
+int fgpu_gnn_create(fgpu_gnn **net, fgpu_ctx *ctx, const unsigned char *load_mask,
+                    unsigned seed);
+void fgpu_gnn_destroy(fgpu_gnn *net);
+int fgpu_gnn_num_params(fgpu_gnn *net);
+/* feature / target normalisation: |u| scale, load-factor scale, output scale */
+int fgpu_gnn_set_scales(fgpu_gnn *net, double u_scale, double lam_scale, double out_scale);
+int fgpu_gnn_get_params(fgpu_gnn *net, float *params);
+int fgpu_gnn_set_params(fgpu_gnn *net, const float *params);
+/* loss and gradient (host) of one sample: state u at lam, target state ut at
+ * lam + dlam; u may be NULL (zero state) */
+int fgpu_gnn_loss_grad(fgpu_gnn *net, const double *u_dev, double lam, double dlam,
+                       const double *ut_dev, double *loss, float *grad);
+/* du = predicted increment of a step lam -> lam + dlam from state u */
+int fgpu_gnn_predict(fgpu_gnn *net, const double *u_dev, double lam, double dlam,
+                     double *du_dev);
+/* Adam training on a converged trajectory: states [nstate * neq] (device) at
+ * load factors lams[nstate] (host, increasing); each step draws a pair
+ * i < j with a log-uniform gap j - i in [1, maxgap] (i = -1: the unloaded
+ * state) and takes one Adam step on its loss.  loss_hist (optional, host):
+ * the loss of every step. */
+int fgpu_gnn_train(fgpu_gnn *net, int nstate, const double *states_dev, const double *lams,
+                   int nsteps, int maxgap, float lr, unsigned seed, float *loss_hist);
+/* fgpu_predict_fn adapter: pass with predict_user = the fgpu_gnn handle */
+int fgpu_gnn_newton_predictor(void *user, int step, const double *u_dev, double lam_prev,
+                              double lam, double *du_dev);
 
 /* Time (ms, HIP events on the context stream) of the last call of each
  * phase: 0 = element K, 1 = assembly, 2 = element f_int (+ gather),

@@ -1,0 +1,192 @@
+"""The learned warm start on the device (fgpu_gnn_*), through the C-ABI:
+forward, loss and hand-written backward against the CPU FP64 oracle
+(oracle/gnn_oracle.py), device training, and what the trained network gains
+in the Newton driver over the same driver without it.
+
+Tolerances: the dense layers take FP16 operands (unit roundoff 2^-11 =
+4.9e-4) with FP32 accumulation through seven products in a row, so the loss,
+the gradient and the prediction are held to 2e-2 normwise relative error
+against FP64; the Newton solves with and without the network must reach the
+same converged states to 1e-6.
+"""
+import numpy as np
+import pytest
+
+from fast_amd import mesh as M
+from oracle import gnn_oracle as G
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def fg():
+    import torch
+    assert torch.cuda.is_available(), "GPU tests need a HIP device"
+    from fast_amd import fgpu
+    fgpu.lib()
+    return fgpu
+
+
+def nrel(a, b):
+    return np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300)
+
+
+def masks(m):
+    bits = np.array([1, 2, 4], dtype=np.uint8)
+    fmask = ((~m.fixed).astype(np.uint8) * bits).sum(1).astype(np.uint8)
+    lmask = ((m.fext != 0).astype(np.uint8) * bits).sum(1).astype(np.uint8)
+    return fmask, lmask
+
+
+def to_eq(m, u):
+    out = np.zeros(m.neq)
+    d = m.dofmap.ravel()
+    k = d > 0
+    out[d[k] - 1] = u.ravel()[k]
+    return out
+
+
+def cube(n):
+    return M.hex8_block(n, n, n, form="tl_neohooke", mat=np.tile((1.0, 1.5), (n ** 3, 1)),
+                        traction=(0.0, 0.0, -1.0))
+
+
+def fields(m, seed):
+    rng = np.random.default_rng(seed)
+    X = m.coords
+    u = 0.03 * np.stack([np.sin(2 * X[:, 0] + X[:, 1]), X[:, 0] * X[:, 2], -X[:, 0] ** 2], 1)
+    ut = u + 0.004 * np.stack([X[:, 0], X[:, 1] * X[:, 0], -2 * X[:, 0] ** 2], 1)
+    ut += 1e-4 * rng.standard_normal(ut.shape)
+    u[m.fixed] = 0
+    ut[m.fixed] = 0
+    return u, ut
+
+
+SC = (0.05, 0.3, 0.2)  # u_scale, lam_scale, out_scale
+
+
+@pytest.mark.parametrize("zero_state", [False, True])
+def test_loss_and_gradient_match_oracle(fg, zero_state):
+    import torch
+    m = cube(6)
+    fmask, lmask = masks(m)
+    model = fg.GpuModel(m)
+    net = fg.Gnn(model, lmask, seed=3)
+    net.set_scales(*SC)
+    u, ut = fields(m, 0)
+    if zero_state:
+        u[:] = 0
+    lam, dlam = 0.12, 0.02
+    du = torch.from_numpy(to_eq(m, u)).cuda()
+    dut = torch.from_numpy(to_eq(m, ut)).cuda()
+    torch.cuda.synchronize()
+    loss, grad = net.loss_grad(None if zero_state else du.data_ptr(), lam, dlam, dut.data_ptr())
+    P = net.params().astype(np.float64)
+    X = G.features(m.coords, u, lam, dlam, fmask, lmask, SC[0], SC[1])
+    Mg = G.graph(m.conn, m.nnode)
+    T = G.target(u, ut, dlam, SC[2], fmask)
+    l_ref, g_ref, _ = G.loss_grad(P, X, Mg, T, fmask, m.neq)
+    assert abs(loss - l_ref) <= 2e-2 * l_ref, (loss, l_ref)
+    assert nrel(grad.astype(np.float64), g_ref) <= 2e-2, nrel(grad, g_ref)
+    # every parameter block receives its gradient (a dropped term shows here)
+    L = G.layout()
+    for k, (o, n) in ((k, v) for k, v in L.items() if k != "total"):
+        gb, rb = grad[o:o + n].astype(np.float64), g_ref[o:o + n]
+        if np.linalg.norm(rb) > 1e-6 * np.linalg.norm(g_ref):
+            assert nrel(gb, rb) <= 5e-2, (k, nrel(gb, rb))
+
+
+def test_prediction_matches_oracle(fg):
+    import torch
+    m = cube(5)
+    fmask, lmask = masks(m)
+    model = fg.GpuModel(m)
+    net = fg.Gnn(model, lmask, seed=7)
+    net.set_scales(*SC)
+    # parameters from the oracle side: the layouts must agree
+    rng = np.random.default_rng(1)
+    P = (0.15 * rng.standard_normal(G.layout()["total"])).astype(np.float32)
+    net.set_params(P)
+    assert np.array_equal(net.params(), P)
+    u, _ = fields(m, 1)
+    lam, dlam = 0.2, 0.03
+    du = torch.from_numpy(to_eq(m, u)).cuda()
+    out = torch.zeros(m.neq, dtype=torch.float64, device="cuda")
+    torch.cuda.synchronize()
+    net.predict(du.data_ptr(), lam, dlam, out.data_ptr())
+    X = G.features(m.coords, u, lam, dlam, fmask, lmask, SC[0], SC[1])
+    ref = G.predict(P.astype(np.float64), X, G.graph(m.conn, m.nnode), dlam, SC[2], fmask)
+    assert nrel(out.cpu().numpy(), to_eq(m, ref)) <= 2e-2
+
+
+def test_training_and_warm_start_gain(fg):
+    """Continuation data from the device Newton driver, device training, then
+    the same load path solved with and without the trained predictor."""
+    import torch
+    m = cube(8)
+    fmask, lmask = masks(m)
+    model = fg.GpuModel(m)
+    q = m.load_vector()
+    rng = np.random.default_rng(0)
+    lams = np.sort(rng.uniform(0.0, 0.3, 200))
+    states = torch.zeros((len(lams), m.neq), dtype=torch.float64, device="cuda")
+    torch.cuda.synchronize()
+    us, st, err = model.newton(q, steps=len(lams), lambdas=lams, maxit=20, tol=1e-8,
+                               pcg_tol=1.0, adaptive_pcg=2, secant=True,
+                               u_steps_dev=states.data_ptr(), keep_host=False)
+    assert err == "" and all(s["converged"] for s in st), err
+    umax = float(states[-1].abs().max())
+    net = fg.Gnn(model, lmask, seed=0)
+    net.set_scales(umax, 0.3, umax / 0.3)
+    hist = net.train(states.data_ptr(), lams, nsteps=1500, maxgap=64, lr=2e-3, seed=0)
+    assert np.all(np.isfinite(hist))
+    assert hist[-100:].mean() < 0.05 * hist[:20].mean(), (hist[:20].mean(), hist[-100:].mean())
+    # held-out load path: 10 equal increments to 0.3
+    kw = dict(steps=10, factor=0.03, maxit=20, tol=1e-8, pcg_tol=1.0, adaptive_pcg=2)
+    u0, s0, e0 = model.newton(q, **kw)
+    u1, s1, e1 = model.newton(q, predict=net, **kw)
+    assert e0 == "" and e1 == "", (e0, e1)
+    for a, b in zip(s0, s1):
+        print("step %d plain: iters %d pcg %d err0 %.2e | net: iters %d pcg %d err0 %.2e"
+              % (a["step"], a["iters"], a["pcg_iters"], a["err_start"], b["iters"],
+                 b["pcg_iters"], b["err_start"]))
+    print("loss %.3e -> %.3e" % (hist[:20].mean(), hist[-100:].mean()))
+    assert nrel(u1[-1], u0[-1]) <= 1e-6
+    assert all(s["predicted"] == 1 for s in s1)
+    # the network's start saves PCG iterations over the plain start
+    assert sum(s["pcg_iters"] for s in s1) < sum(s["pcg_iters"] for s in s0)
+
+
+def test_newton_driver_options(fg):
+    """Load-factor lists, restarts from a device state, the secant predictor
+    and the rejection of a prediction that raises the residual all reach the
+    converged states of the plain fixed-increment run."""
+    import torch
+    m = cube(5)
+    model = fg.GpuModel(m)
+    q = m.load_vector()
+    base = dict(maxit=30, tol=1e-10, pcg_tol=1e-12)
+    uA, sA, eA = model.newton(q, steps=4, factor=0.05, **base)
+    assert eA == ""
+    uB, sB, eB = model.newton(q, steps=4, lambdas=[0.05, 0.1, 0.15, 0.2], secant=True,
+                              maxit=30, tol=1e-10, pcg_tol=1.0, adaptive_pcg=2)
+    assert eB == "" and nrel(uB[-1], uA[-1]) <= 1e-7
+    assert [s["predicted"] for s in sB] == [0, 1, 1, 1]
+    states = torch.zeros((2, m.neq), dtype=torch.float64, device="cuda")
+    torch.cuda.synchronize()
+    uC, sC, eC = model.newton(q, steps=2, factor=0.05, u_steps_dev=states.data_ptr(), **base)
+    assert np.allclose(states.cpu().numpy(), uC, rtol=0, atol=0)
+    uD, sD, eD = model.newton(q, steps=2, factor=0.05, u0_dev=states[1].data_ptr(), lam0=0.1,
+                              **base)
+    assert eD == "" and nrel(uD[-1], uA[-1]) <= 1e-8
+    assert abs(sD[-1]["lam"] - 0.2) < 1e-15
+    ones = torch.ones(m.neq, dtype=torch.float64, device="cuda")
+    torch.cuda.synchronize()
+    L = fg.lib()
+
+    def bad(step, u_dev, lam_prev, lam, du_dev):  # du = K 1: far from equilibrium
+        return bool(L.fgpu_spmv_dev(model.h, ones.data_ptr(), du_dev))
+
+    uE, sE, eE = model.newton(q, steps=4, factor=0.05, predict=bad, **base)
+    assert eE == "" and nrel(uE[-1], uA[-1]) <= 1e-8
+    assert all(s["predicted"] == -1 for s in sE)
