@@ -11,6 +11,8 @@
 #include <algorithm>
 #include <chrono>
 #include <cmath>
+#include <cstdio>
+#include <cstdlib>
 #include <vector>
 
 #include "fgpu_internal.h"
@@ -42,6 +44,13 @@ struct DevVec {
   DevVec(const DevVec &) = delete;
   DevVec &operator=(const DevVec &) = delete;
 };
+
+// y = K x on equation vectors (PCG work vectors as scratch)
+void spmv_eq(Ctx &c, const double *x, double *y) {
+  launch_eq_to_nodal(c, x, c.d_bn);
+  launch_spmv_nodal(c, c.d_bn, c.d_w);
+  launch_nodal_to_eq(c, c.d_w, y);
+}
 
 double now_ms() {
   return std::chrono::duration<double, std::milli>(
@@ -75,7 +84,8 @@ int fgpu_newton(fgpu_ctx *ctx, const double *q, const fgpu_nr_params *prm, doubl
     const int n = c.neq;
     const size_t nb = sizeof(double) * n;
     DevVec dq(n), df(n), dg(n), dr(n), du(n), ddu(n), dpred(n), dlast(n), dinc(n), dgsave(n),
-        dr0(n);
+        dr0(n), db2(prm->galerkin ? n : 1), dw1(prm->galerkin ? n : 1), dw2(prm->galerkin ? n : 1);
+    bool k_valid = false;  // the context holds the tangent of the current state
     FGPU_CHECK(hipMemcpyAsync(dq.p, q, nb, hipMemcpyHostToDevice, c.stream));
     if (prm->u0_dev)
       FGPU_CHECK(hipMemcpyAsync(du.p, prm->u0_dev, nb, hipMemcpyDeviceToDevice, c.stream));
@@ -85,6 +95,8 @@ int fgpu_newton(fgpu_ctx *ctx, const double *q, const fgpu_nr_params *prm, doubl
     launch_eq_to_nodal(c, du.p, unod);
     if (!int_force_dev(c, unod, dg.p)) return 0;  // initial stresses
     const int pcg_maxit = prm->pcg_maxit > 0 ? prm->pcg_maxit : n;
+    const char *venv = std::getenv("FGPU_NR_VERBOSE");  // per-iteration log on stderr
+    const bool verbose = venv && venv[0] == '1';
     double lam = prm->u0_dev ? prm->lam0 : 0.0, lam_last_step = lam;
     bool have_last = false;
     int done_steps = 0;
@@ -100,15 +112,44 @@ int fgpu_newton(fgpu_ctx *ctx, const double *q, const fgpu_nr_params *prm, doubl
       launch_axpby(c, n, 1.0, df.p, -1.0, dg.p, dr.p);  // r = f - g(u_prev)
       const double ref0 = flen > prm->tol ? flen : std::sqrt(dev_dot(c, dg.p, dg.p, n));
       double err = std::sqrt(dev_dot(c, dr.p, dr.p, n)) / std::max(1.0, ref0);
-      FGPU_CHECK(hipMemcpyAsync(dlast.p, du.p, nb, hipMemcpyDeviceToDevice, c.stream));
-      // predictor: u += du_pred; kept only when it lowers the residual
+      // predictor (dlast still holds the state the previous step started from)
       bool predicted = false;
       if (prm->predict) {
-        predicted = prm->predict(prm->predict_user, step, du.p, lam_prev, lam, dpred.p) != 0;
+        predicted = prm->predict(prm->predict_user, step, du.p, lam_prev, lam,
+                                 have_last ? dlast.p : nullptr, lam_last_step, dpred.p) != 0;
       } else if (prm->secant && have_last && lam_prev != lam_last_step) {
         launch_axpby(c, n, (lam - lam_prev) / (lam_prev - lam_last_step), dinc.p, 0.0, nullptr,
                      dpred.p);
         predicted = true;
+      }
+      FGPU_CHECK(hipMemcpyAsync(dlast.p, du.p, nb, hipMemcpyDeviceToDevice, c.stream));
+      if (predicted && prm->galerkin) {
+        // d = sum_i a_i b_i over b = {prediction, secant of the last step}
+        // minimising the quadratic model 1/2 d.K d - d.r of the tangent at
+        // the previous state (a Galerkin projection onto the span)
+        if (!k_valid) {
+          if (!stiffness_dev(c, unod)) return 0;
+          st.ms_stiffness += c.last_ms[0] + c.last_ms[1];
+          k_valid = true;
+        }
+        const bool two = have_last && lam_prev != lam_last_step;
+        if (two)
+          launch_axpby(c, n, (lam - lam_prev) / (lam_prev - lam_last_step), dinc.p, 0.0, nullptr,
+                       db2.p);
+        spmv_eq(c, dpred.p, dw1.p);
+        const double g11 = dev_dot(c, dpred.p, dw1.p, n), r1 = dev_dot(c, dpred.p, dr.p, n);
+        double a1 = g11 > 0 ? r1 / g11 : 1.0, a2 = 0.0;
+        if (two) {
+          spmv_eq(c, db2.p, dw2.p);
+          const double g12 = dev_dot(c, dpred.p, dw2.p, n), g22 = dev_dot(c, db2.p, dw2.p, n);
+          const double r2 = dev_dot(c, db2.p, dr.p, n);
+          const double det = g11 * g22 - g12 * g12;
+          if (g11 > 0 && g22 > 0 && det > 1e-12 * g11 * g22) {
+            a1 = (r1 * g22 - r2 * g12) / det;
+            a2 = (r2 * g11 - r1 * g12) / det;
+          }
+        }
+        launch_axpby(c, n, a1, dpred.p, a2, two ? db2.p : nullptr, dpred.p);
       }
       if (predicted) {
         // accept the prediction d when it lowers the potential energy:
@@ -146,6 +187,7 @@ int fgpu_newton(fgpu_ctx *ctx, const double *q, const fgpu_nr_params *prm, doubl
       int it = 0;
       for (it = 1; it <= prm->maxit && !conv; it++) {
         if (!stiffness_dev(c, unod)) return 0;
+        k_valid = true;
         st.ms_stiffness += c.last_ms[0] + c.last_ms[1];
         double ptol = prm->pcg_tol;
         if (prm->adaptive_pcg == 1 && it > 1) ptol = std::min(ptol, std::max(1e-12, 0.1 * err));
@@ -161,6 +203,9 @@ int fgpu_newton(fgpu_ctx *ctx, const double *q, const fgpu_nr_params *prm, doubl
         FGPU_CHECK(hipEventElapsedTime(&ms, c.ev[5], c.ev[6]));
         st.ms_solve += ms;
         st.pcg_iters += pit;
+        if (verbose)
+          std::fprintf(stderr, "fgpu_newton: step %d lambda %.6g iter %d |r|rel %.3e pcg_tol %.2e "
+                       "pcg %d (%.2f ms)\n", step, lam, it, err, ptol, pit, ms);
         launch_axpby(c, n, 1.0, du.p, 1.0, ddu.p, du.p);  // u += du
         launch_eq_to_nodal(c, du.p, unod);
         if (!int_force_dev(c, unod, dg.p)) {  // stress update failed: stop

@@ -4,17 +4,29 @@
 // (FP16 operands converted on load, FP32 accumulation, FP32 master weights).
 //
 // Graph: the mesh nodes; neighbours = nodes sharing an element (the node-block
-// rows of the global matrix, self included).  Per node features (16, padded):
-//   x, y, z (box-normalised), u_state / u_scale (3), lambda / lam_scale,
-//   dlambda / lam_scale, free-direction flags (3), load flags (3), 1.
-// Network (H = 64, L = 4):
+// rows of the global matrix, self included).  A sample is a converged state u
+// at load factor lambda, an earlier converged state u_h at lambda_h (the
+// unloaded state when there is none) and the step dlambda.  Per node features
+// (32, the last 14 zero):
+//   0-2   x, y, z (box-normalised)       3-5  u / u_scale
+//   6-8   s = u / (lambda out_scale), the slope from the unloaded state
+//   9-11  v = (u - u_h) / ((lambda - lambda_h) out_scale), the slope of the
+//         last step (the secant predictor; s when there is no history)
+//   12    lambda / lam_scale   13  dlambda / lam_scale
+//   14    (lambda - lambda_h) / lam_scale
+//   15    free-direction bit mask / 7    16  load bit mask / 7    17  1
+// (s, v are 0 at lambda = 0.)  Network (H = 64, L = 4):
 //   h0 = relu(X W_in + b_in)
 //   h_{l+1} = h_l + relu(h_l W_s,l + mean_N(h_l) W_n,l + b_l)
-//   d = relu(h_L W_d1 + b_d1);  y = d W_d2 + b_d2   (first 3 columns used)
-//   du = dlambda * out_scale * y   (zero on fixed directions)
-// Loss: mean squared error of y against (u_target - u_state) / (dlambda *
-// out_scale) over free directions.  Every reduction (weight gradients, bias
-// gradients, loss) is a block-partial sum added in a fixed order.
+//   d = relu(h_L W_d1 + b_d1);  y = M M (d W_d2 + b_d2)   (first 3 columns,
+//   output layer in FP32; M = mean over the neighbours: the correction is
+//   smoothed twice after the last rounding to FP16)
+//   du = dlambda * out_scale * (v + y)   (zero on fixed directions)
+// so the network learns the correction of the secant predictor.  Loss: the
+// squared error e of v + y against (u_target - u) / (dlambda out_scale) over
+// free directions, plus beta times the squared roughness |e - M e|^2 over all
+// nodes, divided by neq.  Every reduction (weight gradients, bias gradients,
+// loss) is a block-partial sum added in a fixed order.
 #include <cmath>
 #include <cstring>
 #include <random>
@@ -28,15 +40,17 @@ typedef _Float16 half4 __attribute__((ext_vector_type(4)));
 typedef float floatx4 __attribute__((ext_vector_type(4)));
 
 constexpr int GH = 64;   // hidden width
-constexpr int GF = 16;   // padded feature / output width
+constexpr int GFI = 32;  // input features per node
+constexpr int GF = 16;   // padded output width
 constexpr int GL = 4;    // message-passing layers
+constexpr int GS = 2;    // smoothing passes (mean aggregation) of the decoder
 
 // parameter offsets in the flat FP32 vector
 struct GnnLayout {
   int w_in, b_in, ws[GL], wn[GL], bl[GL], w_d1, b_d1, w_d2, b_d2, total;
   GnnLayout() {
     int o = 0;
-    w_in = o; o += GF * GH;
+    w_in = o; o += GFI * GH;
     b_in = o; o += GH;
     for (int l = 0; l < GL; l++) {
       ws[l] = o; o += GH * GH;
@@ -56,12 +70,13 @@ struct Gnn {
   GnnLayout lay;
   int N = 0;              // nodes
   float *P = nullptr, *G = nullptr, *Mo = nullptr, *Vo = nullptr;  // params, grads, Adam moments
-  float *X = nullptr;     // [N x 16] features
+  float *X = nullptr;     // [N x 32] features
   float *H[GL + 1] = {};  // [N x 64] h_l
   float *A[GL] = {};      // [N x 64] relu outputs of the layers
   float *AG[GL] = {};     // [N x 64] mean aggregations
   float *H0 = nullptr;    // relu(X W_in + b)  (== H[0])
-  float *D1 = nullptr;    // [N x 64]
+  float *D1 = nullptr;    // [N x 64] decoder hidden layer
+  float *YS[2] = {};      // [N x 16] output before / between smoothing passes
   float *Y = nullptr;     // [N x 16]
   float *T = nullptr;     // [N x 16] target (scaled)
   float *dY = nullptr, *dD1 = nullptr, *dH = nullptr, *dZ = nullptr, *tmp = nullptr,
@@ -70,6 +85,8 @@ struct Gnn {
   float *box = nullptr;   // [6] bounding box (min, 1/extent)
   double *dbuf = nullptr; // loss scalar
   double u_scale = 1.0, lam_scale = 1.0, out_scale = 1.0;
+  double beta = 0.0;      // loss weight of the roughness of the error
+  float *E = nullptr, *ME = nullptr;  // [N x 16] error, its aggregation
   int nblk_rows = 0;
   int step = 0;
   unsigned seed = 0;
@@ -78,7 +95,7 @@ struct Gnn {
 namespace {
 
 constexpr int ROWS_PER_BLOCK = 64;   // rowgemm: 4 waves x 16 rows
-constexpr int WG_ROWS = 2048;        // rows per weight-gradient partial
+constexpr int WG_ROWS = 256;         // rows per weight-gradient partial
 
 struct Bop {  // B[k][n] = p[k * sk + n * sm]
   const float *p;
@@ -204,54 +221,89 @@ __global__ __launch_bounds__(256) void colsum_kernel(long N, const float *__rest
   }
 }
 
-// agg[i] = (1/deg_i) sum_{j in N(i)} h[j]   (16 lanes per node, 4 features each)
+// agg[i] = (1/deg_i) sum_{j in N(i)} h[j]   (W/4 lanes per node, 4 features each)
+template <int W>
 __global__ __launch_bounds__(256) void agg_fwd_kernel(int N, const long long *__restrict__ bptr,
                                                       const int *__restrict__ bcol,
                                                       const float *__restrict__ h,
                                                       float *__restrict__ agg) {
+  constexpr int LPN = W / 4;
   const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
-  const int i = (int)(t / 16), f = 4 * (int)(t % 16);
+  const int i = (int)(t / LPN), f = 4 * (int)(t % LPN);
   if (i >= N) return;
   const long long k0 = bptr[i], k1 = bptr[i + 1];
   float s0 = 0, s1 = 0, s2 = 0, s3 = 0;
   for (long long k = k0; k < k1; k++) {
-    const float *x = h + (long)bcol[k] * GH + f;
+    const float *x = h + (long)bcol[k] * W + f;
     s0 += x[0];
     s1 += x[1];
     s2 += x[2];
     s3 += x[3];
   }
   const float inv = 1.f / (float)(k1 - k0);
-  float *o = agg + (long)i * GH + f;
+  float *o = agg + (long)i * W + f;
   o[0] = s0 * inv;
   o[1] = s1 * inv;
   o[2] = s2 * inv;
   o[3] = s3 * inv;
 }
 
-// dh[j] += sum_{i in N(j)} dagg[i] / deg_i  (the graph is symmetric)
+// dh[j] (+)= sum_{i in N(j)} dagg[i] / deg_i  (the graph is symmetric)
+template <int W>
 __global__ __launch_bounds__(256) void agg_bwd_kernel(int N, const long long *__restrict__ bptr,
                                                       const int *__restrict__ bcol,
                                                       const float *__restrict__ dagg,
-                                                      float *__restrict__ dh) {
+                                                      float *__restrict__ dh, int accumulate) {
+  constexpr int LPN = W / 4;
   const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
-  const int j = (int)(t / 16), f = 4 * (int)(t % 16);
+  const int j = (int)(t / LPN), f = 4 * (int)(t % LPN);
   if (j >= N) return;
   float s0 = 0, s1 = 0, s2 = 0, s3 = 0;
   for (long long k = bptr[j]; k < bptr[j + 1]; k++) {
     const int i = bcol[k];
     const float inv = 1.f / (float)(bptr[i + 1] - bptr[i]);
-    const float *x = dagg + (long)i * GH + f;
+    const float *x = dagg + (long)i * W + f;
     s0 += x[0] * inv;
     s1 += x[1] * inv;
     s2 += x[2] * inv;
     s3 += x[3] * inv;
   }
-  float *o = dh + (long)j * GH + f;
-  o[0] += s0;
-  o[1] += s1;
-  o[2] += s2;
-  o[3] += s3;
+  float *o = dh + (long)j * W + f;
+  if (accumulate) {
+    o[0] += s0;
+    o[1] += s1;
+    o[2] += s2;
+    o[3] += s3;
+  } else {
+    o[0] = s0;
+    o[1] = s1;
+    o[2] = s2;
+    o[3] = s3;
+  }
+}
+
+// output layer in FP32: y[i][c] = sum_k d[i][k] W[k][c] + b[c] for the three
+// used columns (0 elsewhere).  It runs on the vector ALUs in FP32 so that no
+// FP16 rounding enters the prediction after its last smoothing input.
+__global__ __launch_bounds__(256) void head_kernel(int N, const float *__restrict__ D,
+                                                   const float *__restrict__ W,
+                                                   const float *__restrict__ b,
+                                                   float *__restrict__ Y) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= N) return;
+  const float *d = D + (long)i * GH;
+  float y0 = b[0], y1 = b[1], y2 = b[2];
+  for (int k = 0; k < GH; k++) {
+    const float dk = d[k];
+    y0 = fmaf(dk, W[k * GF + 0], y0);
+    y1 = fmaf(dk, W[k * GF + 1], y1);
+    y2 = fmaf(dk, W[k * GF + 2], y2);
+  }
+  float *o = Y + (long)i * GF;
+  o[0] = y0;
+  o[1] = y1;
+  o[2] = y2;
+  for (int c = 3; c < GF; c++) o[c] = 0.f;
 }
 
 // dz = dy * (a > 0)
@@ -266,24 +318,35 @@ __global__ void add_kernel(long n, float *__restrict__ y, const float *__restric
   if (i < n) y[i] += x[i];
 }
 
-// features of node i for state u (nodal FP64), lam, dlam
+constexpr int SKIP = 9;  // feature offset of the skip term v
+
+// features of node i (nodal FP64 states; uh == nullptr: the unloaded state)
 __global__ void features_kernel(int N, const double *__restrict__ coords,
-                                const double *__restrict__ u, const unsigned char *__restrict__ fmask,
+                                const double *__restrict__ u, const double *__restrict__ uh,
+                                const unsigned char *__restrict__ fmask,
                                 const unsigned char *__restrict__ lmask,
                                 const float *__restrict__ box, double u_scale, double lam_scale,
-                                double lam, double dlam, float *__restrict__ X) {
+                                double out_scale, double lam, double lamh, double dlam,
+                                float *__restrict__ X) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= N) return;
-  float *x = X + (long)i * GF;
-  for (int d = 0; d < 3; d++) x[d] = (float)((coords[3 * i + d] - box[d]) * box[3 + d]);
-  for (int d = 0; d < 3; d++) x[3 + d] = (float)(u[3 * i + d] / u_scale);
-  x[6] = (float)(lam / lam_scale);
-  x[7] = (float)(dlam / lam_scale);
-  const unsigned fm = fmask[i], lm = lmask[i];
-  for (int d = 0; d < 3; d++) x[8 + d] = (fm >> d) & 1u ? 1.f : 0.f;
-  for (int d = 0; d < 3; d++) x[11 + d] = (lm >> d) & 1u ? 1.f : 0.f;
-  x[14] = 1.f;
-  x[15] = 0.f;
+  float *x = X + (long)i * GFI;
+  const double ss = lam != 0.0 ? 1.0 / (lam * out_scale) : 0.0;
+  const double sv = lam > lamh ? 1.0 / ((lam - lamh) * out_scale) : 0.0;
+  for (int d = 0; d < 3; d++) {
+    const double ui = u[3 * i + d], uhi = uh ? uh[3 * i + d] : 0.0;
+    x[d] = (float)((coords[3 * i + d] - box[d]) * box[3 + d]);
+    x[3 + d] = (float)(ui / u_scale);
+    x[6 + d] = (float)(ui * ss);
+    x[SKIP + d] = (float)((ui - uhi) * sv);
+  }
+  x[12] = (float)(lam / lam_scale);
+  x[13] = (float)(dlam / lam_scale);
+  x[14] = (float)((lam - lamh) / lam_scale);
+  x[15] = (float)fmask[i] / 7.f;
+  x[16] = (float)lmask[i] / 7.f;
+  x[17] = 1.f;
+  for (int k = 18; k < GFI; k++) x[k] = 0.f;
 }
 
 // target T = (u_tgt - u) / (dlam out_scale) on free directions, 0 elsewhere
@@ -299,11 +362,13 @@ __global__ void target_kernel(int N, const double *__restrict__ u, const double 
   }
 }
 
-// dY = 2 (Y - T) / count on free directions; partial loss sums per block
+// e = v + y - T on free directions (0 elsewhere, also stored in E);
+// dY = 2 e (d loss / dy times the count); partial loss sums per block
 __global__ __launch_bounds__(256) void loss_kernel(int N, const float *__restrict__ Y,
+                                                   const float *__restrict__ X,
                                                    const float *__restrict__ T,
                                                    const unsigned char *__restrict__ fmask,
-                                                   double inv_count, float *__restrict__ dY,
+                                                   float *__restrict__ dY, float *__restrict__ E,
                                                    double *__restrict__ part) {
   __shared__ double sh[16];
   double s = 0.0;
@@ -312,17 +377,55 @@ __global__ __launch_bounds__(256) void loss_kernel(int N, const float *__restric
     for (int c = 0; c < GF; c++) {
       float g = 0.f;
       if (c < 3 && ((fmask[i] >> c) & 1u)) {
-        const double e = (double)Y[i * GF + c] - (double)T[i * GF + c];
+        const double e =
+            (double)Y[i * GF + c] + (double)X[i * GFI + SKIP + c] - (double)T[i * GF + c];
         s += e * e;
-        // d(loss)/dy times count: keeps FP16 backward operands O(1); the
-        // count is divided out when gradients are reported (Adam is
-        // invariant to the scale)
+        // the count is divided out when gradients are reported: keeps the
+        // FP16 backward operands O(1) (Adam is invariant to the scale)
         g = (float)(2.0 * e);
       }
       dY[i * GF + c] = g;
+      E[i * GF + c] = 0.5f * g;
     }
   double t = block_sum(s, sh);
   if (threadIdx.x == 0) part[blockIdx.x] = t;
+}
+
+// roughness of the error: D = E - M E (in place over ME); partial sums of D^2
+__global__ __launch_bounds__(256) void rough_kernel(int N, const float *__restrict__ E,
+                                                    float *__restrict__ ME,
+                                                    double *__restrict__ part) {
+  __shared__ double sh[16];
+  double s = 0.0;
+  for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < N;
+       i += (long)gridDim.x * blockDim.x)
+    for (int c = 0; c < 3; c++) {
+      const float d = E[i * GF + c] - ME[i * GF + c];
+      ME[i * GF + c] = d;
+      s += (double)d * (double)d;
+    }
+  double t = block_sum(s, sh);
+  if (threadIdx.x == 0) part[blockIdx.x] = t;
+}
+
+// dY += 2 beta (D - M^T D) on free directions
+__global__ void rough_grad_kernel(int N, const float *__restrict__ D, const float *__restrict__ MTD,
+                                  const unsigned char *__restrict__ fmask, float beta,
+                                  float *__restrict__ dY) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= N) return;
+  for (int c = 0; c < 3; c++)
+    if ((fmask[i] >> c) & 1u)
+      dY[(long)i * GF + c] += 2.f * beta * (D[(long)i * GF + c] - MTD[(long)i * GF + c]);
+}
+
+// out = (sum part[0:RED] + beta sum part[RED:2 RED]) * inv_count
+__global__ void loss_finish2_kernel(const double *__restrict__ part, double beta,
+                                    double inv_count, double *__restrict__ out) {
+  __shared__ double sh[16];
+  double a = reduce_partials(part, RED, sh);
+  double b = reduce_partials(part + RED, RED, sh);
+  if (threadIdx.x == 0) *out = (a + beta * b) * inv_count;
 }
 
 __global__ void loss_finish_kernel(const double *__restrict__ part, double inv_count,
@@ -332,14 +435,16 @@ __global__ void loss_finish_kernel(const double *__restrict__ part, double inv_c
   if (threadIdx.x == 0) *out = s * inv_count;
 }
 
-// du = dlam out_scale y (nodal FP64), zero on fixed directions
-__global__ void output_kernel(int N, const float *__restrict__ Y,
+// du = dlam out_scale (v + y) (nodal FP64), zero on fixed directions
+__global__ void output_kernel(int N, const float *__restrict__ Y, const float *__restrict__ X,
                               const unsigned char *__restrict__ fmask, double scale,
                               double *__restrict__ du) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= N) return;
   for (int c = 0; c < 3; c++)
-    du[3 * i + c] = ((fmask[i] >> c) & 1u) ? scale * (double)Y[(long)i * GF + c] : 0.0;
+    du[3 * i + c] = ((fmask[i] >> c) & 1u)
+                        ? scale * ((double)Y[(long)i * GF + c] + (double)X[(long)i * GFI + SKIP + c])
+                        : 0.0;
 }
 
 __global__ void adam_kernel(int n, float *__restrict__ P, const float *__restrict__ G,
@@ -396,14 +501,16 @@ void bgrad(Gnn &g, const float *dY, float *db) {
                      g.part, db, 0);
 }
 
+template <int W = GH>
 void agg_fwd(Gnn &g, const float *h, float *agg) {
-  hipLaunchKernelGGL(agg_fwd_kernel, dim3(grid_of((long)g.N * 16)), dim3(256), 0, g.c->stream,
-                     g.N, g.c->d_bptr, g.c->d_bcol, h, agg);
+  hipLaunchKernelGGL(agg_fwd_kernel<W>, dim3(grid_of((long)g.N * (W / 4))), dim3(256), 0,
+                     g.c->stream, g.N, g.c->d_bptr, g.c->d_bcol, h, agg);
 }
 
-void agg_bwd(Gnn &g, const float *dagg, float *dh) {
-  hipLaunchKernelGGL(agg_bwd_kernel, dim3(grid_of((long)g.N * 16)), dim3(256), 0, g.c->stream,
-                     g.N, g.c->d_bptr, g.c->d_bcol, dagg, dh);
+template <int W = GH>
+void agg_bwd(Gnn &g, const float *dagg, float *dh, int accumulate = 1) {
+  hipLaunchKernelGGL(agg_bwd_kernel<W>, dim3(grid_of((long)g.N * (W / 4))), dim3(256), 0,
+                     g.c->stream, g.N, g.c->d_bptr, g.c->d_bcol, dagg, dh, accumulate);
 }
 
 }  // namespace
@@ -411,7 +518,7 @@ void agg_bwd(Gnn &g, const float *dagg, float *dh) {
 void gnn_forward(Gnn &g) {
   const GnnLayout &L = g.lay;
   const float *P = g.P;
-  rowgemm<GH, true>(g, g.X, GF, fwdW(P + L.w_in, GH), nullptr, 0, Bop{}, P + L.b_in, nullptr,
+  rowgemm<GH, true>(g, g.X, GFI, fwdW(P + L.w_in, GH), nullptr, 0, Bop{}, P + L.b_in, nullptr,
                     g.H[0], nullptr);
   for (int l = 0; l < GL; l++) {
     agg_fwd(g, g.H[l], g.AG[l]);
@@ -420,8 +527,12 @@ void gnn_forward(Gnn &g) {
   }
   rowgemm<GH, true>(g, g.H[GL], GH, fwdW(P + L.w_d1, GH), nullptr, 0, Bop{}, P + L.b_d1, nullptr,
                     g.D1, nullptr);
-  rowgemm<GF, false>(g, g.D1, GH, fwdW(P + L.w_d2, GF), nullptr, 0, Bop{}, P + L.b_d2, nullptr,
-                     g.Y, nullptr);
+  // smoothed FP32 output: y = M^GS (d1 W_d2 + b_d2) (M: mean over the neighbours)
+  float *out[2] = {g.YS[0], g.YS[1]};
+  hipLaunchKernelGGL(head_kernel, dim3(grid_of(g.N)), dim3(256), 0, g.c->stream, g.N, g.D1,
+                     P + L.w_d2, P + L.b_d2, GS ? out[0] : g.Y);
+  for (int k = 0; k < GS; k++)
+    agg_fwd<GF>(g, out[k % 2], k == GS - 1 ? g.Y : out[(k + 1) % 2]);
   FGPU_CHECK(hipGetLastError());
 }
 
@@ -432,10 +543,16 @@ void gnn_backward(Gnn &g) {
   float *G = g.G;
   const long NH = (long)g.N * GH;
   hipStream_t s = g.c->stream;
-  wgrad<GH, GF>(g, g.D1, g.dY, G + L.w_d2);
-  bgrad<GF>(g, g.dY, G + L.b_d2);
-  // dD1 = (dY W_d2^T) * (D1 > 0)
-  rowgemm<GH, false>(g, g.dY, GF, bwdW(P + L.w_d2, GF), nullptr, 0, Bop{}, nullptr, nullptr,
+  // dY0 = (M^T)^GS dY, the gradient at the unsmoothed output
+  const float *dy0 = g.dY;
+  for (int k = 0; k < GS; k++) {
+    agg_bwd<GF>(g, dy0, g.YS[k % 2], 0);
+    dy0 = g.YS[k % 2];
+  }
+  wgrad<GH, GF>(g, g.D1, dy0, G + L.w_d2);
+  bgrad<GF>(g, dy0, G + L.b_d2);
+  // dD1 = (dY0 W_d2^T) * (D1 > 0)
+  rowgemm<GH, false>(g, dy0, GF, bwdW(P + L.w_d2, GF), nullptr, 0, Bop{}, nullptr, nullptr,
                      g.tmp, nullptr);
   hipLaunchKernelGGL(relu_bwd_kernel, dim3(grid_of(NH)), dim3(256), 0, s, NH, g.tmp, g.D1, g.dD1);
   wgrad<GH, GH>(g, g.H[GL], g.dD1, G + L.w_d1);
@@ -457,27 +574,42 @@ void gnn_backward(Gnn &g) {
   }
   // h0 = relu(X W_in + b_in)
   hipLaunchKernelGGL(relu_bwd_kernel, dim3(grid_of(NH)), dim3(256), 0, s, NH, g.dH, g.H[0], g.dZ);
-  wgrad<GF, GH>(g, g.X, g.dZ, G + L.w_in);
+  wgrad<GFI, GH>(g, g.X, g.dZ, G + L.w_in);
   bgrad<GH>(g, g.dZ, G + L.b_in);
   FGPU_CHECK(hipGetLastError());
 }
 
+// loss = (sum_free e^2 + beta sum_nodes |e - M e|^2) / neq: the second term
+// weights the local roughness of the error, which drives the residual of a
+// predicted state far more than its size
 double gnn_loss(Gnn &g) {
   const double inv = 1.0 / (double)std::max(1, g.c->neq);  // free directions
-  hipLaunchKernelGGL(loss_kernel, dim3(RED), dim3(256), 0, g.c->stream, g.N, g.Y, g.T,
-                     g.c->d_fmask, inv, g.dY, g.c->d_part);
-  hipLaunchKernelGGL(loss_finish_kernel, dim3(1), dim3(256), 0, g.c->stream, g.c->d_part, inv,
-                     g.dbuf);
+  hipStream_t s = g.c->stream;
+  hipLaunchKernelGGL(loss_kernel, dim3(RED), dim3(256), 0, s, g.N, g.Y, g.X, g.T, g.c->d_fmask,
+                     g.dY, g.E, g.c->d_part);
+  if (g.beta > 0) {
+    agg_fwd<GF>(g, g.E, g.ME);
+    hipLaunchKernelGGL(rough_kernel, dim3(RED), dim3(256), 0, s, g.N, g.E, g.ME,
+                       g.c->d_part + RED);
+    agg_bwd<GF>(g, g.ME, g.YS[0], 0);
+    hipLaunchKernelGGL(rough_grad_kernel, dim3(grid_of(g.N)), dim3(256), 0, s, g.N, g.ME,
+                       g.YS[0], g.c->d_fmask, (float)g.beta, g.dY);
+    hipLaunchKernelGGL(loss_finish2_kernel, dim3(1), dim3(256), 0, s, g.c->d_part, g.beta, inv,
+                       g.dbuf);
+  } else {
+    hipLaunchKernelGGL(loss_finish_kernel, dim3(1), dim3(256), 0, s, g.c->d_part, inv, g.dbuf);
+  }
   double h = 0;
   FGPU_CHECK(hipMemcpyAsync(&h, g.dbuf, sizeof(double), hipMemcpyDeviceToHost, g.c->stream));
   FGPU_CHECK(hipStreamSynchronize(g.c->stream));
   return h;
 }
 
-void gnn_features(Gnn &g, const double *u, double lam, double dlam, const unsigned char *lmask) {
+void gnn_features(Gnn &g, const double *u, double lam, const double *uh, double lamh, double dlam,
+                  const unsigned char *lmask) {
   hipLaunchKernelGGL(features_kernel, dim3(grid_of(g.N)), dim3(256), 0, g.c->stream, g.N,
-                     g.c->d_coords, u, g.c->d_fmask, lmask, g.box, g.u_scale, g.lam_scale, lam,
-                     dlam, g.X);
+                     g.c->d_coords, u, uh, g.c->d_fmask, lmask, g.box, g.u_scale, g.lam_scale,
+                     g.out_scale, lam, lamh, dlam, g.X);
 }
 
 void gnn_target(Gnn &g, const double *u, const double *ut, double dlam) {
@@ -486,7 +618,7 @@ void gnn_target(Gnn &g, const double *u, const double *ut, double dlam) {
 }
 
 void gnn_output(Gnn &g, double dlam, double *du) {
-  hipLaunchKernelGGL(output_kernel, dim3(grid_of(g.N)), dim3(256), 0, g.c->stream, g.N, g.Y,
+  hipLaunchKernelGGL(output_kernel, dim3(grid_of(g.N)), dim3(256), 0, g.c->stream, g.N, g.Y, g.X,
                      g.c->d_fmask, dlam * g.out_scale, du);
 }
 
@@ -503,7 +635,7 @@ void gnn_alloc(Gnn &g, Ctx &c, unsigned seed) {
   g.N = c.nnode;
   g.seed = seed;
   g.nblk_rows = (int)((g.N + WG_ROWS - 1) / WG_ROWS);
-  const size_t NH = (size_t)g.N * GH, NF = (size_t)g.N * GF;
+  const size_t NH = (size_t)g.N * GH, NF = (size_t)g.N * GF, NX = (size_t)g.N * GFI;
   auto alloc = [](size_t n) {
     float *p = nullptr;
     FGPU_CHECK(hipMalloc(&p, std::max<size_t>(n, 1) * sizeof(float)));
@@ -513,13 +645,16 @@ void gnn_alloc(Gnn &g, Ctx &c, unsigned seed) {
   g.G = alloc(g.lay.total);
   g.Mo = alloc(g.lay.total);
   g.Vo = alloc(g.lay.total);
-  g.X = alloc(NF);
+  g.X = alloc(NX);
   for (int l = 0; l <= GL; l++) g.H[l] = alloc(NH);
   for (int l = 0; l < GL; l++) {
     g.A[l] = alloc(NH);
     g.AG[l] = alloc(NH);
   }
   g.D1 = alloc(NH);
+  for (int k = 0; k < 2; k++) g.YS[k] = alloc(NF);
+  g.E = alloc(NF);
+  g.ME = alloc(NF);
   g.Y = alloc(NF);
   g.T = alloc(NF);
   g.dY = alloc(NF);
@@ -540,7 +675,7 @@ void gnn_alloc(Gnn &g, Ctx &c, unsigned seed) {
     for (int k = 0; k < fin * fout; k++) P[off + k] = U(rng);
   };
   const GnnLayout &L = g.lay;
-  init(L.w_in, GF, GH);
+  init(L.w_in, GFI, GH);
   for (int l = 0; l < GL; l++) {
     init(L.ws[l], GH, GH);
     init(L.wn[l], GH, GH);
@@ -575,12 +710,14 @@ void gnn_alloc(Gnn &g, Ctx &c, unsigned seed) {
 }
 
 void gnn_free(Gnn &g) {
-  float *bufs[] = {g.P, g.G, g.Mo, g.Vo, g.X, g.D1, g.Y, g.T, g.dY, g.dD1, g.dH, g.dZ, g.tmp,
+  float *bufs[] = {g.P, g.G, g.Mo, g.Vo, g.X, g.D1, g.Y, g.T, g.E, g.ME, g.dY, g.dD1, g.dH, g.dZ, g.tmp,
                    g.tmp2, g.part, g.box};
   for (float *p : bufs)
     if (p) (void)hipFree(p);
   for (int l = 0; l <= GL; l++)
     if (g.H[l]) (void)hipFree(g.H[l]);
+  for (int k = 0; k < 2; k++)
+    if (g.YS[k]) (void)hipFree(g.YS[k]);
   for (int l = 0; l < GL; l++) {
     if (g.A[l]) (void)hipFree(g.A[l]);
     if (g.AG[l]) (void)hipFree(g.AG[l]);
@@ -594,7 +731,7 @@ namespace {
 struct GnnHandle {
   Gnn g;
   unsigned char *lmask = nullptr;  // load flags per node
-  double *un = nullptr, *utn = nullptr, *dun = nullptr;  // nodal scratch
+  double *un = nullptr, *uhn = nullptr, *utn = nullptr, *dun = nullptr;  // nodal scratch
 };
 
 template <class F>
@@ -616,12 +753,15 @@ void to_nodal(GnnHandle &H, const double *u_eq, double *u_nodal) {
     FGPU_CHECK(hipMemsetAsync(u_nodal, 0, sizeof(double) * 3 * H.g.N, H.g.c->stream));
 }
 
-double sample_loss_grad(GnnHandle &H, const double *u_eq, double lam, double dlam,
-                        const double *ut_eq, bool grad) {
+// one sample: state u at lam, history uh at lamh (nullptr: the unloaded
+// state), target ut at lam + dlam (equation vectors; u == nullptr: zero)
+double sample_loss_grad(GnnHandle &H, const double *u_eq, double lam, const double *uh_eq,
+                        double lamh, double dlam, const double *ut_eq, bool grad) {
   Gnn &g = H.g;
   to_nodal(H, u_eq, H.un);
   to_nodal(H, ut_eq, H.utn);
-  gnn_features(g, H.un, lam, dlam, H.lmask);
+  if (uh_eq) to_nodal(H, uh_eq, H.uhn);
+  gnn_features(g, H.un, lam, uh_eq ? H.uhn : nullptr, uh_eq ? lamh : 0.0, dlam, H.lmask);
   gnn_target(g, H.un, H.utn, dlam);
   gnn_forward(g);
   double loss = gnn_loss(g);
@@ -649,7 +789,7 @@ int fgpu_gnn_create(fgpu_gnn **out, fgpu_ctx *ctx, const unsigned char *load_mas
       copy_sync(c, H->lmask, load_mask, c.nnode, hipMemcpyHostToDevice);
     else
       fill_sync(c, H->lmask, 0, c.nnode);
-    for (double **p : {&H->un, &H->utn, &H->dun})
+    for (double **p : {&H->un, &H->uhn, &H->utn, &H->dun})
       FGPU_CHECK(hipMalloc(p, sizeof(double) * 3 * std::max(1, c.nnode)));
   } catch (const Error &e) {
     c.err = e.msg;
@@ -665,7 +805,7 @@ void fgpu_gnn_destroy(fgpu_gnn *h) {
   GnnHandle *H = reinterpret_cast<GnnHandle *>(h);
   if (!H) return;
   gnn_free(H->g);
-  for (void *p : {(void *)H->lmask, (void *)H->un, (void *)H->utn, (void *)H->dun})
+  for (void *p : {(void *)H->lmask, (void *)H->un, (void *)H->uhn, (void *)H->utn, (void *)H->dun})
     if (p) (void)hipFree(p);
   delete H;
 }
@@ -680,6 +820,14 @@ int fgpu_gnn_set_scales(fgpu_gnn *h, double u_scale, double lam_scale, double ou
     H.g.u_scale = u_scale;
     H.g.lam_scale = lam_scale;
     H.g.out_scale = out_scale;
+    return 1;
+  });
+}
+
+int fgpu_gnn_set_loss(fgpu_gnn *h, double beta) {
+  return gguard(h, [&](GnnHandle &H) {
+    if (!(beta >= 0)) throw Error("beta must be >= 0");
+    H.g.beta = beta;
     return 1;
   });
 }
@@ -699,9 +847,10 @@ int fgpu_gnn_set_params(fgpu_gnn *h, const float *p) {
 }
 
 int fgpu_gnn_loss_grad(fgpu_gnn *h, const double *u_dev, double lam, double dlam,
-                       const double *ut_dev, double *loss, float *grad) {
+                       const double *uh_dev, double lam_h, const double *ut_dev, double *loss,
+                       float *grad) {
   return gguard(h, [&](GnnHandle &H) {
-    double l = sample_loss_grad(H, u_dev, lam, dlam, ut_dev, grad != nullptr);
+    double l = sample_loss_grad(H, u_dev, lam, uh_dev, lam_h, dlam, ut_dev, grad != nullptr);
     if (loss) *loss = l;
     if (grad) {
       copy_sync(*H.g.c, grad, H.g.G, sizeof(float) * H.g.lay.total, hipMemcpyDeviceToHost);
@@ -712,11 +861,13 @@ int fgpu_gnn_loss_grad(fgpu_gnn *h, const double *u_dev, double lam, double dlam
   });
 }
 
-int fgpu_gnn_predict(fgpu_gnn *h, const double *u_dev, double lam, double dlam, double *du_dev) {
+int fgpu_gnn_predict(fgpu_gnn *h, const double *u_dev, double lam, double dlam,
+                     const double *uh_dev, double lam_h, double *du_dev) {
   return gguard(h, [&](GnnHandle &H) {
     Gnn &g = H.g;
     to_nodal(H, u_dev, H.un);
-    gnn_features(g, H.un, lam, dlam, H.lmask);
+    if (uh_dev) to_nodal(H, uh_dev, H.uhn);
+    gnn_features(g, H.un, lam, uh_dev ? H.uhn : nullptr, uh_dev ? lam_h : 0.0, dlam, H.lmask);
     gnn_forward(g);
     gnn_output(g, dlam, H.dun);
     launch_nodal_to_eq(*g.c, H.dun, du_dev);
@@ -726,7 +877,8 @@ int fgpu_gnn_predict(fgpu_gnn *h, const double *u_dev, double lam, double dlam, 
 }
 
 int fgpu_gnn_train(fgpu_gnn *h, int nstate, const double *states_dev, const double *lams,
-                   int nsteps, int maxgap, float lr, unsigned seed, float *loss_hist) {
+                   int nsteps, int maxgap, double origin_frac, float lr, unsigned seed,
+                   float *loss_hist) {
   return gguard(h, [&](GnnHandle &H) {
     if (nstate < 2) throw Error("need at least two states");
     Gnn &g = H.g;
@@ -734,16 +886,25 @@ int fgpu_gnn_train(fgpu_gnn *h, int nstate, const double *states_dev, const doub
     std::mt19937 rng(seed);
     maxgap = std::max(1, std::min(maxgap, nstate - 1));
     for (int s = 0; s < nsteps; s++) {
-      // a pair i < j with a log-uniform gap j - i in [1, maxgap]; state
-      // i = -1 is the unloaded state
-      std::uniform_int_distribution<int> Ui(-1, nstate - 2);
+      // a triple h < i < j with log-uniform gaps i - h, j - i in [1, maxgap];
+      // index -1 is the unloaded state.  With probability origin_frac the
+      // sample is a solve from the unloaded state to a uniformly drawn state
+      std::uniform_int_distribution<int> Ui(0, nstate - 2), Uj(0, nstate - 1);
       std::uniform_real_distribution<double> Ug(0.0, std::log((double)maxgap + 1.0));
-      const int i = Ui(rng);
-      const int gap = std::max(1, std::min(maxgap, (int)std::exp(Ug(rng))));
+      std::uniform_real_distribution<double> U01(0.0, 1.0);
+      const bool from_origin = U01(rng) < origin_frac;
+      const int i = from_origin ? -1 : Ui(rng);
+      const int gap = from_origin ? 1 + Uj(rng)
+                                  : std::max(1, std::min(maxgap, (int)std::exp(Ug(rng))));
+      const int gaph = std::max(1, std::min(maxgap, (int)std::exp(Ug(rng))));
       const int j = std::min(nstate - 1, i + gap);
+      const int hh = std::max(-1, i - gaph);
       const double li = i < 0 ? 0.0 : lams[i], lj = lams[j];
       const double *ui = i < 0 ? nullptr : states_dev + (size_t)i * n;
-      double l = sample_loss_grad(H, ui, li, lj - li, states_dev + (size_t)j * n, true);
+      const double *uh = hh < 0 ? nullptr : states_dev + (size_t)hh * n;
+      const double lh = hh < 0 ? 0.0 : lams[hh];
+      double l = sample_loss_grad(H, ui, li, i < 0 ? nullptr : uh, lh, lj - li,
+                                  states_dev + (size_t)j * n, true);
       // cosine-decayed learning rate
       const float f = 0.5f * (1.f + cosf(3.14159265f * (float)s / (float)nsteps));
       gnn_adam(g, lr * (0.05f + 0.95f * f));
@@ -756,10 +917,11 @@ int fgpu_gnn_train(fgpu_gnn *h, int nstate, const double *states_dev, const doub
 
 // fgpu_predict_fn for fgpu_newton: user = the fgpu_gnn handle
 int fgpu_gnn_newton_predictor(void *user, int step, const double *u_dev, double lam_prev,
-                              double lam, double *du_dev) {
+                              double lam, const double *u_hist_dev, double lam_hist,
+                              double *du_dev) {
   (void)step;
   return fgpu_gnn_predict(reinterpret_cast<fgpu_gnn *>(user), u_dev, lam_prev, lam - lam_prev,
-                          du_dev);
+                          u_hist_dev, lam_hist, du_dev);
 }
 
 }  // extern "C"

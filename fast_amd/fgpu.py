@@ -36,7 +36,7 @@ class FgpuMesh(C.Structure):
 
 
 PREDICT_FN = C.CFUNCTYPE(C.c_int, C.c_void_p, C.c_int, C.c_void_p, C.c_double, C.c_double,
-                         C.c_void_p)
+                         C.c_void_p, C.c_double, C.c_void_p)
 
 
 class NrParams(C.Structure):
@@ -44,7 +44,7 @@ class NrParams(C.Structure):
                 ("tol", C.c_double), ("pcg_maxit", C.c_int), ("pcg_tol", C.c_double),
                 ("adaptive_pcg", C.c_int), ("predict", PREDICT_FN), ("predict_user", C.c_void_p),
                 ("lambdas", C.c_void_p), ("u0_dev", C.c_void_p), ("lam0", C.c_double),
-                ("u_steps_dev", C.c_void_p), ("secant", C.c_int)]
+                ("u_steps_dev", C.c_void_p), ("secant", C.c_int), ("galerkin", C.c_int)]
 
 
 class NrStat(C.Structure):
@@ -88,16 +88,18 @@ EXPORTS = {
     "fgpu_gnn_destroy": (None, [C.c_void_p]),
     "fgpu_gnn_num_params": (C.c_int, [C.c_void_p]),
     "fgpu_gnn_set_scales": (C.c_int, [C.c_void_p, C.c_double, C.c_double, C.c_double]),
+    "fgpu_gnn_set_loss": (C.c_int, [C.c_void_p, C.c_double]),
     "fgpu_gnn_get_params": (C.c_int, [C.c_void_p, C.c_void_p]),
     "fgpu_gnn_set_params": (C.c_int, [C.c_void_p, C.c_void_p]),
     "fgpu_gnn_loss_grad": (C.c_int, [C.c_void_p, C.c_void_p, C.c_double, C.c_double,
-                                     C.c_void_p, C.c_void_p, C.c_void_p]),
+                                     C.c_void_p, C.c_double, C.c_void_p, C.c_void_p,
+                                     C.c_void_p]),
     "fgpu_gnn_predict": (C.c_int, [C.c_void_p, C.c_void_p, C.c_double, C.c_double,
-                                   C.c_void_p]),
+                                   C.c_void_p, C.c_double, C.c_void_p]),
     "fgpu_gnn_train": (C.c_int, [C.c_void_p, C.c_int, C.c_void_p, C.c_void_p, C.c_int, C.c_int,
-                                 C.c_float, C.c_uint, C.c_void_p]),
+                                 C.c_double, C.c_float, C.c_uint, C.c_void_p]),
     "fgpu_gnn_newton_predictor": (C.c_int, [C.c_void_p, C.c_int, C.c_void_p, C.c_double,
-                                            C.c_double, C.c_void_p]),
+                                            C.c_double, C.c_void_p, C.c_double, C.c_void_p]),
 }
 
 
@@ -233,11 +235,11 @@ class GpuModel:
 
     def newton(self, q, steps, factor=0.0, maxit=20, tol=1e-5, pcg_tol=1e-5, pcg_maxit=0,
                adaptive_pcg=0, predict=None, lambdas=None, u0_dev=None, lam0=0.0,
-               u_steps_dev=None, secant=False, keep_host=True):
+               u_steps_dev=None, secant=False, galerkin=False, keep_host=True):
         """cCtrlNR::Solver on the device (ctrl/ctrlnr.cpp:47-267).
 
         ``predict`` is a Python callable ``(step, u_dev_ptr, lam_prev, lam,
-        du_dev_ptr) -> bool`` or a ``Gnn`` (its native predictor is passed
+        u_hist_dev_ptr, lam_hist, du_dev_ptr) -> bool`` or a ``Gnn`` (its native predictor is passed
         without a Python hop).  ``lambdas`` replaces the fixed increment
         ``factor`` by a list of load factors; ``u0_dev``/``lam0`` start from
         a converged state; ``u_steps_dev`` (device, steps x neq) receives the
@@ -247,14 +249,15 @@ class GpuModel:
         prm.steps, prm.factor, prm.maxit, prm.tol = steps, factor, maxit, tol
         prm.pcg_maxit, prm.pcg_tol, prm.adaptive_pcg = pcg_maxit, pcg_tol, int(adaptive_pcg)
         prm.secant = int(secant)
+        prm.galerkin = int(galerkin)
         cb = None
         if isinstance(predict, Gnn):
             prm.predict = C.cast(lib().fgpu_gnn_newton_predictor, PREDICT_FN)
             prm.predict_user = predict.h
         elif predict is not None:
-            def _cb(user, step, u_dev, lam_prev, lam, du_dev):
+            def _cb(user, step, u_dev, lam_prev, lam, uh_dev, lam_h, du_dev):
                 try:
-                    return 1 if predict(step, u_dev, lam_prev, lam, du_dev) else 0
+                    return 1 if predict(step, u_dev, lam_prev, lam, uh_dev, lam_h, du_dev) else 0
                 except Exception as exc:  # never let an exception cross the ABI
                     print("predictor failed:", exc)
                     return 0
@@ -332,6 +335,9 @@ class Gnn:
         self._check(lib().fgpu_gnn_set_scales(self.h, u_scale, lam_scale, out_scale),
                     "fgpu_gnn_set_scales")
 
+    def set_loss(self, beta):
+        self._check(lib().fgpu_gnn_set_loss(self.h, beta), "fgpu_gnn_set_loss")
+
     def params(self):
         p = np.zeros(self.nparam, dtype=np.float32)
         self._check(lib().fgpu_gnn_get_params(self.h, _p(p)), "fgpu_gnn_get_params")
@@ -342,19 +348,20 @@ class Gnn:
         assert p.shape == (self.nparam,)
         self._check(lib().fgpu_gnn_set_params(self.h, _p(p)), "fgpu_gnn_set_params")
 
-    def loss_grad(self, u_dev, lam, dlam, ut_dev, grad=True):
+    def loss_grad(self, u_dev, lam, dlam, ut_dev, uh_dev=None, lam_h=0.0, grad=True):
         loss = C.c_double(0.0)
         g = np.zeros(self.nparam, dtype=np.float32) if grad else None
-        self._check(lib().fgpu_gnn_loss_grad(self.h, u_dev, lam, dlam, ut_dev, C.byref(loss),
-                                             _p(g)), "fgpu_gnn_loss_grad")
+        self._check(lib().fgpu_gnn_loss_grad(self.h, u_dev, lam, dlam, uh_dev, lam_h, ut_dev,
+                                             C.byref(loss), _p(g)), "fgpu_gnn_loss_grad")
         return loss.value, g
 
-    def predict(self, u_dev, lam, dlam, du_dev):
-        self._check(lib().fgpu_gnn_predict(self.h, u_dev, lam, dlam, du_dev), "fgpu_gnn_predict")
+    def predict(self, u_dev, lam, dlam, du_dev, uh_dev=None, lam_h=0.0):
+        self._check(lib().fgpu_gnn_predict(self.h, u_dev, lam, dlam, uh_dev, lam_h, du_dev),
+                    "fgpu_gnn_predict")
 
-    def train(self, states_dev, lams, nsteps, maxgap=4, lr=1e-3, seed=0):
+    def train(self, states_dev, lams, nsteps, maxgap=4, lr=1e-3, seed=0, origin_frac=0.1):
         lams = np.ascontiguousarray(lams, dtype=np.float64)
         hist = np.zeros(nsteps, dtype=np.float32)
         self._check(lib().fgpu_gnn_train(self.h, len(lams), states_dev, _p(lams), nsteps, maxgap,
-                                         lr, seed, _p(hist)), "fgpu_gnn_train")
+                                         origin_frac, lr, seed, _p(hist)), "fgpu_gnn_train")
         return hist

@@ -133,13 +133,16 @@ int fgpu_linear_static(fgpu_ctx *ctx, const double *f, double *u, int maxit,
                        double rtol, int *iters, double *relres);
 
 /* Predictor hook for the Newton driver: given the converged equation vector
- * u_dev of the previous step and the load factors, write a predicted
- * displacement increment into du_dev (neq, device).  Return 1 to use it,
- * 0 to start the step from du = 0.  A prediction d is discarded (the step
- * then starts from du = 0) unless it lowers the potential energy by the
- * trapezoidal estimate d . (r(u) + r(u + d)) > 0, r = f - f_int. */
-typedef int (*fgpu_predict_fn)(void *user, int step, const double *u_dev,
-                               double lam_prev, double lam, double *du_dev);
+ * u_dev of the previous step (load factor lam_prev), the converged state
+ * before it (u_hist_dev at lam_hist; NULL at the first step) and the new load
+ * factor lam, write a predicted displacement increment into du_dev (neq,
+ * device).  Return 1 to use it, 0 to start the step from du = 0.  A
+ * prediction d is discarded (the step then starts from du = 0) unless it
+ * lowers the potential energy by the trapezoidal estimate
+ * d . (r(u) + r(u + d)) > 0, r = f - f_int. */
+typedef int (*fgpu_predict_fn)(void *user, int step, const double *u_dev, double lam_prev,
+                               double lam, const double *u_hist_dev, double lam_hist,
+                               double *du_dev);
 
 typedef struct {
   int steps;          /* MaxStep */
@@ -162,6 +165,10 @@ typedef struct {
   double *u_steps_dev;      /* optional [steps * neq] converged states (device) */
   int secant;               /* 1: without a predictor, extrapolate the last
                                converged increment linearly in lambda */
+  int galerkin;             /* 1: replace a prediction d by the combination
+                               of d and the secant increment that minimises
+                               1/2 x.K x - x.r with the tangent K of the
+                               previous state (two SpMVs) */
 } fgpu_nr_params;
 
 typedef struct {
@@ -197,25 +204,38 @@ void fgpu_gnn_destroy(fgpu_gnn *net);
 int fgpu_gnn_num_params(fgpu_gnn *net);
 /* feature / target normalisation: |u| scale, load-factor scale, output scale */
 int fgpu_gnn_set_scales(fgpu_gnn *net, double u_scale, double lam_scale, double out_scale);
+/* loss = (sum_free e^2 + beta sum_nodes |e - M e|^2) / neq, e the error of
+ * the scaled prediction, M the mean over the neighbours: beta > 0 penalises
+ * rough errors, which raise the residual of a predicted state most
+ * (default 0) */
+int fgpu_gnn_set_loss(fgpu_gnn *net, double beta);
 int fgpu_gnn_get_params(fgpu_gnn *net, float *params);
 int fgpu_gnn_set_params(fgpu_gnn *net, const float *params);
-/* loss and gradient (host) of one sample: state u at lam, target state ut at
- * lam + dlam; u may be NULL (zero state) */
+/* One sample: state u at lam, an earlier state uh at lam_h (NULL: the
+ * unloaded state, lam_h ignored), step dlam; equation vectors on the device,
+ * u may be NULL (the unloaded state, lam = 0).  The prediction is the secant
+ * of the last step (u - uh) / (lam - lam_h) times dlam plus the network's
+ * correction. */
+/* loss and gradient (host) against the target state ut at lam + dlam */
 int fgpu_gnn_loss_grad(fgpu_gnn *net, const double *u_dev, double lam, double dlam,
-                       const double *ut_dev, double *loss, float *grad);
-/* du = predicted increment of a step lam -> lam + dlam from state u */
+                       const double *uh_dev, double lam_h, const double *ut_dev, double *loss,
+                       float *grad);
+/* du = predicted increment of the step lam -> lam + dlam */
 int fgpu_gnn_predict(fgpu_gnn *net, const double *u_dev, double lam, double dlam,
-                     double *du_dev);
+                     const double *uh_dev, double lam_h, double *du_dev);
 /* Adam training on a converged trajectory: states [nstate * neq] (device) at
- * load factors lams[nstate] (host, increasing); each step draws a pair
- * i < j with a log-uniform gap j - i in [1, maxgap] (i = -1: the unloaded
- * state) and takes one Adam step on its loss.  loss_hist (optional, host):
- * the loss of every step. */
+ * load factors lams[nstate] (host, increasing).  Each step draws one sample
+ * and takes one Adam step on its loss: with probability origin_frac a solve
+ * from the unloaded state to a uniformly drawn state, else a triple h < i < j
+ * (history, state, target) with log-uniform gaps in [1, maxgap] (index -1 =
+ * the unloaded state).  loss_hist (optional, host): the loss of every step. */
 int fgpu_gnn_train(fgpu_gnn *net, int nstate, const double *states_dev, const double *lams,
-                   int nsteps, int maxgap, float lr, unsigned seed, float *loss_hist);
+                   int nsteps, int maxgap, double origin_frac, float lr, unsigned seed,
+                   float *loss_hist);
 /* fgpu_predict_fn adapter: pass with predict_user = the fgpu_gnn handle */
 int fgpu_gnn_newton_predictor(void *user, int step, const double *u_dev, double lam_prev,
-                              double lam, double *du_dev);
+                              double lam, const double *u_hist_dev, double lam_hist,
+                              double *du_dev);
 
 /* Time (ms, HIP events on the context stream) of the last call of each
  * phase: 0 = element K, 1 = assembly, 2 = element f_int (+ gather),

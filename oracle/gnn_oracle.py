@@ -8,15 +8,22 @@ its definition in plain numpy:
 
   graph      nodes of the mesh, neighbours = nodes sharing an element, self
              included; mean aggregation over the neighbour list
-  features   16 per node: (x - lo) / extent (3), u / u_scale (3),
-             lam / lam_scale, dlam / lam_scale, free-direction flags (3),
-             load flags (3), 1, 0
+  sample     state u at lam, earlier state u_h at lam_h (the unloaded state
+             when there is none), step dlam
+  features   32 per node: p = (x - lo) / extent (3), u / u_scale (3),
+             s = u / (lam out_scale) (3), v = (u - u_h) / ((lam - lam_h)
+             out_scale) (3), lam / lam_scale, dlam / lam_scale,
+             (lam - lam_h) / lam_scale, free-direction bit mask / 7, load bit
+             mask / 7, 1, then zeros; s and v are 0 at lam = 0
   h0         relu(X W_in + b_in)
   layer l    h <- h + relu(h W_s,l + mean_N(h) W_n,l + b_l),  l = 0..3
-  decoder    d = relu(h W_d1 + b_d1);  y = d W_d2 + b_d2
-  loss       sum over free directions of (y[:, c] - t[:, c])^2 / neq,
+  decoder    d = relu(h W_d1 + b_d1);  y = M M (d W_d2 + b_d2), M the
+             mean over the neighbours (two smoothing passes; M has unit row
+             sums, so this equals (M M d) W_d2 + b_d2, the form used here)
+  loss       (sum over free directions of e^2 + beta sum |e - M e|^2) / neq,
+             e = v + y - t on free directions (0 elsewhere),
              t = (u_target - u) / (dlam out_scale)
-  prediction du = dlam out_scale y[:, :3] on free directions
+  prediction du = dlam out_scale (v + y[:, :3]) on free directions
 
 Parameters are one flat vector; every weight W (fan_in x fan_out) is stored
 row-major, W[k][n] at k * fan_out + n.  The backward pass here is written out
@@ -29,8 +36,10 @@ import numpy as np
 import scipy.sparse as sp
 
 GH = 64
+GFI = 32
 GF = 16
 GL = 4
+GS = 2  # smoothing passes of the decoder
 
 
 def layout():
@@ -43,7 +52,7 @@ def layout():
         L[name] = (o, n)
         o += n
 
-    take("w_in", GF * GH)
+    take("w_in", GFI * GH)
     take("b_in", GH)
     for l in range(GL):
         take("ws%d" % l, GH * GH)
@@ -57,7 +66,7 @@ def layout():
     return L
 
 
-SHAPES = {"w_in": (GF, GH), "w_d1": (GH, GH), "w_d2": (GH, GF)}
+SHAPES = {"w_in": (GFI, GH), "w_d1": (GH, GH), "w_d2": (GH, GF)}
 
 
 def _shape(name):
@@ -111,19 +120,27 @@ def box(coords):
     return lo.astype(np.float32).astype(np.float64), inv.astype(np.float32).astype(np.float64)
 
 
-def features(coords, u, lam, dlam, fmask, lmask, u_scale, lam_scale):
-    """coords, u: [nnode, 3]; fmask/lmask: per-node direction bit masks."""
+SKIP = 9  # feature offset of v
+
+
+def features(coords, u, lam, dlam, uh, lamh, fmask, lmask, u_scale, lam_scale, out_scale):
+    """coords, u, uh: [nnode, 3] (uh None: the unloaded state); fmask/lmask:
+    per-node direction bit masks."""
+    if uh is None:
+        uh, lamh = np.zeros_like(u), 0.0
     lo, inv = box(coords)
     n = coords.shape[0]
-    X = np.zeros((n, GF))
+    X = np.zeros((n, GFI))
     X[:, 0:3] = (coords - lo) * inv
     X[:, 3:6] = u / u_scale
-    X[:, 6] = lam / lam_scale
-    X[:, 7] = dlam / lam_scale
-    for d in range(3):
-        X[:, 8 + d] = (fmask >> d) & 1
-        X[:, 11 + d] = (lmask >> d) & 1
-    X[:, 14] = 1.0
+    X[:, 6:9] = u / (lam * out_scale) if lam != 0 else 0.0
+    X[:, 9:12] = (u - uh) / ((lam - lamh) * out_scale) if lam > lamh else 0.0
+    X[:, 12] = lam / lam_scale
+    X[:, 13] = dlam / lam_scale
+    X[:, 14] = (lam - lamh) / lam_scale
+    X[:, 15] = fmask / 7.0
+    X[:, 16] = lmask / 7.0
+    X[:, 17] = 1.0
     return X
 
 
@@ -144,8 +161,12 @@ def forward(P, X, M):
         cache["a"].append(a)
         cache["h"].append(h)
     d1 = relu(h @ W["w_d1"] + W["b_d1"])
-    y = d1 @ W["w_d2"] + W["b_d2"]
+    e = d1
+    for _ in range(GS):
+        e = M @ e
+    y = e @ W["w_d2"] + W["b_d2"]
     cache["d1"] = d1
+    cache["e"] = e
     return y, cache
 
 
@@ -157,12 +178,22 @@ def target(u, ut, dlam, out_scale, fmask):
     return T
 
 
-def loss(y, T, fmask, neq):
+def loss(y, X, T, fmask, neq, M=None, beta=0.0):
+    """(sum_free e^2 + beta sum_nodes |e - M e|^2) / neq and d loss / d y."""
     e = np.zeros_like(y)
     for c in range(3):
         free = ((fmask >> c) & 1).astype(bool)
-        e[free, c] = y[free, c] - T[free, c]
-    return float((e * e).sum() / neq), 2.0 * e / neq
+        e[free, c] = y[free, c] + X[free, SKIP + c] - T[free, c]
+    val = (e * e).sum()
+    dy = 2.0 * e
+    if beta:
+        d = e - M @ e
+        val += beta * (d * d).sum()
+        g = 2.0 * beta * (d - M.T @ d)
+        for c in range(3):
+            free = ((fmask >> c) & 1).astype(bool)
+            dy[free, c] += g[free, c]
+    return float(val / neq), dy / neq
 
 
 def backward(P, M, cache, dy):
@@ -171,9 +202,12 @@ def backward(P, M, cache, dy):
     G = {}
     d1 = cache["d1"]
     h = cache["h"]
-    G["w_d2"] = d1.T @ dy
+    G["w_d2"] = cache["e"].T @ dy
     G["b_d2"] = dy.sum(axis=0)
-    dz = (dy @ W["w_d2"].T) * (d1 > 0)
+    de = dy @ W["w_d2"].T
+    for _ in range(GS):
+        de = M.T @ de
+    dz = de * (d1 > 0)
     G["w_d1"] = h[GL].T @ dz
     G["b_d1"] = dz.sum(axis=0)
     dh = dz @ W["w_d1"].T
@@ -189,9 +223,9 @@ def backward(P, M, cache, dy):
     return pack(G)
 
 
-def loss_grad(P, X, M, T, fmask, neq):
+def loss_grad(P, X, M, T, fmask, neq, beta=0.0):
     y, cache = forward(P, X, M)
-    l, dy = loss(y, T, fmask, neq)
+    l, dy = loss(y, X, T, fmask, neq, M, beta)
     return l, backward(P, M, cache, dy), y
 
 
@@ -200,5 +234,5 @@ def predict(P, X, M, dlam, out_scale, fmask):
     du = np.zeros((X.shape[0], 3))
     for c in range(3):
         free = ((fmask >> c) & 1).astype(bool)
-        du[free, c] = dlam * out_scale * y[free, c]
+        du[free, c] = dlam * out_scale * (y[free, c] + X[free, SKIP + c])
     return du

@@ -65,27 +65,35 @@ def fields(m, seed):
 SC = (0.05, 0.3, 0.2)  # u_scale, lam_scale, out_scale
 
 
-@pytest.mark.parametrize("zero_state", [False, True])
-def test_loss_and_gradient_match_oracle(fg, zero_state):
+@pytest.mark.parametrize("case,beta", [("history", 0.0), ("no_history", 0.0), ("unloaded", 0.0),
+                                       ("history", 20.0)])
+def test_loss_and_gradient_match_oracle(fg, case, beta):
     import torch
     m = cube(6)
     fmask, lmask = masks(m)
     model = fg.GpuModel(m)
     net = fg.Gnn(model, lmask, seed=3)
     net.set_scales(*SC)
+    net.set_loss(beta)
     u, ut = fields(m, 0)
-    if zero_state:
+    uh = 0.7 * u
+    lam, dlam, lamh = 0.12, 0.02, 0.08
+    if case == "unloaded":
         u[:] = 0
-    lam, dlam = 0.12, 0.02
+        lam = 0.0
     du = torch.from_numpy(to_eq(m, u)).cuda()
     dut = torch.from_numpy(to_eq(m, ut)).cuda()
+    duh = torch.from_numpy(to_eq(m, uh)).cuda()
     torch.cuda.synchronize()
-    loss, grad = net.loss_grad(None if zero_state else du.data_ptr(), lam, dlam, dut.data_ptr())
+    hist = case == "history"
+    loss, grad = net.loss_grad(None if case == "unloaded" else du.data_ptr(), lam, dlam,
+                               dut.data_ptr(), duh.data_ptr() if hist else None, lamh)
     P = net.params().astype(np.float64)
-    X = G.features(m.coords, u, lam, dlam, fmask, lmask, SC[0], SC[1])
+    X = G.features(m.coords, u, lam, dlam, uh if hist else None, lamh, fmask, lmask,
+                   SC[0], SC[1], SC[2])
     Mg = G.graph(m.conn, m.nnode)
     T = G.target(u, ut, dlam, SC[2], fmask)
-    l_ref, g_ref, _ = G.loss_grad(P, X, Mg, T, fmask, m.neq)
+    l_ref, g_ref, _ = G.loss_grad(P, X, Mg, T, fmask, m.neq, beta)
     assert abs(loss - l_ref) <= 2e-2 * l_ref, (loss, l_ref)
     assert nrel(grad.astype(np.float64), g_ref) <= 2e-2, nrel(grad, g_ref)
     # every parameter block receives its gradient (a dropped term shows here)
@@ -109,12 +117,14 @@ def test_prediction_matches_oracle(fg):
     net.set_params(P)
     assert np.array_equal(net.params(), P)
     u, _ = fields(m, 1)
-    lam, dlam = 0.2, 0.03
+    uh = 0.8 * u
+    lam, dlam, lamh = 0.2, 0.03, 0.16
     du = torch.from_numpy(to_eq(m, u)).cuda()
+    duh = torch.from_numpy(to_eq(m, uh)).cuda()
     out = torch.zeros(m.neq, dtype=torch.float64, device="cuda")
     torch.cuda.synchronize()
-    net.predict(du.data_ptr(), lam, dlam, out.data_ptr())
-    X = G.features(m.coords, u, lam, dlam, fmask, lmask, SC[0], SC[1])
+    net.predict(du.data_ptr(), lam, dlam, out.data_ptr(), duh.data_ptr(), lamh)
+    X = G.features(m.coords, u, lam, dlam, uh, lamh, fmask, lmask, SC[0], SC[1], SC[2])
     ref = G.predict(P.astype(np.float64), X, G.graph(m.conn, m.nnode), dlam, SC[2], fmask)
     assert nrel(out.cpu().numpy(), to_eq(m, ref)) <= 2e-2
 
@@ -184,7 +194,7 @@ def test_newton_driver_options(fg):
     torch.cuda.synchronize()
     L = fg.lib()
 
-    def bad(step, u_dev, lam_prev, lam, du_dev):  # du = K 1: far from equilibrium
+    def bad(step, u_dev, lam_prev, lam, uh_dev, lam_h, du_dev):  # du = K 1: far off
         return bool(L.fgpu_spmv_dev(model.h, ones.data_ptr(), du_dev))
 
     uE, sE, eE = model.newton(q, steps=4, factor=0.05, predict=bad, **base)
