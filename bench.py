@@ -7,6 +7,13 @@ inputs resident in HBM.  Default workload: cubic B-spline solid patch
 seeded U(-0.05, 0.05) h), 4x4x4 Gauss, E = 1, nu = 0.3.
 ``--workload hex8`` runs the 64^3 neo-Hookean hex8 block instead.
 
+``--workload warmstart`` runs the learned warm start on the same 64^3
+neo-Hookean block: solver-generated training data (2000 load levels
+U(0, 0.3) of a unit traction, seed 0, solved by continuation), device training
+of the graph network, then one load path (10 equal steps to 0.3) solved to
+|r| / max(1, |f|) <= 1e-8 plain, with the secant predictor and with the
+network; the value is the wall-time ratio network / plain.
+
 Prints ONE JSON line (rank 0).  Multi-GPU runs are not part of this build.
 """
 from __future__ import annotations
@@ -111,12 +118,92 @@ def accuracy(model, m, u, nsample=32):
     return float(np.sqrt(num / den)), float(worst_fw)
 
 
+def run_warmstart(args):
+    """Learned warm start: data generation, training, timed solves."""
+    import torch
+    from fast_amd import mesh as M, fgpu
+    n = 64
+    m = M.hex8_block(n, n, n, form="tl_neohooke", mat=np.tile((1.0, 1.5), (n ** 3, 1)),
+                     traction=(0.0, 0.0, -1.0))
+    model = fgpu.GpuModel(m)
+    q = m.load_vector()
+    bits = np.array([1, 2, 4], dtype=np.uint8)
+    lmask = ((m.fext != 0).astype(np.uint8) * bits).sum(1).astype(np.uint8)
+    lams = np.sort(np.random.default_rng(0).uniform(0.0, 0.3, args.nstates))
+    print("bench: warm start, neq %d, generating %d states" % (m.neq, args.nstates),
+          file=sys.stderr, flush=True)
+    states = torch.zeros((args.nstates, m.neq), dtype=torch.float64, device="cuda")
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    chunk = 250  # continuation in chunks, restarting from the last state
+    for c0 in range(0, args.nstates, chunk):
+        c1 = min(args.nstates, c0 + chunk)
+        restart = {} if c0 == 0 else {"u0_dev": states[c0 - 1].data_ptr(), "lam0": lams[c0 - 1]}
+        _, st, err = model.newton(q, steps=c1 - c0, lambdas=lams[c0:c1], maxit=20, tol=1e-8,
+                                  pcg_tol=1.0, adaptive_pcg=2, secant=True,
+                                  u_steps_dev=states[c0].data_ptr(), keep_host=False, **restart)
+        if err:
+            raise RuntimeError("data generation: " + err)
+        print("bench: states %d/%d (%.1f s)" % (c1, args.nstates, time.perf_counter() - t0),
+              file=sys.stderr, flush=True)
+    t_data = time.perf_counter() - t0
+    umax = float(states[-1].abs().max())
+    net = fgpu.Gnn(model, lmask, seed=0)
+    net.set_scales(umax, 0.3, umax / 0.3)
+    net.set_loss(args.beta)
+    print("bench: data %.1f s; training %d steps" % (t_data, args.train_steps), file=sys.stderr,
+          flush=True)
+    os.environ["FGPU_GNN_VERBOSE"] = "1"
+    t0 = time.perf_counter()
+    hist = net.train(states.data_ptr(), lams, nsteps=args.train_steps, maxgap=400, lr=1e-3,
+                     seed=0, origin_frac=0.1)
+    t_train = time.perf_counter() - t0
+    del states
+    torch.cuda.empty_cache()
+    kw = dict(steps=10, factor=0.03, maxit=30, tol=1e-8, pcg_tol=1.0, adaptive_pcg=2)
+    res = {}
+    for rep in range(2):  # the first pass warms up; the second is timed
+        for name, extra in (("plain", {}), ("secant", {"secant": True}), ("network", {"predict": net})):
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            _, s, e = model.newton(q, keep_host=False, **kw, **extra)
+            wall = time.perf_counter() - t0
+            if e:
+                raise RuntimeError("%s: %s" % (name, e))
+            res[name] = {"wall_s": round(wall, 4), "newton_iters": sum(x["iters"] for x in s),
+                         "pcg_iters": sum(x["pcg_iters"] for x in s),
+                         "predictions_used": sum(x["predicted"] == 1 for x in s)}
+            print("bench: %s %s" % (name, res[name]), file=sys.stderr, flush=True)
+    ratio = res["network"]["wall_s"] / res["plain"]["wall_s"]
+    k = max(1, len(hist) // 10)
+    out = {
+        "metric": "solve wall time to 1e-8, NN-warm / plain (10 load steps to 0.3)",
+        "value": round(ratio, 4), "unit": "ratio", "n_gpus": 1, "steps": 10, "warmup": 10,
+        "ms_per_step": round(res["network"]["wall_s"] * 100, 2), "higher_is_better": False,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f64 solve / fp16-mfma network",
+        "data": "synthetic (seeded generated mesh and load levels)",
+        "config": {"workload": "warmstart_hex8_neohooke_64x64x64", "neq": int(m.neq),
+                   "train_states": args.nstates, "train_steps": args.train_steps,
+                   "loss_beta": args.beta, "network": "4 message-passing layers, hidden 64"},
+        "warm_start": {"solves": res, "secant_over_plain": round(res["secant"]["wall_s"] /
+                                                                 res["plain"]["wall_s"], 4),
+                       "datagen_s": round(t_data, 2), "train_s": round(t_train, 2),
+                       "train_ms_per_step": round(1e3 * t_train / args.train_steps, 3),
+                       "loss_curve": [float("%.3e" % hist[i:i + k].mean())
+                                      for i in range(0, len(hist), k)]},
+    }
+    print(json.dumps(out))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--workload", default="iga", choices=["iga", "hex8", "batch"])
+    ap.add_argument("--workload", default="iga", choices=["iga", "hex8", "batch", "warmstart"])
+    ap.add_argument("--nstates", type=int, default=2000, help="warmstart: training states")
+    ap.add_argument("--train-steps", type=int, default=4000, help="warmstart: Adam steps")
+    ap.add_argument("--beta", type=float, default=30.0, help="warmstart: roughness weight")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--solve", action="store_true", help="also time a plain PCG solve to 1e-8")
     args = ap.parse_args()
@@ -129,6 +216,9 @@ def main():
     assert torch.cuda.is_available(), "bench.py needs a HIP device"
     torch.cuda.set_device(0)
     from fast_amd import fgpu
+    if args.workload == "warmstart":
+        run_warmstart(args)
+        return
 
     m, cfg, u = make_workload(args.workload)
     print("bench: workload %s nelem %d neq %d" % (cfg["workload"], m.nelem, m.neq),

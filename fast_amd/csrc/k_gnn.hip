@@ -28,6 +28,8 @@
 // nodes, divided by neq.  Every reduction (weight gradients, bias gradients,
 // loss) is a block-partial sum added in a fixed order.
 #include <cmath>
+#include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <random>
 #include <vector>
@@ -885,6 +887,8 @@ int fgpu_gnn_train(fgpu_gnn *h, int nstate, const double *states_dev, const doub
     const size_t n = (size_t)g.c->neq;
     std::mt19937 rng(seed);
     maxgap = std::max(1, std::min(maxgap, nstate - 1));
+    const char *venv = std::getenv("FGPU_GNN_VERBOSE");  // progress log on stderr
+    const bool verbose = venv && venv[0] == '1';
     for (int s = 0; s < nsteps; s++) {
       // a triple h < i < j with log-uniform gaps i - h, j - i in [1, maxgap];
       // index -1 is the unloaded state.  With probability origin_frac the
@@ -909,6 +913,8 @@ int fgpu_gnn_train(fgpu_gnn *h, int nstate, const double *states_dev, const doub
       const float f = 0.5f * (1.f + cosf(3.14159265f * (float)s / (float)nsteps));
       gnn_adam(g, lr * (0.05f + 0.95f * f));
       if (loss_hist) loss_hist[s] = (float)l;
+      if (verbose && (s + 1) % 250 == 0)
+        std::fprintf(stderr, "fgpu_gnn_train: step %d/%d loss %.3e\n", s + 1, nsteps, l);
     }
     FGPU_CHECK(hipStreamSynchronize(g.c->stream));
     return 1;
