@@ -195,12 +195,38 @@ def run_warmstart(args):
     print(json.dumps(out))
 
 
+def run_copy(args):
+    """Device-to-device copy of a 1 GiB FP64 buffer: the practical HBM
+    bandwidth, and the calibration of the FETCH_SIZE / WRITE_SIZE counters
+    (each copy reads and writes exactly 1 GiB)."""
+    import torch
+    n = (1 << 30) // 8
+    x = torch.ones(n, dtype=torch.float64, device="cuda")
+    y = torch.empty_like(x)
+    for _ in range(args.warmup):
+        y.copy_(x)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        y.copy_(x)
+    torch.cuda.synchronize()
+    dt = (time.perf_counter() - t0) / args.steps
+    gbs = 2 * (1 << 30) / dt / 1e9
+    print(json.dumps({"metric": "device copy bandwidth (read + write)", "value": round(gbs, 1),
+                      "unit": "GB/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+                      "ms_per_step": round(dt * 1e3, 4), "higher_is_better": True,
+                      "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+                      "data": "synthetic", "config": {"workload": "copy_1GiB"},
+                      "bytes_per_copy": 2 * (1 << 30), "frac_of_peak": round(gbs / PEAK_HBM, 4)}))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--workload", default="iga", choices=["iga", "hex8", "batch", "warmstart"])
+    ap.add_argument("--workload", default="iga",
+                    choices=["iga", "hex8", "batch", "warmstart", "copy"])
     ap.add_argument("--nstates", type=int, default=2000, help="warmstart: training states")
     ap.add_argument("--train-steps", type=int, default=4000, help="warmstart: Adam steps")
     ap.add_argument("--beta", type=float, default=30.0, help="warmstart: roughness weight")
@@ -218,6 +244,9 @@ def main():
     from fast_amd import fgpu
     if args.workload == "warmstart":
         run_warmstart(args)
+        return
+    if args.workload == "copy":
+        run_copy(args)
         return
 
     m, cfg, u = make_workload(args.workload)

@@ -13,6 +13,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <numeric>
 #include <thread>
@@ -287,7 +288,8 @@ void destroy(Ctx *c) {
                   c->d_status, c->d_rowptr, c->d_col, c->d_bptr, c->d_bcol,
                   c->d_bcoloff, c->d_bdiag, c->d_fmask, c->d_bval, c->d_bn,
                   c->d_blkA, c->d_blkB, c->d_offAB, c->d_offBA, c->d_cptr,
-                  c->d_contrib, c->d_nptr, c->d_nadj, c->d_u, c->d_b, c->d_x,
+                  c->d_contrib, c->d_nptr, c->d_nadj, c->d_u,
+                  c->d_b, c->d_x,
                   c->d_r, c->d_p, c->d_w, c->d_z, c->d_dinv, c->d_part,
                   c->d_scal, c->d_done, c->d_eqnode};
   for (void *p : bufs)
@@ -347,8 +349,58 @@ int check_status(Ctx &c, const char *what) {
 
 namespace fgpu {
 
+// Autotuner: times every valid launch configuration of the element-stiffness
+// and assembly kernels on this mesh and state (best of 3 runs each, HIP
+// events) and keeps the fastest.  All configurations give identical bits.
+void autotune(Ctx &c, const double *d_u) {
+  c.tuned = true;
+  const char *env = std::getenv("FGPU_AUTOTUNE");
+  if (env && env[0] == '0') return;
+  std::string rep;
+  auto time3 = [&](auto &&launch) {
+    float best = 1e30f;
+    for (int r = 0; r < 3; r++) {
+      FGPU_CHECK(hipEventRecord(c.ev[0], c.stream));
+      launch();
+      FGPU_CHECK(hipEventRecord(c.ev[1], c.stream));
+      FGPU_CHECK(hipEventSynchronize(c.ev[1]));
+      best = std::min(best, (float)elapsed(c, 0, 1));
+    }
+    return best;
+  };
+  char buf[96];
+  if (c.shape == FGPU_HEX8) {
+    float best = 1e30f;
+    int pick = 0;
+    for (int te : hex8_te_options(c)) {
+      c.hex8_te = te;
+      float ms = time3([&] { launch_hex8_K(c, d_u); });
+      std::snprintf(buf, sizeof buf, "hex8_K elements/workgroup %d: %.4f ms\n", te, ms);
+      rep += buf;
+      if (ms < best) best = ms, pick = te;
+    }
+    c.hex8_te = pick;
+    launch_hex8_K(c, d_u);
+  } else {
+    launch_bsp_K(c, d_u);
+  }
+  float best = 1e30f;
+  int pick = 0;
+  for (int mode : assembly_options(c)) {
+    c.asm_mode = mode;
+    float ms = time3([&] { launch_assemble_K(c); });
+    std::snprintf(buf, sizeof buf, "assembly mode %d (loads in batches of %d): %.4f ms\n", mode,
+                  mode == 2 ? 8 : 1, ms);
+    rep += buf;
+    if (ms < best) best = ms, pick = mode;
+  }
+  c.asm_mode = pick;
+  c.tune_report = rep;
+}
+
 int stiffness_dev(Ctx &c, const double *d_u) {
   FGPU_CHECK(hipMemsetAsync(c.d_status, 0, 2 * sizeof(int), c.stream));
+  if (!c.tuned) autotune(c, d_u);
   FGPU_CHECK(hipEventRecord(c.ev[0], c.stream));
   if (c.shape == FGPU_HEX8)
     launch_hex8_K(c, d_u);
@@ -668,6 +720,32 @@ int fgpu_get_element_matrices(fgpu_ctx *ctx, int e0, int ne, float *Ke) {
     (void)hipFree(d);
     return 1;
   });
+}
+
+int fgpu_set_launch(fgpu_ctx *ctx, int hex8_te, int asm_mode) {
+  return guarded(ctx, [&](Ctx &c) {
+    if (hex8_te != 0) {
+      const std::vector<int> ok = hex8_te_options(c);
+      if (c.shape != FGPU_HEX8 || std::find(ok.begin(), ok.end(), hex8_te) == ok.end())
+        throw Error("elements per workgroup " + std::to_string(hex8_te) +
+                    " not available for this mesh");
+    }
+    if (asm_mode != -1) {
+      const std::vector<int> ok = assembly_options(c);
+      if (std::find(ok.begin(), ok.end(), asm_mode) == ok.end())
+        throw Error("assembly mode " + std::to_string(asm_mode) + " not available for this mesh");
+    }
+    c.hex8_te = hex8_te;
+    c.asm_mode = asm_mode;
+    c.tuned = true;  // an explicit choice replaces the autotuner
+    c.tune_report = "set by the caller\n";
+    return 1;
+  });
+}
+
+const char *fgpu_tuning_report(fgpu_ctx *ctx) {
+  Ctx *c = reinterpret_cast<Ctx *>(ctx);
+  return c ? c->tune_report.c_str() : "";
 }
 
 double fgpu_last_time_ms(fgpu_ctx *ctx, int phase) {

@@ -98,6 +98,13 @@ struct Ctx {
   int *d_nptr = nullptr;         // [nnode+1]
   int *d_nadj = nullptr;         // [nelem*nn] e * nn + a
 
+  // ---- launch configuration (picked by the autotuner, identical results)
+  int hex8_te = 0;       // elements per workgroup of the hex8 K kernel (0: largest)
+  int asm_mode = -1;     // assembly: contribution-list loads in batches of 1 (0)
+                         // or 8 (2); -1 default (2)
+  bool tuned = false;
+  std::string tune_report;
+
   // ---- solver workspace
   double *d_u = nullptr;      // nodal vector scratch [nnode*3]
   // PCG vectors on nodes [nnode*3]; d_b is an equation-vector scratch
@@ -134,6 +141,8 @@ void launch_hex8_fint(Ctx &c, const double *d_unodal);
 void launch_bsp_K(Ctx &c, const double *d_unodal);
 void launch_bsp_fint(Ctx &c, const double *d_unodal);
 void launch_assemble_K(Ctx &c);
+std::vector<int> hex8_te_options(const Ctx &c);     // valid hex8 K workgroup sizes
+std::vector<int> assembly_options(const Ctx &c);    // valid asm_mode values
 void launch_gather_fint(Ctx &c, double *d_g);
 void launch_spmv_nodal(Ctx &c, const double *x, double *y);  // node vectors
 void launch_csr_convert(Ctx &c, double *d_csrval, int to_csr);

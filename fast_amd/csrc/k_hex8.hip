@@ -384,19 +384,45 @@ __global__ __launch_bounds__(128) void hex8_fint_kernel(
     for (int i = 0; i < 3; i++) out[3 * a + i] = g[a][i];
 }
 
-template <int FORM, int NIP>
-void launch_K_t(Ctx &c, const double *u) {
+template <int FORM, int NIP, int TE>
+void launch_K_te(Ctx &c, const double *u) {
   constexpr int NQ = NIP * NIP * NIP;
   constexpr int NF = Stage<FORM>::NF;
-  // elements per workgroup: largest divisor of 64 keeping LDS <= 64 KB
-  constexpr int TE0 = (65536 / (NQ * NF * 4));
-  constexpr int TE = TE0 >= 64 ? 64 : TE0 >= 32 ? 32 : TE0 >= 16 ? 16 : TE0 >= 8 ? 8 : 4;
   size_t lds = (size_t)NQ * NF * TE * sizeof(float);
   int grid = (c.nelem + TE - 1) / TE;
   hipLaunchKernelGGL((hex8_K_kernel<FORM, NIP, TE>), dim3(grid), dim3(256), lds, c.stream,
                      c.nelem, c.d_coords, c.d_conn, c.d_mat, u, c.d_gauss, c.d_Ke, c.kel,
                      c.d_status);
   FGPU_CHECK(hipGetLastError());
+}
+
+// largest number of elements per workgroup (a divisor of the 64-element
+// storage tile) whose staged point data fit in 64 KB of LDS
+constexpr int te_max(int nq, int nf) {
+  return (65536 / (nq * nf * 4)) >= 64   ? 64
+         : (65536 / (nq * nf * 4)) >= 32 ? 32
+         : (65536 / (nq * nf * 4)) >= 16 ? 16
+         : (65536 / (nq * nf * 4)) >= 8  ? 8
+                                         : 4;
+}
+
+template <int FORM, int NIP>
+void launch_K_t(Ctx &c, const double *u) {
+  constexpr int TMAX = te_max(NIP * NIP * NIP, Stage<FORM>::NF);
+  const int te = c.hex8_te > 0 && c.hex8_te <= TMAX ? c.hex8_te : TMAX;
+  switch (te) {
+    case 4: launch_K_te<FORM, NIP, 4>(c, u); break;
+    case 8: if constexpr (TMAX >= 8) launch_K_te<FORM, NIP, 8>(c, u); break;
+    case 16: if constexpr (TMAX >= 16) launch_K_te<FORM, NIP, 16>(c, u); break;
+    case 32: if constexpr (TMAX >= 32) launch_K_te<FORM, NIP, 32>(c, u); break;
+    default: if constexpr (TMAX >= 64) launch_K_te<FORM, NIP, 64>(c, u); break;
+  }
+}
+
+template <int FORM>
+int te_limit(int nip) {
+  return nip == 1 ? te_max(1, Stage<FORM>::NF)
+                  : nip == 2 ? te_max(8, Stage<FORM>::NF) : te_max(27, Stage<FORM>::NF);
 }
 
 template <int FORM>
@@ -439,6 +465,15 @@ void launch_hex8_K(Ctx &c, const double *u) {
     case FGPU_TL_SVK: launch_K_f<FGPU_TL_SVK>(c, u); break;
     default: launch_K_f<FGPU_TL_NEOHOOKE>(c, u); break;
   }
+}
+
+std::vector<int> hex8_te_options(const Ctx &c) {
+  int tmax = c.form == FGPU_LINEAR   ? te_limit<FGPU_LINEAR>(c.nip[0])
+             : c.form == FGPU_TL_SVK ? te_limit<FGPU_TL_SVK>(c.nip[0])
+                                     : te_limit<FGPU_TL_NEOHOOKE>(c.nip[0]);
+  std::vector<int> out;
+  for (int te = 4; te <= tmax; te *= 2) out.push_back(te);
+  return out;
 }
 
 void launch_hex8_fint(Ctx &c, const double *u) {

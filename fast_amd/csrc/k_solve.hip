@@ -25,6 +25,10 @@
 namespace fgpu {
 namespace {
 
+// One thread per upper block (A, B): sum its contributions (sorted by
+// element), NB at a time: the NB codes are read, then all NB blocks are
+// loaded, then added in order (the loads of a batch are in flight together).
+template <int NB>
 __global__ __launch_bounds__(256) void assemble_kernel(
     long long nblk, const int *__restrict__ blkA, const int *__restrict__ blkB,
     const int *__restrict__ posAB, const int *__restrict__ posBA,
@@ -35,20 +39,30 @@ __global__ __launch_bounds__(256) void assemble_kernel(
   if (k >= nblk) return;
   double s[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
   const long long c0 = cptr[k], c1 = cptr[k + 1];
-#pragma unroll 4
-  for (long long t = c0; t < c1; t++) {
-    const int code = contrib[t];
-    const int x = code >> 1;
-    const long e = x / L.nsym;
-    const int sb = x - (int)e * L.nsym;
-    const float *p = Ke + L.offset(e, sb);
-    float v[9];
-    for (int m = 0; m < 9; m++) v[m] = p[m];
-    if (code & 1) {
-      for (int i = 0; i < 3; i++)
-        for (int j = 0; j < 3; j++) s[3 * i + j] += (double)v[3 * j + i];
-    } else {
-      for (int m = 0; m < 9; m++) s[m] += (double)v[m];
+  for (long long tb = c0; tb < c1; tb += NB) {
+    int code[NB];
+#pragma unroll
+    for (int u = 0; u < NB; u++) code[u] = tb + u < c1 ? contrib[tb + u] : -1;
+    float v[NB][9];
+#pragma unroll
+    for (int u = 0; u < NB; u++) {
+      if (code[u] < 0) continue;
+      const int x = code[u] >> 1;
+      const long e = x / L.nsym;
+      const int sb = x - (int)e * L.nsym;
+      const float *p = Ke + L.offset(e, sb);
+#pragma unroll
+      for (int m = 0; m < 9; m++) v[u][m] = p[m];
+    }
+#pragma unroll
+    for (int u = 0; u < NB; u++) {
+      if (code[u] < 0) continue;
+      if (code[u] & 1) {
+        for (int i = 0; i < 3; i++)
+          for (int j = 0; j < 3; j++) s[3 * i + j] += (double)v[u][3 * j + i];
+      } else {
+        for (int m = 0; m < 9; m++) s[m] += (double)v[u][m];
+      }
     }
   }
   const int A = blkA[k], B = blkB[k];
@@ -227,11 +241,22 @@ void bspmv_launch(Ctx &c, const double *x, double *y) {
 
 }  // namespace
 
-void launch_assemble_K(Ctx &c) {
-  int grid = (int)((c.nblk + 255) / 256);
-  hipLaunchKernelGGL(assemble_kernel, dim3(grid), dim3(256), 0, c.stream, c.nblk, c.d_blkA,
+// assembly modes: 0, 2: contribution lists with loads in batches of 1, 8
+std::vector<int> assembly_options(const Ctx &) { return {0, 2}; }
+
+template <int NB>
+void launch_contrib(Ctx &c, int grid) {
+  hipLaunchKernelGGL((assemble_kernel<NB>), dim3(grid), dim3(256), 0, c.stream, c.nblk, c.d_blkA,
                      c.d_blkB, c.d_offAB, c.d_offBA, c.d_cptr, c.d_contrib, c.d_Ke, c.kel,
                      c.d_fmask, c.d_bval);
+}
+
+void launch_assemble_K(Ctx &c) {
+  const int grid = (int)((c.nblk + 255) / 256);
+  if (c.asm_mode == 0)
+    launch_contrib<1>(c, grid);
+  else
+    launch_contrib<8>(c, grid);
   FGPU_CHECK(hipGetLastError());
 }
 

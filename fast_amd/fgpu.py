@@ -82,6 +82,8 @@ EXPORTS = {
     "fgpu_get_csr": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),
     "fgpu_set_csr_values": (C.c_int, [C.c_void_p, C.c_void_p]),
     "fgpu_get_element_matrices": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_void_p]),
+    "fgpu_set_launch": (C.c_int, [C.c_void_p, C.c_int, C.c_int]),
+    "fgpu_tuning_report": (C.c_char_p, [C.c_void_p]),
     "fgpu_last_time_ms": (C.c_double, [C.c_void_p, C.c_int]),
     "fgpu_device_buffer": (C.c_void_p, [C.c_void_p, C.c_int]),
     "fgpu_gnn_create": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint]),
@@ -291,6 +293,13 @@ class GpuModel:
         self._check(lib().fgpu_get_element_matrices(self.h, e0, ne, _p(K)),
                     "fgpu_get_element_matrices")
         return K
+
+    def set_launch(self, hex8_te=0, asm_mode=-1):
+        """Fix the launch configuration (replaces the autotuner)."""
+        self._check(lib().fgpu_set_launch(self.h, hex8_te, asm_mode), "fgpu_set_launch")
+
+    def tuning_report(self):
+        return lib().fgpu_tuning_report(self.h).decode()
 
     def last_times(self):
         return [lib().fgpu_last_time_ms(self.h, k) for k in range(4)]

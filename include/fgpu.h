@@ -237,6 +237,18 @@ int fgpu_gnn_newton_predictor(void *user, int step, const double *u_dev, double 
                               double lam, const double *u_hist_dev, double lam_hist,
                               double *du_dev);
 
+/* Launch configuration.  On the first stiffness assembly of a context the
+ * autotuner times every valid configuration of the element-stiffness kernel
+ * (hex8: elements per workgroup 4..64) and of the assembly (contribution
+ * loads in batches of 1, mode 0, or 8, mode 2)
+ * on the actual mesh and state, keeps the fastest and records the timings
+ * (environment FGPU_AUTOTUNE=0 skips it).  All configurations give identical
+ * results, bit for bit.  fgpu_set_launch fixes a configuration instead
+ * (hex8_te 0: the largest; asm_mode -1: the default); 0 if it is not valid
+ * for the mesh.  No counterpart in the reference (a CPU loop). */
+int fgpu_set_launch(fgpu_ctx *ctx, int hex8_te, int asm_mode);
+const char *fgpu_tuning_report(fgpu_ctx *ctx);
+
 /* Time (ms, HIP events on the context stream) of the last call of each
  * phase: 0 = element K, 1 = assembly, 2 = element f_int (+ gather),
  * 3 = solve. */

@@ -280,3 +280,40 @@ def test_linear_static_driver(fg, oracle_mod):
     g = fg.GpuModel(m)
     u, it, rr = g.linear_static(m.load_vector(), tol=1e-10)
     assert nrel(m.scatter(u), d["u"]) <= 1e-5
+
+
+@pytest.mark.parametrize("kind", ["hex8_nh", "bsp"])
+def test_launch_configurations_identical_bits(fg, oracle_mod, kind):
+    """Every launch configuration the autotuner may pick (hex8 elements per
+    workgroup; assembly loads in batches of 1 or 8) gives the same global
+    matrix, bit for bit, and it matches the oracle to the FP32
+    element-matrix tolerance."""
+    if kind == "bsp":
+        m = M.bsp_block(ne=(4, 3, 3), p=3, perturb=0.05, seed=2)
+        u = None
+    else:
+        n = 6
+        m = M.hex8_block(n, n, n, form="tl_neohooke", perturb=0.1, seed=3,
+                         mat=np.tile((1.0, 1.5), (n ** 3, 1)))
+        X = m.coords
+        u = np.stack([0.02 * X[:, 0] * X[:, 2], 0.01 * X[:, 1] ** 2, -0.03 * X[:, 0] ** 2], 1)
+        u[m.fixed] = 0
+    g = fg.GpuModel(m)
+    g.stiffness(u)
+    assert g.tuning_report()  # the autotuner ran on the first assembly
+    ref = g.csr()[2].copy()
+    seen = 0
+    for te in ([0, 4, 8, 16, 32, 64] if kind != "bsp" else [0]):
+        for mode in (0, 2):
+            try:
+                g.set_launch(te, mode)
+            except fg.FgpuError:
+                continue
+            g.stiffness(u)
+            val = g.csr()[2]
+            assert np.array_equal(val, ref), (te, mode)
+            seen += 1
+    assert seen >= 2
+    om = oracle_mod.OracleMesh(m)
+    Ko, ok = om.assemble_K(u)
+    assert nrel(ref, Ko) <= 1e-6
