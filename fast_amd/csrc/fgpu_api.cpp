@@ -447,7 +447,6 @@ int fgpu_create(fgpu_ctx **out, const fgpu_mesh *m) {
       if (m->deg[0] != 3 || m->deg[1] != 3 || m->deg[2] != 3 || m->nip[0] != 4 ||
           m->nip[1] != 4 || m->nip[2] != 4)
         throw Error("B-spline solid kernels support degree 3 with 4x4x4 Gauss");
-      if (m->form != FGPU_LINEAR) throw Error("B-spline solid kernels support the linear form");
       if (!m->ext) throw Error("B-spline solid needs extraction operators");
     } else {
       for (int d = 0; d < 3; d++)

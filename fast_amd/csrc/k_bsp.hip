@@ -21,6 +21,16 @@
 // of a tile sit in the same lanes, so K_ab is formed in registers and each
 // lane stores whole 3x3 blocks; no P matrices go through LDS, which keeps the
 // workgroup small enough for three of them to share a CU.
+//
+// Total-Lagrangian tangents (cElmParamTL::StiffMat, elc/elmparam.cpp
+// 1711-1807, with the same closed forms as the hex8 kernel: St.Venant-
+// Kirchhoff K_ab = sum c [lambda (Fg)(Fh)^T + mu (Fh)(Fg)^T + mu (g.h) F F^T
+// + (g^T S h) I]; neo-Hooke K_ab = sum c [lambda g'h'^T + mu' h'g'^T +
+// mu (g.h) I], g' = F^-T g, mu' = mu - lambda ln J) carry per-point matrices
+// (F F^T, S) between the two gradients, which do not factor into a few
+// products over the points; bsp_Kt_kernel forms them on the vector ALUs
+// instead: phase 1 stages the mapped gradients per point in LDS, phase 2
+// runs one thread per stored block over the 64 points.
 #include "fgpu_internal.h"
 
 namespace fgpu {
@@ -250,18 +260,199 @@ __global__ __launch_bounds__(256, 3) void bsp_K_kernel(
   }
 }
 
+// ---------------------------------------------------------------- TL K
+// (a, b) of the stored blocks in packed upper order: a * 64 + b
+struct SymTable {
+  unsigned short ab[NSYM];
+};
+__constant__ SymTable kSym;
+
+template <int FORM>
+struct KtShared {
+  Geo g;
+  float A[3][NQP][NBAS];                                 // F g (SVK) / F^-T g (neo-Hooke)
+  float G[FORM == FGPU_TL_SVK ? 3 : 1][NQP][NBAS];       // g (SVK only)
+  float pp[NQP][16];  // SVK: c, F F^T (6), S (6);  neo-Hooke: F F^T (6), c lam, c mu', c mu
+};
+
+template <int FORM>
+__global__ __launch_bounds__(256) void bsp_Kt_kernel(
+    int nelem, const double *__restrict__ coords, const int *__restrict__ conn,
+    const double *__restrict__ cpw, const double *__restrict__ ext,
+    const double *__restrict__ mat, const double *__restrict__ unodal,
+    const double *__restrict__ gp, float *__restrict__ Ke, int *__restrict__ status) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  KtShared<FORM> &S = *reinterpret_cast<KtShared<FORM> *>(smem);
+  const int e = blockIdx.x;
+  const int tid = threadIdx.x;
+  const int *cn = conn + (size_t)e * NBAS;
+  geo_load(S.g, e, coords, conn, cpw, ext, gp);
+  __syncthreads();
+  double lam, mu;
+  if constexpr (FORM == FGPU_TL_SVK) {
+    const double E = mat[2 * (size_t)e], nu = mat[2 * (size_t)e + 1];
+    lam = E * nu / ((1.0 + nu) * (1.0 - 2.0 * nu));
+    mu = 0.5 * E / (1.0 + nu);
+  } else {  // neo-Hooke: mat = (mu, lambda)
+    mu = mat[2 * (size_t)e];
+    lam = mat[2 * (size_t)e + 1];
+  }
+  {  // phase 1, thread (q, part): F at q, mapped gradients of 16 nodes
+    const int q = tid / 4, part = tid % 4;
+    PointGeo P;
+    point_geometry(S.g, q, part, gp, P);
+    double L[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
+    #pragma unroll 2
+    for (int k = 0; k < 16; k++) {
+      const int a = part * 16 + k;
+      const double *ua = unodal + 3 * (size_t)cn[a];
+      double dR[3], G[3];
+      rational_dR(S.g, q, a, P, dR);
+      for (int i = 0; i < 3; i++)
+        G[i] = P.Ji[i][0] * dR[0] + P.Ji[i][1] * dR[1] + P.Ji[i][2] * dR[2];
+      for (int i = 0; i < 3; i++)
+        for (int j = 0; j < 3; j++) L[i][j] += ua[i] * G[j];
+    }
+    for (int m = 1; m < 4; m <<= 1)
+      for (int i = 0; i < 3; i++)
+        for (int j = 0; j < 3; j++) L[i][j] += __shfl_xor(L[i][j], m);
+    double F[3][3];
+    for (int i = 0; i < 3; i++)
+      for (int j = 0; j < 3; j++) F[i][j] = (i == j ? 1.0 : 0.0) + L[i][j];
+    double B[6];  // F F^T (xx, yy, zz, xy, xz, yz)
+    B[0] = F[0][0] * F[0][0] + F[0][1] * F[0][1] + F[0][2] * F[0][2];
+    B[1] = F[1][0] * F[1][0] + F[1][1] * F[1][1] + F[1][2] * F[1][2];
+    B[2] = F[2][0] * F[2][0] + F[2][1] * F[2][1] + F[2][2] * F[2][2];
+    B[3] = F[0][0] * F[1][0] + F[0][1] * F[1][1] + F[0][2] * F[1][2];
+    B[4] = F[0][0] * F[2][0] + F[0][1] * F[2][1] + F[0][2] * F[2][2];
+    B[5] = F[1][0] * F[2][0] + F[1][1] * F[2][1] + F[1][2] * F[2][2];
+    double M[3][3];  // the map of the gradients: F (SVK) or F^-T (neo-Hooke)
+    if constexpr (FORM == FGPU_TL_SVK) {
+      for (int i = 0; i < 3; i++)
+        for (int j = 0; j < 3; j++) M[i][j] = F[i][j];
+      if (part == 0) {
+        // Green-Lagrange E = (F^T F - I) / 2 ; S = lambda tr(E) I + 2 mu E
+        double C[3][3];
+        for (int i = 0; i < 3; i++)
+          for (int j = 0; j < 3; j++)
+            C[i][j] = F[0][i] * F[0][j] + F[1][i] * F[1][j] + F[2][i] * F[2][j];
+        const double tr = 0.5 * (C[0][0] - 1.0) + 0.5 * (C[1][1] - 1.0) + 0.5 * (C[2][2] - 1.0);
+        float *pp = S.pp[q];
+        pp[0] = (float)P.c;
+        for (int k = 0; k < 6; k++) pp[1 + k] = (float)B[k];
+        pp[7] = (float)(lam * tr + mu * (C[0][0] - 1.0));
+        pp[8] = (float)(lam * tr + mu * (C[1][1] - 1.0));
+        pp[9] = (float)(lam * tr + mu * (C[2][2] - 1.0));
+        pp[10] = (float)(mu * C[0][1]);
+        pp[11] = (float)(mu * C[0][2]);
+        pp[12] = (float)(mu * C[1][2]);
+      }
+    } else {
+      double Fi[3][3];
+      double J = inv3d(F, Fi);
+      if (!(J > 0.0)) {
+        if (part == 0) atomicAdd(&status[1], 1);
+        J = 1.0;
+      }
+      for (int i = 0; i < 3; i++)
+        for (int j = 0; j < 3; j++) M[i][j] = Fi[j][i];
+      if (part == 0) {
+        float *pp = S.pp[q];
+        for (int k = 0; k < 6; k++) pp[k] = (float)B[k];
+        pp[6] = (float)(P.c * lam);
+        pp[7] = (float)(P.c * (mu - lam * log(J)));
+        pp[8] = (float)(P.c * mu);
+      }
+    }
+    if (part == 0 && !(P.detJ > 0.0)) atomicAdd(&status[0], 1);
+    #pragma unroll 2
+    for (int k = 0; k < 16; k++) {
+      const int a = part * 16 + k;
+      double dR[3], G[3];
+      rational_dR(S.g, q, a, P, dR);
+      for (int i = 0; i < 3; i++)
+        G[i] = P.Ji[i][0] * dR[0] + P.Ji[i][1] * dR[1] + P.Ji[i][2] * dR[2];
+      for (int i = 0; i < 3; i++) {
+        S.A[i][q][a] = (float)(M[i][0] * G[0] + M[i][1] * G[1] + M[i][2] * G[2]);
+        if constexpr (FORM == FGPU_TL_SVK) S.G[i][q][a] = (float)G[i];
+      }
+    }
+  }
+  __syncthreads();
+  // phase 2: one thread per stored block, all 64 points
+  const float flam = (float)lam, fmu = (float)mu;
+  float *out = Ke + (size_t)e * NSYM * 9;
+  for (int blk = tid; blk < NSYM; blk += 256) {
+    const int a = kSym.ab[blk] >> 6, b = kSym.ab[blk] & 63;
+    float K[9];
+    if constexpr (FORM == FGPU_TL_SVK) {
+      float Mm[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0}, Bs[6] = {0, 0, 0, 0, 0, 0}, sg = 0.f;
+      for (int q = 0; q < NQP; q++) {
+        const float *pp = S.pp[q];
+        const float c = pp[0];
+        float fa[3], fb[3], ga[3], gb[3];
+        for (int i = 0; i < 3; i++) {
+          fa[i] = S.A[i][q][a];
+          fb[i] = S.A[i][q][b];
+          ga[i] = c * S.G[i][q][a];
+          gb[i] = S.G[i][q][b];
+        }
+        for (int i = 0; i < 3; i++)
+          for (int j = 0; j < 3; j++) Mm[3 * i + j] += c * (fa[i] * fb[j]);
+        const float gg = ga[0] * gb[0] + ga[1] * gb[1] + ga[2] * gb[2];
+        for (int k = 0; k < 6; k++) Bs[k] += gg * pp[1 + k];
+        sg += ga[0] * (pp[7] * gb[0] + pp[10] * gb[1] + pp[11] * gb[2]) +
+              ga[1] * (pp[10] * gb[0] + pp[8] * gb[1] + pp[12] * gb[2]) +
+              ga[2] * (pp[11] * gb[0] + pp[12] * gb[1] + pp[9] * gb[2]);
+      }
+      const float Bm[9] = {Bs[0], Bs[3], Bs[4], Bs[3], Bs[1], Bs[5], Bs[4], Bs[5], Bs[2]};
+      for (int i = 0; i < 3; i++)
+        for (int j = 0; j < 3; j++)
+          K[3 * i + j] = flam * Mm[3 * i + j] + fmu * Mm[3 * j + i] + fmu * Bm[3 * i + j] +
+                         (i == j ? sg : 0.f);
+    } else {
+      float M1[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0}, M2[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0}, sg = 0.f;
+      for (int q = 0; q < NQP; q++) {
+        const float *pp = S.pp[q];
+        float ga[3], gb[3];
+        for (int i = 0; i < 3; i++) {
+          ga[i] = S.A[i][q][a];
+          gb[i] = S.A[i][q][b];
+        }
+        for (int i = 0; i < 3; i++)
+          for (int j = 0; j < 3; j++) {
+            const float p = ga[i] * gb[j];
+            M1[3 * i + j] += pp[6] * p;
+            M2[3 * i + j] += pp[7] * p;
+          }
+        // g.h = g'^T (F F^T) h'
+        sg += pp[8] * (ga[0] * (pp[0] * gb[0] + pp[3] * gb[1] + pp[4] * gb[2]) +
+                       ga[1] * (pp[3] * gb[0] + pp[1] * gb[1] + pp[5] * gb[2]) +
+                       ga[2] * (pp[4] * gb[0] + pp[5] * gb[1] + pp[2] * gb[2]));
+      }
+      for (int i = 0; i < 3; i++)
+        for (int j = 0; j < 3; j++) K[3 * i + j] = M1[3 * i + j] + M2[3 * j + i] + (i == j ? sg : 0.f);
+    }
+    float *o = out + (size_t)blk * 9;
+    for (int k = 0; k < 9; k++) o[k] = K[k];
+  }
+}
+
 // ---------------------------------------------------------------- f_int
 // Phase 1, thread (q, part): geometry, then grad u at q from its 16 control
-// points (shuffle-reduced), then c * sigma.  Phase 2, thread (a, quarter):
-// g_a = sum_q c sigma_q . grad N_a(q) over 16 points, recomputing grad N_a
-// from the tensor tables and the stored J^-1, reduced over the 4 quarters.
+// points (shuffle-reduced), then c times the stress that multiplies the
+// gradients: sigma (linear) or P = F S (total Lagrangian; the reference's
+// (B + Bl)^T S, elmparam.cpp:1615-1707).  Phase 2, thread (a, quarter):
+// g_a = sum_q c P_q grad N_a(q) over 16 points, recomputing grad N_a from the
+// tensor tables and the stored J^-1, reduced over the 4 quarters.
 struct FShared {
   Geo g;
   double Ji[NQP][9];
   double rs[NQP][4];   // 1 / W, dW/dr, dW/ds, dW/dt
-  double sig[NQP][6];  // c * sigma (Voigt xx yy zz xy xz yz)
+  double sig[NQP][9];  // c * P, row-major
 };
 
+template <int FORM>
 __global__ __launch_bounds__(256) void bsp_fint_kernel(
     int nelem, const double *__restrict__ coords, const int *__restrict__ conn,
     const double *__restrict__ cpw, const double *__restrict__ ext,
@@ -298,15 +489,47 @@ __global__ __launch_bounds__(256) void bsp_fint_kernel(
         for (int j = 0; j < 3; j++) S.Ji[q][3 * i + j] = P.Ji[i][j];
       S.rs[q][0] = P.inv_s;
       for (int k = 0; k < 3; k++) S.rs[q][1 + k] = P.dW[k];
-      const double E = mat[2 * (size_t)e], nu = mat[2 * (size_t)e + 1];
-      const double lam = E * nu / ((1.0 + nu) * (1.0 - 2.0 * nu)), mu = 0.5 * E / (1.0 + nu);
-      const double tr = L[0][0] + L[1][1] + L[2][2], c = P.c;
-      S.sig[q][0] = c * (lam * tr + 2.0 * mu * L[0][0]);
-      S.sig[q][1] = c * (lam * tr + 2.0 * mu * L[1][1]);
-      S.sig[q][2] = c * (lam * tr + 2.0 * mu * L[2][2]);
-      S.sig[q][3] = c * mu * (L[0][1] + L[1][0]);
-      S.sig[q][4] = c * mu * (L[0][2] + L[2][0]);
-      S.sig[q][5] = c * mu * (L[1][2] + L[2][1]);
+      const double c = P.c;
+      double *sg = S.sig[q];
+      if constexpr (FORM == FGPU_LINEAR) {
+        const double E = mat[2 * (size_t)e], nu = mat[2 * (size_t)e + 1];
+        const double lam = E * nu / ((1.0 + nu) * (1.0 - 2.0 * nu)), mu = 0.5 * E / (1.0 + nu);
+        const double tr = L[0][0] + L[1][1] + L[2][2];
+        sg[0] = c * (lam * tr + 2.0 * mu * L[0][0]);
+        sg[4] = c * (lam * tr + 2.0 * mu * L[1][1]);
+        sg[8] = c * (lam * tr + 2.0 * mu * L[2][2]);
+        sg[1] = sg[3] = c * mu * (L[0][1] + L[1][0]);
+        sg[2] = sg[6] = c * mu * (L[0][2] + L[2][0]);
+        sg[5] = sg[7] = c * mu * (L[1][2] + L[2][1]);
+      } else {
+        double F[3][3], C[3][3], Sp[3][3];
+        for (int i = 0; i < 3; i++)
+          for (int j = 0; j < 3; j++) F[i][j] = (i == j ? 1.0 : 0.0) + L[i][j];
+        for (int i = 0; i < 3; i++)
+          for (int j = 0; j < 3; j++)
+            C[i][j] = F[0][i] * F[0][j] + F[1][i] * F[1][j] + F[2][i] * F[2][j];
+        if constexpr (FORM == FGPU_TL_SVK) {
+          const double E = mat[2 * (size_t)e], nu = mat[2 * (size_t)e + 1];
+          const double lam = E * nu / ((1.0 + nu) * (1.0 - 2.0 * nu)),
+                       mu = 0.5 * E / (1.0 + nu);
+          const double tr = 0.5 * (C[0][0] - 1.0) + 0.5 * (C[1][1] - 1.0) + 0.5 * (C[2][2] - 1.0);
+          for (int i = 0; i < 3; i++)
+            for (int j = 0; j < 3; j++)
+              Sp[i][j] = (i == j ? lam * tr : 0.0) + mu * (C[i][j] - (i == j ? 1.0 : 0.0));
+        } else {  // neo-Hooke, mat = (mu, lambda): S = mu (I - C^-1) + lambda ln J C^-1
+          const double mu = mat[2 * (size_t)e], lam = mat[2 * (size_t)e + 1];
+          double Ci[3][3];
+          const double detC = inv3d(C, Ci);
+          if (!(detC > 0.0)) atomicAdd(&status[1], 1);
+          const double lnJ = detC > 0.0 ? 0.5 * log(detC) : 0.0;
+          for (int i = 0; i < 3; i++)
+            for (int j = 0; j < 3; j++)
+              Sp[i][j] = mu * ((i == j ? 1.0 : 0.0) - Ci[i][j]) + lam * lnJ * Ci[i][j];
+        }
+        for (int i = 0; i < 3; i++)
+          for (int j = 0; j < 3; j++)
+            sg[3 * i + j] = c * (F[i][0] * Sp[0][j] + F[i][1] * Sp[1][j] + F[i][2] * Sp[2][j]);
+      }
     }
   }
   __syncthreads();
@@ -323,9 +546,9 @@ __global__ __launch_bounds__(256) void bsp_fint_kernel(
       for (int i = 0; i < 3; i++)
         G[i] = S.Ji[q][3 * i] * dR[0] + S.Ji[q][3 * i + 1] * dR[1] + S.Ji[q][3 * i + 2] * dR[2];
       const double *s = S.sig[q];
-      g0 += s[0] * G[0] + s[3] * G[1] + s[4] * G[2];
-      g1 += s[3] * G[0] + s[1] * G[1] + s[5] * G[2];
-      g2 += s[4] * G[0] + s[5] * G[1] + s[2] * G[2];
+      g0 += s[0] * G[0] + s[1] * G[1] + s[2] * G[2];
+      g1 += s[3] * G[0] + s[4] * G[1] + s[5] * G[2];
+      g2 += s[6] * G[0] + s[7] * G[1] + s[8] * G[2];
     }
     for (int m = 1; m < 4; m <<= 1) {
       g0 += __shfl_xor(g0, m);
@@ -341,13 +564,11 @@ __global__ __launch_bounds__(256) void bsp_fint_kernel(
   }
 }
 
+// dynamic LDS above 64 KB must be allowed per kernel (cheap; set per launch)
 template <class K>
-void set_lds(K kernel, size_t lds, int dev) {
-  static int done_dev = -1;
-  if (done_dev == dev) return;
+void set_lds(K kernel, size_t lds) {
   FGPU_CHECK(hipFuncSetAttribute((const void *)kernel,
                                  hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-  done_dev = dev;
 }
 
 }  // namespace
@@ -364,21 +585,63 @@ void bsp_prepare_tables(Ctx &c, const double *h_ext, const double *h_cpw) {
   }
 }
 
-void launch_bsp_K(Ctx &c, const double *u) {
-  (void)u;
-  const size_t lds = sizeof(KShared);
-  set_lds(bsp_K_kernel, lds, c.device);
-  hipLaunchKernelGGL(bsp_K_kernel, dim3(c.nelem), dim3(256), lds, c.stream, c.nelem, c.d_coords,
-                     c.d_conn, c.d_cpw, c.d_ext, c.d_mat, c.d_gauss64, c.d_Ke, c.d_status);
+namespace {
+
+template <int FORM>
+void launch_Kt(Ctx &c, const double *u) {
+  static int table_dev = -1;
+  if (table_dev != c.device) {  // packed (a, b) of the stored blocks
+    SymTable t;
+    for (int a = 0; a < NBAS; a++)
+      for (int b = a; b < NBAS; b++) t.ab[sym_index(a, b, NBAS)] = (unsigned short)(a * 64 + b);
+    FGPU_CHECK(hipMemcpyToSymbol(HIP_SYMBOL(kSym), &t, sizeof t));
+    table_dev = c.device;
+  }
+  const size_t lds = sizeof(KtShared<FORM>);
+  set_lds(bsp_Kt_kernel<FORM>, lds);
+  hipLaunchKernelGGL(bsp_Kt_kernel<FORM>, dim3(c.nelem), dim3(256), lds, c.stream, c.nelem,
+                     c.d_coords, c.d_conn, c.d_cpw, c.d_ext, c.d_mat, u, c.d_gauss64, c.d_Ke,
+                     c.d_status);
   FGPU_CHECK(hipGetLastError());
 }
 
-void launch_bsp_fint(Ctx &c, const double *u) {
+template <int FORM>
+void launch_fint(Ctx &c, const double *u) {
   const size_t lds = sizeof(FShared);
-  hipLaunchKernelGGL(bsp_fint_kernel, dim3(c.nelem), dim3(256), lds, c.stream, c.nelem,
+  hipLaunchKernelGGL(bsp_fint_kernel<FORM>, dim3(c.nelem), dim3(256), lds, c.stream, c.nelem,
                      c.d_coords, c.d_conn, c.d_cpw, c.d_ext, c.d_mat, u, c.d_gauss64, c.d_ge,
                      c.d_status);
   FGPU_CHECK(hipGetLastError());
+}
+
+}  // namespace
+
+void launch_bsp_K(Ctx &c, const double *u) {
+  if (c.form != FGPU_LINEAR && !u) {
+    // zero displacement: stage a zero nodal field
+    FGPU_CHECK(hipMemsetAsync(c.d_u, 0, sizeof(double) * 3 * c.nnode, c.stream));
+    u = c.d_u;
+  }
+  if (c.form == FGPU_TL_SVK) {
+    launch_Kt<FGPU_TL_SVK>(c, u);
+  } else if (c.form == FGPU_TL_NEOHOOKE) {
+    launch_Kt<FGPU_TL_NEOHOOKE>(c, u);
+  } else {
+    const size_t lds = sizeof(KShared);
+    set_lds(bsp_K_kernel, lds);
+    hipLaunchKernelGGL(bsp_K_kernel, dim3(c.nelem), dim3(256), lds, c.stream, c.nelem,
+                       c.d_coords, c.d_conn, c.d_cpw, c.d_ext, c.d_mat, c.d_gauss64, c.d_Ke,
+                       c.d_status);
+    FGPU_CHECK(hipGetLastError());
+  }
+}
+
+void launch_bsp_fint(Ctx &c, const double *u) {
+  switch (c.form) {
+    case FGPU_TL_SVK: launch_fint<FGPU_TL_SVK>(c, u); break;
+    case FGPU_TL_NEOHOOKE: launch_fint<FGPU_TL_NEOHOOKE>(c, u); break;
+    default: launch_fint<FGPU_LINEAR>(c, u); break;
+  }
 }
 
 }  // namespace fgpu

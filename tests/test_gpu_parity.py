@@ -70,15 +70,34 @@ def test_hex8_element_matrices_vs_oracle(fg, oracle_mod, form, matp, nip):
     elem_check(fg, oracle_mod, m, u, tol=1e-6 if nip > 1 else 2e-6)
 
 
-def test_bsp_element_matrices_vs_reference_fixture(fg, oracle_mod):
-    d = load_golden("elem_bsp3_linear.npz")
+@pytest.mark.parametrize("form", ["linear", "tl_svk"])
+def test_bsp_element_matrices_vs_reference_fixture(fg, oracle_mod, form):
+    d = load_golden("elem_bsp3_%s.npz" % form)
     m = mesh_from_golden(d)
     g = fg.GpuModel(m)
-    g.stiffness()
+    g.stiffness(d["u"] if form != "linear" else None)
     Kg = g.element_matrices(0, m.nelem).astype(np.float64)
     assert nrel(Kg[0], d["K0"]) <= 1e-6
     for e in range(m.nelem):
         assert nrel(Kg[e] @ d["v"], d["Kv"][e]) <= 2e-6
+        assert nrel(np.diag(Kg[e]), d["Kdiag"][e]) <= 1e-6
+
+
+@pytest.mark.parametrize("form,matp", [("tl_svk", (2.6, 0.3)), ("tl_neohooke", (1.0, 1.5))])
+def test_bsp_total_lagrangian_vs_oracle(fg, oracle_mod, form, matp):
+    """B-spline total-Lagrangian tangent (vector-ALU kernel) and internal
+    force against the FP64 oracle on a perturbed, deformed patch."""
+    m = M.bsp_block(ne=(3, 2, 2), p=3, size=(1.0, 0.7, 0.5), form=form, perturb=0.08, seed=5,
+                    mat=np.tile(matp, (12, 1)))
+    X = m.coords
+    u = np.stack([0.04 * X[:, 0] * X[:, 2], 0.03 * X[:, 1] ** 2, -0.05 * X[:, 0] ** 2], 1)
+    u += np.random.default_rng(4).uniform(-0.01, 0.01, u.shape)
+    worst, g, om = elem_check(fg, oracle_mod, m, u, tol=1e-6)
+    print("bsp %s worst element rel err %.2e" % (form, worst))
+    fi = g.int_force(u)
+    f0, ok = om.assemble_fint(u)
+    assert ok
+    assert nrel(fi, f0) <= 1e-11
 
 
 def test_bsp_element_matrices_vs_oracle_perturbed_patch(fg, oracle_mod):
@@ -123,8 +142,9 @@ def test_internal_force_vs_oracle_fp64(fg, oracle_mod, form, matp):
     assert nrel(g, g0) <= 1e-11
 
 
-def test_internal_force_bsp_vs_reference_fixture(fg, oracle_mod):
-    d = load_golden("elem_bsp3_linear.npz")
+@pytest.mark.parametrize("form", ["linear", "tl_svk"])
+def test_internal_force_bsp_vs_reference_fixture(fg, oracle_mod, form):
+    d = load_golden("elem_bsp3_%s.npz" % form)
     m = mesh_from_golden(d)
     gg = fg.GpuModel(m).int_force(d["u"])
     g0 = np.zeros(m.neq)
