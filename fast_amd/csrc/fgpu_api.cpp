@@ -444,9 +444,10 @@ int fgpu_create(fgpu_ctx **out, const fgpu_mesh *m) {
     if (m->shape == FGPU_BSP) {
       if (m->nn != (m->deg[0] + 1) * (m->deg[1] + 1) * (m->deg[2] + 1))
         throw Error("B-spline solid: nn != prod(p+1)");
-      if (m->deg[0] != 3 || m->deg[1] != 3 || m->deg[2] != 3 || m->nip[0] != 4 ||
-          m->nip[1] != 4 || m->nip[2] != 4)
-        throw Error("B-spline solid kernels support degree 3 with 4x4x4 Gauss");
+      if (m->deg[0] != 3 || m->deg[1] != 3 || (m->deg[2] != 3 && m->deg[2] != 2) ||
+          m->nip[0] != 4 || m->nip[1] != 4 || m->nip[2] != 4)
+        throw Error("B-spline solid kernels support degrees (3, 3, 3) and (3, 3, 2) with "
+                    "4x4x4 Gauss");
       if (!m->ext) throw Error("B-spline solid needs extraction operators");
     } else {
       for (int d = 0; d < 3; d++)

@@ -38,9 +38,30 @@ namespace {
 
 typedef float floatx4 __attribute__((ext_vector_type(4)));
 
-constexpr int NB = 4, NQ1 = 4;   // basis and points per direction
-constexpr int NBAS = 64, NQP = 64;
-constexpr int NSYM = NBAS * (NBAS + 1) / 2;  // 2080
+// Degree 3 along r and s; along t degree 3 (NB2 = 4 basis functions) or 2
+// (NB2 = 3).  Points: 4 x 4 x 4 Gauss.
+constexpr int NB = 4, NQ1 = 4;   // basis (at most) and points per direction
+constexpr int NQP = 64;
+template <int NB2>
+struct Bsp {
+  static constexpr int NBAS = 16 * NB2;                  // 64 or 48 control points
+  static constexpr int NSYM = NBAS * (NBAS + 1) / 2;     // 2080 or 1176 stored blocks
+  static constexpr int NPART = NBAS / 4;                 // control points per quarter
+  static constexpr int NT = NBAS / 16;                   // 16-wide tiles
+};
+
+// degree 2: B_i and d B_i / dt (with the 1/2 of t' = (t+1)/2)
+__device__ inline void bernstein2(double u, double *B, double *dB) {
+  double v = 1.0 - u;
+  B[0] = v * v;
+  B[1] = 2.0 * u * v;
+  B[2] = u * u;
+  B[3] = 0.0;
+  dB[0] = 0.5 * 2.0 * (0.0 - v);
+  dB[1] = 0.5 * 2.0 * (v - u);
+  dB[2] = 0.5 * 2.0 * (u - 0.0);
+  dB[3] = 0.0;
+}
 
 __device__ inline void bernstein3(double u, double *B, double *dB) {
   double v = 1.0 - u;
@@ -75,15 +96,19 @@ __device__ inline double inv3d(const double J[3][3], double I[3][3]) {
 
 // Element geometry staged in LDS: univariate tables at the Gauss points,
 // control points (relative to the first) and weights.
+template <int NB2>
 struct Geo {
   double N[3][NQ1][NB], dN[3][NQ1][NB];  // [dir][point][basis]
-  double X[NBAS][3];
-  double W[NBAS];
+  double X[Bsp<NB2>::NBAS][3];
+  double W[Bsp<NB2>::NBAS];
 };
 
-__device__ inline void geo_load(Geo &g, int e, const double *__restrict__ coords,
+// extraction rows on the device: [e][dir][4][4], zero-padded for degree 2
+template <int NB2>
+__device__ inline void geo_load(Geo<NB2> &g, int e, const double *__restrict__ coords,
                                 const int *__restrict__ conn, const double *__restrict__ cpw,
                                 const double *__restrict__ ext, const double *__restrict__ gp) {
+  constexpr int NBAS = Bsp<NB2>::NBAS;
   const int tid = threadIdx.x;
   const int *cn = conn + (size_t)e * NBAS;
   if (tid < NBAS) {
@@ -95,7 +120,10 @@ __device__ inline void geo_load(Geo &g, int e, const double *__restrict__ coords
     int d = t / (NQ1 * NB), q = (t / NB) % NQ1, i = t % NB;
     double u = (gp[(d * 16 + q) * 2] + 1.0) / 2.0;
     double B[4], dB[4];
-    bernstein3(u, B, dB);
+    if (d == 2 && NB2 == 3)
+      bernstein2(u, B, dB);
+    else
+      bernstein3(u, B, dB);
     const double *C = ext + ((size_t)e * 3 + d) * 16 + i * 4;
     double n = 0.0, dn = 0.0;
     for (int j = 0; j < 4; j++) {
@@ -109,7 +137,8 @@ __device__ inline void geo_load(Geo &g, int e, const double *__restrict__ coords
 
 // Numerators (N w, dN w) of the tensor-product basis a at point q;
 // a = i + 4 j + 16 k with i along r, q likewise.
-__device__ inline void tensor(const Geo &g, int q, int a, double &n, double d[3]) {
+template <int NB2>
+__device__ inline void tensor(const Geo<NB2> &g, int q, int a, double &n, double d[3]) {
   const int qr = q % 4, qs = (q / 4) % 4, qt = q / 16;
   const int i = a % 4, j = (a / 4) % 4, k = a / 16;
   const double w = g.W[a];
@@ -130,20 +159,24 @@ struct PointGeo {
 };
 
 // d R_a / d(r,s,t) at point q (rational quotient with the point's sums)
-__device__ inline void rational_dR(const Geo &g, int q, int a, const PointGeo &P, double dR[3]) {
+template <int NB2>
+__device__ inline void rational_dR(const Geo<NB2> &g, int q, int a, const PointGeo &P,
+                                   double dR[3]) {
   double n, d[3];
   tensor(g, q, a, n, d);
   const double R = n * P.inv_s;
   for (int i = 0; i < 3; i++) dR[i] = (d[i] - R * P.dW[i]) * P.inv_s;
 }
 
-__device__ inline void point_geometry(const Geo &g, int q, int part, const double *gp,
+template <int NB2>
+__device__ inline void point_geometry(const Geo<NB2> &g, int q, int part, const double *gp,
                                       PointGeo &P) {
+  constexpr int NPART = Bsp<NB2>::NPART;
   double s = 0.0, ds[3] = {0, 0, 0};
   #pragma unroll 2
-  for (int k = 0; k < 16; k++) {
+  for (int k = 0; k < NPART; k++) {
     double n, d[3];
-    tensor(g, q, part * 16 + k, n, d);
+    tensor(g, q, part * NPART + k, n, d);
     s += n;
     ds[0] += d[0];
     ds[1] += d[1];
@@ -157,8 +190,8 @@ __device__ inline void point_geometry(const Geo &g, int q, int part, const doubl
   for (int k = 0; k < 3; k++) P.dW[k] = ds[k];
   double J[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
   #pragma unroll 2
-  for (int k = 0; k < 16; k++) {
-    const int a = part * 16 + k;
+  for (int k = 0; k < NPART; k++) {
+    const int a = part * NPART + k;
     double dR[3];
     rational_dR(g, q, a, P, dR);
     for (int i = 0; i < 3; i++)
@@ -172,23 +205,23 @@ __device__ inline void point_geometry(const Geo &g, int q, int part, const doubl
   P.c = gp[(0 * 16 + qr) * 2 + 1] * gp[(1 * 16 + qs) * 2 + 1] * gp[(2 * 16 + qt) * 2 + 1] * P.detJ;
 }
 
+template <int NB2>
 struct KShared {
-  Geo g;
-  float G[3][NQP][NBAS];  // dN_a/dx_i at point q
-  float c[NQP];           // w detJ
+  Geo<NB2> g;
+  float G[3][NQP][Bsp<NB2>::NBAS];  // dN_a/dx_i at point q
+  float c[NQP];                      // w detJ
 };
 
-// upper 16x16 tile pairs (ta <= tb) of the 64 x 64 node-block matrix
-__constant__ unsigned char kPairA[10] = {0, 0, 0, 0, 1, 1, 1, 2, 2, 3};
-__constant__ unsigned char kPairB[10] = {0, 1, 2, 3, 1, 2, 3, 2, 3, 3};
-
+template <int NB2>
 __global__ __launch_bounds__(256, 3) void bsp_K_kernel(
     int nelem, const double *__restrict__ coords, const int *__restrict__ conn,
     const double *__restrict__ cpw, const double *__restrict__ ext,
     const double *__restrict__ mat, const double *__restrict__ gp, float *__restrict__ Ke,
     int *__restrict__ status) {
+  constexpr int NBAS = Bsp<NB2>::NBAS, NSYM = Bsp<NB2>::NSYM, NPART = Bsp<NB2>::NPART;
+  constexpr int NT = Bsp<NB2>::NT, NPAIR = NT * (NT + 1) / 2;
   extern __shared__ __align__(16) unsigned char smem[];
-  KShared &S = *reinterpret_cast<KShared *>(smem);
+  KShared<NB2> &S = *reinterpret_cast<KShared<NB2> *>(smem);
   const int e = blockIdx.x;
   const int tid = threadIdx.x;
   geo_load(S.g, e, coords, conn, cpw, ext, gp);
@@ -202,11 +235,11 @@ __global__ __launch_bounds__(256, 3) void bsp_K_kernel(
       if (!(P.detJ > 0.0)) atomicAdd(&status[0], 1);
     }
     #pragma unroll 2
-    for (int k = 0; k < 16; k++) {
+    for (int k = 0; k < NPART; k++) {
       double dR[3];
-      rational_dR(S.g, q, part * 16 + k, P, dR);
+      rational_dR(S.g, q, part * NPART + k, P, dR);
       for (int i = 0; i < 3; i++)
-        S.G[i][q][part * 16 + k] =
+        S.G[i][q][part * NPART + k] =
             (float)(P.Ji[i][0] * dR[0] + P.Ji[i][1] * dR[1] + P.Ji[i][2] * dR[2]);
     }
   }
@@ -218,8 +251,14 @@ __global__ __launch_bounds__(256, 3) void bsp_K_kernel(
   const int wave = tid / 64, lane = tid % 64;
   const int ln = lane % 16, lk = lane / 16;
   float *out = Ke + (size_t)e * NSYM * 9;
-  for (int tp = wave; tp < 10; tp += 4) {
-    const int a0 = 16 * kPairA[tp], b0 = 16 * kPairB[tp];
+  for (int tp = wave; tp < NPAIR; tp += 4) {
+    // upper 16x16 tile pair (ta <= tb) number tp, row-major
+    int ta = 0, r = tp;
+    while (r >= NT - ta) {
+      r -= NT - ta;
+      ta++;
+    }
+    const int a0 = 16 * ta, b0 = 16 * (ta + r);
     floatx4 acc[9];
     for (int t = 0; t < 9; t++)
       for (int r = 0; r < 4; r++) acc[t][r] = 0.f;
@@ -262,27 +301,38 @@ __global__ __launch_bounds__(256, 3) void bsp_K_kernel(
 
 // ---------------------------------------------------------------- TL K
 // (a, b) of the stored blocks in packed upper order: a * 64 + b
+template <int NB2>
 struct SymTable {
-  unsigned short ab[NSYM];
+  unsigned short ab[Bsp<NB2>::NSYM];
 };
-__constant__ SymTable kSym;
+__constant__ SymTable<4> kSym4;
+__constant__ SymTable<3> kSym3;
 
-template <int FORM>
+template <int NB2>
+__device__ inline unsigned short sym_ab(int blk) {
+  if constexpr (NB2 == 4)
+    return kSym4.ab[blk];
+  else
+    return kSym3.ab[blk];
+}
+
+template <int FORM, int NB2>
 struct KtShared {
-  Geo g;
-  float A[3][NQP][NBAS];                                 // F g (SVK) / F^-T g (neo-Hooke)
-  float G[FORM == FGPU_TL_SVK ? 3 : 1][NQP][NBAS];       // g (SVK only)
+  Geo<NB2> g;
+  float A[3][NQP][Bsp<NB2>::NBAS];                            // F g (SVK) / F^-T g (neo-Hooke)
+  float G[FORM == FGPU_TL_SVK ? 3 : 1][NQP][Bsp<NB2>::NBAS];  // g (SVK only)
   float pp[NQP][16];  // SVK: c, F F^T (6), S (6);  neo-Hooke: F F^T (6), c lam, c mu', c mu
 };
 
-template <int FORM>
+template <int FORM, int NB2>
 __global__ __launch_bounds__(256) void bsp_Kt_kernel(
     int nelem, const double *__restrict__ coords, const int *__restrict__ conn,
     const double *__restrict__ cpw, const double *__restrict__ ext,
     const double *__restrict__ mat, const double *__restrict__ unodal,
     const double *__restrict__ gp, float *__restrict__ Ke, int *__restrict__ status) {
+  constexpr int NBAS = Bsp<NB2>::NBAS, NSYM = Bsp<NB2>::NSYM, NPART = Bsp<NB2>::NPART;
   extern __shared__ __align__(16) unsigned char smem[];
-  KtShared<FORM> &S = *reinterpret_cast<KtShared<FORM> *>(smem);
+  KtShared<FORM, NB2> &S = *reinterpret_cast<KtShared<FORM, NB2> *>(smem);
   const int e = blockIdx.x;
   const int tid = threadIdx.x;
   const int *cn = conn + (size_t)e * NBAS;
@@ -303,8 +353,8 @@ __global__ __launch_bounds__(256) void bsp_Kt_kernel(
     point_geometry(S.g, q, part, gp, P);
     double L[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
     #pragma unroll 2
-    for (int k = 0; k < 16; k++) {
-      const int a = part * 16 + k;
+    for (int k = 0; k < NPART; k++) {
+      const int a = part * NPART + k;
       const double *ua = unodal + 3 * (size_t)cn[a];
       double dR[3], G[3];
       rational_dR(S.g, q, a, P, dR);
@@ -366,8 +416,8 @@ __global__ __launch_bounds__(256) void bsp_Kt_kernel(
     }
     if (part == 0 && !(P.detJ > 0.0)) atomicAdd(&status[0], 1);
     #pragma unroll 2
-    for (int k = 0; k < 16; k++) {
-      const int a = part * 16 + k;
+    for (int k = 0; k < NPART; k++) {
+      const int a = part * NPART + k;
       double dR[3], G[3];
       rational_dR(S.g, q, a, P, dR);
       for (int i = 0; i < 3; i++)
@@ -383,7 +433,8 @@ __global__ __launch_bounds__(256) void bsp_Kt_kernel(
   const float flam = (float)lam, fmu = (float)mu;
   float *out = Ke + (size_t)e * NSYM * 9;
   for (int blk = tid; blk < NSYM; blk += 256) {
-    const int a = kSym.ab[blk] >> 6, b = kSym.ab[blk] & 63;
+    const unsigned short ab = sym_ab<NB2>(blk);
+    const int a = ab >> 6, b = ab & 63;
     float K[9];
     if constexpr (FORM == FGPU_TL_SVK) {
       float Mm[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0}, Bs[6] = {0, 0, 0, 0, 0, 0}, sg = 0.f;
@@ -445,21 +496,23 @@ __global__ __launch_bounds__(256) void bsp_Kt_kernel(
 // (B + Bl)^T S, elmparam.cpp:1615-1707).  Phase 2, thread (a, quarter):
 // g_a = sum_q c P_q grad N_a(q) over 16 points, recomputing grad N_a from the
 // tensor tables and the stored J^-1, reduced over the 4 quarters.
+template <int NB2>
 struct FShared {
-  Geo g;
+  Geo<NB2> g;
   double Ji[NQP][9];
   double rs[NQP][4];   // 1 / W, dW/dr, dW/ds, dW/dt
   double sig[NQP][9];  // c * P, row-major
 };
 
-template <int FORM>
+template <int FORM, int NB2>
 __global__ __launch_bounds__(256) void bsp_fint_kernel(
     int nelem, const double *__restrict__ coords, const int *__restrict__ conn,
     const double *__restrict__ cpw, const double *__restrict__ ext,
     const double *__restrict__ mat, const double *__restrict__ unodal,
     const double *__restrict__ gp, double *__restrict__ ge, int *__restrict__ status) {
+  constexpr int NBAS = Bsp<NB2>::NBAS, NPART = Bsp<NB2>::NPART;
   extern __shared__ __align__(16) unsigned char smem[];
-  FShared &S = *reinterpret_cast<FShared *>(smem);
+  FShared<NB2> &S = *reinterpret_cast<FShared<NB2> *>(smem);
   const int e = blockIdx.x;
   const int tid = threadIdx.x;
   const int *cn = conn + (size_t)e * NBAS;
@@ -471,10 +524,10 @@ __global__ __launch_bounds__(256) void bsp_fint_kernel(
     point_geometry(S.g, q, part, gp, P);
     double L[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
     #pragma unroll 2
-    for (int k = 0; k < 16; k++) {
-      const double *ua = unodal + 3 * (size_t)cn[part * 16 + k];
+    for (int k = 0; k < NPART; k++) {
+      const double *ua = unodal + 3 * (size_t)cn[part * NPART + k];
       double dR[3], G[3];
-      rational_dR(S.g, q, part * 16 + k, P, dR);
+      rational_dR(S.g, q, part * NPART + k, P, dR);
       for (int i = 0; i < 3; i++)
         G[i] = P.Ji[i][0] * dR[0] + P.Ji[i][1] * dR[1] + P.Ji[i][2] * dR[2];
       for (int i = 0; i < 3; i++)
@@ -533,7 +586,7 @@ __global__ __launch_bounds__(256) void bsp_fint_kernel(
     }
   }
   __syncthreads();
-  {
+  if (tid < 4 * NBAS) {  // whole waves: 4 NBAS is 256 or 192
     const int a = tid / 4, quarter = tid % 4;
     double g0 = 0.0, g1 = 0.0, g2 = 0.0;
     for (int q = quarter * 16; q < quarter * 16 + 16; q++) {
@@ -574,10 +627,23 @@ void set_lds(K kernel, size_t lds) {
 }  // namespace
 
 void bsp_prepare_tables(Ctx &c, const double *h_ext, const double *h_cpw) {
-  // extraction rows and weights stay FP64 on the device
-  size_t n = (size_t)c.nelem * 3 * 16;
+  // extraction rows stay FP64 on the device, [e][dir][4][4]: the caller's
+  // packed (p_d + 1)^2 per direction, zero-padded for degree 2
+  const size_t n = (size_t)c.nelem * 3 * 16;
+  std::vector<double> pad(n, 0.0);
+  size_t per = 0;
+  for (int d = 0; d < 3; d++) per += (size_t)(c.deg[d] + 1) * (c.deg[d] + 1);
+  for (size_t e = 0; e < (size_t)c.nelem; e++) {
+    const double *src = h_ext + e * per;
+    for (int d = 0; d < 3; d++) {
+      const int q = c.deg[d] + 1;
+      for (int i = 0; i < q; i++)
+        for (int j = 0; j < q; j++) pad[(e * 3 + d) * 16 + i * 4 + j] = src[i * q + j];
+      src += q * q;
+    }
+  }
   FGPU_CHECK(hipMalloc(&c.d_ext, n * sizeof(double)));
-  FGPU_CHECK(hipMemcpyAsync(c.d_ext, h_ext, n * sizeof(double), hipMemcpyHostToDevice, c.stream));
+  FGPU_CHECK(hipMemcpy(c.d_ext, pad.data(), n * sizeof(double), hipMemcpyHostToDevice));
   if (h_cpw) {
     FGPU_CHECK(hipMalloc(&c.d_cpw, c.nnode * sizeof(double)));
     FGPU_CHECK(hipMemcpyAsync(c.d_cpw, h_cpw, c.nnode * sizeof(double), hipMemcpyHostToDevice,
@@ -587,31 +653,69 @@ void bsp_prepare_tables(Ctx &c, const double *h_ext, const double *h_cpw) {
 
 namespace {
 
-template <int FORM>
+template <int NB2>
+void upload_sym_table() {
+  SymTable<NB2> t;
+  constexpr int NBAS = Bsp<NB2>::NBAS;
+  for (int a = 0; a < NBAS; a++)
+    for (int b = a; b < NBAS; b++) t.ab[sym_index(a, b, NBAS)] = (unsigned short)(a * 64 + b);
+  if constexpr (NB2 == 4)
+    FGPU_CHECK(hipMemcpyToSymbol(HIP_SYMBOL(kSym4), &t, sizeof t));
+  else
+    FGPU_CHECK(hipMemcpyToSymbol(HIP_SYMBOL(kSym3), &t, sizeof t));
+}
+
+template <int FORM, int NB2>
 void launch_Kt(Ctx &c, const double *u) {
-  static int table_dev = -1;
-  if (table_dev != c.device) {  // packed (a, b) of the stored blocks
-    SymTable t;
-    for (int a = 0; a < NBAS; a++)
-      for (int b = a; b < NBAS; b++) t.ab[sym_index(a, b, NBAS)] = (unsigned short)(a * 64 + b);
-    FGPU_CHECK(hipMemcpyToSymbol(HIP_SYMBOL(kSym), &t, sizeof t));
+  static int table_dev = -1;  // the packed (a, b) table, once per device
+  if (table_dev != c.device) {
+    upload_sym_table<NB2>();
     table_dev = c.device;
   }
-  const size_t lds = sizeof(KtShared<FORM>);
-  set_lds(bsp_Kt_kernel<FORM>, lds);
-  hipLaunchKernelGGL(bsp_Kt_kernel<FORM>, dim3(c.nelem), dim3(256), lds, c.stream, c.nelem,
-                     c.d_coords, c.d_conn, c.d_cpw, c.d_ext, c.d_mat, u, c.d_gauss64, c.d_Ke,
+  const size_t lds = sizeof(KtShared<FORM, NB2>);
+  set_lds(bsp_Kt_kernel<FORM, NB2>, lds);
+  hipLaunchKernelGGL((bsp_Kt_kernel<FORM, NB2>), dim3(c.nelem), dim3(256), lds, c.stream,
+                     c.nelem, c.d_coords, c.d_conn, c.d_cpw, c.d_ext, c.d_mat, u, c.d_gauss64,
+                     c.d_Ke, c.d_status);
+  FGPU_CHECK(hipGetLastError());
+}
+
+template <int NB2>
+void launch_K_lin(Ctx &c) {
+  const size_t lds = sizeof(KShared<NB2>);
+  set_lds(bsp_K_kernel<NB2>, lds);
+  hipLaunchKernelGGL(bsp_K_kernel<NB2>, dim3(c.nelem), dim3(256), lds, c.stream, c.nelem,
+                     c.d_coords, c.d_conn, c.d_cpw, c.d_ext, c.d_mat, c.d_gauss64, c.d_Ke,
                      c.d_status);
   FGPU_CHECK(hipGetLastError());
 }
 
-template <int FORM>
+template <int FORM, int NB2>
 void launch_fint(Ctx &c, const double *u) {
-  const size_t lds = sizeof(FShared);
-  hipLaunchKernelGGL(bsp_fint_kernel<FORM>, dim3(c.nelem), dim3(256), lds, c.stream, c.nelem,
-                     c.d_coords, c.d_conn, c.d_cpw, c.d_ext, c.d_mat, u, c.d_gauss64, c.d_ge,
-                     c.d_status);
+  const size_t lds = sizeof(FShared<NB2>);
+  hipLaunchKernelGGL((bsp_fint_kernel<FORM, NB2>), dim3(c.nelem), dim3(256), lds, c.stream,
+                     c.nelem, c.d_coords, c.d_conn, c.d_cpw, c.d_ext, c.d_mat, u, c.d_gauss64,
+                     c.d_ge, c.d_status);
   FGPU_CHECK(hipGetLastError());
+}
+
+template <int NB2>
+void launch_K_nb(Ctx &c, const double *u) {
+  if (c.form == FGPU_TL_SVK)
+    launch_Kt<FGPU_TL_SVK, NB2>(c, u);
+  else if (c.form == FGPU_TL_NEOHOOKE)
+    launch_Kt<FGPU_TL_NEOHOOKE, NB2>(c, u);
+  else
+    launch_K_lin<NB2>(c);
+}
+
+template <int NB2>
+void launch_fint_nb(Ctx &c, const double *u) {
+  switch (c.form) {
+    case FGPU_TL_SVK: launch_fint<FGPU_TL_SVK, NB2>(c, u); break;
+    case FGPU_TL_NEOHOOKE: launch_fint<FGPU_TL_NEOHOOKE, NB2>(c, u); break;
+    default: launch_fint<FGPU_LINEAR, NB2>(c, u); break;
+  }
 }
 
 }  // namespace
@@ -622,26 +726,17 @@ void launch_bsp_K(Ctx &c, const double *u) {
     FGPU_CHECK(hipMemsetAsync(c.d_u, 0, sizeof(double) * 3 * c.nnode, c.stream));
     u = c.d_u;
   }
-  if (c.form == FGPU_TL_SVK) {
-    launch_Kt<FGPU_TL_SVK>(c, u);
-  } else if (c.form == FGPU_TL_NEOHOOKE) {
-    launch_Kt<FGPU_TL_NEOHOOKE>(c, u);
-  } else {
-    const size_t lds = sizeof(KShared);
-    set_lds(bsp_K_kernel, lds);
-    hipLaunchKernelGGL(bsp_K_kernel, dim3(c.nelem), dim3(256), lds, c.stream, c.nelem,
-                       c.d_coords, c.d_conn, c.d_cpw, c.d_ext, c.d_mat, c.d_gauss64, c.d_Ke,
-                       c.d_status);
-    FGPU_CHECK(hipGetLastError());
-  }
+  if (c.deg[2] == 2)
+    launch_K_nb<3>(c, u);
+  else
+    launch_K_nb<4>(c, u);
 }
 
 void launch_bsp_fint(Ctx &c, const double *u) {
-  switch (c.form) {
-    case FGPU_TL_SVK: launch_fint<FGPU_TL_SVK>(c, u); break;
-    case FGPU_TL_NEOHOOKE: launch_fint<FGPU_TL_NEOHOOKE>(c, u); break;
-    default: launch_fint<FGPU_LINEAR>(c, u); break;
-  }
+  if (c.deg[2] == 2)
+    launch_fint_nb<3>(c, u);
+  else
+    launch_fint_nb<4>(c, u);
 }
 
 }  // namespace fgpu

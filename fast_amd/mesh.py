@@ -251,14 +251,14 @@ def bsp_block(ne=(32, 32, 8), p=3, size=(1.0, 1.0, 1.0), E=1.0, nu=0.3,
     reference order (k slowest, i fastest).  All weights are 1 and the patch
     is rational, the case the reference evaluates correctly.
     """
+    deg = tuple(p) if np.ndim(p) else (p, p, p)
     if nip is None:
-        nip = p + 1
-    deg = (p, p, p)
-    knots = [open_uniform_knots(p, n) for n in ne]
-    nb = tuple(len(U) - p - 1 for U in knots)
-    gx = greville(p, knots[0]) * size[0]
-    gy = greville(p, knots[1]) * size[1]
-    gz = greville(p, knots[2]) * size[2]
+        nip = max(deg) + 1
+    knots = [open_uniform_knots(deg[d], ne[d]) for d in range(3)]
+    nb = tuple(len(knots[d]) - deg[d] - 1 for d in range(3))
+    gx = greville(deg[0], knots[0]) * size[0]
+    gy = greville(deg[1], knots[1]) * size[1]
+    gz = greville(deg[2], knots[2]) * size[2]
     Z, Y, X = np.meshgrid(gz, gy, gx, indexing="ij")
     coords = np.stack([X.ravel(), Y.ravel(), Z.ravel()], axis=1)
     if perturb:
@@ -271,26 +271,25 @@ def bsp_block(ne=(32, 32, 8), p=3, size=(1.0, 1.0, 1.0), E=1.0, nu=0.3,
                  & (k > 0) & (k < nb[2] - 1))
         d = rng.uniform(-perturb, perturb, size=coords.shape) * h
         coords[inner] += d[inner]
-    exts = [bezier_extraction(p, U) for U in knots]
+    exts = [bezier_extraction(deg[d], knots[d]) for d in range(3)]
     nel = ne[0] * ne[1] * ne[2]
     span = np.zeros((nel, 3), dtype=np.int32)
-    q = p + 1
-    conn = np.zeros((nel, q ** 3), dtype=np.int32)
-    ext = np.zeros((nel, 3, q, q))
+    q = [d + 1 for d in deg]
+    conn = np.zeros((nel, q[0] * q[1] * q[2]), dtype=np.int32)
+    # packed per direction: (p_d + 1)^2 row-major each
+    ext = np.zeros((nel, sum(x * x for x in q)))
     e = 0
-    li = np.arange(q)
     for st in range(ne[2]):
         for ss in range(ne[1]):
             for sr in range(ne[0]):
                 span[e] = (sr, ss, st)
                 # basis index of span s for open knots with simple interior
                 # knots is s (bspbasis.cpp:59-62: FindSpan(s) - p)
-                kk, jj, ii = np.meshgrid(st + li, ss + li, sr + li,
-                                         indexing="ij")
+                kk, jj, ii = np.meshgrid(st + np.arange(q[2]), ss + np.arange(q[1]),
+                                         sr + np.arange(q[0]), indexing="ij")
                 conn[e] = (ii + nb[0] * (jj + nb[1] * kk)).ravel()
-                ext[e, 0] = exts[0][sr]
-                ext[e, 1] = exts[1][ss]
-                ext[e, 2] = exts[2][st]
+                ext[e] = np.concatenate([exts[0][sr].ravel(), exts[1][ss].ravel(),
+                                         exts[2][st].ravel()])
                 e += 1
     nnode = coords.shape[0]
     fixed = np.zeros((nnode, 3), dtype=bool)

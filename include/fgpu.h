@@ -47,13 +47,15 @@ typedef struct {
   int nnode, nelem;
   int shape;            /* FGPU_HEX8 | FGPU_BSP */
   int nn;               /* nodes per element: 8, or (p+1)^3 */
-  int deg[3];           /* FGPU_BSP: degree per direction */
+  int deg[3];           /* FGPU_BSP: degree per direction: (3,3,3) or (3,3,2) */
   int nip[3];           /* Gauss points per direction (intpoint.cpp:579-617) */
   int form;             /* FGPU_LINEAR | FGPU_TL_SVK | FGPU_TL_NEOHOOKE */
   const double *coords; /* [nnode*3] */
   const int *conn;      /* [nelem*nn] 0-based, reference local node order */
   const int *dofmap;    /* [nnode*3] 1-based equation or 0 */
-  const double *ext;    /* FGPU_BSP: [nelem*3*(p+1)^2] extraction rows */
+  const double *ext;    /* FGPU_BSP: per element, per direction d the
+                           (p_d+1) x (p_d+1) extraction matrix, row-major,
+                           packed: [nelem * sum_d (p_d+1)^2] */
   const double *cpw;    /* FGPU_BSP: [nnode] weights (NULL = all 1) */
   const double *mat;    /* [nelem*2] */
   /* Batch of independent meshes (0 / NULL for one mesh): mesh k owns nodes

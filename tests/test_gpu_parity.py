@@ -83,11 +83,12 @@ def test_bsp_element_matrices_vs_reference_fixture(fg, oracle_mod, form):
         assert nrel(np.diag(Kg[e]), d["Kdiag"][e]) <= 1e-6
 
 
+@pytest.mark.parametrize("deg", [3, (3, 3, 2)])
 @pytest.mark.parametrize("form,matp", [("tl_svk", (2.6, 0.3)), ("tl_neohooke", (1.0, 1.5))])
-def test_bsp_total_lagrangian_vs_oracle(fg, oracle_mod, form, matp):
+def test_bsp_total_lagrangian_vs_oracle(fg, oracle_mod, form, matp, deg):
     """B-spline total-Lagrangian tangent (vector-ALU kernel) and internal
     force against the FP64 oracle on a perturbed, deformed patch."""
-    m = M.bsp_block(ne=(3, 2, 2), p=3, size=(1.0, 0.7, 0.5), form=form, perturb=0.08, seed=5,
+    m = M.bsp_block(ne=(3, 2, 2), p=deg, size=(1.0, 0.7, 0.5), form=form, perturb=0.08, seed=5,
                     mat=np.tile(matp, (12, 1)))
     X = m.coords
     u = np.stack([0.04 * X[:, 0] * X[:, 2], 0.03 * X[:, 1] ** 2, -0.05 * X[:, 0] ** 2], 1)
@@ -100,17 +101,23 @@ def test_bsp_total_lagrangian_vs_oracle(fg, oracle_mod, form, matp):
     assert nrel(fi, f0) <= 1e-11
 
 
-def test_bsp_element_matrices_vs_oracle_perturbed_patch(fg, oracle_mod):
-    m = M.bsp_block(ne=(4, 3, 2), p=3, size=(1.0, 0.7, 0.4), E=2.6, nu=0.3, perturb=0.08, seed=21)
-    worst, _, _ = elem_check(fg, oracle_mod, m, tol=1e-6)
-    print("bsp worst element rel err", worst)
+@pytest.mark.parametrize("deg", [3, (3, 3, 2)])
+def test_bsp_element_matrices_vs_oracle_perturbed_patch(fg, oracle_mod, deg):
+    m = M.bsp_block(ne=(4, 3, 2), p=deg, size=(1.0, 0.7, 0.4), E=2.6, nu=0.3, perturb=0.08,
+                    seed=21)
+    worst, g, om = elem_check(fg, oracle_mod, m, tol=1e-6)
+    print("bsp", deg, "worst element rel err", worst)
+    X = m.coords
+    u = np.stack([0.01 * X[:, 0] * X[:, 2], 0.02 * X[:, 1], -0.01 * X[:, 0] ** 2], 1)
+    f0, ok = om.assemble_fint(u)
+    assert nrel(g.int_force(u), f0) <= 1e-11
 
 
-@pytest.mark.parametrize("kind", ["hex8_linear", "hex8_tl", "hex8_nh", "bsp"])
+@pytest.mark.parametrize("kind", ["hex8_linear", "hex8_tl", "hex8_nh", "bsp", "bsp332"])
 def test_assembly_pattern_bitexact_and_values(fg, oracle_mod, kind):
     rng = np.random.default_rng(8)
-    if kind == "bsp":
-        m = M.bsp_block(ne=(3, 3, 2), p=3, perturb=0.05, seed=2)
+    if kind in ("bsp", "bsp332"):
+        m = M.bsp_block(ne=(3, 3, 2), p=3 if kind == "bsp" else (3, 3, 2), perturb=0.05, seed=2)
         u = None
     else:
         form = {"hex8_linear": "linear", "hex8_tl": "tl_svk", "hex8_nh": "tl_neohooke"}[kind]

@@ -293,3 +293,39 @@ def test_pcg_matches_textbook_and_direct(oracle_mod):
     assert k == it
     us, ok = oracle_mod.skyline_solve(rowptr, col, val, f)
     assert ok and np.linalg.norm(us - ud) <= 1e-10 * np.linalg.norm(ud)
+
+
+def test_bsp_mixed_degree_element_invariants(oracle_mod):
+    """Degree (3, 3, 2) B-spline solid (the reference's Scordelis-Lo roof
+    uses it): symmetric K_e, six rigid-body modes, constant-strain patch."""
+    m = M.bsp_block(ne=(2, 2, 1), p=(3, 3, 2), E=2.0, nu=0.3, perturb=0.1, seed=7)
+    assert m.nn == 48
+    om = oracle_mod.OracleMesh(m)
+    X = m.coords
+    for e in range(m.nelem):
+        K, ok = om.elem_stiff(e)
+        assert ok and np.allclose(K, K.T, rtol=0, atol=1e-12 * np.abs(K).max())
+        Xe = X[m.conn[e]]
+        R = []
+        for d in range(3):
+            t = np.zeros_like(Xe)
+            t[:, d] = 1.0
+            R.append(t.ravel())
+        for a, b in ((0, 1), (0, 2), (1, 2)):
+            r = np.zeros_like(Xe)
+            r[:, a], r[:, b] = -Xe[:, b], Xe[:, a]
+            R.append(r.ravel())
+        for r in R:
+            assert np.linalg.norm(K @ r) <= 1e-9 * np.abs(K).max() * np.linalg.norm(r)
+    # linear field u = G x is reproduced: f_int equals the boundary traction
+    # of the constant stress, so interior control points see zero force
+    G = np.array([[1e-3, 2e-4, 0.0], [0.0, -5e-4, 1e-4], [3e-4, 0.0, 2e-4]])
+    u = X @ G.T
+    g, ok = om.assemble_fint(u)
+    nb = m.nbas
+    i = np.arange(m.nnode) % nb[0]
+    j = (np.arange(m.nnode) // nb[0]) % nb[1]
+    k = np.arange(m.nnode) // (nb[0] * nb[1])
+    inner = (i > 0) & (i < nb[0] - 1) & (j > 0) & (j < nb[1] - 1) & (k > 0) & (k < nb[2] - 1)
+    fn = m.scatter(g)
+    assert inner.sum() == 0 or np.abs(fn[inner]).max() <= 1e-12 * np.abs(fn).max()
