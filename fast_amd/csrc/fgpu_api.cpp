@@ -328,13 +328,23 @@ float elapsed(Ctx &c, int a, int b) {
   return ms;
 }
 
+// status[4] of the element kernels (count_jacobian): a zero or non-finite
+// Jacobian, or negative Jacobians at some but not all points of the mesh
+// (inverted elements), fail; all negative is a left-handed parametrization,
+// integrated with |det J|.
 int check_status(Ctx &c, const char *what) {
-  int st[2];
+  int st[4];
   FGPU_CHECK(hipMemcpyAsync(st, c.d_status, sizeof st, hipMemcpyDeviceToHost, c.stream));
   FGPU_CHECK(hipStreamSynchronize(c.stream));
-  if (st[0]) {
-    c.err = std::string(what) + ": " + std::to_string(st[0]) +
-            " integration points with non-positive Jacobian";
+  const long long npts = (long long)c.nelem * c.nip[0] * c.nip[1] * c.nip[2];
+  if (st[3]) {
+    c.err = std::string(what) + ": " + std::to_string(st[3]) +
+            " integration points with a zero or non-finite Jacobian";
+    return 0;
+  }
+  if (st[0] && st[0] != npts) {
+    c.err = std::string(what) + ": " + std::to_string(st[0]) + " of " + std::to_string(npts) +
+            " integration points with a negative Jacobian (inverted elements)";
     return 0;
   }
   if (st[1]) {
@@ -399,8 +409,8 @@ void autotune(Ctx &c, const double *d_u) {
 }
 
 int stiffness_dev(Ctx &c, const double *d_u) {
-  FGPU_CHECK(hipMemsetAsync(c.d_status, 0, 2 * sizeof(int), c.stream));
-  if (!c.tuned) autotune(c, d_u);
+  if (!c.tuned) autotune(c, d_u);  // its launches count into the status: reset after
+  FGPU_CHECK(hipMemsetAsync(c.d_status, 0, 4 * sizeof(int), c.stream));
   FGPU_CHECK(hipEventRecord(c.ev[0], c.stream));
   if (c.shape == FGPU_HEX8)
     launch_hex8_K(c, d_u);
@@ -416,7 +426,7 @@ int stiffness_dev(Ctx &c, const double *d_u) {
 }
 
 int int_force_dev(Ctx &c, const double *d_u, double *d_g) {
-  FGPU_CHECK(hipMemsetAsync(c.d_status, 0, 2 * sizeof(int), c.stream));
+  FGPU_CHECK(hipMemsetAsync(c.d_status, 0, 4 * sizeof(int), c.stream));
   FGPU_CHECK(hipEventRecord(c.ev[3], c.stream));
   if (c.shape == FGPU_HEX8)
     launch_hex8_fint(c, d_u);
@@ -502,7 +512,7 @@ int fgpu_create(fgpu_ctx **out, const fgpu_mesh *m) {
     }
     c->d_gauss64 = dupload(gp, 3 * 2 * 16, s);
     c->d_gauss = dupload(gpf, 3 * 2 * 16, s);
-    c->d_status = dalloc<int>(2);
+    c->d_status = dalloc<int>(4);
     build_maps(*c, m->conn);
     int nsym = nsym_blocks(c->nn);
     c->kel = KeLayout{c->shape == FGPU_HEX8 ? 64 : 1, nsym};
@@ -719,6 +729,35 @@ int fgpu_get_element_matrices(fgpu_ctx *ctx, int e0, int ne, float *Ke) {
     FGPU_CHECK(hipStreamSynchronize(c.stream));
     (void)hipFree(d);
     return 1;
+  });
+}
+
+int fgpu_body_force_dev(fgpu_ctx *ctx, const double *b_dev, double *f_dev) {
+  return guarded(ctx, [&](Ctx &c) {
+    if (c.nbatch > 0) throw Error("body forces of a batch context are not supported");
+    FGPU_CHECK(hipMemsetAsync(c.d_status, 0, 4 * sizeof(int), c.stream));
+    if (c.shape == FGPU_HEX8)
+      launch_hex8_body(c, b_dev);
+    else
+      launch_bsp_body(c, b_dev);
+    launch_gather_fint(c, f_dev);
+    return check_status(c, "body force");
+  });
+}
+
+int fgpu_body_force(fgpu_ctx *ctx, const double *b, double *f) {
+  return guarded(ctx, [&](Ctx &c) {
+    if (!b || !f) throw Error("null argument");
+    double *db = dupload(b, (size_t)c.nelem * 3, c.stream);
+    double *df = dalloc<double>(std::max(c.neq, 1));
+    int ok = fgpu_body_force_dev(ctx, db, df);
+    if (ok) {
+      FGPU_CHECK(hipMemcpyAsync(f, df, sizeof(double) * c.neq, hipMemcpyDeviceToHost, c.stream));
+      FGPU_CHECK(hipStreamSynchronize(c.stream));
+    }
+    (void)hipFree(db);
+    (void)hipFree(df);
+    return ok;
   });
 }
 

@@ -32,6 +32,24 @@ __host__ __device__ inline int sym_index(int a, int b, int nn) {
   return a * nn - (a * (a - 1)) / 2 + (b - a);
 }
 
+// Jacobian bookkeeping of the element kernels, status[4] on the device:
+// [0] points with det J < 0, [1] failed stress updates, [3] points with
+// det J = 0 or not finite.  A mesh whose points all have det J < 0 (a
+// left-handed parametrization) is accepted and integrated with |det J|; some
+// negative points among positive ones mean inverted elements and are an
+// error.  (The reference integrates with the signed det J: its stiffness and
+// body forces then both change sign, so its displacements under body forces
+// agree with these, and under nodal or face loads they come out negated.)
+#ifdef __HIPCC__
+template <class T>
+__device__ inline void count_jacobian(int *status, T det, int n = 1) {
+  if (det < T(0))
+    atomicAdd(&status[0], n);
+  else if (!(det > T(0)))
+    atomicAdd(&status[3], n);
+}
+#endif
+
 // Element-matrix storage: tiles of T elements; inside a tile, block-major:
 //   Ke[((e / T) * nsym + blk) * T * 9 + (e % T) * 9 + (3 i + j)]
 // so that one element's 3x3 block is 9 contiguous floats (the assembly
@@ -140,6 +158,8 @@ void launch_hex8_K(Ctx &c, const double *d_unodal);
 void launch_hex8_fint(Ctx &c, const double *d_unodal);
 void launch_bsp_K(Ctx &c, const double *d_unodal);
 void launch_bsp_fint(Ctx &c, const double *d_unodal);
+void launch_hex8_body(Ctx &c, const double *d_b);   // element vectors into d_ge
+void launch_bsp_body(Ctx &c, const double *d_b);
 void launch_assemble_K(Ctx &c);
 std::vector<int> hex8_te_options(const Ctx &c);     // valid hex8 K workgroup sizes
 std::vector<int> assembly_options(const Ctx &c);    // valid asm_mode values

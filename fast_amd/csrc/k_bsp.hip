@@ -202,7 +202,9 @@ __device__ inline void point_geometry(const Geo<NB2> &g, int q, int part, const 
       for (int l = 0; l < 3; l++) J[i][l] += __shfl_xor(J[i][l], m);
   P.detJ = inv3d(J, P.Ji);
   const int qr = q % 4, qs = (q / 4) % 4, qt = q / 16;
-  P.c = gp[(0 * 16 + qr) * 2 + 1] * gp[(1 * 16 + qs) * 2 + 1] * gp[(2 * 16 + qt) * 2 + 1] * P.detJ;
+  // |det J|: a mesh parametrized left-handed throughout is accepted
+  P.c = gp[(0 * 16 + qr) * 2 + 1] * gp[(1 * 16 + qs) * 2 + 1] * gp[(2 * 16 + qt) * 2 + 1] *
+        fabs(P.detJ);
 }
 
 template <int NB2>
@@ -232,7 +234,7 @@ __global__ __launch_bounds__(256, 3) void bsp_K_kernel(
     point_geometry(S.g, q, part, gp, P);
     if (part == 0) {
       S.c[q] = (float)P.c;
-      if (!(P.detJ > 0.0)) atomicAdd(&status[0], 1);
+      count_jacobian(status, P.detJ);
     }
     #pragma unroll 2
     for (int k = 0; k < NPART; k++) {
@@ -414,7 +416,7 @@ __global__ __launch_bounds__(256) void bsp_Kt_kernel(
         pp[8] = (float)(P.c * mu);
       }
     }
-    if (part == 0 && !(P.detJ > 0.0)) atomicAdd(&status[0], 1);
+    if (part == 0) count_jacobian(status, P.detJ);
     #pragma unroll 2
     for (int k = 0; k < NPART; k++) {
       const int a = part * NPART + k;
@@ -537,7 +539,7 @@ __global__ __launch_bounds__(256) void bsp_fint_kernel(
       for (int i = 0; i < 3; i++)
         for (int j = 0; j < 3; j++) L[i][j] += __shfl_xor(L[i][j], m);
     if (part == 0) {
-      if (!(P.detJ > 0.0)) atomicAdd(&status[0], 1);
+      count_jacobian(status, P.detJ);
       for (int i = 0; i < 3; i++)
         for (int j = 0; j < 3; j++) S.Ji[q][3 * i + j] = P.Ji[i][j];
       S.rs[q][0] = P.inv_s;
@@ -618,6 +620,57 @@ __global__ __launch_bounds__(256) void bsp_fint_kernel(
 }
 
 // dynamic LDS above 64 KB must be allowed per kernel (cheap; set per launch)
+// ---------------------------------------------------------------- body force
+// cElemLoad::ExtForce with cBodyLoad (load/load.cpp:618-691, 969-974):
+// f_a = sum_q w detJ R_a b.  Phase 1, thread (q, part): the point's w detJ
+// and rational denominator; phase 2, thread (a, quarter): sum over 16 points.
+template <int NB2>
+struct BShared {
+  Geo<NB2> g;
+  double c[NQP], inv_s[NQP];
+};
+
+template <int NB2>
+__global__ __launch_bounds__(256) void bsp_body_kernel(
+    int nelem, const double *__restrict__ coords, const int *__restrict__ conn,
+    const double *__restrict__ cpw, const double *__restrict__ ext,
+    const double *__restrict__ bvec, const double *__restrict__ gp, double *__restrict__ ge,
+    int *__restrict__ status) {
+  constexpr int NBAS = Bsp<NB2>::NBAS;
+  extern __shared__ __align__(16) unsigned char smem[];
+  BShared<NB2> &S = *reinterpret_cast<BShared<NB2> *>(smem);
+  const int e = blockIdx.x;
+  const int tid = threadIdx.x;
+  geo_load(S.g, e, coords, conn, cpw, ext, gp);
+  __syncthreads();
+  {
+    const int q = tid / 4, part = tid % 4;
+    PointGeo P;
+    point_geometry(S.g, q, part, gp, P);
+    if (part == 0) {
+      S.c[q] = P.c;
+      S.inv_s[q] = P.inv_s;
+      count_jacobian(status, P.detJ);
+    }
+  }
+  __syncthreads();
+  if (tid < 4 * NBAS) {  // whole waves
+    const int a = tid / 4, quarter = tid % 4;
+    double sN = 0.0;
+    for (int q = quarter * 16; q < quarter * 16 + 16; q++) {
+      double n, d[3];
+      tensor(S.g, q, a, n, d);
+      sN += S.c[q] * (n * S.inv_s[q]);
+    }
+    for (int m = 1; m < 4; m <<= 1) sN += __shfl_xor(sN, m);
+    if (quarter == 0) {
+      const double *b = bvec + 3 * (size_t)e;
+      double *o = ge + ((size_t)e * NBAS + a) * 3;
+      for (int i = 0; i < 3; i++) o[i] = sN * b[i];
+    }
+  }
+}
+
 template <class K>
 void set_lds(K kernel, size_t lds) {
   FGPU_CHECK(hipFuncSetAttribute((const void *)kernel,
@@ -718,7 +771,22 @@ void launch_fint_nb(Ctx &c, const double *u) {
   }
 }
 
+template <int NB2>
+void launch_body_nb(Ctx &c, const double *b) {
+  const size_t lds = sizeof(BShared<NB2>);
+  hipLaunchKernelGGL(bsp_body_kernel<NB2>, dim3(c.nelem), dim3(256), lds, c.stream, c.nelem,
+                     c.d_coords, c.d_conn, c.d_cpw, c.d_ext, b, c.d_gauss64, c.d_ge, c.d_status);
+  FGPU_CHECK(hipGetLastError());
+}
+
 }  // namespace
+
+void launch_bsp_body(Ctx &c, const double *b) {
+  if (c.deg[2] == 2)
+    launch_body_nb<3>(c, b);
+  else
+    launch_body_nb<4>(c, b);
+}
 
 void launch_bsp_K(Ctx &c, const double *u) {
   if (c.form != FGPU_LINEAR && !u) {

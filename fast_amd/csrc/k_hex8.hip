@@ -119,12 +119,12 @@ __global__ __launch_bounds__(256) void hex8_K_kernel(
     }
     float Ji[3][3];
     float det = inv3(J, Ji);
-    if (!(det > 0.f)) atomicAdd(&status[0], 1);
+    count_jacobian(status, det);
     float g[8][3];
     for (int a = 0; a < 8; a++)
       for (int i = 0; i < 3; i++)
         g[a][i] = Ji[i][0] * dN[a][0] + Ji[i][1] * dN[a][1] + Ji[i][2] * dN[a][2];
-    const float c = w * det;
+    const float c = w * fabsf(det);  // |det J|: a left-handed mesh is accepted whole
     if constexpr (FORM == FGPU_LINEAR) {
       for (int a = 0; a < 8; a++)
         for (int i = 0; i < 3; i++) S[(3 * a + i) * TE] = g[a][i];
@@ -304,7 +304,7 @@ __global__ __launch_bounds__(128) void hex8_fint_kernel(
     }
   }
   double p0 = mat[2 * (size_t)e], p1 = mat[2 * (size_t)e + 1];
-  int bad = 0, fail = 0;
+  int neg = 0, zero = 0, fail = 0;
   for (int q = 0; q < NIP * NIP * NIP; q++) {
     const int ir = q % NIP, is = (q / NIP) % NIP, it = q / (NIP * NIP);
     const double r = gp[(0 * 16 + ir) * 2], s = gp[(1 * 16 + is) * 2], t = gp[(2 * 16 + it) * 2];
@@ -318,8 +318,9 @@ __global__ __launch_bounds__(128) void hex8_fint_kernel(
     }
     double Ji[3][3];
     double det = inv3(J, Ji);
-    if (!(det > 0.0)) bad++;
-    const double c = w * det;
+    neg += det < 0.0;
+    zero += !(det > 0.0) && !(det < 0.0);
+    const double c = w * fabs(det);
     // L = grad u (L_ij = d u_i / d X_j)
     double Lg[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
     for (int a = 0; a < 8; a++) {
@@ -377,11 +378,60 @@ __global__ __launch_bounds__(128) void hex8_fint_kernel(
       for (int i = 0; i < 3; i++) g[a][i] += c * (P[i][0] * G[0] + P[i][1] * G[1] + P[i][2] * G[2]);
     }
   }
-  if (bad) atomicAdd(&status[0], bad);
+  if (neg) atomicAdd(&status[0], neg);
+  if (zero) atomicAdd(&status[3], zero);
   if (fail) atomicAdd(&status[1], fail);
   double *out = ge + (size_t)e * 24;
   for (int a = 0; a < 8; a++)
     for (int i = 0; i < 3; i++) out[3 * a + i] = g[a][i];
+}
+
+// Body-force element vectors (cElemLoad::ExtForce with cBodyLoad, load/
+// load.cpp:618-691, 969-974): f_a = sum_q w detJ N_a b, one thread per
+// element, FP64.  b: [nelem * 3].
+template <int NIP>
+__global__ __launch_bounds__(128) void hex8_body_kernel(int nelem, const double *__restrict__ coords,
+                                                        const int *__restrict__ conn,
+                                                        const double *__restrict__ bvec,
+                                                        const double *__restrict__ gp,
+                                                        double *__restrict__ ge,
+                                                        int *__restrict__ status) {
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= nelem) return;
+  const int *cn = conn + (size_t)e * 8;
+  double X[8][3];
+  const int n0 = cn[0];
+  for (int a = 0; a < 8; a++)
+    for (int k = 0; k < 3; k++) X[a][k] = coords[3 * (size_t)cn[a] + k] - coords[3 * (size_t)n0 + k];
+  double sN[8] = {0, 0, 0, 0, 0, 0, 0, 0};  // sum_q w detJ N_a
+  int neg = 0, zero = 0;
+  for (int q = 0; q < NIP * NIP * NIP; q++) {
+    const int ir = q % NIP, is = (q / NIP) % NIP, it = q / (NIP * NIP);
+    const double r = gp[(0 * 16 + ir) * 2], s = gp[(1 * 16 + is) * 2], t = gp[(2 * 16 + it) * 2];
+    const double w = gp[(0 * 16 + ir) * 2 + 1] * gp[(1 * 16 + is) * 2 + 1] * gp[(2 * 16 + it) * 2 + 1];
+    double J[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
+    for (int a = 0; a < 8; a++) {
+      double d[3];
+      hex8_dN(r, s, t, a, d[0], d[1], d[2]);
+      for (int i = 0; i < 3; i++)
+        for (int k = 0; k < 3; k++) J[i][k] += d[i] * X[a][k];
+    }
+    double Ji[3][3];
+    const double det = inv3(J, Ji);
+    neg += det < 0.0;
+    zero += !(det > 0.0) && !(det < 0.0);
+    const double c = w * fabs(det);
+    for (int a = 0; a < 8; a++) {
+      const double N = 0.125 * (1.0 + kNodeR[a] * r) * (1.0 + kNodeS[a] * s) * (1.0 + kNodeT[a] * t);
+      sN[a] += c * N;
+    }
+  }
+  if (neg) atomicAdd(&status[0], neg);
+  if (zero) atomicAdd(&status[3], zero);
+  const double *b = bvec + 3 * (size_t)e;
+  double *out = ge + (size_t)e * 24;
+  for (int a = 0; a < 8; a++)
+    for (int i = 0; i < 3; i++) out[3 * a + i] = sN[a] * b[i];
 }
 
 template <int FORM, int NIP, int TE>
@@ -464,6 +514,21 @@ void launch_hex8_K(Ctx &c, const double *u) {
     case FGPU_LINEAR: launch_K_f<FGPU_LINEAR>(c, u); break;
     case FGPU_TL_SVK: launch_K_f<FGPU_TL_SVK>(c, u); break;
     default: launch_K_f<FGPU_TL_NEOHOOKE>(c, u); break;
+  }
+}
+
+template <int NIP>
+static void launch_body_t(Ctx &c, const double *b) {
+  hipLaunchKernelGGL((hex8_body_kernel<NIP>), dim3((c.nelem + 127) / 128), dim3(128), 0, c.stream,
+                     c.nelem, c.d_coords, c.d_conn, b, c.d_gauss64, c.d_ge, c.d_status);
+  FGPU_CHECK(hipGetLastError());
+}
+
+void launch_hex8_body(Ctx &c, const double *b) {
+  switch (c.nip[0]) {
+    case 1: launch_body_t<1>(c, b); break;
+    case 2: launch_body_t<2>(c, b); break;
+    default: launch_body_t<3>(c, b); break;
   }
 }
 

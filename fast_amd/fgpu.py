@@ -82,6 +82,8 @@ EXPORTS = {
     "fgpu_get_csr": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),
     "fgpu_set_csr_values": (C.c_int, [C.c_void_p, C.c_void_p]),
     "fgpu_get_element_matrices": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_void_p]),
+    "fgpu_body_force": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p]),
+    "fgpu_body_force_dev": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p]),
     "fgpu_set_launch": (C.c_int, [C.c_void_p, C.c_int, C.c_int]),
     "fgpu_tuning_report": (C.c_char_p, [C.c_void_p]),
     "fgpu_last_time_ms": (C.c_double, [C.c_void_p, C.c_int]),
@@ -293,6 +295,14 @@ class GpuModel:
         self._check(lib().fgpu_get_element_matrices(self.h, e0, ne, _p(K)),
                     "fgpu_get_element_matrices")
         return K
+
+    def body_force(self, b_elem):
+        """Load vector of uniform body forces b_elem [nelem, 3]."""
+        b = np.ascontiguousarray(b_elem, dtype=np.float64)
+        assert b.shape == (self.mesh.nelem, 3)
+        f = np.zeros(self.neq)
+        self._check(lib().fgpu_body_force(self.h, _p(b), _p(f)), "fgpu_body_force")
+        return f
 
     def set_launch(self, hex8_te=0, asm_mode=-1):
         """Fix the launch configuration (replaces the autotuner)."""

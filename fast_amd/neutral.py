@@ -1,0 +1,396 @@
+"""Reader for the part of FAST's neutral input file (``.dat``) that this path
+runs, and writer of its nodal-displacement result section (``.pos``).
+
+Supported: solid 8-node bricks (``ELEMENT.BRICK8`` / ``ELEMENT.BRICK8TL``) and
+B-spline solids of degree (3,3,3) or (3,3,2) (``ELEMENT.BSPLINE.SOLID`` /
+``...SOLIDTL``) with isotropic elasticity (``MATERIAL.ISOTROPIC`` through
+``SECTION.HOMOGENEOUS.ISOTROPIC.3D``), supports, the solver node order,
+uniform face loads on bricks in global axes, body forces and nodal forces,
+linear static analysis.  Any other keyword that changes the analysis raises
+``NotImplementedError`` naming it; keywords that only steer printing are
+skipped.
+
+Formats follow the reference's readers (main/input.cpp:70-330 keyword table):
+NODE.COORD / NODE.SUPPORT / CONTROL.POINT.WEIGHT / NODE.SOLVER.ORDER
+(node/node.cpp:60-280), MATERIAL.ISOTROPIC (mat/material.cpp:824-831),
+INTEGRATION.ORDER (elc/element.cpp:194-228), element lines
+(elc/elmparam.cpp:221-245: label, section, order, then the shape's incidence;
+shp/shpbsp.cpp:57-110 for B-spline patch + spans), PATCH
+(geom/patch.cpp:229-264, 1460-1484; knot vectors geom/knotvec.cpp:93-175;
+control points shp/shpiga.cpp:174-215), face loads (load/load.cpp:1440-1480),
+body forces (load/load.cpp:944-966).  Equation numbering follows the solver
+node order (ctrl/ctrl.cpp:2004-2104, node/node.cpp:248-285).
+"""
+from __future__ import annotations
+
+import dataclasses
+import re
+from typing import Dict, List, Optional
+
+import numpy as np
+
+from .mesh import Mesh, bezier_extraction
+
+# keywords that only control printing / output; their data are skipped
+_OUTPUT_ONLY = {
+    "HEADER", "END", "MATERIAL.LABEL", "PATCH.OUTPUT.POINT", "LOAD", "LOAD.CASE",
+    "HEADER.OUTPUT.PRECISION", "HEADER.PRINT.SUPPORT.REACTIONS",
+    "HEADER.OUTPUT.STRESS.EXTRAPOLATION", "HEADER.PRINT.EQUIVALENT.MESH",
+    "HEADER.EQUIVALENT.MESH.DIVISION", "HEADER.OUTPUT.PATCH.STRESS.SMOOTH", "NODE",
+    "ELEMENT", "MATERIAL", "SECTION", "HEADER.ANALYSIS.PRINT.STEPS",
+    # the solver kind: every system here is solved by PCG to a tight tolerance
+    "HEADER.ANALYSIS.MATRIX.TYPE",
+}
+
+_HEX8_FACES = ((0, 1, 2, 3), (4, 5, 6, 7), (0, 1, 5, 4), (3, 2, 6, 7), (0, 3, 7, 4),
+               (1, 2, 6, 5))  # cyclic corner lists of the reference's Brick8
+
+
+@dataclasses.dataclass
+class Deck:
+    """A parsed deck: the mesh handed to the device, the load vector in
+    equation order, and the node labels (row i of every nodal array is the
+    node labelled labels[i])."""
+    mesh: Mesh
+    load: np.ndarray
+    labels: np.ndarray
+    title: str = ""
+    analysis: str = "linear"              # "linear" or "newton" (load control)
+    body: Optional[np.ndarray] = None     # [nelem, 3] body forces (force / volume)
+    steps: int = 1                        # Newton: MAXIMUM.STEPS, STEP.FACTOR,
+    factor: float = 1.0                   # MAXIMUM.ITERATIONS, TOLERANCE
+    maxit: int = 20
+    tol: float = 1e-5
+
+
+def _sections(text: str) -> Dict[str, List[str]]:
+    """Keyword -> whitespace tokens of its data (a quoted label is one token)."""
+    out: Dict[str, List[str]] = {}
+    cur = None
+    for line in text.splitlines():
+        s = line.strip()
+        if s.startswith("%"):
+            cur = s[1:].split()[0].upper()
+            if cur in out and cur not in _OUTPUT_ONLY:
+                raise ValueError("keyword %%%s given twice" % cur)
+            out[cur] = []
+            continue
+        if cur is not None:
+            out[cur] += re.findall(r"'[^']*'|\S+", s)
+    return out
+
+
+class _Toks:
+    def __init__(self, toks: List[str], key: str):
+        self.t, self.i, self.key = toks, 0, key
+
+    def next(self):
+        if self.i >= len(self.t):
+            raise ValueError("%%%s: data ended early" % self.key)
+        self.i += 1
+        return self.t[self.i - 1]
+
+    def int(self):
+        return int(self.next())
+
+    def float(self):
+        return float(self.next())
+
+
+def _knot_vector(tk: _Toks) -> np.ndarray:
+    kind = tk.next().upper()
+    n = tk.int()
+    if kind in ("'GENERAL'", "'G'"):
+        vals = [tk.float() for _ in range(n)]
+    elif kind in ("'UNIFORM'", "'U'"):
+        v0, dv = tk.float(), tk.float()
+        vals = [v0 + dv * k for k in range(n)]
+    else:
+        raise ValueError("unknown knot vector type %s" % kind)
+    mult = [tk.int() for _ in range(n)]
+    return np.repeat(np.asarray(vals, dtype=np.float64), mult)
+
+
+def _span_first_basis(U: np.ndarray, p: int, sid: int) -> int:
+    """First basis function of the sid-th non-empty knot span
+    (geom/knotvec.cpp:303-313, geom/bspbasis.cpp:59-62)."""
+    cur = -1
+    for k in range(1, len(U)):
+        if U[k] - U[k - 1] > 1e-5:
+            cur += 1
+            if cur == sid:
+                return (k - 1) - p
+    raise ValueError("knot span %d does not exist" % (sid + 1))
+
+
+def read(path: str) -> Deck:
+    text = open(path, errors="replace").read()
+    sec = _sections(text)
+    unknown = [k for k in sec if k not in _OUTPUT_ONLY and k not in _HANDLED]
+    if unknown:
+        raise NotImplementedError("keywords outside this path: " + ", ".join(sorted(unknown)))
+    title = " ".join(sec.get("HEADER", []))[:200]
+
+    # nodes
+    tk = _Toks(sec["NODE.COORD"], "NODE.COORD")
+    n = tk.int()
+    lab, xyz = [], []
+    for _ in range(n):
+        lab.append(tk.int())
+        xyz.append([tk.float(), tk.float(), tk.float()])
+    order = np.argsort(lab)
+    labels = np.asarray(lab)[order]
+    coords = np.asarray(xyz, dtype=np.float64)[order]
+    idx = {int(l): i for i, l in enumerate(labels)}
+    nnode = len(labels)
+
+    cpw = np.ones(nnode)
+    if "CONTROL.POINT.WEIGHT" in sec:
+        tk = _Toks(sec["CONTROL.POINT.WEIGHT"], "CONTROL.POINT.WEIGHT")
+        for _ in range(tk.int()):
+            i = idx[tk.int()]
+            cpw[i] = tk.float()
+
+    fixed = np.zeros((nnode, 3), dtype=bool)
+    if "NODE.SUPPORT" in sec:
+        tk = _Toks(sec["NODE.SUPPORT"], "NODE.SUPPORT")
+        for _ in range(tk.int()):
+            i = idx[tk.int()]
+            flags = [tk.int() for _ in range(6)]
+            fixed[i] = np.asarray(flags[:3]) != 0
+
+    # materials, sections, integration orders
+    mats = {}
+    tk = _Toks(sec["MATERIAL.ISOTROPIC"], "MATERIAL.ISOTROPIC")
+    for _ in range(tk.int()):
+        m = tk.int()
+        mats[m] = (tk.float(), tk.float())
+    secs = {}
+    tk = _Toks(sec["SECTION.HOMOGENEOUS.ISOTROPIC.3D"], "SECTION.HOMOGENEOUS.ISOTROPIC.3D")
+    for _ in range(tk.int()):
+        s_id = tk.int()
+        secs[s_id] = tk.int()
+    ords = {}
+    tk = _Toks(sec["INTEGRATION.ORDER"], "INTEGRATION.ORDER")
+    for _ in range(tk.int()):
+        o = tk.int()
+        k = [tk.int() for _ in range(3)]
+        [tk.int() for _ in range(3)]
+        ords[o] = tuple(k)
+
+    # elements (labels 1..nelem)
+    kinds = [k for k in ("ELEMENT.BRICK8", "ELEMENT.BRICK8TL", "ELEMENT.BSPLINE.SOLID",
+                         "ELEMENT.BSPLINE.SOLIDTL") if k in sec]
+    if len(kinds) != 1:
+        raise NotImplementedError("one element type per deck: found %s" % kinds)
+    kind = kinds[0]
+    form = "tl_svk" if kind.endswith("TL") else "linear"
+    tk = _Toks(sec[kind], kind)
+    ne = tk.int()
+    rows = []
+    for _ in range(ne):
+        lab_e, s_id, o_id = tk.int(), tk.int(), tk.int()
+        if kind.startswith("ELEMENT.BRICK8"):
+            inc = [idx[tk.int()] for _ in range(8)]
+        else:
+            inc = [tk.int() for _ in range(4)]  # patch, span r, s, t (1-based)
+        rows.append((lab_e, s_id, o_id, inc))
+    rows.sort(key=lambda r: r[0])
+    mat = np.asarray([mats[secs[r[1]]] for r in rows], dtype=np.float64)
+    nips = {ords[r[2]] for r in rows}
+    if len(nips) != 1:
+        raise NotImplementedError("one integration order per deck")
+    nip = nips.pop()
+
+    if kind.startswith("ELEMENT.BRICK8"):
+        conn = np.asarray([r[3] for r in rows], dtype=np.int32)
+        mesh_kw = dict(shape="hex8")
+        ext = None
+        deg = (1, 1, 1)
+    else:
+        conn, ext, deg = _bspline_elements(sec, rows, idx)
+        mesh_kw = dict(shape="bsp", cpw=cpw)
+
+    # equation numbering in solver order: nodes sorted by their rank
+    rank = np.arange(nnode)
+    if "NODE.SOLVER.ORDER" in sec:
+        tk = _Toks(sec["NODE.SOLVER.ORDER"], "NODE.SOLVER.ORDER")
+        if tk.int() != nnode:
+            raise ValueError("NODE.SOLVER.ORDER must list every node")
+        rank = np.asarray([tk.int() for _ in range(nnode)])  # rank of node i (label order)
+    dofmap = _number_dofs(conn, fixed, nnode, np.argsort(rank, kind="stable"))
+    mesh = Mesh(coords=coords, conn=conn, nip=nip, form=form, mat=mat, fixed=fixed, deg=deg,
+                ext=ext, dofmap=dofmap, **mesh_kw)
+
+    # loads
+    f = np.zeros(mesh.neq)
+    body = None
+    if "LOAD.CASE.NODAL.FORCE" in sec:
+        tk = _Toks(sec["LOAD.CASE.NODAL.FORCE"], "LOAD.CASE.NODAL.FORCE")
+        for _ in range(tk.int()):
+            i = idx[tk.int()]
+            v = [tk.float() for _ in range(6)]
+            for d in range(3):
+                q = dofmap[i, d]
+                if q > 0:
+                    f[q - 1] += v[d]
+    if "LOAD.CASE.AREA.UNIFORM" in sec:
+        if mesh.shape != "hex8":
+            raise NotImplementedError("face loads on B-spline solids")
+        f += _face_loads(mesh, sec["LOAD.CASE.AREA.UNIFORM"], idx)
+    if "LOAD.CASE.BODY.FORCES" in sec:
+        tk = _Toks(sec["LOAD.CASE.BODY.FORCES"], "LOAD.CASE.BODY.FORCES")
+        body = np.zeros((mesh.nelem, 3))
+        for _ in range(tk.int()):
+            e = tk.int() - 1
+            body[e] += [tk.float(), tk.float(), tk.float()]
+    deck = Deck(mesh=mesh, load=f, labels=labels, title=title, body=body)
+    # analysis (ctrl/ctrl.cpp:148-190 ReadAlg and the HEADER.ANALYSIS readers)
+    alg = sec.get("HEADER.ANALYSIS.ALGORITHM", ["'Linear'"])[0].strip("'")
+    if alg in ("Linear", "linear"):
+        deck.analysis = "linear"
+    elif alg in ("LoadControl", "lcm", "NewtonRaphson", "snr"):
+        deck.analysis = "newton"
+        deck.steps = int(sec["HEADER.ANALYSIS.MAXIMUM.STEPS"][0])
+        deck.factor = float(sec["HEADER.ANALYSIS.STEP.FACTOR"][0])
+        deck.maxit = int(sec["HEADER.ANALYSIS.MAXIMUM.ITERATIONS"][0])
+        deck.tol = float(sec["HEADER.ANALYSIS.TOLERANCE"][0])
+    else:
+        raise NotImplementedError("analysis algorithm %s" % alg)
+    return deck
+
+
+_HANDLED = {"HEADER.ANALYSIS.ALGORITHM", "HEADER.ANALYSIS.MAXIMUM.STEPS",
+            "HEADER.ANALYSIS.STEP.FACTOR", "HEADER.ANALYSIS.MAXIMUM.ITERATIONS",
+            "HEADER.ANALYSIS.TOLERANCE",
+            "NODE.COORD", "CONTROL.POINT.WEIGHT", "NODE.SUPPORT", "NODE.SOLVER.ORDER",
+            "MATERIAL.ISOTROPIC", "SECTION.HOMOGENEOUS.ISOTROPIC.3D", "INTEGRATION.ORDER",
+            "ELEMENT.BRICK8", "ELEMENT.BRICK8TL", "ELEMENT.BSPLINE.SOLID",
+            "ELEMENT.BSPLINE.SOLIDTL", "PATCH", "LOAD.CASE.AREA.UNIFORM",
+            "LOAD.CASE.BODY.FORCES", "LOAD.CASE.NODAL.FORCE"}
+
+
+def _number_dofs(conn, fixed, nnode, node_order):
+    """VecDof numbering: active, unsupported directions numbered node by node
+    in solver order, x, y, z (ctrl/ctrl.cpp:2004-2104)."""
+    active = np.zeros((nnode, 3), dtype=bool)
+    active[np.unique(conn.ravel())] = True
+    active &= ~fixed
+    dofmap = np.zeros((nnode, 3), dtype=np.int32)
+    q = 0
+    for i in node_order:
+        for d in range(3):
+            if active[i, d]:
+                q += 1
+                dofmap[i, d] = q
+    return dofmap
+
+
+def _bspline_elements(sec, rows, idx):
+    tk = _Toks(sec["PATCH"], "PATCH")
+    pats = {}
+    for _ in range(tk.int()):
+        pid, ptype = tk.int(), tk.next()
+        if ptype.lower() != "'bspsol'":
+            raise NotImplementedError("patch type %s" % ptype)
+        if tk.int() != 1:
+            raise NotImplementedError("non-rational B-spline patches (the reference's "
+                                      "non-rational solid basis is not evaluated correctly)")
+        bases = []
+        for _d in range(3):
+            p = tk.int()
+            bases.append((p, _knot_vector(tk)))
+        nb = [len(U) - p - 1 for p, U in bases]
+        cps = [idx[tk.int()] for _ in range(nb[0] * nb[1] * nb[2])]
+        pats[pid] = (bases, nb, cps)
+    degs = {tuple(p for p, _ in pats[r[3][0]][0]) for r in rows}
+    if len(degs) != 1:
+        raise NotImplementedError("one B-spline degree per deck")
+    deg = degs.pop()
+    conn, ext = [], []
+    cache = {}
+    for r in rows:
+        pid, span = r[3][0], [s - 1 for s in r[3][1:]]
+        bases, nb, cps = pats[pid]
+        first, cs = [], []
+        for d in range(3):
+            p, U = bases[d]
+            key = (pid, d)
+            if key not in cache:
+                cache[key] = bezier_extraction(p, U)
+            cs.append(cache[key][span[d]])
+            first.append(_span_first_basis(U, p, span[d]))
+        q = [p + 1 for p in deg]
+        c = []
+        for k in range(q[2]):
+            for j in range(q[1]):
+                for i in range(q[0]):
+                    cp = (first[0] + i) + nb[0] * ((first[1] + j) + nb[1] * (first[2] + k))
+                    c.append(cps[cp])
+        conn.append(c)
+        ext.append(np.concatenate([x.ravel() for x in cs]))
+    return np.asarray(conn, dtype=np.int32), np.asarray(ext), deg
+
+
+def _face_loads(mesh, toks, idx):
+    """Uniform face loads on bricks, global axes (cFaceUnifLoad with
+    Local = 0, load/load.cpp:1440-1498; integrated on the element's Gauss
+    order over the bilinear face, cElemLoad::ExtForce, load.cpp:618-691)."""
+    tk = _Toks(toks, "LOAD.CASE.AREA.UNIFORM")
+    f = np.zeros(mesh.neq)
+    n = mesh.nip[0]
+    r, w = np.polynomial.legendre.leggauss(n)
+    for _ in range(tk.int()):
+        e = tk.int() - 1
+        corners = {idx[tk.int()] for _ in range(3)}
+        local = tk.int()
+        qv = np.asarray([tk.float(), tk.float(), tk.float()])
+        if local:
+            raise NotImplementedError("face loads in local axes")
+        cn = mesh.conn[e]
+        face = [fc for fc in _HEX8_FACES if corners <= {int(cn[a]) for a in fc}]
+        if len(face) != 1:
+            raise ValueError("face of element %d not found" % (e + 1))
+        nodes = [int(cn[a]) for a in face[0]]
+        X = mesh.coords[nodes]
+        for i in range(n):
+            for j in range(n):
+                s, t = r[i], r[j]
+                N = 0.25 * np.array([(1 - s) * (1 - t), (1 + s) * (1 - t), (1 + s) * (1 + t),
+                                     (1 - s) * (1 + t)])
+                ds = 0.25 * np.array([-(1 - t), (1 - t), (1 + t), -(1 + t)])
+                dt = 0.25 * np.array([-(1 - s), -(1 + s), (1 + s), (1 - s)])
+                dA = np.linalg.norm(np.cross(ds @ X, dt @ X)) * w[i] * w[j]
+                for a, node in enumerate(nodes):
+                    for d in range(3):
+                        q = mesh.dofmap[node, d]
+                        if q > 0:
+                            f[q - 1] += N[a] * dA * qv[d]
+    return f
+
+
+def write_pos(path: str, labels: np.ndarray, u_nodal: np.ndarray, factor: float = 1.0,
+              digits: int = 16):
+    """Nodal displacements in the result layout of the reference's .pos file
+    (ctrl/ctrl.cpp:1117-1175): one step, label and six components per node."""
+    with open(path, "w") as out:
+        out.write("%RESULT\n1\n1  'result1'\n\n%RESULT.CASE\n1\n\n")
+        out.write("%%RESULT.CASE.STEP\n1\n\n%%RESULT.CASE.STEP.FACTOR\n%.*e\n\n" % (digits, factor))
+        out.write("%%RESULT.CASE.STEP.NODAL.DISPLACEMENT\n%d  'displacement'\n" % len(labels))
+        for lab, u in zip(labels, u_nodal):
+            out.write("%-5d " % lab + "  ".join("%+.*e" % (digits, v) for v in
+                                                  list(u) + [0.0, 0.0, 0.0]) + "  \n")
+        out.write("\n%END\n")
+
+
+def read_pos_displacements(path: str):
+    """Labels and displacements (x, y, z) of the first step of a .pos file."""
+    lines = open(path).read().splitlines()
+    k = next(i for i, l in enumerate(lines) if l.strip() == "%RESULT.CASE.STEP.NODAL.DISPLACEMENT")
+    n = int(lines[k + 1].split()[0])
+    lab, u = [], []
+    for l in lines[k + 2:k + 2 + n]:
+        v = l.split()
+        lab.append(int(v[0]))
+        u.append([float(x) for x in v[1:4]])
+    return np.asarray(lab), np.asarray(u)

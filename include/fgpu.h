@@ -119,7 +119,9 @@ int fgpu_spmv_dev(fgpu_ctx *ctx, const double *x_dev, double *y_dev);
  * (mlib/sysmat.h:60-99) for callers that assemble on their own. */
 int fgpu_set_csr_values(fgpu_ctx *ctx, const double *val);
 /* Read back the CSR (full symmetric pattern, 0-based, rows in equation
- * order).  Any pointer may be NULL to skip it. */
+ * order; within a row the columns follow the node order of the rows'
+ * neighbours, which is ascending equation order unless a solver node order
+ * numbered the equations).  Any pointer may be NULL to skip it. */
 int fgpu_get_csr(fgpu_ctx *ctx, long long *rowptr, int *col, double *val);
 /* Element matrices of elements [e0, e0+ne) as dense row-major
  * (3nn x 3nn) FP32, element-local DOF order (node-major). */
@@ -130,7 +132,12 @@ int fgpu_get_element_matrices(fgpu_ctx *ctx, int e0, int ne, float *Ke);
  * the seam, cf. cLinStat::Solver ctrl/ctrllins.cpp:46-99 and
  * cCtrlNR::Solver ctrl/ctrlnr.cpp:47-267). */
 
-/* Linear static: assemble K, solve K u = f with PCG to rtol (u: neq). */
+/* Linear static: assemble K, solve K u = f to |f - K u| / |f| <= rtol (u:
+ * neq).  K u is evaluated in FP64 element by element (the linear internal
+ * force); PCG on the matrix assembled from FP32 element matrices solves for
+ * the corrections (iterative refinement), so ill-conditioned models reach
+ * FP64 accuracy.  maxit bounds the total PCG iterations (<= 0: 50 neq);
+ * iters returns them, relres the final residual. */
 int fgpu_linear_static(fgpu_ctx *ctx, const double *f, double *u, int maxit,
                        double rtol, int *iters, double *relres);
 
@@ -238,6 +245,13 @@ int fgpu_gnn_train(fgpu_gnn *net, int nstate, const double *states_dev, const do
 int fgpu_gnn_newton_predictor(void *user, int step, const double *u_dev, double lam_prev,
                               double lam, const double *u_hist_dev, double lam_hist,
                               double *du_dev);
+
+/* External load vector of uniform body forces, b [nelem * 3] (force per
+ * unit volume of each element): f = sum_e sum_q w detJ N^T b_e on the
+ * element's Gauss rule (cElemLoad::ExtForce with cBodyLoad, load/load.cpp
+ * 618-705, 969-974).  f: [neq] equation vector. */
+int fgpu_body_force(fgpu_ctx *ctx, const double *b, double *f);
+int fgpu_body_force_dev(fgpu_ctx *ctx, const double *b_dev, double *f_dev);
 
 /* Launch configuration.  On the first stiffness assembly of a context the
  * autotuner times every valid configuration of the element-stiffness kernel

@@ -216,11 +216,13 @@ static void bsp_univariate(int p, const double *C, double t, double *N,
     for (int j = 0; j <= p; j++) dN[i] += C[i * (p + 1) + j] * dB[j];
 }
 
+/* dN: natural derivatives [3 nn]; N (optional): shape values [nn] */
 static void natural_derivs(const orc_etype *et, const double *ext,
-                           const double *w, const double rst[3], double *dN)
+                           const double *w, const double rst[3], double *dN,
+                           double *Nval)
 {
   if (et->shape == ORC_HEX8) {
-    orc_hex8_shape(rst[0], rst[1], rst[2], NULL, dN);
+    orc_hex8_shape(rst[0], rst[1], rst[2], Nval, dN);
     return;
   }
   const int *p = et->deg;
@@ -240,6 +242,7 @@ static void natural_derivs(const orc_etype *et, const double *ext,
           dN[3 * a + 0] = dNr[k] * Ns[j] * Nt[i];
           dN[3 * a + 1] = Nr[k] * dNs[j] * Nt[i];
           dN[3 * a + 2] = Nr[k] * Ns[j] * dNt[i];
+          if (Nval) Nval[a] = Nr[k] * Ns[j] * Nt[i];
         }
     return;
   }
@@ -264,6 +267,8 @@ static void natural_derivs(const orc_etype *et, const double *ext,
     dN[3 * i + 1] = (dN[3 * i + 1] - N[i] * dws) / W;
     dN[3 * i + 2] = (dN[3 * i + 2] - N[i] * dwt) / W;
   }
+  if (Nval)
+    for (int i = 0; i < a; i++) Nval[i] = N[i];
 }
 
 /* Reference: shp/shpsolid.cpp:58-126 (J = dN^T X with rows r,s,t) and
@@ -273,7 +278,7 @@ int orc_shape_dxyz(const orc_etype *et, const double *X, const double *ext,
                    double *dNxyz)
 {
   double dN[3 * MAXNN];
-  natural_derivs(et, ext, w, rst, dN);
+  natural_derivs(et, ext, w, rst, dN, NULL);
   double A[3][3] = {{0}};
   int nn = et->nn;
   for (int i = 0; i < nn; i++) {
@@ -596,6 +601,31 @@ int orc_elem_intforce(const orc_etype *et, const double *X,
   return ok;
 }
 
+/* Element body-force vector: cElemLoad::ExtForce (load/load.cpp:618-691)
+ * with cBodyLoad::GetVal (load.cpp:969-974): per point, coeff = w detJ (the
+ * element's Gauss rule), f += coeff N^T b. */
+int orc_elem_bodyforce(const orc_etype *et, const double *X, const double *ext,
+                       const double *w, const double *b, double *f)
+{
+  int nn = et->nn;
+  double rst[3 * 512], wq[512];
+  int nq = orc_brick_points(et->nip, rst, wq);
+  double Nv[MAXNN], dN[3 * MAXNN], dxyz[3 * MAXNN];
+  memset(f, 0, sizeof(double) * 3 * nn);
+  int ok = 1;
+  for (int q = 0; q < nq; q++) {
+    double detJ;
+    natural_derivs(et, ext, w, rst + 3 * q, dN, Nv);
+    if (!orc_shape_dxyz(et, X, ext, w, rst + 3 * q, &detJ, dxyz)) ok = 0;
+    double coeff = wq[q] * detJ * 1.0;
+    for (int d = 0; d < 3; d++) {
+      double ab = coeff * b[d];
+      for (int a = 0; a < nn; a++) f[3 * a + d] += Nv[a] * ab;
+    }
+  }
+  return ok;
+}
+
 /* ------------------------------------------------------------------------
  * Mesh-level loops.
  * ---------------------------------------------------------------------- */
@@ -694,6 +724,26 @@ int orc_assemble_K(const orc_mesh *m, const double *u_nodal, int neq,
 }
 
 /* Reference: ctrl/ctrl.cpp:719-813 and elc/element.cpp:2953-2971. */
+/* Body-force load vector: element loop in element order, AddGlobVec
+ * (load.cpp:695-705). b: [nelem * 3]. */
+int orc_assemble_body(const orc_mesh *m, const double *b, int neq, double *f)
+{
+  int nn = m->et.nn;
+  size_t es = ext_stride(&m->et);
+  double X[3 * MAXNN], we[MAXNN], fe[3 * MAXNN];
+  int edof[3 * MAXNN];
+  memset(f, 0, sizeof(double) * neq);
+  int ok = 1;
+  for (int e = 0; e < m->nelem; e++) {
+    gather_elem(m, e, NULL, X, NULL, we, edof);
+    const double *ext = m->ext ? m->ext + (size_t)e * es : NULL;
+    if (!orc_elem_bodyforce(&m->et, X, ext, we, b + 3 * (size_t)e, fe)) ok = 0;
+    for (int i = 0; i < 3 * nn; i++)
+      if (edof[i] > 0) f[edof[i] - 1] += fe[i];
+  }
+  return ok;
+}
+
 int orc_assemble_fint(const orc_mesh *m, const double *u_nodal, int neq,
                       double *g, int nthreads, int eb, int ee)
 {
@@ -773,8 +823,8 @@ static void pattern_rows(const orc_mesh *m, int neq, long long *rowptr,
       rowptr[r + 1] = total;
       continue;
     }
-    /* neighbour nodes (sorted, unique), then their active equations:
-     * equations are node-major, so node order gives equation order */
+    /* neighbour nodes (sorted, unique), then their active equations,
+     * sorted (a solver node order numbers nodes out of label order) */
     int k = 0;
     for (int t = cnt[n]; t < cnt[n + 1]; t++) {
       int e = adj[t];
@@ -790,6 +840,7 @@ static void pattern_rows(const orc_mesh *m, int neq, long long *rowptr,
           if (col) col[total + u] = d - 1;
           u++;
         }
+    if (col) qsort(col + total, u, sizeof(int), cmp_int);
     prev_node = n;
     prev_start = total;
     prev_len = u;

@@ -71,6 +71,9 @@ int  orc_elem_intforce(const orc_etype *et, const double *X,
 int  orc_assemble_K(const orc_mesh *m, const double *u_nodal, int neq,
                     const long long *rowptr, const int *col, double *val,
                     int nthreads, int elem_begin, int elem_end);
+int  orc_elem_bodyforce(const orc_etype *et, const double *X, const double *ext,
+                        const double *w, const double *b, double *f);
+int  orc_assemble_body(const orc_mesh *m, const double *b, int neq, double *f);
 int  orc_assemble_fint(const orc_mesh *m, const double *u_nodal, int neq,
                        double *g, int nthreads, int elem_begin, int elem_end);
 /* Full CSR sparsity pattern of the equations (two-pass). */

@@ -53,6 +53,8 @@ def lib():
         L.orc_elem_intforce.argtypes = [vp, vp, vp, vp, vp, vp, vp]
         L.orc_assemble_K.argtypes = [vp, vp, i, vp, vp, vp, i, i, i]
         L.orc_assemble_fint.argtypes = [vp, vp, i, vp, i, i, i]
+        L.orc_elem_bodyforce.argtypes = [vp, vp, vp, vp, vp, vp]
+        L.orc_assemble_body.argtypes = [vp, vp, i, vp]
         L.orc_csr_count.argtypes = [vp, i, vp]
         L.orc_csr_count.restype = C.c_longlong
         L.orc_csr_fill.argtypes = [vp, i, vp, vp]
@@ -164,6 +166,22 @@ class OracleMesh:
         ok = lib().orc_elem_intforce(C.byref(self.et), _p(X), _p(ext), _p(w),
                                      _p(mat), _p(ue), _p(g))
         return g, ok
+
+    def elem_bodyforce(self, e, b):
+        """cElemLoad::ExtForce of a uniform body force b on element e
+        (load/load.cpp:618-691)."""
+        X, ext, w, _ = self.elem_data(e)
+        f = np.zeros(3 * self.mesh.nn)
+        b = np.ascontiguousarray(b, dtype=np.float64)
+        ok = lib().orc_elem_bodyforce(C.byref(self.et), _p(X), _p(ext), _p(w), _p(b), _p(f))
+        return f, ok
+
+    def assemble_body(self, b_elem):
+        """Equation-ordered body-force load vector, b_elem [nelem, 3]."""
+        f = np.zeros(self.mesh.neq)
+        b = np.ascontiguousarray(b_elem, dtype=np.float64)
+        ok = lib().orc_assemble_body(C.byref(self.om), _p(b), self.mesh.neq, _p(f))
+        return f, ok
 
     def shape_dxyz(self, e, rst):
         X, ext, w, _ = self.elem_data(e)

@@ -1,0 +1,74 @@
+"""The reference's own example decks through the device path
+(fast_amd/deck.py: neutral-file reader, device body forces, device solve)
+against the displacements the reference program printed for them
+(tests/golden/deck_*.npz), and the orientation rules of the element kernels.
+
+Tolerance: the reference solves with a direct (skyline) solver; the device
+path with Jacobi-PCG to 1e-12 and FP32 element matrices (1e-7 relative per
+element): displacements agree to 1e-6 of the largest."""
+import os
+
+import numpy as np
+import pytest
+
+from conftest import load_golden
+from fast_amd import mesh as M
+from test_neutral import DECKS, assert_same_displacements
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def fg():
+    import torch
+    assert torch.cuda.is_available(), "GPU tests need a HIP device"
+    from fast_amd import fgpu
+    fgpu.lib()
+    return fgpu
+
+
+@pytest.mark.parametrize("name", ["TestB8", "ScordelisLoDeg3"])
+def test_deck_matches_reference_run(fg, tmp_path, name):
+    from fast_amd import deck, neutral
+    out = str(tmp_path / (name + ".pos"))
+    d, un = deck.run(os.path.join(DECKS, name + ".dat"), out=out)
+    ref = load_golden("deck_%s.npz" % name)
+    assert_same_displacements(d, un, ref["u"], name, tol=1e-6)
+    labels, u2 = neutral.read_pos_displacements(out)
+    assert np.array_equal(labels, d.labels) and np.allclose(u2, un, rtol=1e-15, atol=0)
+
+
+def test_body_force_vs_oracle(fg, oracle_mod):
+    for m in (M.hex8_block(3, 2, 2, size=(1.0, 0.7, 0.4), perturb=0.2, seed=1),
+              M.bsp_block(ne=(3, 2, 2), p=3, perturb=0.1, seed=2),
+              M.bsp_block(ne=(2, 2, 1), p=(3, 3, 2), perturb=0.1, seed=2)):
+        b = np.random.default_rng(0).uniform(-1, 1, (m.nelem, 3))
+        f = fg.GpuModel(m).body_force(b)
+        f0, ok = oracle_mod.OracleMesh(m).assemble_body(b)
+        assert ok and np.abs(f - f0).max() <= 1e-13 * np.abs(f0).max()
+
+
+def _mirror(conn):
+    """Reference local order with the t = +1 and t = -1 faces swapped:
+    a left-handed parametrization of the same brick."""
+    return np.ascontiguousarray(conn[:, [4, 5, 6, 7, 0, 1, 2, 3]])
+
+
+def test_left_handed_mesh_gives_the_same_solution(fg):
+    m = M.hex8_block(4, 3, 3, perturb=0.2, seed=5, traction=(0.0, 0.0, -1.0))
+    f = m.load_vector()
+    u0, _, _ = fg.GpuModel(m).linear_static(f, tol=1e-12)
+    ml = M.hex8_block(4, 3, 3, perturb=0.2, seed=5, traction=(0.0, 0.0, -1.0))
+    ml.conn = _mirror(ml.conn)
+    u1, _, _ = fg.GpuModel(ml).linear_static(f, tol=1e-12)
+    assert np.abs(u1 - u0).max() <= 1e-8 * np.abs(u0).max()
+
+
+def test_inverted_element_is_reported(fg):
+    m = M.hex8_block(4, 3, 3, perturb=0.1, seed=5)
+    m.conn[7] = m.conn[7][[4, 5, 6, 7, 0, 1, 2, 3]]
+    g = fg.GpuModel(m)
+    with pytest.raises(fg.FgpuError, match="inverted elements"):
+        g.stiffness()
+    with pytest.raises(fg.FgpuError, match="inverted elements"):
+        g.int_force(np.zeros((m.nnode, 3)))

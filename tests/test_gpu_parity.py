@@ -306,7 +306,9 @@ def test_linear_static_driver(fg, oracle_mod):
     m = mesh_from_golden(d)
     g = fg.GpuModel(m)
     u, it, rr = g.linear_static(m.load_vector(), tol=1e-10)
-    assert nrel(m.scatter(u), d["u"]) <= 1e-5
+    # FP64 residual refinement: the FP32 element matrices limit nothing
+    assert rr <= 1e-10
+    assert nrel(m.scatter(u), d["u"]) <= 1e-8
 
 
 @pytest.mark.parametrize("kind", ["hex8_nh", "bsp"])
