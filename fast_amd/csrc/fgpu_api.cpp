@@ -400,7 +400,7 @@ void autotune(Ctx &c, const double *d_u) {
     c.asm_mode = mode;
     float ms = time3([&] { launch_assemble_K(c); });
     std::snprintf(buf, sizeof buf, "assembly mode %d (loads in batches of %d): %.4f ms\n", mode,
-                  mode == 2 ? 8 : 1, ms);
+                  mode == 0 ? 1 : 2 << mode, ms);
     rep += buf;
     if (ms < best) best = ms, pick = mode;
   }

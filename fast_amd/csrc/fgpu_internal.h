@@ -118,8 +118,8 @@ struct Ctx {
 
   // ---- launch configuration (picked by the autotuner, identical results)
   int hex8_te = 0;       // elements per workgroup of the hex8 K kernel (0: largest)
-  int asm_mode = -1;     // assembly: contribution-list loads in batches of 1 (0)
-                         // or 8 (2); -1 default (2)
+  int asm_mode = -1;     // assembly: contribution-list loads in batches of 1, 4, 8,
+                         // 16 (modes 0..3); -1 default (2)
   bool tuned = false;
   std::string tune_report;
 

@@ -241,8 +241,8 @@ void bspmv_launch(Ctx &c, const double *x, double *y) {
 
 }  // namespace
 
-// assembly modes: 0, 2: contribution lists with loads in batches of 1, 8
-std::vector<int> assembly_options(const Ctx &) { return {0, 2}; }
+// assembly modes 0, 1, 2, 3: contribution loads in batches of 1, 4, 8, 16
+std::vector<int> assembly_options(const Ctx &) { return {0, 1, 2, 3}; }
 
 template <int NB>
 void launch_contrib(Ctx &c, int grid) {
@@ -253,10 +253,12 @@ void launch_contrib(Ctx &c, int grid) {
 
 void launch_assemble_K(Ctx &c) {
   const int grid = (int)((c.nblk + 255) / 256);
-  if (c.asm_mode == 0)
-    launch_contrib<1>(c, grid);
-  else
-    launch_contrib<8>(c, grid);
+  switch (c.asm_mode) {
+    case 0: launch_contrib<1>(c, grid); break;
+    case 1: launch_contrib<4>(c, grid); break;
+    case 3: launch_contrib<16>(c, grid); break;
+    default: launch_contrib<8>(c, grid); break;
+  }
   FGPU_CHECK(hipGetLastError());
 }
 

@@ -314,7 +314,7 @@ def test_linear_static_driver(fg, oracle_mod):
 @pytest.mark.parametrize("kind", ["hex8_nh", "bsp"])
 def test_launch_configurations_identical_bits(fg, oracle_mod, kind):
     """Every launch configuration the autotuner may pick (hex8 elements per
-    workgroup; assembly loads in batches of 1 or 8) gives the same global
+    workgroup; assembly loads in batches of 1, 4, 8 or 16) gives the same global
     matrix, bit for bit, and it matches the oracle to the FP32
     element-matrix tolerance."""
     if kind == "bsp":
@@ -333,7 +333,7 @@ def test_launch_configurations_identical_bits(fg, oracle_mod, kind):
     ref = g.csr()[2].copy()
     seen = 0
     for te in ([0, 4, 8, 16, 32, 64] if kind != "bsp" else [0]):
-        for mode in (0, 2):
+        for mode in (0, 1, 2, 3):
             try:
                 g.set_launch(te, mode)
             except fg.FgpuError:
@@ -346,3 +346,30 @@ def test_launch_configurations_identical_bits(fg, oracle_mod, kind):
     om = oracle_mod.OracleMesh(m)
     Ko, ok = om.assemble_K(u)
     assert nrel(ref, Ko) <= 1e-6
+
+
+@pytest.mark.parametrize("form,matp,deg", [("tl_svk", (2.6, 0.3), 3),
+                                           ("tl_neohooke", (1.0, 1.5), (3, 3, 2))])
+def test_newton_bspline_vs_oracle(fg, oracle_mod, form, matp, deg):
+    """Load-control Newton on a B-spline solid cantilever (total Lagrangian)
+    against the oracle's restatement of ctrlnr.cpp on the same model: same
+    converged states per step (the device tangent is FP32, the residual FP64,
+    so the converged states agree to the Newton tolerance)."""
+    m = M.bsp_block(ne=(4, 2, 2), p=deg, size=(2.0, 0.5, 0.5), form=form, perturb=0.05, seed=3,
+                    mat=np.tile(matp, (16, 1)))
+    tip = np.isclose(m.coords[:, 0], 2.0)
+    fn = np.zeros((m.nnode, 3))
+    fn[tip, 2] = -0.002 / tip.sum()
+    q = np.zeros(m.neq)
+    dm = m.dofmap
+    q[dm[tip, 2] - 1] = fn[tip, 2]
+    om = oracle_mod.OracleMesh(m)
+    u_ref, it_ref = oracle_mod.newton_raphson(om, q, 3, 1.0, 30, 1e-10)
+    assert len(u_ref) == 3
+    g = fg.GpuModel(m)
+    us, st, err = g.newton(q, steps=3, factor=1.0, maxit=30, tol=1e-10, pcg_tol=1e-12)
+    assert err == "" and len(us) == 3, err
+    for s in range(3):
+        assert nrel(us[s], u_ref[s]) <= 1e-8, (s, nrel(us[s], u_ref[s]))
+        assert st[s]["converged"]
+    print("Newton iterations oracle", it_ref, "device", [x["iters"] for x in st])
