@@ -399,8 +399,9 @@ void autotune(Ctx &c, const double *d_u) {
   for (int mode : assembly_options(c)) {
     c.asm_mode = mode;
     float ms = time3([&] { launch_assemble_K(c); });
-    std::snprintf(buf, sizeof buf, "assembly mode %d (loads in batches of %d): %.4f ms\n", mode,
-                  mode == 0 ? 1 : 2 << mode, ms);
+    const int nb = (mode & 3) == 0 ? 1 : 2 << (mode & 3);
+    std::snprintf(buf, sizeof buf, "assembly mode %d (loads in batches of %d%s): %.4f ms\n", mode,
+                  nb, mode & 4 ? ", XCD-contiguous workgroups" : "", ms);
     rep += buf;
     if (ms < best) best = ms, pick = mode;
   }

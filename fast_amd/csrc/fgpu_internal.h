@@ -119,7 +119,8 @@ struct Ctx {
   // ---- launch configuration (picked by the autotuner, identical results)
   int hex8_te = 0;       // elements per workgroup of the hex8 K kernel (0: largest)
   int asm_mode = -1;     // assembly: contribution-list loads in batches of 1, 4, 8,
-                         // 16 (modes 0..3); -1 default (2)
+                         // 16 (mode & 3), XCD-contiguous workgroups (mode & 4);
+                         // -1 default (2)
   bool tuned = false;
   std::string tune_report;
 

@@ -498,12 +498,13 @@ __global__ __launch_bounds__(256) void bsp_Kt_kernel(
 // (B + Bl)^T S, elmparam.cpp:1615-1707).  Phase 2, thread (a, quarter):
 // g_a = sum_q c P_q grad N_a(q) over 16 points, recomputing grad N_a from the
 // tensor tables and the stored J^-1, reduced over the 4 quarters.
-template <int NB2>
+template <int NB2, int FORM>
 struct FShared {
   Geo<NB2> g;
   double Ji[NQP][9];
   double rs[NQP][4];   // 1 / W, dW/dr, dW/ds, dW/dt
-  double sig[NQP][9];  // c * P, row-major
+  // c * sigma (Voigt, linear) or c * P (row-major, total-Lagrangian)
+  double sig[NQP][FORM == FGPU_LINEAR ? 6 : 9];
 };
 
 template <int FORM, int NB2>
@@ -514,7 +515,7 @@ __global__ __launch_bounds__(256) void bsp_fint_kernel(
     const double *__restrict__ gp, double *__restrict__ ge, int *__restrict__ status) {
   constexpr int NBAS = Bsp<NB2>::NBAS, NPART = Bsp<NB2>::NPART;
   extern __shared__ __align__(16) unsigned char smem[];
-  FShared<NB2> &S = *reinterpret_cast<FShared<NB2> *>(smem);
+  FShared<NB2, FORM> &S = *reinterpret_cast<FShared<NB2, FORM> *>(smem);
   const int e = blockIdx.x;
   const int tid = threadIdx.x;
   const int *cn = conn + (size_t)e * NBAS;
@@ -550,12 +551,13 @@ __global__ __launch_bounds__(256) void bsp_fint_kernel(
         const double E = mat[2 * (size_t)e], nu = mat[2 * (size_t)e + 1];
         const double lam = E * nu / ((1.0 + nu) * (1.0 - 2.0 * nu)), mu = 0.5 * E / (1.0 + nu);
         const double tr = L[0][0] + L[1][1] + L[2][2];
+        // symmetric: Voigt order xx yy zz xy xz yz
         sg[0] = c * (lam * tr + 2.0 * mu * L[0][0]);
-        sg[4] = c * (lam * tr + 2.0 * mu * L[1][1]);
-        sg[8] = c * (lam * tr + 2.0 * mu * L[2][2]);
-        sg[1] = sg[3] = c * mu * (L[0][1] + L[1][0]);
-        sg[2] = sg[6] = c * mu * (L[0][2] + L[2][0]);
-        sg[5] = sg[7] = c * mu * (L[1][2] + L[2][1]);
+        sg[1] = c * (lam * tr + 2.0 * mu * L[1][1]);
+        sg[2] = c * (lam * tr + 2.0 * mu * L[2][2]);
+        sg[3] = c * mu * (L[0][1] + L[1][0]);
+        sg[4] = c * mu * (L[0][2] + L[2][0]);
+        sg[5] = c * mu * (L[1][2] + L[2][1]);
       } else {
         double F[3][3], C[3][3], Sp[3][3];
         for (int i = 0; i < 3; i++)
@@ -591,6 +593,7 @@ __global__ __launch_bounds__(256) void bsp_fint_kernel(
   if (tid < 4 * NBAS) {  // whole waves: 4 NBAS is 256 or 192
     const int a = tid / 4, quarter = tid % 4;
     double g0 = 0.0, g1 = 0.0, g2 = 0.0;
+    #pragma unroll
     for (int q = quarter * 16; q < quarter * 16 + 16; q++) {
       double n, d[3];
       tensor(S.g, q, a, n, d);
@@ -601,9 +604,15 @@ __global__ __launch_bounds__(256) void bsp_fint_kernel(
       for (int i = 0; i < 3; i++)
         G[i] = S.Ji[q][3 * i] * dR[0] + S.Ji[q][3 * i + 1] * dR[1] + S.Ji[q][3 * i + 2] * dR[2];
       const double *s = S.sig[q];
-      g0 += s[0] * G[0] + s[1] * G[1] + s[2] * G[2];
-      g1 += s[3] * G[0] + s[4] * G[1] + s[5] * G[2];
-      g2 += s[6] * G[0] + s[7] * G[1] + s[8] * G[2];
+      if constexpr (FORM == FGPU_LINEAR) {
+        g0 += s[0] * G[0] + s[3] * G[1] + s[4] * G[2];
+        g1 += s[3] * G[0] + s[1] * G[1] + s[5] * G[2];
+        g2 += s[4] * G[0] + s[5] * G[1] + s[2] * G[2];
+      } else {
+        g0 += s[0] * G[0] + s[1] * G[1] + s[2] * G[2];
+        g1 += s[3] * G[0] + s[4] * G[1] + s[5] * G[2];
+        g2 += s[6] * G[0] + s[7] * G[1] + s[8] * G[2];
+      }
     }
     for (int m = 1; m < 4; m <<= 1) {
       g0 += __shfl_xor(g0, m);
@@ -657,6 +666,7 @@ __global__ __launch_bounds__(256) void bsp_body_kernel(
   if (tid < 4 * NBAS) {  // whole waves
     const int a = tid / 4, quarter = tid % 4;
     double sN = 0.0;
+    #pragma unroll
     for (int q = quarter * 16; q < quarter * 16 + 16; q++) {
       double n, d[3];
       tensor(S.g, q, a, n, d);
@@ -745,7 +755,7 @@ void launch_K_lin(Ctx &c) {
 
 template <int FORM, int NB2>
 void launch_fint(Ctx &c, const double *u) {
-  const size_t lds = sizeof(FShared<NB2>);
+  const size_t lds = sizeof(FShared<NB2, FORM>);
   hipLaunchKernelGGL((bsp_fint_kernel<FORM, NB2>), dim3(c.nelem), dim3(256), lds, c.stream,
                      c.nelem, c.d_coords, c.d_conn, c.d_cpw, c.d_ext, c.d_mat, u, c.d_gauss64,
                      c.d_ge, c.d_status);

@@ -28,14 +28,21 @@ namespace {
 // One thread per upper block (A, B): sum its contributions (sorted by
 // element), NB at a time: the NB codes are read, then all NB blocks are
 // loaded, then added in order (the loads of a batch are in flight together).
-template <int NB>
+// XCD: consecutive workgroups are dispatched round-robin to the 8 XCDs, each
+// with its own L2; with XCD set, workgroup w takes the range
+// (w % 8) * (grid / 8) + w / 8 (grid a multiple of 8), so each XCD sweeps one
+// contiguous eighth of the block rows and the element blocks its rows share
+// stay in that XCD's L2.
+template <int NB, bool XCD>
 __global__ __launch_bounds__(256) void assemble_kernel(
     long long nblk, const int *__restrict__ blkA, const int *__restrict__ blkB,
     const int *__restrict__ posAB, const int *__restrict__ posBA,
     const long long *__restrict__ cptr, const int *__restrict__ contrib,
     const float *__restrict__ Ke, KeLayout L, const unsigned char *__restrict__ fmask,
     double *__restrict__ bval) {
-  long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  const long long w = XCD ? (long long)(blockIdx.x & 7) * (gridDim.x >> 3) + (blockIdx.x >> 3)
+                          : (long long)blockIdx.x;
+  long long k = w * blockDim.x + threadIdx.x;
   if (k >= nblk) return;
   double s[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
   const long long c0 = cptr[k], c1 = cptr[k + 1];
@@ -241,23 +248,29 @@ void bspmv_launch(Ctx &c, const double *x, double *y) {
 
 }  // namespace
 
-// assembly modes 0, 1, 2, 3: contribution loads in batches of 1, 4, 8, 16
-std::vector<int> assembly_options(const Ctx &) { return {0, 1, 2, 3}; }
+// assembly modes 0..7: contribution loads in batches of 1, 4, 8, 16
+// (mode & 3), XCD-contiguous workgroup order when mode & 4
+std::vector<int> assembly_options(const Ctx &) { return {0, 1, 2, 3, 4, 5, 6, 7}; }
 
-template <int NB>
-void launch_contrib(Ctx &c, int grid) {
-  hipLaunchKernelGGL((assemble_kernel<NB>), dim3(grid), dim3(256), 0, c.stream, c.nblk, c.d_blkA,
-                     c.d_blkB, c.d_offAB, c.d_offBA, c.d_cptr, c.d_contrib, c.d_Ke, c.kel,
-                     c.d_fmask, c.d_bval);
+template <int NB, bool XCD>
+void launch_contrib(Ctx &c) {
+  long long grid = (c.nblk + 255) / 256;
+  if (XCD) grid = (grid + 7) / 8 * 8;  // the remap is a bijection on multiples of 8
+  hipLaunchKernelGGL((assemble_kernel<NB, XCD>), dim3((unsigned)grid), dim3(256), 0, c.stream,
+                     c.nblk, c.d_blkA, c.d_blkB, c.d_offAB, c.d_offBA, c.d_cptr, c.d_contrib,
+                     c.d_Ke, c.kel, c.d_fmask, c.d_bval);
 }
 
 void launch_assemble_K(Ctx &c) {
-  const int grid = (int)((c.nblk + 255) / 256);
-  switch (c.asm_mode) {
-    case 0: launch_contrib<1>(c, grid); break;
-    case 1: launch_contrib<4>(c, grid); break;
-    case 3: launch_contrib<16>(c, grid); break;
-    default: launch_contrib<8>(c, grid); break;
+  switch (c.asm_mode < 0 ? 2 : c.asm_mode) {
+    case 0: launch_contrib<1, false>(c); break;
+    case 1: launch_contrib<4, false>(c); break;
+    case 3: launch_contrib<16, false>(c); break;
+    case 4: launch_contrib<1, true>(c); break;
+    case 5: launch_contrib<4, true>(c); break;
+    case 6: launch_contrib<8, true>(c); break;
+    case 7: launch_contrib<16, true>(c); break;
+    default: launch_contrib<8, false>(c); break;
   }
   FGPU_CHECK(hipGetLastError());
 }

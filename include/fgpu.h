@@ -256,7 +256,8 @@ int fgpu_body_force_dev(fgpu_ctx *ctx, const double *b_dev, double *f_dev);
 /* Launch configuration.  On the first stiffness assembly of a context the
  * autotuner times every valid configuration of the element-stiffness kernel
  * (hex8: elements per workgroup 4..64) and of the assembly (contribution
- * loads in batches of 1, 4, 8 or 16: modes 0..3)
+ * loads in batches of 1, 4, 8 or 16: mode & 3; mode & 4: workgroups ordered
+ * so that each of the 8 XCDs sweeps a contiguous range of block rows)
  * on the actual mesh and state, keeps the fastest and records the timings
  * (environment FGPU_AUTOTUNE=0 skips it).  All configurations give identical
  * results, bit for bit.  fgpu_set_launch fixes a configuration instead
