@@ -291,7 +291,7 @@ void destroy(Ctx *c) {
                   c->d_contrib, c->d_nptr, c->d_nadj, c->d_u,
                   c->d_b, c->d_x,
                   c->d_r, c->d_p, c->d_w, c->d_z, c->d_dinv, c->d_part,
-                  c->d_scal, c->d_done, c->d_eqnode};
+                  c->d_scal, c->d_done, c->d_eqnode, c->d_TR};
   for (void *p : bufs)
     if (p) (void)hipFree(p);
   if (c->h_done) (void)hipHostFree(c->h_done);
@@ -730,6 +730,120 @@ int fgpu_get_element_matrices(fgpu_ctx *ctx, int e0, int ne, float *Ke) {
     FGPU_CHECK(hipStreamSynchronize(c.stream));
     (void)hipFree(d);
     return 1;
+  });
+}
+
+// Extrapolation of point values to the element's nodes (cElmParam::
+// CompTRMatrix, elc/elmparam.cpp:1444-1535): least-squares fit of
+// a + b r + c s + d t over the Gauss points, evaluated at the nodes' natural
+// coordinates (bricks: cShapeBrick8::NodeNatCoord, shp/shpsolid.cpp:173-185;
+// B-spline solids: a uniform (p+1)^3 grid on [-1, 1]^3, r fastest,
+// shp/shpbsp.cpp:906-923); one point along r: every node takes the mean.
+static void upload_stress_extrap(Ctx &c) {
+  if (c.d_TR) return;
+  std::vector<double> pts;  // (r, s, t) per point, r fastest
+  double g[3][16], w[16];
+  for (int d = 0; d < 3; d++) gauss_rule(c.nip[d], g[d], w);
+  for (int k = 0; k < c.nip[2]; k++)
+    for (int j = 0; j < c.nip[1]; j++)
+      for (int i = 0; i < c.nip[0]; i++) pts.insert(pts.end(), {g[0][i], g[1][j], g[2][k]});
+  const int nq = (int)pts.size() / 3, nn = c.nn;
+  std::vector<double> nodes;
+  if (c.shape == FGPU_HEX8) {
+    nodes = {-1, -1, 1, 1, -1, 1, 1, 1, 1, -1, 1, 1, -1, -1, -1, 1, -1, -1, 1, 1, -1, -1, 1, -1};
+  } else {
+    const int nb[3] = {c.deg[0] + 1, c.deg[1] + 1, c.deg[2] + 1};
+    for (int k = 0; k < nb[2]; k++)
+      for (int j = 0; j < nb[1]; j++)
+        for (int i = 0; i < nb[0]; i++)
+          nodes.insert(nodes.end(), {-1.0 + 2.0 * i / (nb[0] - 1), -1.0 + 2.0 * j / (nb[1] - 1),
+                                     -1.0 + 2.0 * k / (nb[2] - 1)});
+  }
+  if ((int)nodes.size() != 3 * nn) throw Error("stress extrapolation: node count");
+  std::vector<double> TR((size_t)nn * nq, 1.0 / nq);
+  if (c.nip[0] > 1) {
+    // normal matrix A = sum u u^T, u = (1, r, s, t), inverted by Gauss-Jordan
+    double A[4][8] = {};
+    for (int q = 0; q < nq; q++) {
+      const double u[4] = {1.0, pts[3 * q], pts[3 * q + 1], pts[3 * q + 2]};
+      for (int a = 0; a < 4; a++)
+        for (int b = 0; b < 4; b++) A[a][b] += u[a] * u[b];
+    }
+    for (int a = 0; a < 4; a++) A[a][4 + a] = 1.0;
+    for (int k = 0; k < 4; k++) {
+      const double p = A[k][k];
+      if (!(p > 0.0)) throw Error("stress extrapolation: singular point set");
+      for (int l = 0; l < 8; l++) A[k][l] /= p;
+      for (int r = 0; r < 4; r++)
+        if (r != k) {
+          const double f = A[r][k];
+          for (int l = 0; l < 8; l++) A[r][l] -= f * A[k][l];
+        }
+    }
+    for (int n = 0; n < nn; n++) {
+      const double cn[4] = {1.0, nodes[3 * n], nodes[3 * n + 1], nodes[3 * n + 2]};
+      double ci[4] = {0, 0, 0, 0};
+      for (int a = 0; a < 4; a++)
+        for (int b = 0; b < 4; b++) ci[b] += cn[a] * A[a][4 + b];
+      for (int q = 0; q < nq; q++)
+        TR[(size_t)n * nq + q] =
+            ci[0] + ci[1] * pts[3 * q] + ci[2] * pts[3 * q + 1] + ci[3] * pts[3 * q + 2];
+    }
+  }
+  c.d_TR = dupload(TR.data(), TR.size(), c.stream);
+  FGPU_CHECK(hipStreamSynchronize(c.stream));
+}
+
+int fgpu_num_points(fgpu_ctx *ctx) {
+  return guarded(ctx, [&](Ctx &c) { return c.nip[0] * c.nip[1] * c.nip[2]; });
+}
+
+int fgpu_stresses_dev(fgpu_ctx *ctx, const double *u_dev, double *sig_ip_dev,
+                      double *sig_node_dev, double *sig_avg_dev) {
+  return guarded(ctx, [&](Ctx &c) {
+    if (!u_dev || !sig_ip_dev) throw Error("null argument");
+    upload_stress_extrap(c);
+    double *tmp = nullptr;
+    if (sig_avg_dev && !sig_node_dev)
+      sig_node_dev = tmp = dalloc<double>((size_t)c.nelem * c.nn * 6);
+    FGPU_CHECK(hipMemsetAsync(c.d_status, 0, 4 * sizeof(int), c.stream));
+    if (c.shape == FGPU_HEX8)
+      launch_hex8_stress(c, u_dev, sig_ip_dev, sig_node_dev);
+    else
+      launch_bsp_stress(c, u_dev, sig_ip_dev, sig_node_dev);
+    if (sig_avg_dev) launch_avg_stress(c, sig_node_dev, sig_avg_dev);
+    const int ok = check_status(c, "stresses");
+    if (tmp) (void)hipFree(tmp);
+    return ok;
+  });
+}
+
+int fgpu_stresses(fgpu_ctx *ctx, const double *u, double *sig_ip, double *sig_node,
+                  double *sig_avg) {
+  return guarded(ctx, [&](Ctx &c) {
+    if (!u || !sig_ip) throw Error("null argument");
+    const size_t nq = (size_t)c.nip[0] * c.nip[1] * c.nip[2];
+    const size_t n_ip = (size_t)c.nelem * nq * 6, n_node = (size_t)c.nelem * c.nn * 6,
+                 n_avg = (size_t)c.nnode * 6;
+    FGPU_CHECK(hipMemcpyAsync(c.d_u, u, sizeof(double) * 3 * c.nnode, hipMemcpyHostToDevice,
+                              c.stream));
+    double *d_ip = dalloc<double>(n_ip);
+    double *d_node = sig_node || sig_avg ? dalloc<double>(n_node) : nullptr;
+    double *d_avg = sig_avg ? dalloc<double>(n_avg) : nullptr;
+    int ok = fgpu_stresses_dev(ctx, c.d_u, d_ip, d_node, d_avg);
+    FGPU_CHECK(hipMemcpyAsync(sig_ip, d_ip, sizeof(double) * n_ip, hipMemcpyDeviceToHost,
+                              c.stream));
+    if (sig_node)
+      FGPU_CHECK(hipMemcpyAsync(sig_node, d_node, sizeof(double) * n_node,
+                                hipMemcpyDeviceToHost, c.stream));
+    if (sig_avg)
+      FGPU_CHECK(hipMemcpyAsync(sig_avg, d_avg, sizeof(double) * n_avg, hipMemcpyDeviceToHost,
+                                c.stream));
+    FGPU_CHECK(hipStreamSynchronize(c.stream));
+    (void)hipFree(d_ip);
+    if (d_node) (void)hipFree(d_node);
+    if (d_avg) (void)hipFree(d_avg);
+    return ok;
   });
 }
 

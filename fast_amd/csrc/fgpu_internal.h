@@ -48,6 +48,53 @@ __device__ inline void count_jacobian(int *status, T det, int n = 1) {
   else if (!(det > T(0)))
     atomicAdd(&status[3], n);
 }
+
+// Stress output at one point from L = grad u (L_ij = d u_i / d X_j), Voigt
+// order xx yy zz xy xz yz (tensor components).  Linear (p0 = E, p1 = nu):
+// sigma = lambda tr(eps) I + 2 mu eps, eps = (L + L^T) / 2.  St.Venant-
+// Kirchhoff: second Piola-Kirchhoff S = lambda tr(E) I + 2 mu E with the
+// Green strain E = (L + L^T + L^T L) / 2, the reference's s = C e with
+// engineering shears.  Neo-Hooke (p0 = mu, p1 = lambda): S = mu (I - C^-1) +
+// lambda ln J C^-1, C = F^T F.  Returns false when det C <= 0 (S then 0).
+template <int FORM>
+__device__ inline bool point_stress(const double L[3][3], double p0, double p1, double s[6]) {
+  if (FORM != FGPU_TL_NEOHOOKE) {
+    const double lam = p0 * p1 / ((1.0 + p1) * (1.0 - 2.0 * p1)), mu = 0.5 * p0 / (1.0 + p1);
+    double E[3][3];
+    for (int i = 0; i < 3; i++)
+      for (int j = 0; j < 3; j++) {
+        E[i][j] = 0.5 * (L[i][j] + L[j][i]);
+        if (FORM == FGPU_TL_SVK)
+          E[i][j] += 0.5 * (L[0][i] * L[0][j] + L[1][i] * L[1][j] + L[2][i] * L[2][j]);
+      }
+    const double tr = E[0][0] + E[1][1] + E[2][2];
+    s[0] = lam * tr + 2.0 * mu * E[0][0];
+    s[1] = lam * tr + 2.0 * mu * E[1][1];
+    s[2] = lam * tr + 2.0 * mu * E[2][2];
+    s[3] = 2.0 * mu * E[0][1];
+    s[4] = 2.0 * mu * E[0][2];
+    s[5] = 2.0 * mu * E[1][2];
+    return true;
+  }
+  double F[3][3], C[3][3];
+  for (int i = 0; i < 3; i++)
+    for (int j = 0; j < 3; j++) F[i][j] = L[i][j] + (i == j ? 1.0 : 0.0);
+  for (int i = 0; i < 3; i++)
+    for (int j = 0; j < 3; j++) C[i][j] = F[0][i] * F[0][j] + F[1][i] * F[1][j] + F[2][i] * F[2][j];
+  // adjugate of the symmetric C
+  const double a00 = C[1][1] * C[2][2] - C[1][2] * C[2][1], a01 = C[0][2] * C[2][1] - C[0][1] * C[2][2],
+               a02 = C[0][1] * C[1][2] - C[0][2] * C[1][1], a11 = C[0][0] * C[2][2] - C[0][2] * C[2][0],
+               a12 = C[0][2] * C[1][0] - C[0][0] * C[1][2], a22 = C[0][0] * C[1][1] - C[0][1] * C[1][0];
+  const double det = C[0][0] * a00 + C[1][0] * a01 + C[2][0] * a02;
+  if (!(det > 0.0)) {
+    for (int k = 0; k < 6; k++) s[k] = 0.0;
+    return false;
+  }
+  const double id = 1.0 / det, lnJ = 0.5 * log(det);
+  const double ci[6] = {a00 * id, a11 * id, a22 * id, a01 * id, a02 * id, a12 * id};
+  for (int k = 0; k < 6; k++) s[k] = (k < 3 ? p0 : 0.0) + (p1 * lnJ - p0) * ci[k];
+  return true;
+}
 #endif
 
 // Element-matrix storage: tiles of T elements; inside a tile, block-major:
@@ -88,6 +135,7 @@ struct Ctx {
   float *d_Ke = nullptr;        // element matrices (upper node blocks)
   double *d_ge = nullptr;       // element internal forces [nelem*nn*3]
   int *d_status = nullptr;      // [2]: bad-Jacobian count, stress failure
+  double *d_TR = nullptr;       // stress extrapolation points -> nodes [nn*nq]
 
   // ---- global matrix: node-block rows of 3x3 FP64 blocks (all nodes; a
   // fixed direction has a unit diagonal and no coupling), the solver's
@@ -161,6 +209,11 @@ void launch_bsp_K(Ctx &c, const double *d_unodal);
 void launch_bsp_fint(Ctx &c, const double *d_unodal);
 void launch_hex8_body(Ctx &c, const double *d_b);   // element vectors into d_ge
 void launch_bsp_body(Ctx &c, const double *d_b);
+// stresses at the points (sig_ip [nelem*nq*6]) and, when sig_node is not
+// null, extrapolated to the element nodes with d_TR ([nelem*nn*6])
+void launch_hex8_stress(Ctx &c, const double *d_unodal, double *sig_ip, double *sig_node);
+void launch_bsp_stress(Ctx &c, const double *d_unodal, double *sig_ip, double *sig_node);
+void launch_avg_stress(Ctx &c, const double *sig_node, double *sig_avg);
 void launch_assemble_K(Ctx &c);
 std::vector<int> hex8_te_options(const Ctx &c);     // valid hex8 K workgroup sizes
 std::vector<int> assembly_options(const Ctx &c);    // valid asm_mode values

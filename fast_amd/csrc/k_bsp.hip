@@ -681,6 +681,83 @@ __global__ __launch_bounds__(256) void bsp_body_kernel(
   }
 }
 
+// ---------------------------------------------------------------- stresses
+// cElmParam::IntPntStress (elc/elmparam.cpp:859-937) / cElmParamTL::
+// IntPntStress (1811-1885) at the 64 points and, when sig_node is not null,
+// cElmParam::NodalStress (1304-1333): s_a = sum_q TR[a][q] s_q with the
+// extrapolation matrix of CompTRMatrix (1444-1535).  Phase 1, thread
+// (q, part): grad u at q as in the f_int kernel, then the point's stress;
+// phase 2, thread (a, quarter): 16 points of the extrapolation sum.
+template <int NB2>
+struct SShared {
+  Geo<NB2> g;
+  double sig[NQP][6];
+};
+
+template <int FORM, int NB2>
+__global__ __launch_bounds__(256) void bsp_stress_kernel(
+    int nelem, const double *__restrict__ coords, const int *__restrict__ conn,
+    const double *__restrict__ cpw, const double *__restrict__ ext,
+    const double *__restrict__ mat, const double *__restrict__ unodal,
+    const double *__restrict__ gp, const double *__restrict__ TR, double *__restrict__ sig_ip,
+    double *__restrict__ sig_node, int *__restrict__ status) {
+  constexpr int NBAS = Bsp<NB2>::NBAS, NPART = Bsp<NB2>::NPART;
+  extern __shared__ __align__(16) unsigned char smem[];
+  SShared<NB2> &S = *reinterpret_cast<SShared<NB2> *>(smem);
+  const int e = blockIdx.x;
+  const int tid = threadIdx.x;
+  const int *cn = conn + (size_t)e * NBAS;
+  geo_load(S.g, e, coords, conn, cpw, ext, gp);
+  __syncthreads();
+  {
+    const int q = tid / 4, part = tid % 4;
+    PointGeo P;
+    point_geometry(S.g, q, part, gp, P);
+    double L[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
+    #pragma unroll 2
+    for (int k = 0; k < NPART; k++) {
+      const double *ua = unodal + 3 * (size_t)cn[part * NPART + k];
+      double dR[3], G[3];
+      rational_dR(S.g, q, part * NPART + k, P, dR);
+      for (int i = 0; i < 3; i++)
+        G[i] = P.Ji[i][0] * dR[0] + P.Ji[i][1] * dR[1] + P.Ji[i][2] * dR[2];
+      for (int i = 0; i < 3; i++)
+        for (int j = 0; j < 3; j++) L[i][j] += ua[i] * G[j];
+    }
+    for (int m = 1; m < 4; m <<= 1)
+      for (int i = 0; i < 3; i++)
+        for (int j = 0; j < 3; j++) L[i][j] += __shfl_xor(L[i][j], m);
+    if (part == 0) {
+      count_jacobian(status, P.detJ);
+      double sg[6];
+      if (!point_stress<FORM>(L, mat[2 * (size_t)e], mat[2 * (size_t)e + 1], sg))
+        atomicAdd(&status[1], 1);
+      double *o = sig_ip + ((size_t)e * NQP + q) * 6;
+      for (int k = 0; k < 6; k++) {
+        o[k] = sg[k];
+        S.sig[q][k] = sg[k];
+      }
+    }
+  }
+  if (!sig_node) return;
+  __syncthreads();
+  if (tid < 4 * NBAS) {  // whole waves
+    const int a = tid / 4, quarter = tid % 4;
+    double acc[6] = {0, 0, 0, 0, 0, 0};
+    #pragma unroll
+    for (int q = quarter * 16; q < quarter * 16 + 16; q++) {
+      const double w = TR[a * NQP + q];
+      for (int k = 0; k < 6; k++) acc[k] += w * S.sig[q][k];
+    }
+    for (int m = 1; m < 4; m <<= 1)
+      for (int k = 0; k < 6; k++) acc[k] += __shfl_xor(acc[k], m);
+    if (quarter == 0) {
+      double *o = sig_node + ((size_t)e * NBAS + a) * 6;
+      for (int k = 0; k < 6; k++) o[k] = acc[k];
+    }
+  }
+}
+
 template <class K>
 void set_lds(K kernel, size_t lds) {
   FGPU_CHECK(hipFuncSetAttribute((const void *)kernel,
@@ -789,7 +866,32 @@ void launch_body_nb(Ctx &c, const double *b) {
   FGPU_CHECK(hipGetLastError());
 }
 
+template <int FORM, int NB2>
+void launch_stress(Ctx &c, const double *u, double *sig_ip, double *sig_node) {
+  const size_t lds = sizeof(SShared<NB2>);
+  hipLaunchKernelGGL((bsp_stress_kernel<FORM, NB2>), dim3(c.nelem), dim3(256), lds, c.stream,
+                     c.nelem, c.d_coords, c.d_conn, c.d_cpw, c.d_ext, c.d_mat, u, c.d_gauss64,
+                     c.d_TR, sig_ip, sig_node, c.d_status);
+  FGPU_CHECK(hipGetLastError());
+}
+
+template <int NB2>
+void launch_stress_nb(Ctx &c, const double *u, double *sig_ip, double *sig_node) {
+  switch (c.form) {
+    case FGPU_TL_SVK: launch_stress<FGPU_TL_SVK, NB2>(c, u, sig_ip, sig_node); break;
+    case FGPU_TL_NEOHOOKE: launch_stress<FGPU_TL_NEOHOOKE, NB2>(c, u, sig_ip, sig_node); break;
+    default: launch_stress<FGPU_LINEAR, NB2>(c, u, sig_ip, sig_node); break;
+  }
+}
+
 }  // namespace
+
+void launch_bsp_stress(Ctx &c, const double *u, double *sig_ip, double *sig_node) {
+  if (c.deg[2] == 2)
+    launch_stress_nb<3>(c, u, sig_ip, sig_node);
+  else
+    launch_stress_nb<4>(c, u, sig_ip, sig_node);
+}
 
 void launch_bsp_body(Ctx &c, const double *b) {
   if (c.deg[2] == 2)

@@ -434,6 +434,76 @@ __global__ __launch_bounds__(128) void hex8_body_kernel(int nelem, const double 
     for (int i = 0; i < 3; i++) out[3 * a + i] = sN[a] * b[i];
 }
 
+// Stress output: cElmParam::IntPntStress (elc/elmparam.cpp:859-937) and
+// cElmParamTL::IntPntStress (1811-1885) at every point, and, when sig_node is
+// not null, cElmParam::NodalStress (1304-1333) by the extrapolation matrix TR
+// [8][NIP^3] (CompTRMatrix, 1444-1535).  One thread per element, FP64.
+template <int FORM, int NIP>
+__global__ __launch_bounds__(128) void hex8_stress_kernel(
+    int nelem, const double *__restrict__ coords, const int *__restrict__ conn,
+    const double *__restrict__ mat, const double *__restrict__ unodal,
+    const double *__restrict__ gp, const double *__restrict__ TR, double *__restrict__ sig_ip,
+    double *__restrict__ sig_node, int *__restrict__ status) {
+  constexpr int NQ = NIP * NIP * NIP;
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= nelem) return;
+  const int *cn = conn + (size_t)e * 8;
+  double X[8][3], U[8][3];
+  const int n0 = cn[0];
+  for (int a = 0; a < 8; a++) {
+    const int n = cn[a];
+    for (int k = 0; k < 3; k++) {
+      X[a][k] = coords[3 * (size_t)n + k] - coords[3 * (size_t)n0 + k];
+      U[a][k] = unodal[3 * (size_t)n + k];
+    }
+  }
+  const double p0 = mat[2 * (size_t)e], p1 = mat[2 * (size_t)e + 1];
+  double acc[8][6];
+  for (int a = 0; a < 8; a++)
+    for (int k = 0; k < 6; k++) acc[a][k] = 0.0;
+  int neg = 0, zero = 0, fail = 0;
+  for (int q = 0; q < NQ; q++) {
+    const int ir = q % NIP, is = (q / NIP) % NIP, it = q / (NIP * NIP);
+    const double r = gp[(0 * 16 + ir) * 2], s = gp[(1 * 16 + is) * 2], t = gp[(2 * 16 + it) * 2];
+    double J[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
+    for (int a = 0; a < 8; a++) {
+      double d[3];
+      hex8_dN(r, s, t, a, d[0], d[1], d[2]);
+      for (int i = 0; i < 3; i++)
+        for (int k = 0; k < 3; k++) J[i][k] += d[i] * X[a][k];
+    }
+    double Ji[3][3];
+    const double det = inv3(J, Ji);
+    neg += det < 0.0;
+    zero += !(det > 0.0) && !(det < 0.0);
+    double Lg[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
+    for (int a = 0; a < 8; a++) {
+      double d[3], G[3];
+      hex8_dN(r, s, t, a, d[0], d[1], d[2]);
+      for (int i = 0; i < 3; i++) G[i] = Ji[i][0] * d[0] + Ji[i][1] * d[1] + Ji[i][2] * d[2];
+      for (int i = 0; i < 3; i++)
+        for (int j = 0; j < 3; j++) Lg[i][j] += U[a][i] * G[j];
+    }
+    double sg[6];
+    fail += !point_stress<FORM>(Lg, p0, p1, sg);
+    double *o = sig_ip + ((size_t)e * NQ + q) * 6;
+    for (int k = 0; k < 6; k++) o[k] = sg[k];
+    if (sig_node)
+      for (int a = 0; a < 8; a++) {
+        const double w = TR[a * NQ + q];
+        for (int k = 0; k < 6; k++) acc[a][k] += w * sg[k];
+      }
+  }
+  if (sig_node) {
+    double *o = sig_node + (size_t)e * 48;
+    for (int a = 0; a < 8; a++)
+      for (int k = 0; k < 6; k++) o[6 * a + k] = acc[a][k];
+  }
+  if (neg) atomicAdd(&status[0], neg);
+  if (zero) atomicAdd(&status[3], zero);
+  if (fail) atomicAdd(&status[1], fail);
+}
+
 template <int FORM, int NIP, int TE>
 void launch_K_te(Ctx &c, const double *u) {
   constexpr int NQ = NIP * NIP * NIP;
@@ -522,6 +592,31 @@ static void launch_body_t(Ctx &c, const double *b) {
   hipLaunchKernelGGL((hex8_body_kernel<NIP>), dim3((c.nelem + 127) / 128), dim3(128), 0, c.stream,
                      c.nelem, c.d_coords, c.d_conn, b, c.d_gauss64, c.d_ge, c.d_status);
   FGPU_CHECK(hipGetLastError());
+}
+
+template <int FORM, int NIP>
+static void launch_stress_t(Ctx &c, const double *u, double *sig_ip, double *sig_node) {
+  hipLaunchKernelGGL((hex8_stress_kernel<FORM, NIP>), dim3((c.nelem + 127) / 128), dim3(128), 0,
+                     c.stream, c.nelem, c.d_coords, c.d_conn, c.d_mat, u, c.d_gauss64, c.d_TR,
+                     sig_ip, sig_node, c.d_status);
+  FGPU_CHECK(hipGetLastError());
+}
+
+template <int FORM>
+static void launch_stress_f(Ctx &c, const double *u, double *sig_ip, double *sig_node) {
+  switch (c.nip[0]) {
+    case 1: launch_stress_t<FORM, 1>(c, u, sig_ip, sig_node); break;
+    case 2: launch_stress_t<FORM, 2>(c, u, sig_ip, sig_node); break;
+    default: launch_stress_t<FORM, 3>(c, u, sig_ip, sig_node); break;
+  }
+}
+
+void launch_hex8_stress(Ctx &c, const double *u, double *sig_ip, double *sig_node) {
+  switch (c.form) {
+    case FGPU_LINEAR: launch_stress_f<FGPU_LINEAR>(c, u, sig_ip, sig_node); break;
+    case FGPU_TL_SVK: launch_stress_f<FGPU_TL_SVK>(c, u, sig_ip, sig_node); break;
+    default: launch_stress_f<FGPU_TL_NEOHOOKE>(c, u, sig_ip, sig_node); break;
+  }
 }
 
 void launch_hex8_body(Ctx &c, const double *b) {

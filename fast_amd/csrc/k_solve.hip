@@ -106,6 +106,24 @@ __global__ __launch_bounds__(256) void gather_fint_kernel(
   if ((q = eqmap[3 * A + 2]) >= 0) g[q] = s2;
 }
 
+// cControl::PrintAvrNodalStress (ctrl/ctrl.cpp:2397-2470): the element-nodal
+// stresses of each node summed in element order, divided by their count.
+__global__ __launch_bounds__(256) void avg_stress_kernel(int nnode, const int *__restrict__ nptr,
+                                                         const int *__restrict__ nadj,
+                                                         const double *__restrict__ sig_node,
+                                                         double *__restrict__ out) {
+  const int A = blockIdx.x * blockDim.x + threadIdx.x;
+  if (A >= nnode) return;
+  double s[6] = {0, 0, 0, 0, 0, 0};
+  const int t0 = nptr[A], t1 = nptr[A + 1];
+  for (int t = t0; t < t1; t++) {
+    const double *p = sig_node + (size_t)nadj[t] * 6;
+    for (int k = 0; k < 6; k++) s[k] += p[k];
+  }
+  const int n = t1 - t0;
+  for (int k = 0; k < 6; k++) out[6 * (size_t)A + k] = n > 0 ? s[k] / (double)n : 0.0;
+}
+
 __global__ void add_diag_kernel(int n, const int *__restrict__ eq, const double *__restrict__ v,
                                 const int *__restrict__ eqnode,
                                 const long long *__restrict__ bdiag, double *__restrict__ bval) {
@@ -278,6 +296,12 @@ void launch_assemble_K(Ctx &c) {
 void launch_gather_fint(Ctx &c, double *g) {
   hipLaunchKernelGGL(gather_fint_kernel, dim3((c.nnode + 255) / 256), dim3(256), 0, c.stream,
                      c.nnode, c.nn, c.d_nptr, c.d_nadj, c.d_ge, c.d_eqmap, g);
+  FGPU_CHECK(hipGetLastError());
+}
+
+void launch_avg_stress(Ctx &c, const double *sig_node, double *sig_avg) {
+  hipLaunchKernelGGL(avg_stress_kernel, dim3((c.nnode + 255) / 256), dim3(256), 0, c.stream,
+                     c.nnode, c.d_nptr, c.d_nadj, sig_node, sig_avg);
   FGPU_CHECK(hipGetLastError());
 }
 

@@ -1,7 +1,8 @@
 """Run a FAST neutral-file deck on the device: read (fast_amd.neutral), form
 the load vector (nodal and face loads on the host, body forces on the
-device), solve (linear static, or load-control Newton-Raphson), write the
-nodal displacements in the reference's result layout.
+device), solve (linear static, or load-control Newton-Raphson), compute the
+stresses on the device, and write displacements and stresses in the
+reference's result layout.
 
 usage: python -m fast_amd.deck <deck.dat> [--out <file.pos>] [--tol 1e-9]
 """
@@ -16,9 +17,14 @@ from . import neutral
 from .fgpu import FgpuError, GpuModel
 
 
-def run(path: str, out: str | None = None, tol: float = 1e-9):
-    """Returns (deck, nodal displacements [nnode, 3] of the last step)."""
+def run(path: str, out: str | None = None, tol: float = 1e-9, stresses: bool = True):
+    """Returns (deck, nodal displacements [nnode, 3] of the last step, stresses)
+    with stresses = (integration points [nelem, nq, 6], element nodes
+    [nelem, nn, 6], node averages [nnode, 6] or None) of that step, or None."""
     d = neutral.read(path)
+    if stresses and not d.stress_extrapolation:
+        raise NotImplementedError("nodal stresses without extrapolation "
+                                  "(HEADER.OUTPUT.STRESS.EXTRAPOLATION 0)")
     model = GpuModel(d.mesh)
     f = d.load.copy()
     if d.body is not None:
@@ -27,18 +33,24 @@ def run(path: str, out: str | None = None, tol: float = 1e-9):
         u, it, rr = model.linear_static(f, maxit=max(100000, 200 * d.mesh.neq), tol=tol)
         factor = 1.0
     else:
-        # ctrl/ctrlnr.cpp: f = lambda q per step, Newton to TOLERANCE
+        # ctrl/ctrlnr.cpp: f = lambda q per step, Newton to TOLERANCE.  The
+        # reference solves each correction directly; PCG stands in for that
+        # here, so its iteration limit is that of a direct-quality solve.
         us, st, err = model.newton(f, steps=d.steps, factor=d.factor, maxit=d.maxit, tol=d.tol,
-                                   pcg_tol=tol)
+                                   pcg_tol=tol, pcg_maxit=max(100000, 200 * d.mesh.neq))
         if err:
             raise FgpuError(err)
         u = us[-1]
         factor = d.steps * d.factor
     un = d.mesh.scatter(u)
+    sig = model.stresses(un, nodal=True, average=d.average_stress) if stresses else None
     if out:
-        neutral.write_pos(out, d.labels, un, factor)
+        kw = {}
+        if sig is not None:
+            kw = dict(elem_labels=d.elem_labels, sig_ip=sig[0], sig_node=sig[1], sig_avg=sig[2])
+        neutral.write_pos(out, d.labels, un, factor, **kw)
     model.close()
-    return d, un
+    return d, un, sig
 
 
 def main(argv=None):
@@ -47,7 +59,7 @@ def main(argv=None):
     ap.add_argument("--out", default=None)
     ap.add_argument("--tol", type=float, default=1e-9)
     a = ap.parse_args(argv)
-    d, un = run(a.deck, a.out, a.tol)
+    d, un, sig = run(a.deck, a.out, a.tol)
     k = int(np.argmax(np.abs(un).max(axis=1)))
     print("%s: %d nodes, %d equations; largest displacement at node %d: %s"
           % (a.deck, d.mesh.nnode, d.mesh.neq, d.labels[k], un[k]))

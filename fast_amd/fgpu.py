@@ -84,6 +84,9 @@ EXPORTS = {
     "fgpu_get_element_matrices": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_void_p]),
     "fgpu_body_force": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p]),
     "fgpu_body_force_dev": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p]),
+    "fgpu_num_points": (C.c_int, [C.c_void_p]),
+    "fgpu_stresses": (C.c_int, [C.c_void_p] + [C.c_void_p] * 4),
+    "fgpu_stresses_dev": (C.c_int, [C.c_void_p] + [C.c_void_p] * 4),
     "fgpu_set_launch": (C.c_int, [C.c_void_p, C.c_int, C.c_int]),
     "fgpu_tuning_report": (C.c_char_p, [C.c_void_p]),
     "fgpu_last_time_ms": (C.c_double, [C.c_void_p, C.c_int]),
@@ -303,6 +306,20 @@ class GpuModel:
         f = np.zeros(self.neq)
         self._check(lib().fgpu_body_force(self.h, _p(b), _p(f)), "fgpu_body_force")
         return f
+
+    def stresses(self, u_nodal, nodal=True, average=False):
+        """Stresses of the nodal state u [nnode, 3] (Voigt xx yy zz xy xz yz):
+        (integration points [nelem, nq, 6], element nodes [nelem, nn, 6] or
+        None, node averages [nnode, 6] or None)."""
+        m = self.mesh
+        u = np.ascontiguousarray(u_nodal, dtype=np.float64)
+        assert u.shape == (m.nnode, 3)
+        nq = lib().fgpu_num_points(self.h)
+        si = np.zeros((m.nelem, nq, 6))
+        sn = np.zeros((m.nelem, m.nn, 6)) if nodal else None
+        sa = np.zeros((m.nnode, 6)) if average else None
+        self._check(lib().fgpu_stresses(self.h, _p(u), _p(si), _p(sn), _p(sa)), "fgpu_stresses")
+        return si, sn, sa
 
     def set_launch(self, hex8_te=0, asm_mode=-1):
         """Fix the launch configuration (replaces the autotuner)."""

@@ -39,7 +39,7 @@ _OUTPUT_ONLY = {
     "HEADER.EQUIVALENT.MESH.DIVISION", "HEADER.OUTPUT.PATCH.STRESS.SMOOTH", "NODE",
     "ELEMENT", "MATERIAL", "SECTION", "HEADER.ANALYSIS.PRINT.STEPS",
     # the solver kind: every system here is solved by PCG to a tight tolerance
-    "HEADER.ANALYSIS.MATRIX.TYPE",
+    "HEADER.ANALYSIS.MATRIX.TYPE", "HEADER.PRINT.AVERAGE.NODAL.STRESS",
 }
 
 _HEX8_FACES = ((0, 1, 2, 3), (4, 5, 6, 7), (0, 1, 5, 4), (3, 2, 6, 7), (0, 3, 7, 4),
@@ -61,6 +61,9 @@ class Deck:
     factor: float = 1.0                   # MAXIMUM.ITERATIONS, TOLERANCE
     maxit: int = 20
     tol: float = 1e-5
+    elem_labels: Optional[np.ndarray] = None  # row e of every element array
+    stress_extrapolation: bool = True     # HEADER.OUTPUT.STRESS.EXTRAPOLATION
+    average_stress: bool = False          # HEADER.PRINT.AVERAGE.NODAL.STRESS
 
 
 def _sections(text: str) -> Dict[str, List[str]]:
@@ -244,7 +247,14 @@ def read(path: str) -> Deck:
         for _ in range(tk.int()):
             e = tk.int() - 1
             body[e] += [tk.float(), tk.float(), tk.float()]
-    deck = Deck(mesh=mesh, load=f, labels=labels, title=title, body=body)
+    deck = Deck(mesh=mesh, load=f, labels=labels, title=title, body=body,
+                elem_labels=np.asarray([r[0] for r in rows]))
+    # stress output flags (ctrl/ctrl.cpp:354-385 ReadPrintAverageStress /
+    # ReadStrExtFlag; extrapolation on and averages off by default)
+    if "HEADER.OUTPUT.STRESS.EXTRAPOLATION" in sec:
+        deck.stress_extrapolation = int(sec["HEADER.OUTPUT.STRESS.EXTRAPOLATION"][0].split()[0]) != 0
+    if "HEADER.PRINT.AVERAGE.NODAL.STRESS" in sec:
+        deck.average_stress = int(sec["HEADER.PRINT.AVERAGE.NODAL.STRESS"][0].split()[0]) != 0
     # analysis (ctrl/ctrl.cpp:148-190 ReadAlg and the HEADER.ANALYSIS readers)
     alg = sec.get("HEADER.ANALYSIS.ALGORITHM", ["'Linear'"])[0].strip("'")
     if alg in ("Linear", "linear"):
@@ -369,28 +379,126 @@ def _face_loads(mesh, toks, idx):
     return f
 
 
+STRESS_LABELS = ("STRESS_XX", "STRESS_YY", "STRESS_ZZ", "STRESS_XY", "STRESS_XZ", "STRESS_YZ")
+
+
+def _fmt(v, digits):
+    return "%+.*e" % (digits, v)
+
+
 def write_pos(path: str, labels: np.ndarray, u_nodal: np.ndarray, factor: float = 1.0,
-              digits: int = 16):
-    """Nodal displacements in the result layout of the reference's .pos file
-    (ctrl/ctrl.cpp:1117-1175): one step, label and six components per node."""
+              digits: int = 16, elem_labels=None, sig_ip=None, sig_node=None, sig_avg=None):
+    """One step in the result layout of the reference's .pos file: nodal
+    displacements (ctrl/ctrl.cpp:1117-1175: label and six components per
+    node) and, when given, the stresses of cControl::PrintIntPntStress
+    (ctrl.cpp:2189-2259: per element its label, point count and one line per
+    point), PrintNodalStress (2264-2330: per element its label and one line per
+    element node) and PrintAvrNodalStress (2397-2470: one line per node).
+    Stress components in Voigt order xx yy zz xy xz yz."""
     with open(path, "w") as out:
         out.write("%RESULT\n1\n1  'result1'\n\n%RESULT.CASE\n1\n\n")
         out.write("%%RESULT.CASE.STEP\n1\n\n%%RESULT.CASE.STEP.FACTOR\n%.*e\n\n" % (digits, factor))
         out.write("%%RESULT.CASE.STEP.NODAL.DISPLACEMENT\n%d  'displacement'\n" % len(labels))
         for lab, u in zip(labels, u_nodal):
-            out.write("%-5d " % lab + "  ".join("%+.*e" % (digits, v) for v in
+            out.write("%-5d " % lab + "  ".join(_fmt(v, digits) for v in
                                                   list(u) + [0.0, 0.0, 0.0]) + "  \n")
-        out.write("\n%END\n")
+        out.write("\n")
+        head = "6\n" + "  ".join("'%s'" % k for k in STRESS_LABELS) + "  \n\n"
+        if elem_labels is None and (sig_ip is not None or sig_node is not None):
+            elem_labels = np.arange(1, len(sig_ip if sig_ip is not None else sig_node) + 1)
+        if sig_ip is not None:
+            out.write("%RESULT.CASE.STEP.ELEMENT.GAUSS.SCALAR\n" + head)
+            out.write("%%RESULT.CASE.STEP.ELEMENT.GAUSS.SCALAR.DATA\n%d\n" % len(sig_ip))
+            for lab, s in zip(elem_labels, sig_ip):
+                out.write("%d\n%d\n" % (lab, len(s)))
+                for row in s:
+                    out.write(" ".join(_fmt(v, digits) for v in row) + " \n")
+            out.write("\n")
+        if sig_node is not None:
+            out.write("%RESULT.CASE.STEP.ELEMENT.NODAL.SCALAR\n" + head)
+            out.write("%%RESULT.CASE.STEP.ELEMENT.NODAL.SCALAR.DATA\n%d\n" % len(sig_node))
+            for lab, s in zip(elem_labels, sig_node):
+                out.write("%d\n" % lab)
+                for row in s:
+                    out.write(" ".join(_fmt(v, digits) for v in row) + " \n")
+            out.write("\n")
+        if sig_avg is not None:
+            out.write("%RESULT.CASE.STEP.NODAL.SCALAR\n" + head)
+            out.write("%%RESULT.CASE.STEP.NODAL.SCALAR.DATA\n%d\n" % len(sig_avg))
+            for lab, row in zip(labels, sig_avg):
+                out.write("%-5d " % lab + "  ".join(_fmt(v, digits) for v in row) + "  \n")
+            out.write("\n")
+        out.write("%END\n")
 
 
-def read_pos_displacements(path: str):
-    """Labels and displacements (x, y, z) of the first step of a .pos file."""
-    lines = open(path).read().splitlines()
-    k = next(i for i, l in enumerate(lines) if l.strip() == "%RESULT.CASE.STEP.NODAL.DISPLACEMENT")
-    n = int(lines[k + 1].split()[0])
-    lab, u = [], []
-    for l in lines[k + 2:k + 2 + n]:
-        v = l.split()
-        lab.append(int(v[0]))
-        u.append([float(x) for x in v[1:4]])
-    return np.asarray(lab), np.asarray(u)
+def read_pos(path: str):
+    """Every result step of a .pos file: a list of dicts with the step's
+    ``factor``, nodal ``labels`` and displacements ``u`` (x, y, z), and where
+    present ``elem_labels`` with ``sig_ip`` (list of [npt, 6] arrays),
+    ``sig_node`` (list of [nen, 6] arrays) and ``sig_avg`` ([nnode, 6])."""
+    lines = [l.strip() for l in open(path).read().splitlines()]
+    steps = []
+    cur = None
+    i = 0
+
+    def vals(l):
+        return [float(x) for x in l.split()]
+
+    while i < len(lines):
+        l = lines[i]
+        if l == "%RESULT.CASE.STEP.FACTOR":
+            cur = {"factor": float(lines[i + 1].split()[0])}
+            steps.append(cur)
+            i += 2
+            continue
+        if l == "%RESULT.CASE.STEP.NODAL.DISPLACEMENT" and cur is not None:
+            n = int(lines[i + 1].split()[0])
+            rows = [lines[i + 2 + k].split() for k in range(n)]
+            cur["labels"] = np.asarray([int(r[0]) for r in rows])
+            cur["u"] = np.asarray([[float(x) for x in r[1:4]] for r in rows])
+            i += 2 + n
+            continue
+        if l == "%RESULT.CASE.STEP.ELEMENT.GAUSS.SCALAR.DATA" and cur is not None:
+            ne = int(lines[i + 1])
+            j = i + 2
+            labs, sig = [], []
+            for _ in range(ne):
+                labs.append(int(lines[j]))
+                npt = int(lines[j + 1])
+                sig.append(np.asarray([vals(lines[j + 2 + k]) for k in range(npt)]))
+                j += 2 + npt
+            cur["elem_labels"] = np.asarray(labs)
+            cur["sig_ip"] = sig
+            i = j
+            continue
+        if l == "%RESULT.CASE.STEP.ELEMENT.NODAL.SCALAR.DATA" and cur is not None:
+            ne = int(lines[i + 1])
+            j = i + 2
+            labs, sig = [], []
+            for _ in range(ne):
+                labs.append(int(lines[j]))
+                j += 1
+                rows = []
+                while j < len(lines) and lines[j] and (lines[j][0] in "+-"):
+                    rows.append(vals(lines[j]))
+                    j += 1
+                sig.append(np.asarray(rows))
+            cur["elem_labels"] = np.asarray(labs)
+            cur["sig_node"] = sig
+            i = j
+            continue
+        if l == "%RESULT.CASE.STEP.NODAL.SCALAR.DATA" and cur is not None:
+            n = int(lines[i + 1])
+            rows = [vals(lines[i + 2 + k])[1:] for k in range(n)]
+            cur["sig_avg"] = np.asarray(rows)
+            i += 2 + n
+            continue
+        i += 1
+    return steps
+
+
+def read_pos_displacements(path: str, step: int = 0):
+    """Labels and displacements (x, y, z) of one result step of a .pos file
+    (the first by default; -1 the last)."""
+    st = read_pos(path)[step]
+    return st["labels"], st["u"]

@@ -253,6 +253,30 @@ int fgpu_gnn_newton_predictor(void *user, int step, const double *u_dev, double 
 int fgpu_body_force(fgpu_ctx *ctx, const double *b, double *f);
 int fgpu_body_force_dev(fgpu_ctx *ctx, const double *b_dev, double *f_dev);
 
+/* Stresses of the nodal state u [nnode * 3] (FP64), in Voigt order xx yy zz
+ * xy xz yz: the second Piola-Kirchhoff stress of the Green strain for the
+ * total-Lagrangian forms, the Cauchy stress of the small strain for the
+ * linear form.
+ *   sig_ip   [nelem * nq * 6]  at the integration points, in the brick
+ *            rule's order (r fastest); nq = fgpu_num_points.  Replaces
+ *            cElmParam::IntPntStress / cElmParamTL::IntPntStress
+ *            (elc/elmparam.cpp:859-937, 1811-1885) over the element loop of
+ *            cControl::PrintIntPntStress (ctrl/ctrl.cpp:2189-2259).
+ *   sig_node [nelem * nn * 6]  extrapolated to each element's nodes by the
+ *            least-squares linear fit of cElmParam::NodalStress /
+ *            CompTRMatrix (elmparam.cpp:1304-1333, 1444-1535; the reference's
+ *            default, HEADER.OUTPUT.STRESS.EXTRAPOLATION 1), as printed by
+ *            cControl::PrintNodalStress (ctrl.cpp:2264-2330); may be NULL.
+ *   sig_avg  [nnode * 6]  element-nodal values averaged per node in element
+ *            order (cControl::PrintAvrNodalStress, ctrl.cpp:2397-2470); may be
+ *            NULL.
+ * Returns 0 on a failed stress update or an invalid Jacobian. */
+int fgpu_num_points(fgpu_ctx *ctx);
+int fgpu_stresses(fgpu_ctx *ctx, const double *u_nodal, double *sig_ip, double *sig_node,
+                  double *sig_avg);
+int fgpu_stresses_dev(fgpu_ctx *ctx, const double *u_nodal_dev, double *sig_ip_dev,
+                      double *sig_node_dev, double *sig_avg_dev);
+
 /* Launch configuration.  On the first stiffness assembly of a context the
  * autotuner times every valid configuration of the element-stiffness kernel
  * (hex8: elements per workgroup 4..64) and of the assembly (contribution

@@ -626,6 +626,109 @@ int orc_elem_bodyforce(const orc_etype *et, const double *X, const double *ext,
   return ok;
 }
 
+/* Integration-point stresses.  Linear: cElmParam::IntPntStress
+ * (elc/elmparam.cpp:859-937): e = B u, s = C e per point.  Total Lagrangian:
+ * cElmParamTL::IntPntStress (elmparam.cpp:1811-1885): Green strain
+ * e = B u + 0.5 Bl u (GreenStrain, 1899-1919), s = C e (second
+ * Piola-Kirchhoff); neo-Hooke: the second Piola-Kirchhoff stress of the
+ * project's model.  Voigt order xx yy zz xy xz yz; points in the brick
+ * rule's order (r fastest).  sig: [nq*6]. */
+int orc_elem_stress(const orc_etype *et, const double *X, const double *ext,
+                    const double *w, const double *mat, const double *u,
+                    double *sig)
+{
+  int nn = et->nn, nd = 3 * nn;
+  double rst[3 * 512], wq[512];
+  int nq = orc_brick_points(et->nip, rst, wq);
+  double *B = malloc(sizeof(double) * 6 * nd);
+  double *Bl = malloc(sizeof(double) * 6 * nd);
+  double *d = malloc(sizeof(double) * 3 * nn);
+  int ok = 1;
+  for (int q = 0; q < nq; q++) {
+    double detJ, e[6];
+    if (!orc_shape_dxyz(et, X, ext, w, rst + 3 * q, &detJ, d)) ok = 0;
+    bmatrix(nn, d, B);
+    if (et->form == ORC_LINEAR) {
+      matvec(6, nd, B, u, e);
+    } else {
+      blmatrix(nn, d, u, Bl);
+      green_strain(nd, B, Bl, u, e);
+    }
+    if (!material(et->form == ORC_TL_SVK ? ORC_LINEAR : et->form, mat, e,
+                  sig + 6 * q, NULL))
+      ok = 0;
+  }
+  free(B); free(Bl); free(d);
+  return ok;
+}
+
+/* Natural coordinates of the element's nodes: cShapeBrick8::NodeNatCoord
+ * (shp/shpsolid.cpp:173-185) and cShapeBspSolid::NodeNatCoord
+ * (shp/shpbsp.cpp:906-923: a uniform (p+1)^3 grid on [-1, 1]^3, r fastest). */
+static void node_nat_coords(const orc_etype *et, double *c)
+{
+  if (et->shape == ORC_HEX8) {
+    static const double h[8][3] = {{-1, -1, 1}, {1, -1, 1}, {1, 1, 1}, {-1, 1, 1},
+                                   {-1, -1, -1}, {1, -1, -1}, {1, 1, -1}, {-1, 1, -1}};
+    memcpy(c, h, sizeof h);
+    return;
+  }
+  int nb[3] = {et->deg[0] + 1, et->deg[1] + 1, et->deg[2] + 1}, n = 0;
+  for (int i = 0; i < nb[2]; i++)
+    for (int j = 0; j < nb[1]; j++)
+      for (int k = 0; k < nb[0]; k++, n++) {
+        c[3 * n] = -1.0 + (2.0 / (nb[0] - 1)) * k;
+        c[3 * n + 1] = -1.0 + (2.0 / (nb[1] - 1)) * j;
+        c[3 * n + 2] = -1.0 + (2.0 / (nb[2] - 1)) * i;
+      }
+}
+
+/* Extrapolation of integration-point values to the nodes:
+ * cElmParam::CompTRMatrix (elc/elmparam.cpp:1444-1535).  Least-squares fit
+ * of a + b r + c s + d t over the points (normal matrix A = sum u u^T,
+ * u = (1, r, s, t)), evaluated at the nodes: TR = CN A^-1 [u_1 .. u_nq].
+ * One point along r: every node takes the mean.  TR: [nn*nq] row-major. */
+int orc_stress_extrap(const orc_etype *et, double *TR)
+{
+  int nn = et->nn;
+  double rst[3 * 512], wq[512];
+  int nq = orc_brick_points(et->nip, rst, wq);
+  if (et->nip[0] == 1) {
+    for (int i = 0; i < nn * nq; i++) TR[i] = 1.0 / (double)nq;
+    return nq;
+  }
+  double A[4][4] = {{0}};
+  for (int q = 0; q < nq; q++) {
+    double v[4] = {1.0, rst[3 * q], rst[3 * q + 1], rst[3 * q + 2]};
+    for (int k = 0; k < 4; k++)
+      for (int l = 0; l < 4; l++) A[k][l] += v[k] * v[l];
+  }
+  /* A^-1 by Gauss-Jordan elimination (A is symmetric positive definite) */
+  double I[4][4] = {{1, 0, 0, 0}, {0, 1, 0, 0}, {0, 0, 1, 0}, {0, 0, 0, 1}};
+  for (int k = 0; k < 4; k++) {
+    double p = A[k][k];
+    if (!(p > 0.0)) return 0;
+    for (int l = 0; l < 4; l++) { A[k][l] /= p; I[k][l] /= p; }
+    for (int r = 0; r < 4; r++) {
+      if (r == k) continue;
+      double f = A[r][k];
+      for (int l = 0; l < 4; l++) { A[r][l] -= f * A[k][l]; I[r][l] -= f * I[k][l]; }
+    }
+  }
+  double cn[3 * MAXNN];
+  node_nat_coords(et, cn);
+  for (int a = 0; a < nn; a++) {
+    double c[4] = {1.0, cn[3 * a], cn[3 * a + 1], cn[3 * a + 2]}, ci[4] = {0, 0, 0, 0};
+    for (int k = 0; k < 4; k++)
+      for (int l = 0; l < 4; l++) ci[l] += c[k] * I[k][l];
+    for (int q = 0; q < nq; q++) {
+      double v[4] = {1.0, rst[3 * q], rst[3 * q + 1], rst[3 * q + 2]};
+      TR[(size_t)a * nq + q] = ci[0] * v[0] + ci[1] * v[1] + ci[2] * v[2] + ci[3] * v[3];
+    }
+  }
+  return nq;
+}
+
 /* ------------------------------------------------------------------------
  * Mesh-level loops.
  * ---------------------------------------------------------------------- */
@@ -741,6 +844,56 @@ int orc_assemble_body(const orc_mesh *m, const double *b, int neq, double *f)
     for (int i = 0; i < 3 * nn; i++)
       if (edof[i] > 0) f[edof[i] - 1] += fe[i];
   }
+  return ok;
+}
+
+/* Stress output of the whole mesh: cControl::PrintIntPntStress
+ * (ctrl/ctrl.cpp:2189-2259: IntPntStress per element), PrintNodalStress
+ * (2264-2330: cElmParam::NodalStress, elmparam.cpp:1304-1333, with the
+ * default extrapolation [S]node = TR [S]ipt) and PrintAvrNodalStress
+ * (2397-2470: element-nodal values summed per node in element order, then
+ * divided by their count).  sig_ip [nelem*nq*6], sig_node [nelem*nn*6],
+ * sig_avg [nnode*6]; sig_node / sig_avg may be NULL. */
+int orc_stresses(const orc_mesh *m, const double *u_nodal, double *sig_ip,
+                 double *sig_node, double *sig_avg)
+{
+  int nn = m->et.nn;
+  size_t es = ext_stride(&m->et);
+  double rst[3 * 512], wq[512];
+  int nq = orc_brick_points(m->et.nip, rst, wq);
+  double *TR = malloc(sizeof(double) * (size_t)nn * nq);
+  int ok = orc_stress_extrap(&m->et, TR) > 0;
+  double X[3 * MAXNN], ue[3 * MAXNN], we[MAXNN];
+  double *cnt = NULL;
+  if (sig_avg) {
+    memset(sig_avg, 0, sizeof(double) * 6 * (size_t)m->nnode);
+    cnt = calloc((size_t)m->nnode, sizeof(double));
+  }
+  double *sn = malloc(sizeof(double) * 6 * (size_t)nn);
+  for (int e = 0; e < m->nelem; e++) {
+    gather_elem(m, e, u_nodal, X, ue, we, NULL);
+    const double *ext = m->ext ? m->ext + (size_t)e * es : NULL;
+    double *si = sig_ip + (size_t)e * nq * 6;
+    if (!orc_elem_stress(&m->et, X, ext, we, m->mat + 2 * (size_t)e, ue, si)) ok = 0;
+    for (int a = 0; a < nn; a++)
+      for (int c = 0; c < 6; c++) {
+        double v = 0.0;
+        for (int q = 0; q < nq; q++) v += TR[(size_t)a * nq + q] * si[6 * q + c];
+        sn[6 * a + c] = v;
+      }
+    if (sig_node) memcpy(sig_node + (size_t)e * nn * 6, sn, sizeof(double) * 6 * nn);
+    if (sig_avg)
+      for (int a = 0; a < nn; a++) {
+        int n = m->conn[(size_t)e * nn + a];
+        for (int c = 0; c < 6; c++) sig_avg[6 * (size_t)n + c] += sn[6 * a + c];
+        cnt[n] += 1.0;
+      }
+  }
+  if (sig_avg)
+    for (int n = 0; n < m->nnode; n++)
+      if (cnt[n] > 0.0)
+        for (int c = 0; c < 6; c++) sig_avg[6 * (size_t)n + c] /= cnt[n];
+  free(cnt); free(sn); free(TR);
   return ok;
 }
 

@@ -76,6 +76,16 @@ int  orc_elem_bodyforce(const orc_etype *et, const double *X, const double *ext,
 int  orc_assemble_body(const orc_mesh *m, const double *b, int neq, double *f);
 int  orc_assemble_fint(const orc_mesh *m, const double *u_nodal, int neq,
                        double *g, int nthreads, int elem_begin, int elem_end);
+/* Stresses: integration-point values (Voigt xx yy zz xy xz yz, points in
+ * the brick rule's order), the extrapolation matrix to the element's nodes
+ * [nn*nq], and the whole-mesh output (element-nodal and node-averaged values
+ * may be NULL). */
+int  orc_elem_stress(const orc_etype *et, const double *X, const double *ext,
+                     const double *w, const double *mat, const double *u,
+                     double *sig);
+int  orc_stress_extrap(const orc_etype *et, double *TR);
+int  orc_stresses(const orc_mesh *m, const double *u_nodal, double *sig_ip,
+                  double *sig_node, double *sig_avg);
 /* Full CSR sparsity pattern of the equations (two-pass). */
 long long orc_csr_count(const orc_mesh *m, int neq, long long *rowptr);
 void orc_csr_fill(const orc_mesh *m, int neq, const long long *rowptr, int *col);

@@ -55,6 +55,9 @@ def lib():
         L.orc_assemble_fint.argtypes = [vp, vp, i, vp, i, i, i]
         L.orc_elem_bodyforce.argtypes = [vp, vp, vp, vp, vp, vp]
         L.orc_assemble_body.argtypes = [vp, vp, i, vp]
+        L.orc_elem_stress.argtypes = [vp, vp, vp, vp, vp, vp, vp]
+        L.orc_stress_extrap.argtypes = [vp, vp]
+        L.orc_stresses.argtypes = [vp, vp, vp, vp, vp]
         L.orc_csr_count.argtypes = [vp, i, vp]
         L.orc_csr_count.restype = C.c_longlong
         L.orc_csr_fill.argtypes = [vp, i, vp, vp]
@@ -182,6 +185,36 @@ class OracleMesh:
         b = np.ascontiguousarray(b_elem, dtype=np.float64)
         ok = lib().orc_assemble_body(C.byref(self.om), _p(b), self.mesh.neq, _p(f))
         return f, ok
+
+    def elem_stress(self, e, u_nodal):
+        """Integration-point stresses of element e: [nq, 6] (Voigt xx yy zz xy
+        xz yz), and the success flag."""
+        X, ext, w, mat = self.elem_data(e)
+        ue = np.ascontiguousarray(np.asarray(u_nodal, dtype=np.float64)[self.mesh.conn[e]].ravel())
+        nq = int(np.prod(self.mesh.nip))
+        sig = np.zeros((nq, 6))
+        ok = lib().orc_elem_stress(C.byref(self.et), _p(X), _p(ext), _p(w), _p(mat), _p(ue),
+                                   _p(sig))
+        return sig, ok
+
+    def stress_extrap(self):
+        """Integration points -> element nodes extrapolation matrix [nn, nq]."""
+        nq = int(np.prod(self.mesh.nip))
+        TR = np.zeros((self.mesh.nn, nq))
+        assert lib().orc_stress_extrap(C.byref(self.et), _p(TR)) == nq
+        return TR
+
+    def stresses(self, u_nodal):
+        """(integration-point [nelem, nq, 6], element-nodal [nelem, nn, 6],
+        node-averaged [nnode, 6]) stresses of the nodal state u, and the flag."""
+        m = self.mesh
+        nq = int(np.prod(m.nip))
+        u = np.ascontiguousarray(u_nodal, dtype=np.float64)
+        si = np.zeros((m.nelem, nq, 6))
+        sn = np.zeros((m.nelem, m.nn, 6))
+        sa = np.zeros((m.nnode, 6))
+        ok = lib().orc_stresses(C.byref(self.om), _p(u), _p(si), _p(sn), _p(sa))
+        return si, sn, sa, ok
 
     def shape_dxyz(self, e, rst):
         X, ext, w, _ = self.elem_data(e)

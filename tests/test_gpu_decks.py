@@ -1,7 +1,8 @@
 """The reference's own example decks through the device path
 (fast_amd/deck.py: neutral-file reader, device body forces, device solve)
-against the displacements the reference program printed for them
-(tests/golden/deck_*.npz), and the orientation rules of the element kernels.
+against the displacements and stresses the reference program printed for
+them (tests/golden/deck_*.npz), and the orientation rules of the element
+kernels.
 
 Tolerance: the reference solves with a direct (skyline) solver; the device
 path with Jacobi-PCG to 1e-12 and FP32 element matrices (1e-7 relative per
@@ -13,7 +14,7 @@ import pytest
 
 from conftest import load_golden
 from fast_amd import mesh as M
-from test_neutral import DECKS, assert_same_displacements
+from test_neutral import ALL, DECKS, assert_same_displacements
 
 pytestmark = pytest.mark.gpu
 
@@ -27,15 +28,24 @@ def fg():
     return fgpu
 
 
-@pytest.mark.parametrize("name", ["TestB8", "ScordelisLoDeg3"])
+@pytest.mark.parametrize("name", ALL)
 def test_deck_matches_reference_run(fg, tmp_path, name):
+    """Displacements and stresses of the last step (linear static, or load
+    control for the TL decks) against the reference program's output; the
+    written .pos file reads back the same."""
     from fast_amd import deck, neutral
     out = str(tmp_path / (name + ".pos"))
-    d, un = deck.run(os.path.join(DECKS, name + ".dat"), out=out)
+    d, un, sig = deck.run(os.path.join(DECKS, name + ".dat"), out=out)
     ref = load_golden("deck_%s.npz" % name)
     assert_same_displacements(d, un, ref["u"], name, tol=1e-6)
-    labels, u2 = neutral.read_pos_displacements(out)
-    assert np.array_equal(labels, d.labels) and np.allclose(u2, un, rtol=1e-15, atol=0)
+    # (TestB8's free rigid rotation carries no small-strain stress)
+    scale = np.abs(ref["sig_ip"]).max()
+    assert np.abs(sig[0] - ref["sig_ip"]).max() <= 1e-5 * scale
+    assert np.abs(sig[1] - ref["sig_node"]).max() <= 1e-5 * scale
+    st = neutral.read_pos(out)[-1]
+    assert np.array_equal(st["labels"], d.labels) and np.array_equal(st["u"], un)
+    assert np.array_equal(np.stack(st["sig_ip"]), sig[0])
+    assert np.array_equal(np.stack(st["sig_node"]), sig[1])
 
 
 def test_body_force_vs_oracle(fg, oracle_mod):

@@ -52,20 +52,76 @@ def test_solver_order_numbering():
     assert dm.max() == d.mesh.neq
 
 
-@pytest.mark.parametrize("name", ["TestB8", "ScordelisLoDeg3"])
-def test_oracle_solution_matches_reference_run(oracle_mod, name):
-    d = neutral.read(os.path.join(DECKS, name + ".dat"))
+ALL = ["TestB8", "ScordelisLoDeg3", "BeamB8", "BeamB8TL", "ScordelisLoDeg3TL"]
+
+
+def oracle_solution(oracle_mod, d):
+    """Nodal displacements of the deck's last step by the oracle."""
     om = oracle_mod.OracleMesh(d.mesh)
     f = d.load.copy()
     if d.body is not None:
         fb, ok = om.assemble_body(d.body)
         assert ok
         f += fb
-    u = oracle_mod.linear_static(om, f)
+    if d.analysis == "linear":
+        u = oracle_mod.linear_static(om, f)
+    else:
+        us, its = oracle_mod.newton_raphson(om, f, d.steps, d.factor, d.maxit, d.tol)[:2]
+        assert len(us) == d.steps
+        u = us[-1]
+    return om, d.mesh.scatter(u)
+
+
+def reference_state(d, ref):
+    """The reference's printed displacements in the reader's node order."""
+    idx = {int(l): i for i, l in enumerate(d.labels)}
+    u = np.zeros((d.mesh.nnode, 3))
+    for l, v in zip(ref["labels"], ref["u"]):
+        u[idx[int(l)]] = v
+    return u
+
+
+@pytest.mark.parametrize("name", ALL)
+def test_oracle_solution_matches_reference_run(oracle_mod, name):
+    d = neutral.read(os.path.join(DECKS, name + ".dat"))
     ref = load_golden("deck_%s.npz" % name)
     assert np.array_equal(ref["labels"], d.labels)
-    un = d.mesh.scatter(u)
-    assert_same_displacements(d, un, ref["u"], name)
+    assert int(ref["nsteps"]) == d.steps
+    om, un = oracle_solution(oracle_mod, d)
+    assert_same_displacements(d, un, ref["u"], name, tol=1e-9 if "TL" in name else 1e-10)
+
+
+@pytest.mark.parametrize("name", ALL)
+def test_oracle_stresses_match_reference_run(oracle_mod, name):
+    """Integration-point and element-nodal stresses of the reference's own
+    displacements (cElmParam[TL]::IntPntStress, NodalStress with the
+    default extrapolation) against what the reference printed for them."""
+    d = neutral.read(os.path.join(DECKS, name + ".dat"))
+    ref = load_golden("deck_%s.npz" % name)
+    assert np.array_equal(ref["elem_labels"], d.elem_labels)
+    si, sn, sa, ok = oracle_mod.OracleMesh(d.mesh).stresses(reference_state(d, ref))
+    assert ok
+    scale = np.abs(ref["sig_ip"]).max()
+    assert np.abs(si - ref["sig_ip"]).max() <= 1e-13 * scale
+    assert np.abs(sn - ref["sig_node"]).max() <= 1e-13 * scale
+
+
+def test_pos_round_trip(tmp_path):
+    rng = np.random.default_rng(3)
+    labels = np.arange(1, 9)
+    u = rng.standard_normal((8, 3))
+    si, sn, sa = rng.standard_normal((2, 8, 6)), rng.standard_normal((2, 8, 6)), \
+        rng.standard_normal((8, 6))
+    out = str(tmp_path / "a.pos")
+    neutral.write_pos(out, labels, u, 0.5, elem_labels=np.array([4, 7]), sig_ip=si,
+                      sig_node=sn, sig_avg=sa)
+    st = neutral.read_pos(out)
+    assert len(st) == 1 and st[0]["factor"] == 0.5
+    assert np.array_equal(st[0]["labels"], labels) and np.array_equal(st[0]["u"], u)
+    assert np.array_equal(st[0]["elem_labels"], [4, 7])
+    assert np.array_equal(np.stack(st[0]["sig_ip"]), si)
+    assert np.array_equal(np.stack(st[0]["sig_node"]), sn)
+    assert np.array_equal(st[0]["sig_avg"], sa)
 
 
 def assert_same_displacements(d, un, ref, name, tol=1e-10):

@@ -329,3 +329,84 @@ def test_bsp_mixed_degree_element_invariants(oracle_mod):
     inner = (i > 0) & (i < nb[0] - 1) & (j > 0) & (j < nb[1] - 1) & (k > 0) & (k < nb[2] - 1)
     fn = m.scatter(g)
     assert inner.sum() == 0 or np.abs(fn[inner]).max() <= 1e-12 * np.abs(fn).max()
+
+
+def _stress_meshes(M, form, mat):
+    return [M.hex8_block(3, 2, 2, size=(1.0, 0.6, 0.5), perturb=0.25, seed=3, form=form,
+                         mat=np.tile(mat, (12, 1))),
+            M.hex8_block(2, 2, 1, perturb=0.2, seed=4, form=form, nip=3, mat=np.tile(mat, (4, 1))),
+            M.bsp_block(ne=(2, 2, 2), p=3, perturb=0.1, seed=2, form=form, mat=np.tile(mat, (8, 1))),
+            M.bsp_block(ne=(2, 1, 2), p=(3, 3, 2), perturb=0.1, seed=5, form=form,
+                        mat=np.tile(mat, (4, 1)))]
+
+
+def test_stress_extrapolation_matrix(oracle_mod):
+    """CompTRMatrix: a least-squares linear fit, so rows sum to one, linear
+    fields at the points come out exactly at the nodes, and a single point
+    along r gives the mean."""
+    from fast_amd import mesh as M
+    rng = np.random.default_rng(0)
+    for m in _stress_meshes(M, "linear", (1.0, 0.3)):
+        om = oracle_mod.OracleMesh(m)
+        TR = om.stress_extrap()
+        pts = oracle_mod.brick_points(m.nip)[0].reshape(-1, 3)
+        assert TR.shape == (m.nn, len(pts))
+        assert np.allclose(TR.sum(axis=1), 1.0, rtol=0, atol=1e-14)
+        if m.shape == "hex8":
+            nodes = np.array([[-1, -1, 1], [1, -1, 1], [1, 1, 1], [-1, 1, 1],
+                              [-1, -1, -1], [1, -1, -1], [1, 1, -1], [-1, 1, -1]], float)
+        else:
+            g = [np.linspace(-1, 1, p + 1) for p in m.deg]
+            nodes = np.array([[r, s, t] for t in g[2] for s in g[1] for r in g[0]])
+        c = rng.standard_normal(4)
+        f = c[0] + pts @ c[1:]
+        assert np.allclose(TR @ f, c[0] + nodes @ c[1:], rtol=0, atol=1e-13)
+    m = M.hex8_block(2, 1, 1, nip=1)
+    assert np.allclose(oracle_mod.OracleMesh(m).stress_extrap(), 1.0)
+
+
+def _homogeneous(m, A):
+    return (m.coords - m.coords.mean(axis=0)) @ A.T
+
+
+@pytest.mark.parametrize("form,mat", [("linear", (2.0, 0.3)), ("tl_svk", (2.0, 0.3)),
+                                      ("tl_neohooke", (1.0, 1.5))])
+def test_stress_of_homogeneous_deformation(oracle_mod, form, mat):
+    """u = (F - I) X reproduces F exactly on every element shape, so every
+    point carries the closed-form stress: sigma = lambda tr(eps) I + 2 mu eps
+    (linear), S = lambda tr(E) I + 2 mu E with E = (F^T F - I) / 2 (SVK),
+    S = mu (I - C^-1) + lambda ln J C^-1 (neo-Hooke); the element-nodal and
+    averaged values equal it too.  A rigid rotation leaves S = 0 for the
+    total-Lagrangian forms."""
+    from fast_amd import mesh as M
+    rng = np.random.default_rng(1)
+    H = 0.08 * rng.standard_normal((3, 3))
+    F = np.eye(3) + H
+    if form == "linear":
+        E, nu = mat
+        lam, mu = E * nu / ((1 + nu) * (1 - 2 * nu)), E / (2 * (1 + nu))
+        eps = 0.5 * (H + H.T)
+        S = lam * np.trace(eps) * np.eye(3) + 2 * mu * eps
+    elif form == "tl_svk":
+        E, nu = mat
+        lam, mu = E * nu / ((1 + nu) * (1 - 2 * nu)), E / (2 * (1 + nu))
+        Eg = 0.5 * (F.T @ F - np.eye(3))
+        S = lam * np.trace(Eg) * np.eye(3) + 2 * mu * Eg
+    else:
+        mu, lam = mat
+        Cm = F.T @ F
+        Ci = np.linalg.inv(Cm)
+        S = mu * (np.eye(3) - Ci) + lam * 0.5 * np.log(np.linalg.det(Cm)) * Ci
+    voigt = np.array([S[0, 0], S[1, 1], S[2, 2], S[0, 1], S[0, 2], S[1, 2]])
+    th = 0.4
+    Q = np.array([[np.cos(th), -np.sin(th), 0], [np.sin(th), np.cos(th), 0], [0, 0, 1]])
+    for m in _stress_meshes(M, form, mat):
+        om = oracle_mod.OracleMesh(m)
+        si, sn, sa, ok = om.stresses(_homogeneous(m, H))
+        assert ok
+        assert np.abs(si - voigt).max() <= 1e-12 * np.abs(voigt).max()
+        assert np.abs(sn - voigt).max() <= 1e-12 * np.abs(voigt).max()
+        assert np.abs(sa - voigt).max() <= 1e-12 * np.abs(voigt).max()
+        if form != "linear":
+            si, _, _, ok = om.stresses(_homogeneous(m, Q - np.eye(3)))
+            assert ok and np.abs(si).max() <= 1e-12
