@@ -291,7 +291,7 @@ void destroy(Ctx *c) {
                   c->d_contrib, c->d_nptr, c->d_nadj, c->d_u,
                   c->d_b, c->d_x,
                   c->d_r, c->d_p, c->d_w, c->d_z, c->d_dinv, c->d_part,
-                  c->d_scal, c->d_done, c->d_eqnode, c->d_TR};
+                  c->d_scal, c->d_done, c->d_eqnode, c->d_TR, c->d_band, c->d_finfo};
   for (void *p : bufs)
     if (p) (void)hipFree(p);
   if (c->h_done) (void)hipHostFree(c->h_done);
@@ -411,6 +411,7 @@ void autotune(Ctx &c, const double *d_u) {
 
 int stiffness_dev(Ctx &c, const double *d_u) {
   if (!c.tuned) autotune(c, d_u);  // its launches count into the status: reset after
+  c.factored = false;
   FGPU_CHECK(hipMemsetAsync(c.d_status, 0, 4 * sizeof(int), c.stream));
   FGPU_CHECK(hipEventRecord(c.ev[0], c.stream));
   if (c.shape == FGPU_HEX8)
@@ -629,6 +630,7 @@ int fgpu_int_force(fgpu_ctx *ctx, const double *u, double *g) {
 int fgpu_add_diag(fgpu_ctx *ctx, int n, const int *eq, const double *v) {
   return guarded(ctx, [&](Ctx &c) {
     if (n <= 0) return 1;
+    c.factored = false;
     for (int k = 0; k < n; k++)
       if (eq[k] < 1 || eq[k] > c.neq) throw Error("fgpu_add_diag: equation out of range");
     int *d_eq = dupload(eq, n, c.stream);
@@ -711,6 +713,7 @@ int fgpu_get_csr(fgpu_ctx *ctx, long long *rowptr, int *col, double *val) {
 int fgpu_set_csr_values(fgpu_ctx *ctx, const double *val) {
   return guarded(ctx, [&](Ctx &c) {
     if (!val) throw Error("null argument");
+    c.factored = false;
     double *d = dupload(val, c.nnz, c.stream);
     launch_csr_convert(c, d, 0);
     FGPU_CHECK(hipStreamSynchronize(c.stream));
@@ -792,6 +795,62 @@ static void upload_stress_extrap(Ctx &c) {
   }
   c.d_TR = dupload(TR.data(), TR.size(), c.stream);
   FGPU_CHECK(hipStreamSynchronize(c.stream));
+}
+
+int fgpu_factor(fgpu_ctx *ctx) {
+  return guarded(ctx, [&](Ctx &c) { return band_factor(c); });
+}
+
+int fgpu_factor_solve_dev(fgpu_ctx *ctx, const double *b_dev, double *x_dev) {
+  return guarded(ctx, [&](Ctx &c) {
+    if (!b_dev || !x_dev) throw Error("null argument");
+    if (!c.factored)
+      throw Error("no factorization of the current matrix (call fgpu_factor after assembly)");
+    FGPU_CHECK(hipEventRecord(c.ev[5], c.stream));
+    band_solve(c, b_dev, x_dev);
+    FGPU_CHECK(hipEventRecord(c.ev[6], c.stream));
+    FGPU_CHECK(hipStreamSynchronize(c.stream));
+    c.last_ms[3] = elapsed(c, 5, 6);
+    return 1;
+  });
+}
+
+int fgpu_factor_solve(fgpu_ctx *ctx, const double *b, double *x) {
+  return guarded(ctx, [&](Ctx &c) {
+    if (!b || !x) throw Error("null argument");
+    double *d = dupload(b, std::max(c.neq, 1), c.stream);
+    int ok = fgpu_factor_solve_dev(ctx, d, d);
+    if (ok) {
+      FGPU_CHECK(hipMemcpyAsync(x, d, sizeof(double) * c.neq, hipMemcpyDeviceToHost, c.stream));
+      FGPU_CHECK(hipStreamSynchronize(c.stream));
+    }
+    (void)hipFree(d);
+    return ok;
+  });
+}
+
+int fgpu_direct_solve_dev(fgpu_ctx *ctx, const double *b_dev, double *x_dev) {
+  return guarded(ctx, [&](Ctx &c) {
+    if (!c.factored && !band_factor(c)) return 0;
+    return fgpu_factor_solve_dev(ctx, b_dev, x_dev);
+  });
+}
+
+int fgpu_direct_solve(fgpu_ctx *ctx, const double *b, double *x) {
+  return guarded(ctx, [&](Ctx &c) {
+    if (!c.factored && !band_factor(c)) return 0;
+    return fgpu_factor_solve(ctx, b, x);
+  });
+}
+
+int fgpu_factor_info(fgpu_ctx *ctx, int *half_band, int *neg_pivots, long long *bytes) {
+  return guarded(ctx, [&](Ctx &c) {
+    const int bw = band_width(c);
+    if (half_band) *half_band = bw;
+    if (neg_pivots) *neg_pivots = c.factored ? c.neg_pivots : -1;
+    if (bytes) *bytes = (long long)c.neq * (bw + 1) * (long long)sizeof(double);
+    return 1;
+  });
 }
 
 int fgpu_num_points(fgpu_ctx *ctx) {

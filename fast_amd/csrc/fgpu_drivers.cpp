@@ -115,6 +115,56 @@ int fgpu_linear_static(fgpu_ctx *ctx, const double *f, double *u, int maxit, dou
   });
 }
 
+int fgpu_linear_static_direct(fgpu_ctx *ctx, const double *f, double *u, double rtol,
+                              int *refinements, double *relres) {
+  return guarded(ctx, [&](Ctx &c) {
+    if (!f || !u) throw Error("null argument");
+    const int n = c.neq;
+    const size_t nb = sizeof(double) * n;
+    DevVec df(n), du(n), dr(n), dd(n), dg(n);
+    FGPU_CHECK(hipMemcpyAsync(df.p, f, nb, hipMemcpyHostToDevice, c.stream));
+    FGPU_CHECK(hipMemsetAsync(du.p, 0, nb, c.stream));
+    FGPU_CHECK(hipMemcpyAsync(dr.p, df.p, nb, hipMemcpyDeviceToDevice, c.stream));
+    if (!stiffness_dev(c, nullptr)) return 0;
+    if (!band_factor(c)) return 0;
+    // refinement as in fgpu_linear_static, each correction by the factors of
+    // the matrix assembled from FP32 element matrices
+    const double flen = std::sqrt(dev_dot(c, df.p, df.p, n));
+    double rel = flen > 0.0 ? 1.0 : 0.0;
+    int k = 0, conv = 0;
+    for (; k < 50; k++) {
+      if (rel <= rtol) {
+        conv = 1;
+        break;
+      }
+      band_solve(c, dr.p, dd.p);
+      launch_axpby(c, n, 1.0, du.p, 1.0, dd.p, du.p);
+      launch_eq_to_nodal(c, du.p, c.d_u);
+      if (!int_force_dev(c, c.d_u, dg.p)) return 0;
+      launch_axpby(c, n, 1.0, df.p, -1.0, dg.p, dr.p);
+      const double prev = rel;
+      rel = std::sqrt(dev_dot(c, dr.p, dr.p, n)) / flen;
+      if (rel > 0.9 * prev) {  // at the floor of the FP64 residual
+        k++;
+        break;
+      }
+    }
+    if (!conv && rel <= rtol) conv = 1;
+    FGPU_CHECK(hipMemcpyAsync(u, du.p, nb, hipMemcpyDeviceToHost, c.stream));
+    FGPU_CHECK(hipStreamSynchronize(c.stream));
+    if (refinements) *refinements = k;
+    if (relres) *relres = rel;
+    if (!conv) {
+      char msg[160];
+      std::snprintf(msg, sizeof msg,
+                    "linear static (direct): residual %.3e above the tolerance after %d solves",
+                    rel, k);
+      c.err = msg;
+    }
+    return conv;
+  });
+}
+
 int fgpu_newton(fgpu_ctx *ctx, const double *q, const fgpu_nr_params *prm, double *u_steps,
                 fgpu_nr_stat *stats) {
   return guarded(ctx, [&](Ctx &c) {
@@ -234,7 +284,12 @@ int fgpu_newton(fgpu_ctx *ctx, const double *q, const fgpu_nr_params *prm, doubl
         int pit = 0;
         double prr = 0;
         FGPU_CHECK(hipEventRecord(c.ev[5], c.stream));
-        run_pcg(c, dr.p, ddu.p, nullptr, pcg_maxit, ptol, &pit, &prr, prm->adaptive_pcg == 0);
+        if (prm->direct) {
+          if (!band_factor(c)) return 0;
+          band_solve(c, dr.p, ddu.p);
+        } else {
+          run_pcg(c, dr.p, ddu.p, nullptr, pcg_maxit, ptol, &pit, &prr, prm->adaptive_pcg == 0);
+        }
         FGPU_CHECK(hipEventRecord(c.ev[6], c.stream));
         FGPU_CHECK(hipEventSynchronize(c.ev[6]));
         float ms = 0;

@@ -197,6 +197,14 @@ struct Ctx {
   int *d_mdone = nullptr;             // [nbatch*4]
   int *d_ndone = nullptr;             // finished meshes
 
+  // ---- direct solve: LDL^T of the equation matrix in FP64 band storage,
+  // A(i, j) (j <= i <= j + bw) at d_band[j * (bw + 1) + i - j]
+  int band_bw = -1;            // half bandwidth (-1: not computed yet)
+  double *d_band = nullptr;    // [neq * (bw + 1)]
+  int *d_finfo = nullptr;      // [2] first pivot below the tolerance + 1, negative pivots
+  bool factored = false;       // d_band holds the factors of the current matrix
+  int neg_pivots = 0;
+
   // ---- timing
   hipEvent_t ev[8] = {};
   double last_ms[4] = {0, 0, 0, 0};
@@ -231,6 +239,12 @@ void ensure_eq_csr(Ctx &c);
 void launch_nodal_to_eq(Ctx &c, const double *nodal, double *eq);
 void launch_eq_to_nodal(Ctx &c, const double *eq, double *nodal);
 void launch_add_diag(Ctx &c, int n, const int *d_eq, const double *d_v);
+// direct solve (k_direct.hip): half bandwidth of the equation matrix; factor
+// the current global matrix into d_band (0 + c.err on a tiny pivot); solve
+// with the factors (b, x equation vectors, may alias)
+int band_width(Ctx &c);
+int band_factor(Ctx &c);
+void band_solve(Ctx &c, const double *b, double *x);
 void launch_expand_Ke(Ctx &c, int e0, int ne, float *d_out);
 void bsp_prepare_tables(Ctx &c, const double *h_ext, const double *h_cpw);
 void launch_axpby(Ctx &c, int n, double a, const double *x, double b, const double *y,

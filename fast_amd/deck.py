@@ -17,27 +17,43 @@ from . import neutral
 from .fgpu import FgpuError, GpuModel
 
 
-def run(path: str, out: str | None = None, tol: float = 1e-9, stresses: bool = True):
+MAX_DIRECT_BAND = 16384  # widest half band of the device direct solve
+
+
+def run(path: str, out: str | None = None, tol: float = 1e-9, stresses: bool = True,
+        solver: str | None = None):
     """Returns (deck, nodal displacements [nnode, 3] of the last step, stresses)
     with stresses = (integration points [nelem, nq, 6], element nodes
-    [nelem, nn, 6], node averages [nnode, 6] or None) of that step, or None."""
+    [nelem, nn, 6], node averages [nnode, 6] or None) of that step, or None.
+
+    solver: "direct" (the device LDL^T, the deck's default skyline matrix),
+    "pcg" (the deck's sparse matrix types), or None to follow the deck; a
+    band wider than the direct solve allows falls back to PCG."""
     d = neutral.read(path)
     if stresses and not d.stress_extrapolation:
         raise NotImplementedError("nodal stresses without extrapolation "
                                   "(HEADER.OUTPUT.STRESS.EXTRAPOLATION 0)")
     model = GpuModel(d.mesh)
+    if solver is None:
+        solver = "direct" if d.matrix_type.endswith("skyline") else "pcg"
+    if solver == "direct" and model.factor_info()[0] > MAX_DIRECT_BAND:
+        solver = "pcg"
     f = d.load.copy()
     if d.body is not None:
         f += model.body_force(d.body)
     if d.analysis == "linear":
-        u, it, rr = model.linear_static(f, maxit=max(100000, 200 * d.mesh.neq), tol=tol)
+        if solver == "direct":
+            u, it, rr = model.linear_static_direct(f, tol=tol)
+        else:
+            u, it, rr = model.linear_static(f, maxit=max(100000, 200 * d.mesh.neq), tol=tol)
         factor = 1.0
     else:
-        # ctrl/ctrlnr.cpp: f = lambda q per step, Newton to TOLERANCE.  The
-        # reference solves each correction directly; PCG stands in for that
-        # here, so its iteration limit is that of a direct-quality solve.
+        # ctrl/ctrlnr.cpp: f = lambda q per step, Newton to TOLERANCE.  With
+        # PCG for the corrections its iteration limit is that of a
+        # direct-quality solve.
         us, st, err = model.newton(f, steps=d.steps, factor=d.factor, maxit=d.maxit, tol=d.tol,
-                                   pcg_tol=tol, pcg_maxit=max(100000, 200 * d.mesh.neq))
+                                   pcg_tol=tol, pcg_maxit=max(100000, 200 * d.mesh.neq),
+                                   direct=solver == "direct")
         if err:
             raise FgpuError(err)
         u = us[-1]
@@ -58,8 +74,10 @@ def main(argv=None):
     ap.add_argument("deck")
     ap.add_argument("--out", default=None)
     ap.add_argument("--tol", type=float, default=1e-9)
+    ap.add_argument("--solver", choices=["direct", "pcg"], default=None,
+                    help="default: the deck's matrix type (skyline: direct)")
     a = ap.parse_args(argv)
-    d, un, sig = run(a.deck, a.out, a.tol)
+    d, un, sig = run(a.deck, a.out, a.tol, solver=a.solver)
     k = int(np.argmax(np.abs(un).max(axis=1)))
     print("%s: %d nodes, %d equations; largest displacement at node %d: %s"
           % (a.deck, d.mesh.nnode, d.mesh.neq, d.labels[k], un[k]))

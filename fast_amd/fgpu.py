@@ -44,7 +44,8 @@ class NrParams(C.Structure):
                 ("tol", C.c_double), ("pcg_maxit", C.c_int), ("pcg_tol", C.c_double),
                 ("adaptive_pcg", C.c_int), ("predict", PREDICT_FN), ("predict_user", C.c_void_p),
                 ("lambdas", C.c_void_p), ("u0_dev", C.c_void_p), ("lam0", C.c_double),
-                ("u_steps_dev", C.c_void_p), ("secant", C.c_int), ("galerkin", C.c_int)]
+                ("u_steps_dev", C.c_void_p), ("secant", C.c_int), ("galerkin", C.c_int),
+                ("direct", C.c_int)]
 
 
 class NrStat(C.Structure):
@@ -84,6 +85,14 @@ EXPORTS = {
     "fgpu_get_element_matrices": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_void_p]),
     "fgpu_body_force": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p]),
     "fgpu_body_force_dev": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p]),
+    "fgpu_factor": (C.c_int, [C.c_void_p]),
+    "fgpu_linear_static_direct": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_double,
+                                            C.c_void_p, C.c_void_p]),
+    "fgpu_factor_solve": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p]),
+    "fgpu_factor_solve_dev": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p]),
+    "fgpu_direct_solve": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p]),
+    "fgpu_direct_solve_dev": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p]),
+    "fgpu_factor_info": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),
     "fgpu_num_points": (C.c_int, [C.c_void_p]),
     "fgpu_stresses": (C.c_int, [C.c_void_p] + [C.c_void_p] * 4),
     "fgpu_stresses_dev": (C.c_int, [C.c_void_p] + [C.c_void_p] * 4),
@@ -240,9 +249,19 @@ class GpuModel:
         self._check(ok, "fgpu_linear_static")
         return u, it.value, rr.value
 
+    def linear_static_direct(self, f, tol=1e-10):
+        """cLinStat::Solver with the device LDL^T: (u, refinements, relres)."""
+        f = np.ascontiguousarray(f, dtype=np.float64)
+        u = np.zeros(self.neq)
+        k = C.c_int(0)
+        rr = C.c_double(0.0)
+        ok = lib().fgpu_linear_static_direct(self.h, _p(f), _p(u), tol, C.byref(k), C.byref(rr))
+        self._check(ok, "fgpu_linear_static_direct")
+        return u, k.value, rr.value
+
     def newton(self, q, steps, factor=0.0, maxit=20, tol=1e-5, pcg_tol=1e-5, pcg_maxit=0,
                adaptive_pcg=0, predict=None, lambdas=None, u0_dev=None, lam0=0.0,
-               u_steps_dev=None, secant=False, galerkin=False, keep_host=True):
+               u_steps_dev=None, secant=False, galerkin=False, direct=False, keep_host=True):
         """cCtrlNR::Solver on the device (ctrl/ctrlnr.cpp:47-267).
 
         ``predict`` is a Python callable ``(step, u_dev_ptr, lam_prev, lam,
@@ -250,13 +269,15 @@ class GpuModel:
         without a Python hop).  ``lambdas`` replaces the fixed increment
         ``factor`` by a list of load factors; ``u0_dev``/``lam0`` start from
         a converged state; ``u_steps_dev`` (device, steps x neq) receives the
-        converged displacement of every step."""
+        converged displacement of every step; ``direct`` solves each
+        correction with the device LDL^T instead of PCG."""
         q = np.ascontiguousarray(q, dtype=np.float64)
         prm = NrParams()
         prm.steps, prm.factor, prm.maxit, prm.tol = steps, factor, maxit, tol
         prm.pcg_maxit, prm.pcg_tol, prm.adaptive_pcg = pcg_maxit, pcg_tol, int(adaptive_pcg)
         prm.secant = int(secant)
         prm.galerkin = int(galerkin)
+        prm.direct = int(direct)
         cb = None
         if isinstance(predict, Gnn):
             prm.predict = C.cast(lib().fgpu_gnn_newton_predictor, PREDICT_FN)
@@ -306,6 +327,31 @@ class GpuModel:
         f = np.zeros(self.neq)
         self._check(lib().fgpu_body_force(self.h, _p(b), _p(f)), "fgpu_body_force")
         return f
+
+    def direct_solve(self, b):
+        """Solve K x = b with the device LDL^T of the current matrix (factorized
+        on first use after an assembly)."""
+        b = np.ascontiguousarray(b, dtype=np.float64)
+        x = np.zeros(self.neq)
+        self._check(lib().fgpu_direct_solve(self.h, _p(b), _p(x)), "fgpu_direct_solve")
+        return x
+
+    def factor(self):
+        self._check(lib().fgpu_factor(self.h), "fgpu_factor")
+
+    def factor_solve(self, b):
+        b = np.ascontiguousarray(b, dtype=np.float64)
+        x = np.zeros(self.neq)
+        self._check(lib().fgpu_factor_solve(self.h, _p(b), _p(x)), "fgpu_factor_solve")
+        return x
+
+    def factor_info(self):
+        """(half bandwidth, negative pivots of the last factorization or -1,
+        band bytes)."""
+        bw, neg, nb = C.c_int(0), C.c_int(0), C.c_longlong(0)
+        self._check(lib().fgpu_factor_info(self.h, C.byref(bw), C.byref(neg), C.byref(nb)),
+                    "fgpu_factor_info")
+        return bw.value, neg.value, nb.value
 
     def stresses(self, u_nodal, nodal=True, average=False):
         """Stresses of the nodal state u [nnode, 3] (Voigt xx yy zz xy xz yz):

@@ -38,7 +38,7 @@ _OUTPUT_ONLY = {
     "HEADER.OUTPUT.STRESS.EXTRAPOLATION", "HEADER.PRINT.EQUIVALENT.MESH",
     "HEADER.EQUIVALENT.MESH.DIVISION", "HEADER.OUTPUT.PATCH.STRESS.SMOOTH", "NODE",
     "ELEMENT", "MATERIAL", "SECTION", "HEADER.ANALYSIS.PRINT.STEPS",
-    # the solver kind: every system here is solved by PCG to a tight tolerance
+    # read below, with their data, for the runner's choices
     "HEADER.ANALYSIS.MATRIX.TYPE", "HEADER.PRINT.AVERAGE.NODAL.STRESS",
 }
 
@@ -64,6 +64,7 @@ class Deck:
     elem_labels: Optional[np.ndarray] = None  # row e of every element array
     stress_extrapolation: bool = True     # HEADER.OUTPUT.STRESS.EXTRAPOLATION
     average_stress: bool = False          # HEADER.PRINT.AVERAGE.NODAL.STRESS
+    matrix_type: str = "skyline"          # HEADER.ANALYSIS.MATRIX.TYPE
 
 
 def _sections(text: str) -> Dict[str, List[str]]:
@@ -253,6 +254,12 @@ def read(path: str) -> Deck:
     # ReadStrExtFlag; extrapolation on and averages off by default)
     if "HEADER.OUTPUT.STRESS.EXTRAPOLATION" in sec:
         deck.stress_extrapolation = int(sec["HEADER.OUTPUT.STRESS.EXTRAPOLATION"][0].split()[0]) != 0
+    # system matrix (ctrl/ctrl.cpp:213-238 ReadMatrixType: skyline, the
+    # direct solver, unless the deck names a sparse one)
+    if "HEADER.ANALYSIS.MATRIX.TYPE" in sec:
+        mt = sec["HEADER.ANALYSIS.MATRIX.TYPE"][0].split()[0].strip("'")
+        deck.matrix_type = mt if mt in ("skyline", "unsym_skyline", "sparse", "unsym_sparse",
+                                        "eigen_unsym_sparse") else "skyline"
     if "HEADER.PRINT.AVERAGE.NODAL.STRESS" in sec:
         deck.average_stress = int(sec["HEADER.PRINT.AVERAGE.NODAL.STRESS"][0].split()[0]) != 0
     # analysis (ctrl/ctrl.cpp:148-190 ReadAlg and the HEADER.ANALYSIS readers)

@@ -101,6 +101,29 @@ int fgpu_solve_dev(fgpu_ctx *ctx, const double *b_dev, double *x_dev,
                    const double *x0_dev, int maxit, double rtol, int *iters,
                    double *relres);
 
+/* Direct solve of the assembled global matrix, the reference's skyline
+ * solver: cSymSkylMatrix::Solve (mlib/sysmat.cpp:265-277) with CroutSolver
+ * (mlib/matvec.cpp:609-664), LDL^T without pivoting in FP64.  Storage is a
+ * band over the equation numbering (half bandwidth = the widest row
+ * profile, which contains the skyline and all fill); the factors stay on the
+ * device until the matrix changes (assembly, fgpu_add_diag,
+ * fgpu_set_csr_values).
+ *   fgpu_factor        factorize (CroutSolver op 2); returns 0 when a pivot
+ *                      is below 1e-12 in magnitude (the reference's TOLC,
+ *                      "singular matrix") or the half bandwidth exceeds 16384
+ *   fgpu_factor_solve  solve with the current factors (op 3)
+ *   fgpu_direct_solve  factorize when needed, then solve (op 1)
+ *   fgpu_factor_info   half bandwidth, negative pivots of the last
+ *                      factorization (cSymSkylMatrix::GetNgtPvt; -1 if none),
+ *                      band bytes
+ * b and x are equation vectors and may be the same array. */
+int fgpu_factor(fgpu_ctx *ctx);
+int fgpu_factor_solve(fgpu_ctx *ctx, const double *b, double *x);
+int fgpu_factor_solve_dev(fgpu_ctx *ctx, const double *b_dev, double *x_dev);
+int fgpu_direct_solve(fgpu_ctx *ctx, const double *b, double *x);
+int fgpu_direct_solve_dev(fgpu_ctx *ctx, const double *b_dev, double *x_dev);
+int fgpu_factor_info(fgpu_ctx *ctx, int *half_band, int *neg_pivots, long long *bytes);
+
 /* Batched solve: one Jacobi-PCG per mesh of a batch context, each with its
  * own |b|, scalars and stopping test (the reference solves each model on its
  * own), all meshes advancing in the same launches.  iters / relres (optional)
@@ -140,6 +163,13 @@ int fgpu_get_element_matrices(fgpu_ctx *ctx, int e0, int ne, float *Ke);
  * iters returns them, relres the final residual. */
 int fgpu_linear_static(fgpu_ctx *ctx, const double *f, double *u, int maxit,
                        double rtol, int *iters, double *relres);
+/* The same with the device LDL^T (fgpu_factor) in place of PCG: one
+ * factorization, then refinements with FP64 residuals until
+ * |f - K u| / |f| <= rtol or the residual stops falling (cLinStat::Solver
+ * with the reference's default skyline matrix).  refinements returns the
+ * number of solves with the factors. */
+int fgpu_linear_static_direct(fgpu_ctx *ctx, const double *f, double *u, double rtol,
+                              int *refinements, double *relres);
 
 /* Predictor hook for the Newton driver: given the converged equation vector
  * u_dev of the previous step (load factor lam_prev), the converged state
@@ -178,6 +208,9 @@ typedef struct {
                                of d and the secant increment that minimises
                                1/2 x.K x - x.r with the tangent K of the
                                previous state (two SpMVs) */
+  int direct;               /* 1: solve each correction with the device LDL^T
+                               of the tangent (fgpu_factor; the reference's
+                               skyline solver) instead of PCG */
 } fgpu_nr_params;
 
 typedef struct {

@@ -28,14 +28,16 @@ def fg():
     return fgpu
 
 
+@pytest.mark.parametrize("solver", ["direct", "pcg"])
 @pytest.mark.parametrize("name", ALL)
-def test_deck_matches_reference_run(fg, tmp_path, name):
+def test_deck_matches_reference_run(fg, tmp_path, name, solver):
     """Displacements and stresses of the last step (linear static, or load
-    control for the TL decks) against the reference program's output; the
-    written .pos file reads back the same."""
+    control for the TL decks), with the device direct solver (the decks'
+    default skyline matrix) and with PCG, against the reference program's
+    output; the written .pos file reads back the same."""
     from fast_amd import deck, neutral
     out = str(tmp_path / (name + ".pos"))
-    d, un, sig = deck.run(os.path.join(DECKS, name + ".dat"), out=out)
+    d, un, sig = deck.run(os.path.join(DECKS, name + ".dat"), out=out, solver=solver)
     ref = load_golden("deck_%s.npz" % name)
     assert_same_displacements(d, un, ref["u"], name, tol=1e-6)
     # (TestB8's free rigid rotation carries no small-strain stress)
