@@ -256,6 +256,55 @@ def run_warmstart(args):
     print(json.dumps(out))
 
 
+def run_newton(args):
+    """The 64^3 neo-Hookean cantilever under load control: 20 increments to
+    the tip traction 0.2, Newton to |r| / |f| <= 1e-8 with inexact PCG
+    corrections (FP32 element tangents, FP64 f_int and PCG)."""
+    import torch
+    from fast_amd import mesh as M, fgpu
+    n = 64
+    m = M.hex8_block(n, n, n, form="tl_neohooke", mat=np.tile((1.0, 1.5), (n ** 3, 1)),
+                     traction=(0.0, 0.0, -0.2))
+    model = fgpu.GpuModel(m)
+    q = m.load_vector()
+    kw = dict(steps=20, factor=0.05, maxit=30, tol=1e-8, pcg_tol=1.0, adaptive_pcg=2,
+              keep_host=False)
+    print("bench: newton, neq %d, 20 increments" % m.neq, file=sys.stderr, flush=True)
+    model.newton(q, **dict(kw, steps=1))  # warm-up: tuning, first launches
+    res = {}
+    for name, extra in (("plain", {}), ("secant", {"secant": True}),
+                        ("plain_mat32", {}), ("secant_mat32", {"secant": True})):
+        model.set_matrix_precision(32 if name.endswith("mat32") else 64)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        us, st, err = model.newton(q, **kw, **extra)
+        wall = time.perf_counter() - t0
+        if err:
+            raise RuntimeError("%s: %s" % (name, err))
+        res[name] = {"wall_s": round(wall, 4), "newton_iters": sum(x["iters"] for x in st),
+                     "pcg_iters": sum(x["pcg_iters"] for x in st),
+                     "ms_stiffness": round(sum(x["ms_stiffness"] for x in st), 2),
+                     "ms_fint": round(sum(x["ms_fint"] for x in st), 2),
+                     "ms_solve": round(sum(x["ms_solve"] for x in st), 2),
+                     "final_relres": float("%.3e" % st[-1]["err"])}
+        print("bench: %s %s" % (name, res[name]), file=sys.stderr, flush=True)
+    # headline: plain Newton (no predictor), PCG on the FP32 copy of the
+    # matrix; the FP64-matrix runs reach the same states (same iterations)
+    out = {
+        "metric": "load-control Newton wall time, 20 increments to traction 0.2",
+        "value": res["plain_mat32"]["wall_s"], "unit": "s", "n_gpus": 1, "steps": 20,
+        "warmup": 1, "ms_per_step": round(res["plain_mat32"]["wall_s"] * 1e3 / 20, 2),
+        "higher_is_better": False,
+        "scaling": "weak", "vs_baseline": None, "dtype": "fp32-mfma/f64",
+        "data": "synthetic (generated mesh)",
+        "config": {"workload": "newton_hex8_neohooke_64x64x64", "neq": int(m.neq),
+                   "mu": 1.0, "lambda": 1.5, "traction": 0.2, "increments": 20, "tol": 1e-8,
+                   "pcg_matrix": "fp32 copy of the fp64 assembly"},
+        "newton": res,
+    }
+    print(json.dumps(out))
+
+
 def run_copy(args):
     """Device-to-device copy of a 1 GiB FP64 buffer: the practical HBM
     bandwidth, and the calibration of the FETCH_SIZE / WRITE_SIZE counters
@@ -287,12 +336,14 @@ def main():
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--workload", default="iga",
-                    choices=["iga", "hex8", "batch", "warmstart", "copy"])
+                    choices=["iga", "hex8", "batch", "warmstart", "newton", "copy"])
     ap.add_argument("--nstates", type=int, default=2000, help="warmstart: training states")
     ap.add_argument("--train-steps", type=int, default=4000, help="warmstart: Adam steps")
     ap.add_argument("--beta", type=float, default=30.0, help="warmstart: roughness weight")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--solve", action="store_true", help="also time a plain PCG solve to 1e-8")
+    ap.add_argument("--serial", action="store_true",
+                    help="K, then f_int (default: f_int beside K on a second stream)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py: this build measures one GPU only (multi-GPU runs are not part of "
@@ -309,6 +360,9 @@ def main():
     if args.workload == "copy":
         run_copy(args)
         return
+    if args.workload == "newton":
+        run_newton(args)
+        return
 
     m, cfg, u = make_workload(args.workload)
     print("bench: workload %s nelem %d neq %d" % (cfg["workload"], m.nelem, m.neq),
@@ -321,13 +375,16 @@ def main():
     torch.cuda.synchronize()  # the library runs on its own non-blocking stream
 
     def step():
-        if not L.fgpu_assemble_K_dev(model.h, d_u):
+        if args.serial:
+            if not L.fgpu_assemble_K_dev(model.h, d_u):
+                raise fgpu.FgpuError(L.fgpu_last_error(model.h).decode())
+            if not L.fgpu_int_force_dev(model.h, d_u, g.data_ptr()):
+                raise fgpu.FgpuError(L.fgpu_last_error(model.h).decode())
+        # K (element + assembly) and f_int of the state, f_int on the
+        # context's second stream beside the stiffness
+        elif not L.fgpu_assemble_K_fint_dev(model.h, d_u, g.data_ptr()):
             raise fgpu.FgpuError(L.fgpu_last_error(model.h).decode())
-        tk, ta = L.fgpu_last_time_ms(model.h, 0), L.fgpu_last_time_ms(model.h, 1)
-        if not L.fgpu_int_force_dev(model.h, d_u, g.data_ptr()):
-            raise fgpu.FgpuError(L.fgpu_last_error(model.h).decode())
-        tf = L.fgpu_last_time_ms(model.h, 2)
-        return tk, ta, tf
+        return tuple(L.fgpu_last_time_ms(model.h, k) for k in range(3))
 
     for _ in range(args.warmup):
         step()
@@ -392,6 +449,7 @@ def main():
         "data": "synthetic (seeded generated mesh)", "config": cfg,
         "phase_ms": {"element_K": round(float(tk), 4), "assembly": round(float(ta), 4),
                      "f_int": round(float(tf), 4)},
+        "f_int_beside_K": not args.serial,
         "accuracy": {"Ku_normwise_rel": nrm, "Ku_forward_rel_max": fwd},
         "roofline": roof, "cpu_baseline": cpu, "solve_plain_pcg": solve,
     }

@@ -11,6 +11,7 @@
 //   * the node -> (element, local node) lists of the internal-force gather.
 // All integer maps are exact; no floating-point atomics are used anywhere.
 #include <algorithm>
+#include <chrono>
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -291,7 +292,8 @@ void destroy(Ctx *c) {
                   c->d_contrib, c->d_nptr, c->d_nadj, c->d_u,
                   c->d_b, c->d_x,
                   c->d_r, c->d_p, c->d_w, c->d_z, c->d_dinv, c->d_part,
-                  c->d_scal, c->d_done, c->d_eqnode, c->d_TR, c->d_band, c->d_finfo};
+                  c->d_scal, c->d_done, c->d_eqnode, c->d_TR, c->d_band, c->d_finfo,
+                  c->d_bval32};
   for (void *p : bufs)
     if (p) (void)hipFree(p);
   if (c->h_done) (void)hipHostFree(c->h_done);
@@ -305,6 +307,9 @@ void destroy(Ctx *c) {
   if (c->g_single) (void)hipGraphExecDestroy(c->g_single);
   if (c->g_batch) (void)hipGraphExecDestroy(c->g_batch);
   if (c->stream) (void)hipStreamDestroy(c->stream);
+  if (c->stream2) (void)hipStreamDestroy(c->stream2);
+  if (c->ev_fork) (void)hipEventDestroy(c->ev_fork);
+  if (c->d_status2) (void)hipFree(c->d_status2);
   delete c;
 }
 
@@ -412,6 +417,7 @@ void autotune(Ctx &c, const double *d_u) {
 int stiffness_dev(Ctx &c, const double *d_u) {
   if (!c.tuned) autotune(c, d_u);  // its launches count into the status: reset after
   c.factored = false;
+  c.bval32_valid = false;
   FGPU_CHECK(hipMemsetAsync(c.d_status, 0, 4 * sizeof(int), c.stream));
   FGPU_CHECK(hipEventRecord(c.ev[0], c.stream));
   if (c.shape == FGPU_HEX8)
@@ -439,6 +445,96 @@ int int_force_dev(Ctx &c, const double *d_u, double *d_g) {
   FGPU_CHECK(hipEventSynchronize(c.ev[4]));
   c.last_ms[2] = elapsed(c, 3, 4);
   return check_status(c, "internal force");
+}
+
+namespace {
+// launches issued while it lives go to the second stream and count into the
+// second status array (the launchers read c.stream / c.d_status)
+struct OnStream2 {
+  Ctx &c;
+  explicit OnStream2(Ctx &c_) : c(c_) {
+    std::swap(c.stream, c.stream2);
+    std::swap(c.d_status, c.d_status2);
+  }
+  ~OnStream2() {
+    std::swap(c.stream, c.stream2);
+    std::swap(c.d_status, c.d_status2);
+  }
+};
+}  // namespace
+
+int stiffness_fint_overlapped(Ctx &c, const double *d_u, double *d_g);
+
+// the overlapped and the serial order give the same bits; the first call
+// times both (unless autotuning is off) and keeps the faster
+int stiffness_fint_dev(Ctx &c, const double *d_u, double *d_g) {
+  if (!c.tuned) autotune(c, d_u);
+  if (c.overlap < 0) {
+    const char *env = std::getenv("FGPU_AUTOTUNE");
+    if (env && env[0] == '0') {
+      c.overlap = 0;
+    } else {
+      double best[2] = {1e30, 1e30};
+      for (int r = 0; r < 3; r++)
+        for (int mode = 0; mode < 2; mode++) {
+          FGPU_CHECK(hipStreamSynchronize(c.stream));
+          const auto t0 = std::chrono::steady_clock::now();
+          const int ok = mode ? stiffness_fint_overlapped(c, d_u, d_g)
+                              : (stiffness_dev(c, d_u) && int_force_dev(c, d_u, d_g));
+          if (!ok) return 0;
+          const double ms =
+              std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0)
+                  .count();
+          best[mode] = std::min(best[mode], ms);
+        }
+      c.overlap = best[1] < best[0] ? 1 : 0;
+      char buf[128];
+      std::snprintf(buf, sizeof buf, "K then f_int: %.4f ms; f_int beside the assembly: %.4f ms\n",
+                    best[0], best[1]);
+      c.tune_report += buf;
+    }
+  }
+  if (!c.overlap) return stiffness_dev(c, d_u) && int_force_dev(c, d_u, d_g);
+  return stiffness_fint_overlapped(c, d_u, d_g);
+}
+
+int stiffness_fint_overlapped(Ctx &c, const double *d_u, double *d_g) {
+  c.factored = false;
+  c.bval32_valid = false;
+  FGPU_CHECK(hipMemsetAsync(c.d_status, 0, 4 * sizeof(int), c.stream));
+  FGPU_CHECK(hipMemsetAsync(c.d_status2, 0, 4 * sizeof(int), c.stream));
+  FGPU_CHECK(hipEventRecord(c.ev[0], c.stream));
+  if (c.shape == FGPU_HEX8)
+    launch_hex8_K(c, d_u);
+  else
+    launch_bsp_K(c, d_u);
+  FGPU_CHECK(hipEventRecord(c.ev[1], c.stream));
+  // the element kernels of K and f_int both fill every CU with arithmetic;
+  // f_int starts with the assembly, which waits on memory
+  FGPU_CHECK(hipEventRecord(c.ev_fork, c.stream));
+  FGPU_CHECK(hipStreamWaitEvent(c.stream2, c.ev_fork, 0));
+  {
+    OnStream2 on(c);
+    FGPU_CHECK(hipEventRecord(c.ev[3], c.stream));
+    if (c.shape == FGPU_HEX8)
+      launch_hex8_fint(c, d_u);
+    else
+      launch_bsp_fint(c, d_u);
+    launch_gather_fint(c, d_g);
+    FGPU_CHECK(hipEventRecord(c.ev[4], c.stream));
+  }
+  launch_assemble_K(c);
+  FGPU_CHECK(hipEventRecord(c.ev[2], c.stream));
+  FGPU_CHECK(hipStreamWaitEvent(c.stream, c.ev[4], 0));  // join
+  FGPU_CHECK(hipStreamSynchronize(c.stream));
+  c.last_ms[0] = elapsed(c, 0, 1);
+  c.last_ms[1] = elapsed(c, 1, 2);
+  c.last_ms[2] = elapsed(c, 3, 4);
+  if (!check_status(c, "stiffness")) return 0;
+  std::swap(c.d_status, c.d_status2);
+  const int ok = check_status(c, "internal force");
+  std::swap(c.d_status, c.d_status2);
+  return ok;
 }
 
 }  // namespace fgpu
@@ -487,7 +583,9 @@ int fgpu_create(fgpu_ctx **out, const fgpu_mesh *m) {
     c->neq = neq;
     FGPU_CHECK(hipGetDevice(&c->device));
     FGPU_CHECK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+    FGPU_CHECK(hipStreamCreateWithFlags(&c->stream2, hipStreamNonBlocking));
     for (auto &e : c->ev) FGPU_CHECK(hipEventCreate(&e));
+    FGPU_CHECK(hipEventCreateWithFlags(&c->ev_fork, hipEventDisableTiming));
     hipStream_t s = c->stream;
     c->d_coords = dupload(m->coords, (size_t)m->nnode * 3, s);
     c->d_conn = dupload(m->conn, (size_t)m->nelem * m->nn, s);
@@ -515,6 +613,7 @@ int fgpu_create(fgpu_ctx **out, const fgpu_mesh *m) {
     c->d_gauss64 = dupload(gp, 3 * 2 * 16, s);
     c->d_gauss = dupload(gpf, 3 * 2 * 16, s);
     c->d_status = dalloc<int>(4);
+    c->d_status2 = dalloc<int>(4);
     build_maps(*c, m->conn);
     int nsym = nsym_blocks(c->nn);
     c->kel = KeLayout{c->shape == FGPU_HEX8 ? 64 : 1, nsym};
@@ -610,6 +709,26 @@ int fgpu_assemble_K(fgpu_ctx *ctx, const double *u) {
   });
 }
 
+int fgpu_assemble_K_fint_dev(fgpu_ctx *ctx, const double *d_u, double *d_g) {
+  return guarded(ctx, [&](Ctx &c) {
+    if (!d_u || !d_g) throw Error("null argument");
+    return stiffness_fint_dev(c, d_u, d_g);
+  });
+}
+
+int fgpu_assemble_K_fint(fgpu_ctx *ctx, const double *u, double *g) {
+  return guarded(ctx, [&](Ctx &c) {
+    if (!u || !g) throw Error("null argument");
+    FGPU_CHECK(hipMemcpyAsync(c.d_u, u, sizeof(double) * 3 * c.nnode,
+                              hipMemcpyHostToDevice, c.stream));
+    int ok = stiffness_fint_dev(c, c.d_u, c.d_b);
+    FGPU_CHECK(hipMemcpyAsync(g, c.d_b, sizeof(double) * c.neq, hipMemcpyDeviceToHost,
+                              c.stream));
+    FGPU_CHECK(hipStreamSynchronize(c.stream));
+    return ok;
+  });
+}
+
 int fgpu_int_force_dev(fgpu_ctx *ctx, const double *d_u, double *d_g) {
   return guarded(ctx, [&](Ctx &c) { return int_force_dev(c, d_u, d_g); });
 }
@@ -631,6 +750,7 @@ int fgpu_add_diag(fgpu_ctx *ctx, int n, const int *eq, const double *v) {
   return guarded(ctx, [&](Ctx &c) {
     if (n <= 0) return 1;
     c.factored = false;
+    c.bval32_valid = false;
     for (int k = 0; k < n; k++)
       if (eq[k] < 1 || eq[k] > c.neq) throw Error("fgpu_add_diag: equation out of range");
     int *d_eq = dupload(eq, n, c.stream);
@@ -714,6 +834,7 @@ int fgpu_set_csr_values(fgpu_ctx *ctx, const double *val) {
   return guarded(ctx, [&](Ctx &c) {
     if (!val) throw Error("null argument");
     c.factored = false;
+    c.bval32_valid = false;
     double *d = dupload(val, c.nnz, c.stream);
     launch_csr_convert(c, d, 0);
     FGPU_CHECK(hipStreamSynchronize(c.stream));
@@ -795,6 +916,14 @@ static void upload_stress_extrap(Ctx &c) {
   }
   c.d_TR = dupload(TR.data(), TR.size(), c.stream);
   FGPU_CHECK(hipStreamSynchronize(c.stream));
+}
+
+int fgpu_set_matrix_precision(fgpu_ctx *ctx, int bits) {
+  return guarded(ctx, [&](Ctx &c) {
+    if (bits != 32 && bits != 64) throw Error("matrix precision: 32 or 64 bits");
+    c.mat_bits = bits;
+    return 1;
+  });
 }
 
 int fgpu_factor(fgpu_ctx *ctx) {

@@ -36,12 +36,13 @@ __device__ inline double reduce_partials(const double *__restrict__ part, int n,
 }
 
 // Rows [r0, r1) of y = K x, G lanes per row; returns this thread's share of
-// dot(x, y) (nonzero only in lane 0 of a row group).
-template <int G>
+// dot(x, y) (nonzero only in lane 0 of a row group).  T: the stored value
+// type (the FP64 matrix, or its FP32 copy); the arithmetic is FP64.
+template <int G, class T = double>
 __device__ inline double bspmv_rows(long long r0, long long r1, long long stride,
                                     const long long *__restrict__ bptr,
                                     const int *__restrict__ bcol,
-                                    const double *__restrict__ bval,
+                                    const T *__restrict__ bval,
                                     const double *__restrict__ x, double *__restrict__ y) {
   const int sub = threadIdx.x % G;
   double dot = 0.0;
@@ -49,7 +50,10 @@ __device__ inline double bspmv_rows(long long r0, long long r1, long long stride
     double y0 = 0.0, y1 = 0.0, y2 = 0.0;
     const long long k1 = bptr[r + 1];
     for (long long k = bptr[r] + sub; k < k1; k += G) {
-      const double *v = bval + 9 * k;
+      const T *vp = bval + 9 * k;
+      double v[9];
+#pragma unroll
+      for (int m = 0; m < 9; m++) v[m] = (double)vp[m];
       const int B = bcol[k];
       const double x0 = x[3 * B], x1 = x[3 * B + 1], x2 = x[3 * B + 2];
       y0 += v[0] * x0 + v[1] * x1 + v[2] * x2;

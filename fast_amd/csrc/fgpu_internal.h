@@ -118,6 +118,11 @@ struct Ctx {
   std::string err;
   int device = 0;
   hipStream_t stream = nullptr;
+  // second stream (default priority) for f_int beside K, its status counters
+  // and the fork event
+  hipStream_t stream2 = nullptr;
+  int *d_status2 = nullptr;
+  hipEvent_t ev_fork = nullptr;
 
   // ---- device mesh
   double *d_coords = nullptr;   // [nnode*3]
@@ -148,6 +153,11 @@ struct Ctx {
   long long *d_bdiag = nullptr;    // [nnode] position of the diagonal block
   unsigned char *d_fmask = nullptr;  // [nnode] bit d set = direction d free
   double *d_bval = nullptr;        // [nbf*9]
+  // FP32 copy of d_bval for the PCG's products (mat_bits 32), refreshed on
+  // the first solve after the matrix changes
+  int mat_bits = 64;
+  float *d_bval32 = nullptr;
+  bool bval32_valid = false;
   long long nnz = 0;
   long long *d_rowptr = nullptr;  // [neq+1] equation CSR pattern
   int *d_col = nullptr;           // [nnz]
@@ -170,6 +180,7 @@ struct Ctx {
                          // 16 (mode & 3), XCD-contiguous workgroups (mode & 4);
                          // -1 default (2)
   bool tuned = false;
+  int overlap = -1;      // f_int beside the assembly (1) or after K (0); -1 untimed
   std::string tune_report;
 
   // ---- solver workspace
@@ -185,6 +196,7 @@ struct Ctx {
   int nblocks_red = 0;        // reduction blocks
   bool use_graphs = true;     // capture PCG iteration chunks into hipGraphs
   hipGraphExec_t g_single = nullptr, g_batch = nullptr;
+  int g_single_bits = 0, g_batch_bits = 0;  // matrix precision they were captured for
 
   // ---- batch of independent meshes (contiguous node ranges)
   int nbatch = 0;
@@ -239,6 +251,7 @@ void ensure_eq_csr(Ctx &c);
 void launch_nodal_to_eq(Ctx &c, const double *nodal, double *eq);
 void launch_eq_to_nodal(Ctx &c, const double *eq, double *nodal);
 void launch_add_diag(Ctx &c, int n, const int *d_eq, const double *d_v);
+void ensure_bval32(Ctx &c);  // d_bval32 = (float) d_bval when stale
 // direct solve (k_direct.hip): half bandwidth of the equation matrix; factor
 // the current global matrix into d_band (0 + c.err on a tiny pivot); solve
 // with the factors (b, x equation vectors, may alias)
@@ -254,5 +267,7 @@ double dev_dot(Ctx &c, const double *x, const double *y, int n);
 // the status check (returns 0 and sets c.err on failure)
 int stiffness_dev(Ctx &c, const double *d_unodal);
 int int_force_dev(Ctx &c, const double *d_unodal, double *d_g);
+// both for the same state, f_int on the second stream beside K
+int stiffness_fint_dev(Ctx &c, const double *d_unodal, double *d_g);
 
 }  // namespace fgpu

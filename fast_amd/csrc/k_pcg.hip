@@ -28,15 +28,15 @@ constexpr int CHUNK_IT = 16;  // iterations per captured graph
 // scal: [0] lenf  [1],[2] z.r by iteration parity  [3] |r|/lenf  [4] |b|
 //       [6] tol ; done[0] = iteration at convergence, done[1] = breakdown,
 //       done[2] = current iteration
-template <int G>
+template <int G, class T>
 __global__ __launch_bounds__(256) void pcg_spmv_kernel(
     int nnode, const long long *__restrict__ bptr, const int *__restrict__ bcol,
-    const double *__restrict__ bval, const double *__restrict__ p, double *__restrict__ w,
+    const T *__restrict__ bval, const double *__restrict__ p, double *__restrict__ w,
     double *__restrict__ part, const int *__restrict__ done) {
   __shared__ double sh[16];
   if (done[0]) return;
   const int rows_per_block = blockDim.x / G;
-  double dot = bspmv_rows<G>((long long)blockIdx.x * rows_per_block, nnode,
+  double dot = bspmv_rows<G, T>((long long)blockIdx.x * rows_per_block, nnode,
                              (long long)gridDim.x * rows_per_block, bptr, bcol, bval, p, w);
   double t = block_sum(dot, sh);
   if (threadIdx.x == 0) part[blockIdx.x] = t;
@@ -165,8 +165,12 @@ __global__ void residual_kernel(int n, const double *__restrict__ b, const doubl
 
 template <int G>
 void pcg_spmv_launch(Ctx &c) {
-  hipLaunchKernelGGL(pcg_spmv_kernel<G>, dim3(RED), dim3(TPB), 0, c.stream, c.nnode, c.d_bptr,
-                     c.d_bcol, c.d_bval, c.d_p, c.d_w, c.d_part, c.d_done);
+  if (c.mat_bits == 32)
+    hipLaunchKernelGGL((pcg_spmv_kernel<G, float>), dim3(RED), dim3(TPB), 0, c.stream, c.nnode,
+                       c.d_bptr, c.d_bcol, c.d_bval32, c.d_p, c.d_w, c.d_part, c.d_done);
+  else
+    hipLaunchKernelGGL((pcg_spmv_kernel<G, double>), dim3(RED), dim3(TPB), 0, c.stream, c.nnode,
+                       c.d_bptr, c.d_bcol, c.d_bval, c.d_p, c.d_w, c.d_part, c.d_done);
 }
 
 void enqueue_iteration(Ctx &c) {
@@ -195,18 +199,18 @@ __device__ inline double mesh_sum(const double *__restrict__ part, int c0, int c
   return s;
 }
 
-template <int G>
+template <int G, class T>
 __global__ __launch_bounds__(256) void bpcg_spmv_kernel(
     const long long *__restrict__ chn0, const long long *__restrict__ chn1,
     const int *__restrict__ chmesh, const long long *__restrict__ bptr,
-    const int *__restrict__ bcol, const double *__restrict__ bval,
+    const int *__restrict__ bcol, const T *__restrict__ bval,
     const double *__restrict__ p, double *__restrict__ w, double *__restrict__ part,
     const int *__restrict__ mdone) {
   __shared__ double sh[16];
   const int ch = blockIdx.x;
   if (mdone[4 * chmesh[ch]]) return;
   const int rows_per_block = blockDim.x / G;
-  double dot = bspmv_rows<G>(chn0[ch], chn1[ch], rows_per_block, bptr, bcol, bval, p, w);
+  double dot = bspmv_rows<G, T>(chn0[ch], chn1[ch], rows_per_block, bptr, bcol, bval, p, w);
   double t = block_sum(dot, sh);
   if (threadIdx.x == 0) part[ch] = t;
 }
@@ -345,9 +349,14 @@ __global__ void bpcg_step_kernel(int nmesh, const int *__restrict__ meshch,
 
 template <int G>
 void bpcg_spmv_launch(Ctx &c) {
-  hipLaunchKernelGGL(bpcg_spmv_kernel<G>, dim3(c.nchunk), dim3(TPB), 0, c.stream, c.d_chn0,
-                     c.d_chn1, c.d_chmesh, c.d_bptr, c.d_bcol, c.d_bval, c.d_p, c.d_w, c.d_part,
-                     c.d_mdone);
+  if (c.mat_bits == 32)
+    hipLaunchKernelGGL((bpcg_spmv_kernel<G, float>), dim3(c.nchunk), dim3(TPB), 0, c.stream,
+                       c.d_chn0, c.d_chn1, c.d_chmesh, c.d_bptr, c.d_bcol, c.d_bval32, c.d_p,
+                       c.d_w, c.d_part, c.d_mdone);
+  else
+    hipLaunchKernelGGL((bpcg_spmv_kernel<G, double>), dim3(c.nchunk), dim3(TPB), 0, c.stream,
+                       c.d_chn0, c.d_chn1, c.d_chmesh, c.d_bptr, c.d_bcol, c.d_bval, c.d_p,
+                       c.d_w, c.d_part, c.d_mdone);
 }
 
 void enqueue_batch_iteration(Ctx &c) {
@@ -376,7 +385,13 @@ void run_chunk(Ctx &c, bool batch) {
     FGPU_CHECK(hipGetLastError());
     return;
   }
+  int &bits = batch ? c.g_batch_bits : c.g_single_bits;
+  if (ge && bits != c.mat_bits) {  // captured for the other matrix precision
+    FGPU_CHECK(hipGraphExecDestroy(ge));
+    ge = nullptr;
+  }
   if (!ge) {
+    bits = c.mat_bits;
     hipGraph_t g;
     FGPU_CHECK(hipStreamBeginCapture(c.stream, hipStreamCaptureModeThreadLocal));
     for (int k = 0; k < CHUNK_IT; k++) batch ? enqueue_batch_iteration(c) : enqueue_iteration(c);
@@ -391,6 +406,7 @@ void prepare(Ctx &c, const double *b, const double *x0) {
   const int n = 3 * c.nnode;
   const int eb = (n + 255) / 256;
   hipStream_t s = c.stream;
+  if (c.mat_bits == 32) ensure_bval32(c);
   hipLaunchKernelGGL(diag_inv_kernel, dim3(eb), dim3(256), 0, s, c.nnode, c.d_bdiag, c.d_bval,
                      c.d_dinv);
   launch_eq_to_nodal(c, b, c.d_bn);

@@ -106,6 +106,12 @@ __global__ __launch_bounds__(256) void gather_fint_kernel(
   if ((q = eqmap[3 * A + 2]) >= 0) g[q] = s2;
 }
 
+__global__ __launch_bounds__(256) void to_f32_kernel(long long n, const double *__restrict__ a,
+                                                     float *__restrict__ b) {
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) b[i] = (float)a[i];
+}
+
 // cControl::PrintAvrNodalStress (ctrl/ctrl.cpp:2397-2470): the element-nodal
 // stresses of each node summed in element order, divided by their count.
 __global__ __launch_bounds__(256) void avg_stress_kernel(int nnode, const int *__restrict__ nptr,
@@ -297,6 +303,16 @@ void launch_gather_fint(Ctx &c, double *g) {
   hipLaunchKernelGGL(gather_fint_kernel, dim3((c.nnode + 255) / 256), dim3(256), 0, c.stream,
                      c.nnode, c.nn, c.d_nptr, c.d_nadj, c.d_ge, c.d_eqmap, g);
   FGPU_CHECK(hipGetLastError());
+}
+
+void ensure_bval32(Ctx &c) {
+  if (c.bval32_valid) return;
+  const long long n = c.nbf * 9;
+  if (!c.d_bval32) FGPU_CHECK(hipMalloc(&c.d_bval32, sizeof(float) * n));
+  hipLaunchKernelGGL(to_f32_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, c.stream, n,
+                     c.d_bval, c.d_bval32);
+  FGPU_CHECK(hipGetLastError());
+  c.bval32_valid = true;
 }
 
 void launch_avg_stress(Ctx &c, const double *sig_node, double *sig_avg) {

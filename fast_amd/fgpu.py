@@ -85,6 +85,9 @@ EXPORTS = {
     "fgpu_get_element_matrices": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_void_p]),
     "fgpu_body_force": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p]),
     "fgpu_body_force_dev": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p]),
+    "fgpu_assemble_K_fint": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p]),
+    "fgpu_assemble_K_fint_dev": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p]),
+    "fgpu_set_matrix_precision": (C.c_int, [C.c_void_p, C.c_int]),
     "fgpu_factor": (C.c_int, [C.c_void_p]),
     "fgpu_linear_static_direct": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_double,
                                             C.c_void_p, C.c_void_p]),
@@ -194,6 +197,14 @@ class GpuModel:
     def stiffness(self, u_nodal=None):
         u = None if u_nodal is None else np.ascontiguousarray(u_nodal, dtype=np.float64)
         self._check(lib().fgpu_assemble_K(self.h, _p(u)), "fgpu_assemble_K")
+
+    def stiffness_and_int_force(self, u_nodal):
+        """K of the state (kept in the context) and its internal force, the
+        internal force computed beside the stiffness on a second stream."""
+        u = np.ascontiguousarray(u_nodal, dtype=np.float64)
+        g = np.zeros(self.neq)
+        self._check(lib().fgpu_assemble_K_fint(self.h, _p(u), _p(g)), "fgpu_assemble_K_fint")
+        return g
 
     def int_force(self, u_nodal):
         u = np.ascontiguousarray(u_nodal, dtype=np.float64)
@@ -327,6 +338,11 @@ class GpuModel:
         f = np.zeros(self.neq)
         self._check(lib().fgpu_body_force(self.h, _p(b), _p(f)), "fgpu_body_force")
         return f
+
+    def set_matrix_precision(self, bits):
+        """32: the PCG multiplies by an FP32 copy of the matrix; 64: FP64."""
+        self._check(lib().fgpu_set_matrix_precision(self.h, int(bits)),
+                    "fgpu_set_matrix_precision")
 
     def direct_solve(self, b):
         """Solve K x = b with the device LDL^T of the current matrix (factorized

@@ -88,6 +88,16 @@ int fgpu_assemble_K_dev(fgpu_ctx *ctx, const double *u_nodal_dev);
 int fgpu_int_force(fgpu_ctx *ctx, const double *u_nodal, double *g);
 int fgpu_int_force_dev(fgpu_ctx *ctx, const double *u_nodal_dev, double *g_dev);
 
+/* Tangent and internal force of the same state in one call: the element
+ * stiffness and the assembly run on the context stream while the internal
+ * force (element vectors + gather) runs beside them on a second stream of
+ * the context; the results equal those of fgpu_assemble_K followed by
+ * fgpu_int_force, bit for bit.  What cCtrlNR::Solver needs of each
+ * updated state (ctrl/ctrlnr.cpp:203 IntForceVector, then 157
+ * StiffnessMatrix in the next iteration). */
+int fgpu_assemble_K_fint(fgpu_ctx *ctx, const double *u_nodal, double *g);
+int fgpu_assemble_K_fint_dev(fgpu_ctx *ctx, const double *u_nodal_dev, double *g_dev);
+
 /* Add v[k] to the diagonal of equation eq[k] (1-based), as the prescribed-DOF
  * penalty of ctrl/ctrl.cpp:960-974 does with cSysMatrix::Add. */
 int fgpu_add_diag(fgpu_ctx *ctx, int n, const int *eq, const double *v);
@@ -100,6 +110,16 @@ int fgpu_solve(fgpu_ctx *ctx, const double *b, double *x, const double *x0,
 int fgpu_solve_dev(fgpu_ctx *ctx, const double *b_dev, double *x_dev,
                    const double *x0_dev, int maxit, double rtol, int *iters,
                    double *relres);
+
+/* Precision of the matrix values the PCG reads (64, the default, or 32):
+ * with 32 every PCG of the context (fgpu_solve, the drivers) multiplies by an
+ * FP32 copy of the assembled FP64 matrix, refreshed on the first solve after
+ * the matrix changes; arithmetic, vectors and dot products stay FP64.  The
+ * matrix is assembled from FP32 element matrices, so the copy adds a
+ * rounding of the same order; the drivers' FP64 residuals correct it, and a
+ * plain fgpu_solve then solves the rounded matrix.  Halves the bytes of each
+ * iteration's matrix-vector product.  No counterpart in the reference. */
+int fgpu_set_matrix_precision(fgpu_ctx *ctx, int bits);
 
 /* Direct solve of the assembled global matrix, the reference's skyline
  * solver: cSymSkylMatrix::Solve (mlib/sysmat.cpp:265-277) with CroutSolver

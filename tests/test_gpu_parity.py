@@ -311,6 +311,37 @@ def test_linear_static_driver(fg, oracle_mod):
     assert nrel(m.scatter(u), d["u"]) <= 1e-8
 
 
+@pytest.mark.parametrize("kind", ["hex8_nh", "bsp_svk", "bsp332_linear"])
+def test_stiffness_beside_internal_force_identical_bits(fg, kind):
+    """fgpu_assemble_K_fint (f_int on the second stream beside K) gives the
+    matrix and the force of fgpu_assemble_K + fgpu_int_force, bit for bit,
+    and reports a failed stress update of either side."""
+    if kind == "hex8_nh":
+        m = M.hex8_block(5, 4, 3, form="tl_neohooke", perturb=0.2, seed=4,
+                         mat=np.tile((1.0, 1.5), (60, 1)))
+    elif kind == "bsp_svk":
+        m = M.bsp_block(ne=(3, 3, 2), p=3, perturb=0.05, seed=2, form="tl_svk",
+                        mat=np.tile((2.6, 0.3), (18, 1)))
+    else:
+        m = M.bsp_block(ne=(3, 2, 2), p=(3, 3, 2), perturb=0.05, seed=3)
+    X = m.coords
+    u = np.stack([0.02 * X[:, 0] * X[:, 2], 0.01 * X[:, 1] ** 2, -0.03 * X[:, 0] ** 2], 1)
+    u[m.fixed] = 0
+    g = fg.GpuModel(m)
+    g.stiffness(u)
+    K1 = g.csr()[2].copy()
+    f1 = g.int_force(u)
+    f2 = g.stiffness_and_int_force(u)
+    K2 = g.csr()[2]
+    assert np.array_equal(K1, K2) and np.array_equal(f1, f2)
+    if kind == "hex8_nh":
+        bad = u.copy()
+        bad[:, 0] = -X[:, 0]  # F_xx = 0: det C = 0 everywhere
+        with pytest.raises(fg.FgpuError, match="stress update failed"):
+            g.stiffness_and_int_force(bad)
+    g.close()
+
+
 @pytest.mark.parametrize("kind", ["hex8_nh", "bsp"])
 def test_launch_configurations_identical_bits(fg, oracle_mod, kind):
     """Every launch configuration the autotuner may pick (hex8 elements per
