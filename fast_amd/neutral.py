@@ -5,8 +5,9 @@ Supported: solid 8-node bricks (``ELEMENT.BRICK8`` / ``ELEMENT.BRICK8TL``) and
 B-spline solids of degree (3,3,3) or (3,3,2) (``ELEMENT.BSPLINE.SOLID`` /
 ``...SOLIDTL``) with isotropic elasticity (``MATERIAL.ISOTROPIC`` through
 ``SECTION.HOMOGENEOUS.ISOTROPIC.3D``), supports, the solver node order,
-uniform face loads on bricks in global axes, body forces and nodal forces,
-linear static analysis.  Any other keyword that changes the analysis raises
+uniform face loads on bricks in global or local axes, body forces and
+nodal forces, linear static and load-control Newton analyses, and the
+stress output of the result file.  Any other keyword that changes the analysis raises
 ``NotImplementedError`` naming it; keywords that only steer printing are
 skipped.
 
@@ -42,8 +43,11 @@ _OUTPUT_ONLY = {
     "HEADER.ANALYSIS.MATRIX.TYPE", "HEADER.PRINT.AVERAGE.NODAL.STRESS",
 }
 
-_HEX8_FACES = ((0, 1, 2, 3), (4, 5, 6, 7), (0, 1, 5, 4), (3, 2, 6, 7), (0, 3, 7, 4),
-               (1, 2, 6, 5))  # cyclic corner lists of the reference's Brick8
+# faces of the reference's Brick8 in its table order and node order
+# (cShapeBrick8::GetFace, shp/shpsolid.cpp:264-276): the node order fixes the
+# face's natural axes r, s and with them the local load axes
+_HEX8_FACES = ((0, 1, 2, 3), (3, 2, 6, 7), (1, 5, 6, 2), (0, 4, 5, 1), (0, 3, 7, 4),
+               (4, 7, 6, 5))
 
 
 @dataclasses.dataclass
@@ -350,9 +354,12 @@ def _bspline_elements(sec, rows, idx):
 
 
 def _face_loads(mesh, toks, idx):
-    """Uniform face loads on bricks, global axes (cFaceUnifLoad with
-    Local = 0, load/load.cpp:1440-1498; integrated on the element's Gauss
-    order over the bilinear face, cElemLoad::ExtForce, load.cpp:618-691)."""
+    """Uniform face loads on bricks (cFaceUnifLoad, load/load.cpp:1440-1498;
+    integrated on the element's Gauss order over the bilinear face,
+    cElemLoad::ExtForce, load.cpp:618-691).  Local = 1: the load components
+    act along the face's local axes at each point (cShapeSurf::LocalSys,
+    shp/shpsurf.cpp:121-164): u = dx/dr / |dx/dr|, v' = dx/ds / |dx/ds|,
+    w = u x v' (not normalised), v = w x u, load = q0 u + q1 v + q2 w."""
     tk = _Toks(toks, "LOAD.CASE.AREA.UNIFORM")
     f = np.zeros(mesh.neq)
     n = mesh.nip[0]
@@ -362,13 +369,11 @@ def _face_loads(mesh, toks, idx):
         corners = {idx[tk.int()] for _ in range(3)}
         local = tk.int()
         qv = np.asarray([tk.float(), tk.float(), tk.float()])
-        if local:
-            raise NotImplementedError("face loads in local axes")
         cn = mesh.conn[e]
         face = [fc for fc in _HEX8_FACES if corners <= {int(cn[a]) for a in fc}]
-        if len(face) != 1:
+        if not face:
             raise ValueError("face of element %d not found" % (e + 1))
-        nodes = [int(cn[a]) for a in face[0]]
+        nodes = [int(cn[a]) for a in face[0]]  # the first face in table order
         X = mesh.coords[nodes]
         for i in range(n):
             for j in range(n):
@@ -377,12 +382,19 @@ def _face_loads(mesh, toks, idx):
                                      (1 - s) * (1 + t)])
                 ds = 0.25 * np.array([-(1 - t), (1 - t), (1 + t), -(1 + t)])
                 dt = 0.25 * np.array([-(1 - s), -(1 + s), (1 + s), (1 - s)])
-                dA = np.linalg.norm(np.cross(ds @ X, dt @ X)) * w[i] * w[j]
+                xr, xs = ds @ X, dt @ X
+                dA = np.linalg.norm(np.cross(xr, xs)) * w[i] * w[j]
+                load = qv
+                if local:
+                    uu = xr / np.linalg.norm(xr)
+                    ww = np.cross(uu, xs / np.linalg.norm(xs))
+                    vv = np.cross(ww, uu)
+                    load = qv[0] * uu + qv[1] * vv + qv[2] * ww
                 for a, node in enumerate(nodes):
                     for d in range(3):
                         q = mesh.dofmap[node, d]
                         if q > 0:
-                            f[q - 1] += N[a] * dA * qv[d]
+                            f[q - 1] += N[a] * dA * load[d]
     return f
 
 

@@ -8,6 +8,8 @@ Decks in tests/golden/decks/:
   BeamB8.dat, BeamB8TL.dat  written by this script: a 4x2x2 cantilever of
       distorted bricks under tip forces, linear and total-Lagrangian
       (the latter under load control, 4 steps to 4 times the load);
+  BeamB8Face.dat  the same beam under a uniform load on its distorted top
+      faces in the faces' local axes (tangential and normal components);
   ScordelisLoDeg3TL.dat  the Scordelis-Lo deck with total-Lagrangian
       B-spline solids under load control (2 steps).
 This script runs the reference program built from its sources
@@ -37,15 +39,16 @@ sys.path.insert(0, ROOT)
 from fast_amd import neutral  # noqa: E402
 
 REF = os.path.join(ROOT, "oracle", "_ref", "fast")
-DECKS = ("TestB8", "ScordelisLoDeg3", "BeamB8", "BeamB8TL", "ScordelisLoDeg3TL")
+DECKS = ("TestB8", "ScordelisLoDeg3", "BeamB8", "BeamB8TL", "BeamB8Face", "ScordelisLoDeg3TL")
 LCM = "%HEADER.ANALYSIS.ALGORITHM\n'lcm'\n\n%HEADER.ANALYSIS.MAXIMUM.STEPS\n{steps}\n\n" \
       "%HEADER.ANALYSIS.STEP.FACTOR\n{factor}\n\n%HEADER.ANALYSIS.MAXIMUM.ITERATIONS\n30\n\n" \
       "%HEADER.ANALYSIS.TOLERANCE\n{tol}\n\n"
 
 
-def beam_deck(tl: bool) -> str:
+def beam_deck(tl: bool, face: bool = False) -> str:
     """4x2x2 bricks on [0,4]x[0,1]x[0,1], interior nodes moved by a seeded
-    +-0.1 h, clamped at x = 0, tip forces (y and z) at x = 4."""
+    +-0.1 h, clamped at x = 0, tip forces (y and z) at x = 4, or (face) a
+    uniform load in local axes on the top faces."""
     nx, ny, nz = 4, 2, 2
     rng = np.random.default_rng(11)
     nid = lambda i, j, k: 1 + i + (nx + 1) * (j + (ny + 1) * k)
@@ -78,9 +81,16 @@ def beam_deck(tl: bool) -> str:
                      nid(i, j + 1, k + 1), nid(i, j, k), nid(i + 1, j, k), nid(i + 1, j + 1, k),
                      nid(i, j + 1, k)]
                 out.append("%d  1  1  " % e + "  ".join(str(n) for n in c))
-    tip = [nid(nx, j, k) for k in range(nz + 1) for j in range(ny + 1)]
-    out += ["", "%LOAD", "1", "1  'tip'", "%LOAD.CASE", "1", "%LOAD.CASE.NODAL.FORCE", str(len(tip))]
-    out += ["%d  0.0  %.6g  %.6g  0.0  0.0  0.0" % (n, 0.05, -0.25) for n in tip]
+    out += ["", "%LOAD", "1", "1  'tip'", "%LOAD.CASE", "1"]
+    if face:
+        top = [(1 + i + nx * (j + ny * (nz - 1)), nid(i, j, nz), nid(i + 1, j, nz),
+                nid(i + 1, j + 1, nz)) for j in range(ny) for i in range(nx)]
+        out += ["%LOAD.CASE.AREA.UNIFORM", str(len(top))]
+        out += ["%d  %d  %d  %d  1  0.02  -0.01  -0.3" % t for t in top]
+    else:
+        tip = [nid(nx, j, k) for k in range(nz + 1) for j in range(ny + 1)]
+        out += ["%LOAD.CASE.NODAL.FORCE", str(len(tip))]
+        out += ["%d  0.0  %.6g  %.6g  0.0  0.0  0.0" % (n, 0.05, -0.25) for n in tip]
     out += ["", "%END", ""]
     return "\n".join(out)
 
@@ -90,6 +100,8 @@ def deck_text(name: str) -> str:
         return beam_deck(False)
     if name == "BeamB8TL":
         return beam_deck(True)
+    if name == "BeamB8Face":
+        return beam_deck(False, face=True)
     if name == "ScordelisLoDeg3TL":
         text = open(os.path.join(HERE, "decks", "ScordelisLoDeg3.dat")).read()
         text = text.replace("%ELEMENT.BSPLINE.SOLID\n", "%ELEMENT.BSPLINE.SOLIDTL\n", 1)

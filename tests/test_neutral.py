@@ -52,7 +52,7 @@ def test_solver_order_numbering():
     assert dm.max() == d.mesh.neq
 
 
-ALL = ["TestB8", "ScordelisLoDeg3", "BeamB8", "BeamB8TL", "ScordelisLoDeg3TL"]
+ALL = ["TestB8", "ScordelisLoDeg3", "BeamB8", "BeamB8TL", "BeamB8Face", "ScordelisLoDeg3TL"]
 
 
 def oracle_solution(oracle_mod, d):
