@@ -293,7 +293,8 @@ void destroy(Ctx *c) {
                   c->d_b, c->d_x,
                   c->d_r, c->d_p, c->d_w, c->d_z, c->d_dinv, c->d_part,
                   c->d_scal, c->d_done, c->d_eqnode, c->d_TR, c->d_band, c->d_finfo,
-                  c->d_bval32};
+                  c->d_bval32, c->d_gsym, c->d_gG, c->d_gR, c->d_gH, c->d_gPQ,
+                  c->d_gnodes};
   for (void *p : bufs)
     if (p) (void)hipFree(p);
   if (c->h_done) (void)hipHostFree(c->h_done);
@@ -552,10 +553,11 @@ int fgpu_create(fgpu_ctx **out, const fgpu_mesh *m) {
     if (m->shape == FGPU_BSP) {
       if (m->nn != (m->deg[0] + 1) * (m->deg[1] + 1) * (m->deg[2] + 1))
         throw Error("B-spline solid: nn != prod(p+1)");
-      if (m->deg[0] != 3 || m->deg[1] != 3 || (m->deg[2] != 3 && m->deg[2] != 2) ||
-          m->nip[0] != 4 || m->nip[1] != 4 || m->nip[2] != 4)
-        throw Error("B-spline solid kernels support degrees (3, 3, 3) and (3, 3, 2) with "
-                    "4x4x4 Gauss");
+      for (int d = 0; d < 3; d++)
+        if (m->deg[d] < 1 || m->deg[d] > 7 || m->nip[d] < 1 || m->nip[d] > 16)
+          throw Error("B-spline solids: degrees 1..7 and 1..16 Gauss points per direction");
+      if (m->nn > 512 || m->nip[0] * m->nip[1] * m->nip[2] > 512)
+        throw Error("B-spline solids: at most 512 control points and 512 points per element");
       if (!m->ext) throw Error("B-spline solid needs extraction operators");
     } else {
       for (int d = 0; d < 3; d++)
@@ -982,6 +984,15 @@ int fgpu_factor_info(fgpu_ctx *ctx, int *half_band, int *neg_pivots, long long *
   });
 }
 
+int fgpu_set_stress_nodes(fgpu_ctx *ctx, int extrapolate) {
+  return guarded(ctx, [&](Ctx &c) {
+    if (!extrapolate && (c.shape != FGPU_BSP || c.form == FGPU_TL_NEOHOOKE))
+      throw Error("nodal stresses evaluated at the nodes: B-spline solids with (E, nu) only");
+    c.stress_nodes = extrapolate ? 1 : 0;
+    return 1;
+  });
+}
+
 int fgpu_num_points(fgpu_ctx *ctx) {
   return guarded(ctx, [&](Ctx &c) { return c.nip[0] * c.nip[1] * c.nip[2]; });
 }
@@ -995,10 +1006,12 @@ int fgpu_stresses_dev(fgpu_ctx *ctx, const double *u_dev, double *sig_ip_dev,
     if (sig_avg_dev && !sig_node_dev)
       sig_node_dev = tmp = dalloc<double>((size_t)c.nelem * c.nn * 6);
     FGPU_CHECK(hipMemsetAsync(c.d_status, 0, 4 * sizeof(int), c.stream));
+    const bool at_nodes = c.stress_nodes == 0 && sig_node_dev;
     if (c.shape == FGPU_HEX8)
-      launch_hex8_stress(c, u_dev, sig_ip_dev, sig_node_dev);
+      launch_hex8_stress(c, u_dev, sig_ip_dev, at_nodes ? nullptr : sig_node_dev);
     else
-      launch_bsp_stress(c, u_dev, sig_ip_dev, sig_node_dev);
+      launch_bsp_stress(c, u_dev, sig_ip_dev, at_nodes ? nullptr : sig_node_dev);
+    if (at_nodes) gen_launch_node_stress(c, u_dev, sig_node_dev);
     if (sig_avg_dev) launch_avg_stress(c, sig_node_dev, sig_avg_dev);
     const int ok = check_status(c, "stresses");
     if (tmp) (void)hipFree(tmp);

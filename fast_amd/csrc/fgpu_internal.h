@@ -141,6 +141,14 @@ struct Ctx {
   double *d_ge = nullptr;       // element internal forces [nelem*nn*3]
   int *d_status = nullptr;      // [2]: bad-Jacobian count, stress failure
   double *d_TR = nullptr;       // stress extrapolation points -> nodes [nn*nq]
+  // B-spline solids outside the tuned cubic kernels (k_bspg.hip): upper block
+  // pairs, per-chunk scratch (gradients, basis values, F g / F^-T g, point data)
+  int *d_gsym = nullptr;        // [nsym] int2 (a, b)
+  double *d_gG = nullptr, *d_gR = nullptr, *d_gH = nullptr, *d_gPQ = nullptr;
+  int gen_chunk = 0;            // elements per scratch chunk
+  size_t gen_cap = 0, gen_cap_pq = 0;  // entries the scratch was sized for
+  double *d_gnodes = nullptr;   // natural coordinates of the nodes, per direction
+  int stress_nodes = 1;         // 1: extrapolated nodal stresses, 0: evaluated at the nodes
 
   // ---- global matrix: node-block rows of 3x3 FP64 blocks (all nodes; a
   // fixed direction has a unit diagonal and no coupling), the solver's
@@ -234,6 +242,14 @@ void launch_bsp_body(Ctx &c, const double *d_b);
 void launch_hex8_stress(Ctx &c, const double *d_unodal, double *sig_ip, double *sig_node);
 void launch_bsp_stress(Ctx &c, const double *d_unodal, double *sig_ip, double *sig_node);
 void launch_avg_stress(Ctx &c, const double *sig_node, double *sig_avg);
+// B-spline solids of any degree / Gauss order (k_bspg.hip); bsp_generic is
+// true when the tuned cubic kernels do not cover the element
+bool bsp_generic(const Ctx &c);
+void gen_prepare(Ctx &c, const double *h_ext);
+void gen_launch_K(Ctx &c, const double *d_unodal);
+void gen_launch_vec(Ctx &c, const double *d_unodal, const double *d_b);  // f_int, or body if b
+void gen_launch_stress(Ctx &c, const double *d_unodal, double *sig_ip, double *sig_node);
+void gen_launch_node_stress(Ctx &c, const double *d_unodal, double *sig_node);
 void launch_assemble_K(Ctx &c);
 std::vector<int> hex8_te_options(const Ctx &c);     // valid hex8 K workgroup sizes
 std::vector<int> assembly_options(const Ctx &c);    // valid asm_mode values

@@ -767,6 +767,15 @@ void set_lds(K kernel, size_t lds) {
 }  // namespace
 
 void bsp_prepare_tables(Ctx &c, const double *h_ext, const double *h_cpw) {
+  if (h_cpw) {
+    FGPU_CHECK(hipMalloc(&c.d_cpw, c.nnode * sizeof(double)));
+    FGPU_CHECK(hipMemcpyAsync(c.d_cpw, h_cpw, c.nnode * sizeof(double), hipMemcpyHostToDevice,
+                              c.stream));
+  }
+  if (bsp_generic(c)) {
+    gen_prepare(c, h_ext);  // the caller's packed layout as it is
+    return;
+  }
   // extraction rows stay FP64 on the device, [e][dir][4][4]: the caller's
   // packed (p_d + 1)^2 per direction, zero-padded for degree 2
   const size_t n = (size_t)c.nelem * 3 * 16;
@@ -783,12 +792,9 @@ void bsp_prepare_tables(Ctx &c, const double *h_ext, const double *h_cpw) {
     }
   }
   FGPU_CHECK(hipMalloc(&c.d_ext, n * sizeof(double)));
-  FGPU_CHECK(hipMemcpy(c.d_ext, pad.data(), n * sizeof(double), hipMemcpyHostToDevice));
-  if (h_cpw) {
-    FGPU_CHECK(hipMalloc(&c.d_cpw, c.nnode * sizeof(double)));
-    FGPU_CHECK(hipMemcpyAsync(c.d_cpw, h_cpw, c.nnode * sizeof(double), hipMemcpyHostToDevice,
-                              c.stream));
-  }
+  FGPU_CHECK(hipMemcpyAsync(c.d_ext, pad.data(), n * sizeof(double), hipMemcpyHostToDevice,
+                            c.stream));
+  FGPU_CHECK(hipStreamSynchronize(c.stream));
 }
 
 namespace {
@@ -887,14 +893,18 @@ void launch_stress_nb(Ctx &c, const double *u, double *sig_ip, double *sig_node)
 }  // namespace
 
 void launch_bsp_stress(Ctx &c, const double *u, double *sig_ip, double *sig_node) {
-  if (c.deg[2] == 2)
+  if (bsp_generic(c))
+    gen_launch_stress(c, u, sig_ip, sig_node);
+  else if (c.deg[2] == 2)
     launch_stress_nb<3>(c, u, sig_ip, sig_node);
   else
     launch_stress_nb<4>(c, u, sig_ip, sig_node);
 }
 
 void launch_bsp_body(Ctx &c, const double *b) {
-  if (c.deg[2] == 2)
+  if (bsp_generic(c))
+    gen_launch_vec(c, nullptr, b);
+  else if (c.deg[2] == 2)
     launch_body_nb<3>(c, b);
   else
     launch_body_nb<4>(c, b);
@@ -906,14 +916,18 @@ void launch_bsp_K(Ctx &c, const double *u) {
     FGPU_CHECK(hipMemsetAsync(c.d_u, 0, sizeof(double) * 3 * c.nnode, c.stream));
     u = c.d_u;
   }
-  if (c.deg[2] == 2)
+  if (bsp_generic(c))
+    gen_launch_K(c, u);
+  else if (c.deg[2] == 2)
     launch_K_nb<3>(c, u);
   else
     launch_K_nb<4>(c, u);
 }
 
 void launch_bsp_fint(Ctx &c, const double *u) {
-  if (c.deg[2] == 2)
+  if (bsp_generic(c))
+    gen_launch_vec(c, u, nullptr);
+  else if (c.deg[2] == 2)
     launch_fint_nb<3>(c, u);
   else
     launch_fint_nb<4>(c, u);

@@ -30,10 +30,9 @@ def run(path: str, out: str | None = None, tol: float = 1e-9, stresses: bool = T
     "pcg" (the deck's sparse matrix types), or None to follow the deck; a
     band wider than the direct solve allows falls back to PCG."""
     d = neutral.read(path)
-    if stresses and not d.stress_extrapolation:
-        raise NotImplementedError("nodal stresses without extrapolation "
-                                  "(HEADER.OUTPUT.STRESS.EXTRAPOLATION 0)")
     model = GpuModel(d.mesh)
+    if stresses and not d.stress_extrapolation:
+        model.set_stress_nodes(False)  # HEADER.OUTPUT.STRESS.EXTRAPOLATION 0
     if solver is None:
         solver = "direct" if d.matrix_type.endswith("skyline") else "pcg"
     if solver == "direct" and model.factor_info()[0] > MAX_DIRECT_BAND:

@@ -96,6 +96,7 @@ EXPORTS = {
     "fgpu_direct_solve": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p]),
     "fgpu_direct_solve_dev": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p]),
     "fgpu_factor_info": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),
+    "fgpu_set_stress_nodes": (C.c_int, [C.c_void_p, C.c_int]),
     "fgpu_num_points": (C.c_int, [C.c_void_p]),
     "fgpu_stresses": (C.c_int, [C.c_void_p] + [C.c_void_p] * 4),
     "fgpu_stresses_dev": (C.c_int, [C.c_void_p] + [C.c_void_p] * 4),
@@ -368,6 +369,12 @@ class GpuModel:
         self._check(lib().fgpu_factor_info(self.h, C.byref(bw), C.byref(neg), C.byref(nb)),
                     "fgpu_factor_info")
         return bw.value, neg.value, nb.value
+
+    def set_stress_nodes(self, extrapolate):
+        """True: nodal stresses by extrapolation from the points (default);
+        False: evaluated at the nodes (B-spline solids)."""
+        self._check(lib().fgpu_set_stress_nodes(self.h, int(bool(extrapolate))),
+                    "fgpu_set_stress_nodes")
 
     def stresses(self, u_nodal, nodal=True, average=False):
         """Stresses of the nodal state u [nnode, 3] (Voigt xx yy zz xy xz yz):

@@ -325,6 +325,12 @@ int fgpu_body_force_dev(fgpu_ctx *ctx, const double *b_dev, double *f_dev);
  *            NULL.
  * Returns 0 on a failed stress update or an invalid Jacobian. */
 int fgpu_num_points(fgpu_ctx *ctx);
+/* extrapolate 0: sig_node holds the stresses evaluated at each element's
+ * nodes instead (HEADER.OUTPUT.STRESS.EXTRAPOLATION 0: NodalStress through
+ * cElmParam::PntStress, elmparam.cpp:1206-1290, at the nodes' natural
+ * coordinates with the linear strain B u and the linear elastic material);
+ * B-spline solids with (E, nu) materials only.  1 (default): extrapolation. */
+int fgpu_set_stress_nodes(fgpu_ctx *ctx, int extrapolate);
 int fgpu_stresses(fgpu_ctx *ctx, const double *u_nodal, double *sig_ip, double *sig_node,
                   double *sig_avg);
 int fgpu_stresses_dev(fgpu_ctx *ctx, const double *u_nodal_dev, double *sig_ip_dev,

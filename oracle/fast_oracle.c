@@ -847,6 +847,38 @@ int orc_assemble_body(const orc_mesh *m, const double *b, int neq, double *f)
   return ok;
 }
 
+/* Nodal stresses without extrapolation (HEADER.OUTPUT.STRESS.EXTRAPOLATION
+ * 0): cElmParam::NodalStress (elc/elmparam.cpp:1313-1333) through PntStress
+ * (1206-1290) at the nodes' natural coordinates: e = B u (the linear strain,
+ * also for total-Lagrangian elements, which do not override PntStress) and
+ * s = C e of the linear elastic material.  sig_node [nelem*nn*6]. */
+int orc_node_stresses(const orc_mesh *m, const double *u_nodal, double *sig_node)
+{
+  int nn = m->et.nn, nd = 3 * nn;
+  size_t es = ext_stride(&m->et);
+  double cn[3 * MAXNN];
+  node_nat_coords(&m->et, cn);
+  double X[3 * MAXNN], ue[3 * MAXNN], we[MAXNN];
+  double *B = malloc(sizeof(double) * 6 * nd);
+  double *d = malloc(sizeof(double) * 3 * nn);
+  int ok = 1;
+  for (int e = 0; e < m->nelem; e++) {
+    gather_elem(m, e, u_nodal, X, ue, we, NULL);
+    const double *ext = m->ext ? m->ext + (size_t)e * es : NULL;
+    for (int a = 0; a < nn; a++) {
+      double detJ, eps[6];
+      orc_shape_dxyz(&m->et, X, ext, we, cn + 3 * a, &detJ, d);
+      bmatrix(nn, d, B);
+      matvec(6, nd, B, ue, eps);
+      if (!material(ORC_LINEAR, m->mat + 2 * (size_t)e, eps,
+                    sig_node + ((size_t)e * nn + a) * 6, NULL))
+        ok = 0;
+    }
+  }
+  free(B); free(d);
+  return ok;
+}
+
 /* Stress output of the whole mesh: cControl::PrintIntPntStress
  * (ctrl/ctrl.cpp:2189-2259: IntPntStress per element), PrintNodalStress
  * (2264-2330: cElmParam::NodalStress, elmparam.cpp:1304-1333, with the

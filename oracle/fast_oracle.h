@@ -86,6 +86,7 @@ int  orc_elem_stress(const orc_etype *et, const double *X, const double *ext,
 int  orc_stress_extrap(const orc_etype *et, double *TR);
 int  orc_stresses(const orc_mesh *m, const double *u_nodal, double *sig_ip,
                   double *sig_node, double *sig_avg);
+int  orc_node_stresses(const orc_mesh *m, const double *u_nodal, double *sig_node);
 /* Full CSR sparsity pattern of the equations (two-pass). */
 long long orc_csr_count(const orc_mesh *m, int neq, long long *rowptr);
 void orc_csr_fill(const orc_mesh *m, int neq, const long long *rowptr, int *col);

@@ -58,6 +58,7 @@ def lib():
         L.orc_elem_stress.argtypes = [vp, vp, vp, vp, vp, vp, vp]
         L.orc_stress_extrap.argtypes = [vp, vp]
         L.orc_stresses.argtypes = [vp, vp, vp, vp, vp]
+        L.orc_node_stresses.argtypes = [vp, vp, vp]
         L.orc_csr_count.argtypes = [vp, i, vp]
         L.orc_csr_count.restype = C.c_longlong
         L.orc_csr_fill.argtypes = [vp, i, vp, vp]
@@ -215,6 +216,15 @@ class OracleMesh:
         sa = np.zeros((m.nnode, 6))
         ok = lib().orc_stresses(C.byref(self.om), _p(u), _p(si), _p(sn), _p(sa))
         return si, sn, sa, ok
+
+    def node_stresses(self, u_nodal):
+        """Stresses evaluated at the element nodes' natural coordinates with
+        the linear strain (no extrapolation): [nelem, nn, 6] and the flag."""
+        m = self.mesh
+        u = np.ascontiguousarray(u_nodal, dtype=np.float64)
+        sn = np.zeros((m.nelem, m.nn, 6))
+        ok = lib().orc_node_stresses(C.byref(self.om), _p(u), _p(sn))
+        return sn, ok
 
     def shape_dxyz(self, e, rst):
         X, ext, w, _ = self.elem_data(e)
