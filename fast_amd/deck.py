@@ -45,7 +45,7 @@ def run(path: str, out: str | None = None, tol: float = 1e-9, stresses: bool = T
             u, it, rr = model.linear_static_direct(f, tol=tol)
         else:
             u, it, rr = model.linear_static(f, maxit=max(100000, 200 * d.mesh.neq), tol=tol)
-        factor = 1.0
+        us, factors = [u], [1.0]
     else:
         # ctrl/ctrlnr.cpp: f = lambda q per step, Newton to TOLERANCE.  With
         # PCG for the corrections its iteration limit is that of a
@@ -55,17 +55,20 @@ def run(path: str, out: str | None = None, tol: float = 1e-9, stresses: bool = T
                                    direct=solver == "direct")
         if err:
             raise FgpuError(err)
-        u = us[-1]
-        factor = d.steps * d.factor
-    un = d.mesh.scatter(u)
-    sig = model.stresses(un, nodal=True, average=d.average_stress) if stresses else None
+        factors = [s["lam"] for s in st]
+    # every step is printed, as the reference does at its default print step
+    steps = []
+    for u, lam in zip(us, factors):
+        un = d.mesh.scatter(u)
+        sig = model.stresses(un, nodal=True, average=d.average_stress) if stresses else None
+        steps.append(dict(factor=lam, u=un, sig_ip=sig[0] if sig else None,
+                          sig_node=sig[1] if sig else None, sig_avg=sig[2] if sig else None))
     if out:
-        kw = {}
-        if sig is not None:
-            kw = dict(elem_labels=d.elem_labels, sig_ip=sig[0], sig_node=sig[1], sig_avg=sig[2])
-        neutral.write_pos(out, d.labels, un, factor, **kw)
+        neutral.write_pos_steps(out, d.labels, steps, elem_labels=d.elem_labels)
     model.close()
-    return d, un, sig
+    last = steps[-1]
+    sig = (last["sig_ip"], last["sig_node"], last["sig_avg"]) if stresses else None
+    return d, last["u"], sig
 
 
 def main(argv=None):

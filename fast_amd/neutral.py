@@ -407,46 +407,63 @@ def _fmt(v, digits):
 
 def write_pos(path: str, labels: np.ndarray, u_nodal: np.ndarray, factor: float = 1.0,
               digits: int = 16, elem_labels=None, sig_ip=None, sig_node=None, sig_avg=None):
-    """One step in the result layout of the reference's .pos file: nodal
-    displacements (ctrl/ctrl.cpp:1117-1175: label and six components per
-    node) and, when given, the stresses of cControl::PrintIntPntStress
+    """One step in the result layout of the reference's .pos file (see
+    write_pos_steps)."""
+    write_pos_steps(path, labels, [dict(factor=factor, u=u_nodal, sig_ip=sig_ip,
+                                        sig_node=sig_node, sig_avg=sig_avg)],
+                    elem_labels=elem_labels, digits=digits)
+
+
+def write_pos_steps(path: str, labels: np.ndarray, steps, elem_labels=None, digits: int = 16):
+    """Result steps in the layout of the reference's .pos file: the case
+    header listing the steps (ctrl/ctrl.cpp:1117-1175), then per step its
+    load factor, the nodal displacements (label and six components per node)
+    and, when given, the stresses of cControl::PrintIntPntStress
     (ctrl.cpp:2189-2259: per element its label, point count and one line per
-    point), PrintNodalStress (2264-2330: per element its label and one line per
-    element node) and PrintAvrNodalStress (2397-2470: one line per node).
-    Stress components in Voigt order xx yy zz xy xz yz."""
+    point), PrintNodalStress (2264-2330: per element its label and one line
+    per element node) and PrintAvrNodalStress (2397-2470: one line per
+    node).  steps: dicts with factor, u [nnode, 3] and optional sig_ip,
+    sig_node, sig_avg; stress components in Voigt order xx yy zz xy xz yz."""
+    head = "6\n" + "  ".join("'%s'" % k for k in STRESS_LABELS) + "  \n\n"
     with open(path, "w") as out:
-        out.write("%RESULT\n1\n1  'result1'\n\n%RESULT.CASE\n1\n\n")
-        out.write("%%RESULT.CASE.STEP\n1\n\n%%RESULT.CASE.STEP.FACTOR\n%.*e\n\n" % (digits, factor))
-        out.write("%%RESULT.CASE.STEP.NODAL.DISPLACEMENT\n%d  'displacement'\n" % len(labels))
-        for lab, u in zip(labels, u_nodal):
-            out.write("%-5d " % lab + "  ".join(_fmt(v, digits) for v in
-                                                  list(u) + [0.0, 0.0, 0.0]) + "  \n")
+        out.write("%%RESULT\n1\n1  'result1'\n\n%%RESULT.CASE\n1   %d\n" % len(steps))
+        for k in range(len(steps)):
+            out.write("%d   'step%d'\n" % (k + 1, k + 1))
         out.write("\n")
-        head = "6\n" + "  ".join("'%s'" % k for k in STRESS_LABELS) + "  \n\n"
-        if elem_labels is None and (sig_ip is not None or sig_node is not None):
-            elem_labels = np.arange(1, len(sig_ip if sig_ip is not None else sig_node) + 1)
-        if sig_ip is not None:
-            out.write("%RESULT.CASE.STEP.ELEMENT.GAUSS.SCALAR\n" + head)
-            out.write("%%RESULT.CASE.STEP.ELEMENT.GAUSS.SCALAR.DATA\n%d\n" % len(sig_ip))
-            for lab, s in zip(elem_labels, sig_ip):
-                out.write("%d\n%d\n" % (lab, len(s)))
-                for row in s:
-                    out.write(" ".join(_fmt(v, digits) for v in row) + " \n")
+        for k, st in enumerate(steps):
+            out.write("%%RESULT.CASE.STEP\n%d\n\n%%RESULT.CASE.STEP.FACTOR\n%.*e\n\n"
+                      % (k + 1, digits, st["factor"]))
+            out.write("%%RESULT.CASE.STEP.NODAL.DISPLACEMENT\n%d  'displacement'\n" % len(labels))
+            for lab, u in zip(labels, st["u"]):
+                out.write("%-5d " % lab + "  ".join(_fmt(v, digits) for v in
+                                                      list(u) + [0.0, 0.0, 0.0]) + "  \n")
             out.write("\n")
-        if sig_node is not None:
-            out.write("%RESULT.CASE.STEP.ELEMENT.NODAL.SCALAR\n" + head)
-            out.write("%%RESULT.CASE.STEP.ELEMENT.NODAL.SCALAR.DATA\n%d\n" % len(sig_node))
-            for lab, s in zip(elem_labels, sig_node):
-                out.write("%d\n" % lab)
-                for row in s:
-                    out.write(" ".join(_fmt(v, digits) for v in row) + " \n")
-            out.write("\n")
-        if sig_avg is not None:
-            out.write("%RESULT.CASE.STEP.NODAL.SCALAR\n" + head)
-            out.write("%%RESULT.CASE.STEP.NODAL.SCALAR.DATA\n%d\n" % len(sig_avg))
-            for lab, row in zip(labels, sig_avg):
-                out.write("%-5d " % lab + "  ".join(_fmt(v, digits) for v in row) + "  \n")
-            out.write("\n")
+            sig_ip, sig_node, sig_avg = st.get("sig_ip"), st.get("sig_node"), st.get("sig_avg")
+            el = elem_labels
+            if el is None and (sig_ip is not None or sig_node is not None):
+                el = np.arange(1, len(sig_ip if sig_ip is not None else sig_node) + 1)
+            if sig_ip is not None:
+                out.write("%RESULT.CASE.STEP.ELEMENT.GAUSS.SCALAR\n" + head)
+                out.write("%%RESULT.CASE.STEP.ELEMENT.GAUSS.SCALAR.DATA\n%d\n" % len(sig_ip))
+                for lab, sg in zip(el, sig_ip):
+                    out.write("%d\n%d\n" % (lab, len(sg)))
+                    for row in sg:
+                        out.write(" ".join(_fmt(v, digits) for v in row) + " \n")
+                out.write("\n")
+            if sig_node is not None:
+                out.write("%RESULT.CASE.STEP.ELEMENT.NODAL.SCALAR\n" + head)
+                out.write("%%RESULT.CASE.STEP.ELEMENT.NODAL.SCALAR.DATA\n%d\n" % len(sig_node))
+                for lab, sg in zip(el, sig_node):
+                    out.write("%d\n" % lab)
+                    for row in sg:
+                        out.write(" ".join(_fmt(v, digits) for v in row) + " \n")
+                out.write("\n")
+            if sig_avg is not None:
+                out.write("%RESULT.CASE.STEP.NODAL.SCALAR\n" + head)
+                out.write("%%RESULT.CASE.STEP.NODAL.SCALAR.DATA\n%d\n" % len(sig_avg))
+                for lab, row in zip(labels, sig_avg):
+                    out.write("%-5d " % lab + "  ".join(_fmt(v, digits) for v in row) + "  \n")
+                out.write("\n")
         out.write("%END\n")
 
 

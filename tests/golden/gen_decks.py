@@ -18,7 +18,8 @@ This script runs the reference program built from its sources
 (``make -C oracle ref`` -> oracle/_ref/fast) on a temporary copy of each deck
 with ``%HEADER.OUTPUT.PRECISION 16`` added, and stores what it prints for the
 last step as tests/golden/deck_<name>.npz: node labels, displacements u, the
-load factor, element labels, integration-point stresses sig_ip [nelem, npt, 6]
+load factor, every step's displacements u_steps and load factors, element
+labels, integration-point stresses sig_ip [nelem, npt, 6]
 and element-nodal stresses sig_node [nelem, nen, 6] (Voigt xx yy zz xy xz
 yz), and the number of steps.
 
@@ -129,6 +130,8 @@ def main():
             st = steps[-1]
             np.savez_compressed(os.path.join(HERE, "deck_%s.npz" % name), labels=st["labels"],
                                 u=st["u"], factor=st["factor"], nsteps=len(steps),
+                                u_steps=np.stack([x["u"] for x in steps]),
+                                factors=np.array([x["factor"] for x in steps]),
                                 elem_labels=st["elem_labels"], sig_ip=np.stack(st["sig_ip"]),
                                 sig_node=np.stack(st["sig_node"]))
             print(name, len(st["labels"]), "nodes,", len(steps), "steps, max |u|",

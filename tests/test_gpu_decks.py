@@ -44,10 +44,16 @@ def test_deck_matches_reference_run(fg, tmp_path, name, solver):
     scale = np.abs(ref["sig_ip"]).max()
     assert np.abs(sig[0] - ref["sig_ip"]).max() <= 1e-5 * scale
     assert np.abs(sig[1] - ref["sig_node"]).max() <= 1e-5 * scale
-    st = neutral.read_pos(out)[-1]
+    steps = neutral.read_pos(out)
+    st = steps[-1]
     assert np.array_equal(st["labels"], d.labels) and np.array_equal(st["u"], un)
     assert np.array_equal(np.stack(st["sig_ip"]), sig[0])
     assert np.array_equal(np.stack(st["sig_node"]), sig[1])
+    # every load step is printed, as the reference prints them
+    assert len(steps) == int(ref["nsteps"])
+    for k, s in enumerate(steps):
+        assert abs(s["factor"] - ref["factors"][k]) <= 1e-12 * abs(ref["factors"][k])
+        assert_same_displacements(d, s["u"], ref["u_steps"][k], name, tol=1e-6)
 
 
 def test_body_force_vs_oracle(fg, oracle_mod):
