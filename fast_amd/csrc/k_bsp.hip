@@ -135,6 +135,18 @@ __device__ inline void geo_load(Geo<NB2> &g, int e, const double *__restrict__ c
   }
 }
 
+// The element's nodal displacements into LDS by threads 128.. (idle in
+// geo_load), so that their gathers overlap the geometry's loads.
+template <int NB2>
+__device__ inline void stage_u(double (*U)[3], const int *__restrict__ cn,
+                               const double *__restrict__ unodal) {
+  const int t = threadIdx.x - 128;
+  if (t >= 0 && t < Bsp<NB2>::NBAS) {
+    const int n = cn[t];
+    for (int k = 0; k < 3; k++) U[t][k] = unodal[3 * (size_t)n + k];
+  }
+}
+
 // Numerators (N w, dN w) of the tensor-product basis a at point q;
 // a = i + 4 j + 16 k with i along r, q likewise.
 template <int NB2>
@@ -321,6 +333,7 @@ __device__ inline unsigned short sym_ab(int blk) {
 template <int FORM, int NB2>
 struct KtShared {
   Geo<NB2> g;
+  double U[Bsp<NB2>::NBAS][3];
   float A[3][NQP][Bsp<NB2>::NBAS];                            // F g (SVK) / F^-T g (neo-Hooke)
   float G[FORM == FGPU_TL_SVK ? 3 : 1][NQP][Bsp<NB2>::NBAS];  // g (SVK only)
   float pp[NQP][16];  // SVK: c, F F^T (6), S (6);  neo-Hooke: F F^T (6), c lam, c mu', c mu
@@ -339,6 +352,7 @@ __global__ __launch_bounds__(256) void bsp_Kt_kernel(
   const int tid = threadIdx.x;
   const int *cn = conn + (size_t)e * NBAS;
   geo_load(S.g, e, coords, conn, cpw, ext, gp);
+  stage_u<NB2>(S.U, cn, unodal);
   __syncthreads();
   double lam, mu;
   if constexpr (FORM == FGPU_TL_SVK) {
@@ -357,7 +371,7 @@ __global__ __launch_bounds__(256) void bsp_Kt_kernel(
     #pragma unroll 2
     for (int k = 0; k < NPART; k++) {
       const int a = part * NPART + k;
-      const double *ua = unodal + 3 * (size_t)cn[a];
+      const double *ua = S.U[a];
       double dR[3], G[3];
       rational_dR(S.g, q, a, P, dR);
       for (int i = 0; i < 3; i++)
@@ -501,6 +515,7 @@ __global__ __launch_bounds__(256) void bsp_Kt_kernel(
 template <int NB2, int FORM>
 struct FShared {
   Geo<NB2> g;
+  double U[Bsp<NB2>::NBAS][3];  // the element's nodal displacements
   double Ji[NQP][9];
   double rs[NQP][4];   // 1 / W, dW/dr, dW/ds, dW/dt
   // c * sigma (Voigt, linear) or c * P (row-major, total-Lagrangian)
@@ -520,6 +535,7 @@ __global__ __launch_bounds__(256) void bsp_fint_kernel(
   const int tid = threadIdx.x;
   const int *cn = conn + (size_t)e * NBAS;
   geo_load(S.g, e, coords, conn, cpw, ext, gp);
+  stage_u<NB2>(S.U, cn, unodal);
   __syncthreads();
   {
     const int q = tid / 4, part = tid % 4;
@@ -528,7 +544,7 @@ __global__ __launch_bounds__(256) void bsp_fint_kernel(
     double L[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
     #pragma unroll 2
     for (int k = 0; k < NPART; k++) {
-      const double *ua = unodal + 3 * (size_t)cn[part * NPART + k];
+      const double *ua = S.U[part * NPART + k];
       double dR[3], G[3];
       rational_dR(S.g, q, part * NPART + k, P, dR);
       for (int i = 0; i < 3; i++)
@@ -691,6 +707,7 @@ __global__ __launch_bounds__(256) void bsp_body_kernel(
 template <int NB2>
 struct SShared {
   Geo<NB2> g;
+  double U[Bsp<NB2>::NBAS][3];
   double sig[NQP][6];
 };
 
@@ -708,6 +725,7 @@ __global__ __launch_bounds__(256) void bsp_stress_kernel(
   const int tid = threadIdx.x;
   const int *cn = conn + (size_t)e * NBAS;
   geo_load(S.g, e, coords, conn, cpw, ext, gp);
+  stage_u<NB2>(S.U, cn, unodal);
   __syncthreads();
   {
     const int q = tid / 4, part = tid % 4;
@@ -716,7 +734,7 @@ __global__ __launch_bounds__(256) void bsp_stress_kernel(
     double L[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
     #pragma unroll 2
     for (int k = 0; k < NPART; k++) {
-      const double *ua = unodal + 3 * (size_t)cn[part * NPART + k];
+      const double *ua = S.U[part * NPART + k];
       double dR[3], G[3];
       rational_dR(S.g, q, part * NPART + k, P, dR);
       for (int i = 0; i < 3; i++)
