@@ -510,8 +510,10 @@ __global__ __launch_bounds__(256) void bsp_Kt_kernel(
 // points (shuffle-reduced), then c times the stress that multiplies the
 // gradients: sigma (linear) or P = F S (total Lagrangian; the reference's
 // (B + Bl)^T S, elmparam.cpp:1615-1707).  Phase 2, thread (a, quarter):
-// g_a = sum_q c P_q grad N_a(q) over 16 points, recomputing grad N_a from the
-// tensor tables and the stored J^-1, reduced over the 4 quarters.
+// g_a = sum_q c P_q grad N_a(q) over every fourth point (q = quarter mod 4:
+// the four quarters of a wave read neighbouring rows of the per-point LDS
+// arrays, in distinct banks), recomputing grad N_a from the tensor tables
+// and the stored J^-1, reduced over the 4 quarters.
 template <int NB2, int FORM>
 struct FShared {
   Geo<NB2> g;
@@ -610,7 +612,7 @@ __global__ __launch_bounds__(256) void bsp_fint_kernel(
     const int a = tid / 4, quarter = tid % 4;
     double g0 = 0.0, g1 = 0.0, g2 = 0.0;
     #pragma unroll
-    for (int q = quarter * 16; q < quarter * 16 + 16; q++) {
+    for (int q = quarter; q < NQP; q += 4) {  // the quarters' points side by side in LDS
       double n, d[3];
       tensor(S.g, q, a, n, d);
       const double is = S.rs[q][0];
@@ -648,7 +650,7 @@ __global__ __launch_bounds__(256) void bsp_fint_kernel(
 // ---------------------------------------------------------------- body force
 // cElemLoad::ExtForce with cBodyLoad (load/load.cpp:618-691, 969-974):
 // f_a = sum_q w detJ R_a b.  Phase 1, thread (q, part): the point's w detJ
-// and rational denominator; phase 2, thread (a, quarter): sum over 16 points.
+// and rational denominator; phase 2, thread (a, quarter): sum over every fourth point.
 template <int NB2>
 struct BShared {
   Geo<NB2> g;
@@ -683,7 +685,7 @@ __global__ __launch_bounds__(256) void bsp_body_kernel(
     const int a = tid / 4, quarter = tid % 4;
     double sN = 0.0;
     #pragma unroll
-    for (int q = quarter * 16; q < quarter * 16 + 16; q++) {
+    for (int q = quarter; q < NQP; q += 4) {  // the quarters' points side by side in LDS
       double n, d[3];
       tensor(S.g, q, a, n, d);
       sN += S.c[q] * (n * S.inv_s[q]);
@@ -703,7 +705,7 @@ __global__ __launch_bounds__(256) void bsp_body_kernel(
 // cElmParam::NodalStress (1304-1333): s_a = sum_q TR[a][q] s_q with the
 // extrapolation matrix of CompTRMatrix (1444-1535).  Phase 1, thread
 // (q, part): grad u at q as in the f_int kernel, then the point's stress;
-// phase 2, thread (a, quarter): 16 points of the extrapolation sum.
+// phase 2, thread (a, quarter): every fourth point of the extrapolation sum.
 template <int NB2>
 struct SShared {
   Geo<NB2> g;
@@ -763,7 +765,7 @@ __global__ __launch_bounds__(256) void bsp_stress_kernel(
     const int a = tid / 4, quarter = tid % 4;
     double acc[6] = {0, 0, 0, 0, 0, 0};
     #pragma unroll
-    for (int q = quarter * 16; q < quarter * 16 + 16; q++) {
+    for (int q = quarter; q < NQP; q += 4) {  // the quarters' points side by side in LDS
       const double w = TR[a * NQP + q];
       for (int k = 0; k < 6; k++) acc[k] += w * S.sig[q][k];
     }
