@@ -20,6 +20,7 @@ from __future__ import annotations
 
 import argparse
 import json
+import re
 import os
 import sys
 import time
@@ -404,6 +405,16 @@ def main():
     dom = "assembly" if ta >= tk else "element_K"
     roof = dict(rooflines[dom])
     roof["other"] = rooflines["element_K" if dom == "assembly" else "assembly"]
+    # the context picks the faster order on its first combined call: with f_int
+    # beside the assembly, the assembly's launches share the GPU with it and
+    # its durations (and so the roofline) are those of the shared run
+    overlap = False
+    if not args.serial:
+        mm = re.search(r"K then f_int: ([0-9.]+) ms; f_int beside the assembly: ([0-9.]+) ms",
+                       model.tuning_report())
+        overlap = bool(mm) and float(mm.group(2)) < float(mm.group(1))
+    if overlap and dom == "assembly":
+        roof["shares_gpu_with"] = "f_int (second stream)"
     nrm, fwd = accuracy(model, m, u)
     solve = None
     if args.solve:
@@ -449,7 +460,7 @@ def main():
         "data": "synthetic (seeded generated mesh)", "config": cfg,
         "phase_ms": {"element_K": round(float(tk), 4), "assembly": round(float(ta), 4),
                      "f_int": round(float(tf), 4)},
-        "f_int_beside_K": not args.serial,
+        "f_int_beside_assembly": overlap,
         "accuracy": {"Ku_normwise_rel": nrm, "Ku_forward_rel_max": fwd},
         "roofline": roof, "cpu_baseline": cpu, "solve_plain_pcg": solve,
     }
