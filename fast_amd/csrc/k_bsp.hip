@@ -265,7 +265,10 @@ __global__ __launch_bounds__(256, 3) void bsp_K_kernel(
   const int wave = tid / 64, lane = tid % 64;
   const int ln = lane % 16, lk = lane / 16;
   float *out = Ke + (size_t)e * NSYM * 9;
-  for (int tp = wave; tp < NPAIR; tp += 4) {
+  // waves take the tile pairs round-robin from a start that rotates with the
+  // element, so that the waves holding one pair more than the others are not
+  // always on the same SIMDs of a CU
+  for (int tp = (wave + e) % 4; tp < NPAIR; tp += 4) {
     // upper 16x16 tile pair (ta <= tb) number tp, row-major
     int ta = 0, r = tp;
     while (r >= NT - ta) {
