@@ -285,7 +285,7 @@ void setup_batch(Ctx &c, const fgpu_mesh &m) {
 void destroy(Ctx *c) {
   if (!c) return;
   void *bufs[] = {c->d_coords, c->d_conn, c->d_eqmap, c->d_mat, c->d_ext,
-                  c->d_cpw, c->d_gauss, c->d_gauss64, c->d_Ke, c->d_ge,
+                  c->d_cpw, c->d_gauss, c->d_gauss64, c->d_Ke, c->d_Ke64, c->d_ge,
                   c->d_status, c->d_rowptr, c->d_col, c->d_bptr, c->d_bcol,
                   c->d_bcoloff, c->d_bdiag, c->d_fmask, c->d_bval, c->d_bn,
                   c->d_blkA, c->d_blkB, c->d_offAB, c->d_offBA, c->d_cptr,
@@ -620,7 +620,10 @@ int fgpu_create(fgpu_ctx **out, const fgpu_mesh *m) {
     int nsym = nsym_blocks(c->nn);
     c->kel = KeLayout{c->shape == FGPU_HEX8 ? 64 : 1, nsym};
     size_t ntile = (size_t)(c->nelem + c->kel.T - 1) / c->kel.T;
-    c->d_Ke = dalloc<float>(ntile * c->kel.T * (size_t)nsym * 9);
+    if (c->shape == FGPU_BSP && bsp_generic(*c))
+      c->d_Ke64 = dalloc<double>(ntile * c->kel.T * (size_t)nsym * 9);
+    else
+      c->d_Ke = dalloc<float>(ntile * c->kel.T * (size_t)nsym * 9);
     c->d_ge = dalloc<double>((size_t)c->nelem * c->nn * 3);
     if (c->shape == FGPU_BSP) bsp_prepare_tables(*c, m->ext, m->cpw);
     c->d_u = dalloc<double>((size_t)c->nnode * 3);

@@ -138,6 +138,8 @@ struct Ctx {
   // ---- element outputs
   KeLayout kel{64, 36};
   float *d_Ke = nullptr;        // element matrices (upper node blocks)
+  double *d_Ke64 = nullptr;     // the same in FP64, for element kernels that compute in FP64
+                                // (B-spline solids outside the tuned shapes); d_Ke unused then
   double *d_ge = nullptr;       // element internal forces [nelem*nn*3]
   int *d_status = nullptr;      // [2]: bad-Jacobian count, stress failure
   double *d_TR = nullptr;       // stress extrapolation points -> nodes [nn*nq]

@@ -14,8 +14,9 @@
 // Jacobian and inverse, then per (point, control point) R_a and
 // G_a = dR_a/dx into global scratch), then, for a state, per point
 // L = grad u and the stress terms, then one thread per output entry: a
-// stored K block (sums over the points in FP64, stored FP32 like every
-// element matrix), an f_int or body-force vector, a point stress.  The
+// stored K block (sums over the points in FP64, kept in FP64: these
+// elements serve small, often slender models whose Newton iterations need
+// the full tangent), an f_int or body-force vector, a point stress.  The
 // K closed forms are those of the hex8 and cubic kernels:
 //   linear  K_ab = sum c [lambda g h^T + mu h g^T + mu (g.h) I]
 //   SVK     K_ab = sum c [lambda (Fg)(Fh)^T + mu (Fh)(Fg)^T + mu (g.h) F F^T + (g^T S h) I]
@@ -263,7 +264,7 @@ template <int FORM>
 __global__ __launch_bounds__(256) void gen_K_kernel(
     int e0, int ne, GDims D, int nsym, const int2 *__restrict__ sym,
     const double *__restrict__ mat, const double *__restrict__ G, const double *__restrict__ H,
-    const double *__restrict__ PQ, float *__restrict__ Ke) {
+    const double *__restrict__ PQ, double *__restrict__ Ke) {
   const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= (long long)ne * nsym) return;
   const int le = (int)(t / nsym), sb = (int)(t % nsym), e = e0 + le, nn = D.nn;
@@ -317,8 +318,8 @@ __global__ __launch_bounds__(256) void gen_K_kernel(
     for (int i = 0; i < 3; i++)
       for (int j = 0; j < 3; j++) K[3 * i + j] = M[3 * i + j] + M2[3 * j + i] + (i == j ? sc : 0.0);
   }
-  float *out = Ke + ((size_t)e * nsym + sb) * 9;
-  for (int k = 0; k < 9; k++) out[k] = (float)K[k];
+  double *out = Ke + ((size_t)e * nsym + sb) * 9;
+  for (int k = 0; k < 9; k++) out[k] = K[k];
 }
 
 // g_a = sum_q c P G_a (internal force) or f_a = sum_q c R_a b (body force)
@@ -452,7 +453,7 @@ void launch_K_chunk(Ctx &c, const GDims &D, int e0, int ne) {
                        dim3(256), 0, c.stream, ne, D, c.d_gG, c.d_gPQ, c.d_gH);
   hipLaunchKernelGGL((gen_K_kernel<FORM>), dim3(grid_of((long long)ne * nsym)), dim3(256), 0,
                      c.stream, e0, ne, D, nsym, (const int2 *)c.d_gsym, c.d_mat, c.d_gG, c.d_gH,
-                     c.d_gPQ, c.d_Ke);
+                     c.d_gPQ, c.d_Ke64);
 }
 
 }  // namespace

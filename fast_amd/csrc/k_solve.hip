@@ -33,12 +33,14 @@ namespace {
 // (w % 8) * (grid / 8) + w / 8 (grid a multiple of 8), so each XCD sweeps one
 // contiguous eighth of the block rows and the element blocks its rows share
 // stay in that XCD's L2.
-template <int NB, bool XCD>
+// T: the element matrices' storage type (FP32; FP64 where the element
+// kernels compute and keep FP64)
+template <int NB, bool XCD, class T = float>
 __global__ __launch_bounds__(256) void assemble_kernel(
     long long nblk, const int *__restrict__ blkA, const int *__restrict__ blkB,
     const int *__restrict__ posAB, const int *__restrict__ posBA,
     const long long *__restrict__ cptr, const int *__restrict__ contrib,
-    const float *__restrict__ Ke, KeLayout L, const unsigned char *__restrict__ fmask,
+    const T *__restrict__ Ke, KeLayout L, const unsigned char *__restrict__ fmask,
     double *__restrict__ bval) {
   const long long w = XCD ? (long long)(blockIdx.x & 7) * (gridDim.x >> 3) + (blockIdx.x >> 3)
                           : (long long)blockIdx.x;
@@ -50,14 +52,14 @@ __global__ __launch_bounds__(256) void assemble_kernel(
     int code[NB];
 #pragma unroll
     for (int u = 0; u < NB; u++) code[u] = tb + u < c1 ? contrib[tb + u] : -1;
-    float v[NB][9];
+    T v[NB][9];
 #pragma unroll
     for (int u = 0; u < NB; u++) {
       if (code[u] < 0) continue;
       const int x = code[u] >> 1;
       const long e = x / L.nsym;
       const int sb = x - (int)e * L.nsym;
-      const float *p = Ke + L.offset(e, sb);
+      const T *p = Ke + L.offset(e, sb);
 #pragma unroll
       for (int m = 0; m < 9; m++) v[u][m] = p[m];
     }
@@ -141,7 +143,8 @@ __global__ void add_diag_kernel(int n, const int *__restrict__ eq, const double 
     }
 }
 
-__global__ void expand_Ke_kernel(int e0, int ne, int nn, KeLayout L, const float *__restrict__ Ke,
+template <class T>
+__global__ void expand_Ke_kernel(int e0, int ne, int nn, KeLayout L, const T *__restrict__ Ke,
                                  float *__restrict__ out) {
   const long long nd = 3LL * nn;
   long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -151,8 +154,8 @@ __global__ void expand_Ke_kernel(int e0, int ne, int nn, KeLayout L, const float
   const int r = rc / (int)nd, cc = rc % (int)nd;
   const int a = r / 3, i = r % 3, b = cc / 3, j = cc % 3;
   const int lo = a < b ? a : b, hi = a < b ? b : a;
-  const float *p = Ke + L.offset(e0 + el, sym_index(lo, hi, nn));
-  out[idx] = a <= b ? p[3 * i + j] : p[3 * j + i];
+  const T *p = Ke + L.offset(e0 + el, sym_index(lo, hi, nn));
+  out[idx] = (float)(a <= b ? p[3 * i + j] : p[3 * j + i]);
 }
 
 __global__ void eq_to_nodal_kernel(int nnode, const int *__restrict__ eqmap,
@@ -280,9 +283,14 @@ template <int NB, bool XCD>
 void launch_contrib(Ctx &c) {
   long long grid = (c.nblk + 255) / 256;
   if (XCD) grid = (grid + 7) / 8 * 8;  // the remap is a bijection on multiples of 8
-  hipLaunchKernelGGL((assemble_kernel<NB, XCD>), dim3((unsigned)grid), dim3(256), 0, c.stream,
-                     c.nblk, c.d_blkA, c.d_blkB, c.d_offAB, c.d_offBA, c.d_cptr, c.d_contrib,
-                     c.d_Ke, c.kel, c.d_fmask, c.d_bval);
+  if (c.d_Ke64)
+    hipLaunchKernelGGL((assemble_kernel<NB, XCD, double>), dim3((unsigned)grid), dim3(256), 0,
+                       c.stream, c.nblk, c.d_blkA, c.d_blkB, c.d_offAB, c.d_offBA, c.d_cptr,
+                       c.d_contrib, c.d_Ke64, c.kel, c.d_fmask, c.d_bval);
+  else
+    hipLaunchKernelGGL((assemble_kernel<NB, XCD, float>), dim3((unsigned)grid), dim3(256), 0,
+                       c.stream, c.nblk, c.d_blkA, c.d_blkB, c.d_offAB, c.d_offBA, c.d_cptr,
+                       c.d_contrib, c.d_Ke, c.kel, c.d_fmask, c.d_bval);
 }
 
 void launch_assemble_K(Ctx &c) {
@@ -329,8 +337,12 @@ void launch_add_diag(Ctx &c, int n, const int *d_eq, const double *d_v) {
 
 void launch_expand_Ke(Ctx &c, int e0, int ne, float *out) {
   long long tot = (long long)ne * 9 * c.nn * c.nn;
-  hipLaunchKernelGGL(expand_Ke_kernel, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0,
-                     c.stream, e0, ne, c.nn, c.kel, c.d_Ke, out);
+  if (c.d_Ke64)
+    hipLaunchKernelGGL(expand_Ke_kernel<double>, dim3((unsigned)((tot + 255) / 256)), dim3(256),
+                       0, c.stream, e0, ne, c.nn, c.kel, c.d_Ke64, out);
+  else
+    hipLaunchKernelGGL(expand_Ke_kernel<float>, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0,
+                       c.stream, e0, ne, c.nn, c.kel, c.d_Ke, out);
   FGPU_CHECK(hipGetLastError());
 }
 

@@ -216,7 +216,7 @@ def read(path: str) -> Deck:
         ext = None
         deg = (1, 1, 1)
     else:
-        conn, ext, deg = _bspline_elements(sec, rows, idx)
+        conn, ext, deg, iga = _bspline_elements(sec, rows, idx)
         mesh_kw = dict(shape="bsp", cpw=cpw)
 
     # equation numbering in solver order: nodes sorted by their rank
@@ -246,6 +246,11 @@ def read(path: str) -> Deck:
         if mesh.shape != "hex8":
             raise NotImplementedError("face loads on B-spline solids")
         f += _face_loads(mesh, sec["LOAD.CASE.AREA.UNIFORM"], idx)
+    if "LOAD.CASE.AREA.IGA.UNIFORM" in sec:
+        if mesh.shape != "bsp":
+            raise NotImplementedError("IGA face loads need B-spline solids")
+        elab = {r[0]: k for k, r in enumerate(rows)}
+        f += _iga_face_loads(mesh, sec["LOAD.CASE.AREA.IGA.UNIFORM"], elab, iga)
     if "LOAD.CASE.BODY.FORCES" in sec:
         tk = _Toks(sec["LOAD.CASE.BODY.FORCES"], "LOAD.CASE.BODY.FORCES")
         body = np.zeros((mesh.nelem, 3))
@@ -288,6 +293,7 @@ _HANDLED = {"HEADER.ANALYSIS.ALGORITHM", "HEADER.ANALYSIS.MAXIMUM.STEPS",
             "MATERIAL.ISOTROPIC", "SECTION.HOMOGENEOUS.ISOTROPIC.3D", "INTEGRATION.ORDER",
             "ELEMENT.BRICK8", "ELEMENT.BRICK8TL", "ELEMENT.BSPLINE.SOLID",
             "ELEMENT.BSPLINE.SOLIDTL", "PATCH", "LOAD.CASE.AREA.UNIFORM",
+            "LOAD.CASE.AREA.IGA.UNIFORM",
             "LOAD.CASE.BODY.FORCES", "LOAD.CASE.NODAL.FORCE"}
 
 
@@ -328,7 +334,7 @@ def _bspline_elements(sec, rows, idx):
     if len(degs) != 1:
         raise NotImplementedError("one B-spline degree per deck")
     deg = degs.pop()
-    conn, ext = [], []
+    conn, ext, einfo = [], [], []
     cache = {}
     for r in rows:
         pid, span = r[3][0], [s - 1 for s in r[3][1:]]
@@ -350,7 +356,120 @@ def _bspline_elements(sec, rows, idx):
                     c.append(cps[cp])
         conn.append(c)
         ext.append(np.concatenate([x.ravel() for x in cs]))
-    return np.asarray(conn, dtype=np.int32), np.asarray(ext), deg
+        einfo.append((pid, first, [np.asarray(x) for x in cs]))
+    return np.asarray(conn, dtype=np.int32), np.asarray(ext), deg, (pats, einfo)
+
+
+# IGA faces (shp/shpiga.h:169-176 eFaceType; geom/subpat.cpp:123-190
+# nSurfAlg::CreateSubPatch): the fixed parametric direction and whether the
+# face lies at its start (0) or end (1); the surface's parameters are the
+# other two directions in increasing order
+_IGA_FACES = {"Front": (2, 0), "Back": (2, 1), "Left": (0, 0), "Right": (0, 1),
+              "Above": (1, 1), "Below": (1, 0)}
+
+
+def _bernstein(p, u):
+    """B_j(u) and dB_j/du of degree p (binomial form)."""
+    from math import comb
+    B = np.array([comb(p, j) * u ** j * (1 - u) ** (p - j) for j in range(p + 1)])
+    if p == 0:
+        return B, np.zeros(1)
+    lo = np.array([comb(p - 1, j) * u ** j * (1 - u) ** (p - 1 - j) for j in range(p)])
+    dB = p * (np.concatenate([[0.0], lo]) - np.concatenate([lo, [0.0]]))
+    return B, dB
+
+
+def _iga_face_loads(mesh, toks, elab, iga):
+    """Uniform loads on a face of a B-spline solid patch (cFaceUnifLoadIGA,
+    load/loadiga.cpp:305-354): the face is the patch's boundary surface
+    (geom/subpat.cpp CreateSubPatch), taken over the element's knot spans in
+    its two parameters; integrated with max(K) x max(K) Gauss points
+    (cElmParam::GetIntOrd, elc/elmparam.cpp:349-360; cElemLoad::ExtForce,
+    load/load.cpp:618-691) with |x_r x x_s| as the area factor
+    (cShapeSurf::DrvShpXYZ, shp/shpsurf.cpp:56-116) and, for Local = 1, the
+    face frame of cShapeSurf::LocalSys (shpsurf.cpp:121-164)."""
+    pats, einfo = iga
+    tk = _Toks(toks, "LOAD.CASE.AREA.IGA.UNIFORM")
+    f = np.zeros(mesh.neq)
+    n = max(mesh.nip)
+    r, w = gauss_table(n)
+    for _ in range(tk.int()):
+        e = elab[tk.int()]
+        face = tk.next().strip("'")
+        local = tk.int()
+        qv = np.asarray([tk.float(), tk.float(), tk.float()])
+        if face not in _IGA_FACES:
+            raise ValueError("unknown face %s" % face)
+        fd, end = _IGA_FACES[face]
+        pid, first, cs = einfo[e]
+        bases, nb, cps = pats[pid]
+        d1, d2 = [d for d in range(3) if d != fd]
+        p1, p2 = bases[d1][0], bases[d2][0]
+        fixed = 0 if end == 0 else nb[fd] - 1
+        nodes = np.zeros((p1 + 1, p2 + 1), dtype=np.int64)
+        for i in range(p1 + 1):
+            for j in range(p2 + 1):
+                ijk = [0, 0, 0]
+                ijk[fd], ijk[d1], ijk[d2] = fixed, first[d1] + i, first[d2] + j
+                nodes[i, j] = cps[ijk[0] + nb[0] * (ijk[1] + nb[1] * ijk[2])]
+        X = mesh.coords[nodes]                         # [p1+1, p2+1, 3]
+        W = mesh.cpw[nodes] if mesh.cpw is not None else np.ones(nodes.shape)
+        for a in range(n):
+            B1, dB1 = _bernstein(p1, (r[a] + 1.0) / 2.0)
+            N1, dN1 = cs[d1] @ B1, 0.5 * (cs[d1] @ dB1)
+            for b in range(n):
+                B2, dB2 = _bernstein(p2, (r[b] + 1.0) / 2.0)
+                N2, dN2 = cs[d2] @ B2, 0.5 * (cs[d2] @ dB2)
+                num = np.outer(N1, N2) * W
+                sw = num.sum()
+                R = num / sw
+                dR1 = (np.outer(dN1, N2) * W - R * (np.outer(dN1, N2) * W).sum()) / sw
+                dR2 = (np.outer(N1, dN2) * W - R * (np.outer(N1, dN2) * W).sum()) / sw
+                x1 = np.einsum("ij,ijk->k", dR1, X)
+                x2 = np.einsum("ij,ijk->k", dR2, X)
+                dA = np.linalg.norm(np.cross(x1, x2)) * w[a] * w[b]
+                load = qv
+                if local:
+                    uu = x1 / np.linalg.norm(x1)
+                    ww = np.cross(uu, x2 / np.linalg.norm(x2))
+                    vv = np.cross(ww, uu)
+                    load = qv[0] * uu + qv[1] * vv + qv[2] * ww
+                for i in range(p1 + 1):
+                    for j in range(p2 + 1):
+                        if R[i, j] == 0.0:
+                            continue
+                        for d in range(3):
+                            eq = mesh.dofmap[nodes[i, j], d]
+                            if eq > 0:
+                                f[eq - 1] += R[i, j] * dA * load[d]
+    return f
+
+
+def gauss_table(n):
+    """Gauss-Legendre points and weights rounded to 15 decimals, the values of
+    the reference's quadrature table (elc/intpoint.cpp:741-920), as the
+    device kernels use them: Newton's method on P_n in 40-digit decimal
+    arithmetic from the double-precision roots, then rounded."""
+    import decimal
+    ctx = decimal.Context(prec=40)
+    r0, _ = np.polynomial.legendre.leggauss(n)
+    rs, ws = [], []
+    for x0 in r0:
+        x = ctx.create_decimal(repr(float(x0)))
+        for _ in range(8):
+            p0, p1 = ctx.create_decimal(1), x
+            for k in range(2, n + 1):
+                p0, p1 = p1, ctx.divide(ctx.subtract(ctx.multiply((2 * k - 1) * x, p1),
+                                                     ctx.multiply(k - 1, p0)), k)
+            if n == 1:
+                p0, p1 = ctx.create_decimal(1), x
+            dp = ctx.divide(ctx.multiply(n, ctx.subtract(ctx.multiply(x, p1), p0)),
+                            ctx.subtract(ctx.multiply(x, x), 1))
+            x = ctx.subtract(x, ctx.divide(p1, dp))
+        wv = ctx.divide(2, ctx.multiply(ctx.subtract(1, ctx.multiply(x, x)), ctx.multiply(dp, dp)))
+        rs.append(float(round(x, 15)) + 0.0)
+        ws.append(float(round(wv, 15)))
+    return np.array(rs), np.array(ws)
 
 
 def _face_loads(mesh, toks, idx):
@@ -363,7 +482,7 @@ def _face_loads(mesh, toks, idx):
     tk = _Toks(toks, "LOAD.CASE.AREA.UNIFORM")
     f = np.zeros(mesh.neq)
     n = mesh.nip[0]
-    r, w = np.polynomial.legendre.leggauss(n)
+    r, w = gauss_table(n)
     for _ in range(tk.int()):
         e = tk.int() - 1
         corners = {idx[tk.int()] for _ in range(3)}

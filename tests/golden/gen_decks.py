@@ -1,12 +1,16 @@
 """Fixtures of decks run by the reference program.
 
 Decks in tests/golden/decks/:
-  TestB8.dat, ScordelisLoDeg3.dat, PinCylDeg4.dat  the reference's own
+  TestB8.dat, ScordelisLoDeg3.dat, PinCylDeg4.dat, ThinCylinderDeg5.dat,
+  CurvedBeamDeg4.dat  the reference's own
       example decks (dat/continuum/TestB8.dat: one brick under a uniform end
       traction; dat/iga/ScordelisLoDeg3.dat: the Scordelis-Lo roof, 64 NURBS
       solids of degree (3,3,2) under self weight; dat/iga/PinCylDeg4.dat: a
       pinched cylinder, 64 NURBS solids of degree (4,2,4) on 5x5x5 Gauss
-      points, nodal stresses evaluated at the nodes);
+      points, nodal stresses evaluated at the nodes; dat/iga/ThinCylinderDeg5:
+      a thin cylinder under a normal pressure on its inner faces, 80 NURBS
+      solids of degree (5,2,5); dat/iga/CurvedBeamDeg4: a curved
+      total-Lagrangian beam under an end-face load, 200 load steps);
   BeamB8.dat, BeamB8TL.dat  written by this script: a 4x2x2 cantilever of
       distorted bricks under tip forces, linear and total-Lagrangian
       (the latter under load control, 4 steps to 4 times the load);
@@ -42,8 +46,8 @@ sys.path.insert(0, ROOT)
 from fast_amd import neutral  # noqa: E402
 
 REF = os.path.join(ROOT, "oracle", "_ref", "fast")
-DECKS = ("TestB8", "ScordelisLoDeg3", "PinCylDeg4", "BeamB8", "BeamB8TL", "BeamB8Face",
-         "ScordelisLoDeg3TL")
+DECKS = ("TestB8", "ScordelisLoDeg3", "PinCylDeg4", "ThinCylinderDeg5", "CurvedBeamDeg4",
+         "BeamB8", "BeamB8TL", "BeamB8Face", "ScordelisLoDeg3TL")
 LCM = "%HEADER.ANALYSIS.ALGORITHM\n'lcm'\n\n%HEADER.ANALYSIS.MAXIMUM.STEPS\n{steps}\n\n" \
       "%HEADER.ANALYSIS.STEP.FACTOR\n{factor}\n\n%HEADER.ANALYSIS.MAXIMUM.ITERATIONS\n30\n\n" \
       "%HEADER.ANALYSIS.TOLERANCE\n{tol}\n\n"

@@ -1,8 +1,9 @@
 """B-spline solids outside the tuned cubic kernels (k_bspg.hip: any degree up
-to 7 per direction, any Gauss order) against the CPU FP64 oracle: element
-matrices (FP64 sums stored in FP32: 1e-6 normwise), assembled matrix,
-internal and body forces (FP64: 1e-11 / 1e-13), stresses at the points and
-at the nodes (1e-12), and a total-Lagrangian Newton solve."""
+to 7 per direction, any Gauss order) against the CPU FP64 oracle; these
+kernels compute and keep FP64 throughout: element matrices and the
+assembled matrix (1e-12 normwise), internal and body forces (1e-11 /
+1e-13), stresses at the points and at the nodes (1e-12), and a
+total-Lagrangian Newton solve."""
 import numpy as np
 import pytest
 
@@ -53,10 +54,10 @@ def test_generic_kernels_vs_oracle(fg, oracle_mod, p, nip, form):
     Ke = g.element_matrices(0, m.nelem)
     for e in (0, m.nelem - 1):
         K0, ok = om.elem_stiff(e, uk)
-        assert ok and nrel(Ke[e].astype(np.float64), K0) <= 1e-6, (p, nip, form, e)
+        assert ok and nrel(Ke[e].astype(np.float64), K0) <= 1e-6, (p, nip, form, e)  # (FP32 export)
     val = g.csr()[2]
     v0, ok = om.assemble_K(uk)
-    assert ok and nrel(val, v0) <= 1e-6
+    assert ok and nrel(val, v0) <= 1e-12
     f = g.int_force(u)
     f0, ok = om.assemble_fint(u)
     assert ok and nrel(f, f0) <= 1e-11

@@ -52,7 +52,8 @@ def test_solver_order_numbering():
     assert dm.max() == d.mesh.neq
 
 
-ALL = ["TestB8", "ScordelisLoDeg3", "PinCylDeg4", "BeamB8", "BeamB8TL", "BeamB8Face",
+ALL = ["TestB8", "ScordelisLoDeg3", "PinCylDeg4", "ThinCylinderDeg5", "CurvedBeamDeg4", "BeamB8",
+       "BeamB8TL", "BeamB8Face",
        "ScordelisLoDeg3TL"]
 
 
@@ -89,7 +90,10 @@ def test_oracle_solution_matches_reference_run(oracle_mod, name):
     assert np.array_equal(ref["labels"], d.labels)
     assert int(ref["nsteps"]) == d.steps
     om, un = oracle_solution(oracle_mod, d)
-    assert_same_displacements(d, un, ref["u"], name, tol=1e-9 if "TL" in name else 1e-10)
+    # ThinCylinderDeg5: both direct solutions leave a relative residual of
+    # 7e-12 in the same FP64 system (they are each accurate to ~1e-9 there)
+    tol = 1e-8 if name == "ThinCylinderDeg5" else 1e-9 if "TL" in name or "Curved" in name else 1e-10
+    assert_same_displacements(d, un, ref["u"], name, tol=tol)
 
 
 @pytest.mark.parametrize("name", ALL)
@@ -109,8 +113,10 @@ def test_oracle_stresses_match_reference_run(oracle_mod, name):
         sn, ok = om.node_stresses(u)
         assert ok
     scale = np.abs(ref["sig_ip"]).max()
-    assert np.abs(si - ref["sig_ip"]).max() <= 1e-13 * scale
-    assert np.abs(sn - ref["sig_node"]).max() <= 1e-13 * scale
+    # (degree 5: the oracle's basis derivatives round differently at 1e-12)
+    tol = 1e-11 if name == "ThinCylinderDeg5" else 1e-13
+    assert np.abs(si - ref["sig_ip"]).max() <= tol * scale
+    assert np.abs(sn - ref["sig_node"]).max() <= tol * scale
 
 
 def test_pos_round_trip(tmp_path):
