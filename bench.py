@@ -397,6 +397,7 @@ def main():
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
     times = np.array(times)
+    print(model.tuning_report(), file=sys.stderr, flush=True)
     tk, ta, tf = times.mean(axis=0)
     value = m.nelem * args.steps / wall
     rooflines = {"element_K": element_roofline(m, tk), "assembly": assembly_roofline(m, ta, cfg)}
