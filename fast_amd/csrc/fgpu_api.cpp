@@ -236,6 +236,34 @@ void build_maps(Ctx &c, const int *conn) {
   c.d_contrib = dupload(contrib.data(), contrib.size(), s);
   c.d_nptr = dupload(nptr.data(), nptr.size(), s);
   c.d_nadj = dupload(nadj.data(), nadj.size(), s);
+  // row assembly tables: B-spline solids of at most 64 control points per
+  // element whose rows fit the kernel's LDS (ROW_MAXS upper blocks)
+  constexpr long long ROW_MAXS = 896;  // 9 doubles each: 63 KB of LDS
+  long long maxs = 0;
+  for (int n = 0; n < nnode; n++) maxs = std::max(maxs, nb_up[n]);
+  bool distinct = true;  // an element's nodes must be distinct (one column per lane)
+  for (int n = 0; n < nnode && distinct; n++)
+    for (int t = nptr[n]; t < nptr[n + 1] - 1; t++)
+      if (nadj[t] / nn == nadj[t + 1] / nn) distinct = false;
+  if (c.shape == FGPU_BSP && nn <= 64 && maxs <= ROW_MAXS && distinct) {
+    std::vector<unsigned short> rslot(nadj.size() * (size_t)nn);
+    parallel_for(nnode, [&](long Al) {
+      const int A = (int)Al;
+      const int *r0 = blkB.data() + bptr[A], *r1 = blkB.data() + bptr[A + 1];
+      for (int t = nptr[A]; t < nptr[A + 1]; t++) {
+        const int e = nadj[t] / nn;
+        for (int b = 0; b < nn; b++) {
+          const int B = conn[(size_t)e * nn + b];
+          rslot[(size_t)t * nn + b] =
+              B < A ? (unsigned short)0xFFFF
+                    : (unsigned short)(std::lower_bound(r0, r1, B) - r0);
+        }
+      }
+    });
+    c.d_kptr = dupload(bptr.data(), bptr.size(), s);
+    c.d_rslot = dupload(rslot.data(), rslot.size(), s);
+    c.row_maxs = (int)maxs;
+  }
   FGPU_CHECK(hipStreamSynchronize(s));
 }
 
@@ -289,7 +317,7 @@ void destroy(Ctx *c) {
                   c->d_status, c->d_rowptr, c->d_col, c->d_bptr, c->d_bcol,
                   c->d_bcoloff, c->d_bdiag, c->d_fmask, c->d_bval, c->d_bn,
                   c->d_blkA, c->d_blkB, c->d_offAB, c->d_offBA, c->d_cptr,
-                  c->d_contrib, c->d_nptr, c->d_nadj, c->d_u,
+                  c->d_contrib, c->d_nptr, c->d_nadj, c->d_kptr, c->d_rslot, c->d_u,
                   c->d_b, c->d_x,
                   c->d_r, c->d_p, c->d_w, c->d_z, c->d_dinv, c->d_part,
                   c->d_scal, c->d_done, c->d_eqnode, c->d_TR, c->d_band, c->d_finfo,
@@ -406,12 +434,17 @@ void autotune(Ctx &c, const double *d_u) {
     c.asm_mode = mode;
     float ms = time3([&] { launch_assemble_K(c); });
     const int nb = (mode & 3) == 0 ? 1 : 2 << (mode & 3);
-    std::snprintf(buf, sizeof buf, "assembly mode %d (loads in batches of %d%s): %.4f ms\n", mode,
-                  nb, mode & 4 ? ", XCD-contiguous workgroups" : "", ms);
+    if (mode == 8)
+      std::snprintf(buf, sizeof buf, "assembly mode 8 (one wave per node row): %.4f ms\n", ms);
+    else
+      std::snprintf(buf, sizeof buf, "assembly mode %d (loads in batches of %d%s): %.4f ms\n",
+                    mode, nb, mode & 4 ? ", XCD-contiguous workgroups" : "", ms);
     rep += buf;
     if (ms < best) best = ms, pick = mode;
   }
   c.asm_mode = pick;
+  std::snprintf(buf, sizeof buf, "picked assembly mode %d\n", pick);
+  rep += buf;
   c.tune_report = rep;
 }
 

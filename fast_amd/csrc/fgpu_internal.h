@@ -183,6 +183,12 @@ struct Ctx {
   // node -> (element, local node) adjacency for the f_int gather
   int *d_nptr = nullptr;         // [nnode+1]
   int *d_nadj = nullptr;         // [nelem*nn] e * nn + a
+  // row assembly (elements of at most 64 nodes): upper-block row pointer,
+  // and for adjacency entry t and local node b the column of block (a, b)
+  // among the row's upper blocks (0xFFFF: node b precedes the row's node)
+  long long *d_kptr = nullptr;          // [nnode+1]
+  unsigned short *d_rslot = nullptr;    // [nelem*nn*nn]
+  int row_maxs = 0;                     // longest row (upper blocks)
 
   // ---- launch configuration (picked by the autotuner, identical results)
   int hex8_te = 0;       // elements per workgroup of the hex8 K kernel (0: largest)

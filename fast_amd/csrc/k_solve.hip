@@ -25,6 +25,26 @@
 namespace fgpu {
 namespace {
 
+// the summed upper block (A, B): fixed directions replaced by the identity,
+// stored at (A, B) and transposed at (B, A)
+__device__ inline void store_block(int A, int B, int pAB, int pBA, double s[9],
+                                   const unsigned char *__restrict__ fmask,
+                                   double *__restrict__ bval) {
+  const unsigned mA = fmask[A], mB = fmask[B];
+  for (int i = 0; i < 3; i++)
+    for (int j = 0; j < 3; j++) {
+      const bool live = ((mA >> i) & 1u) && ((mB >> j) & 1u);
+      s[3 * i + j] = live ? s[3 * i + j] : ((A == B && i == j) ? 1.0 : 0.0);
+    }
+  double *o = bval + 9 * (size_t)pAB;
+  for (int m = 0; m < 9; m++) o[m] = s[m];
+  if (A != B) {
+    double *t = bval + 9 * (size_t)pBA;
+    for (int i = 0; i < 3; i++)
+      for (int j = 0; j < 3; j++) t[3 * j + i] = s[3 * i + j];
+  }
+}
+
 // One thread per upper block (A, B): sum its contributions (sorted by
 // element), NB at a time: the NB codes are read, then all NB blocks are
 // loaded, then added in order (the loads of a batch are in flight together).
@@ -74,19 +94,80 @@ __global__ __launch_bounds__(256) void assemble_kernel(
       }
     }
   }
-  const int A = blkA[k], B = blkB[k];
-  const unsigned mA = fmask[A], mB = fmask[B];
-  for (int i = 0; i < 3; i++)
-    for (int j = 0; j < 3; j++) {
-      const bool live = ((mA >> i) & 1u) && ((mB >> j) & 1u);
-      s[3 * i + j] = live ? s[3 * i + j] : ((A == B && i == j) ? 1.0 : 0.0);
+  store_block(blkA[k], blkB[k], posAB[k], posBA[k], s, fmask, bval);
+}
+
+// Row assembly (elements of at most 64 nodes): one wave per node row A, its
+// upper blocks accumulated in LDS, 9 doubles per block, one column per block
+// in the row's order.  The wave walks the elements of A in element order, U
+// at a time; for element e with A at local node a, lane b reads the stored
+// block coupling a and b (row a of the element's blocks when b >= a, read as
+// it lies; column a, transposed, when b < a) and adds it to the column
+// rslot[(t, b)] names, skipping nodes b that precede A (0xFFFF).  Lanes of
+// one element add to distinct columns; elements follow one another in
+// order, so every column receives its contributions in element order, as the
+// per-block lists of assemble_kernel hold them: the same sums, bit for bit.
+// A lane's reads of consecutive b are consecutive blocks of one element.
+template <class T, int U>
+__global__ __launch_bounds__(64) void assemble_rows_kernel(
+    int nn, const int *__restrict__ nptr, const int *__restrict__ nadj,
+    const unsigned short *__restrict__ rslot, const long long *__restrict__ kptr,
+    const int *__restrict__ blkB, const int *__restrict__ posAB, const int *__restrict__ posBA,
+    const T *__restrict__ Ke, KeLayout L, const unsigned char *__restrict__ fmask,
+    double *__restrict__ bval, int maxs) {
+  extern __shared__ double acc[];  // [9][maxs]
+  const int A = blockIdx.x;
+  const int b = threadIdx.x;
+  const long long k0 = kptr[A];
+  const int ns = (int)(kptr[A + 1] - k0);
+  for (int j = b; j < ns; j += 64)
+    for (int m = 0; m < 9; m++) acc[m * maxs + j] = 0.0;
+  __syncthreads();
+  const int t0 = nptr[A], t1 = nptr[A + 1];
+  for (int t = t0; t < t1; t += U) {
+    int col[U], tr[U];
+    T v[U][9];
+#pragma unroll
+    for (int u = 0; u < U; u++) {
+      col[u] = 0xFFFF;
+      tr[u] = 0;
+      if (t + u < t1 && b < nn) {
+        const int x = nadj[t + u];
+        const int e = x / nn, a = x - e * nn;
+        col[u] = rslot[(size_t)(t + u) * nn + b];
+        // block (a, b) lies in row a of the element's blocks; it is read
+        // without waiting on the column table (its column exists unless the
+        // numbering runs against the element's local order)
+        if (b >= a) {
+          const T *p = Ke + L.offset(e, sym_index(a, b, nn));
+#pragma unroll
+          for (int m = 0; m < 9; m++) v[u][m] = p[m];
+        } else if (col[u] != 0xFFFF) {  // block (b, a), read transposed
+          tr[u] = 1;
+          const T *p = Ke + L.offset(e, sym_index(b, a, nn));
+#pragma unroll
+          for (int m = 0; m < 9; m++) v[u][m] = p[m];
+        }
+      }
     }
-  double *o = bval + 9 * (size_t)posAB[k];
-  for (int m = 0; m < 9; m++) o[m] = s[m];
-  if (A != B) {
-    double *t = bval + 9 * (size_t)posBA[k];
-    for (int i = 0; i < 3; i++)
-      for (int j = 0; j < 3; j++) t[3 * j + i] = s[3 * i + j];
+#pragma unroll
+    for (int u = 0; u < U; u++) {
+      if (col[u] == 0xFFFF) continue;
+      double *c = acc + col[u];
+      if (tr[u]) {
+        for (int q = 0; q < 3; q++)
+          for (int r = 0; r < 3; r++) c[(3 * q + r) * maxs] += (double)v[u][3 * r + q];
+      } else {
+        for (int m = 0; m < 9; m++) c[m * maxs] += (double)v[u][m];
+      }
+    }
+  }
+  __syncthreads();
+  for (int j = b; j < ns; j += 64) {
+    double sum[9];
+    for (int m = 0; m < 9; m++) sum[m] = acc[m * maxs + j];
+    const long long k = k0 + j;
+    store_block(A, blkB[k], posAB[k], posBA[k], sum, fmask, bval);
   }
 }
 
@@ -276,8 +357,28 @@ void bspmv_launch(Ctx &c, const double *x, double *y) {
 }  // namespace
 
 // assembly modes 0..7: contribution loads in batches of 1, 4, 8, 16
-// (mode & 3), XCD-contiguous workgroup order when mode & 4
-std::vector<int> assembly_options(const Ctx &) { return {0, 1, 2, 3, 4, 5, 6, 7}; }
+// (mode & 3), XCD-contiguous workgroup order when mode & 4; mode 8: one wave
+// per node row (meshes with the row tables)
+std::vector<int> assembly_options(const Ctx &c) {
+  std::vector<int> m = {0, 1, 2, 3, 4, 5, 6, 7};
+  if (c.d_rslot) m.push_back(8);
+  return m;
+}
+
+// U elements per batch of loads (FP32 element matrices; FP64 ones take half)
+template <int U>
+void launch_rows(Ctx &c) {
+  if (!c.d_rslot) throw Error("row assembly: no row tables for this mesh");
+  const size_t lds = sizeof(double) * 9 * (size_t)c.row_maxs;
+  if (c.d_Ke64)
+    hipLaunchKernelGGL((assemble_rows_kernel<double, U / 2>), dim3(c.nnode), dim3(64), lds,
+                       c.stream, c.nn, c.d_nptr, c.d_nadj, c.d_rslot, c.d_kptr, c.d_blkB,
+                       c.d_offAB, c.d_offBA, c.d_Ke64, c.kel, c.d_fmask, c.d_bval, c.row_maxs);
+  else
+    hipLaunchKernelGGL((assemble_rows_kernel<float, U>), dim3(c.nnode), dim3(64), lds, c.stream,
+                       c.nn, c.d_nptr, c.d_nadj, c.d_rslot, c.d_kptr, c.d_blkB, c.d_offAB,
+                       c.d_offBA, c.d_Ke, c.kel, c.d_fmask, c.d_bval, c.row_maxs);
+}
 
 template <int NB, bool XCD>
 void launch_contrib(Ctx &c) {
@@ -302,6 +403,7 @@ void launch_assemble_K(Ctx &c) {
     case 5: launch_contrib<4, true>(c); break;
     case 6: launch_contrib<8, true>(c); break;
     case 7: launch_contrib<16, true>(c); break;
+    case 8: launch_rows<8>(c); break;
     default: launch_contrib<8, false>(c); break;
   }
   FGPU_CHECK(hipGetLastError());

@@ -340,8 +340,10 @@ int fgpu_stresses_dev(fgpu_ctx *ctx, const double *u_nodal_dev, double *sig_ip_d
  * autotuner times every valid configuration of the element-stiffness kernel
  * (hex8: elements per workgroup 4..64) and of the assembly (contribution
  * loads in batches of 1, 4, 8 or 16: mode & 3; mode & 4: workgroups ordered
- * so that each of the 8 XCDs sweeps a contiguous range of block rows)
- * on the actual mesh and state, keeps the fastest and records the timings
+ * so that each of the 8 XCDs sweeps a contiguous range of block rows; mode
+ * 8, B-spline solids of at most 64 control points per element: one wave per
+ * node row, the row accumulated in LDS element by element) on the actual
+ * mesh and state, keeps the fastest and records the timings
  * (environment FGPU_AUTOTUNE=0 skips it).  All configurations give identical
  * results, bit for bit.  fgpu_set_launch fixes a configuration instead
  * (hex8_te 0: the largest; asm_mode -1: the default); 0 if it is not valid

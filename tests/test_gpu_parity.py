@@ -365,7 +365,7 @@ def test_launch_configurations_identical_bits(fg, oracle_mod, kind):
     ref = g.csr()[2].copy()
     seen = 0
     for te in ([0, 4, 8, 16, 32, 64] if kind != "bsp" else [0]):
-        for mode in range(8):
+        for mode in range(9):
             try:
                 g.set_launch(te, mode)
             except fg.FgpuError:
@@ -378,6 +378,47 @@ def test_launch_configurations_identical_bits(fg, oracle_mod, kind):
     om = oracle_mod.OracleMesh(m)
     Ko, ok = om.assemble_K(u)
     assert nrel(ref, Ko) <= 1e-6
+
+
+@pytest.mark.parametrize("kind", ["cubic_renumbered", "p332_svk", "quadratic_f64"])
+def test_row_assembly_identical_bits(fg, kind):
+    """The row assembly (one wave per node row, mode 8) gives the global
+    matrix of the per-block kernel bit for bit: on a mesh whose control
+    points are renumbered at random (so that element blocks are read
+    transposed as well), on the cubic-cubic-quadratic total-Lagrangian
+    kernel, and on FP64 element matrices (degree 2)."""
+    import dataclasses
+    if kind == "cubic_renumbered":
+        m = M.bsp_block(ne=(4, 3, 3), p=3, perturb=0.05, seed=12)
+        perm = np.random.default_rng(1).permutation(m.nnode).astype(np.int32)
+        coords = np.empty_like(m.coords)
+        coords[perm] = m.coords
+        fixed = np.empty_like(m.fixed)
+        fixed[perm] = m.fixed
+        cpw = None
+        if m.cpw is not None:
+            cpw = np.empty_like(m.cpw)
+            cpw[perm] = m.cpw
+        m = dataclasses.replace(m, coords=coords, conn=perm[m.conn], fixed=fixed, cpw=cpw,
+                                dofmap=None, fext=None)
+        u = None
+    elif kind == "p332_svk":
+        m = M.bsp_block(ne=(3, 3, 3), p=(3, 3, 2), perturb=0.05, seed=13, form="tl_svk",
+                        mat=np.tile((2.6, 0.3), (27, 1)))
+        X = m.coords
+        u = np.stack([0.02 * X[:, 0] * X[:, 2], 0.01 * X[:, 1] ** 2, -0.03 * X[:, 0] ** 2], 1)
+        u[m.fixed] = 0
+    else:
+        m = M.bsp_block(ne=(4, 3, 3), p=2, perturb=0.05, seed=14)
+        u = None
+    g = fg.GpuModel(m)
+    g.set_launch(0, 2)
+    g.stiffness(u)
+    ref = g.csr()[2].copy()
+    g.set_launch(0, 8)
+    g.stiffness(u)
+    assert np.array_equal(g.csr()[2], ref)
+    g.close()
 
 
 @pytest.mark.parametrize("form,matp,deg", [("tl_svk", (2.6, 0.3), 3),
