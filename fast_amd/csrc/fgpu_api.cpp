@@ -238,14 +238,20 @@ void build_maps(Ctx &c, const int *conn) {
   c.d_nadj = dupload(nadj.data(), nadj.size(), s);
   // row assembly tables: B-spline solids of at most 64 control points per
   // element whose rows fit the kernel's LDS (ROW_MAXS upper blocks)
-  constexpr long long ROW_MAXS = 896;  // 9 doubles each: 63 KB of LDS
+  constexpr long long ROW_MAXS = 848;  // 77 bytes of LDS each: 63.8 KB
   long long maxs = 0;
   for (int n = 0; n < nnode; n++) maxs = std::max(maxs, nb_up[n]);
-  bool distinct = true;  // an element's nodes must be distinct (one column per lane)
-  for (int n = 0; n < nnode && distinct; n++)
+  // the kernel relies on: an element's nodes distinct (one column per lane);
+  // a row's upper blocks consecutive in the node-block rows, the diagonal first
+  bool fits = true;
+  for (int n = 0; n < nnode && fits; n++) {
     for (int t = nptr[n]; t < nptr[n + 1] - 1; t++)
-      if (nadj[t] / nn == nadj[t + 1] / nn) distinct = false;
-  if (c.shape == FGPU_BSP && nn <= 64 && maxs <= ROW_MAXS && distinct) {
+      if (nadj[t] / nn == nadj[t + 1] / nn) fits = false;
+    for (long long k = bptr[n]; k < bptr[n + 1]; k++)
+      if (offAB[k] != offAB[bptr[n]] + (k - bptr[n])) fits = false;
+    if (bptr[n + 1] > bptr[n] && blkB[bptr[n]] != n) fits = false;
+  }
+  if (c.shape == FGPU_BSP && nn <= 64 && maxs <= ROW_MAXS && fits) {
     std::vector<unsigned short> rslot(nadj.size() * (size_t)nn);
     parallel_for(nnode, [&](long Al) {
       const int A = (int)Al;

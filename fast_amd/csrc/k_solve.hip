@@ -115,13 +115,18 @@ __global__ __launch_bounds__(64) void assemble_rows_kernel(
     const int *__restrict__ blkB, const int *__restrict__ posAB, const int *__restrict__ posBA,
     const T *__restrict__ Ke, KeLayout L, const unsigned char *__restrict__ fmask,
     double *__restrict__ bval, int maxs) {
-  extern __shared__ double acc[];  // [9][maxs]
+  extern __shared__ double acc[];  // [9][maxs], then [maxs] positions of (B, A), [maxs] masks
+  int *sBA = reinterpret_cast<int *>(acc + 9 * maxs);
+  unsigned char *sMB = reinterpret_cast<unsigned char *>(sBA + maxs);
   const int A = blockIdx.x;
   const int b = threadIdx.x;
   const long long k0 = kptr[A];
   const int ns = (int)(kptr[A + 1] - k0);
-  for (int j = b; j < ns; j += 64)
+  for (int j = b; j < ns; j += 64) {
     for (int m = 0; m < 9; m++) acc[m * maxs + j] = 0.0;
+    sBA[j] = posBA[k0 + j];
+    sMB[j] = fmask[blkB[k0 + j]];
+  }
   __syncthreads();
   const int t0 = nptr[A], t1 = nptr[A + 1];
   for (int t = t0; t < t1; t += U) {
@@ -163,11 +168,23 @@ __global__ __launch_bounds__(64) void assemble_rows_kernel(
     }
   }
   __syncthreads();
-  for (int j = b; j < ns; j += 64) {
-    double sum[9];
-    for (int m = 0; m < 9; m++) sum[m] = acc[m * maxs + j];
-    const long long k = k0 + j;
-    store_block(A, blkB[k], posAB[k], posBA[k], sum, fmask, bval);
+  // write-out value by value, so that consecutive lanes store consecutive
+  // doubles: value i is component m = i % 9 of block j = i / 9.  The row's
+  // upper blocks are consecutive in the node-block rows, the first being the
+  // diagonal (A, A); the transposed copy (B, A) of block j goes to sBA[j].
+  // Fixed directions as in store_block.
+  const unsigned mA = fmask[A];
+  const size_t pAB0 = ns > 0 ? 9 * (size_t)posAB[k0] : 0;
+  for (int i = b; i < 9 * ns; i += 64) {
+    const int j = i / 9, m = i - 9 * j;
+    const int q = m / 3, r = m - 3 * q;
+    const unsigned mB = sMB[j];
+    const bool live = (mA >> q) & (mB >> r) & 1u;
+    bval[pAB0 + i] = live ? acc[m * maxs + j] : ((j == 0 && q == r) ? 1.0 : 0.0);
+    if (j > 0) {  // (B, A)[q][r] = (A, B)[r][q]
+      const bool liveT = (mA >> r) & (mB >> q) & 1u;
+      bval[9 * (size_t)sBA[j] + m] = liveT ? acc[(3 * r + q) * maxs + j] : 0.0;
+    }
   }
 }
 
@@ -369,7 +386,7 @@ std::vector<int> assembly_options(const Ctx &c) {
 template <int U>
 void launch_rows(Ctx &c) {
   if (!c.d_rslot) throw Error("row assembly: no row tables for this mesh");
-  const size_t lds = sizeof(double) * 9 * (size_t)c.row_maxs;
+  const size_t lds = (sizeof(double) * 9 + sizeof(int) + 1) * (size_t)c.row_maxs;
   if (c.d_Ke64)
     hipLaunchKernelGGL((assemble_rows_kernel<double, U / 2>), dim3(c.nnode), dim3(64), lds,
                        c.stream, c.nn, c.d_nptr, c.d_nadj, c.d_rslot, c.d_kptr, c.d_blkB,
