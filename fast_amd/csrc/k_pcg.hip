@@ -27,14 +27,18 @@ constexpr int CHUNK_IT = 16;  // iterations per captured graph
 // ---------------------------------------------------------------- single
 // scal: [0] lenf  [1],[2] z.r by iteration parity  [3] |r|/lenf  [4] |b|
 //       [6] tol ; done[0] = iteration at convergence, done[1] = breakdown,
-//       done[2] = current iteration
+//       done[2] = current iteration.  An iteration is three launches: the
+//       product (whose block 0 opens the iteration, done[2] += 1; no other
+//       block of it reads done[2]), the update of x, r, z and the direction
+//       (whose blocks all find convergence alike, block 0 recording it).
 template <int G, class T>
 __global__ __launch_bounds__(256) void pcg_spmv_kernel(
     int nnode, const long long *__restrict__ bptr, const int *__restrict__ bcol,
     const T *__restrict__ bval, const double *__restrict__ p, double *__restrict__ w,
-    double *__restrict__ part, const int *__restrict__ done) {
+    double *__restrict__ part, int *__restrict__ done) {
   __shared__ double sh[16];
   if (done[0]) return;
+  if (blockIdx.x == 0 && threadIdx.x == 0) done[2] = done[2] + 1;
   const int rows_per_block = blockDim.x / G;
   double dot = bspmv_rows<G, T>((long long)blockIdx.x * rows_per_block, nnode,
                              (long long)gridDim.x * rows_per_block, bptr, bcol, bval, p, w);
@@ -82,7 +86,7 @@ __global__ void pcg_start_kernel(const double *__restrict__ part, double *__rest
     scal[6] = tol;
     done[0] = 0;
     done[1] = 0;
-    done[2] = 1;
+    done[2] = 0;
   }
 }
 
@@ -128,8 +132,9 @@ __global__ __launch_bounds__(256) void pcg_dir_kernel(
     if (blockIdx.x == 0 && threadIdx.x == 0) {
       scal[3] = err;
       done[1] = !(err == err);
+      done[0] = it;  // the other blocks reach the same test and return alike
     }
-    return;  // done[0] is set by the step kernel, after every block has read it
+    return;
   }
   const double beta = zr / scal[1 + ((it - 1) & 1)];
   for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n;
@@ -139,16 +144,6 @@ __global__ __launch_bounds__(256) void pcg_dir_kernel(
     scal[1 + (it & 1)] = zr;
     scal[3] = err;
   }
-}
-
-// closes iteration `it`: converged -> done[0] = it, else it + 1
-__global__ void pcg_step_kernel(const double *__restrict__ scal, int *__restrict__ done) {
-  if (threadIdx.x != 0 || done[0]) return;
-  const double err = scal[3];
-  if (err < scal[6] || done[1])
-    done[0] = done[2];
-  else
-    done[2] = done[2] + 1;
 }
 
 __global__ void diag_inv_kernel(int nnode, const long long *__restrict__ bdiag,
@@ -186,7 +181,6 @@ void enqueue_iteration(Ctx &c) {
                      c.d_z, c.d_p, c.d_w, c.d_dinv, c.d_part, c.d_scal, c.d_done);
   hipLaunchKernelGGL(pcg_dir_kernel, dim3(RED), dim3(TPB), 0, c.stream, n, c.d_p, c.d_z,
                      c.d_part, c.d_scal, c.d_done);
-  hipLaunchKernelGGL(pcg_step_kernel, dim3(1), dim3(64), 0, c.stream, c.d_scal, c.d_done);
 }
 
 // ---------------------------------------------------------------- batch
@@ -453,7 +447,7 @@ int run_pcg(Ctx &c, const double *b, double *x, const double *x0, int maxit, dou
       broke = c.h_done[1];
       break;
     }
-    if (c.h_done[2] > maxit) break;
+    if (c.h_done[2] >= maxit) break;
   }
   // a chunk may run past maxit: the iteration count is what the device did
   launch_nodal_to_eq(c, c.d_x, x);
@@ -462,7 +456,7 @@ int run_pcg(Ctx &c, const double *b, double *x, const double *x0, int maxit, dou
     if (relres) *relres = (abs_exit && c.h_scal[4] < rtol) || c.h_scal[4] == 0.0 ? 0.0 : c.h_scal[3];
     return 1;
   }
-  if (iters) *iters = conv_it ? conv_it : c.h_done[2] - 1;
+  if (iters) *iters = conv_it ? conv_it : c.h_done[2];
   if (relres) *relres = c.h_scal[3];
   return conv_it > 0 && conv_it <= maxit && !broke;
 }
