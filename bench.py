@@ -165,6 +165,44 @@ def cpu_baseline(m, u, budget_s=12.0):
     return res, ncores
 
 
+def reference_baseline(cfg, ne=(8, 8, 8)):
+    """The reference's own code on the CPU: oracle/_ref/ref_probe (linked from
+    the reference's object files, built from its sources as it ships: -O3, one
+    thread) times cControl::StiffnessMatrix (element StiffMat + AddGlobMat into
+    its sparse system matrix, ctrl/ctrl.cpp:886-975) and
+    cControl::IntForceVector (ctrl/ctrl.cpp:719-813) on a smaller patch of the
+    same discretization.  None where the build or the workload's model is
+    missing from the reference."""
+    import subprocess
+    import tempfile
+    exe = os.path.join(ROOT, "oracle", "_ref", "ref_probe")
+    if not os.path.exists(exe) or not cfg["workload"].startswith("iga_cubic_bspline"):
+        return None
+    from fast_amd import mesh as M
+    sys.path.insert(0, os.path.join(ROOT, "tests", "golden"))
+    from gen_golden import write_deck
+    sample = M.bsp_block(ne=ne, p=3, perturb=0.05, seed=0)
+    with tempfile.TemporaryDirectory() as d:
+        base = os.path.join(d, "sample")
+        write_deck(base + ".dat", sample, matrix="sparse")
+        try:
+            out = subprocess.run([exe, base, "-", "--time", "1"], capture_output=True, text=True,
+                                 timeout=600, check=True).stdout
+        except (subprocess.SubprocessError, OSError) as err:
+            print("bench: reference baseline failed (%s); the oracle's stands" % err,
+                  file=sys.stderr)
+            return None
+    f = out.split()
+    nelem, tk, tf = int(f[1]), float(f[3]), float(f[5])
+    return {"value": round(nelem / (tk + tf), 2), "unit": "elements/s", "cores": 1,
+            "kind": "reference",
+            "sample": "the reference's cControl::StiffnessMatrix + IntForceVector (its own "
+                      "code, built from its sources, one thread as shipped) on a %dx%dx%d cubic "
+                      "B-spline patch (%d elements, 4x4x4 Gauss, same generator), sparse system "
+                      "matrix: %.2f s + %.2f s" % (ne + (nelem, tk, tf)),
+            "stiffness_s": tk, "int_force_s": tf}
+
+
 def accuracy(model, m, u, nsample=32):
     """Relative error of K_e u_e (FP32 path) against the FP64 oracle on a
     sample of elements, normwise and forward."""
@@ -454,10 +492,17 @@ def main():
         res, ncores = cpu_baseline(m, u)
         v, ne, dt = res[ncores]
         v1 = res[1]
-        cpu = {"value": round(v, 2), "unit": "elements/s", "cores": ncores, "kind": "port",
-               "sample": "oracle K_e+assembly+f_int over the first %d elements (%.1f s) of the "
-                         "same workload" % (ne, dt),
-               "single_thread": round(v1[0], 2)}
+        port = {"value": round(v, 2), "unit": "elements/s", "cores": ncores, "kind": "port",
+                "sample": "oracle K_e+assembly+f_int over the first %d elements (%.1f s) of the "
+                          "same workload" % (ne, dt),
+                "single_thread": round(v1[0], 2)}
+        # the reference's own code where it builds and has the workload's model;
+        # the oracle (a restatement of its loops, threaded) beside it
+        cpu = reference_baseline(cfg)
+        if cpu is None:
+            cpu = port
+        else:
+            cpu["port"] = port
     out = {
         "metric": "elements/s (K+f_int+assembly)",
         "value": round(value, 1), "unit": "elements/s", "n_gpus": 1, "steps": args.steps,

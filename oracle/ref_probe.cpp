@@ -10,12 +10,19 @@
 // for a prescribed nodal displacement field.
 //
 // usage: ref_probe <deck-basename> <nodal-displ-file|-> <out.bin>
+//        ref_probe <deck-basename> <nodal-displ-file|-> --time <reps>
 //   nodal-displ-file: text, 3 doubles per node in node order (or "-" = zero)
+//   --time: instead of the dump, times the reference's own assembly loops
+//   cControl::StiffnessMatrix (element StiffMat + AddGlobMat into the system
+//   matrix the deck names, ctrl/ctrl.cpp:886-975) and
+//   cControl::IntForceVector (ctrl/ctrl.cpp:719-813), best of <reps> calls
+//   each, and prints "nelem <n> stiffness_s <t> int_force_s <t>".
 // out.bin layout (little endian):
 //   int32 nnode, int32 numeq, int32 nelem
 //   int32 dof[nnode*3]                 (node VecDof[0..2], 1-based, 0 = fixed)
 //   per element: int32 ndof, int32 edof[ndof], f64 K[ndof*ndof] (row major),
 //                f64 g[ndof], int32 ok
+#include <chrono>
 #include <cstdio>
 #include <cstdlib>
 #include <string>
@@ -32,6 +39,19 @@ using namespace std;
 #include "vec.h"
 #include "mat.h"
 #include "shpiga.h"
+
+// access to the analysis set-up (system-matrix type, equations, profile)
+// that cControl keeps for its drivers
+struct cProbeCtrl : public cControl
+{
+  static cSysMatrix *NewMatrix(void) { return cSysMatrix :: CreateMatrix(SysMatType, NumEq, Profile); }
+  static int Neq(void) { return NumEq; }
+};
+
+static double Seconds(chrono::steady_clock::time_point t0)
+{
+  return chrono::duration<double>(chrono::steady_clock::now( ) - t0).count( );
+}
 
 ifstream in;
 ofstream out, outem;
@@ -63,6 +83,29 @@ int main(int argc, char **argv)
   {
     cNode *nd = cNode :: GetNode(i);
     for (int j = 0; j < 6; j++) nd->SetDispl(j, j < 3 ? ud[3*i+j] : 0.0);
+  }
+
+  if (string(argv[3]) == "--time")
+  {
+    int reps = argc > 4 ? atoi(argv[4]) : 1;
+    if (reps < 1) reps = 1;
+    cSysMatrix *K = cProbeCtrl :: NewMatrix( );
+    cVector g(cProbeCtrl :: Neq( ));
+    double tk = 1e30, tf = 1e30;
+    for (int r = 0; r < reps; r++)
+    {
+      chrono::steady_clock::time_point t0 = chrono::steady_clock::now( );
+      cControl :: StiffnessMatrix(K);
+      double t = Seconds(t0);
+      if (t < tk) tk = t;
+      t0 = chrono::steady_clock::now( );
+      cControl :: IntForceVector(g);
+      t = Seconds(t0);
+      if (t < tf) tf = t;
+    }
+    printf("nelem %d stiffness_s %.6f int_force_s %.6f\n", cElement :: GetNumElm( ), tk, tf);
+    delete K;
+    return 0;
   }
 
   FILE *fo = fopen(argv[3], "wb");
