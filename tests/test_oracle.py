@@ -1,5 +1,6 @@
 """The CPU oracle pinned against the reference's outputs and against the
 mathematics (closed forms, invariants, brute force).  No GPU needed."""
+import os
 import numpy as np
 import pytest
 
@@ -410,3 +411,17 @@ def test_stress_of_homogeneous_deformation(oracle_mod, form, mat):
         if form != "linear":
             si, _, _, ok = om.stresses(_homogeneous(m, Q - np.eye(3)))
             assert ok and np.abs(si).max() <= 1e-12
+
+
+@pytest.mark.skipif(not os.path.exists(os.path.join(
+    os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "oracle", "_ref", "ref_probe")),
+    reason="the reference build (oracle/_ref) is not present")
+def test_reference_cpu_baseline_runs():
+    """bench.py's CPU baseline runs the reference's own assembly loops
+    (ref_probe's timing mode) on a small patch and reports their rate."""
+    import bench
+    res = bench.reference_baseline({"workload": "iga_cubic_bspline_32x32x8"}, ne=(2, 2, 2))
+    assert res["kind"] == "reference" and res["cores"] == 1
+    assert res["value"] > 0 and "8 elements" in res["sample"]
+    assert res["stiffness_s"] > 0 and res["int_force_s"] >= 0
+    assert bench.reference_baseline({"workload": "hex8_neohooke_64x64x64"}) is None
