@@ -470,23 +470,32 @@ def main():
         it = fgpu.C.c_int(0)
         rr = fgpu.C.c_double(0.0)
         model.stiffness(u)
-        if m.batch_nodeptr is not None:
-            nb = len(m.batch_nodeptr) - 1
-            its = np.zeros(nb, dtype=np.int32)
-            rrs = np.zeros(nb)
-            nconv = L.fgpu_solve_batch_dev(model.h, b.data_ptr(), x.data_ptr(), 100000, 1e-8,
-                                           its.ctypes.data, rrs.ctypes.data)
-            ms = L.fgpu_last_time_ms(model.h, 3)
-            solve = {"rtol": 1e-8, "meshes": nb, "converged": int(nconv),
-                     "iters_max": int(its.max()), "iters_mean": float(its.mean()),
-                     "relres_max": float(rrs.max()), "ms": round(ms, 3)}
-        else:
-            conv = L.fgpu_solve_dev(model.h, b.data_ptr(), x.data_ptr(), None, 100000, 1e-8,
-                                    fgpu.C.byref(it), fgpu.C.byref(rr))
-            ms = L.fgpu_last_time_ms(model.h, 3)
-            solve = {"rtol": 1e-8, "iters": it.value, "relres": rr.value,
-                     "converged": bool(conv), "ms": round(ms, 3),
-                     "ms_per_iter": round(ms / max(it.value, 1), 4)}
+        solve = {}
+        # the PCG on the FP64 matrix, then on its FP32 copy (FP64 vectors and
+        # dot products either way)
+        for bits in (64, 32):
+            model.set_matrix_precision(bits)
+            x.zero_()
+            torch.cuda.synchronize()
+            key = "fp%d_matrix" % bits
+            if m.batch_nodeptr is not None:
+                nb = len(m.batch_nodeptr) - 1
+                its = np.zeros(nb, dtype=np.int32)
+                rrs = np.zeros(nb)
+                nconv = L.fgpu_solve_batch_dev(model.h, b.data_ptr(), x.data_ptr(), 100000, 1e-8,
+                                               its.ctypes.data, rrs.ctypes.data)
+                ms = L.fgpu_last_time_ms(model.h, 3)
+                solve[key] = {"rtol": 1e-8, "meshes": nb, "converged": int(nconv),
+                              "iters_max": int(its.max()), "iters_mean": float(its.mean()),
+                              "relres_max": float(rrs.max()), "ms": round(ms, 3)}
+            else:
+                conv = L.fgpu_solve_dev(model.h, b.data_ptr(), x.data_ptr(), None, 100000, 1e-8,
+                                        fgpu.C.byref(it), fgpu.C.byref(rr))
+                ms = L.fgpu_last_time_ms(model.h, 3)
+                solve[key] = {"rtol": 1e-8, "iters": it.value, "relres": rr.value,
+                              "converged": bool(conv), "ms": round(ms, 3),
+                              "ms_per_iter": round(ms / max(it.value, 1), 4)}
+        model.set_matrix_precision(64)
     cpu = None
     if not args.no_cpu:
         res, ncores = cpu_baseline(m, u)
