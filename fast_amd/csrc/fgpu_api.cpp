@@ -344,7 +344,7 @@ void destroy(Ctx *c) {
   if (c->stream) (void)hipStreamDestroy(c->stream);
   if (c->stream2) (void)hipStreamDestroy(c->stream2);
   if (c->ev_fork) (void)hipEventDestroy(c->ev_fork);
-  if (c->d_status2) (void)hipFree(c->d_status2);
+  if (c->h_status) (void)hipHostFree(c->h_status);
   delete c;
 }
 
@@ -368,14 +368,17 @@ float elapsed(Ctx &c, int a, int b) {
   return ms;
 }
 
+// status[4] of the element kernels (count_jacobian), read into pinned host
+// memory behind the kernels that count into it (one synchronization)
+void enqueue_status_read(Ctx &c, const int *d_st, int n) {
+  FGPU_CHECK(hipMemcpyAsync(c.h_status, d_st, n * sizeof(int), hipMemcpyDeviceToHost, c.stream));
+}
+
 // status[4] of the element kernels (count_jacobian): a zero or non-finite
 // Jacobian, or negative Jacobians at some but not all points of the mesh
 // (inverted elements), fail; all negative is a left-handed parametrization,
 // integrated with |det J|.
-int check_status(Ctx &c, const char *what) {
-  int st[4];
-  FGPU_CHECK(hipMemcpyAsync(st, c.d_status, sizeof st, hipMemcpyDeviceToHost, c.stream));
-  FGPU_CHECK(hipStreamSynchronize(c.stream));
+int check_status_values(Ctx &c, const char *what, const int *st) {
   const long long npts = (long long)c.nelem * c.nip[0] * c.nip[1] * c.nip[2];
   if (st[3]) {
     c.err = std::string(what) + ": " + std::to_string(st[3]) +
@@ -393,6 +396,12 @@ int check_status(Ctx &c, const char *what) {
     return 0;
   }
   return 1;
+}
+
+int check_status(Ctx &c, const char *what) {
+  enqueue_status_read(c, c.d_status, 4);
+  FGPU_CHECK(hipStreamSynchronize(c.stream));
+  return check_status_values(c, what, c.h_status);
 }
 
 }  // namespace
@@ -467,10 +476,11 @@ int stiffness_dev(Ctx &c, const double *d_u) {
   FGPU_CHECK(hipEventRecord(c.ev[1], c.stream));
   launch_assemble_K(c);
   FGPU_CHECK(hipEventRecord(c.ev[2], c.stream));
-  FGPU_CHECK(hipEventSynchronize(c.ev[2]));
+  enqueue_status_read(c, c.d_status, 4);
+  FGPU_CHECK(hipStreamSynchronize(c.stream));
   c.last_ms[0] = elapsed(c, 0, 1);
   c.last_ms[1] = elapsed(c, 1, 2);
-  return check_status(c, "stiffness");
+  return check_status_values(c, "stiffness", c.h_status);
 }
 
 int int_force_dev(Ctx &c, const double *d_u, double *d_g) {
@@ -482,9 +492,10 @@ int int_force_dev(Ctx &c, const double *d_u, double *d_g) {
     launch_bsp_fint(c, d_u);
   launch_gather_fint(c, d_g);
   FGPU_CHECK(hipEventRecord(c.ev[4], c.stream));
-  FGPU_CHECK(hipEventSynchronize(c.ev[4]));
+  enqueue_status_read(c, c.d_status, 4);
+  FGPU_CHECK(hipStreamSynchronize(c.stream));
   c.last_ms[2] = elapsed(c, 3, 4);
-  return check_status(c, "internal force");
+  return check_status_values(c, "internal force", c.h_status);
 }
 
 namespace {
@@ -541,8 +552,8 @@ int stiffness_fint_dev(Ctx &c, const double *d_u, double *d_g) {
 int stiffness_fint_overlapped(Ctx &c, const double *d_u, double *d_g) {
   c.factored = false;
   c.bval32_valid = false;
-  FGPU_CHECK(hipMemsetAsync(c.d_status, 0, 4 * sizeof(int), c.stream));
-  FGPU_CHECK(hipMemsetAsync(c.d_status2, 0, 4 * sizeof(int), c.stream));
+  // the two status arrays are adjacent (d_status2 = d_status + 4)
+  FGPU_CHECK(hipMemsetAsync(c.d_status, 0, 8 * sizeof(int), c.stream));
   FGPU_CHECK(hipEventRecord(c.ev[0], c.stream));
   if (c.shape == FGPU_HEX8)
     launch_hex8_K(c, d_u);
@@ -566,15 +577,13 @@ int stiffness_fint_overlapped(Ctx &c, const double *d_u, double *d_g) {
   launch_assemble_K(c);
   FGPU_CHECK(hipEventRecord(c.ev[2], c.stream));
   FGPU_CHECK(hipStreamWaitEvent(c.stream, c.ev[4], 0));  // join
+  enqueue_status_read(c, c.d_status, 8);  // both arrays
   FGPU_CHECK(hipStreamSynchronize(c.stream));
   c.last_ms[0] = elapsed(c, 0, 1);
   c.last_ms[1] = elapsed(c, 1, 2);
   c.last_ms[2] = elapsed(c, 3, 4);
-  if (!check_status(c, "stiffness")) return 0;
-  std::swap(c.d_status, c.d_status2);
-  const int ok = check_status(c, "internal force");
-  std::swap(c.d_status, c.d_status2);
-  return ok;
+  if (!check_status_values(c, "stiffness", c.h_status)) return 0;
+  return check_status_values(c, "internal force", c.h_status + 4);
 }
 
 }  // namespace fgpu
@@ -653,8 +662,8 @@ int fgpu_create(fgpu_ctx **out, const fgpu_mesh *m) {
     }
     c->d_gauss64 = dupload(gp, 3 * 2 * 16, s);
     c->d_gauss = dupload(gpf, 3 * 2 * 16, s);
-    c->d_status = dalloc<int>(4);
-    c->d_status2 = dalloc<int>(4);
+    c->d_status = dalloc<int>(8);  // the second stream's counters follow
+    c->d_status2 = c->d_status + 4;
     build_maps(*c, m->conn);
     int nsym = nsym_blocks(c->nn);
     c->kel = KeLayout{c->shape == FGPU_HEX8 ? 64 : 1, nsym};
@@ -683,6 +692,7 @@ int fgpu_create(fgpu_ctx **out, const fgpu_mesh *m) {
     c->d_scal = dalloc<double>(16);
     c->d_done = dalloc<int>(4);
     FGPU_CHECK(hipHostMalloc(&c->h_done, 4 * sizeof(int)));
+    FGPU_CHECK(hipHostMalloc(&c->h_status, 8 * sizeof(int)));
     FGPU_CHECK(hipHostMalloc(&c->h_scal, 16 * sizeof(double)));
     FGPU_CHECK(hipStreamSynchronize(s));
   } catch (const Error &e) {

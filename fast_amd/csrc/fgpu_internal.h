@@ -121,7 +121,8 @@ struct Ctx {
   // second stream (default priority) for f_int beside K, its status counters
   // and the fork event
   hipStream_t stream2 = nullptr;
-  int *d_status2 = nullptr;
+  int *d_status2 = nullptr;  // d_status + 4 (one allocation)
+  int *h_status = nullptr;   // pinned mirror of both status arrays [8]
   hipEvent_t ev_fork = nullptr;
 
   // ---- device mesh
