@@ -251,7 +251,9 @@ void build_maps(Ctx &c, const int *conn) {
       if (offAB[k] != offAB[bptr[n]] + (k - bptr[n])) fits = false;
     if (bptr[n + 1] > bptr[n] && blkB[bptr[n]] != n) fits = false;
   }
-  if (c.shape == FGPU_BSP && nn <= 64 && maxs <= ROW_MAXS && fits) {
+  // the table holds 2 bytes per element, local node pair: built up to 4 GB
+  const bool small = (double)nadj.size() * nn * 2.0 <= 4.0 * (1 << 30);
+  if (c.shape == FGPU_BSP && nn <= 64 && maxs <= ROW_MAXS && fits && small) {
     std::vector<unsigned short> rslot(nadj.size() * (size_t)nn);
     parallel_for(nnode, [&](long Al) {
       const int A = (int)Al;
