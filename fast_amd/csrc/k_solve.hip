@@ -25,25 +25,9 @@
 namespace fgpu {
 namespace {
 
-// the summed upper block (A, B): fixed directions replaced by the identity,
-// stored at (A, B) and transposed at (B, A)
-__device__ inline void store_block(int A, int B, int pAB, int pBA, double s[9],
-                                   const unsigned char *__restrict__ fmask,
-                                   double *__restrict__ bval) {
-  const unsigned mA = fmask[A], mB = fmask[B];
-  for (int i = 0; i < 3; i++)
-    for (int j = 0; j < 3; j++) {
-      const bool live = ((mA >> i) & 1u) && ((mB >> j) & 1u);
-      s[3 * i + j] = live ? s[3 * i + j] : ((A == B && i == j) ? 1.0 : 0.0);
-    }
-  double *o = bval + 9 * (size_t)pAB;
-  for (int m = 0; m < 9; m++) o[m] = s[m];
-  if (A != B) {
-    double *t = bval + 9 * (size_t)pBA;
-    for (int i = 0; i < 3; i++)
-      for (int j = 0; j < 3; j++) t[3 * j + i] = s[3 * i + j];
-  }
-}
+// Fixed directions, in both assembly kernels: entry (i, j) of block (A, B)
+// is kept where direction i of A and direction j of B are free (fmask bits),
+// otherwise it is 1 on the diagonal of a diagonal block and 0 elsewhere.
 
 // One thread per upper block (A, B): sum its contributions (sorted by
 // element), NB at a time: the NB codes are read, then all NB blocks are
@@ -62,12 +46,17 @@ __global__ __launch_bounds__(256) void assemble_kernel(
     const long long *__restrict__ cptr, const int *__restrict__ contrib,
     const T *__restrict__ Ke, KeLayout L, const unsigned char *__restrict__ fmask,
     double *__restrict__ bval) {
+  __shared__ double sv[256 * 9];  // the workgroup's summed blocks, masked
+  __shared__ int sAB[256], sBA[256];
   const long long w = XCD ? (long long)(blockIdx.x & 7) * (gridDim.x >> 3) + (blockIdx.x >> 3)
                           : (long long)blockIdx.x;
-  long long k = w * blockDim.x + threadIdx.x;
-  if (k >= nblk) return;
+  const int tid = threadIdx.x;
+  const long long k0 = w * blockDim.x;
+  const long long k = k0 + tid;
+  const int nk = (int)min((long long)blockDim.x, nblk - k0);  // blocks of this workgroup
+  if (nk <= 0) return;  // the whole workgroup
   double s[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
-  const long long c0 = cptr[k], c1 = cptr[k + 1];
+  const long long c0 = k < nblk ? cptr[k] : 0, c1 = k < nblk ? cptr[k + 1] : 0;
   for (long long tb = c0; tb < c1; tb += NB) {
     int code[NB];
 #pragma unroll
@@ -94,7 +83,26 @@ __global__ __launch_bounds__(256) void assemble_kernel(
       }
     }
   }
-  store_block(blkA[k], blkB[k], posAB[k], posBA[k], s, fmask, bval);
+  if (k < nblk) {
+    const int A = blkA[k], B = blkB[k];
+    const unsigned mA = fmask[A], mB = fmask[B];
+    for (int i = 0; i < 3; i++)
+      for (int j = 0; j < 3; j++) {
+        const bool live = ((mA >> i) & 1u) && ((mB >> j) & 1u);
+        sv[9 * tid + 3 * i + j] = live ? s[3 * i + j] : ((A == B && i == j) ? 1.0 : 0.0);
+      }
+    sAB[tid] = posAB[k];
+    sBA[tid] = A != B ? posBA[k] : -1;
+  }
+  __syncthreads();
+  // write-out value by value (consecutive lanes store consecutive doubles):
+  // value i is component m = i % 9 of the workgroup's block t = i / 9, at
+  // (A, B) as summed and at (B, A) transposed
+  for (int i = tid; i < 9 * nk; i += blockDim.x) {
+    const int t = i / 9, m = i - 9 * t;
+    bval[9 * (size_t)sAB[t] + m] = sv[i];
+    if (sBA[t] >= 0) bval[9 * (size_t)sBA[t] + m] = sv[9 * t + 3 * (m % 3) + m / 3];
+  }
 }
 
 // Row assembly (elements of at most 64 nodes): one wave per node row A, its
@@ -172,7 +180,7 @@ __global__ __launch_bounds__(64) void assemble_rows_kernel(
   // doubles: value i is component m = i % 9 of block j = i / 9.  The row's
   // upper blocks are consecutive in the node-block rows, the first being the
   // diagonal (A, A); the transposed copy (B, A) of block j goes to sBA[j].
-  // Fixed directions as in store_block.
+  // Fixed directions as above.
   const unsigned mA = fmask[A];
   const size_t pAB0 = ns > 0 ? 9 * (size_t)posAB[k0] : 0;
   for (int i = b; i < 9 * ns; i += 64) {
