@@ -196,36 +196,28 @@ __global__ __launch_bounds__(64) void assemble_rows_kernel(
   }
 }
 
+// One thread per node direction (3 nnode threads; the three of a node read
+// one 24-byte element vector entry together), its element contributions
+// added in adjacency order, eight in flight at a time.
 __global__ __launch_bounds__(256) void gather_fint_kernel(
     int nnode, int nn, const int *__restrict__ nptr, const int *__restrict__ nadj,
     const double *__restrict__ ge, const int *__restrict__ eqmap, double *__restrict__ g) {
-  const int A = blockIdx.x * blockDim.x + threadIdx.x;
-  if (A >= nnode) return;
-  double s0 = 0, s1 = 0, s2 = 0;
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= 3 * nnode) return;
+  const int A = i / 3, d = i - 3 * A;
+  double s = 0;
   const int t0 = nptr[A], t1 = nptr[A + 1];
-  // eight element vectors in flight at a time, added in adjacency order
   for (int tb = t0; tb < t1; tb += 8) {
-    double v[8][3];
+    double v[8];
 #pragma unroll
     for (int u = 0; u < 8; u++)
-      if (tb + u < t1) {
-        const double *p = ge + (size_t)nadj[tb + u] * 3;
-        v[u][0] = p[0];
-        v[u][1] = p[1];
-        v[u][2] = p[2];
-      }
+      if (tb + u < t1) v[u] = ge[(size_t)nadj[tb + u] * 3 + d];
 #pragma unroll
     for (int u = 0; u < 8; u++)
-      if (tb + u < t1) {
-        s0 += v[u][0];
-        s1 += v[u][1];
-        s2 += v[u][2];
-      }
+      if (tb + u < t1) s += v[u];
   }
-  int q;
-  if ((q = eqmap[3 * A]) >= 0) g[q] = s0;
-  if ((q = eqmap[3 * A + 1]) >= 0) g[q] = s1;
-  if ((q = eqmap[3 * A + 2]) >= 0) g[q] = s2;
+  const int q = eqmap[i];
+  if (q >= 0) g[q] = s;
 }
 
 __global__ __launch_bounds__(256) void to_f32_kernel(long long n, const double *__restrict__ a,
@@ -450,9 +442,8 @@ void launch_assemble_K(Ctx &c) {
 
 void launch_gather_fint(Ctx &c, double *g) {
   // 64-thread workgroups: a B-spline mesh has few nodes (13,475 on the
-  // 32x32x8 cubic patch), and 211 workgroups spread over the CUs where 53
-  // would leave most of them idle
-  hipLaunchKernelGGL(gather_fint_kernel, dim3((c.nnode + 63) / 64), dim3(64), 0, c.stream,
+  // 32x32x8 cubic patch), and its 632 workgroups spread over the CUs
+  hipLaunchKernelGGL(gather_fint_kernel, dim3((3 * c.nnode + 63) / 64), dim3(64), 0, c.stream,
                      c.nnode, c.nn, c.d_nptr, c.d_nadj, c.d_ge, c.d_eqmap, g);
   FGPU_CHECK(hipGetLastError());
 }
