@@ -166,15 +166,12 @@ __global__ __launch_bounds__(64) void assemble_rows_kernel(
 #pragma unroll
     for (int u = 0; u < U; u++) {
       if (col[u] == 0xFFFF) continue;
-      // LDS atomic adds (ds_add_f64): the lanes of one element hit distinct
-      // columns, and one wave's LDS operations complete in program order,
-      // so each column still sums its contributions in element order
       double *c = acc + col[u];
       if (tr[u]) {
         for (int q = 0; q < 3; q++)
-          for (int r = 0; r < 3; r++) atomicAdd(c + (3 * q + r) * maxs, (double)v[u][3 * r + q]);
+          for (int r = 0; r < 3; r++) c[(3 * q + r) * maxs] += (double)v[u][3 * r + q];
       } else {
-        for (int m = 0; m < 9; m++) atomicAdd(c + m * maxs, (double)v[u][m]);
+        for (int m = 0; m < 9; m++) c[m * maxs] += (double)v[u][m];
       }
     }
   }
