@@ -321,7 +321,7 @@ void setup_batch(Ctx &c, const fgpu_mesh &m) {
 void destroy(Ctx *c) {
   if (!c) return;
   void *bufs[] = {c->d_coords, c->d_conn, c->d_eqmap, c->d_mat, c->d_ext,
-                  c->d_cpw, c->d_gauss, c->d_gauss64, c->d_Ke, c->d_Ke64, c->d_ge,
+                  c->d_cpw, c->d_gauss, c->d_gauss64, c->d_Ke, c->d_Ke64, c->d_ge, c->d_pts,
                   c->d_status, c->d_rowptr, c->d_col, c->d_bptr, c->d_bcol,
                   c->d_bcoloff, c->d_bdiag, c->d_fmask, c->d_bval, c->d_bn,
                   c->d_blkA, c->d_blkB, c->d_offAB, c->d_offBA, c->d_cptr,
@@ -557,8 +557,13 @@ int stiffness_fint_overlapped(Ctx &c, const double *d_u, double *d_g) {
   // the two status arrays are adjacent (d_status2 = d_status + 4)
   FGPU_CHECK(hipMemsetAsync(c.d_status, 0, 8 * sizeof(int), c.stream));
   FGPU_CHECK(hipEventRecord(c.ev[0], c.stream));
+  // linear tuned B-spline kernels: the stiffness kernel also writes the
+  // points' internal-force records, and f_int only finishes from them
+  const bool fused = bsp_fused_fint(c);
   if (c.shape == FGPU_HEX8)
     launch_hex8_K(c, d_u);
+  else if (fused)
+    launch_bsp_K_pts(c, d_u);
   else
     launch_bsp_K(c, d_u);
   FGPU_CHECK(hipEventRecord(c.ev[1], c.stream));
@@ -571,6 +576,8 @@ int stiffness_fint_overlapped(Ctx &c, const double *d_u, double *d_g) {
     FGPU_CHECK(hipEventRecord(c.ev[3], c.stream));
     if (c.shape == FGPU_HEX8)
       launch_hex8_fint(c, d_u);
+    else if (fused)
+      launch_bsp_fint_pts(c);
     else
       launch_bsp_fint(c, d_u);
     launch_gather_fint(c, d_g);

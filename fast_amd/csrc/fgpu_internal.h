@@ -142,6 +142,7 @@ struct Ctx {
   double *d_Ke64 = nullptr;     // the same in FP64, for element kernels that compute in FP64
                                 // (B-spline solids outside the tuned shapes); d_Ke unused then
   double *d_ge = nullptr;       // element internal forces [nelem*nn*3]
+  double *d_pts = nullptr;      // per-point internal-force records (fused B-spline path)
   int *d_status = nullptr;      // [2]: bad-Jacobian count, stress failure
   double *d_TR = nullptr;       // stress extrapolation points -> nodes [nn*nq]
   // B-spline solids outside the tuned cubic kernels (k_bspg.hip): upper block
@@ -244,6 +245,11 @@ void launch_hex8_K(Ctx &c, const double *d_unodal);
 void launch_hex8_fint(Ctx &c, const double *d_unodal);
 void launch_bsp_K(Ctx &c, const double *d_unodal);
 void launch_bsp_fint(Ctx &c, const double *d_unodal);
+// linear tuned B-spline kernels: the stiffness kernel also writes the points'
+// internal-force records (d_pts), from which launch_bsp_fint_pts finishes f_int
+bool bsp_fused_fint(const Ctx &c);
+void launch_bsp_K_pts(Ctx &c, const double *d_unodal);
+void launch_bsp_fint_pts(Ctx &c);
 void launch_hex8_body(Ctx &c, const double *d_b);   // element vectors into d_ge
 void launch_bsp_body(Ctx &c, const double *d_b);
 // stresses at the points (sig_ip [nelem*nq*6]) and, when sig_node is not

@@ -219,26 +219,52 @@ __device__ inline void point_geometry(const Geo<NB2> &g, int q, int part, const 
         fabs(P.detJ);
 }
 
-template <int NB2>
+// c sigma (Voigt order xx yy zz xy xz yz) of the linear form at a point with
+// displacement gradient L
+__device__ inline void linear_csig(const double *__restrict__ mat, int e, double c,
+                                   const double L[3][3], double *sg) {
+  const double E = mat[2 * (size_t)e], nu = mat[2 * (size_t)e + 1];
+  const double lam = E * nu / ((1.0 + nu) * (1.0 - 2.0 * nu)), mu = 0.5 * E / (1.0 + nu);
+  const double tr = L[0][0] + L[1][1] + L[2][2];
+  sg[0] = c * (lam * tr + 2.0 * mu * L[0][0]);
+  sg[1] = c * (lam * tr + 2.0 * mu * L[1][1]);
+  sg[2] = c * (lam * tr + 2.0 * mu * L[2][2]);
+  sg[3] = c * mu * (L[0][1] + L[1][0]);
+  sg[4] = c * mu * (L[0][2] + L[2][0]);
+  sg[5] = c * mu * (L[1][2] + L[2][1]);
+}
+
+template <int NB2, bool FP>
 struct KShared {
   Geo<NB2> g;
   float G[3][NQP][Bsp<NB2>::NBAS];  // dN_a/dx_i at point q
   float c[NQP];                      // w detJ
+  double U[FP ? Bsp<NB2>::NBAS : 1][3];  // FP: the element's nodal displacements
 };
 
-template <int NB2>
+// Per-point record of the internal force, [element][point][PREC]: J^-1 (9),
+// 1 / W and dW/d(r,s,t) (4), c sigma (Voigt, 6) -- what the internal-force
+// kernel's second phase reads (FShared's Ji, rs, sig)
+constexpr int PREC = 19;
+
+// FP (linear form): the geometry phase also forms grad u = sum_a u_a (x) G_a
+// from the same FP64 gradients, in the order of bsp_fint_kernel's first
+// phase, and writes each point's record to pts; bsp_fint_pts_kernel then
+// finishes the internal force without recomputing the points' geometry.
+template <int NB2, bool FP>
 __global__ __launch_bounds__(256, 3) void bsp_K_kernel(
     int nelem, const double *__restrict__ coords, const int *__restrict__ conn,
     const double *__restrict__ cpw, const double *__restrict__ ext,
     const double *__restrict__ mat, const double *__restrict__ gp, float *__restrict__ Ke,
-    int *__restrict__ status) {
+    int *__restrict__ status, const double *__restrict__ unodal, double *__restrict__ pts) {
   constexpr int NBAS = Bsp<NB2>::NBAS, NSYM = Bsp<NB2>::NSYM, NPART = Bsp<NB2>::NPART;
   constexpr int NT = Bsp<NB2>::NT, NPAIR = NT * (NT + 1) / 2;
   extern __shared__ __align__(16) unsigned char smem[];
-  KShared<NB2> &S = *reinterpret_cast<KShared<NB2> *>(smem);
+  KShared<NB2, FP> &S = *reinterpret_cast<KShared<NB2, FP> *>(smem);
   const int e = blockIdx.x;
   const int tid = threadIdx.x;
   geo_load(S.g, e, coords, conn, cpw, ext, gp);
+  if constexpr (FP) stage_u<NB2>(S.U, conn + (size_t)e * NBAS, unodal);
   __syncthreads();
   {  // phase 1: G and c
     const int q = tid / 4, part = tid % 4;
@@ -248,13 +274,32 @@ __global__ __launch_bounds__(256, 3) void bsp_K_kernel(
       S.c[q] = (float)P.c;
       count_jacobian(status, P.detJ);
     }
+    double L[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
     #pragma unroll 2
     for (int k = 0; k < NPART; k++) {
-      double dR[3];
+      double dR[3], G[3];
       rational_dR(S.g, q, part * NPART + k, P, dR);
       for (int i = 0; i < 3; i++)
-        S.G[i][q][part * NPART + k] =
-            (float)(P.Ji[i][0] * dR[0] + P.Ji[i][1] * dR[1] + P.Ji[i][2] * dR[2]);
+        G[i] = P.Ji[i][0] * dR[0] + P.Ji[i][1] * dR[1] + P.Ji[i][2] * dR[2];
+      for (int i = 0; i < 3; i++) S.G[i][q][part * NPART + k] = (float)G[i];
+      if constexpr (FP) {
+        const double *ua = S.U[part * NPART + k];
+        for (int i = 0; i < 3; i++)
+          for (int j = 0; j < 3; j++) L[i][j] += ua[i] * G[j];
+      }
+    }
+    if constexpr (FP) {
+      for (int m = 1; m < 4; m <<= 1)
+        for (int i = 0; i < 3; i++)
+          for (int j = 0; j < 3; j++) L[i][j] += __shfl_xor(L[i][j], m);
+      if (part == 0) {
+        double *r = pts + ((size_t)e * NQP + q) * PREC;
+        for (int i = 0; i < 3; i++)
+          for (int j = 0; j < 3; j++) r[3 * i + j] = P.Ji[i][j];
+        r[9] = P.inv_s;
+        for (int k = 0; k < 3; k++) r[10 + k] = P.dW[k];
+        linear_csig(mat, e, P.c, L, r + 13);
+      }
     }
   }
   __syncthreads();
@@ -527,6 +572,53 @@ struct FShared {
   double sig[NQP][FORM == FGPU_LINEAR ? 6 : 9];
 };
 
+// Second phase of the internal force, thread (a, quarter): g_a = sum over
+// every fourth point of (c sigma or c P) grad N_a, grad N_a recomputed from
+// the tensor tables, the point's J^-1 and rational sums; the quarters reduce
+// with shuffles.
+template <int NB2, int FORM>
+__device__ inline void fint_points(const FShared<NB2, FORM> &S, int e, double *__restrict__ ge) {
+  constexpr int NBAS = Bsp<NB2>::NBAS;
+  const int tid = threadIdx.x;
+  if (tid < 4 * NBAS) {  // whole waves: 4 NBAS is 256 or 192
+    const int a = tid / 4, quarter = tid % 4;
+    double g0 = 0.0, g1 = 0.0, g2 = 0.0;
+    #pragma unroll
+    for (int q = quarter; q < NQP; q += 4) {  // the quarters' points side by side in LDS
+      double n, d[3];
+      tensor(S.g, q, a, n, d);
+      const double is = S.rs[q][0];
+      const double R = n * is;
+      double dR[3], G[3];
+      for (int i = 0; i < 3; i++) dR[i] = (d[i] - R * S.rs[q][1 + i]) * is;
+      for (int i = 0; i < 3; i++)
+        G[i] = S.Ji[q][3 * i] * dR[0] + S.Ji[q][3 * i + 1] * dR[1] + S.Ji[q][3 * i + 2] * dR[2];
+      const double *s = S.sig[q];
+      if constexpr (FORM == FGPU_LINEAR) {
+        g0 += s[0] * G[0] + s[3] * G[1] + s[4] * G[2];
+        g1 += s[3] * G[0] + s[1] * G[1] + s[5] * G[2];
+        g2 += s[4] * G[0] + s[5] * G[1] + s[2] * G[2];
+      } else {
+        g0 += s[0] * G[0] + s[1] * G[1] + s[2] * G[2];
+        g1 += s[3] * G[0] + s[4] * G[1] + s[5] * G[2];
+        g2 += s[6] * G[0] + s[7] * G[1] + s[8] * G[2];
+      }
+    }
+    for (int m = 1; m < 4; m <<= 1) {
+      g0 += __shfl_xor(g0, m);
+      g1 += __shfl_xor(g1, m);
+      g2 += __shfl_xor(g2, m);
+    }
+    if (quarter == 0) {
+      double *o = ge + ((size_t)e * NBAS + a) * 3;
+      o[0] = g0;
+      o[1] = g1;
+      o[2] = g2;
+    }
+  }
+}
+
+
 template <int FORM, int NB2>
 __global__ __launch_bounds__(256) void bsp_fint_kernel(
     int nelem, const double *__restrict__ coords, const int *__restrict__ conn,
@@ -569,16 +661,7 @@ __global__ __launch_bounds__(256) void bsp_fint_kernel(
       const double c = P.c;
       double *sg = S.sig[q];
       if constexpr (FORM == FGPU_LINEAR) {
-        const double E = mat[2 * (size_t)e], nu = mat[2 * (size_t)e + 1];
-        const double lam = E * nu / ((1.0 + nu) * (1.0 - 2.0 * nu)), mu = 0.5 * E / (1.0 + nu);
-        const double tr = L[0][0] + L[1][1] + L[2][2];
-        // symmetric: Voigt order xx yy zz xy xz yz
-        sg[0] = c * (lam * tr + 2.0 * mu * L[0][0]);
-        sg[1] = c * (lam * tr + 2.0 * mu * L[1][1]);
-        sg[2] = c * (lam * tr + 2.0 * mu * L[2][2]);
-        sg[3] = c * mu * (L[0][1] + L[1][0]);
-        sg[4] = c * mu * (L[0][2] + L[2][0]);
-        sg[5] = c * mu * (L[1][2] + L[2][1]);
+        linear_csig(mat, e, c, L, sg);
       } else {
         double F[3][3], C[3][3], Sp[3][3];
         for (int i = 0; i < 3; i++)
@@ -611,42 +694,34 @@ __global__ __launch_bounds__(256) void bsp_fint_kernel(
     }
   }
   __syncthreads();
-  if (tid < 4 * NBAS) {  // whole waves: 4 NBAS is 256 or 192
-    const int a = tid / 4, quarter = tid % 4;
-    double g0 = 0.0, g1 = 0.0, g2 = 0.0;
-    #pragma unroll
-    for (int q = quarter; q < NQP; q += 4) {  // the quarters' points side by side in LDS
-      double n, d[3];
-      tensor(S.g, q, a, n, d);
-      const double is = S.rs[q][0];
-      const double R = n * is;
-      double dR[3], G[3];
-      for (int i = 0; i < 3; i++) dR[i] = (d[i] - R * S.rs[q][1 + i]) * is;
-      for (int i = 0; i < 3; i++)
-        G[i] = S.Ji[q][3 * i] * dR[0] + S.Ji[q][3 * i + 1] * dR[1] + S.Ji[q][3 * i + 2] * dR[2];
-      const double *s = S.sig[q];
-      if constexpr (FORM == FGPU_LINEAR) {
-        g0 += s[0] * G[0] + s[3] * G[1] + s[4] * G[2];
-        g1 += s[3] * G[0] + s[1] * G[1] + s[5] * G[2];
-        g2 += s[4] * G[0] + s[5] * G[1] + s[2] * G[2];
-      } else {
-        g0 += s[0] * G[0] + s[1] * G[1] + s[2] * G[2];
-        g1 += s[3] * G[0] + s[4] * G[1] + s[5] * G[2];
-        g2 += s[6] * G[0] + s[7] * G[1] + s[8] * G[2];
-      }
-    }
-    for (int m = 1; m < 4; m <<= 1) {
-      g0 += __shfl_xor(g0, m);
-      g1 += __shfl_xor(g1, m);
-      g2 += __shfl_xor(g2, m);
-    }
-    if (quarter == 0) {
-      double *o = ge + ((size_t)e * NBAS + a) * 3;
-      o[0] = g0;
-      o[1] = g1;
-      o[2] = g2;
-    }
+  fint_points<NB2, FORM>(S, e, ge);
+}
+
+// The internal force of the linear form from the per-point records the
+// stiffness kernel wrote (bsp_K_kernel<NB2, true>): the second phase of
+// bsp_fint_kernel on the same data, so the same result.
+template <int NB2>
+__global__ __launch_bounds__(256) void bsp_fint_pts_kernel(
+    const double *__restrict__ coords, const int *__restrict__ conn,
+    const double *__restrict__ cpw, const double *__restrict__ ext,
+    const double *__restrict__ gp, const double *__restrict__ pts, double *__restrict__ ge) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  FShared<NB2, FGPU_LINEAR> &S = *reinterpret_cast<FShared<NB2, FGPU_LINEAR> *>(smem);
+  const int e = blockIdx.x;
+  geo_load(S.g, e, coords, conn, cpw, ext, gp);
+  const double *r0 = pts + (size_t)e * NQP * PREC;
+  for (int i = threadIdx.x; i < NQP * PREC; i += blockDim.x) {
+    const int q = i / PREC, f = i - PREC * q;
+    const double v = r0[i];
+    if (f < 9)
+      S.Ji[q][f] = v;
+    else if (f < 13)
+      S.rs[q][f - 9] = v;
+    else
+      S.sig[q][f - 13] = v;
   }
+  __syncthreads();
+  fint_points<NB2, FGPU_LINEAR>(S, e, ge);
 }
 
 // dynamic LDS above 64 KB must be allowed per kernel (cheap; set per launch)
@@ -851,11 +926,29 @@ void launch_Kt(Ctx &c, const double *u) {
 
 template <int NB2>
 void launch_K_lin(Ctx &c) {
-  const size_t lds = sizeof(KShared<NB2>);
-  set_lds(bsp_K_kernel<NB2>, lds);
-  hipLaunchKernelGGL(bsp_K_kernel<NB2>, dim3(c.nelem), dim3(256), lds, c.stream, c.nelem,
-                     c.d_coords, c.d_conn, c.d_cpw, c.d_ext, c.d_mat, c.d_gauss64, c.d_Ke,
-                     c.d_status);
+  const size_t lds = sizeof(KShared<NB2, false>);
+  set_lds(bsp_K_kernel<NB2, false>, lds);
+  hipLaunchKernelGGL((bsp_K_kernel<NB2, false>), dim3(c.nelem), dim3(256), lds, c.stream,
+                     c.nelem, c.d_coords, c.d_conn, c.d_cpw, c.d_ext, c.d_mat, c.d_gauss64,
+                     c.d_Ke, c.d_status, nullptr, nullptr);
+  FGPU_CHECK(hipGetLastError());
+}
+
+template <int NB2>
+void launch_K_lin_pts(Ctx &c, const double *u) {
+  const size_t lds = sizeof(KShared<NB2, true>);
+  set_lds(bsp_K_kernel<NB2, true>, lds);
+  hipLaunchKernelGGL((bsp_K_kernel<NB2, true>), dim3(c.nelem), dim3(256), lds, c.stream,
+                     c.nelem, c.d_coords, c.d_conn, c.d_cpw, c.d_ext, c.d_mat, c.d_gauss64,
+                     c.d_Ke, c.d_status, u, c.d_pts);
+  FGPU_CHECK(hipGetLastError());
+}
+
+template <int NB2>
+void launch_fint_pts(Ctx &c) {
+  const size_t lds = sizeof(FShared<NB2, FGPU_LINEAR>);
+  hipLaunchKernelGGL(bsp_fint_pts_kernel<NB2>, dim3(c.nelem), dim3(256), lds, c.stream,
+                     c.d_coords, c.d_conn, c.d_cpw, c.d_ext, c.d_gauss64, c.d_pts, c.d_ge);
   FGPU_CHECK(hipGetLastError());
 }
 
@@ -945,6 +1038,26 @@ void launch_bsp_K(Ctx &c, const double *u) {
     launch_K_nb<3>(c, u);
   else
     launch_K_nb<4>(c, u);
+}
+
+bool bsp_fused_fint(const Ctx &c) {
+  return c.shape == FGPU_BSP && !bsp_generic(c) && c.form == FGPU_LINEAR;
+}
+
+void launch_bsp_K_pts(Ctx &c, const double *u) {
+  if (!bsp_fused_fint(c)) throw Error("fused internal force: linear tuned B-spline kernels only");
+  if (!c.d_pts) FGPU_CHECK(hipMalloc(&c.d_pts, sizeof(double) * (size_t)c.nelem * NQP * PREC));
+  if (c.deg[2] == 2)
+    launch_K_lin_pts<3>(c, u);
+  else
+    launch_K_lin_pts<4>(c, u);
+}
+
+void launch_bsp_fint_pts(Ctx &c) {
+  if (c.deg[2] == 2)
+    launch_fint_pts<3>(c);
+  else
+    launch_fint_pts<4>(c);
 }
 
 void launch_bsp_fint(Ctx &c, const double *u) {

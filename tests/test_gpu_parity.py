@@ -311,17 +311,21 @@ def test_linear_static_driver(fg, oracle_mod):
     assert nrel(m.scatter(u), d["u"]) <= 1e-8
 
 
-@pytest.mark.parametrize("kind", ["hex8_nh", "bsp_svk", "bsp332_linear"])
+@pytest.mark.parametrize("kind", ["hex8_nh", "bsp_svk", "bsp332_linear", "bsp_linear"])
 def test_stiffness_beside_internal_force_identical_bits(fg, kind):
-    """fgpu_assemble_K_fint (f_int on the second stream beside K) gives the
-    matrix and the force of fgpu_assemble_K + fgpu_int_force, bit for bit,
-    and reports a failed stress update of either side."""
+    """fgpu_assemble_K_fint (f_int on the second stream beside K; for the
+    linear tuned B-spline kernels finished from the per-point records the
+    stiffness kernel writes) gives the matrix and the force of
+    fgpu_assemble_K + fgpu_int_force, bit for bit, and reports a failed
+    stress update of either side."""
     if kind == "hex8_nh":
         m = M.hex8_block(5, 4, 3, form="tl_neohooke", perturb=0.2, seed=4,
                          mat=np.tile((1.0, 1.5), (60, 1)))
     elif kind == "bsp_svk":
         m = M.bsp_block(ne=(3, 3, 2), p=3, perturb=0.05, seed=2, form="tl_svk",
                         mat=np.tile((2.6, 0.3), (18, 1)))
+    elif kind == "bsp_linear":
+        m = M.bsp_block(ne=(3, 3, 2), p=3, perturb=0.05, seed=5)
     else:
         m = M.bsp_block(ne=(3, 2, 2), p=(3, 3, 2), perturb=0.05, seed=3)
     X = m.coords
