@@ -682,7 +682,10 @@ int fgpu_create(fgpu_ctx **out, const fgpu_mesh *m) {
     else
       c->d_Ke = dalloc<float>(ntile * c->kel.T * (size_t)nsym * 9);
     c->d_ge = dalloc<double>((size_t)c->nelem * c->nn * 3);
-    if (c->shape == FGPU_BSP) bsp_prepare_tables(*c, m->ext, m->cpw);
+    if (c->shape == FGPU_BSP) {
+      bsp_prepare_tables(*c, m->ext, m->cpw);
+      bsp_alloc_pts(*c);
+    }
     c->d_u = dalloc<double>((size_t)c->nnode * 3);
     size_t n = (size_t)std::max(3 * c->nnode, std::max(c->neq, 1));
     c->d_bn = dalloc<double>(n);
@@ -755,6 +758,14 @@ int fgpu_solve_batch(fgpu_ctx *ctx, const double *b, double *x, int maxit, doubl
 }
 
 long long fgpu_nnz(fgpu_ctx *ctx) { return ctx ? reinterpret_cast<Ctx *>(ctx)->nnz : 0; }
+
+int fgpu_block_counts(fgpu_ctx *ctx, long long *nbf, long long *nblk) {
+  return guarded(ctx, [&](Ctx &c) {
+    if (nbf) *nbf = c.nbf;
+    if (nblk) *nblk = c.nblk;
+    return 1;
+  });
+}
 
 int fgpu_assemble_K_dev(fgpu_ctx *ctx, const double *d_u) {
   return guarded(ctx, [&](Ctx &c) { return stiffness_dev(c, d_u); });
@@ -1155,6 +1166,14 @@ int fgpu_set_launch(fgpu_ctx *ctx, int hex8_te, int asm_mode) {
     c.asm_mode = asm_mode;
     c.tuned = true;  // an explicit choice replaces the autotuner
     c.tune_report = "set by the caller\n";
+    return 1;
+  });
+}
+
+int fgpu_set_overlap(fgpu_ctx *ctx, int mode) {
+  return guarded(ctx, [&](Ctx &c) {
+    if (mode < -1 || mode > 1) throw Error("overlap mode: -1, 0 or 1");
+    c.overlap = mode;
     return 1;
   });
 }

@@ -271,7 +271,10 @@ int fgpu_newton(fgpu_ctx *ctx, const double *q, const fgpu_nr_params *prm, doubl
         }
       }
       st.err_start = err;
-      bool conv = err < prm->tol;
+      // the reference builds K, solves and updates at least once per step
+      // (ctrl/ctrlnr.cpp:153-227); only an accepted prediction that already
+      // meets the tolerance ends the step without an iteration
+      bool conv = st.predicted == 1 && err < prm->tol;
       int it = 0;
       for (it = 1; it <= prm->maxit && !conv; it++) {
         if (!stiffness_dev(c, unod)) return 0;

@@ -248,6 +248,7 @@ void launch_bsp_fint(Ctx &c, const double *d_unodal);
 // linear tuned B-spline kernels: the stiffness kernel also writes the points'
 // internal-force records (d_pts), from which launch_bsp_fint_pts finishes f_int
 bool bsp_fused_fint(const Ctx &c);
+void bsp_alloc_pts(Ctx &c);  // the records, up to 4 GB, at context creation
 void launch_bsp_K_pts(Ctx &c, const double *d_unodal);
 void launch_bsp_fint_pts(Ctx &c);
 void launch_hex8_body(Ctx &c, const double *d_b);   // element vectors into d_ge

@@ -1040,13 +1040,25 @@ void launch_bsp_K(Ctx &c, const double *u) {
     launch_K_nb<4>(c, u);
 }
 
+// the point records are allocated with the context (bsp_alloc_pts); without
+// them (over the budget, or the allocation failed) f_int takes the unfused
+// kernels, which need no extra memory
 bool bsp_fused_fint(const Ctx &c) {
-  return c.shape == FGPU_BSP && !bsp_generic(c) && c.form == FGPU_LINEAR;
+  return c.shape == FGPU_BSP && !bsp_generic(c) && c.form == FGPU_LINEAR && c.d_pts;
+}
+
+void bsp_alloc_pts(Ctx &c) {
+  if (c.shape != FGPU_BSP || bsp_generic(c) || c.form != FGPU_LINEAR || c.d_pts) return;
+  const size_t bytes = sizeof(double) * (size_t)c.nelem * NQP * PREC;  // 9.7 KB per element
+  if ((double)bytes > 4.0 * (1 << 30)) return;
+  if (hipMalloc(&c.d_pts, bytes) != hipSuccess) {
+    c.d_pts = nullptr;
+    (void)hipGetLastError();  // clear the failed allocation's error
+  }
 }
 
 void launch_bsp_K_pts(Ctx &c, const double *u) {
   if (!bsp_fused_fint(c)) throw Error("fused internal force: linear tuned B-spline kernels only");
-  if (!c.d_pts) FGPU_CHECK(hipMalloc(&c.d_pts, sizeof(double) * (size_t)c.nelem * NQP * PREC));
   if (c.deg[2] == 2)
     launch_K_lin_pts<3>(c, u);
   else

@@ -65,6 +65,7 @@ EXPORTS = {
     "fgpu_last_error": (C.c_char_p, [C.c_void_p]),
     "fgpu_neq": (C.c_int, [C.c_void_p]),
     "fgpu_nnz": (C.c_longlong, [C.c_void_p]),
+    "fgpu_block_counts": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p]),
     "fgpu_assemble_K": (C.c_int, [C.c_void_p, C.c_void_p]),
     "fgpu_assemble_K_dev": (C.c_int, [C.c_void_p, C.c_void_p]),
     "fgpu_int_force": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p]),
@@ -101,6 +102,7 @@ EXPORTS = {
     "fgpu_stresses": (C.c_int, [C.c_void_p] + [C.c_void_p] * 4),
     "fgpu_stresses_dev": (C.c_int, [C.c_void_p] + [C.c_void_p] * 4),
     "fgpu_set_launch": (C.c_int, [C.c_void_p, C.c_int, C.c_int]),
+    "fgpu_set_overlap": (C.c_int, [C.c_void_p, C.c_int]),
     "fgpu_tuning_report": (C.c_char_p, [C.c_void_p]),
     "fgpu_last_time_ms": (C.c_double, [C.c_void_p, C.c_int]),
     "fgpu_device_buffer": (C.c_void_p, [C.c_void_p, C.c_int]),
@@ -393,6 +395,18 @@ class GpuModel:
     def set_launch(self, hex8_te=0, asm_mode=-1):
         """Fix the launch configuration (replaces the autotuner)."""
         self._check(lib().fgpu_set_launch(self.h, hex8_te, asm_mode), "fgpu_set_launch")
+
+    def set_overlap(self, mode):
+        """Order of stiffness_and_int_force: 0 serial, 1 f_int beside the
+        assembly (fused point records where the kernels have them), -1 timed
+        on the first call."""
+        self._check(lib().fgpu_set_overlap(self.h, mode), "fgpu_set_overlap")
+
+    def block_counts(self):
+        """(stored node blocks, upper node blocks) of the global matrix."""
+        a, b = C.c_longlong(0), C.c_longlong(0)
+        self._check(lib().fgpu_block_counts(self.h, C.byref(a), C.byref(b)), "fgpu_block_counts")
+        return a.value, b.value
 
     def tuning_report(self):
         return lib().fgpu_tuning_report(self.h).decode()

@@ -46,9 +46,13 @@ enum { FGPU_LINEAR = 0, FGPU_TL_SVK = 1, FGPU_TL_NEOHOOKE = 2 };
 typedef struct {
   int nnode, nelem;
   int shape;            /* FGPU_HEX8 | FGPU_BSP */
-  int nn;               /* nodes per element: 8, or (p+1)^3 */
-  int deg[3];           /* FGPU_BSP: degree per direction: (3,3,3) or (3,3,2) */
-  int nip[3];           /* Gauss points per direction (intpoint.cpp:579-617) */
+  int nn;               /* nodes per element: 8, or prod(p_d+1) <= 512 */
+  int deg[3];           /* FGPU_BSP: degree per direction, 1..7; (3,3,3) and
+                           (3,3,2) on 4x4x4 Gauss run the tuned FP32 MFMA
+                           kernels, every other shape FP64 kernels */
+  int nip[3];           /* Gauss points per direction (intpoint.cpp:579-617):
+                           hex8 isotropic 1..3; FGPU_BSP 1..16 each, at most
+                           512 points per element */
   int form;             /* FGPU_LINEAR | FGPU_TL_SVK | FGPU_TL_NEOHOOKE */
   const double *coords; /* [nnode*3] */
   const int *conn;      /* [nelem*nn] 0-based, reference local node order */
@@ -56,7 +60,9 @@ typedef struct {
   const double *ext;    /* FGPU_BSP: per element, per direction d the
                            (p_d+1) x (p_d+1) extraction matrix, row-major,
                            packed: [nelem * sum_d (p_d+1)^2] */
-  const double *cpw;    /* FGPU_BSP: [nnode] weights (NULL = all 1) */
+  const double *cpw;    /* FGPU_BSP: [nnode] control-point weights
+                           (cNode::GetW); NULL = all weights 1, i.e. a
+                           polynomial (non-rational) B-spline solid */
   const double *mat;    /* [nelem*2] */
   /* Batch of independent meshes (0 / NULL for one mesh): mesh k owns nodes
    * [batch_nodeptr[k], batch_nodeptr[k+1]), no element couples two meshes,
@@ -73,6 +79,10 @@ void fgpu_destroy(fgpu_ctx *ctx);
 const char *fgpu_last_error(fgpu_ctx *ctx);
 int fgpu_neq(fgpu_ctx *ctx);
 long long fgpu_nnz(fgpu_ctx *ctx);
+/* Stored 3x3 node blocks of the global matrix (every pair of nodes sharing
+ * an element, both orders) and upper node blocks (A <= B) the assembly sums:
+ * the sizes its algorithmic byte count is made of. */
+int fgpu_block_counts(fgpu_ctx *ctx, long long *nbf, long long *nblk);
 
 /* Global stiffness: element K_e (FP32) + deterministic assembly into the FP64
  * CSR.  Replaces cControl::StiffnessMatrix (ctrl/ctrl.cpp:886-975) and the
@@ -349,6 +359,14 @@ int fgpu_stresses_dev(fgpu_ctx *ctx, const double *u_nodal_dev, double *sig_ip_d
  * (hex8_te 0: the largest; asm_mode -1: the default); 0 if it is not valid
  * for the mesh.  No counterpart in the reference (a CPU loop). */
 int fgpu_set_launch(fgpu_ctx *ctx, int hex8_te, int asm_mode);
+/* Order of fgpu_assemble_K_fint[_dev]: 0 = the element stiffness and the
+ * assembly, then the internal force (element vectors + gather), all on the
+ * context stream; 1 = the internal force on the second stream beside the
+ * assembly, and for the linear tuned B-spline kernels (degrees (3,3,3) and
+ * (3,3,2), 4x4x4 Gauss) finished from the per-point records the stiffness
+ * kernel writes; -1 (default) = the first call times both and keeps the
+ * faster (FGPU_AUTOTUNE=0: 0).  Both orders give the same bits. */
+int fgpu_set_overlap(fgpu_ctx *ctx, int mode);
 const char *fgpu_tuning_report(fgpu_ctx *ctx);
 
 /* Time (ms, HIP events on the context stream) of the last call of each

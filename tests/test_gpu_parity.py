@@ -311,13 +311,15 @@ def test_linear_static_driver(fg, oracle_mod):
     assert nrel(m.scatter(u), d["u"]) <= 1e-8
 
 
+@pytest.mark.parametrize("overlap", [0, 1])
 @pytest.mark.parametrize("kind", ["hex8_nh", "bsp_svk", "bsp332_linear", "bsp_linear"])
-def test_stiffness_beside_internal_force_identical_bits(fg, kind):
-    """fgpu_assemble_K_fint (f_int on the second stream beside K; for the
-    linear tuned B-spline kernels finished from the per-point records the
-    stiffness kernel writes) gives the matrix and the force of
-    fgpu_assemble_K + fgpu_int_force, bit for bit, and reports a failed
-    stress update of either side."""
+def test_stiffness_beside_internal_force_identical_bits(fg, kind, overlap):
+    """fgpu_assemble_K_fint in both orders, forced (0: K then f_int on one
+    stream; 1: f_int on the second stream beside the assembly, for the linear
+    tuned B-spline kernels of degree (3,3,3) and (3,3,2) finished from the
+    per-point records the stiffness kernel writes) gives the matrix and the
+    force of fgpu_assemble_K + fgpu_int_force, bit for bit, and reports a
+    failed stress update of either side."""
     if kind == "hex8_nh":
         m = M.hex8_block(5, 4, 3, form="tl_neohooke", perturb=0.2, seed=4,
                          mat=np.tile((1.0, 1.5), (60, 1)))
@@ -332,10 +334,12 @@ def test_stiffness_beside_internal_force_identical_bits(fg, kind):
     u = np.stack([0.02 * X[:, 0] * X[:, 2], 0.01 * X[:, 1] ** 2, -0.03 * X[:, 0] ** 2], 1)
     u[m.fixed] = 0
     g = fg.GpuModel(m)
+    g.set_overlap(overlap)
     g.stiffness(u)
     K1 = g.csr()[2].copy()
     f1 = g.int_force(u)
     f2 = g.stiffness_and_int_force(u)
+    assert "f_int beside" not in g.tuning_report()  # the forced order, untimed
     K2 = g.csr()[2]
     assert np.array_equal(K1, K2) and np.array_equal(f1, f2)
     if kind == "hex8_nh":

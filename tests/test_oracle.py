@@ -418,10 +418,20 @@ def test_stress_of_homogeneous_deformation(oracle_mod, form, mat):
     reason="the reference build (oracle/_ref) is not present")
 def test_reference_cpu_baseline_runs():
     """bench.py's CPU baseline runs the reference's own assembly loops
-    (ref_probe's timing mode) on a small patch and reports their rate."""
+    (ref_probe's timing mode) on a small sample of each step workload, the
+    oracle on the same sample beside it, and reports their rates; the
+    batch's sample is its first cubes, with their own edges and faces."""
     import bench
-    res = bench.reference_baseline({"workload": "iga_cubic_bspline_32x32x8"}, ne=(2, 2, 2))
+    res = bench.reference_baseline(M.bsp_block(ne=(2, 2, 2), p=3, perturb=0.05, seed=0),
+                                   "a 2x2x2 patch")
     assert res["kind"] == "reference" and res["cores"] == 1
-    assert res["value"] > 0 and "8 elements" in res["sample"]
+    assert res["value"] > 0 and "2x2x2 patch" in res["sample"]
     assert res["stiffness_s"] > 0 and res["int_force_s"] >= 0
-    assert bench.reference_baseline({"workload": "hex8_neohooke_64x64x64"}) is None
+    _, prm = M.cube_batch(8, nmin=4, nmax=6, seed=3)
+    sample, what = bench.reference_sample("batch", prm, 200)
+    assert sample.nelem >= 200 and sample.nelem == sum(int(n) ** 3 for n in
+                                                       prm["n"][:len(sample.batch_nodeptr) - 1])
+    res = bench.cpu_baselines("batch", prm, target=200)
+    assert res["kind"] == "reference" and res["port"]["kind"] == "port"
+    assert res["host"]["nproc"] >= 1
+    assert bench.cpu_baselines("hex8", None) is None  # neo-Hookean: no reference model
