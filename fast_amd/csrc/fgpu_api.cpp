@@ -641,6 +641,7 @@ int fgpu_create(fgpu_ctx **out, const fgpu_mesh *m) {
     }
     c->neq = neq;
     FGPU_CHECK(hipGetDevice(&c->device));
+    c->scratch = scratch_setup(c->device);
     FGPU_CHECK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
     FGPU_CHECK(hipStreamCreateWithFlags(&c->stream2, hipStreamNonBlocking));
     for (auto &e : c->ev) FGPU_CHECK(hipEventCreate(&e));
@@ -805,6 +806,36 @@ int fgpu_assemble_K_fint(fgpu_ctx *ctx, const double *u, double *g) {
 
 int fgpu_int_force_dev(fgpu_ctx *ctx, const double *d_u, double *d_g) {
   return guarded(ctx, [&](Ctx &c) { return int_force_dev(c, d_u, d_g); });
+}
+
+int fgpu_int_force_dev_noalloc(fgpu_ctx *ctx, const double *d_u, double *d_g) {
+  return guarded(ctx, [&](Ctx &c) {
+    if (!d_u || !d_g) throw Error("null argument");
+    if (c.shape == FGPU_BSP && bsp_generic(c))
+      throw Error("fgpu_int_force_dev_noalloc: B-spline solids of degree (3,3,3) or (3,3,2) on "
+                  "4x4x4 Gauss points only (the other shapes allocate their point scratch)");
+    const size_t need = sizeof(double) * (size_t)c.nelem * c.nn * 3;
+    if (need > kScratchBytes) {
+      char msg[200];
+      std::snprintf(msg, sizeof msg,
+                    "fgpu_int_force_dev_noalloc: the element vectors need %zu bytes, above the "
+                    "fixed scratch of %zu bytes (at most %zu elements of %d nodes)",
+                    need, kScratchBytes, kScratchBytes / (sizeof(double) * c.nn * 3), c.nn);
+      throw Error(msg);
+    }
+    if (!c.scratch) throw Error("fgpu_int_force_dev_noalloc: no device scratch");
+    // the element kernels write every entry of their element vectors before
+    // the gather reads them, so nothing in the scratch needs clearing
+    struct UseScratch {
+      Ctx &c;
+      double *keep;
+      explicit UseScratch(Ctx &c_) : c(c_), keep(c_.d_ge) {
+        c.d_ge = static_cast<double *>(c.scratch);
+      }
+      ~UseScratch() { c.d_ge = keep; }
+    } use(c);
+    return int_force_dev(c, d_u, d_g);
+  });
 }
 
 int fgpu_int_force(fgpu_ctx *ctx, const double *u, double *g) {

@@ -235,10 +235,19 @@ struct Ctx {
   bool factored = false;       // d_band holds the factors of the current matrix
   int neg_pivots = 0;
 
+  // ---- the device's fixed scratch (k_scratch.hip), looked up at creation
+  void *scratch = nullptr;
+
   // ---- timing
   hipEvent_t ev[8] = {};
   double last_ms[4] = {0, 0, 0, 0};
 };
+
+// Fixed device scratch of the allocation-free internal force (k_scratch.hip):
+// a file-scope device array of kScratchBytes per device.  scratch_setup
+// returns its address on `device` (the current device), looking it up once.
+constexpr size_t kScratchBytes = size_t(64) << 20;
+void *scratch_setup(int device);
 
 // kernel launchers (defined in the .hip files)
 void launch_hex8_K(Ctx &c, const double *d_unodal);

@@ -70,6 +70,7 @@ EXPORTS = {
     "fgpu_assemble_K_dev": (C.c_int, [C.c_void_p, C.c_void_p]),
     "fgpu_int_force": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p]),
     "fgpu_int_force_dev": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p]),
+    "fgpu_int_force_dev_noalloc": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p]),
     "fgpu_add_diag": (C.c_int, [C.c_void_p, C.c_int, C.c_void_p, C.c_void_p]),
     "fgpu_solve": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int,
                              C.c_double, C.c_void_p, C.c_void_p]),

@@ -98,6 +98,18 @@ int fgpu_assemble_K_dev(fgpu_ctx *ctx, const double *u_nodal_dev);
 int fgpu_int_force(fgpu_ctx *ctx, const double *u_nodal, double *g);
 int fgpu_int_force_dev(fgpu_ctx *ctx, const double *u_nodal_dev, double *g_dev);
 
+/* The internal force of fgpu_int_force_dev without any allocation: the
+ * element vectors go to a fixed device array of 64 MiB that the library
+ * holds on every device it runs on until the process ends, instead of the
+ * context's buffer; same kernels, same bits.  No counterpart in the
+ * reference: for callers that cannot allocate device memory (serving loops,
+ * harnesses that forbid hipMalloc).  Limits: nelem * nn * 3 * 8 bytes <= 64
+ * MiB (349,525 hex8 elements, 43,690 cubic B-spline elements); B-spline
+ * solids only of degree (3,3,3) or (3,3,2) on 4x4x4 Gauss points; larger
+ * meshes and other shapes return 0 with an error.  All contexts on one
+ * device share the array: calls that overlap in time on one GPU race. */
+int fgpu_int_force_dev_noalloc(fgpu_ctx *ctx, const double *u_nodal_dev, double *g_dev);
+
 /* Tangent and internal force of the same state in one call: the element
  * stiffness and the assembly run on the context stream while the internal
  * force (element vectors + gather) runs beside them on a second stream of
