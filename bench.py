@@ -405,12 +405,16 @@ def run_step(name, steps, warmup, serial=False, with_accuracy=True):
     dom = "assembly" if ta >= tk else "element_K"
     roof = dict(rooflines[dom])
     roof["other"] = rooflines["element_K" if dom == "assembly" else "assembly"]
-    overlap = False
+    overlap = None
     if not serial:
-        mm = re.search(r"K then f_int: ([0-9.]+) ms; f_int beside the assembly: ([0-9.]+) ms",
+        mm = re.search(r"K then f_int: ([0-9.]+) ms; f_int beside the assembly: ([0-9.]+) ms; "
+                       r"f_int element vectors beside the assembly, gather after: ([0-9.]+) ms",
                        report)
-        overlap = bool(mm) and float(mm.group(2)) < float(mm.group(1))
-    if overlap and dom == "assembly":
+        if mm:
+            t = [float(mm.group(k)) for k in (1, 2, 3)]
+            overlap = ["serial", "f_int beside the assembly",
+                       "f_int element vectors beside the assembly, gather after"][t.index(min(t))]
+    if overlap not in (None, "serial") and dom == "assembly":
         roof["shares_gpu_with"] = "f_int (second stream)"
     rec = {
         "metric": "elements/s (K+f_int+assembly)",
@@ -421,7 +425,7 @@ def run_step(name, steps, warmup, serial=False, with_accuracy=True):
         "data": "synthetic (seeded generated mesh)", "config": cfg,
         "phase_ms": {"element_K": round(float(tk), 4), "assembly": round(float(ta), 4),
                      "f_int": round(float(tf), 4)},
-        "f_int_beside_assembly": overlap,
+        "step_order": overlap or "serial",
         "roofline": roof,
         "f_int_roofline": fint_roofline(m, tf),
     }
@@ -598,12 +602,29 @@ def run_copy(args):
     torch.cuda.synchronize()
     dt = (time.perf_counter() - t0) / args.steps
     gbs = 2 * (1 << 30) / dt / 1e9
+
+    def rate(f, nbytes):  # GB/s of f over args.steps runs
+        f()
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        for _ in range(args.steps):
+            f()
+        torch.cuda.synchronize()
+        return round(nbytes * args.steps / (time.perf_counter() - t1) / 1e9, 1)
+
+    # write-only (fill) and read-only (sum) rates of an 8 GiB buffer
+    big = torch.empty((8 << 30) // 8, dtype=torch.float64, device="cuda")
+    big.fill_(1.0)
+    fill = rate(lambda: big.fill_(2.0), 8 << 30)
+    read = rate(lambda: big.sum(), 8 << 30)
+    del big
     print(json.dumps({"metric": "device copy bandwidth (read + write)", "value": round(gbs, 1),
                       "unit": "GB/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
                       "ms_per_step": round(dt * 1e3, 4), "higher_is_better": True,
                       "scaling": "weak", "vs_baseline": None, "dtype": "f64",
                       "data": "synthetic", "config": {"workload": "copy_1GiB"},
-                      "bytes_per_copy": 2 * (1 << 30), "frac_of_peak": round(gbs / PEAK_HBM, 4)}))
+                      "bytes_per_copy": 2 * (1 << 30), "frac_of_peak": round(gbs / PEAK_HBM, 4),
+                      "write_only_fill_8GiB_GBs": fill, "read_only_sum_8GiB_GBs": read}))
 
 
 def main():
@@ -654,7 +675,7 @@ def main():
             r, _, _, _ = run_step(name, 10, 2)
             extras[r["config"]["workload"]] = {
                 "elements_per_s": r["value"], "ms_per_step": r["ms_per_step"],
-                "phase_ms": r["phase_ms"], "f_int_beside_assembly": r["f_int_beside_assembly"],
+                "phase_ms": r["phase_ms"], "step_order": r["step_order"],
                 "roofline": r["roofline"], "f_int_roofline": r["f_int_roofline"],
                 "accuracy": r["accuracy"]}
         nr = newton_record()

@@ -516,10 +516,14 @@ struct OnStream2 {
 };
 }  // namespace
 
-int stiffness_fint_overlapped(Ctx &c, const double *d_u, double *d_g);
+int stiffness_fint_overlapped(Ctx &c, const double *d_u, double *d_g, int mode);
 
-// the overlapped and the serial order give the same bits; the first call
-// times both (unless autotuning is off) and keeps the faster
+// The orders of the combined call, all giving the same bits: 0 = K, then
+// f_int, on the context stream; 1 = f_int (element vectors and gather) on the
+// second stream beside the assembly; 2 = only f_int's element vectors beside
+// the assembly (arithmetic beside a memory-bound kernel), the gather after
+// it.  The first call times the three (unless autotuning is off) and keeps
+// the fastest.
 int stiffness_fint_dev(Ctx &c, const double *d_u, double *d_g) {
   if (!c.tuned) autotune(c, d_u);
   if (c.overlap < 0) {
@@ -527,12 +531,12 @@ int stiffness_fint_dev(Ctx &c, const double *d_u, double *d_g) {
     if (env && env[0] == '0') {
       c.overlap = 0;
     } else {
-      double best[2] = {1e30, 1e30};
+      double best[3] = {1e30, 1e30, 1e30};
       for (int r = 0; r < 3; r++)
-        for (int mode = 0; mode < 2; mode++) {
+        for (int mode = 0; mode < 3; mode++) {
           FGPU_CHECK(hipStreamSynchronize(c.stream));
           const auto t0 = std::chrono::steady_clock::now();
-          const int ok = mode ? stiffness_fint_overlapped(c, d_u, d_g)
+          const int ok = mode ? stiffness_fint_overlapped(c, d_u, d_g, mode)
                               : (stiffness_dev(c, d_u) && int_force_dev(c, d_u, d_g));
           if (!ok) return 0;
           const double ms =
@@ -540,18 +544,22 @@ int stiffness_fint_dev(Ctx &c, const double *d_u, double *d_g) {
                   .count();
           best[mode] = std::min(best[mode], ms);
         }
-      c.overlap = best[1] < best[0] ? 1 : 0;
-      char buf[128];
-      std::snprintf(buf, sizeof buf, "K then f_int: %.4f ms; f_int beside the assembly: %.4f ms\n",
-                    best[0], best[1]);
+      c.overlap = 0;
+      for (int mode = 1; mode < 3; mode++)
+        if (best[mode] < best[c.overlap]) c.overlap = mode;
+      char buf[192];
+      std::snprintf(buf, sizeof buf,
+                    "K then f_int: %.4f ms; f_int beside the assembly: %.4f ms; f_int element "
+                    "vectors beside the assembly, gather after: %.4f ms\n",
+                    best[0], best[1], best[2]);
       c.tune_report += buf;
     }
   }
   if (!c.overlap) return stiffness_dev(c, d_u) && int_force_dev(c, d_u, d_g);
-  return stiffness_fint_overlapped(c, d_u, d_g);
+  return stiffness_fint_overlapped(c, d_u, d_g, c.overlap);
 }
 
-int stiffness_fint_overlapped(Ctx &c, const double *d_u, double *d_g) {
+int stiffness_fint_overlapped(Ctx &c, const double *d_u, double *d_g, int mode) {
   c.factored = false;
   c.bval32_valid = false;
   // the two status arrays are adjacent (d_status2 = d_status + 4)
@@ -580,17 +588,21 @@ int stiffness_fint_overlapped(Ctx &c, const double *d_u, double *d_g) {
       launch_bsp_fint_pts(c);
     else
       launch_bsp_fint(c, d_u);
-    launch_gather_fint(c, d_g);
+    if (mode == 1) launch_gather_fint(c, d_g);
     FGPU_CHECK(hipEventRecord(c.ev[4], c.stream));
   }
   launch_assemble_K(c);
   FGPU_CHECK(hipEventRecord(c.ev[2], c.stream));
   FGPU_CHECK(hipStreamWaitEvent(c.stream, c.ev[4], 0));  // join
+  if (mode == 2) {
+    launch_gather_fint(c, d_g);
+    FGPU_CHECK(hipEventRecord(c.ev[7], c.stream));
+  }
   enqueue_status_read(c, c.d_status, 8);  // both arrays
   FGPU_CHECK(hipStreamSynchronize(c.stream));
   c.last_ms[0] = elapsed(c, 0, 1);
   c.last_ms[1] = elapsed(c, 1, 2);
-  c.last_ms[2] = elapsed(c, 3, 4);
+  c.last_ms[2] = elapsed(c, 3, 4) + (mode == 2 ? elapsed(c, 2, 7) : 0.0);
   if (!check_status_values(c, "stiffness", c.h_status)) return 0;
   return check_status_values(c, "internal force", c.h_status + 4);
 }
@@ -1203,7 +1215,7 @@ int fgpu_set_launch(fgpu_ctx *ctx, int hex8_te, int asm_mode) {
 
 int fgpu_set_overlap(fgpu_ctx *ctx, int mode) {
   return guarded(ctx, [&](Ctx &c) {
-    if (mode < -1 || mode > 1) throw Error("overlap mode: -1, 0 or 1");
+    if (mode < -1 || mode > 2) throw Error("overlap mode: -1, 0, 1 or 2");
     c.overlap = mode;
     return 1;
   });

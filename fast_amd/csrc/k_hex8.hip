@@ -62,17 +62,75 @@ __device__ inline T inv3(const T J[3][3], T I[3][3]) {
 }
 
 // Per element-point vectors staged in LDS (floats per point):
-//   LINEAR:   g (24), c = w detJ
+//   LINEAR:   sqrt(c) g (24), c = w detJ: M_ab = sum_q (sqrt(c) g_a)(sqrt(c) g_b)^T
+//             costs 9 FMA per point and block, and a diagonal block comes out
+//             exactly symmetric (each product and sum is formed alike for ij, ji)
 //   SVK:      Fg (24), g (24), FF^T (6), S (6), c
 //   NEOHOOKE: g' (24), FF^T (6), c*lambda, c*mu', c*mu
 template <int FORM> struct Stage;
-template <> struct Stage<FGPU_LINEAR> { static constexpr int NF = 25; };
+template <> struct Stage<FGPU_LINEAR> { static constexpr int NF = 24; };
 template <> struct Stage<FGPU_TL_SVK> { static constexpr int NF = 61; };
 template <> struct Stage<FGPU_TL_NEOHOOKE> { static constexpr int NF = 33; };
 
 __device__ inline void lame(double E, double nu, float &lam, float &mu) {
   lam = (float)(E * nu / ((1.0 + nu) * (1.0 - 2.0 * nu)));
   mu = (float)(0.5 * E / (1.0 + nu));
+}
+
+// Per element in LDS beside the staged point data: node coordinates
+// relative to node 0 (FP64 subtraction, then FP32) and, for the
+// total-Lagrangian forms, nodal displacements, 8 x 3 floats each, padded to
+// 25 per element so that consecutive elements fall in different banks; the
+// two material constants.
+constexpr int XPAD = 25;
+template <int FORM>
+constexpr int elem_lds_floats() { return XPAD * (FORM == FGPU_LINEAR ? 1 : 2) + 2; }
+
+// Linear blocks of rows p and 7 - p (p = 0..3: 8 - p and p + 1 blocks, 9 in
+// all) from the staged vectors s_a = sqrt(c) grad N_a of each point (S: the
+// element's column of the [q][f][TE] stage): M_ab = sum_q s_a s_b^T,
+// K_ab = lambda M + mu M^T + mu tr(M) I, stored as upper node blocks.
+template <int P>
+__device__ inline void hex8_lin_rows(const float *S, int TE, int NQ, float lam, float mu,
+                                     float *__restrict__ Ke, KeLayout L, int e) {
+  constexpr int p = P, a1 = P, a2 = 7 - P;
+  float M[9][9];
+  for (int k = 0; k < 9; k++)
+    for (int m = 0; m < 9; m++) M[k][m] = 0.f;
+#pragma unroll 1
+  for (int q = 0; q < NQ; q++) {
+    const float *Sq = S + (size_t)q * 24 * TE;
+    float v[8][3];
+#pragma unroll
+    for (int b = 0; b < 8; b++)
+      if (b >= p)
+        for (int i = 0; i < 3; i++) v[b][i] = Sq[(3 * b + i) * TE];
+#pragma unroll
+    for (int b = 0; b < 8; b++) {
+      // block slot: row a1 holds b = a1..7 in slots 0..7-a1, row a2 the rest
+      if (b >= a1) {
+        float *Mk = M[b - a1];
+        for (int i = 0; i < 3; i++)
+          for (int j = 0; j < 3; j++) Mk[3 * i + j] = fmaf(v[a1][i], v[b][j], Mk[3 * i + j]);
+      }
+      if (b >= a2) {
+        float *Mk = M[8 - a1 + b - a2];
+        for (int i = 0; i < 3; i++)
+          for (int j = 0; j < 3; j++) Mk[3 * i + j] = fmaf(v[a2][i], v[b][j], Mk[3 * i + j]);
+      }
+    }
+  }
+#pragma unroll
+  for (int k = 0; k < 9; k++) {
+    const int a = k < 8 - a1 ? a1 : a2;
+    const int b = k < 8 - a1 ? a1 + k : a2 + (k - (8 - a1));
+    const float *Mk = M[k];
+    const float tr = mu * (Mk[0] + Mk[4] + Mk[8]);
+    float *out = Ke + L.offset(e, sym_index(a, b, 8));
+    for (int i = 0; i < 3; i++)
+      for (int j = 0; j < 3; j++)
+        out[3 * i + j] = lam * Mk[3 * i + j] + mu * Mk[3 * j + i] + (i == j ? tr : 0.f);
+  }
 }
 
 template <int FORM, int NIP, int TE>
@@ -83,9 +141,37 @@ __global__ __launch_bounds__(256) void hex8_K_kernel(
     int *__restrict__ status) {
   constexpr int NQ = NIP * NIP * NIP;
   constexpr int NF = Stage<FORM>::NF;
-  extern __shared__ float sh[];  // [q][f][TE]
+  extern __shared__ float sh[];  // [q][f][TE], then the per-element data
+  float *sX = sh + (size_t)NQ * NF * TE;                       // [TE][XPAD]
+  float *sU = sX + (size_t)TE * XPAD;                          // [TE][XPAD] (TL)
+  float *sM = sX + (size_t)TE * XPAD * (FORM == FGPU_LINEAR ? 1 : 2);  // [TE][2]
   const int e0 = blockIdx.x * TE;
   const int tid = threadIdx.x;
+
+  // ---------------- phase 0: one thread per (element, node): each node's
+  // coordinates (and displacement) are read once per element; consecutive
+  // threads read consecutive connectivity entries
+  for (int task = tid; task < TE * 8; task += blockDim.x) {
+    const int el = task / 8, a = task % 8;
+    const int e = e0 + el;
+    if (e >= nelem) continue;
+    const size_t n = conn[(size_t)e * 8 + a], n0 = conn[(size_t)e * 8];
+    for (int k = 0; k < 3; k++) sX[el * XPAD + 3 * a + k] = (float)(coords[3 * n + k] - coords[3 * n0 + k]);
+    if constexpr (FORM != FGPU_LINEAR)
+      for (int k = 0; k < 3; k++) sU[el * XPAD + 3 * a + k] = (float)unodal[3 * n + k];
+    if (a == 0) {
+      float lam, mu;
+      if constexpr (FORM == FGPU_TL_NEOHOOKE) {  // mat = (mu, lambda)
+        mu = (float)mat[2 * (size_t)e];
+        lam = (float)mat[2 * (size_t)e + 1];
+      } else {
+        lame(mat[2 * (size_t)e], mat[2 * (size_t)e + 1], lam, mu);
+      }
+      sM[2 * el] = lam;
+      sM[2 * el + 1] = mu;
+    }
+  }
+  __syncthreads();
 
   // ---------------- phase 1: one thread per (element, point)
   for (int task = tid; task < TE * NQ; task += blockDim.x) {
@@ -96,17 +182,7 @@ __global__ __launch_bounds__(256) void hex8_K_kernel(
       for (int f = 0; f < NF; f++) S[f * TE] = 0.f;
       continue;
     }
-    const int *cn = conn + (size_t)e * 8;
-    int nd[8];
-    for (int a = 0; a < 8; a++) nd[a] = cn[a];
-    // coordinates relative to node 0 (FP64 subtraction, then FP32)
-    double x0 = coords[3 * nd[0]], y0 = coords[3 * nd[0] + 1], z0 = coords[3 * nd[0] + 2];
-    float X[8][3];
-    for (int a = 0; a < 8; a++) {
-      X[a][0] = (float)(coords[3 * nd[a]] - x0);
-      X[a][1] = (float)(coords[3 * nd[a] + 1] - y0);
-      X[a][2] = (float)(coords[3 * nd[a] + 2] - z0);
-    }
+    const float *X = sX + el * XPAD;
     const int ir = q % NIP, is = (q / NIP) % NIP, it = q / (NIP * NIP);
     const float r = gp[(0 * 16 + ir) * 2], s = gp[(1 * 16 + is) * 2], t = gp[(2 * 16 + it) * 2];
     const float w = gp[(0 * 16 + ir) * 2 + 1] * gp[(1 * 16 + is) * 2 + 1] * gp[(2 * 16 + it) * 2 + 1];
@@ -115,7 +191,7 @@ __global__ __launch_bounds__(256) void hex8_K_kernel(
     for (int a = 0; a < 8; a++) {
       hex8_dN(r, s, t, a, dN[a][0], dN[a][1], dN[a][2]);
       for (int i = 0; i < 3; i++)
-        for (int k = 0; k < 3; k++) J[i][k] += dN[a][i] * X[a][k];
+        for (int k = 0; k < 3; k++) J[i][k] += dN[a][i] * X[3 * a + k];
     }
     float Ji[3][3];
     float det = inv3(J, Ji);
@@ -126,14 +202,14 @@ __global__ __launch_bounds__(256) void hex8_K_kernel(
         g[a][i] = Ji[i][0] * dN[a][0] + Ji[i][1] * dN[a][1] + Ji[i][2] * dN[a][2];
     const float c = w * fabsf(det);  // |det J|: a left-handed mesh is accepted whole
     if constexpr (FORM == FGPU_LINEAR) {
+      const float sc = sqrtf(c);
       for (int a = 0; a < 8; a++)
-        for (int i = 0; i < 3; i++) S[(3 * a + i) * TE] = g[a][i];
-      S[24 * TE] = c;
+        for (int i = 0; i < 3; i++) S[(3 * a + i) * TE] = sc * g[a][i];
     } else {
       float F[3][3] = {{1, 0, 0}, {0, 1, 0}, {0, 0, 1}};
+      const float *Ue = sU + el * XPAD;
       for (int a = 0; a < 8; a++) {
-        const double *ua = unodal + 3 * (size_t)nd[a];
-        float u0 = (float)ua[0], u1 = (float)ua[1], u2 = (float)ua[2];
+        const float u0 = Ue[3 * a], u1 = Ue[3 * a + 1], u2 = Ue[3 * a + 2];
         for (int j = 0; j < 3; j++) {
           F[0][j] += u0 * g[a][j];
           F[1][j] += u1 * g[a][j];
@@ -147,9 +223,8 @@ __global__ __launch_bounds__(256) void hex8_K_kernel(
       B[3] = F[0][0] * F[1][0] + F[0][1] * F[1][1] + F[0][2] * F[1][2];
       B[4] = F[0][0] * F[2][0] + F[0][1] * F[2][1] + F[0][2] * F[2][2];
       B[5] = F[1][0] * F[2][0] + F[1][1] * F[2][1] + F[1][2] * F[2][2];
-      float lam, mu;
+      const float lam = sM[2 * el], mu = sM[2 * el + 1];
       if constexpr (FORM == FGPU_TL_SVK) {
-        lame(mat[2 * (size_t)e], mat[2 * (size_t)e + 1], lam, mu);
         // Green-Lagrange E = (F^T F - I) / 2 ; S = lambda tr(E) I + 2 mu E
         float C00 = F[0][0] * F[0][0] + F[1][0] * F[1][0] + F[2][0] * F[2][0];
         float C11 = F[0][1] * F[0][1] + F[1][1] * F[1][1] + F[2][1] * F[2][1];
@@ -171,9 +246,7 @@ __global__ __launch_bounds__(256) void hex8_K_kernel(
           S[(54 + k) * TE] = Sv[k];
         }
         S[60 * TE] = c;
-      } else {  // neo-Hooke: mat = (mu, lambda)
-        mu = (float)mat[2 * (size_t)e];
-        lam = (float)mat[2 * (size_t)e + 1];
+      } else {  // neo-Hooke
         float Fi[3][3];
         float Jd = inv3(F, Fi);
         if (!(Jd > 0.f)) {
@@ -193,6 +266,25 @@ __global__ __launch_bounds__(256) void hex8_K_kernel(
   }
   __syncthreads();
 
+  if constexpr (FORM == FGPU_LINEAR) {
+    // ---------------- phase 2 (linear): one thread per (element, row pair):
+    // rows p and 7 - p of the element's upper blocks, 9 blocks together,
+    // share their loads of the staged vectors; consecutive threads are
+    // consecutive elements, so each block store of a wave is contiguous
+    for (int task = tid; task < TE * 4; task += blockDim.x) {
+      const int el = task % TE, p = task / TE;
+      const int e = e0 + el;
+      if (e >= nelem) continue;
+      const float lam = sM[2 * el], mu = sM[2 * el + 1];
+      switch (p) {  // uniform across a wave when TE >= 64
+        case 0: hex8_lin_rows<0>(sh + el, TE, NQ, lam, mu, Ke, L, e); break;
+        case 1: hex8_lin_rows<1>(sh + el, TE, NQ, lam, mu, Ke, L, e); break;
+        case 2: hex8_lin_rows<2>(sh + el, TE, NQ, lam, mu, Ke, L, e); break;
+        default: hex8_lin_rows<3>(sh + el, TE, NQ, lam, mu, Ke, L, e); break;
+      }
+    }
+    return;
+  }
   // ---------------- phase 2: one thread per (element, upper node block)
   for (int task = tid; task < TE * 36; task += blockDim.x) {
     const int el = task % TE, blk = task / TE;
@@ -200,25 +292,7 @@ __global__ __launch_bounds__(256) void hex8_K_kernel(
     if (e >= nelem) continue;
     const int a = kBlkA[blk], b = kBlkB[blk];
     float K[9];
-    if constexpr (FORM == FGPU_LINEAR) {
-      float M[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
-      for (int q = 0; q < NQ; q++) {
-        const float *S = sh + (size_t)q * NF * TE + el;
-        // c * (g_i h_j): the product is formed before the weight so that a
-        // diagonal block (a == b) comes out exactly symmetric
-        float c = S[24 * TE];
-        float ga[3] = {S[(3 * a) * TE], S[(3 * a + 1) * TE], S[(3 * a + 2) * TE]};
-        float hb[3] = {S[(3 * b) * TE], S[(3 * b + 1) * TE], S[(3 * b + 2) * TE]};
-        for (int i = 0; i < 3; i++)
-          for (int j = 0; j < 3; j++) M[3 * i + j] += c * (ga[i] * hb[j]);
-      }
-      float lam, mu;
-      lame(mat[2 * (size_t)e], mat[2 * (size_t)e + 1], lam, mu);
-      float tr = mu * (M[0] + M[4] + M[8]);
-      for (int i = 0; i < 3; i++)
-        for (int j = 0; j < 3; j++)
-          K[3 * i + j] = lam * M[3 * i + j] + mu * M[3 * j + i] + (i == j ? tr : 0.f);
-    } else if constexpr (FORM == FGPU_TL_SVK) {
+    if constexpr (FORM == FGPU_TL_SVK) {
       float M[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
       float Bs[6] = {0, 0, 0, 0, 0, 0};
       float s = 0.f;
@@ -242,8 +316,7 @@ __global__ __launch_bounds__(256) void hex8_K_kernel(
              ga[1] * (s3 * gb[0] + s1 * gb[1] + s5 * gb[2]) +
              ga[2] * (s4 * gb[0] + s5 * gb[1] + s2 * gb[2]);
       }
-      float lam, mu;
-      lame(mat[2 * (size_t)e], mat[2 * (size_t)e + 1], lam, mu);
+      const float lam = sM[2 * el], mu = sM[2 * el + 1];
       const float Bm[9] = {Bs[0], Bs[3], Bs[4], Bs[3], Bs[1], Bs[5], Bs[4], Bs[5], Bs[2]};
       for (int i = 0; i < 3; i++)
         for (int j = 0; j < 3; j++)
@@ -508,7 +581,7 @@ template <int FORM, int NIP, int TE>
 void launch_K_te(Ctx &c, const double *u) {
   constexpr int NQ = NIP * NIP * NIP;
   constexpr int NF = Stage<FORM>::NF;
-  size_t lds = (size_t)NQ * NF * TE * sizeof(float);
+  size_t lds = ((size_t)NQ * NF + elem_lds_floats<FORM>()) * TE * sizeof(float);
   int grid = (c.nelem + TE - 1) / TE;
   hipLaunchKernelGGL((hex8_K_kernel<FORM, NIP, TE>), dim3(grid), dim3(256), lds, c.stream,
                      c.nelem, c.d_coords, c.d_conn, c.d_mat, u, c.d_gauss, c.d_Ke, c.kel,
@@ -517,18 +590,19 @@ void launch_K_te(Ctx &c, const double *u) {
 }
 
 // largest number of elements per workgroup (a divisor of the 64-element
-// storage tile) whose staged point data fit in 64 KB of LDS
-constexpr int te_max(int nq, int nf) {
-  return (65536 / (nq * nf * 4)) >= 64   ? 64
-         : (65536 / (nq * nf * 4)) >= 32 ? 32
-         : (65536 / (nq * nf * 4)) >= 16 ? 16
-         : (65536 / (nq * nf * 4)) >= 8  ? 8
-                                         : 4;
+// storage tile) whose staged point data and per-element data fit in 64 KB of
+// LDS (ne floats per element beside the nq * nf of the points)
+constexpr int te_max(int nq, int nf, int ne) {
+  return (65536 / ((nq * nf + ne) * 4)) >= 64   ? 64
+         : (65536 / ((nq * nf + ne) * 4)) >= 32 ? 32
+         : (65536 / ((nq * nf + ne) * 4)) >= 16 ? 16
+         : (65536 / ((nq * nf + ne) * 4)) >= 8  ? 8
+                                                : 4;
 }
 
 template <int FORM, int NIP>
 void launch_K_t(Ctx &c, const double *u) {
-  constexpr int TMAX = te_max(NIP * NIP * NIP, Stage<FORM>::NF);
+  constexpr int TMAX = te_max(NIP * NIP * NIP, Stage<FORM>::NF, elem_lds_floats<FORM>());
   const int te = c.hex8_te > 0 && c.hex8_te <= TMAX ? c.hex8_te : TMAX;
   switch (te) {
     case 4: launch_K_te<FORM, NIP, 4>(c, u); break;
@@ -541,8 +615,9 @@ void launch_K_t(Ctx &c, const double *u) {
 
 template <int FORM>
 int te_limit(int nip) {
-  return nip == 1 ? te_max(1, Stage<FORM>::NF)
-                  : nip == 2 ? te_max(8, Stage<FORM>::NF) : te_max(27, Stage<FORM>::NF);
+  constexpr int ne = elem_lds_floats<FORM>();
+  return nip == 1 ? te_max(1, Stage<FORM>::NF, ne)
+                  : nip == 2 ? te_max(8, Stage<FORM>::NF, ne) : te_max(27, Stage<FORM>::NF, ne);
 }
 
 template <int FORM>

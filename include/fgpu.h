@@ -376,8 +376,9 @@ int fgpu_set_launch(fgpu_ctx *ctx, int hex8_te, int asm_mode);
  * context stream; 1 = the internal force on the second stream beside the
  * assembly, and for the linear tuned B-spline kernels (degrees (3,3,3) and
  * (3,3,2), 4x4x4 Gauss) finished from the per-point records the stiffness
- * kernel writes; -1 (default) = the first call times both and keeps the
- * faster (FGPU_AUTOTUNE=0: 0).  Both orders give the same bits. */
+ * kernel writes; 2 = as 1, but only the element vectors beside the assembly
+ * and the gather after it; -1 (default) = the first call times the three and
+ * keeps the fastest (FGPU_AUTOTUNE=0: 0).  All orders give the same bits. */
 int fgpu_set_overlap(fgpu_ctx *ctx, int mode);
 const char *fgpu_tuning_report(fgpu_ctx *ctx);
 

@@ -311,11 +311,13 @@ def test_linear_static_driver(fg, oracle_mod):
     assert nrel(m.scatter(u), d["u"]) <= 1e-8
 
 
-@pytest.mark.parametrize("overlap", [0, 1])
-@pytest.mark.parametrize("kind", ["hex8_nh", "bsp_svk", "bsp332_linear", "bsp_linear"])
+@pytest.mark.parametrize("overlap", [0, 1, 2])
+@pytest.mark.parametrize("kind", ["hex8_nh", "hex8_linear", "bsp_svk", "bsp332_linear",
+                                  "bsp_linear"])
 def test_stiffness_beside_internal_force_identical_bits(fg, kind, overlap):
-    """fgpu_assemble_K_fint in both orders, forced (0: K then f_int on one
-    stream; 1: f_int on the second stream beside the assembly, for the linear
+    """fgpu_assemble_K_fint in every order, forced (0: K then f_int on one
+    stream; 1: f_int on the second stream beside the assembly; 2: its element
+    vectors beside the assembly, the gather after it; for the linear
     tuned B-spline kernels of degree (3,3,3) and (3,3,2) finished from the
     per-point records the stiffness kernel writes) gives the matrix and the
     force of fgpu_assemble_K + fgpu_int_force, bit for bit, and reports a
@@ -323,6 +325,8 @@ def test_stiffness_beside_internal_force_identical_bits(fg, kind, overlap):
     if kind == "hex8_nh":
         m = M.hex8_block(5, 4, 3, form="tl_neohooke", perturb=0.2, seed=4,
                          mat=np.tile((1.0, 1.5), (60, 1)))
+    elif kind == "hex8_linear":
+        m = M.hex8_block(5, 4, 3, perturb=0.2, seed=6)
     elif kind == "bsp_svk":
         m = M.bsp_block(ne=(3, 3, 2), p=3, perturb=0.05, seed=2, form="tl_svk",
                         mat=np.tile((2.6, 0.3), (18, 1)))
@@ -350,15 +354,18 @@ def test_stiffness_beside_internal_force_identical_bits(fg, kind, overlap):
     g.close()
 
 
-@pytest.mark.parametrize("kind", ["hex8_nh", "bsp"])
+@pytest.mark.parametrize("kind", ["hex8_nh", "hex8_linear", "bsp"])
 def test_launch_configurations_identical_bits(fg, oracle_mod, kind):
     """Every launch configuration the autotuner may pick (hex8 elements per
     workgroup; assembly loads in batches of 1, 4, 8 or 16, with and without the
-    XCD-contiguous workgroup order) gives the same global
-    matrix, bit for bit, and it matches the oracle to the FP32
-    element-matrix tolerance."""
+    XCD-contiguous workgroup order) gives the same global matrix, bit for bit,
+    on neo-Hookean and linear bricks and on a cubic B-spline patch, and it
+    matches the oracle to the FP32 element-matrix tolerance."""
     if kind == "bsp":
         m = M.bsp_block(ne=(4, 3, 3), p=3, perturb=0.05, seed=2)
+        u = None
+    elif kind == "hex8_linear":
+        m = M.hex8_block(7, 6, 5, perturb=0.2, seed=8, E=2.0, nu=0.35)
         u = None
     else:
         n = 6
