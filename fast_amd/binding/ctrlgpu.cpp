@@ -194,9 +194,18 @@ int GpuInit(void)
     return Fail("GpuInit: fgpu_create failed (see stderr)");
   }
 
+  // the penalty of the prescribed equations stays on the device: every
+  // stiffness adds it to the diagonal, every internal force adds
+  // PENALTY_STIFFNESS times the current value (ctrl/ctrl.cpp:958-974,
+  // 794-810)
   int npd = cPrescDof :: GetNumPrescDofs( );
   PrDof.assign(npd, 0);
-  if (npd > 0) cPrescDof :: GetPrescDofs(PrDof.data( ));
+  if (npd > 0)
+  {
+    cPrescDof :: GetPrescDofs(PrDof.data( ));
+    std::vector<double> pen(npd, PENALTY_STIFFNESS);
+    if (!fgpu_set_penalty(Ctx, npd, PrDof.data( ), pen.data( ))) return FailCtx("GpuInit");
+  }
   return 1;
 }
 
@@ -218,12 +227,6 @@ int GpuStiffnessMatrix(void)
   if (!Ctx) return Fail("GpuStiffnessMatrix: no context (GpuInit)");
   std::vector<double> u = NodalDispl( );
   if (!fgpu_assemble_K(Ctx, u.data( ))) return FailCtx("GpuStiffnessMatrix");
-  if (!PrDof.empty( ))
-  {
-    std::vector<double> pen(PrDof.size( ), PENALTY_STIFFNESS);
-    if (!fgpu_add_diag(Ctx, (int)PrDof.size( ), PrDof.data( ), pen.data( )))
-      return FailCtx("GpuStiffnessMatrix");
-  }
   return 1;
 }
 
@@ -234,13 +237,6 @@ int GpuIntForceVector(cVector &g)
   if (!Ctx) return Fail("GpuIntForceVector: no context (GpuInit)");
   std::vector<double> u = NodalDispl( );
   if (!fgpu_int_force(Ctx, u.data( ), &g[0])) return FailCtx("GpuIntForceVector");
-  int npd = (int)PrDof.size( );
-  if (npd > 0)
-  {
-    cVector crval(npd);
-    cPrescDof :: CurrNodalVals(crval);
-    for (int i = 0; i < npd; i++) g[PrDof[i] - 1] += PENALTY_STIFFNESS * crval[i];
-  }
   return 1;
 }
 

@@ -341,6 +341,14 @@ void destroy(Ctx *c) {
                    c->d_ndone};
   for (void *p : bbufs)
     if (p) (void)hipFree(p);
+  for (auto &g : c->sgroups) {
+    void *gb[] = {g.d_nodes, g.d_ext, g.d_local, g.d_q, g.d_gp};
+    for (void *p : gb)
+      if (p) (void)hipFree(p);
+  }
+  void *lbufs[] = {c->d_sfv, c->d_sptr, c->d_sadj, c->d_pnd, c->d_pval};
+  for (void *p : lbufs)
+    if (p) (void)hipFree(p);
   if (c->g_single) (void)hipGraphExecDestroy(c->g_single);
   if (c->g_batch) (void)hipGraphExecDestroy(c->g_batch);
   if (c->stream) (void)hipStreamDestroy(c->stream);
@@ -477,6 +485,7 @@ int stiffness_dev(Ctx &c, const double *d_u) {
     launch_bsp_K(c, d_u);
   FGPU_CHECK(hipEventRecord(c.ev[1], c.stream));
   launch_assemble_K(c);
+  launch_penalty_diag(c);
   FGPU_CHECK(hipEventRecord(c.ev[2], c.stream));
   enqueue_status_read(c, c.d_status, 4);
   FGPU_CHECK(hipStreamSynchronize(c.stream));
@@ -493,6 +502,7 @@ int int_force_dev(Ctx &c, const double *d_u, double *d_g) {
   else
     launch_bsp_fint(c, d_u);
   launch_gather_fint(c, d_g);
+  launch_penalty_force(c, d_u, d_g);
   FGPU_CHECK(hipEventRecord(c.ev[4], c.stream));
   enqueue_status_read(c, c.d_status, 4);
   FGPU_CHECK(hipStreamSynchronize(c.stream));
@@ -588,14 +598,19 @@ int stiffness_fint_overlapped(Ctx &c, const double *d_u, double *d_g, int mode) 
       launch_bsp_fint_pts(c);
     else
       launch_bsp_fint(c, d_u);
-    if (mode == 1) launch_gather_fint(c, d_g);
+    if (mode == 1) {
+      launch_gather_fint(c, d_g);
+      launch_penalty_force(c, d_u, d_g);
+    }
     FGPU_CHECK(hipEventRecord(c.ev[4], c.stream));
   }
   launch_assemble_K(c);
+  launch_penalty_diag(c);
   FGPU_CHECK(hipEventRecord(c.ev[2], c.stream));
   FGPU_CHECK(hipStreamWaitEvent(c.stream, c.ev[4], 0));  // join
   if (mode == 2) {
     launch_gather_fint(c, d_g);
+    launch_penalty_force(c, d_u, d_g);
     FGPU_CHECK(hipEventRecord(c.ev[7], c.stream));
   }
   enqueue_status_read(c, c.d_status, 8);  // both arrays
@@ -1209,6 +1224,140 @@ int fgpu_set_launch(fgpu_ctx *ctx, int hex8_te, int asm_mode) {
     c.asm_mode = asm_mode;
     c.tuned = true;  // an explicit choice replaces the autotuner
     c.tune_report = "set by the caller\n";
+    return 1;
+  });
+}
+
+int fgpu_add_surface_loads(fgpu_ctx *ctx, const fgpu_surface *sl) {
+  return guarded(ctx, [&](Ctx &c) {
+    if (!sl) throw Error("null argument");
+    if (sl->nface < 0) throw Error("surface loads: negative count");
+    if (sl->nface == 0) return 1;
+    for (int d = 0; d < 2; d++)
+      if (sl->deg[d] < 1 || sl->deg[d] > 7) throw Error("surface loads: degrees 1..7");
+    if (sl->nip < 1 || sl->nip > 16) throw Error("surface loads: 1..16 Gauss points");
+    if (!sl->nodes || !sl->q) throw Error("surface loads: nodes and q are required");
+    const int n1 = sl->deg[0] + 1, n2 = sl->deg[1] + 1, nb = n1 * n2;
+    const size_t nn = (size_t)sl->nface * nb;
+    for (size_t k = 0; k < nn; k++)
+      if (sl->nodes[k] < 0 || sl->nodes[k] >= c.nnode)
+        throw Error("surface loads: node out of range");
+    if (c.nsrow + (long long)nn >= (1LL << 31)) throw Error("surface loads: too many face nodes");
+    SurfaceGroup g;
+    g.nface = sl->nface;
+    g.deg[0] = sl->deg[0];
+    g.deg[1] = sl->deg[1];
+    g.nip = sl->nip;
+    g.row0 = c.nsrow;
+    g.h_nodes.assign(sl->nodes, sl->nodes + nn);
+    g.d_nodes = dupload(sl->nodes, nn, c.stream);
+    if (sl->ext) g.d_ext = dupload(sl->ext, (size_t)sl->nface * (n1 * n1 + n2 * n2), c.stream);
+    if (sl->local) g.d_local = dupload(sl->local, (size_t)sl->nface, c.stream);
+    g.d_q = dupload(sl->q, (size_t)sl->nface * 3, c.stream);
+    double r[16], w[16], gp[32];
+    gauss_rule(sl->nip, r, w);
+    for (int k = 0; k < sl->nip; k++) {
+      gp[2 * k] = r[k];
+      gp[2 * k + 1] = w[k];
+    }
+    g.d_gp = dupload(gp, 2 * (size_t)sl->nip, c.stream);
+    c.sgroups.push_back(std::move(g));
+    c.nsrow += (long long)nn;
+    // face-vector rows of every group and the gather lists: each node's rows
+    // in increasing row order, i.e. in the order the entries were given
+    std::vector<int> ptr(c.nnode + 1, 0), adj(c.nsrow);
+    for (const auto &gr : c.sgroups)
+      for (int n : gr.h_nodes) ptr[n + 1]++;
+    for (int n = 0; n < c.nnode; n++) ptr[n + 1] += ptr[n];
+    std::vector<int> pos(ptr.begin(), ptr.end() - 1);
+    for (const auto &gr : c.sgroups)
+      for (size_t k = 0; k < gr.h_nodes.size(); k++)
+        adj[pos[gr.h_nodes[k]]++] = (int)(gr.row0 + (long long)k);
+    for (void *p : {(void *)c.d_sfv, (void *)c.d_sptr, (void *)c.d_sadj})
+      if (p) (void)hipFree(p);
+    c.d_sfv = dalloc<double>((size_t)c.nsrow * 3);
+    c.d_sptr = dupload(ptr.data(), ptr.size(), c.stream);
+    c.d_sadj = dupload(adj.data(), adj.size(), c.stream);
+    FGPU_CHECK(hipStreamSynchronize(c.stream));
+    return 1;
+  });
+}
+
+int fgpu_clear_surface_loads(fgpu_ctx *ctx) {
+  return guarded(ctx, [&](Ctx &c) {
+    FGPU_CHECK(hipStreamSynchronize(c.stream));
+    for (auto &g : c.sgroups) {
+      void *gb[] = {g.d_nodes, g.d_ext, g.d_local, g.d_q, g.d_gp};
+      for (void *p : gb)
+        if (p) (void)hipFree(p);
+    }
+    c.sgroups.clear();
+    for (void *p : {(void *)c.d_sfv, (void *)c.d_sptr, (void *)c.d_sadj})
+      if (p) (void)hipFree(p);
+    c.d_sfv = nullptr;
+    c.d_sptr = c.d_sadj = nullptr;
+    c.nsrow = 0;
+    return 1;
+  });
+}
+
+int fgpu_surface_load_dev(fgpu_ctx *ctx, double *f_dev) {
+  return guarded(ctx, [&](Ctx &c) {
+    if (!f_dev) throw Error("null argument");
+    if (c.sgroups.empty()) {
+      FGPU_CHECK(hipMemsetAsync(f_dev, 0, sizeof(double) * c.neq, c.stream));
+    } else {
+      for (const auto &g : c.sgroups) launch_surface_group(c, g, c.d_sfv + 3 * g.row0);
+      launch_gather_surface(c, f_dev);
+    }
+    FGPU_CHECK(hipStreamSynchronize(c.stream));
+    return 1;
+  });
+}
+
+int fgpu_surface_load(fgpu_ctx *ctx, double *f) {
+  return guarded(ctx, [&](Ctx &c) {
+    if (!f) throw Error("null argument");
+    if (!fgpu_surface_load_dev(ctx, c.d_b)) return 0;
+    FGPU_CHECK(hipMemcpy(f, c.d_b, sizeof(double) * c.neq, hipMemcpyDeviceToHost));
+    return 1;
+  });
+}
+
+int fgpu_set_penalty(fgpu_ctx *ctx, int n, const int *eq, const double *v) {
+  return guarded(ctx, [&](Ctx &c) {
+    if (n < 0 || (n > 0 && (!eq || !v))) throw Error("fgpu_set_penalty: bad arguments");
+    std::vector<int> eqnode(c.neq, -1);
+    for (int i = 0; i < 3 * c.nnode; i++)
+      if (c.eqmap[i] >= 0) eqnode[c.eqmap[i]] = i;
+    // one entry per equation, repeated equations summed in the given order
+    std::vector<int> nd;
+    std::vector<double> val;
+    std::vector<int> slot(3 * (size_t)c.nnode, -1);
+    for (int k = 0; k < n; k++) {
+      if (eq[k] < 1 || eq[k] > c.neq) throw Error("fgpu_set_penalty: equation out of range");
+      const int i = eqnode[eq[k] - 1];
+      if (slot[i] < 0) {
+        slot[i] = (int)nd.size();
+        nd.push_back(i);
+        val.push_back(v[k]);
+      } else {
+        val[slot[i]] += v[k];
+      }
+    }
+    FGPU_CHECK(hipStreamSynchronize(c.stream));
+    if (c.d_pnd) (void)hipFree(c.d_pnd);
+    if (c.d_pval) (void)hipFree(c.d_pval);
+    c.d_pnd = nullptr;
+    c.d_pval = nullptr;
+    c.npen = (int)nd.size();
+    if (c.npen) {
+      c.d_pnd = dupload(nd.data(), nd.size(), c.stream);
+      c.d_pval = dupload(val.data(), val.size(), c.stream);
+      FGPU_CHECK(hipStreamSynchronize(c.stream));
+    }
+    c.factored = false;
+    c.bval32_valid = false;
     return 1;
   });
 }

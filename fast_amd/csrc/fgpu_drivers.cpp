@@ -2,11 +2,12 @@
 // load-control Newton-Raphson, with every vector resident on the device.
 //
 // Linear static follows cLinStat::Solver (ctrl/ctrllins.cpp:46-99):
-// assemble, solve; the solve refines with FP64 residuals (below).  Newton-Raphson follows cCtrlNR::Solver
-// (ctrl/ctrlnr.cpp:47-267): g = f_int(0); per step lambda += factor,
-// f = lambda q, r = f - g; per iteration K(u), K du = r, u += du,
-// g = f_int(u), r = f - g, converged when |r| / max(1, |f|) < Tol
-// (ctrl/ctrl.cpp:1366-1380; |g| replaces |f| when |f| <= Tol); the loop stops
+// assemble, solve; the solve refines with FP64 residuals (below).
+// Newton-Raphson follows cCtrlNR::Solver (ctrl/ctrlnr.cpp:47-267):
+// g = f_int(0); per step lambda += factor, f = lambda q, r = f - g; per
+// iteration K(u), K du = r, u += du, g = f_int(u), r = f - g, converged when
+// |r| / max(1, |f|) < Tol (ctrl/ctrl.cpp:1366-1413; |g| replaces |f| when
+// |f| <= Tol; the norms leave the prescribed equations out); the loop stops
 // at the first step that does not converge.
 #include <algorithm>
 #include <chrono>
@@ -45,6 +46,29 @@ struct DevVec {
   DevVec &operator=(const DevVec &) = delete;
 };
 
+// |x| over the free equations: x copied to tmp with the prescribed (penalty)
+// entries zeroed (cControl::Convergence, ctrl/ctrl.cpp:1391-1413)
+double free_len(Ctx &c, const double *x, double *tmp, int n) {
+  if (!c.npen) return std::sqrt(dev_dot(c, x, x, n));
+  FGPU_CHECK(hipMemcpyAsync(tmp, x, sizeof(double) * n, hipMemcpyDeviceToDevice, c.stream));
+  launch_zero_penalty(c, tmp);
+  return std::sqrt(dev_dot(c, tmp, tmp, n));
+}
+
+// |g| of the elements' internal force alone (penalty left out) at the nodal
+// state u: when the prescribed values are the only load, the reactions it
+// holds give the force scale of the residual test
+double element_force_len(Ctx &c, const double *u_nodal, double *tmp, int n) {
+  struct Restore {
+    Ctx &c;
+    int npen;
+    ~Restore() { c.npen = npen; }
+  } keep{c, c.npen};
+  c.npen = 0;
+  if (!int_force_dev(c, u_nodal, tmp)) throw Error(c.err);
+  return std::sqrt(dev_dot(c, tmp, tmp, n));
+}
+
 // y = K x on equation vectors (PCG work vectors as scratch)
 void spmv_eq(Ctx &c, const double *x, double *y) {
   launch_eq_to_nodal(c, x, c.d_bn);
@@ -76,11 +100,14 @@ int fgpu_linear_static(fgpu_ctx *ctx, const double *f, double *u, int maxit, dou
     // Iterative refinement: the PCG matrix is assembled from FP32 element
     // matrices; the residual r = f - K u is formed in FP64 element by element
     // (f_int of the linear model is K u), and each PCG solve only needs to
-    // reduce it by the factor the FP32 matrix can deliver.
-    const double flen = std::sqrt(dev_dot(c, df.p, df.p, n));
+    // reduce it by the factor the FP32 matrix can deliver.  With prescribed
+    // equations the test runs over the free ones: after the first solve the
+    // prescribed equations hold their values to the penalty's accuracy, so
+    // the corrections see residuals with those entries zeroed.
+    double scale = free_len(c, df.p, dd.p, n);
     const int budget = maxit > 0 ? maxit : 50 * n;
     int total = 0, conv = 0;
-    double rel = flen > 0.0 ? 1.0 : 0.0;
+    double rel = scale > 0.0 || c.npen ? 1.0 : 0.0;
     for (int k = 0; k < 50 && total < budget; k++) {
       if (rel <= rtol) {
         conv = 1;
@@ -95,9 +122,13 @@ int fgpu_linear_static(fgpu_ctx *ctx, const double *f, double *u, int maxit, dou
       launch_eq_to_nodal(c, du.p, c.d_u);
       if (!int_force_dev(c, c.d_u, dg.p)) return 0;
       launch_axpby(c, n, 1.0, df.p, -1.0, dg.p, dr.p);
+      launch_zero_penalty(c, dr.p);
+      if (!(scale > 0.0)) scale = element_force_len(c, c.d_u, dd.p, n);
       const double prev = rel;
-      rel = std::sqrt(dev_dot(c, dr.p, dr.p, n)) / flen;
-      if (rel > 0.9 * prev) break;  // at the floor of the FP64 residual
+      rel = scale > 0.0 ? std::sqrt(dev_dot(c, dr.p, dr.p, n)) / scale : 0.0;
+      // at the floor of the FP64 residual (a first solve with prescribed
+      // equations is dominated by their penalty rows and does not count)
+      if (rel > 0.9 * prev && (k > 0 || !c.npen)) break;
     }
     if (!conv && rel <= rtol) conv = 1;
     FGPU_CHECK(hipMemcpyAsync(u, du.p, nb, hipMemcpyDeviceToHost, c.stream));
@@ -129,8 +160,8 @@ int fgpu_linear_static_direct(fgpu_ctx *ctx, const double *f, double *u, double 
     if (!band_factor(c)) return 0;
     // refinement as in fgpu_linear_static, each correction by the factors of
     // the matrix assembled from FP32 element matrices
-    const double flen = std::sqrt(dev_dot(c, df.p, df.p, n));
-    double rel = flen > 0.0 ? 1.0 : 0.0;
+    double scale = free_len(c, df.p, dd.p, n);
+    double rel = scale > 0.0 || c.npen ? 1.0 : 0.0;
     int k = 0, conv = 0;
     for (; k < 50; k++) {
       if (rel <= rtol) {
@@ -142,9 +173,11 @@ int fgpu_linear_static_direct(fgpu_ctx *ctx, const double *f, double *u, double 
       launch_eq_to_nodal(c, du.p, c.d_u);
       if (!int_force_dev(c, c.d_u, dg.p)) return 0;
       launch_axpby(c, n, 1.0, df.p, -1.0, dg.p, dr.p);
+      launch_zero_penalty(c, dr.p);
+      if (!(scale > 0.0)) scale = element_force_len(c, c.d_u, dd.p, n);
       const double prev = rel;
-      rel = std::sqrt(dev_dot(c, dr.p, dr.p, n)) / flen;
-      if (rel > 0.9 * prev) {  // at the floor of the FP64 residual
+      rel = scale > 0.0 ? std::sqrt(dev_dot(c, dr.p, dr.p, n)) / scale : 0.0;
+      if (rel > 0.9 * prev && (k > 0 || !c.npen)) {  // at the floor of the FP64 residual
         k++;
         break;
       }
@@ -172,7 +205,10 @@ int fgpu_newton(fgpu_ctx *ctx, const double *q, const fgpu_nr_params *prm, doubl
     const int n = c.neq;
     const size_t nb = sizeof(double) * n;
     DevVec dq(n), df(n), dg(n), dr(n), du(n), ddu(n), dpred(n), dlast(n), dinc(n), dgsave(n),
-        dr0(n), db2(prm->galerkin ? n : 1), dw1(prm->galerkin ? n : 1), dw2(prm->galerkin ? n : 1);
+        dr0(n), db2(prm->galerkin ? n : 1), dw1(prm->galerkin ? n : 1), dw2(prm->galerkin ? n : 1),
+        dfree(c.npen ? n : 1);
+    // the prescribed equations are left out of the convergence test
+    auto len = [&](const double *x) { return free_len(c, x, dfree.p, n); };
     bool k_valid = false;  // the context holds the tangent of the current state
     FGPU_CHECK(hipMemcpyAsync(dq.p, q, nb, hipMemcpyHostToDevice, c.stream));
     if (prm->u0_dev)
@@ -196,10 +232,10 @@ int fgpu_newton(fgpu_ctx *ctx, const double *q, const fgpu_nr_params *prm, doubl
       lam = prm->lambdas ? prm->lambdas[step - 1] : lam + prm->factor;
       st.lam = lam;
       launch_axpby(c, n, lam, dq.p, 0.0, nullptr, df.p);  // f = lambda q
-      const double flen = std::sqrt(dev_dot(c, df.p, df.p, n));
+      const double flen = len(df.p);
       launch_axpby(c, n, 1.0, df.p, -1.0, dg.p, dr.p);  // r = f - g(u_prev)
-      const double ref0 = flen > prm->tol ? flen : std::sqrt(dev_dot(c, dg.p, dg.p, n));
-      double err = std::sqrt(dev_dot(c, dr.p, dr.p, n)) / std::max(1.0, ref0);
+      const double ref0 = flen > prm->tol ? flen : len(dg.p);
+      double err = len(dr.p) / std::max(1.0, ref0);
       // predictor (dlast still holds the state the previous step started from)
       bool predicted = false;
       if (prm->predict) {
@@ -254,8 +290,8 @@ int fgpu_newton(fgpu_ctx *ctx, const double *q, const fgpu_nr_params *prm, doubl
         double errp = 0, work = 0;
         if (ok) {
           launch_axpby(c, n, 1.0, df.p, -1.0, dg.p, dr.p);
-          const double refp = flen > prm->tol ? flen : std::sqrt(dev_dot(c, dg.p, dg.p, n));
-          errp = std::sqrt(dev_dot(c, dr.p, dr.p, n)) / std::max(1.0, refp);
+          const double refp = flen > prm->tol ? flen : len(dg.p);
+          errp = len(dr.p) / std::max(1.0, refp);
           work = dev_dot(c, dpred.p, dr0.p, n) + dev_dot(c, dpred.p, dr.p, n);
         }
         if (ok && std::isfinite(errp) && work > 0) {
@@ -311,9 +347,9 @@ int fgpu_newton(fgpu_ctx *ctx, const double *q, const fgpu_nr_params *prm, doubl
         }
         st.ms_fint += c.last_ms[2];
         launch_axpby(c, n, 1.0, df.p, -1.0, dg.p, dr.p);
-        const double rlen = std::sqrt(dev_dot(c, dr.p, dr.p, n));
+        const double rlen = len(dr.p);
         double ref = flen;
-        if (!(flen > prm->tol)) ref = std::sqrt(dev_dot(c, dg.p, dg.p, n));
+        if (!(flen > prm->tol)) ref = len(dg.p);
         err = rlen / std::max(1.0, ref);
         if (err < prm->tol) conv = true;
       }

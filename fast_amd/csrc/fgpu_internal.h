@@ -109,6 +109,18 @@ struct KeLayout {
   }
 };
 
+// One group of uniform surface loads (fgpu_add_surface_loads), on the device
+struct SurfaceGroup {
+  int nface = 0, deg[2] = {1, 1}, nip = 2;
+  long long row0 = 0;         // first face-vector row of the group (faces x nodes)
+  int *d_nodes = nullptr;     // [nface * (p1+1)(p2+1)]
+  double *d_ext = nullptr;    // [nface * ((p1+1)^2 + (p2+1)^2)] or null (identity)
+  int *d_local = nullptr;     // [nface]
+  double *d_q = nullptr;      // [nface * 3]
+  double *d_gp = nullptr;     // [nip][2] Gauss points and weights (FP64)
+  std::vector<int> h_nodes;   // host copy (the gather lists are rebuilt from it)
+};
+
 struct Ctx {
   // ---- mesh (host copies)
   int nnode = 0, nelem = 0, shape = 0, nn = 0, form = 0;
@@ -235,6 +247,17 @@ struct Ctx {
   bool factored = false;       // d_band holds the factors of the current matrix
   int neg_pivots = 0;
 
+  // ---- external loads: surface-load groups, their face vectors [rows * 3]
+  // and the node -> face-vector-row lists (entry order) of their gather;
+  // prescribed equations with their penalty (node-direction index, value)
+  std::vector<SurfaceGroup> sgroups;
+  long long nsrow = 0;
+  double *d_sfv = nullptr;
+  int *d_sptr = nullptr, *d_sadj = nullptr;
+  int npen = 0;
+  int *d_pnd = nullptr;
+  double *d_pval = nullptr;
+
   // ---- the device's fixed scratch (k_scratch.hip), looked up at creation
   void *scratch = nullptr;
 
@@ -279,6 +302,12 @@ void launch_assemble_K(Ctx &c);
 std::vector<int> hex8_te_options(const Ctx &c);     // valid hex8 K workgroup sizes
 std::vector<int> assembly_options(const Ctx &c);    // valid asm_mode values
 void launch_gather_fint(Ctx &c, double *d_g);
+// external loads (k_load.hip)
+void launch_surface_group(Ctx &c, const SurfaceGroup &g, double *fv);
+void launch_gather_surface(Ctx &c, double *f);
+void launch_penalty_diag(Ctx &c);                               // after the assembly
+void launch_penalty_force(Ctx &c, const double *u, double *g);  // after the f_int gather
+void launch_zero_penalty(Ctx &c, double *x);  // x = 0 at the prescribed equations
 void launch_spmv_nodal(Ctx &c, const double *x, double *y);  // node vectors
 void launch_csr_convert(Ctx &c, double *d_csrval, int to_csr);
 // abs_exit: the reference's early return when |b| < rtol (an absolute

@@ -1,6 +1,7 @@
 """Run a FAST neutral-file deck on the device: read (fast_amd.neutral), form
-the load vector (nodal and face loads on the host, body forces on the
-device), solve (linear static, or load-control Newton-Raphson), compute the
+the load vector (nodal forces from the deck, surface and body loads
+integrated on the device, the penalty force of prescribed displacements),
+solve (linear static, or load-control Newton-Raphson), compute the
 stresses on the device, and write displacements and stresses in the
 reference's result layout.
 
@@ -18,6 +19,37 @@ from .fgpu import FgpuError, GpuModel
 
 
 MAX_DIRECT_BAND = 16384  # widest half band of the device direct solve
+PENALTY_STIFFNESS = 1.0e30  # main/gbldef.h:148
+
+
+def prepare(d, model):
+    """Hand the deck's surface loads and prescribed equations to the model:
+    the surface groups are uploaded for device integration and every
+    prescribed equation gets the penalty stiffness on its diagonal and the
+    penalty force in the internal force (ctrl/ctrl.cpp:958-974, 794-810)."""
+    model.clear_surface_loads()
+    for g in d.surfaces:
+        model.add_surface_loads(g)
+    if d.prescribed is not None:
+        eq, _ = d.prescribed
+        model.set_penalty(eq, np.full(len(eq), PENALTY_STIFFNESS))
+
+
+def load_vector(d, model):
+    """The deck's external load vector, cControl::ExtLoadVector
+    (ctrl/ctrl.cpp:666-716) at t = 1: nodal forces, the surface loads and
+    body forces integrated on the device, and the penalty force
+    PENALTY_STIFFNESS * value of each prescribed equation.  ``prepare`` must
+    have been called on the model."""
+    f = d.load.copy()
+    if d.surfaces:
+        f += model.surface_load()
+    if d.body is not None:
+        f += model.body_force(d.body)
+    if d.prescribed is not None:
+        for eq, v in zip(*d.prescribed):
+            f[eq - 1] += PENALTY_STIFFNESS * v
+    return f
 
 
 def run(path: str, out: str | None = None, tol: float = 1e-9, stresses: bool = True,
@@ -28,18 +60,24 @@ def run(path: str, out: str | None = None, tol: float = 1e-9, stresses: bool = T
 
     solver: "direct" (the device LDL^T, the deck's default skyline matrix),
     "pcg" (the deck's sparse matrix types), or None to follow the deck; a
-    band wider than the direct solve allows falls back to PCG."""
+    band wider than the direct solve allows falls back to PCG.  Decks with
+    prescribed displacements use the direct solve (the penalty diagonal
+    leaves the PCG's residual test to the prescribed equations)."""
     d = neutral.read(path)
     model = GpuModel(d.mesh)
+    prepare(d, model)
     if stresses and not d.stress_extrapolation:
         model.set_stress_nodes(False)  # HEADER.OUTPUT.STRESS.EXTRAPOLATION 0
     if solver is None:
         solver = "direct" if d.matrix_type.endswith("skyline") else "pcg"
-    if solver == "direct" and model.factor_info()[0] > MAX_DIRECT_BAND:
+    if d.prescribed is not None:
+        solver = "direct"
+        if model.factor_info()[0] > MAX_DIRECT_BAND:
+            raise FgpuError("prescribed displacements need the direct solve; the band is "
+                            "wider than %d" % MAX_DIRECT_BAND)
+    elif solver == "direct" and model.factor_info()[0] > MAX_DIRECT_BAND:
         solver = "pcg"
-    f = d.load.copy()
-    if d.body is not None:
-        f += model.body_force(d.body)
+    f = load_vector(d, model)
     if d.analysis == "linear":
         if solver == "direct":
             u, it, rr = model.linear_static_direct(f, tol=tol)

@@ -39,6 +39,12 @@ PREDICT_FN = C.CFUNCTYPE(C.c_int, C.c_void_p, C.c_int, C.c_void_p, C.c_double, C
                          C.c_void_p, C.c_double, C.c_void_p)
 
 
+class FgpuSurface(C.Structure):
+    _fields_ = [("nface", C.c_int), ("deg", C.c_int * 2), ("nip", C.c_int),
+                ("nodes", C.c_void_p), ("ext", C.c_void_p), ("local", C.c_void_p),
+                ("q", C.c_void_p)]
+
+
 class NrParams(C.Structure):
     _fields_ = [("steps", C.c_int), ("factor", C.c_double), ("maxit", C.c_int),
                 ("tol", C.c_double), ("pcg_maxit", C.c_int), ("pcg_tol", C.c_double),
@@ -104,6 +110,11 @@ EXPORTS = {
     "fgpu_stresses_dev": (C.c_int, [C.c_void_p] + [C.c_void_p] * 4),
     "fgpu_set_launch": (C.c_int, [C.c_void_p, C.c_int, C.c_int]),
     "fgpu_set_overlap": (C.c_int, [C.c_void_p, C.c_int]),
+    "fgpu_add_surface_loads": (C.c_int, [C.c_void_p, C.c_void_p]),
+    "fgpu_clear_surface_loads": (C.c_int, [C.c_void_p]),
+    "fgpu_surface_load": (C.c_int, [C.c_void_p, C.c_void_p]),
+    "fgpu_surface_load_dev": (C.c_int, [C.c_void_p, C.c_void_p]),
+    "fgpu_set_penalty": (C.c_int, [C.c_void_p, C.c_int, C.c_void_p, C.c_void_p]),
     "fgpu_tuning_report": (C.c_char_p, [C.c_void_p]),
     "fgpu_last_time_ms": (C.c_double, [C.c_void_p, C.c_int]),
     "fgpu_device_buffer": (C.c_void_p, [C.c_void_p, C.c_int]),
@@ -397,10 +408,41 @@ class GpuModel:
         """Fix the launch configuration (replaces the autotuner)."""
         self._check(lib().fgpu_set_launch(self.h, hex8_te, asm_mode), "fgpu_set_launch")
 
+    def add_surface_loads(self, group):
+        """Append a group of surface loads (a SurfaceGroup of fast_amd.mesh:
+        deg, nip, nodes [nface, (p1+1)(p2+1)], ext or None, local, q)."""
+        nodes = np.ascontiguousarray(group.nodes, dtype=np.int32)
+        ext = None if group.ext is None else np.ascontiguousarray(group.ext, dtype=np.float64)
+        local = (None if group.local is None
+                 else np.ascontiguousarray(group.local, dtype=np.int32))
+        q = np.ascontiguousarray(group.q, dtype=np.float64)
+        s = FgpuSurface()
+        s.nface = len(nodes)
+        s.deg[0], s.deg[1] = int(group.deg[0]), int(group.deg[1])
+        s.nip = int(group.nip)
+        s.nodes, s.ext, s.local, s.q = _p(nodes), _p(ext), _p(local), _p(q)
+        self._check(lib().fgpu_add_surface_loads(self.h, C.byref(s)), "fgpu_add_surface_loads")
+
+    def clear_surface_loads(self):
+        self._check(lib().fgpu_clear_surface_loads(self.h), "fgpu_clear_surface_loads")
+
+    def surface_load(self):
+        """Equation vector of the context's surface loads (device integration)."""
+        f = np.zeros(self.neq)
+        self._check(lib().fgpu_surface_load(self.h, _p(f)), "fgpu_surface_load")
+        return f
+
+    def set_penalty(self, eq, v):
+        """Prescribed equations (1-based) and their penalty (fgpu_set_penalty)."""
+        eq = np.ascontiguousarray(eq, dtype=np.int32)
+        v = np.ascontiguousarray(v, dtype=np.float64)
+        self._check(lib().fgpu_set_penalty(self.h, len(eq), _p(eq), _p(v)), "fgpu_set_penalty")
+
     def set_overlap(self, mode):
         """Order of stiffness_and_int_force: 0 serial, 1 f_int beside the
-        assembly (fused point records where the kernels have them), -1 timed
-        on the first call."""
+        assembly (fused point records where the kernels have them), 2 only
+        f_int's element vectors beside the assembly and its gather after it,
+        -1 timed on the first call."""
         self._check(lib().fgpu_set_overlap(self.h, mode), "fgpu_set_overlap")
 
     def block_counts(self):

@@ -99,6 +99,37 @@ class Mesh:
         return u
 
 
+@dataclasses.dataclass
+class SurfaceGroup:
+    """Uniform surface loads on faces of one degree and Gauss rule
+    (``%LOAD.CASE.AREA.UNIFORM``, load/load.cpp:1440-1500, and
+    ``%LOAD.CASE.AREA.IGA.UNIFORM``, load/loadiga.cpp:305-354).  Each face is
+    a tensor-product surface of degree ``deg`` over (p1+1)(p2+1) nodes, the
+    first surface parameter fastest; ``ext`` holds its Bezier extraction
+    rows of the two parameters, (p1+1)^2 then (p2+1)^2 per face, or is None
+    for brick faces (degree (1, 1), nodes n0 n1 n3 n2 of the face).
+    ``local[f]`` = 1 gives the load in the face's local axes, ``q`` is the
+    load per unit area."""
+    deg: tuple
+    nip: int
+    nodes: np.ndarray                  # (nface, nb) int32
+    q: np.ndarray                      # (nface, 3) float64
+    ext: Optional[np.ndarray] = None   # (nface, (p1+1)^2 + (p2+1)^2) float64
+    local: Optional[np.ndarray] = None  # (nface,) int32
+
+    def __post_init__(self):
+        self.nodes = np.ascontiguousarray(self.nodes, dtype=np.int32)
+        self.q = np.ascontiguousarray(self.q, dtype=np.float64).reshape(-1, 3)
+        if self.ext is not None:
+            self.ext = np.ascontiguousarray(self.ext, dtype=np.float64).reshape(len(self.q), -1)
+        if self.local is not None:
+            self.local = np.ascontiguousarray(self.local, dtype=np.int32)
+
+    @property
+    def nface(self) -> int:
+        return len(self.q)
+
+
 def number_dofs(conn: np.ndarray, fixed: np.ndarray, nnode: int) -> np.ndarray:
     """Equation numbering of ctrl/ctrl.cpp:2004-2104 (InitDofs + CompEqs)."""
     active = np.zeros((nnode, 3), dtype=bool)

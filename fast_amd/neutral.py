@@ -2,14 +2,15 @@
 runs, and writer of its nodal-displacement result section (``.pos``).
 
 Supported: solid 8-node bricks (``ELEMENT.BRICK8`` / ``ELEMENT.BRICK8TL``) and
-B-spline solids of degree (3,3,3) or (3,3,2) (``ELEMENT.BSPLINE.SOLID`` /
+B-spline solids of degree up to 7 per direction (``ELEMENT.BSPLINE.SOLID`` /
 ``...SOLIDTL``) with isotropic elasticity (``MATERIAL.ISOTROPIC`` through
 ``SECTION.HOMOGENEOUS.ISOTROPIC.3D``), supports, the solver node order,
-uniform face loads on bricks in global or local axes, body forces and
-nodal forces, linear static and load-control Newton analyses, and the
-stress output of the result file.  Any other keyword that changes the analysis raises
-``NotImplementedError`` naming it; keywords that only steer printing are
-skipped.
+uniform face loads on bricks and on B-spline patch faces in global or local
+axes (read into surface groups that the device integrates), body forces,
+nodal forces, prescribed displacements (the penalty method), linear static
+and load-control Newton analyses, and the stress output of the result file.
+Any other keyword that changes the analysis raises ``NotImplementedError``
+naming it; keywords that only steer printing are skipped.
 
 Formats follow the reference's readers (main/input.cpp:70-330 keyword table):
 NODE.COORD / NODE.SUPPORT / CONTROL.POINT.WEIGHT / NODE.SOLVER.ORDER
@@ -30,7 +31,7 @@ from typing import Dict, List, Optional
 
 import numpy as np
 
-from .mesh import Mesh, bezier_extraction
+from .mesh import Mesh, SurfaceGroup, bezier_extraction
 
 # keywords that only control printing / output; their data are skipped
 _OUTPUT_ONLY = {
@@ -52,11 +53,14 @@ _HEX8_FACES = ((0, 1, 2, 3), (3, 2, 6, 7), (1, 5, 6, 2), (0, 4, 5, 1), (0, 3, 7,
 
 @dataclasses.dataclass
 class Deck:
-    """A parsed deck: the mesh handed to the device, the load vector in
-    equation order, and the node labels (row i of every nodal array is the
-    node labelled labels[i])."""
+    """A parsed deck: the mesh handed to the device, its loads and the node
+    labels (row i of every nodal array is the node labelled labels[i]).  The
+    loads are the nodal forces in equation order (``load``), the surface
+    loads (``surfaces``, integrated on the device) and the body forces
+    (``body``, integrated on the device); fast_amd.deck.load_vector sums
+    them."""
     mesh: Mesh
-    load: np.ndarray
+    load: np.ndarray                      # nodal forces, [neq]
     labels: np.ndarray
     title: str = ""
     analysis: str = "linear"              # "linear" or "newton" (load control)
@@ -69,6 +73,9 @@ class Deck:
     stress_extrapolation: bool = True     # HEADER.OUTPUT.STRESS.EXTRAPOLATION
     average_stress: bool = False          # HEADER.PRINT.AVERAGE.NODAL.STRESS
     matrix_type: str = "skyline"          # HEADER.ANALYSIS.MATRIX.TYPE
+    surfaces: List[SurfaceGroup] = dataclasses.field(default_factory=list)
+    # LOAD.CASE.PRESCRIBED.DOF: (equations, 1-based, and values) in deck order
+    prescribed: Optional[tuple] = None
 
 
 def _sections(text: str) -> Dict[str, List[str]]:
@@ -242,15 +249,33 @@ def read(path: str) -> Deck:
                 q = dofmap[i, d]
                 if q > 0:
                     f[q - 1] += v[d]
+    surfaces = []
     if "LOAD.CASE.AREA.UNIFORM" in sec:
         if mesh.shape != "hex8":
             raise NotImplementedError("face loads on B-spline solids")
-        f += _face_loads(mesh, sec["LOAD.CASE.AREA.UNIFORM"], idx)
+        surfaces.append(_face_group(mesh, sec["LOAD.CASE.AREA.UNIFORM"], idx))
     if "LOAD.CASE.AREA.IGA.UNIFORM" in sec:
         if mesh.shape != "bsp":
             raise NotImplementedError("IGA face loads need B-spline solids")
         elab = {r[0]: k for k, r in enumerate(rows)}
-        f += _iga_face_loads(mesh, sec["LOAD.CASE.AREA.IGA.UNIFORM"], elab, iga)
+        surfaces.append(_iga_face_group(mesh, sec["LOAD.CASE.AREA.IGA.UNIFORM"], elab, iga))
+    prescribed = None
+    if "LOAD.CASE.PRESCRIBED.DOF" in sec:
+        # cPrescDof::ReadPrescDofs (load/prescdof.cpp:35-70): node, direction
+        # (1-based), value; the direction must hold an equation
+        # (GetPrescDofs, prescdof.cpp:113-128)
+        tk = _Toks(sec["LOAD.CASE.PRESCRIBED.DOF"], "LOAD.CASE.PRESCRIBED.DOF")
+        eqs, vals = [], []
+        for _ in range(tk.int()):
+            lab, dr, v = tk.int(), tk.int(), tk.float()
+            if not 1 <= dr <= 3:
+                raise NotImplementedError("prescribed direction %d on a solid" % dr)
+            eq = int(dofmap[idx[lab], dr - 1])
+            if eq <= 0:
+                raise ValueError("invalid prescribed dof at node %d" % lab)
+            eqs.append(eq)
+            vals.append(v)
+        prescribed = (np.asarray(eqs, dtype=np.int32), np.asarray(vals))
     if "LOAD.CASE.BODY.FORCES" in sec:
         tk = _Toks(sec["LOAD.CASE.BODY.FORCES"], "LOAD.CASE.BODY.FORCES")
         body = np.zeros((mesh.nelem, 3))
@@ -258,7 +283,8 @@ def read(path: str) -> Deck:
             e = tk.int() - 1
             body[e] += [tk.float(), tk.float(), tk.float()]
     deck = Deck(mesh=mesh, load=f, labels=labels, title=title, body=body,
-                elem_labels=np.asarray([r[0] for r in rows]))
+                elem_labels=np.asarray([r[0] for r in rows]), surfaces=surfaces,
+                prescribed=prescribed)
     # stress output flags (ctrl/ctrl.cpp:354-385 ReadPrintAverageStress /
     # ReadStrExtFlag; extrapolation on and averages off by default)
     if "HEADER.OUTPUT.STRESS.EXTRAPOLATION" in sec:
@@ -294,7 +320,7 @@ _HANDLED = {"HEADER.ANALYSIS.ALGORITHM", "HEADER.ANALYSIS.MAXIMUM.STEPS",
             "ELEMENT.BRICK8", "ELEMENT.BRICK8TL", "ELEMENT.BSPLINE.SOLID",
             "ELEMENT.BSPLINE.SOLIDTL", "PATCH", "LOAD.CASE.AREA.UNIFORM",
             "LOAD.CASE.AREA.IGA.UNIFORM",
-            "LOAD.CASE.BODY.FORCES", "LOAD.CASE.NODAL.FORCE"}
+            "LOAD.CASE.BODY.FORCES", "LOAD.CASE.NODAL.FORCE", "LOAD.CASE.PRESCRIBED.DOF"}
 
 
 def _number_dofs(conn, fixed, nnode, node_order):
@@ -368,36 +394,21 @@ _IGA_FACES = {"Front": (2, 0), "Back": (2, 1), "Left": (0, 0), "Right": (0, 1),
               "Above": (1, 1), "Below": (1, 0)}
 
 
-def _bernstein(p, u):
-    """B_j(u) and dB_j/du of degree p (binomial form)."""
-    from math import comb
-    B = np.array([comb(p, j) * u ** j * (1 - u) ** (p - j) for j in range(p + 1)])
-    if p == 0:
-        return B, np.zeros(1)
-    lo = np.array([comb(p - 1, j) * u ** j * (1 - u) ** (p - 1 - j) for j in range(p)])
-    dB = p * (np.concatenate([[0.0], lo]) - np.concatenate([lo, [0.0]]))
-    return B, dB
-
-
-def _iga_face_loads(mesh, toks, elab, iga):
-    """Uniform loads on a face of a B-spline solid patch (cFaceUnifLoadIGA,
-    load/loadiga.cpp:305-354): the face is the patch's boundary surface
-    (geom/subpat.cpp CreateSubPatch), taken over the element's knot spans in
-    its two parameters; integrated with max(K) x max(K) Gauss points
-    (cElmParam::GetIntOrd, elc/elmparam.cpp:349-360; cElemLoad::ExtForce,
-    load/load.cpp:618-691) with |x_r x x_s| as the area factor
-    (cShapeSurf::DrvShpXYZ, shp/shpsurf.cpp:56-116) and, for Local = 1, the
-    face frame of cShapeSurf::LocalSys (shpsurf.cpp:121-164)."""
+def _iga_face_group(mesh, toks, elab, iga):
+    """Uniform loads on faces of a B-spline solid patch (cFaceUnifLoadIGA,
+    load/loadiga.cpp:305-354) as one surface group: each face is the patch's
+    boundary surface (geom/subpat.cpp CreateSubPatch) over the element's
+    knot spans in its two other parameters (the lower one first), with
+    those parameters' extraction rows, integrated on max(K) x max(K) Gauss
+    points (cElmParam::GetIntOrd, elc/elmparam.cpp:349-360)."""
     pats, einfo = iga
     tk = _Toks(toks, "LOAD.CASE.AREA.IGA.UNIFORM")
-    f = np.zeros(mesh.neq)
-    n = max(mesh.nip)
-    r, w = gauss_table(n)
+    nodes, ext, local, q, deg = [], [], [], [], None
     for _ in range(tk.int()):
         e = elab[tk.int()]
         face = tk.next().strip("'")
-        local = tk.int()
-        qv = np.asarray([tk.float(), tk.float(), tk.float()])
+        local.append(tk.int())
+        q.append([tk.float(), tk.float(), tk.float()])
         if face not in _IGA_FACES:
             raise ValueError("unknown face %s" % face)
         fd, end = _IGA_FACES[face]
@@ -405,116 +416,45 @@ def _iga_face_loads(mesh, toks, elab, iga):
         bases, nb, cps = pats[pid]
         d1, d2 = [d for d in range(3) if d != fd]
         p1, p2 = bases[d1][0], bases[d2][0]
+        if deg is None:
+            deg = (p1, p2)
+        elif deg != (p1, p2):
+            raise NotImplementedError("IGA face loads of different degrees in one deck")
         fixed = 0 if end == 0 else nb[fd] - 1
-        nodes = np.zeros((p1 + 1, p2 + 1), dtype=np.int64)
-        for i in range(p1 + 1):
-            for j in range(p2 + 1):
+        fn = []
+        for j in range(p2 + 1):      # first surface parameter fastest
+            for i in range(p1 + 1):
                 ijk = [0, 0, 0]
                 ijk[fd], ijk[d1], ijk[d2] = fixed, first[d1] + i, first[d2] + j
-                nodes[i, j] = cps[ijk[0] + nb[0] * (ijk[1] + nb[1] * ijk[2])]
-        X = mesh.coords[nodes]                         # [p1+1, p2+1, 3]
-        W = mesh.cpw[nodes] if mesh.cpw is not None else np.ones(nodes.shape)
-        for a in range(n):
-            B1, dB1 = _bernstein(p1, (r[a] + 1.0) / 2.0)
-            N1, dN1 = cs[d1] @ B1, 0.5 * (cs[d1] @ dB1)
-            for b in range(n):
-                B2, dB2 = _bernstein(p2, (r[b] + 1.0) / 2.0)
-                N2, dN2 = cs[d2] @ B2, 0.5 * (cs[d2] @ dB2)
-                num = np.outer(N1, N2) * W
-                sw = num.sum()
-                R = num / sw
-                dR1 = (np.outer(dN1, N2) * W - R * (np.outer(dN1, N2) * W).sum()) / sw
-                dR2 = (np.outer(N1, dN2) * W - R * (np.outer(N1, dN2) * W).sum()) / sw
-                x1 = np.einsum("ij,ijk->k", dR1, X)
-                x2 = np.einsum("ij,ijk->k", dR2, X)
-                dA = np.linalg.norm(np.cross(x1, x2)) * w[a] * w[b]
-                load = qv
-                if local:
-                    uu = x1 / np.linalg.norm(x1)
-                    ww = np.cross(uu, x2 / np.linalg.norm(x2))
-                    vv = np.cross(ww, uu)
-                    load = qv[0] * uu + qv[1] * vv + qv[2] * ww
-                for i in range(p1 + 1):
-                    for j in range(p2 + 1):
-                        if R[i, j] == 0.0:
-                            continue
-                        for d in range(3):
-                            eq = mesh.dofmap[nodes[i, j], d]
-                            if eq > 0:
-                                f[eq - 1] += R[i, j] * dA * load[d]
-    return f
+                fn.append(cps[ijk[0] + nb[0] * (ijk[1] + nb[1] * ijk[2])])
+        nodes.append(fn)
+        ext.append(np.concatenate([np.ravel(cs[d1]), np.ravel(cs[d2])]))
+    return SurfaceGroup(deg=deg, nip=max(mesh.nip), nodes=np.asarray(nodes), q=np.asarray(q),
+                        ext=np.asarray(ext), local=np.asarray(local))
 
 
-def gauss_table(n):
-    """Gauss-Legendre points and weights rounded to 15 decimals, the values of
-    the reference's quadrature table (elc/intpoint.cpp:741-920), as the
-    device kernels use them: Newton's method on P_n in 40-digit decimal
-    arithmetic from the double-precision roots, then rounded."""
-    import decimal
-    ctx = decimal.Context(prec=40)
-    r0, _ = np.polynomial.legendre.leggauss(n)
-    rs, ws = [], []
-    for x0 in r0:
-        x = ctx.create_decimal(repr(float(x0)))
-        for _ in range(8):
-            p0, p1 = ctx.create_decimal(1), x
-            for k in range(2, n + 1):
-                p0, p1 = p1, ctx.divide(ctx.subtract(ctx.multiply((2 * k - 1) * x, p1),
-                                                     ctx.multiply(k - 1, p0)), k)
-            if n == 1:
-                p0, p1 = ctx.create_decimal(1), x
-            dp = ctx.divide(ctx.multiply(n, ctx.subtract(ctx.multiply(x, p1), p0)),
-                            ctx.subtract(ctx.multiply(x, x), 1))
-            x = ctx.subtract(x, ctx.divide(p1, dp))
-        wv = ctx.divide(2, ctx.multiply(ctx.subtract(1, ctx.multiply(x, x)), ctx.multiply(dp, dp)))
-        rs.append(float(round(x, 15)) + 0.0)
-        ws.append(float(round(wv, 15)))
-    return np.array(rs), np.array(ws)
-
-
-def _face_loads(mesh, toks, idx):
-    """Uniform face loads on bricks (cFaceUnifLoad, load/load.cpp:1440-1498;
-    integrated on the element's Gauss order over the bilinear face,
-    cElemLoad::ExtForce, load.cpp:618-691).  Local = 1: the load components
-    act along the face's local axes at each point (cShapeSurf::LocalSys,
-    shp/shpsurf.cpp:121-164): u = dx/dr / |dx/dr|, v' = dx/ds / |dx/ds|,
-    w = u x v' (not normalised), v = w x u, load = q0 u + q1 v + q2 w."""
+def _face_group(mesh, toks, idx):
+    """Uniform face loads on bricks (cFaceUnifLoad, load/load.cpp:1440-1498)
+    as one surface group: the face is the first of the brick's faces in
+    table order that holds the three given corners (cShapeBrick8::GetFace),
+    its nodes n0 n1 n2 n3 given as n0 n1 n3 n2 (the bilinear surface, first
+    parameter fastest), on the element's Gauss order per direction
+    (cElemLoad::ExtForce, load.cpp:618-691)."""
     tk = _Toks(toks, "LOAD.CASE.AREA.UNIFORM")
-    f = np.zeros(mesh.neq)
-    n = mesh.nip[0]
-    r, w = gauss_table(n)
+    nodes, local, q = [], [], []
     for _ in range(tk.int()):
         e = tk.int() - 1
         corners = {idx[tk.int()] for _ in range(3)}
-        local = tk.int()
-        qv = np.asarray([tk.float(), tk.float(), tk.float()])
+        local.append(tk.int())
+        q.append([tk.float(), tk.float(), tk.float()])
         cn = mesh.conn[e]
         face = [fc for fc in _HEX8_FACES if corners <= {int(cn[a]) for a in fc}]
         if not face:
             raise ValueError("face of element %d not found" % (e + 1))
-        nodes = [int(cn[a]) for a in face[0]]  # the first face in table order
-        X = mesh.coords[nodes]
-        for i in range(n):
-            for j in range(n):
-                s, t = r[i], r[j]
-                N = 0.25 * np.array([(1 - s) * (1 - t), (1 + s) * (1 - t), (1 + s) * (1 + t),
-                                     (1 - s) * (1 + t)])
-                ds = 0.25 * np.array([-(1 - t), (1 - t), (1 + t), -(1 + t)])
-                dt = 0.25 * np.array([-(1 - s), -(1 + s), (1 + s), (1 - s)])
-                xr, xs = ds @ X, dt @ X
-                dA = np.linalg.norm(np.cross(xr, xs)) * w[i] * w[j]
-                load = qv
-                if local:
-                    uu = xr / np.linalg.norm(xr)
-                    ww = np.cross(uu, xs / np.linalg.norm(xs))
-                    vv = np.cross(ww, uu)
-                    load = qv[0] * uu + qv[1] * vv + qv[2] * ww
-                for a, node in enumerate(nodes):
-                    for d in range(3):
-                        q = mesh.dofmap[node, d]
-                        if q > 0:
-                            f[q - 1] += N[a] * dA * load[d]
-    return f
+        n0, n1, n2, n3 = (int(cn[a]) for a in face[0])
+        nodes.append([n0, n1, n3, n2])
+    return SurfaceGroup(deg=(1, 1), nip=mesh.nip[0], nodes=np.asarray(nodes), q=np.asarray(q),
+                        local=np.asarray(local))
 
 
 STRESS_LABELS = ("STRESS_XX", "STRESS_YY", "STRESS_ZZ", "STRESS_XY", "STRESS_XZ", "STRESS_YZ")

@@ -321,6 +321,49 @@ int fgpu_gnn_newton_predictor(void *user, int step, const double *u_dev, double 
                               double lam, const double *u_hist_dev, double lam_hist,
                               double *du_dev);
 
+/* Uniform surface loads on element faces, integrated on the device
+ * (cFaceUnifLoad, load/load.cpp:1440-1500; cFaceUnifLoadIGA, load/loadiga.cpp
+ * 305-354; both through cElemLoad::ExtForce, load/load.cpp:618-691).  Each
+ * entry is one face: a tensor-product surface of degree (p1, p2) over
+ * (p1+1)(p2+1) nodes, the first surface parameter fastest, with the Bezier
+ * extraction rows of its two parameters (N_i = sum_j C_ij B_j, as `ext` of
+ * fgpu_mesh) and the context's control-point weights.  At nip x nip Gauss
+ * points (first parameter fastest, cIntPoint::CreateQuadPoints,
+ * elc/intpoint.cpp:266-300) f_a += w |x_,s x x_,t| R_a load, with
+ * load = q in global axes, or (local = 1) q0 u + q1 v + q2 w in the face's
+ * frame of cShapeSurf::LocalSys (shp/shpsurf.cpp:121-164): u = x_,s/|x_,s|,
+ * w = u x x_,t/|x_,t|, v = w x u.  A brick face is the degree-(1,1) surface
+ * over its four corners with identity extraction (ext NULL): the nodes of
+ * cShapeBrick8's face n0 n1 n2 n3 (shp/shpsolid.cpp:264-276) given as
+ * n0 n1 n3 n2.  An IGA face is the patch's boundary surface over the
+ * element's two spans. */
+typedef struct {
+  int nface;           /* faces of this group */
+  int deg[2];          /* degree per surface parameter, 1..7 */
+  int nip;             /* Gauss points per parameter, 1..16 */
+  const int *nodes;    /* [nface * (p1+1)(p2+1)] 0-based nodes */
+  const double *ext;   /* [nface * ((p1+1)^2 + (p2+1)^2)] row-major, or NULL
+                          (identity: the Bernstein polynomials are the basis) */
+  const int *local;    /* [nface] 1 = local axes, or NULL (all global) */
+  const double *q;     /* [nface * 3] load per unit area */
+} fgpu_surface;
+/* Append a group of surface loads to the context (uploaded once; the
+ * entries keep their order across groups); clear all of them. */
+int fgpu_add_surface_loads(fgpu_ctx *ctx, const fgpu_surface *loads);
+int fgpu_clear_surface_loads(fgpu_ctx *ctx);
+/* f = the equation vector of the context's surface loads (zero elsewhere),
+ * every node's face contributions summed in entry order (FP64). */
+int fgpu_surface_load(fgpu_ctx *ctx, double *f);
+int fgpu_surface_load_dev(fgpu_ctx *ctx, double *f_dev);
+
+/* Prescribed equations (cPrescDof) with the penalty of ctrl/ctrl.cpp
+ * 958-974 and 794-810: after this call every stiffness assembly adds v[k]
+ * to the diagonal of equation eq[k] (1-based) after the element
+ * contributions, and every internal force adds v[k] * u at that equation
+ * after the element gather (u the current nodal displacement; repeated
+ * equations add their values).  n = 0 removes the penalty. */
+int fgpu_set_penalty(fgpu_ctx *ctx, int n, const int *eq, const double *v);
+
 /* External load vector of uniform body forces, b [nelem * 3] (force per
  * unit volume of each element): f = sum_e sum_q w detJ N^T b_e on the
  * element's Gauss rule (cElemLoad::ExtForce with cBodyLoad, load/load.cpp

@@ -847,6 +847,130 @@ int orc_assemble_body(const orc_mesh *m, const double *b, int neq, double *f)
   return ok;
 }
 
+/* Uniform surface loads, one group, added into f in entry order.
+ * cElemLoad::ExtForce (load/load.cpp:618-691) over the face shape of
+ * cFaceUnifLoad (load.cpp:1440-1500: cShapeQ4Surf over the brick face's
+ * nodes, shp/shpsolid.cpp:264-276) or of cFaceUnifLoadIGA (loadiga.cpp
+ * 305-354: the patch's boundary surface over the element's spans):
+ *   points of cIntPoint::CreateQuadPoints (elc/intpoint.cpp:266-300: r
+ *   fastest, weight w_r w_s); coeff = weight * detJ * 1 with detJ =
+ *   |x_,r x x_,s| (cShapeSurf::DrvShpXYZ, shp/shpsurf.cpp:56-116);
+ *   load = q, or R q with R = [u v w] of cShapeSurf::LocalSys
+ *   (shpsurf.cpp:121-164: u, v the unit tangents, w = u x v, v = w x u)
+ *   when the entry is local (cFaceUnifLoad::GetVal, load.cpp:1487-1500);
+ *   fe += N^T (coeff load) (MultTAcc), then f[eq] += fe (AddGlobVec).
+ * A group without extraction rows and of degree (1, 1) is a brick face:
+ * its nodes are given first parameter fastest (n0 n1 n3 n2) and the face
+ * is integrated with the bilinear functions of cShapeQ4 (shp/shpplane.cpp
+ * 363-390) over n0 n1 n2 n3.  Otherwise the basis is the rational
+ * Bezier-extracted tensor product of the element's two parameters (the
+ * first fastest), as for the solid (shp/shpbsp.cpp:254-300, 985-1067). */
+int orc_surface_load(const orc_mesh *m, const orc_surface *s, double *f)
+{
+  const int p1 = s->deg[0], p2 = s->deg[1], n1 = p1 + 1, n2 = p2 + 1, nb = n1 * n2;
+  const int q4 = !s->ext && p1 == 1 && p2 == 1;
+  double r[16], wr[16];
+  if (nb > 64 || s->nip < 1 || s->nip > 16 || !orc_gauss(s->nip, r, wr)) return 0;
+  for (int e = 0; e < s->nface; e++) {
+    const int *nd = s->nodes + (size_t)e * nb;
+    const double *C1 = s->ext ? s->ext + (size_t)e * (n1 * n1 + n2 * n2) : NULL;
+    const double *C2 = C1 ? C1 + n1 * n1 : NULL;
+    const double *qv = s->q + 3 * (size_t)e;
+    const int loc = s->local && s->local[e];
+    double fe[3 * 64], X[3 * 64], W[64];
+    int perm[64];  /* shape-function index -> entry node index */
+    for (int a = 0; a < nb; a++) perm[a] = a;
+    if (q4) { perm[2] = 3; perm[3] = 2; }
+    for (int a = 0; a < nb; a++) {
+      const int n = nd[perm[a]];
+      for (int k = 0; k < 3; k++) X[3 * a + k] = m->coords[3 * (size_t)n + k];
+      W[a] = m->cpw ? m->cpw[n] : 1.0;
+    }
+    memset(fe, 0, sizeof(double) * 3 * nb);
+    for (int i = 0; i < s->nip; i++)
+      for (int j = 0; j < s->nip; j++) {
+        const double pr = r[j], ps = r[i], wgt = wr[j] * wr[i];
+        double N[64], dr[64], ds[64];
+        if (q4) {
+          N[0] = 0.25 * (1.0 - pr) * (1.0 - ps);
+          N[1] = 0.25 * (1.0 + pr) * (1.0 - ps);
+          N[2] = 0.25 * (1.0 + pr) * (1.0 + ps);
+          N[3] = 0.25 * (1.0 - pr) * (1.0 + ps);
+          dr[0] = -0.25 * (1.0 - ps); dr[1] = 0.25 * (1.0 - ps);
+          dr[2] = 0.25 * (1.0 + ps);  dr[3] = -0.25 * (1.0 + ps);
+          ds[0] = -0.25 * (1.0 - pr); ds[1] = -0.25 * (1.0 + pr);
+          ds[2] = 0.25 * (1.0 + pr);  ds[3] = 0.25 * (1.0 - pr);
+        } else {
+          static const double I[64] = {0};
+          double Nr[16], dNr[16], Ns[16], dNs[16], Id1[64], Id2[64];
+          const double *c1 = C1, *c2 = C2;
+          if (!c1) {  /* identity extraction: the Bernstein polynomials */
+            memcpy(Id1, I, sizeof(Id1));
+            memcpy(Id2, I, sizeof(Id2));
+            for (int a = 0; a < n1; a++) Id1[a * n1 + a] = 1.0;
+            for (int a = 0; a < n2; a++) Id2[a * n2 + a] = 1.0;
+            c1 = Id1;
+            c2 = Id2;
+          }
+          bsp_univariate(p1, c1, pr, Nr, dNr);
+          bsp_univariate(p2, c2, ps, Ns, dNs);
+          double sw = 0.0, swr = 0.0, sws = 0.0;
+          for (int b = 0; b < n2; b++)
+            for (int a = 0; a < n1; a++) {
+              const int k = a + n1 * b;
+              N[k] = Nr[a] * Ns[b] * W[k];
+              dr[k] = dNr[a] * Ns[b] * W[k];
+              ds[k] = Nr[a] * dNs[b] * W[k];
+              sw += N[k];
+              swr += dr[k];
+              sws += ds[k];
+            }
+          for (int k = 0; k < nb; k++) {
+            N[k] /= sw;
+            dr[k] = (dr[k] - N[k] * swr) / sw;
+            ds[k] = (ds[k] - N[k] * sws) / sw;
+          }
+        }
+        double J0[3] = {0, 0, 0}, J1[3] = {0, 0, 0};
+        for (int a = 0; a < nb; a++)
+          for (int k = 0; k < 3; k++) {
+            J0[k] += dr[a] * X[3 * a + k];
+            J1[k] += ds[a] * X[3 * a + k];
+          }
+        const double nrm[3] = {J0[1] * J1[2] - J0[2] * J1[1], J0[2] * J1[0] - J0[0] * J1[2],
+                               J0[0] * J1[1] - J0[1] * J1[0]};
+        const double detJ = sqrt(nrm[0] * nrm[0] + nrm[1] * nrm[1] + nrm[2] * nrm[2]);
+        double load[3] = {qv[0], qv[1], qv[2]};
+        if (loc) {
+          double u[3], v[3], w[3];
+          const double lu = sqrt(J0[0] * J0[0] + J0[1] * J0[1] + J0[2] * J0[2]);
+          const double lv = sqrt(J1[0] * J1[0] + J1[1] * J1[1] + J1[2] * J1[2]);
+          for (int k = 0; k < 3; k++) { u[k] = J0[k] / lu; v[k] = J1[k] / lv; }
+          w[0] = u[1] * v[2] - u[2] * v[1];
+          w[1] = u[2] * v[0] - u[0] * v[2];
+          w[2] = u[0] * v[1] - u[1] * v[0];
+          v[0] = w[1] * u[2] - w[2] * u[1];
+          v[1] = w[2] * u[0] - w[0] * u[2];
+          v[2] = w[0] * u[1] - w[1] * u[0];
+          for (int k = 0; k < 3; k++) load[k] = u[k] * qv[0] + v[k] * qv[1] + w[k] * qv[2];
+        }
+        const double coeff = wgt * detJ * 1.0;
+        for (int d = 0; d < 3; d++) {
+          const double cl = coeff * load[d];
+          for (int a = 0; a < nb; a++) fe[3 * a + d] += N[a] * cl;
+        }
+      }
+    for (int a = 0; a < nb; a++) {
+      const int n = nd[perm[a]];
+      for (int d = 0; d < 3; d++) {
+        const int eq = m->dofmap[3 * (size_t)n + d];
+        if (eq > 0) f[eq - 1] += fe[3 * a + d];
+      }
+    }
+  }
+  return 1;
+}
+
 /* Nodal stresses without extrapolation (HEADER.OUTPUT.STRESS.EXTRAPOLATION
  * 0): cElmParam::NodalStress (elc/elmparam.cpp:1313-1333) through PntStress
  * (1206-1290) at the nodes' natural coordinates: e = B u (the linear strain,

@@ -76,6 +76,21 @@ int  orc_elem_bodyforce(const orc_etype *et, const double *X, const double *ext,
 int  orc_assemble_body(const orc_mesh *m, const double *b, int neq, double *f);
 int  orc_assemble_fint(const orc_mesh *m, const double *u_nodal, int neq,
                        double *g, int nthreads, int elem_begin, int elem_end);
+/* Uniform surface loads on faces (cElemLoad::ExtForce over cFaceUnifLoad /
+ * cFaceUnifLoadIGA faces): one group of faces of degree deg, nip x nip
+ * Gauss points, nodes first surface parameter fastest, per-face extraction
+ * rows of the two parameters (NULL: a brick face of degree (1, 1)), local
+ * flags (NULL: global) and loads q [nface*3].  Adds into f [neq]. */
+typedef struct {
+  int nface;
+  int deg[2];
+  int nip;
+  const int *nodes;
+  const double *ext;
+  const int *local;
+  const double *q;
+} orc_surface;
+int  orc_surface_load(const orc_mesh *m, const orc_surface *s, double *f);
 /* Stresses: integration-point values (Voigt xx yy zz xy xz yz, points in
  * the brick rule's order), the extrapolation matrix to the element's nodes
  * [nn*nq], and the whole-mesh output (element-nodal and node-averaged values

@@ -55,6 +55,7 @@ def lib():
         L.orc_assemble_fint.argtypes = [vp, vp, i, vp, i, i, i]
         L.orc_elem_bodyforce.argtypes = [vp, vp, vp, vp, vp, vp]
         L.orc_assemble_body.argtypes = [vp, vp, i, vp]
+        L.orc_surface_load.argtypes = [vp, vp, vp]
         L.orc_elem_stress.argtypes = [vp, vp, vp, vp, vp, vp, vp]
         L.orc_stress_extrap.argtypes = [vp, vp]
         L.orc_stresses.argtypes = [vp, vp, vp, vp, vp]
@@ -105,6 +106,12 @@ def hex8_shape(r, s, t):
     N, dN = np.zeros(8), np.zeros((8, 3))
     lib().orc_hex8_shape(r, s, t, _p(N), _p(dN))
     return N, dN
+
+
+class OSurface(C.Structure):
+    _fields_ = [("nface", C.c_int), ("deg", C.c_int * 2), ("nip", C.c_int),
+                ("nodes", C.c_void_p), ("ext", C.c_void_p), ("local", C.c_void_p),
+                ("q", C.c_void_p)]
 
 
 class OracleMesh:
@@ -186,6 +193,27 @@ class OracleMesh:
         b = np.ascontiguousarray(b_elem, dtype=np.float64)
         ok = lib().orc_assemble_body(C.byref(self.om), _p(b), self.mesh.neq, _p(f))
         return f, ok
+
+    def surface_load(self, groups):
+        """Equation-ordered load vector of uniform surface loads, groups in
+        order (each with deg, nip, nodes [nface, nb] first parameter fastest,
+        ext [nface, (p1+1)^2 + (p2+1)^2] or None, local [nface] or None and
+        q [nface, 3]): cElemLoad::ExtForce over each face, load/load.cpp
+        618-691."""
+        f = np.zeros(self.mesh.neq)
+        for g in groups:
+            nodes = np.ascontiguousarray(g.nodes, dtype=np.int32)
+            ext = None if g.ext is None else np.ascontiguousarray(g.ext, dtype=np.float64)
+            loc = None if g.local is None else np.ascontiguousarray(g.local, dtype=np.int32)
+            q = np.ascontiguousarray(g.q, dtype=np.float64)
+            s = OSurface()
+            s.nface = len(q)
+            s.deg[0], s.deg[1] = int(g.deg[0]), int(g.deg[1])
+            s.nip = int(g.nip)
+            s.nodes, s.ext, s.local, s.q = _p(nodes), _p(ext), _p(loc), _p(q)
+            if not lib().orc_surface_load(C.byref(self.om), C.byref(s), _p(f)):
+                raise ValueError("surface load group rejected by the oracle")
+        return f
 
     def elem_stress(self, e, u_nodal):
         """Integration-point stresses of element e: [nq, 6] (Voigt xx yy zz xy

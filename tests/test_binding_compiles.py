@@ -28,7 +28,8 @@ def test_binding_compiles_against_reference_headers(tmp_path):
     nm = subprocess.run(["nm", "-C", "--undefined-only", obj], capture_output=True,
                         text=True).stdout
     assert "fgpu_create" in nm and "fgpu_assemble_K" in nm and "fgpu_int_force" in nm
-    assert "cPrescDof::CurrNodalVals" in nm and "cBspPat::getExtMat" in nm
+    assert "cPrescDof::GetPrescDofs" in nm and "fgpu_set_penalty" in nm
+    assert "cBspPat::getExtMat" in nm
 
 
 @pytest.mark.skipif(not os.path.exists(os.path.join(ROOT, "oracle", "_ref", "ref_gpu")),

@@ -113,12 +113,11 @@ def test_singular_matrix_is_reported(fg, oracle_mod):
 
 @pytest.mark.parametrize("name", ["ScordelisLoDeg3", "BeamB8"])
 def test_linear_static_direct_vs_reference_run(fg, name):
-    from fast_amd import neutral
+    from fast_amd import deck, neutral
     d = neutral.read(os.path.join(DECKS, name + ".dat"))
     g = fg.GpuModel(d.mesh)
-    f = d.load.copy()
-    if d.body is not None:
-        f += g.body_force(d.body)
+    deck.prepare(d, g)
+    f = deck.load_vector(d, g)
     # (the FP64 residual of the Scordelis-Lo shell bottoms out near 1.6e-10)
     u, k, rr = g.linear_static_direct(f, tol=1e-9)
     assert rr <= 1e-9 and k <= 30, (k, rr)
@@ -130,12 +129,11 @@ def test_linear_static_direct_vs_reference_run(fg, name):
 
 @pytest.mark.parametrize("name", ["BeamB8TL", "ScordelisLoDeg3TL"])
 def test_newton_direct_vs_reference_run(fg, oracle_mod, name):
-    from fast_amd import neutral
+    from fast_amd import deck, neutral
     d = neutral.read(os.path.join(DECKS, name + ".dat"))
     g = fg.GpuModel(d.mesh)
-    f = d.load.copy()
-    if d.body is not None:
-        f += g.body_force(d.body)
+    deck.prepare(d, g)
+    f = deck.load_vector(d, g)
     us, st, err = g.newton(f, steps=d.steps, factor=d.factor, maxit=d.maxit, tol=d.tol,
                            direct=True)
     assert err == "" and all(s["converged"] for s in st), err

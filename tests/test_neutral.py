@@ -13,13 +13,16 @@ from fast_amd import neutral
 DECKS = os.path.join(os.path.dirname(__file__), "golden", "decks")
 
 
-def test_testb8_model_and_face_load():
+def test_testb8_model_and_face_load(oracle_mod):
     d = neutral.read(os.path.join(DECKS, "TestB8.dat"))
     m = d.mesh
     assert m.shape == "hex8" and m.nelem == 1 and m.nnode == 8 and m.neq == 18
     assert m.nip == (2, 2, 2) and np.allclose(m.mat, [[200e9, 0.3]])
+    assert not d.load.any() and len(d.surfaces) == 1
+    g = d.surfaces[0]
+    assert g.deg == (1, 1) and g.nip == 2 and g.nodes.shape == (1, 4) and g.ext is None
     # 100 MPa on the 1 m x 1 m end face: a quarter on each corner, along x
-    fn = m.scatter(d.load)
+    fn = m.scatter(oracle_mod.OracleMesh(m).surface_load(d.surfaces))
     assert np.allclose(fn[[1, 2, 5, 6], 0], 25e6, rtol=1e-14)
     assert np.abs(fn).sum() == pytest.approx(100e6, rel=1e-14)
 
@@ -61,6 +64,8 @@ def oracle_solution(oracle_mod, d):
     """Nodal displacements of the deck's last step by the oracle."""
     om = oracle_mod.OracleMesh(d.mesh)
     f = d.load.copy()
+    if d.surfaces:
+        f += om.surface_load(d.surfaces)
     if d.body is not None:
         fb, ok = om.assemble_body(d.body)
         assert ok
