@@ -384,6 +384,16 @@ void enqueue_status_read(Ctx &c, const int *d_st, int n) {
   FGPU_CHECK(hipMemcpyAsync(c.h_status, d_st, n * sizeof(int), hipMemcpyDeviceToHost, c.stream));
 }
 
+int check_status(Ctx &c, const char *what) {
+  enqueue_status_read(c, c.d_status, 4);
+  FGPU_CHECK(hipStreamSynchronize(c.stream));
+  return check_status_values(c, what, c.h_status);
+}
+
+}  // namespace
+
+namespace fgpu {
+
 // status[4] of the element kernels (count_jacobian): a zero or non-finite
 // Jacobian, or negative Jacobians at some but not all points of the mesh
 // (inverted elements), fail; all negative is a left-handed parametrization,
@@ -408,15 +418,6 @@ int check_status_values(Ctx &c, const char *what, const int *st) {
   return 1;
 }
 
-int check_status(Ctx &c, const char *what) {
-  enqueue_status_read(c, c.d_status, 4);
-  FGPU_CHECK(hipStreamSynchronize(c.stream));
-  return check_status_values(c, what, c.h_status);
-}
-
-}  // namespace
-
-namespace fgpu {
 
 // Autotuner: times every valid launch configuration of the element-stiffness
 // and assembly kernels on this mesh and state (best of 3 runs each, HIP
@@ -471,6 +472,35 @@ void autotune(Ctx &c, const double *d_u) {
   std::snprintf(buf, sizeof buf, "picked assembly mode %d\n", pick);
   rep += buf;
   c.tune_report = rep;
+}
+
+// The stiffness of the nodal state d_u, enqueued only: status counters
+// cleared by a fill kernel (graph-safe), element kernel, assembly, penalty.
+// The autotuner must have run (it synchronises).
+void enqueue_stiffness(Ctx &c, const double *d_u) {
+  c.factored = false;
+  c.bval32_valid = false;
+  launch_fill(c, c.d_status, 4, 0);
+  if (c.shape == FGPU_HEX8)
+    launch_hex8_K(c, d_u);
+  else
+    launch_bsp_K(c, d_u);
+  launch_assemble_K(c);
+  launch_penalty_diag(c);
+}
+
+// The internal force of d_u into d_g, enqueued only, counting into the
+// second status array.
+void enqueue_int_force(Ctx &c, const double *d_u, double *d_g) {
+  std::swap(c.d_status, c.d_status2);
+  launch_fill(c, c.d_status, 4, 0);
+  if (c.shape == FGPU_HEX8)
+    launch_hex8_fint(c, d_u);
+  else
+    launch_bsp_fint(c, d_u);
+  launch_gather_fint(c, d_g);
+  launch_penalty_force(c, d_u, d_g);
+  std::swap(c.d_status, c.d_status2);
 }
 
 int stiffness_dev(Ctx &c, const double *d_u) {

@@ -69,6 +69,44 @@ double element_force_len(Ctx &c, const double *u_nodal, double *tmp, int n) {
   return std::sqrt(dev_dot(c, tmp, tmp, n));
 }
 
+// Graphs of one Newton iteration, captured on first use in a call and
+// replayed for the rest of it; destroyed with the call.
+struct IterGraphs {
+  hipGraphExec_t head = nullptr, tail = nullptr;
+  hipEvent_t ev[4] = {};
+  ~IterGraphs() {
+    if (head) (void)hipGraphExecDestroy(head);
+    if (tail) (void)hipGraphExecDestroy(tail);
+    for (hipEvent_t e : ev)
+      if (e) (void)hipEventDestroy(e);
+  }
+};
+
+// captures what enqueue() launches on the context stream into an
+// executable graph; a failure inside ends the capture before it propagates
+template <class F>
+void capture(Ctx &c, hipGraphExec_t &ge, F &&enqueue) {
+  hipGraph_t g = nullptr;
+  FGPU_CHECK(hipStreamBeginCapture(c.stream, hipStreamCaptureModeRelaxed));
+  try {
+    enqueue();
+  } catch (...) {
+    (void)hipStreamEndCapture(c.stream, &g);
+    if (g) (void)hipGraphDestroy(g);
+    throw;
+  }
+  FGPU_CHECK(hipStreamEndCapture(c.stream, &g));
+  const hipError_t e = hipGraphInstantiate(&ge, g, nullptr, nullptr, 0);
+  (void)hipGraphDestroy(g);
+  FGPU_CHECK(e);
+}
+
+float ev_ms(hipEvent_t a, hipEvent_t b) {
+  float ms = 0;
+  FGPU_CHECK(hipEventElapsedTime(&ms, a, b));
+  return ms;
+}
+
 // y = K x on equation vectors (PCG work vectors as scratch)
 void spmv_eq(Ctx &c, const double *x, double *y) {
   launch_eq_to_nodal(c, x, c.d_bn);
@@ -219,6 +257,36 @@ int fgpu_newton(fgpu_ctx *ctx, const double *q, const fgpu_nr_params *prm, doubl
     launch_eq_to_nodal(c, du.p, unod);
     if (!int_force_dev(c, unod, dg.p)) return 0;  // initial stresses
     const int pcg_maxit = prm->pcg_maxit > 0 ? prm->pcg_maxit : n;
+    // Captured iterations (prm->graph): after one eager iteration has done
+    // the lazy set-up (autotuning, scratch buffers), each Newton iteration
+    // is replayed from graphs -- with PCG: head = tangent + PCG start + its
+    // first CHUNK iterations, then PCG chunk graphs while the device's flag
+    // says running, then tail = update + f_int + residual + norms; with the
+    // direct solver one graph holds the whole iteration.  One host read per
+    // iteration (and one per PCG chunk) replaces the eager path's
+    // synchronisations; the kernels and their order are the eager path's.
+    const bool use_graph = prm->graph != 0 && c.use_graphs;
+    const int abs_exit = prm->adaptive_pcg == 0;
+    IterGraphs G;
+    bool warm = false;
+    double *d_tol = c.d_scal + 10;                      // PCG tolerance of the solve
+    double *d_rr = c.d_scal + 11, *d_gg = c.d_scal + 12;  // |r|^2, |g|^2 (free equations)
+    auto enqueue_tail = [&](bool pcg_x) {
+      if (pcg_x) launch_nodal_to_eq(c, c.d_x, ddu.p);
+      launch_axpby(c, n, 1.0, du.p, 1.0, ddu.p, du.p);  // u += du
+      launch_eq_to_nodal(c, du.p, unod);
+      enqueue_int_force(c, unod, dg.p);
+      launch_axpby(c, n, 1.0, df.p, -1.0, dg.p, dr.p);
+      for (int k = 0; k < 2; k++) {
+        const double *x = k ? dg.p : dr.p;
+        if (c.npen) {
+          launch_copy(c, x, dfree.p, n);
+          launch_zero_penalty(c, dfree.p);
+          x = dfree.p;
+        }
+        enqueue_dot(c, x, x, n, k ? d_gg : d_rr);
+      }
+    };
     const char *venv = std::getenv("FGPU_NR_VERBOSE");  // per-iteration log on stderr
     const bool verbose = venv && venv[0] == '1';
     double lam = prm->u0_dev ? prm->lam0 : 0.0, lam_last_step = lam;
@@ -313,15 +381,82 @@ int fgpu_newton(fgpu_ctx *ctx, const double *q, const fgpu_nr_params *prm, doubl
       bool conv = st.predicted == 1 && err < prm->tol;
       int it = 0;
       for (it = 1; it <= prm->maxit && !conv; it++) {
-        if (!stiffness_dev(c, unod)) return 0;
-        k_valid = true;
-        st.ms_stiffness += c.last_ms[0] + c.last_ms[1];
         double ptol = prm->pcg_tol;
         if (prm->adaptive_pcg == 1 && it > 1) ptol = std::min(ptol, std::max(1e-12, 0.1 * err));
         if (prm->adaptive_pcg == 2)
           ptol = std::min(prm->pcg_tol, std::max(1e-12, 0.5 * prm->tol / std::max(err, 1e-300)));
         int pit = 0;
         double prr = 0;
+        if (use_graph && warm) {
+          if (!G.head) {
+            for (hipEvent_t &e : G.ev) FGPU_CHECK(hipEventCreate(&e));
+            if (prm->direct) {
+              if (!band_setup(c)) return 0;
+              band_solve_setup(c);
+              capture(c, G.head, [&] {
+                enqueue_stiffness(c, unod);
+                enqueue_band_factor(c);
+                enqueue_band_solve(c, dr.p, ddu.p, c.d_b);
+                enqueue_tail(false);
+              });
+            } else {
+              capture(c, G.head, [&] {
+                enqueue_stiffness(c, unod);
+                enqueue_pcg_start(c, dr.p, d_tol, abs_exit);
+                enqueue_pcg_chunk(c);
+              });
+              capture(c, G.tail, [&] { enqueue_tail(true); });
+            }
+          }
+          k_valid = true;
+          if (!prm->direct) launch_fill(c, d_tol, 1, ptol);
+          FGPU_CHECK(hipEventRecord(G.ev[0], c.stream));
+          FGPU_CHECK(hipGraphLaunch(G.head, c.stream));
+          FGPU_CHECK(hipEventRecord(G.ev[1], c.stream));
+          if (!prm->direct) {
+            pcg_continue(c, pcg_maxit, ptol, abs_exit, &pit, &prr);
+            FGPU_CHECK(hipEventRecord(G.ev[2], c.stream));
+            FGPU_CHECK(hipGraphLaunch(G.tail, c.stream));
+          } else {
+            FGPU_CHECK(hipEventRecord(G.ev[2], c.stream));
+          }
+          FGPU_CHECK(hipEventRecord(G.ev[3], c.stream));
+          int info[2] = {0x7fffffff, 0};
+          FGPU_CHECK(hipMemcpyAsync(c.h_status, c.d_status, 8 * sizeof(int),
+                                    hipMemcpyDeviceToHost, c.stream));
+          if (prm->direct)
+            FGPU_CHECK(hipMemcpyAsync(info, c.d_finfo, sizeof info, hipMemcpyDeviceToHost,
+                                      c.stream));
+          FGPU_CHECK(hipMemcpyAsync(c.h_scal + 11, d_rr, 2 * sizeof(double),
+                                    hipMemcpyDeviceToHost, c.stream));
+          FGPU_CHECK(hipStreamSynchronize(c.stream));
+          const float t_head = ev_ms(G.ev[0], G.ev[1]), t_mid = ev_ms(G.ev[1], G.ev[2]),
+                      t_tail = ev_ms(G.ev[2], G.ev[3]);
+          st.ms_stiffness += t_head;  // the whole iteration with the direct solver
+          st.ms_solve += t_mid;
+          st.ms_fint += t_tail;
+          st.pcg_iters += pit;
+          if (!check_status_values(c, "stiffness", c.h_status)) return 0;
+          if (prm->direct && !band_factor_result(c, info)) return 0;
+          if (verbose)
+            std::fprintf(stderr, "fgpu_newton: step %d lambda %.6g iter %d |r|rel %.3e pcg_tol "
+                         "%.2e pcg %d (%.2f ms, graph)\n", step, lam, it, err, ptol, pit,
+                         t_head + t_mid + t_tail);
+          if (!check_status_values(c, "internal force", c.h_status + 4)) {
+            st.iters = it;
+            if (stats) stats[step - 1] = st;
+            return 0;
+          }
+          const double rlen = std::sqrt(c.h_scal[11]);
+          double ref = flen;
+          if (!(flen > prm->tol)) ref = std::sqrt(c.h_scal[12]);
+          err = rlen / std::max(1.0, ref);
+          if (err < prm->tol) conv = true;
+          continue;
+        }
+        if (!stiffness_dev(c, unod)) return 0;
+        k_valid = true;
+        st.ms_stiffness += c.last_ms[0] + c.last_ms[1];
         FGPU_CHECK(hipEventRecord(c.ev[5], c.stream));
         if (prm->direct) {
           if (!band_factor(c)) return 0;
@@ -345,6 +480,7 @@ int fgpu_newton(fgpu_ctx *ctx, const double *q, const fgpu_nr_params *prm, doubl
           if (stats) stats[step - 1] = st;
           return 0;
         }
+        warm = true;
         st.ms_fint += c.last_ms[2];
         launch_axpby(c, n, 1.0, df.p, -1.0, dg.p, dr.p);
         const double rlen = len(dr.p);

@@ -333,6 +333,22 @@ void bsp_prepare_tables(Ctx &c, const double *h_ext, const double *h_cpw);
 void launch_axpby(Ctx &c, int n, double a, const double *x, double b, const double *y,
                   double *z);
 double dev_dot(Ctx &c, const double *x, const double *y, int n);
+// graph-safe pieces (no host synchronisation, no memset nodes)
+void launch_fill(Ctx &c, double *p, long long n, double v);
+void launch_fill(Ctx &c, int *p, long long n, int v);
+void launch_copy(Ctx &c, const double *x, double *y, long long n);  // y = x
+void enqueue_dot(Ctx &c, const double *x, const double *y, int n, double *d_out);
+void enqueue_stiffness(Ctx &c, const double *d_unodal);     // status into d_status
+void enqueue_int_force(Ctx &c, const double *d_unodal, double *d_g);  // into d_status2
+int check_status_values(Ctx &c, const char *what, const int *st);
+int band_setup(Ctx &c);                 // width check, buffers; 0 with c.err
+void enqueue_band_factor(Ctx &c);       // info into d_finfo
+int band_factor_result(Ctx &c, const int *info);  // info read back: 1 ok, 0 singular
+void band_solve_setup(Ctx &c);
+void enqueue_band_solve(Ctx &c, const double *b, double *x, double *z);  // z: n scratch
+void enqueue_pcg_start(Ctx &c, const double *b, const double *d_tol, int abs_exit);
+void enqueue_pcg_chunk(Ctx &c);
+int pcg_continue(Ctx &c, int maxit, double rtol, int abs_exit, int *iters, double *relres);
 // element K + assembly / element f_int + gather on the context stream, with
 // the status check (returns 0 and sets c.err on failure)
 int stiffness_dev(Ctx &c, const double *d_unodal);

@@ -248,26 +248,36 @@ int band_width(Ctx &c) {
   return bw;
 }
 
-int band_factor(Ctx &c) {
+// Width check and buffers of the band factorization (host work; before any
+// graph capture).  0 with c.err when the band is too wide.
+int band_setup(Ctx &c) {
   const int n = c.neq, bw = band_width(c);
-  if (n == 0) {
-    c.factored = true;
-    return 1;
-  }
+  if (n == 0) return 1;
   if (bw > MAX_BW_SOLVE) {
     c.err = "direct solve: half bandwidth " + std::to_string(bw) + " above the limit " +
             std::to_string(MAX_BW_SOLVE) + " (renumber the equations or use PCG)";
     return 0;
   }
-  const size_t entries = (size_t)n * (bw + 1);
   if (!c.d_band) {
+    const size_t entries = (size_t)n * (bw + 1);
     FGPU_CHECK(hipMalloc(&c.d_band, entries * sizeof(double)));
     FGPU_CHECK(hipMalloc(&c.d_finfo, 2 * sizeof(int)));
   }
+  return 1;
+}
+
+// The factorization's launches only (graph-safe: fills by kernels): the
+// band is cleared and filled from the node blocks, then panels of NB
+// columns; d_finfo[0] = first tiny pivot (or INT_MAX), d_finfo[1] = the
+// number of negative pivots.
+void enqueue_band_factor(Ctx &c) {
+  const int n = c.neq, bw = c.band_bw;
+  if (n == 0) return;
+  const size_t entries = (size_t)n * (bw + 1);
   Band B{c.d_band, n, bw, (long long)bw + 1};
-  FGPU_CHECK(hipMemsetAsync(c.d_band, 0, entries * sizeof(double), c.stream));
-  FGPU_CHECK(hipMemsetD32Async(c.d_finfo, 0x7fffffff, 1, c.stream));
-  FGPU_CHECK(hipMemsetD32Async(c.d_finfo + 1, 0, 1, c.stream));
+  launch_fill(c, c.d_band, (long long)entries, 0.0);
+  launch_fill(c, c.d_finfo, 1, 0x7fffffff);
+  launch_fill(c, c.d_finfo + 1, 1, 0);
   hipLaunchKernelGGL(band_fill_kernel, dim3((c.nnode + 255) / 256), dim3(256), 0, c.stream,
                      c.nnode, c.d_bptr, c.d_bcol, c.d_bval, c.d_eqmap, B);
   FGPU_CHECK(hipGetLastError());
@@ -285,9 +295,9 @@ int band_factor(Ctx &c) {
     }
   }
   FGPU_CHECK(hipGetLastError());
-  int info[2];
-  FGPU_CHECK(hipMemcpyAsync(info, c.d_finfo, sizeof info, hipMemcpyDeviceToHost, c.stream));
-  FGPU_CHECK(hipStreamSynchronize(c.stream));
+}
+
+int band_factor_result(Ctx &c, const int *info) {
   c.neg_pivots = info[1];
   if (info[0] != 0x7fffffff) {
     c.factored = false;
@@ -299,24 +309,51 @@ int band_factor(Ctx &c) {
   return 1;
 }
 
-void band_solve(Ctx &c, const double *b, double *x) {
+int band_factor(Ctx &c) {
+  if (c.neq == 0) {
+    c.factored = true;
+    return 1;
+  }
+  if (!band_setup(c)) return 0;
+  enqueue_band_factor(c);
+  int info[2];
+  FGPU_CHECK(hipMemcpyAsync(info, c.d_finfo, sizeof info, hipMemcpyDeviceToHost, c.stream));
+  FGPU_CHECK(hipStreamSynchronize(c.stream));
+  return band_factor_result(c, info);
+}
+
+// the solves' LDS size for the current band (host work, before a capture)
+void band_solve_setup(Ctx &c) {
+  const size_t lds = sizeof(double) * (size_t)(c.band_bw + NB);
+  FGPU_CHECK(hipFuncSetAttribute((const void *)band_forward_kernel,
+                                 hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+  FGPU_CHECK(hipFuncSetAttribute((const void *)band_backward_kernel,
+                                 hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+}
+
+// forward and backward substitution, x = (L D L^T)^-1 b through z (n
+// doubles, distinct from b and x); launches only
+void enqueue_band_solve(Ctx &c, const double *b, double *x, double *z) {
   const int n = c.neq, bw = c.band_bw;
   if (n == 0) return;
   Band B{c.d_band, n, bw, (long long)bw + 1};
   const size_t lds = sizeof(double) * (size_t)(bw + NB);
+  hipLaunchKernelGGL(band_forward_kernel, dim3(1), dim3(1024), lds, c.stream, B, b, z);
+  hipLaunchKernelGGL(band_backward_kernel, dim3(1), dim3(1024), lds, c.stream, B, z, x);
+  FGPU_CHECK(hipGetLastError());
+}
+
+void band_solve(Ctx &c, const double *b, double *x) {
+  const int n = c.neq;
+  if (n == 0) return;
   double *z = c.d_b == b || c.d_b == x ? nullptr : c.d_b;
   double *tmp = nullptr;
   if (!z) {
     FGPU_CHECK(hipMalloc(&tmp, sizeof(double) * n));
     z = tmp;
   }
-  FGPU_CHECK(hipFuncSetAttribute((const void *)band_forward_kernel,
-                                 hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-  FGPU_CHECK(hipFuncSetAttribute((const void *)band_backward_kernel,
-                                 hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-  hipLaunchKernelGGL(band_forward_kernel, dim3(1), dim3(1024), lds, c.stream, B, b, z);
-  hipLaunchKernelGGL(band_backward_kernel, dim3(1), dim3(1024), lds, c.stream, B, z, x);
-  FGPU_CHECK(hipGetLastError());
+  band_solve_setup(c);
+  enqueue_band_solve(c, b, x, z);
   if (tmp) {
     FGPU_CHECK(hipStreamSynchronize(c.stream));
     (void)hipFree(tmp);

@@ -90,6 +90,33 @@ __global__ void pcg_start_kernel(const double *__restrict__ part, double *__rest
   }
 }
 
+// The start of a solve whose tolerance lives on the device (a replayed
+// graph): as pcg_start_kernel, and the immediate exits of run_pcg (|b| < tol
+// under abs_exit, |b| = 0, or the start already converged) mark done[0] = -1,
+// which turns every iteration kernel into a no-op.
+__global__ void pcg_start_dev_kernel(const double *__restrict__ part, double *__restrict__ scal,
+                                     int *__restrict__ done, const double *__restrict__ tolp,
+                                     int abs_exit) {
+  __shared__ double sh[16];
+  double zr = reduce_partials(part, RED, sh);
+  double bb = reduce_partials(part + RED, RED, sh);
+  double rr = reduce_partials(part + 2 * RED, RED, sh);
+  if (threadIdx.x == 0) {
+    const double tol = *tolp;
+    double lenf = sqrt(bb);
+    scal[4] = lenf;
+    if (lenf > 10e+10) lenf = 1.0;
+    scal[0] = lenf;
+    scal[1] = zr;
+    scal[3] = sqrt(rr) / lenf;
+    scal[6] = tol;
+    const bool early = (abs_exit && scal[4] < tol) || scal[4] == 0.0 || scal[3] < tol;
+    done[0] = early ? -1 : 0;
+    done[1] = 0;
+    done[2] = 0;
+  }
+}
+
 __global__ __launch_bounds__(256) void pcg_update_kernel(
     int n, double *__restrict__ x, double *__restrict__ r, double *__restrict__ z,
     const double *__restrict__ p, const double *__restrict__ w, const double *__restrict__ dinv,
@@ -416,6 +443,57 @@ void prepare(Ctx &c, const double *b, const double *x0) {
 }
 
 }  // namespace
+
+// Graph-safe start of a solve of K x = b from x0 = 0 (the Newton
+// corrections): no memset or copy nodes, the tolerance read from d_tol.
+void enqueue_pcg_start(Ctx &c, const double *b, const double *d_tol, int abs_exit) {
+  const int n = 3 * c.nnode;
+  const int eb = (n + 255) / 256;
+  hipStream_t s = c.stream;
+  if (c.mat_bits == 32) ensure_bval32(c);
+  hipLaunchKernelGGL(diag_inv_kernel, dim3(eb), dim3(256), 0, s, c.nnode, c.d_bdiag, c.d_bval,
+                     c.d_dinv);
+  launch_eq_to_nodal(c, b, c.d_bn);
+  launch_fill(c, c.d_x, n, 0.0);
+  launch_copy(c, c.d_bn, c.d_r, n);
+  hipLaunchKernelGGL(pcg_init_kernel, dim3(RED), dim3(TPB), 0, s, n, c.d_bn, c.d_r, c.d_dinv,
+                     c.d_z, c.d_p, c.d_part);
+  hipLaunchKernelGGL(pcg_start_dev_kernel, dim3(1), dim3(TPB), 0, s, c.d_part, c.d_scal,
+                     c.d_done, d_tol, abs_exit);
+  FGPU_CHECK(hipGetLastError());
+}
+
+// CHUNK_IT iterations, launches only (inside a graph being captured)
+void enqueue_pcg_chunk(Ctx &c) {
+  for (int k = 0; k < CHUNK_IT; k++) enqueue_iteration(c);
+  FGPU_CHECK(hipGetLastError());
+}
+
+// After enqueue_pcg_start and one chunk: further chunks until the device
+// flags convergence or maxit iterations ran, with run_pcg's results (the
+// solution stays in the node vector d_x).
+int pcg_continue(Ctx &c, int maxit, double rtol, int abs_exit, int *iters, double *relres) {
+  hipStream_t s = c.stream;
+  int launched = CHUNK_IT;
+  while (true) {
+    FGPU_CHECK(hipMemcpyAsync(c.h_done, c.d_done, 3 * sizeof(int), hipMemcpyDeviceToHost, s));
+    FGPU_CHECK(hipMemcpyAsync(c.h_scal, c.d_scal, 5 * sizeof(double), hipMemcpyDeviceToHost, s));
+    FGPU_CHECK(hipStreamSynchronize(s));
+    if (c.h_done[0] || c.h_done[2] >= maxit || launched >= maxit) break;
+    run_chunk(c, false);
+    launched += CHUNK_IT;
+  }
+  if (c.h_done[0] < 0) {  // the immediate exits of run_pcg
+    if (iters) *iters = 0;
+    if (relres)
+      *relres = (abs_exit && c.h_scal[4] < rtol) || c.h_scal[4] == 0.0 ? 0.0 : c.h_scal[3];
+    return 1;
+  }
+  const int conv_it = c.h_done[0], broke = c.h_done[1];
+  if (iters) *iters = conv_it ? conv_it : c.h_done[2];
+  if (relres) *relres = c.h_scal[3];
+  return conv_it > 0 && conv_it <= maxit && !broke;
+}
 
 // b, x, x0: equation vectors on the device.  The iteration runs on node
 // vectors (3 nnode entries, fixed directions identically zero).

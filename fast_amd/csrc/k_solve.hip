@@ -20,6 +20,8 @@
 // The PCG itself lives in k_pcg.hip.
 #include <cmath>
 
+#include <algorithm>
+
 #include "fgpu_dev.h"
 
 namespace fgpu {
@@ -364,6 +366,21 @@ __global__ void dot_finish_kernel(const double *__restrict__ part, double *__res
   if (threadIdx.x == 0) *out = s;
 }
 
+__global__ __launch_bounds__(256) void copy_kernel(long long n, const double *__restrict__ x,
+                                                   double *__restrict__ y) {
+  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (long long)gridDim.x * blockDim.x)
+    y[i] = x[i];
+}
+
+// fills for code that may be captured into a graph (no memset nodes there)
+template <class T>
+__global__ __launch_bounds__(256) void fill_kernel(long long n, T v, T *__restrict__ p) {
+  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (long long)gridDim.x * blockDim.x)
+    p[i] = v;
+}
+
 
 template <int G>
 __global__ __launch_bounds__(256) void bspmv_kernel(int nnode, const long long *__restrict__ bptr,
@@ -527,12 +544,32 @@ void launch_axpby(Ctx &c, int n, double a, const double *x, double b, const doub
   FGPU_CHECK(hipGetLastError());
 }
 
-double dev_dot(Ctx &c, const double *x, const double *y, int n) {
+template <class T>
+static void fill_launch(Ctx &c, T *p, long long n, T v) {
+  if (n <= 0) return;
+  const long long blocks = std::min<long long>(4 * RED, (n + 255) / 256);
+  hipLaunchKernelGGL((fill_kernel<T>), dim3((unsigned)blocks), dim3(256), 0, c.stream, n, v, p);
+  FGPU_CHECK(hipGetLastError());
+}
+void launch_fill(Ctx &c, double *p, long long n, double v) { fill_launch(c, p, n, v); }
+void launch_fill(Ctx &c, int *p, long long n, int v) { fill_launch(c, p, n, v); }
+
+void launch_copy(Ctx &c, const double *x, double *y, long long n) {
+  if (n <= 0) return;
+  const long long blocks = std::min<long long>(4 * RED, (n + 255) / 256);
+  hipLaunchKernelGGL(copy_kernel, dim3((unsigned)blocks), dim3(256), 0, c.stream, n, x, y);
+  FGPU_CHECK(hipGetLastError());
+}
+
+void enqueue_dot(Ctx &c, const double *x, const double *y, int n, double *out) {
   hipLaunchKernelGGL(dot_partial_kernel, dim3(RED), dim3(256), 0, c.stream, n, x, y,
                      c.d_part + 3 * RED);
-  hipLaunchKernelGGL(dot_finish_kernel, dim3(1), dim3(256), 0, c.stream, c.d_part + 3 * RED,
-                     c.d_scal + 8);
+  hipLaunchKernelGGL(dot_finish_kernel, dim3(1), dim3(256), 0, c.stream, c.d_part + 3 * RED, out);
   FGPU_CHECK(hipGetLastError());
+}
+
+double dev_dot(Ctx &c, const double *x, const double *y, int n) {
+  enqueue_dot(c, x, y, n, c.d_scal + 8);
   FGPU_CHECK(hipMemcpyAsync(c.h_scal + 8, c.d_scal + 8, sizeof(double), hipMemcpyDeviceToHost,
                             c.stream));
   FGPU_CHECK(hipStreamSynchronize(c.stream));

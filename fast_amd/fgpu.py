@@ -51,7 +51,7 @@ class NrParams(C.Structure):
                 ("adaptive_pcg", C.c_int), ("predict", PREDICT_FN), ("predict_user", C.c_void_p),
                 ("lambdas", C.c_void_p), ("u0_dev", C.c_void_p), ("lam0", C.c_double),
                 ("u_steps_dev", C.c_void_p), ("secant", C.c_int), ("galerkin", C.c_int),
-                ("direct", C.c_int)]
+                ("direct", C.c_int), ("graph", C.c_int)]
 
 
 class NrStat(C.Structure):
@@ -287,7 +287,8 @@ class GpuModel:
 
     def newton(self, q, steps, factor=0.0, maxit=20, tol=1e-5, pcg_tol=1e-5, pcg_maxit=0,
                adaptive_pcg=0, predict=None, lambdas=None, u0_dev=None, lam0=0.0,
-               u_steps_dev=None, secant=False, galerkin=False, direct=False, keep_host=True):
+               u_steps_dev=None, secant=False, galerkin=False, direct=False, keep_host=True,
+               graph=True):
         """cCtrlNR::Solver on the device (ctrl/ctrlnr.cpp:47-267).
 
         ``predict`` is a Python callable ``(step, u_dev_ptr, lam_prev, lam,
@@ -296,7 +297,9 @@ class GpuModel:
         ``factor`` by a list of load factors; ``u0_dev``/``lam0`` start from
         a converged state; ``u_steps_dev`` (device, steps x neq) receives the
         converged displacement of every step; ``direct`` solves each
-        correction with the device LDL^T instead of PCG."""
+        correction with the device LDL^T instead of PCG; ``graph`` replays
+        each Newton iteration from captured hipGraphs (bit-identical to the
+        eager launches)."""
         q = np.ascontiguousarray(q, dtype=np.float64)
         prm = NrParams()
         prm.steps, prm.factor, prm.maxit, prm.tol = steps, factor, maxit, tol
@@ -304,6 +307,7 @@ class GpuModel:
         prm.secant = int(secant)
         prm.galerkin = int(galerkin)
         prm.direct = int(direct)
+        prm.graph = int(graph)
         cb = None
         if isinstance(predict, Gnn):
             prm.predict = C.cast(lib().fgpu_gnn_newton_predictor, PREDICT_FN)

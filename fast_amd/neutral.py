@@ -312,7 +312,12 @@ def read(path: str) -> Deck:
     return deck
 
 
+# HEADER.ANALYSIS.TARGET.ITERATIONS is read by the reference
+# (cControl::ReadTargetIter, ctrl/ctrl.cpp:264-271) but only its arc-length
+# controls use it (ctrl/ctrlarclen.cpp:125-281); load control ignores it, and
+# the other algorithms are refused below
 _HANDLED = {"HEADER.ANALYSIS.ALGORITHM", "HEADER.ANALYSIS.MAXIMUM.STEPS",
+            "HEADER.ANALYSIS.TARGET.ITERATIONS",
             "HEADER.ANALYSIS.STEP.FACTOR", "HEADER.ANALYSIS.MAXIMUM.ITERATIONS",
             "HEADER.ANALYSIS.TOLERANCE",
             "NODE.COORD", "CONTROL.POINT.WEIGHT", "NODE.SUPPORT", "NODE.SOLVER.ORDER",

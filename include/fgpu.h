@@ -253,6 +253,13 @@ typedef struct {
   int direct;               /* 1: solve each correction with the device LDL^T
                                of the tangent (fgpu_factor; the reference's
                                skyline solver) instead of PCG */
+  int graph;                /* 1: after the first iteration, replay each
+                               Newton iteration from captured hipGraphs (the
+                               whole iteration with the direct solver; the
+                               tangent + PCG start and the update + f_int +
+                               residual around the PCG chunk graphs), one
+                               host read per iteration; bit-identical to 0
+                               (eager launches) */
 } fgpu_nr_params;
 
 typedef struct {

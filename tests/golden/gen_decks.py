@@ -10,7 +10,13 @@ Decks in tests/golden/decks/:
       points, nodal stresses evaluated at the nodes; dat/iga/ThinCylinderDeg5:
       a thin cylinder under a normal pressure on its inner faces, 80 NURBS
       solids of degree (5,2,5); dat/iga/CurvedBeamDeg4: a curved
-      total-Lagrangian beam under an end-face load, 200 load steps);
+      total-Lagrangian beam under an end-face load, 200 load steps;
+      dat/iga/PinSemiCylDeg4.dat: the pinched semi-cylinder of Sze et al.,
+      256 total-Lagrangian NURBS solids of degree (4,4,2) on 5x5x5 Gauss
+      points under a point load in load-control steps of 0.025 -- run for
+      the 40 steps of the deck's header table (load factors 0.05 to 1.0;
+      the deck itself goes on to 100 steps) with every second step printed,
+      keeping the stresses of its first 32 elements);
   BeamB8.dat, BeamB8TL.dat  written by this script: a 4x2x2 cantilever of
       distorted bricks under tip forces, linear and total-Lagrangian
       (the latter under load control, 4 steps to 4 times the load);
@@ -27,7 +33,7 @@ labels, integration-point stresses sig_ip [nelem, npt, 6]
 and element-nodal stresses sig_node [nelem, nen, 6] (Voigt xx yy zz xy xz
 yz), and the number of steps.
 
-usage: python tests/golden/gen_decks.py
+usage: python tests/golden/gen_decks.py [deck names (default: all)]
 """
 from __future__ import annotations
 
@@ -47,7 +53,10 @@ from fast_amd import neutral  # noqa: E402
 
 REF = os.path.join(ROOT, "oracle", "_ref", "fast")
 DECKS = ("TestB8", "ScordelisLoDeg3", "PinCylDeg4", "ThinCylinderDeg5", "CurvedBeamDeg4",
-         "BeamB8", "BeamB8TL", "BeamB8Face", "ScordelisLoDeg3TL")
+         "BeamB8", "BeamB8TL", "BeamB8Face", "ScordelisLoDeg3TL", "PinSemiCylDeg4")
+PRINT_STEP = {"PinSemiCylDeg4": 2}   # HEADER.ANALYSIS.PRINT.STEPS of the run
+RUN_STEPS = {"PinSemiCylDeg4": 40}   # HEADER.ANALYSIS.MAXIMUM.STEPS of the run
+SIG_ELEMS = {"PinSemiCylDeg4": 32}   # stresses kept for the first elements only
 LCM = "%HEADER.ANALYSIS.ALGORITHM\n'lcm'\n\n%HEADER.ANALYSIS.MAXIMUM.STEPS\n{steps}\n\n" \
       "%HEADER.ANALYSIS.STEP.FACTOR\n{factor}\n\n%HEADER.ANALYSIS.MAXIMUM.ITERATIONS\n30\n\n" \
       "%HEADER.ANALYSIS.TOLERANCE\n{tol}\n\n"
@@ -117,30 +126,42 @@ def deck_text(name: str) -> str:
     return open(os.path.join(HERE, "decks", name + ".dat")).read()
 
 
-def main():
+def main(names=None):
     if not os.path.exists(REF):
         subprocess.check_call(["make", "-C", os.path.join(ROOT, "oracle"), "ref", "-j8"])
     with tempfile.TemporaryDirectory() as wd:
-        for name in DECKS:
+        for name in names or DECKS:
             text = deck_text(name)
             with open(os.path.join(HERE, "decks", name + ".dat"), "w") as f:
                 f.write(text)
             # full precision in the result file (ctrl/ctrl.cpp ReadOutPrec)
-            text = text.replace("%NODE\n", "%HEADER.OUTPUT.PRECISION\n16\n\n%NODE\n", 1)
+            extra = "%HEADER.OUTPUT.PRECISION\n16\n\n"
+            pstep = PRINT_STEP.get(name, 1)
+            if pstep > 1:
+                extra += "%%HEADER.ANALYSIS.PRINT.STEPS\n%d\n\n" % pstep
+            text = text.replace("%NODE\n", extra + "%NODE\n", 1)
+            if name in RUN_STEPS:
+                k = text.index("%HEADER.ANALYSIS.MAXIMUM.STEPS")
+                e = text.index("%", k + 1)
+                text = text[:k] + "%%HEADER.ANALYSIS.MAXIMUM.STEPS\n%d\n\n" % RUN_STEPS[name] + \
+                    text[e:]
             with open(os.path.join(wd, name + ".dat"), "w") as f:
                 f.write(text)
             subprocess.check_call([REF, name], cwd=wd, stdout=subprocess.DEVNULL)
             steps = neutral.read_pos(os.path.join(wd, name + ".pos"))
             st = steps[-1]
+            nsteps = len(steps) * pstep
+            ne = SIG_ELEMS.get(name, len(st["sig_ip"]))
             np.savez_compressed(os.path.join(HERE, "deck_%s.npz" % name), labels=st["labels"],
-                                u=st["u"], factor=st["factor"], nsteps=len(steps),
+                                u=st["u"], factor=st["factor"], nsteps=nsteps,
                                 u_steps=np.stack([x["u"] for x in steps]),
                                 factors=np.array([x["factor"] for x in steps]),
-                                elem_labels=st["elem_labels"], sig_ip=np.stack(st["sig_ip"]),
-                                sig_node=np.stack(st["sig_node"]))
+                                elem_labels=st["elem_labels"],
+                                sig_ip=np.stack(st["sig_ip"][:ne]),
+                                sig_node=np.stack(st["sig_node"][:ne]))
             print(name, len(st["labels"]), "nodes,", len(steps), "steps, max |u|",
                   np.abs(st["u"]).max(), "max |sig|", np.abs(np.stack(st["sig_ip"])).max())
 
 
 if __name__ == "__main__":
-    main()
+    main(sys.argv[1:])
