@@ -757,6 +757,8 @@ int fgpu_create(fgpu_ctx **out, const fgpu_mesh *m) {
     c->nblocks_red = 1024;
     const char *ng = getenv("FGPU_NO_GRAPH");
     c->use_graphs = !(ng && ng[0] == '1');
+    const char *tv = getenv("FGPU_BSP_TL_VALU");  // 1: B-spline TL tangents on the VALU kernel
+    c->bsp_tl_valu = tv && tv[0] == '1';
     if (m->nbatch > 0) setup_batch(*c, *m);
     c->d_part = dalloc<double>(std::max<size_t>(4 * c->nblocks_red, 3 * (size_t)c->nchunk + 8));
     c->d_scal = dalloc<double>(16);

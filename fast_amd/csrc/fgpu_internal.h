@@ -225,6 +225,7 @@ struct Ctx {
   double *h_scal = nullptr;   // pinned mirror
   int nblocks_red = 0;        // reduction blocks
   bool use_graphs = true;     // capture PCG iteration chunks into hipGraphs
+  bool bsp_tl_valu = false;   // B-spline TL tangents on the vector-ALU kernel
   hipGraphExec_t g_single = nullptr, g_batch = nullptr;
   int g_single_bits = 0, g_batch_bits = 0;  // matrix precision they were captured for
 

@@ -553,6 +553,258 @@ __global__ __launch_bounds__(256) void bsp_Kt_kernel(
   }
 }
 
+// ------------------------------------------------------- TL K on MFMA
+// The total-Lagrangian tangents as sums of weighted products over the 64
+// points, each a 16x16x64 MFMA chain over a tile pair like the linear
+// kernel's nine:
+//   St.Venant-Kirchhoff: with f = F g,
+//     M_ij  = sum_q c f_ai f_bj                      (9 chains)
+//     BB_ij = sum_q sum_k (c (F F^T)_ij g_ak) g_bk    (6 x 3 chains)
+//     sg    = sum_q sum_l (sum_k c S_kl g_ak) g_bl    (3 chains)
+//     K_ab  = lambda M + mu M^T + mu BB + sg I
+//   (the initial-stress term sg has its own chains: folded into the
+//   diagonal BB chains, its operands would be rounded against the much
+//   larger mu B terms, and thin shells lose their geometric stiffness)
+//   neo-Hooke: with g' = F^-T g,
+//     M1_ij = sum_q (c lambda g'_ai) g'_bj,  M2_ij = sum_q (c mu' g'_ai) g'_bj
+//     sg    = sum_q sum_l (sum_k c mu (F F^T)_kl g'_ak) g'_bl
+//     K_ab  = M1 + M2^T + sg I
+// Phase 1 stages the gradients (g for SVK, g' for neo-Hooke) in FP32 and the
+// point's factors (SVK: sqrt(c) F, c F F^T, c S; neo-Hooke: c lambda, c mu',
+// c mu F F^T) in the LDS the geometry used; phase 2 forms the operands on the
+// vector ALUs per k-step (f = sqrt(c) F g for both sides of M) and runs the
+// chains (30 per k-step for SVK, 21 for neo-Hooke).  Upper tile pairs,
+// diagonal blocks made exactly symmetric, blocks stored as the linear
+// kernel stores them.
+template <int FORM>
+struct KtmPoint {
+  static constexpr int NPP = FORM == FGPU_TL_SVK ? 21 : 8;  // floats per point
+};
+template <int FORM, int NB2>
+struct KtmShared {
+  union {
+    struct {
+      Geo<NB2> g;
+      double U[Bsp<NB2>::NBAS][3];
+    } in;                                      // phase 1
+    float pp[NQP][KtmPoint<FORM>::NPP];        // phase 2
+  } u;
+  float G[3][NQP][Bsp<NB2>::NBAS];  // g (SVK) or F^-T g (neo-Hooke)
+};
+
+template <int FORM, int NB2>
+__global__ __launch_bounds__(256, 3) void bsp_Ktm_kernel(
+    int nelem, const double *__restrict__ coords, const int *__restrict__ conn,
+    const double *__restrict__ cpw, const double *__restrict__ ext,
+    const double *__restrict__ mat, const double *__restrict__ unodal,
+    const double *__restrict__ gp, float *__restrict__ Ke, int *__restrict__ status) {
+  constexpr int NBAS = Bsp<NB2>::NBAS, NSYM = Bsp<NB2>::NSYM, NPART = Bsp<NB2>::NPART;
+  constexpr int NT = Bsp<NB2>::NT, NPAIR = NT * (NT + 1) / 2;
+  constexpr int NPP = KtmPoint<FORM>::NPP;
+  constexpr bool SVK = FORM == FGPU_TL_SVK;
+  extern __shared__ __align__(16) unsigned char smem[];
+  KtmShared<FORM, NB2> &S = *reinterpret_cast<KtmShared<FORM, NB2> *>(smem);
+  const int e = blockIdx.x;
+  const int tid = threadIdx.x;
+  const int *cn = conn + (size_t)e * NBAS;
+  geo_load(S.u.in.g, e, coords, conn, cpw, ext, gp);
+  stage_u<NB2>(S.u.in.U, cn, unodal);
+  __syncthreads();
+  double lam, mu;
+  if constexpr (SVK) {
+    const double E = mat[2 * (size_t)e], nu = mat[2 * (size_t)e + 1];
+    lam = E * nu / ((1.0 + nu) * (1.0 - 2.0 * nu));
+    mu = 0.5 * E / (1.0 + nu);
+  } else {  // neo-Hooke: mat = (mu, lambda)
+    mu = mat[2 * (size_t)e];
+    lam = mat[2 * (size_t)e + 1];
+  }
+  const int q1 = tid / 4, part = tid % 4;
+  float ppv[NPP];  // the point's factors (lanes with part == 0)
+  {  // phase 1, thread (q1, part): F at q1, mapped gradients of 16 nodes
+    const Geo<NB2> &g = S.u.in.g;
+    PointGeo P;
+    point_geometry(g, q1, part, gp, P);
+    double L[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
+    #pragma unroll 2
+    for (int k = 0; k < NPART; k++) {
+      const int a = part * NPART + k;
+      const double *ua = S.u.in.U[a];
+      double dR[3], G[3];
+      rational_dR(g, q1, a, P, dR);
+      for (int i = 0; i < 3; i++)
+        G[i] = P.Ji[i][0] * dR[0] + P.Ji[i][1] * dR[1] + P.Ji[i][2] * dR[2];
+      for (int i = 0; i < 3; i++)
+        for (int j = 0; j < 3; j++) L[i][j] += ua[i] * G[j];
+    }
+    for (int m = 1; m < 4; m <<= 1)
+      for (int i = 0; i < 3; i++)
+        for (int j = 0; j < 3; j++) L[i][j] += __shfl_xor(L[i][j], m);
+    double F[3][3];
+    for (int i = 0; i < 3; i++)
+      for (int j = 0; j < 3; j++) F[i][j] = (i == j ? 1.0 : 0.0) + L[i][j];
+    double B[6];  // F F^T (xx, yy, zz, xy, xz, yz)
+    B[0] = F[0][0] * F[0][0] + F[0][1] * F[0][1] + F[0][2] * F[0][2];
+    B[1] = F[1][0] * F[1][0] + F[1][1] * F[1][1] + F[1][2] * F[1][2];
+    B[2] = F[2][0] * F[2][0] + F[2][1] * F[2][1] + F[2][2] * F[2][2];
+    B[3] = F[0][0] * F[1][0] + F[0][1] * F[1][1] + F[0][2] * F[1][2];
+    B[4] = F[0][0] * F[2][0] + F[0][1] * F[2][1] + F[0][2] * F[2][2];
+    B[5] = F[1][0] * F[2][0] + F[1][1] * F[2][1] + F[1][2] * F[2][2];
+    double M[3][3];  // the map of the staged gradients: I (SVK) or F^-T (neo-Hooke)
+    if constexpr (SVK) {
+      for (int i = 0; i < 3; i++)
+        for (int j = 0; j < 3; j++) M[i][j] = i == j ? 1.0 : 0.0;
+      // Green-Lagrange E = (F^T F - I) / 2 ; S = lambda tr(E) I + 2 mu E
+      double C[3][3];
+      for (int i = 0; i < 3; i++)
+        for (int j = 0; j < 3; j++)
+          C[i][j] = F[0][i] * F[0][j] + F[1][i] * F[1][j] + F[2][i] * F[2][j];
+      const double tr = 0.5 * (C[0][0] - 1.0) + 0.5 * (C[1][1] - 1.0) + 0.5 * (C[2][2] - 1.0);
+      const double rc = sqrt(P.c);
+      for (int i = 0; i < 3; i++)
+        for (int j = 0; j < 3; j++) ppv[3 * i + j] = (float)(rc * F[i][j]);
+      for (int k = 0; k < 6; k++) ppv[9 + k] = (float)(P.c * B[k]);
+      ppv[15] = (float)(P.c * (lam * tr + mu * (C[0][0] - 1.0)));
+      ppv[16] = (float)(P.c * (lam * tr + mu * (C[1][1] - 1.0)));
+      ppv[17] = (float)(P.c * (lam * tr + mu * (C[2][2] - 1.0)));
+      ppv[18] = (float)(P.c * (mu * C[0][1]));
+      ppv[19] = (float)(P.c * (mu * C[0][2]));
+      ppv[20] = (float)(P.c * (mu * C[1][2]));
+    } else {
+      double Fi[3][3];
+      double J = inv3d(F, Fi);
+      if (!(J > 0.0)) {
+        if (part == 0) atomicAdd(&status[1], 1);
+        J = 1.0;
+      }
+      for (int i = 0; i < 3; i++)
+        for (int j = 0; j < 3; j++) M[i][j] = Fi[j][i];
+      ppv[0] = (float)(P.c * lam);
+      ppv[1] = (float)(P.c * (mu - lam * log(J)));
+      for (int k = 0; k < 6; k++) ppv[2 + k] = (float)(P.c * mu * B[k]);
+    }
+    if (part == 0) count_jacobian(status, P.detJ);
+    #pragma unroll 2
+    for (int k = 0; k < NPART; k++) {
+      const int a = part * NPART + k;
+      double dR[3], G[3];
+      rational_dR(g, q1, a, P, dR);
+      for (int i = 0; i < 3; i++)
+        G[i] = P.Ji[i][0] * dR[0] + P.Ji[i][1] * dR[1] + P.Ji[i][2] * dR[2];
+      for (int i = 0; i < 3; i++)
+        S.G[i][q1][a] = (float)(M[i][0] * G[0] + M[i][1] * G[1] + M[i][2] * G[2]);
+    }
+  }
+  __syncthreads();  // the geometry is dead: its LDS takes the points' factors
+  if (part == 0)
+    for (int k = 0; k < NPP; k++) S.u.pp[q1][k] = ppv[k];
+  __syncthreads();
+  const float flam = (float)lam, fmu = (float)mu;
+  const int wave = tid / 64, lane = tid % 64;
+  const int ln = lane % 16, lk = lane / 16;
+  float *out = Ke + (size_t)e * NSYM * 9;
+  for (int tp = (wave + e) % 4; tp < NPAIR; tp += 4) {
+    int ta = 0, r = tp;
+    while (r >= NT - ta) {
+      r -= NT - ta;
+      ta++;
+    }
+    const int a0 = 16 * ta, b0 = 16 * (ta + r);
+    constexpr int NACC = SVK ? 16 : 19;
+    floatx4 acc[NACC];
+    for (int t = 0; t < NACC; t++)
+      for (int m = 0; m < 4; m++) acc[t][m] = 0.f;
+#pragma unroll 2
+    for (int k0 = 0; k0 < NQP; k0 += 4) {
+      const int q = k0 + lk;
+      const float *pp = S.u.pp[q];
+      float ga[3], gb[3];
+      for (int i = 0; i < 3; i++) {
+        ga[i] = S.G[i][q][a0 + ln];
+        gb[i] = S.G[i][q][b0 + ln];
+      }
+      if constexpr (SVK) {
+        float fa[3], fb[3];
+        for (int i = 0; i < 3; i++) {
+          fa[i] = pp[3 * i] * ga[0] + pp[3 * i + 1] * ga[1] + pp[3 * i + 2] * ga[2];
+          fb[i] = pp[3 * i] * gb[0] + pp[3 * i + 1] * gb[1] + pp[3 * i + 2] * gb[2];
+        }
+        for (int i = 0; i < 3; i++)
+          for (int j = 0; j < 3; j++)
+            acc[3 * i + j] = __builtin_amdgcn_mfma_f32_16x16x4f32(fa[i], fb[j], acc[3 * i + j], 0, 0, 0);
+        for (int ij = 0; ij < 6; ij++)
+          for (int k = 0; k < 3; k++)
+            acc[9 + ij] = __builtin_amdgcn_mfma_f32_16x16x4f32(pp[9 + ij] * ga[k], gb[k], acc[9 + ij], 0, 0, 0);
+        // c S g_a (S symmetric: xx yy zz xy xz yz at pp[15..20])
+        const float s0 = pp[15] * ga[0] + pp[18] * ga[1] + pp[19] * ga[2];
+        const float s1 = pp[18] * ga[0] + pp[16] * ga[1] + pp[20] * ga[2];
+        const float s2 = pp[19] * ga[0] + pp[20] * ga[1] + pp[17] * ga[2];
+        acc[15] = __builtin_amdgcn_mfma_f32_16x16x4f32(s0, gb[0], acc[15], 0, 0, 0);
+        acc[15] = __builtin_amdgcn_mfma_f32_16x16x4f32(s1, gb[1], acc[15], 0, 0, 0);
+        acc[15] = __builtin_amdgcn_mfma_f32_16x16x4f32(s2, gb[2], acc[15], 0, 0, 0);
+      } else {
+        for (int i = 0; i < 3; i++) {
+          const float a1 = pp[0] * ga[i], a2 = pp[1] * ga[i];
+          for (int j = 0; j < 3; j++) {
+            acc[3 * i + j] = __builtin_amdgcn_mfma_f32_16x16x4f32(a1, gb[j], acc[3 * i + j], 0, 0, 0);
+            acc[9 + 3 * i + j] = __builtin_amdgcn_mfma_f32_16x16x4f32(a2, gb[j], acc[9 + 3 * i + j], 0, 0, 0);
+          }
+        }
+        // c mu (F F^T) g'_a (xx yy zz xy xz yz at pp[2..7])
+        const float s0 = pp[2] * ga[0] + pp[5] * ga[1] + pp[6] * ga[2];
+        const float s1 = pp[5] * ga[0] + pp[3] * ga[1] + pp[7] * ga[2];
+        const float s2 = pp[6] * ga[0] + pp[7] * ga[1] + pp[4] * ga[2];
+        acc[18] = __builtin_amdgcn_mfma_f32_16x16x4f32(s0, gb[0], acc[18], 0, 0, 0);
+        acc[18] = __builtin_amdgcn_mfma_f32_16x16x4f32(s1, gb[1], acc[18], 0, 0, 0);
+        acc[18] = __builtin_amdgcn_mfma_f32_16x16x4f32(s2, gb[2], acc[18], 0, 0, 0);
+      }
+    }
+    // lane holds D[m][n] with n = lane % 16, m = 4 (lane / 16) + r
+    for (int r = 0; r < 4; r++) {
+      const int a = a0 + 4 * lk + r, b = b0 + ln;
+      if (a > b) continue;
+      float K[9];
+      if constexpr (SVK) {
+        float P[3][3];
+        for (int i = 0; i < 3; i++)
+          for (int j = 0; j < 3; j++) P[i][j] = acc[3 * i + j][r];
+        if (a == b)
+          for (int i = 0; i < 3; i++)
+            for (int j = i + 1; j < 3; j++) {
+              const float s = 0.5f * (P[i][j] + P[j][i]);
+              P[i][j] = s;
+              P[j][i] = s;
+            }
+        const float Bm[9] = {acc[9][r],  acc[12][r], acc[13][r], acc[12][r], acc[10][r],
+                             acc[14][r], acc[13][r], acc[14][r], acc[11][r]};
+        const float sg = acc[15][r];
+        for (int i = 0; i < 3; i++)
+          for (int j = 0; j < 3; j++)
+            K[3 * i + j] = flam * P[i][j] + fmu * P[j][i] + fmu * Bm[3 * i + j] + (i == j ? sg : 0.f);
+      } else {
+        float P1[3][3], P2[3][3];
+        for (int i = 0; i < 3; i++)
+          for (int j = 0; j < 3; j++) {
+            P1[i][j] = acc[3 * i + j][r];
+            P2[i][j] = acc[9 + 3 * i + j][r];
+          }
+        if (a == b)
+          for (int i = 0; i < 3; i++)
+            for (int j = i + 1; j < 3; j++) {
+              const float s1 = 0.5f * (P1[i][j] + P1[j][i]), s2 = 0.5f * (P2[i][j] + P2[j][i]);
+              P1[i][j] = P1[j][i] = s1;
+              P2[i][j] = P2[j][i] = s2;
+            }
+        const float sg = acc[18][r];
+        for (int i = 0; i < 3; i++)
+          for (int j = 0; j < 3; j++) K[3 * i + j] = P1[i][j] + P2[j][i] + (i == j ? sg : 0.f);
+      }
+      float *o = out + (size_t)sym_index(a, b, NBAS) * 9;
+      for (int k = 0; k < 9; k++) o[k] = K[k];
+    }
+  }
+}
+
 // ---------------------------------------------------------------- f_int
 // Phase 1, thread (q, part): geometry, then grad u at q from its 16 control
 // points (shuffle-reduced), then c times the stress that multiplies the
@@ -916,11 +1168,19 @@ void launch_Kt(Ctx &c, const double *u) {
     upload_sym_table<NB2>();
     table_dev = c.device;
   }
-  const size_t lds = sizeof(KtShared<FORM, NB2>);
-  set_lds(bsp_Kt_kernel<FORM, NB2>, lds);
-  hipLaunchKernelGGL((bsp_Kt_kernel<FORM, NB2>), dim3(c.nelem), dim3(256), lds, c.stream,
-                     c.nelem, c.d_coords, c.d_conn, c.d_cpw, c.d_ext, c.d_mat, u, c.d_gauss64,
-                     c.d_Ke, c.d_status);
+  if (c.bsp_tl_valu) {  // the vector-ALU kernel, kept for comparison
+    const size_t lds = sizeof(KtShared<FORM, NB2>);
+    set_lds(bsp_Kt_kernel<FORM, NB2>, lds);
+    hipLaunchKernelGGL((bsp_Kt_kernel<FORM, NB2>), dim3(c.nelem), dim3(256), lds, c.stream,
+                       c.nelem, c.d_coords, c.d_conn, c.d_cpw, c.d_ext, c.d_mat, u, c.d_gauss64,
+                       c.d_Ke, c.d_status);
+  } else {
+    const size_t lds = sizeof(KtmShared<FORM, NB2>);
+    set_lds(bsp_Ktm_kernel<FORM, NB2>, lds);
+    hipLaunchKernelGGL((bsp_Ktm_kernel<FORM, NB2>), dim3(c.nelem), dim3(256), lds, c.stream,
+                       c.nelem, c.d_coords, c.d_conn, c.d_cpw, c.d_ext, c.d_mat, u, c.d_gauss64,
+                       c.d_Ke, c.d_status);
+  }
   FGPU_CHECK(hipGetLastError());
 }
 
