@@ -403,6 +403,14 @@ def run_step(name, steps, warmup, serial=False, with_accuracy=True):
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
     times = np.array(times)
+    # f_int alone (element vectors + gather, nothing beside it), for its
+    # roofline; the step above may run it beside the assembly
+    tfa = []
+    for _ in range(max(3, min(steps, 10))):
+        if not L.fgpu_int_force_dev(model.h, d_u, g.data_ptr()):
+            raise fgpu.FgpuError(L.fgpu_last_error(model.h).decode())
+        tfa.append(L.fgpu_last_time_ms(model.h, 2))
+    tf_alone = float(np.mean(tfa))
     report = model.tuning_report()
     log(report.strip().replace("\n", " | "))
     tk, ta, tf = times.mean(axis=0)
@@ -415,13 +423,15 @@ def run_step(name, steps, warmup, serial=False, with_accuracy=True):
     overlap = None
     if not serial:
         mm = re.search(r"K then f_int: ([0-9.]+) ms; f_int beside the assembly: ([0-9.]+) ms; "
-                       r"f_int element vectors beside the assembly, gather after: ([0-9.]+) ms",
+                       r"f_int element vectors beside the assembly, gather after: ([0-9.]+) ms; "
+                       r"f_int element vectors first, gather beside the element K: ([0-9.]+) ms",
                        report)
         if mm:
-            t = [float(mm.group(k)) for k in (1, 2, 3)]
+            t = [float(mm.group(k)) for k in (1, 2, 3, 4)]
             overlap = ["serial", "f_int beside the assembly",
-                       "f_int element vectors beside the assembly, gather after"][t.index(min(t))]
-    if overlap not in (None, "serial") and dom == "assembly":
+                       "f_int element vectors beside the assembly, gather after",
+                       "f_int element vectors first, gather beside the element K"][t.index(min(t))]
+    if overlap not in (None, "serial") and dom == "assembly" and "first" not in overlap:
         roof["shares_gpu_with"] = "f_int (second stream)"
     rec = {
         "metric": "elements/s (K+f_int+assembly)",
@@ -431,11 +441,15 @@ def run_step(name, steps, warmup, serial=False, with_accuracy=True):
         "dtype": "fp32 element matrices (MFMA for B-spline) / fp64 assembly, f_int",
         "data": "synthetic (seeded generated mesh)", "config": cfg,
         "phase_ms": {"element_K": round(float(tk), 4), "assembly": round(float(ta), 4),
-                     "f_int": round(float(tf), 4)},
+                     "f_int": round(float(tf), 4), "f_int_alone": round(tf_alone, 4)},
         "step_order": overlap or "serial",
         "roofline": roof,
-        "f_int_roofline": fint_roofline(m, tf),
+        "f_int_roofline": fint_roofline(m, tf_alone),
     }
+    if overlap not in (None, "serial"):
+        rec["phase_ms"]["note"] = ("f_int: its span in the step, parts of it beside other "
+                                   "kernels on the second stream; f_int_alone: its kernels "
+                                   "with the GPU to themselves (the f_int roofline's time)")
     if with_accuracy:
         acc = accuracy_Ku(model, m, u)
         gh = g.cpu().numpy()

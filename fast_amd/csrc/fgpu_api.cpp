@@ -571,9 +571,9 @@ int stiffness_fint_dev(Ctx &c, const double *d_u, double *d_g) {
     if (env && env[0] == '0') {
       c.overlap = 0;
     } else {
-      double best[3] = {1e30, 1e30, 1e30};
+      double best[4] = {1e30, 1e30, 1e30, 1e30};
       for (int r = 0; r < 3; r++)
-        for (int mode = 0; mode < 3; mode++) {
+        for (int mode = 0; mode < 4; mode++) {
           FGPU_CHECK(hipStreamSynchronize(c.stream));
           const auto t0 = std::chrono::steady_clock::now();
           const int ok = mode ? stiffness_fint_overlapped(c, d_u, d_g, mode)
@@ -585,13 +585,14 @@ int stiffness_fint_dev(Ctx &c, const double *d_u, double *d_g) {
           best[mode] = std::min(best[mode], ms);
         }
       c.overlap = 0;
-      for (int mode = 1; mode < 3; mode++)
+      for (int mode = 1; mode < 4; mode++)
         if (best[mode] < best[c.overlap]) c.overlap = mode;
-      char buf[192];
+      char buf[256];
       std::snprintf(buf, sizeof buf,
                     "K then f_int: %.4f ms; f_int beside the assembly: %.4f ms; f_int element "
-                    "vectors beside the assembly, gather after: %.4f ms\n",
-                    best[0], best[1], best[2]);
+                    "vectors beside the assembly, gather after: %.4f ms; f_int element vectors "
+                    "first, gather beside the element K: %.4f ms\n",
+                    best[0], best[1], best[2], best[3]);
       c.tune_report += buf;
     }
   }
@@ -599,7 +600,50 @@ int stiffness_fint_dev(Ctx &c, const double *d_u, double *d_g) {
   return stiffness_fint_overlapped(c, d_u, d_g, c.overlap);
 }
 
+// Order 3: f_int's element vectors first (arithmetic), then its gather
+// (memory-bound) on the second stream beside the stiffness element kernel
+// (arithmetic), then the assembly.
+int stiffness_fint_fint_first(Ctx &c, const double *d_u, double *d_g) {
+  c.factored = false;
+  c.bval32_valid = false;
+  FGPU_CHECK(hipMemsetAsync(c.d_status, 0, 8 * sizeof(int), c.stream));
+  std::swap(c.d_status, c.d_status2);  // f_int counts into the second array
+  FGPU_CHECK(hipEventRecord(c.ev[3], c.stream));
+  if (c.shape == FGPU_HEX8)
+    launch_hex8_fint(c, d_u);
+  else
+    launch_bsp_fint(c, d_u);
+  std::swap(c.d_status, c.d_status2);
+  FGPU_CHECK(hipEventRecord(c.ev[4], c.stream));
+  FGPU_CHECK(hipEventRecord(c.ev_fork, c.stream));
+  FGPU_CHECK(hipStreamWaitEvent(c.stream2, c.ev_fork, 0));
+  {
+    OnStream2 on(c);
+    launch_gather_fint(c, d_g);
+    launch_penalty_force(c, d_u, d_g);
+    FGPU_CHECK(hipEventRecord(c.ev[7], c.stream));
+  }
+  FGPU_CHECK(hipEventRecord(c.ev[0], c.stream));
+  if (c.shape == FGPU_HEX8)
+    launch_hex8_K(c, d_u);
+  else
+    launch_bsp_K(c, d_u);
+  FGPU_CHECK(hipEventRecord(c.ev[1], c.stream));
+  launch_assemble_K(c);
+  launch_penalty_diag(c);
+  FGPU_CHECK(hipEventRecord(c.ev[2], c.stream));
+  FGPU_CHECK(hipStreamWaitEvent(c.stream, c.ev[7], 0));  // join
+  enqueue_status_read(c, c.d_status, 8);
+  FGPU_CHECK(hipStreamSynchronize(c.stream));
+  c.last_ms[0] = elapsed(c, 0, 1);
+  c.last_ms[1] = elapsed(c, 1, 2);
+  c.last_ms[2] = elapsed(c, 3, 4) + elapsed(c, 4, 7);
+  if (!check_status_values(c, "stiffness", c.h_status)) return 0;
+  return check_status_values(c, "internal force", c.h_status + 4);
+}
+
 int stiffness_fint_overlapped(Ctx &c, const double *d_u, double *d_g, int mode) {
+  if (mode == 3) return stiffness_fint_fint_first(c, d_u, d_g);
   c.factored = false;
   c.bval32_valid = false;
   // the two status arrays are adjacent (d_status2 = d_status + 4)
@@ -1396,7 +1440,7 @@ int fgpu_set_penalty(fgpu_ctx *ctx, int n, const int *eq, const double *v) {
 
 int fgpu_set_overlap(fgpu_ctx *ctx, int mode) {
   return guarded(ctx, [&](Ctx &c) {
-    if (mode < -1 || mode > 2) throw Error("overlap mode: -1, 0, 1 or 2");
+    if (mode < -1 || mode > 3) throw Error("overlap mode: -1, 0, 1, 2 or 3");
     c.overlap = mode;
     return 1;
   });

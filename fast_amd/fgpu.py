@@ -446,7 +446,8 @@ class GpuModel:
         """Order of stiffness_and_int_force: 0 serial, 1 f_int beside the
         assembly (fused point records where the kernels have them), 2 only
         f_int's element vectors beside the assembly and its gather after it,
-        -1 timed on the first call."""
+        3 f_int's element vectors first and its gather beside the element
+        stiffness, -1 timed on the first call."""
         self._check(lib().fgpu_set_overlap(self.h, mode), "fgpu_set_overlap")
 
     def block_counts(self):

@@ -427,8 +427,10 @@ int fgpu_set_launch(fgpu_ctx *ctx, int hex8_te, int asm_mode);
  * assembly, and for the linear tuned B-spline kernels (degrees (3,3,3) and
  * (3,3,2), 4x4x4 Gauss) finished from the per-point records the stiffness
  * kernel writes; 2 = as 1, but only the element vectors beside the assembly
- * and the gather after it; -1 (default) = the first call times the three and
- * keeps the fastest (FGPU_AUTOTUNE=0: 0).  All orders give the same bits. */
+ * and the gather after it; 3 = the internal force's element vectors first,
+ * then its gather on the second stream beside the element stiffness, then
+ * the assembly; -1 (default) = the first call times the four and keeps the
+ * fastest (FGPU_AUTOTUNE=0: 0).  All orders give the same bits. */
 int fgpu_set_overlap(fgpu_ctx *ctx, int mode);
 const char *fgpu_tuning_report(fgpu_ctx *ctx);
 

@@ -311,13 +311,14 @@ def test_linear_static_driver(fg, oracle_mod):
     assert nrel(m.scatter(u), d["u"]) <= 1e-8
 
 
-@pytest.mark.parametrize("overlap", [0, 1, 2])
+@pytest.mark.parametrize("overlap", [0, 1, 2, 3])
 @pytest.mark.parametrize("kind", ["hex8_nh", "hex8_linear", "bsp_svk", "bsp332_linear",
                                   "bsp_linear"])
 def test_stiffness_beside_internal_force_identical_bits(fg, kind, overlap):
     """fgpu_assemble_K_fint in every order, forced (0: K then f_int on one
     stream; 1: f_int on the second stream beside the assembly; 2: its element
-    vectors beside the assembly, the gather after it; for the linear
+    vectors beside the assembly, the gather after it; 3: its element vectors
+    first, the gather beside the element stiffness; for the linear
     tuned B-spline kernels of degree (3,3,3) and (3,3,2) finished from the
     per-point records the stiffness kernel writes) gives the matrix and the
     force of fgpu_assemble_K + fgpu_int_force, bit for bit, and reports a
