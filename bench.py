@@ -14,8 +14,9 @@ and its 20-increment Newton solve, the learned warm start's solve-time ratios
 (plain / secant / network), and the relative errors of K.u, f_int and u
 against the FP64 oracle for every configuration.
 
-``--workload iga|hex8|batch`` runs one step workload alone; ``warmstart``,
-``newton`` and ``copy`` run those measurements alone.
+``--workload iga|iga_svk|hex8|batch`` runs one step workload alone (iga_svk:
+the cubic patch with total-Lagrangian St.Venant-Kirchhoff elements);
+``warmstart``, ``newton`` and ``copy`` run those measurements alone.
 
 Prints ONE JSON line (rank 0).  Multi-GPU runs are not part of this build.
 """
@@ -40,7 +41,8 @@ PEAK_FP64 = 78.6        # TFLOP/s, MI355X FP64 vector
 PEAK_HBM = 8000.0       # GB/s
 
 STEP_WORKLOADS = {"batch": "hex8_batch_1024_cubes", "iga": "iga_cubic_bspline_32x32x8",
-                  "hex8": "hex8_neohooke_64x64x64"}
+                  "hex8": "hex8_neohooke_64x64x64",
+                  "iga_svk": "iga_cubic_bspline_32x32x8_tl_svk"}
 
 
 def log(msg):
@@ -55,11 +57,12 @@ def smooth_field(X):
 def make_workload(name):
     from fast_amd import mesh as M
     prm = None
-    if name == "iga":
-        m = M.bsp_block(ne=(32, 32, 8), p=3, perturb=0.05, seed=0, E=1.0, nu=0.3)
+    if name in ("iga", "iga_svk"):
+        form = "tl_svk" if name == "iga_svk" else "linear"
+        m = M.bsp_block(ne=(32, 32, 8), p=3, perturb=0.05, seed=0, E=1.0, nu=0.3, form=form)
         cfg = {"workload": STEP_WORKLOADS[name], "elements": m.nelem, "nqp": 64,
                "ndof_elem": 192, "neq": int(m.neq), "E": 1.0, "nu": 0.3, "gauss": "4x4x4",
-               "perturb": "U(-0.05,0.05)*h seed 0"}
+               "perturb": "U(-0.05,0.05)*h seed 0", "form": form}
         u = smooth_field(m.coords)
     elif name == "batch":
         m, prm = M.cube_batch(1024, nmin=4, nmax=32, seed=0)
@@ -80,18 +83,26 @@ def make_workload(name):
 
 
 # ---------------------------------------------------------------- rooflines
+# weighted products per stored block and point of the B-spline stiffness
+# kernels: linear 9 (P_ij = sum_q c_q G_i[q][a] G_j[q][b]); total Lagrangian
+# St.Venant-Kirchhoff 30 (9 of F g, 6 x 3 of (g.g) F F^T, 3 of g^T S g) and
+# neo-Hooke 21 (9 + 9 of F^-T g, 3 of g'^T F F^T g')
+BSP_CHAINS = {"linear": 9, "tl_svk": 30, "tl_neohooke": 21}
+
+
 def kernel_flops(m):
-    """Algorithmic flops of the B-spline element-stiffness kernel: the nine
-    sums P_ij[a][b] = sum_q c_q G_i[q][a] G_j[q][b] for every stored node
-    block a <= b (2080), 64 points, multiply + add."""
+    """Algorithmic flops of the B-spline element-stiffness kernel: its
+    weighted sums for every stored node block a <= b (2080), 64 points,
+    multiply + add."""
     if m.shape == "bsp":
-        return 2080 * 9 * 64 * 2
+        return 2080 * BSP_CHAINS[m.form] * 64 * 2
     return None
 
 
-# what the B-spline kernel issues on MFMA: 10 upper 16x16 tile pairs x 9
-# products, 16x16 outputs, K = 64 (full tiles on the diagonal)
-BSP_EXECUTED_FLOPS = 10 * 9 * 16 * 16 * 64 * 2
+def bsp_executed_flops(m):
+    """What the B-spline kernel issues on MFMA: 10 upper 16x16 tile pairs x
+    its chains, 16x16 outputs, K = 64 (full tiles on the diagonal)."""
+    return 10 * BSP_CHAINS[m.form] * 16 * 16 * 64 * 2
 
 
 def assembly_bytes(m, nbf, nblk):
@@ -119,8 +130,9 @@ def element_roofline(m, tk):
         achieved = fl * m.nelem / (tk * 1e-3) / 1e12
         return {"bound": "mfma", "achieved": round(achieved, 2), "peak": PEAK_FP32_MFMA,
                 "unit": "TFLOP/s", "frac": round(achieved / PEAK_FP32_MFMA, 4), "traffic": None,
-                "kernel": "bsp_K_kernel", "flops_per_elem": fl,
-                "executed_frac": round(BSP_EXECUTED_FLOPS * m.nelem / (tk * 1e-3) / 1e12
+                "kernel": "bsp_K_kernel" if m.form == "linear" else "bsp_Ktm_kernel",
+                "flops_per_elem": fl,
+                "executed_frac": round(bsp_executed_flops(m) * m.nelem / (tk * 1e-3) / 1e12
                                        / PEAK_FP32_MFMA, 4),
                 "kernel_ms": round(float(tk), 4)}
     # hex8: element kernel bound by its FP32 element-matrix stores
@@ -654,7 +666,7 @@ def main():
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--workload", default="batch",
-                    choices=["iga", "hex8", "batch", "warmstart", "newton", "copy"])
+                    choices=["iga", "iga_svk", "hex8", "batch", "warmstart", "newton", "copy"])
     ap.add_argument("--nstates", type=int, default=2000, help="warmstart: training states")
     ap.add_argument("--train-steps", type=int, default=4000, help="warmstart: Adam steps")
     ap.add_argument("--beta", type=float, default=30.0, help="warmstart: roughness weight")
