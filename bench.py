@@ -162,9 +162,21 @@ def fint_roofline(m, tf):
     per_elem = m.nn * (4 + 48 + 48 + 4)
     b = m.nelem * per_elem + m.nnode * 24
     achieved = b / (tf * 1e-3) / 1e9
-    return {"bound": "hbm", "achieved": round(achieved, 1), "peak": PEAK_HBM, "unit": "GB/s",
-            "frac": round(achieved / PEAK_HBM, 4), "algorithmic_bytes": int(b),
-            "kernel_ms": round(float(tf), 4)}
+    rec = {"bound": "hbm", "achieved": round(achieved, 1), "peak": PEAK_HBM, "unit": "GB/s",
+           "frac": round(achieved / PEAK_HBM, 4), "algorithmic_bytes": int(b),
+           "kernel_ms": round(float(tf), 4)}
+    if m.shape == "bsp":
+        # B-spline f_int is FP64 arithmetic: per point grad u = sum_a u_a (x) g_a
+        # and f_a += c sigma g_a (9 multiply-adds per node each), the basis
+        # gradients' evaluation not counted
+        nq = int(np.prod(m.nip))
+        fl = m.nelem * nq * (2 * m.nn * 9 * 2)
+        t = fl / (tf * 1e-3) / 1e12
+        rec = {"bound": "fp64", "achieved": round(t, 2), "peak": PEAK_FP64, "unit": "TFLOP/s",
+               "frac": round(t / PEAK_FP64, 4), "flops_per_elem": fl // m.nelem,
+               "hbm_GBs": round(achieved, 1), "algorithmic_bytes": int(b),
+               "kernel_ms": round(float(tf), 4)}
+    return rec
 
 
 # ---------------------------------------------------------------- CPU baselines
