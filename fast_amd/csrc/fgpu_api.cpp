@@ -236,6 +236,12 @@ void build_maps(Ctx &c, const int *conn) {
   c.d_contrib = dupload(contrib.data(), contrib.size(), s);
   c.d_nptr = dupload(nptr.data(), nptr.size(), s);
   c.d_nadj = dupload(nadj.data(), nadj.size(), s);
+  if (c.shape == FGPU_HEX8) {  // inverse of nadj: where each element entry goes
+    std::vector<int> epos(nadj.size());
+    for (size_t t = 0; t < nadj.size(); t++) epos[nadj[t]] = (int)t;
+    c.d_epos = dupload(epos.data(), epos.size(), s);
+    FGPU_CHECK(hipStreamSynchronize(s));
+  }
   // row assembly tables: B-spline solids of at most 64 control points per
   // element whose rows fit the kernel's LDS (ROW_MAXS upper blocks)
   constexpr long long ROW_MAXS = 848;  // 77 bytes of LDS each: 63.8 KB
@@ -322,6 +328,7 @@ void destroy(Ctx *c) {
   if (!c) return;
   void *bufs[] = {c->d_coords, c->d_conn, c->d_eqmap, c->d_mat, c->d_ext,
                   c->d_cpw, c->d_gauss, c->d_gauss64, c->d_Ke, c->d_Ke64, c->d_ge, c->d_pts,
+                  c->d_epos,
                   c->d_status, c->d_rowptr, c->d_col, c->d_bptr, c->d_bcol,
                   c->d_bcoloff, c->d_bdiag, c->d_fmask, c->d_bval, c->d_bn,
                   c->d_blkA, c->d_blkB, c->d_offAB, c->d_offBA, c->d_cptr,

@@ -226,6 +226,12 @@ struct Ctx {
   int nblocks_red = 0;        // reduction blocks
   bool use_graphs = true;     // capture PCG iteration chunks into hipGraphs
   bool bsp_tl_valu = false;   // B-spline TL tangents on the vector-ALU kernel
+  // hex8 internal force: element vector entry (e, a) written at position
+  // epos[e * nn + a] of the node-sorted adjacency (nadj), so that the gather
+  // reads each node's entries contiguously; ge_sorted says which layout the
+  // last writer of d_ge used
+  int *d_epos = nullptr;
+  bool ge_sorted = false;
   hipGraphExec_t g_single = nullptr, g_batch = nullptr;
   int g_single_bits = 0, g_batch_bits = 0;  // matrix precision they were captured for
 

@@ -362,7 +362,8 @@ template <int FORM, int NIP>
 __global__ __launch_bounds__(128) void hex8_fint_kernel(
     int nelem, const double *__restrict__ coords, const int *__restrict__ conn,
     const double *__restrict__ mat, const double *__restrict__ unodal,
-    const double *__restrict__ gp, double *__restrict__ ge, int *__restrict__ status) {
+    const double *__restrict__ gp, double *__restrict__ ge, int *__restrict__ status,
+    const int *__restrict__ epos) {
   const int e = blockIdx.x * blockDim.x + threadIdx.x;
   if (e >= nelem) return;
   const int *cn = conn + (size_t)e * 8;
@@ -454,9 +455,16 @@ __global__ __launch_bounds__(128) void hex8_fint_kernel(
   if (neg) atomicAdd(&status[0], neg);
   if (zero) atomicAdd(&status[3], zero);
   if (fail) atomicAdd(&status[1], fail);
-  double *out = ge + (size_t)e * 24;
-  for (int a = 0; a < 8; a++)
-    for (int i = 0; i < 3; i++) out[3 * a + i] = g[a][i];
+  if (epos) {  // node-sorted: entry (e, a) at its place in the node's list
+    for (int a = 0; a < 8; a++) {
+      double *out = ge + 3 * (size_t)epos[(size_t)e * 8 + a];
+      for (int i = 0; i < 3; i++) out[i] = g[a][i];
+    }
+  } else {
+    double *out = ge + (size_t)e * 24;
+    for (int a = 0; a < 8; a++)
+      for (int i = 0; i < 3; i++) out[3 * a + i] = g[a][i];
+  }
 }
 
 // Body-force element vectors (cElemLoad::ExtForce with cBodyLoad, load/
@@ -634,8 +642,9 @@ void launch_f_t(Ctx &c, const double *u) {
   int grid = (c.nelem + 127) / 128;
   hipLaunchKernelGGL((hex8_fint_kernel<FORM, NIP>), dim3(grid), dim3(128), 0, c.stream,
                      c.nelem, c.d_coords, c.d_conn, c.d_mat, u, c.d_gauss64, c.d_ge,
-                     c.d_status);
+                     c.d_status, c.d_epos);
   FGPU_CHECK(hipGetLastError());
+  c.ge_sorted = c.d_epos != nullptr;
 }
 
 template <int FORM>
@@ -667,6 +676,7 @@ static void launch_body_t(Ctx &c, const double *b) {
   hipLaunchKernelGGL((hex8_body_kernel<NIP>), dim3((c.nelem + 127) / 128), dim3(128), 0, c.stream,
                      c.nelem, c.d_coords, c.d_conn, b, c.d_gauss64, c.d_ge, c.d_status);
   FGPU_CHECK(hipGetLastError());
+  c.ge_sorted = false;
 }
 
 template <int FORM, int NIP>

@@ -222,6 +222,30 @@ __global__ __launch_bounds__(256) void gather_fint_kernel(
   if (q >= 0) g[q] = s;
 }
 
+// The same sums from node-sorted element vectors (entry t of node A's
+// adjacency at ge[3 t]): each node reads one contiguous run, in the same
+// order, so the results are the unsorted gather's bits.
+__global__ __launch_bounds__(256) void gather_sorted_kernel(
+    int nnode, const int *__restrict__ nptr, const double *__restrict__ ge,
+    const int *__restrict__ eqmap, double *__restrict__ g) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= 3 * nnode) return;
+  const int A = i / 3, d = i - 3 * A;
+  double s = 0;
+  const int t0 = nptr[A], t1 = nptr[A + 1];
+  for (int tb = t0; tb < t1; tb += 8) {
+    double v[8];
+#pragma unroll
+    for (int u = 0; u < 8; u++)
+      if (tb + u < t1) v[u] = ge[(size_t)(tb + u) * 3 + d];
+#pragma unroll
+    for (int u = 0; u < 8; u++)
+      if (tb + u < t1) s += v[u];
+  }
+  const int q = eqmap[i];
+  if (q >= 0) g[q] = s;
+}
+
 __global__ __launch_bounds__(256) void to_f32_kernel(long long n, const double *__restrict__ a,
                                                      float *__restrict__ b) {
   const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -460,8 +484,12 @@ void launch_assemble_K(Ctx &c) {
 void launch_gather_fint(Ctx &c, double *g) {
   // 64-thread workgroups: a B-spline mesh has few nodes (13,475 on the
   // 32x32x8 cubic patch), and its 632 workgroups spread over the CUs
-  hipLaunchKernelGGL(gather_fint_kernel, dim3((3 * c.nnode + 63) / 64), dim3(64), 0, c.stream,
-                     c.nnode, c.nn, c.d_nptr, c.d_nadj, c.d_ge, c.d_eqmap, g);
+  if (c.ge_sorted)
+    hipLaunchKernelGGL(gather_sorted_kernel, dim3((3 * c.nnode + 63) / 64), dim3(64), 0,
+                       c.stream, c.nnode, c.d_nptr, c.d_ge, c.d_eqmap, g);
+  else
+    hipLaunchKernelGGL(gather_fint_kernel, dim3((3 * c.nnode + 63) / 64), dim3(64), 0,
+                       c.stream, c.nnode, c.nn, c.d_nptr, c.d_nadj, c.d_ge, c.d_eqmap, g);
   FGPU_CHECK(hipGetLastError());
 }
 
