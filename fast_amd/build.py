@@ -18,7 +18,8 @@ CSRC = os.path.join(HERE, "csrc")
 OUT = os.path.join(HERE, "libfgpu.so")
 BUILD = os.path.join(HERE, "_build")
 SOURCES = ["fgpu_api.cpp", "fgpu_drivers.cpp", "k_hex8.hip", "k_bsp.hip", "k_solve.hip",
-           "k_pcg.hip", "k_gnn.hip", "k_direct.hip", "k_bspg.hip", "k_scratch.hip", "k_load.hip"]
+           "k_pcg.hip", "k_gnn.hip", "k_direct.hip", "k_bspg.hip", "k_scratch.hip", "k_load.hip",
+           "k_hex20.hip"]
 ARCH = os.environ.get("FGPU_ARCH", "gfx950")
 
 

@@ -426,6 +426,26 @@ int check_status_values(Ctx &c, const char *what, const int *st) {
 }
 
 
+// element kernels of the context's shape (the B-spline launchers pick the
+// tuned or the generic kernels themselves)
+void launch_elem_K(Ctx &c, const double *d_u) {
+  if (c.shape == FGPU_HEX8)
+    launch_hex8_K(c, d_u);
+  else if (c.shape == FGPU_HEX20)
+    launch_hex20_K(c, d_u);
+  else
+    launch_bsp_K(c, d_u);
+}
+
+void launch_elem_fint(Ctx &c, const double *d_u) {
+  if (c.shape == FGPU_HEX8)
+    launch_hex8_fint(c, d_u);
+  else if (c.shape == FGPU_HEX20)
+    launch_hex20_fint(c, d_u);
+  else
+    launch_bsp_fint(c, d_u);
+}
+
 // Autotuner: times every valid launch configuration of the element-stiffness
 // and assembly kernels on this mesh and state (best of 3 runs each, HIP
 // events) and keeps the fastest.  All configurations give identical bits.
@@ -459,7 +479,7 @@ void autotune(Ctx &c, const double *d_u) {
     c.hex8_te = pick;
     launch_hex8_K(c, d_u);
   } else {
-    launch_bsp_K(c, d_u);
+    launch_elem_K(c, d_u);
   }
   float best = 1e30f;
   int pick = 0;
@@ -488,10 +508,7 @@ void enqueue_stiffness(Ctx &c, const double *d_u) {
   c.factored = false;
   c.bval32_valid = false;
   launch_fill(c, c.d_status, 4, 0);
-  if (c.shape == FGPU_HEX8)
-    launch_hex8_K(c, d_u);
-  else
-    launch_bsp_K(c, d_u);
+  launch_elem_K(c, d_u);
   launch_assemble_K(c);
   launch_penalty_diag(c);
 }
@@ -501,10 +518,7 @@ void enqueue_stiffness(Ctx &c, const double *d_u) {
 void enqueue_int_force(Ctx &c, const double *d_u, double *d_g) {
   std::swap(c.d_status, c.d_status2);
   launch_fill(c, c.d_status, 4, 0);
-  if (c.shape == FGPU_HEX8)
-    launch_hex8_fint(c, d_u);
-  else
-    launch_bsp_fint(c, d_u);
+  launch_elem_fint(c, d_u);
   launch_gather_fint(c, d_g);
   launch_penalty_force(c, d_u, d_g);
   std::swap(c.d_status, c.d_status2);
@@ -516,10 +530,7 @@ int stiffness_dev(Ctx &c, const double *d_u) {
   c.bval32_valid = false;
   FGPU_CHECK(hipMemsetAsync(c.d_status, 0, 4 * sizeof(int), c.stream));
   FGPU_CHECK(hipEventRecord(c.ev[0], c.stream));
-  if (c.shape == FGPU_HEX8)
-    launch_hex8_K(c, d_u);
-  else
-    launch_bsp_K(c, d_u);
+  launch_elem_K(c, d_u);
   FGPU_CHECK(hipEventRecord(c.ev[1], c.stream));
   launch_assemble_K(c);
   launch_penalty_diag(c);
@@ -534,10 +545,7 @@ int stiffness_dev(Ctx &c, const double *d_u) {
 int int_force_dev(Ctx &c, const double *d_u, double *d_g) {
   FGPU_CHECK(hipMemsetAsync(c.d_status, 0, 4 * sizeof(int), c.stream));
   FGPU_CHECK(hipEventRecord(c.ev[3], c.stream));
-  if (c.shape == FGPU_HEX8)
-    launch_hex8_fint(c, d_u);
-  else
-    launch_bsp_fint(c, d_u);
+  launch_elem_fint(c, d_u);
   launch_gather_fint(c, d_g);
   launch_penalty_force(c, d_u, d_g);
   FGPU_CHECK(hipEventRecord(c.ev[4], c.stream));
@@ -616,10 +624,7 @@ int stiffness_fint_fint_first(Ctx &c, const double *d_u, double *d_g) {
   FGPU_CHECK(hipMemsetAsync(c.d_status, 0, 8 * sizeof(int), c.stream));
   std::swap(c.d_status, c.d_status2);  // f_int counts into the second array
   FGPU_CHECK(hipEventRecord(c.ev[3], c.stream));
-  if (c.shape == FGPU_HEX8)
-    launch_hex8_fint(c, d_u);
-  else
-    launch_bsp_fint(c, d_u);
+  launch_elem_fint(c, d_u);
   std::swap(c.d_status, c.d_status2);
   FGPU_CHECK(hipEventRecord(c.ev[4], c.stream));
   FGPU_CHECK(hipEventRecord(c.ev_fork, c.stream));
@@ -631,10 +636,7 @@ int stiffness_fint_fint_first(Ctx &c, const double *d_u, double *d_g) {
     FGPU_CHECK(hipEventRecord(c.ev[7], c.stream));
   }
   FGPU_CHECK(hipEventRecord(c.ev[0], c.stream));
-  if (c.shape == FGPU_HEX8)
-    launch_hex8_K(c, d_u);
-  else
-    launch_bsp_K(c, d_u);
+  launch_elem_K(c, d_u);
   FGPU_CHECK(hipEventRecord(c.ev[1], c.stream));
   launch_assemble_K(c);
   launch_penalty_diag(c);
@@ -659,12 +661,10 @@ int stiffness_fint_overlapped(Ctx &c, const double *d_u, double *d_g, int mode) 
   // linear tuned B-spline kernels: the stiffness kernel also writes the
   // points' internal-force records, and f_int only finishes from them
   const bool fused = bsp_fused_fint(c);
-  if (c.shape == FGPU_HEX8)
-    launch_hex8_K(c, d_u);
-  else if (fused)
+  if (fused)
     launch_bsp_K_pts(c, d_u);
   else
-    launch_bsp_K(c, d_u);
+    launch_elem_K(c, d_u);
   FGPU_CHECK(hipEventRecord(c.ev[1], c.stream));
   // the element kernels of K and f_int both fill every CU with arithmetic;
   // f_int starts with the assembly, which waits on memory
@@ -673,12 +673,10 @@ int stiffness_fint_overlapped(Ctx &c, const double *d_u, double *d_g, int mode) 
   {
     OnStream2 on(c);
     FGPU_CHECK(hipEventRecord(c.ev[3], c.stream));
-    if (c.shape == FGPU_HEX8)
-      launch_hex8_fint(c, d_u);
-    else if (fused)
+    if (fused)
       launch_bsp_fint_pts(c);
     else
-      launch_bsp_fint(c, d_u);
+      launch_elem_fint(c, d_u);
     if (mode == 1) {
       launch_gather_fint(c, d_g);
       launch_penalty_force(c, d_u, d_g);
@@ -712,9 +710,11 @@ int fgpu_create(fgpu_ctx **out, const fgpu_mesh *m) {
   *out = nullptr;
   Ctx *c = new Ctx();
   try {
-    if (m->shape != FGPU_HEX8 && m->shape != FGPU_BSP) throw Error("unknown shape");
+    if (m->shape != FGPU_HEX8 && m->shape != FGPU_BSP && m->shape != FGPU_HEX20)
+      throw Error("unknown shape");
     if (m->form < FGPU_LINEAR || m->form > FGPU_TL_NEOHOOKE) throw Error("unknown form");
     if (m->shape == FGPU_HEX8 && m->nn != 8) throw Error("hex8 needs nn = 8");
+    if (m->shape == FGPU_HEX20 && m->nn != 20) throw Error("hex20 needs nn = 20");
     if (m->shape == FGPU_BSP) {
       if (m->nn != (m->deg[0] + 1) * (m->deg[1] + 1) * (m->deg[2] + 1))
         throw Error("B-spline solid: nn != prod(p+1)");
@@ -784,9 +784,9 @@ int fgpu_create(fgpu_ctx **out, const fgpu_mesh *m) {
     c->d_status2 = c->d_status + 4;
     build_maps(*c, m->conn);
     int nsym = nsym_blocks(c->nn);
-    c->kel = KeLayout{c->shape == FGPU_HEX8 ? 64 : 1, nsym};
+    c->kel = KeLayout{c->shape == FGPU_BSP ? 1 : 64, nsym};
     size_t ntile = (size_t)(c->nelem + c->kel.T - 1) / c->kel.T;
-    if (c->shape == FGPU_BSP && bsp_generic(*c))
+    if ((c->shape == FGPU_BSP && bsp_generic(*c)) || c->shape == FGPU_HEX20)
       c->d_Ke64 = dalloc<double>(ntile * c->kel.T * (size_t)nsym * 9);
     else
       c->d_Ke = dalloc<float>(ntile * c->kel.T * (size_t)nsym * 9);
@@ -1090,6 +1090,10 @@ static void upload_stress_extrap(Ctx &c) {
   std::vector<double> nodes;
   if (c.shape == FGPU_HEX8) {
     nodes = {-1, -1, 1, 1, -1, 1, 1, 1, 1, -1, 1, 1, -1, -1, -1, 1, -1, -1, 1, 1, -1, -1, 1, -1};
+  } else if (c.shape == FGPU_HEX20) {  // cShapeBrick20::NodeNatCoord
+    nodes = {-1, -1, 1, 0, -1, 1, 1, -1, 1, 1, 0, 1, 1, 1, 1, 0, 1, 1, -1, 1, 1,
+             -1, 0, 1, -1, -1, 0, 1, -1, 0, 1, 1, 0, -1, 1, 0, -1, -1, -1, 0, -1, -1,
+             1, -1, -1, 1, 0, -1, 1, 1, -1, 0, 1, -1, -1, 1, -1, -1, 0, -1};
   } else {
     const int nb[3] = {c.deg[0] + 1, c.deg[1] + 1, c.deg[2] + 1};
     for (int k = 0; k < nb[2]; k++)
@@ -1222,6 +1226,8 @@ int fgpu_stresses_dev(fgpu_ctx *ctx, const double *u_dev, double *sig_ip_dev,
     const bool at_nodes = c.stress_nodes == 0 && sig_node_dev;
     if (c.shape == FGPU_HEX8)
       launch_hex8_stress(c, u_dev, sig_ip_dev, at_nodes ? nullptr : sig_node_dev);
+    else if (c.shape == FGPU_HEX20)
+      launch_hex20_stress(c, u_dev, sig_ip_dev, sig_node_dev);
     else
       launch_bsp_stress(c, u_dev, sig_ip_dev, at_nodes ? nullptr : sig_node_dev);
     if (at_nodes) gen_launch_node_stress(c, u_dev, sig_node_dev);
@@ -1267,6 +1273,8 @@ int fgpu_body_force_dev(fgpu_ctx *ctx, const double *b_dev, double *f_dev) {
     FGPU_CHECK(hipMemsetAsync(c.d_status, 0, 4 * sizeof(int), c.stream));
     if (c.shape == FGPU_HEX8)
       launch_hex8_body(c, b_dev);
+    else if (c.shape == FGPU_HEX20)
+      launch_hex20_body(c, b_dev);
     else
       launch_bsp_body(c, b_dev);
     launch_gather_fint(c, f_dev);
@@ -1320,7 +1328,10 @@ int fgpu_add_surface_loads(fgpu_ctx *ctx, const fgpu_surface *sl) {
       if (sl->deg[d] < 1 || sl->deg[d] > 7) throw Error("surface loads: degrees 1..7");
     if (sl->nip < 1 || sl->nip > 16) throw Error("surface loads: 1..16 Gauss points");
     if (!sl->nodes || !sl->q) throw Error("surface loads: nodes and q are required");
-    const int n1 = sl->deg[0] + 1, n2 = sl->deg[1] + 1, nb = n1 * n2;
+    if (sl->shape != 0 && sl->shape != 1) throw Error("surface loads: shape 0 or 1");
+    if (sl->shape == 1 && (sl->deg[0] != 2 || sl->deg[1] != 2 || sl->ext))
+      throw Error("surface loads: serendipity faces are degree (2, 2) without extraction");
+    const int n1 = sl->deg[0] + 1, n2 = sl->deg[1] + 1, nb = sl->shape == 1 ? 8 : n1 * n2;
     const size_t nn = (size_t)sl->nface * nb;
     for (size_t k = 0; k < nn; k++)
       if (sl->nodes[k] < 0 || sl->nodes[k] >= c.nnode)
@@ -1331,6 +1342,7 @@ int fgpu_add_surface_loads(fgpu_ctx *ctx, const fgpu_surface *sl) {
     g.deg[0] = sl->deg[0];
     g.deg[1] = sl->deg[1];
     g.nip = sl->nip;
+    g.shape = sl->shape;
     g.row0 = c.nsrow;
     g.h_nodes.assign(sl->nodes, sl->nodes + nn);
     g.d_nodes = dupload(sl->nodes, nn, c.stream);

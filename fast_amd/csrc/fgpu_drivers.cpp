@@ -69,6 +69,25 @@ double element_force_len(Ctx &c, const double *u_nodal, double *tmp, int n) {
   return std::sqrt(dev_dot(c, tmp, tmp, n));
 }
 
+// Normwise backward error |r| / (| |K| |u| | + |f|) over the free equations
+// (2-norms) of the equation residual r (prescribed entries zeroed) at the
+// nodal state u.  The FP64 residual of an ill-conditioned system stalls at
+// a relative size near cond(K) times the rounding of the sums it is formed
+// from; there the backward error still shows whether the solution is as
+// accurate as FP64 allows.
+double backward_error(Ctx &c, const double *r, const double *u_nodal, double f_len, double *tmp,
+                      int n) {
+  launch_abs_spmv_nodal(c, u_nodal, c.d_w);
+  launch_nodal_to_eq(c, c.d_w, tmp);
+  launch_zero_penalty(c, tmp);
+  const double ku = std::sqrt(dev_dot(c, tmp, tmp, n));
+  const double rl = std::sqrt(dev_dot(c, r, r, n));
+  return ku + f_len > 0.0 ? rl / (ku + f_len) : 0.0;
+}
+// a refinement stalled at the FP64 floor with this backward error has
+// converged (the solution solves a system within this relative distance)
+constexpr double kBackwardTol = 1e-13;
+
 // Graphs of one Newton iteration, captured on first use in a call and
 // replayed for the rest of it; destroyed with the call.
 struct IterGraphs {
@@ -165,8 +184,14 @@ int fgpu_linear_static(fgpu_ctx *ctx, const double *f, double *u, int maxit, dou
       const double prev = rel;
       rel = scale > 0.0 ? std::sqrt(dev_dot(c, dr.p, dr.p, n)) / scale : 0.0;
       // at the floor of the FP64 residual (a first solve with prescribed
-      // equations is dominated by their penalty rows and does not count)
-      if (rel > 0.9 * prev && (k > 0 || !c.npen)) break;
+      // equations is dominated by their penalty rows and does not count):
+      // converged if the backward error is at the rounding level
+      if (rel > 0.9 * prev && (k > 0 || !c.npen)) {
+        if (rel > rtol &&
+            backward_error(c, dr.p, c.d_u, free_len(c, df.p, dd.p, n), dd.p, n) <= kBackwardTol)
+          conv = 1;
+        break;
+      }
     }
     if (!conv && rel <= rtol) conv = 1;
     FGPU_CHECK(hipMemcpyAsync(u, du.p, nb, hipMemcpyDeviceToHost, c.stream));
@@ -217,6 +242,9 @@ int fgpu_linear_static_direct(fgpu_ctx *ctx, const double *f, double *u, double 
       rel = scale > 0.0 ? std::sqrt(dev_dot(c, dr.p, dr.p, n)) / scale : 0.0;
       if (rel > 0.9 * prev && (k > 0 || !c.npen)) {  // at the floor of the FP64 residual
         k++;
+        if (rel > rtol &&
+            backward_error(c, dr.p, c.d_u, free_len(c, df.p, dd.p, n), dd.p, n) <= kBackwardTol)
+          conv = 1;
         break;
       }
     }

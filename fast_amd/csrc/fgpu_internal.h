@@ -112,6 +112,7 @@ struct KeLayout {
 // One group of uniform surface loads (fgpu_add_surface_loads), on the device
 struct SurfaceGroup {
   int nface = 0, deg[2] = {1, 1}, nip = 2;
+  int shape = 0;              // 0 tensor product, 1 8-node serendipity (20-node brick faces)
   long long row0 = 0;         // first face-vector row of the group (faces x nodes)
   int *d_nodes = nullptr;     // [nface * (p1+1)(p2+1)]
   double *d_ext = nullptr;    // [nface * ((p1+1)^2 + (p2+1)^2)] or null (identity)
@@ -291,6 +292,11 @@ void bsp_alloc_pts(Ctx &c);  // the records, up to 4 GB, at context creation
 void launch_bsp_K_pts(Ctx &c, const double *d_unodal);
 void launch_bsp_fint_pts(Ctx &c);
 void launch_hex8_body(Ctx &c, const double *d_b);   // element vectors into d_ge
+// 20-node bricks (k_hex20.hip): the same four entry points
+void launch_hex20_K(Ctx &c, const double *d_unodal);
+void launch_hex20_fint(Ctx &c, const double *d_unodal);
+void launch_hex20_body(Ctx &c, const double *d_b);
+void launch_hex20_stress(Ctx &c, const double *d_unodal, double *sig_ip, double *sig_node);
 void launch_bsp_body(Ctx &c, const double *d_b);
 // stresses at the points (sig_ip [nelem*nq*6]) and, when sig_node is not
 // null, extrapolated to the element nodes with d_TR ([nelem*nn*6])
@@ -316,6 +322,7 @@ void launch_penalty_diag(Ctx &c);                               // after the ass
 void launch_penalty_force(Ctx &c, const double *u, double *g);  // after the f_int gather
 void launch_zero_penalty(Ctx &c, double *x);  // x = 0 at the prescribed equations
 void launch_spmv_nodal(Ctx &c, const double *x, double *y);  // node vectors
+void launch_abs_spmv_nodal(Ctx &c, const double *x, double *y);  // |K| |x|, node vectors
 void launch_csr_convert(Ctx &c, double *d_csrval, int to_csr);
 // abs_exit: the reference's early return when |b| < rtol (an absolute
 // test, mlib/sysmat.cpp PCGSolver); callers that pass a relative tolerance

@@ -16,7 +16,8 @@
 // cShapeSurf::LocalSys, shp/shpsurf.cpp:121-164): u = x_,s / |x_,s|,
 // w = u x x_,t / |x_,t| (not normalised), v = w x u, load = q0 u + q1 v +
 // q2 w.  A brick face is the degree-(1,1) surface over its four corners with
-// identity extraction.  One thread per face accumulates its nodes' vectors in
+// identity extraction; a face of a 20-node brick is the 8-node serendipity
+// quadrilateral (surface_q8_kernel).  One thread per face accumulates its nodes' vectors in
 // point order; a gather sums every node's entries in entry order (FP64, no
 // atomics).
 #include "fgpu_dev.h"
@@ -142,6 +143,78 @@ __global__ __launch_bounds__(64) void surface_load_kernel(
   }
 }
 
+// Faces of 20-node bricks: the 8-node serendipity quadrilateral of cShapeQ8
+// (shp/shpplane.cpp:428-486), nodes counterclockwise from (-1, -1), corners
+// and mid-edge nodes alternating.  Corner c: N = (1 + r cr)(1 + s cs)
+// (r cr + s cs - 1) / 4; mid-edge node with coordinate k = 0:
+// N = (1 - x_k^2)(1 + x_j c_j) / 2.  Points, dA and the local frame as in
+// surface_load_kernel.
+__constant__ signed char kQ8[8][2] = {{-1, -1}, {0, -1}, {1, -1}, {1, 0},
+                                      {1, 1},   {0, 1},  {-1, 1}, {-1, 0}};
+
+__device__ inline double q8_shape(double r, double s, int a, double &dr, double &ds) {
+  const double cr = kQ8[a][0], cs = kQ8[a][1];
+  const double fr = 1.0 + r * cr, fs = 1.0 + s * cs;
+  if (cr != 0.0 && cs != 0.0) {
+    const double g = r * cr + s * cs - 1.0;
+    dr = 0.25 * cr * fs * (g + fr);
+    ds = 0.25 * cs * fr * (g + fs);
+    return 0.25 * fr * fs * g;
+  }
+  if (cr == 0.0) {
+    dr = -r * fs;
+    ds = 0.5 * (1.0 - r * r) * cs;
+    return 0.5 * (1.0 - r * r) * fs;
+  }
+  dr = 0.5 * (1.0 - s * s) * cr;
+  ds = -s * fr;
+  return 0.5 * (1.0 - s * s) * fr;
+}
+
+__global__ __launch_bounds__(64) void surface_q8_kernel(
+    int nface, int nip, const int *__restrict__ nodes, const int *__restrict__ local,
+    const double *__restrict__ q, const double *__restrict__ coords,
+    const double *__restrict__ gp, double *__restrict__ fv) {
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= nface) return;
+  const int *nd = nodes + (size_t)e * 8;
+  double *out = fv + (size_t)e * 24;
+  for (int k = 0; k < 24; k++) out[k] = 0.0;
+  const double qx = q[3 * e], qy = q[3 * e + 1], qz = q[3 * e + 2];
+  const bool loc = local && local[e];
+  for (int b = 0; b < nip; b++)
+    for (int a = 0; a < nip; a++) {
+      const double r = gp[2 * a], s = gp[2 * b];
+      double N[8], x1[3] = {0, 0, 0}, x2[3] = {0, 0, 0};
+      for (int k = 0; k < 8; k++) {
+        double dr, ds;
+        N[k] = q8_shape(r, s, k, dr, ds);
+        const double *X = coords + 3 * (size_t)nd[k];
+        for (int d = 0; d < 3; d++) {
+          x1[d] += dr * X[d];
+          x2[d] += ds * X[d];
+        }
+      }
+      const double cx = x1[1] * x2[2] - x1[2] * x2[1], cy = x1[2] * x2[0] - x1[0] * x2[2],
+                   cz = x1[0] * x2[1] - x1[1] * x2[0];
+      const double dA = (gp[2 * a + 1] * gp[2 * b + 1]) * sqrt(cx * cx + cy * cy + cz * cz);
+      double L[3] = {qx, qy, qz};
+      if (loc) {
+        const double l1 = sqrt(x1[0] * x1[0] + x1[1] * x1[1] + x1[2] * x1[2]);
+        const double l2 = sqrt(x2[0] * x2[0] + x2[1] * x2[1] + x2[2] * x2[2]);
+        const double u[3] = {x1[0] / l1, x1[1] / l1, x1[2] / l1};
+        const double t[3] = {x2[0] / l2, x2[1] / l2, x2[2] / l2};
+        const double w[3] = {u[1] * t[2] - u[2] * t[1], u[2] * t[0] - u[0] * t[2],
+                             u[0] * t[1] - u[1] * t[0]};
+        const double v[3] = {w[1] * u[2] - w[2] * u[1], w[2] * u[0] - w[0] * u[2],
+                             w[0] * u[1] - w[1] * u[0]};
+        for (int k = 0; k < 3; k++) L[k] = qx * u[k] + qy * v[k] + qz * w[k];
+      }
+      for (int k = 0; k < 8; k++)
+        for (int d = 0; d < 3; d++) out[3 * k + d] += N[k] * (dA * L[d]);
+    }
+}
+
 // One thread per node direction: the node's face-vector entries summed in
 // entry order (the adjacency lists them so), written to its equation.
 __global__ __launch_bounds__(256) void gather_surface_kernel(
@@ -209,6 +282,12 @@ void launch_penalty_force(Ctx &c, const double *u, double *g) {
 
 void launch_surface_group(Ctx &c, const SurfaceGroup &g, double *fv) {
   if (g.nface == 0) return;
+  if (g.shape == 1) {
+    hipLaunchKernelGGL(surface_q8_kernel, dim3((g.nface + 63) / 64), dim3(64), 0, c.stream,
+                       g.nface, g.nip, g.d_nodes, g.d_local, g.d_q, c.d_coords, g.d_gp, fv);
+    FGPU_CHECK(hipGetLastError());
+    return;
+  }
   hipLaunchKernelGGL(surface_load_kernel, dim3((g.nface + 63) / 64), dim3(64), 0, c.stream,
                      g.nface, g.deg[0], g.deg[1], g.nip, g.d_nodes, g.d_ext, g.d_local, g.d_q,
                      c.d_coords, c.d_cpw, g.d_gp, fv);

@@ -417,6 +417,25 @@ __global__ __launch_bounds__(256) void bspmv_kernel(int nnode, const long long *
                 (long long)gridDim.x * rows_per_block, bptr, bcol, bval, x, y);
 }
 
+// y = |K| |x| on node vectors (one thread per node direction): the scale of
+// the rounding in a residual f - K x, for backward-error tests
+__global__ __launch_bounds__(256) void abs_spmv_kernel(int nnode, const long long *__restrict__ bptr,
+                                                       const int *__restrict__ bcol,
+                                                       const double *__restrict__ bval,
+                                                       const double *__restrict__ x,
+                                                       double *__restrict__ y) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= 3 * nnode) return;
+  const int A = i / 3, d = i - 3 * A;
+  double s = 0.0;
+  for (long long k = bptr[A]; k < bptr[A + 1]; k++) {
+    const double *b = bval + 9 * k + 3 * d;
+    const double *xb = x + 3 * (size_t)bcol[k];
+    s += fabs(b[0]) * fabs(xb[0]) + fabs(b[1]) * fabs(xb[1]) + fabs(b[2]) * fabs(xb[2]);
+  }
+  y[i] = s;
+}
+
 template <int G>
 void bspmv_launch(Ctx &c, const double *x, double *y) {
   const int rows_per_block = TPB / G;
@@ -561,6 +580,12 @@ void launch_spmv_nodal(Ctx &c, const double *x, double *y) {
     case 8: bspmv_launch<8>(c, x, y); break;
     default: bspmv_launch<4>(c, x, y); break;
   }
+  FGPU_CHECK(hipGetLastError());
+}
+
+void launch_abs_spmv_nodal(Ctx &c, const double *x, double *y) {
+  hipLaunchKernelGGL(abs_spmv_kernel, dim3((3 * c.nnode + 255) / 256), dim3(256), 0, c.stream,
+                     c.nnode, c.d_bptr, c.d_bcol, c.d_bval, x, y);
   FGPU_CHECK(hipGetLastError());
 }
 

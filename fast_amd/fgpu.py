@@ -42,7 +42,7 @@ PREDICT_FN = C.CFUNCTYPE(C.c_int, C.c_void_p, C.c_int, C.c_void_p, C.c_double, C
 class FgpuSurface(C.Structure):
     _fields_ = [("nface", C.c_int), ("deg", C.c_int * 2), ("nip", C.c_int),
                 ("nodes", C.c_void_p), ("ext", C.c_void_p), ("local", C.c_void_p),
-                ("q", C.c_void_p)]
+                ("q", C.c_void_p), ("shape", C.c_int)]
 
 
 class NrParams(C.Structure):
@@ -414,7 +414,8 @@ class GpuModel:
 
     def add_surface_loads(self, group):
         """Append a group of surface loads (a SurfaceGroup of fast_amd.mesh:
-        deg, nip, nodes [nface, (p1+1)(p2+1)], ext or None, local, q)."""
+        deg, nip, nodes [nface, (p1+1)(p2+1)] (8 for "q8" faces), ext or
+        None, local, q, shape)."""
         nodes = np.ascontiguousarray(group.nodes, dtype=np.int32)
         ext = None if group.ext is None else np.ascontiguousarray(group.ext, dtype=np.float64)
         local = (None if group.local is None
@@ -425,6 +426,7 @@ class GpuModel:
         s.deg[0], s.deg[1] = int(group.deg[0]), int(group.deg[1])
         s.nip = int(group.nip)
         s.nodes, s.ext, s.local, s.q = _p(nodes), _p(ext), _p(local), _p(q)
+        s.shape = 1 if getattr(group, "shape", "tensor") == "q8" else 0
         self._check(lib().fgpu_add_surface_loads(self.h, C.byref(s)), "fgpu_add_surface_loads")
 
     def clear_surface_loads(self):

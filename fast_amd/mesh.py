@@ -5,8 +5,9 @@ The data model mirrors what FAST's element plugins read from a neutral file:
 * nodes (control points) with coordinates and supports
   (node/node.h:295-369, ``%NODE.COORD`` / ``%NODE.SUPPORT``);
 * one element family per mesh: 8-node hexahedra (``%ELEMENT.BRICK8[TL]``,
-  elc/element.cpp:1985-2041) or B-spline solids over one patch
-  (``%ELEMENT.BSPLINE.SOLID[TL]``, elc/elmparam.cpp:152-190);
+  elc/element.cpp:1985-2041), 20-node serendipity hexahedra
+  (``%ELEMENT.BRICK20[TL]``, elc/element.cpp:2043-2100) or B-spline solids
+  over one patch (``%ELEMENT.BSPLINE.SOLID[TL]``, elc/elmparam.cpp:152-190);
 * per-element isotropic material (E, nu) -- or (mu, lambda) for the
   neo-Hookean total-Lagrangian form, which the reference does not have;
 * the equation map ``dofmap[node, dir]`` with the semantics of
@@ -14,7 +15,9 @@ The data model mirrors what FAST's element plugins read from a neutral file:
   active when an element uses the node, supports deactivate it, and active
   directions are numbered 1.. in node order; 0 marks a fixed direction.
 
-Node order of a hex8 follows shp/shpsolid.cpp:173-184 (t = +1 face first).
+Node order of a hex8 follows shp/shpsolid.cpp:173-184 (t = +1 face first),
+of a hex20 shp/shpsolid.cpp:301-325 (t = +1 face counterclockwise, corners
+and mid-edge nodes alternating, the four mid-height nodes, the t = -1 face).
 Control points of a B-spline element follow shp/shpbsp.cpp:873-902
 (k(t) slowest, i(r) fastest) and patch indexing geom/patch.cpp:1221-1231.
 """
@@ -26,14 +29,14 @@ from typing import Optional, Sequence
 import numpy as np
 
 FORMS = {"linear": 0, "tl_svk": 1, "tl_neohooke": 2}
-SHAPES = {"hex8": 0, "bsp": 1}
+SHAPES = {"hex8": 0, "bsp": 1, "hex20": 2}
 
 
 @dataclasses.dataclass
 class Mesh:
     coords: np.ndarray                 # (nnode, 3) float64
     conn: np.ndarray                   # (nelem, nn) int32, 0-based
-    shape: str                         # "hex8" | "bsp"
+    shape: str                         # "hex8" | "bsp" | "hex20"
     nip: tuple                         # Gauss points per direction
     form: str                          # "linear" | "tl_svk" | "tl_neohooke"
     mat: np.ndarray                    # (nelem, 2) float64
@@ -108,14 +111,17 @@ class SurfaceGroup:
     first surface parameter fastest; ``ext`` holds its Bezier extraction
     rows of the two parameters, (p1+1)^2 then (p2+1)^2 per face, or is None
     for brick faces (degree (1, 1), nodes n0 n1 n3 n2 of the face).
-    ``local[f]`` = 1 gives the load in the face's local axes, ``q`` is the
-    load per unit area."""
+    ``shape`` "q8": faces of 20-node bricks, the 8-node serendipity
+    quadrilateral (deg (2, 2), the face's nodes in cShapeBrick20::GetFace
+    order).  ``local[f]`` = 1 gives the load in the face's local axes, ``q``
+    is the load per unit area."""
     deg: tuple
     nip: int
     nodes: np.ndarray                  # (nface, nb) int32
     q: np.ndarray                      # (nface, 3) float64
     ext: Optional[np.ndarray] = None   # (nface, (p1+1)^2 + (p2+1)^2) float64
     local: Optional[np.ndarray] = None  # (nface,) int32
+    shape: str = "tensor"              # "tensor" | "q8" (faces of 20-node bricks)
 
     def __post_init__(self):
         self.nodes = np.ascontiguousarray(self.nodes, dtype=np.int32)
@@ -199,6 +205,61 @@ def hex8_block(nx: int, ny: int, nz: int, size=(1.0, 1.0, 1.0), E=1.0,
                     fext[nid(nx, jj, kk)] += t * area / 4.0
     return Mesh(coords=coords, conn=conn, shape="hex8", nip=(nip, nip, nip),
                 form=form, mat=mat, fixed=fixed, fext=fext, grid=(nx, ny, nz))
+
+
+# ---------------------------------------------------------------- hex20 ---
+
+# natural coordinates of a 20-node brick's nodes (cShapeBrick20 node order)
+HEX20_NAT = np.array([
+    (-1, -1, 1), (0, -1, 1), (1, -1, 1), (1, 0, 1), (1, 1, 1), (0, 1, 1), (-1, 1, 1),
+    (-1, 0, 1), (-1, -1, 0), (1, -1, 0), (1, 1, 0), (-1, 1, 0), (-1, -1, -1),
+    (0, -1, -1), (1, -1, -1), (1, 0, -1), (1, 1, -1), (0, 1, -1), (-1, 1, -1),
+    (-1, 0, -1)], dtype=np.int64)
+
+# the eight nodes of each face, in the order of cShapeBrick20::GetFace
+# (shp/shpsolid.cpp:459-470; corners at positions 0, 2, 4, 6)
+HEX20_FACES = ((0, 1, 2, 3, 4, 5, 6, 7), (2, 1, 0, 8, 12, 13, 14, 9),
+               (4, 3, 2, 9, 14, 15, 16, 10), (6, 5, 4, 10, 16, 17, 18, 11),
+               (12, 19, 18, 17, 16, 15, 14, 13), (18, 11, 6, 7, 0, 8, 12, 19))
+
+
+def hex20_block(nx: int, ny: int, nz: int, size=(1.0, 1.0, 1.0), E=1.0, nu=0.3,
+                form="linear", nip=3, perturb=0.0, seed=0, clamp_x0=True,
+                mat=None) -> Mesh:
+    """Structured block of nx*ny*nz 20-node bricks.  Nodes: the points of the
+    (2nx+1)(2ny+1)(2nz+1) half-step lattice that a serendipity brick uses
+    (at most one odd lattice index), numbered x fastest.  ``perturb`` moves
+    interior nodes by U(-perturb, perturb) * h / 2 per direction."""
+    lx, ly, lz = size
+    mx, my, mz = 2 * nx + 1, 2 * ny + 1, 2 * nz + 1
+    k, j, i = np.meshgrid(np.arange(mz), np.arange(my), np.arange(mx), indexing="ij")
+    i, j, k = i.ravel(), j.ravel(), k.ravel()
+    keep = (i % 2) + (j % 2) + (k % 2) <= 1
+    lattice_id = np.full(mx * my * mz, -1, dtype=np.int64)
+    lattice_id[keep] = np.arange(int(keep.sum()))
+    coords = np.stack([i[keep] * (lx / (2 * nx)), j[keep] * (ly / (2 * ny)),
+                       k[keep] * (lz / (2 * nz))], axis=1).astype(np.float64)
+    if perturb:
+        rng = np.random.default_rng(seed)
+        ii, jj, kk = i[keep], j[keep], k[keep]
+        inner = (ii > 0) & (ii < mx - 1) & (jj > 0) & (jj < my - 1) & (kk > 0) & (kk < mz - 1)
+        h = np.array([lx / nx, ly / ny, lz / nz]) / 2.0
+        d = rng.uniform(-perturb, perturb, size=coords.shape) * h
+        coords[inner] += d[inner]
+    ez, ey, ex = np.meshgrid(np.arange(nz), np.arange(ny), np.arange(nx), indexing="ij")
+    ex, ey, ez = ex.ravel(), ey.ravel(), ez.ravel()
+    conn = np.empty((nx * ny * nz, 20), dtype=np.int32)
+    for a, (cr, cs, ct) in enumerate(HEX20_NAT):
+        li = (2 * ex + 1 + cr) + mx * ((2 * ey + 1 + cs) + my * (2 * ez + 1 + ct))
+        conn[:, a] = lattice_id[li]
+    nnode = coords.shape[0]
+    fixed = np.zeros((nnode, 3), dtype=bool)
+    if clamp_x0:
+        fixed[np.isclose(coords[:, 0], 0.0)] = True
+    if mat is None:
+        mat = np.tile([E, nu], (conn.shape[0], 1))
+    return Mesh(coords=coords, conn=conn, shape="hex20", nip=(nip, nip, nip), form=form,
+                mat=mat, fixed=fixed, grid=(nx, ny, nz))
 
 
 # ---------------------------------------------------------------- B-spline --

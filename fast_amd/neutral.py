@@ -1,8 +1,8 @@
 """Reader for the part of FAST's neutral input file (``.dat``) that this path
 runs, and writer of its nodal-displacement result section (``.pos``).
 
-Supported: solid 8-node bricks (``ELEMENT.BRICK8`` / ``ELEMENT.BRICK8TL``) and
-B-spline solids of degree up to 7 per direction (``ELEMENT.BSPLINE.SOLID`` /
+Supported: solid 8-node and 20-node bricks (``ELEMENT.BRICK8[TL]``,
+``ELEMENT.BRICK20[TL]``) and B-spline solids of degree up to 7 per direction (``ELEMENT.BSPLINE.SOLID`` /
 ``...SOLIDTL``) with isotropic elasticity (``MATERIAL.ISOTROPIC`` through
 ``SECTION.HOMOGENEOUS.ISOTROPIC.3D``), supports, the solver node order,
 uniform face loads on bricks and on B-spline patch faces in global or local
@@ -16,7 +16,8 @@ Formats follow the reference's readers (main/input.cpp:70-330 keyword table):
 NODE.COORD / NODE.SUPPORT / CONTROL.POINT.WEIGHT / NODE.SOLVER.ORDER
 (node/node.cpp:60-280), MATERIAL.ISOTROPIC (mat/material.cpp:824-831),
 INTEGRATION.ORDER (elc/element.cpp:194-228), element lines
-(elc/elmparam.cpp:221-245: label, section, order, then the shape's incidence;
+(elc/elmparam.cpp:221-245: label, section, order, then the shape's incidence,
+elc/element.cpp:2043-2100 for 20-node bricks;
 shp/shpbsp.cpp:57-110 for B-spline patch + spans), PATCH
 (geom/patch.cpp:229-264, 1460-1484; knot vectors geom/knotvec.cpp:93-175;
 control points shp/shpiga.cpp:174-215), face loads (load/load.cpp:1440-1480),
@@ -31,7 +32,7 @@ from typing import Dict, List, Optional
 
 import numpy as np
 
-from .mesh import Mesh, SurfaceGroup, bezier_extraction
+from .mesh import HEX20_FACES, Mesh, SurfaceGroup, bezier_extraction
 
 # keywords that only control printing / output; their data are skipped
 _OUTPUT_ONLY = {
@@ -194,7 +195,8 @@ def read(path: str) -> Deck:
         ords[o] = tuple(k)
 
     # elements (labels 1..nelem)
-    kinds = [k for k in ("ELEMENT.BRICK8", "ELEMENT.BRICK8TL", "ELEMENT.BSPLINE.SOLID",
+    kinds = [k for k in ("ELEMENT.BRICK8", "ELEMENT.BRICK8TL", "ELEMENT.BRICK20",
+                         "ELEMENT.BRICK20TL", "ELEMENT.BSPLINE.SOLID",
                          "ELEMENT.BSPLINE.SOLIDTL") if k in sec]
     if len(kinds) != 1:
         raise NotImplementedError("one element type per deck: found %s" % kinds)
@@ -205,8 +207,8 @@ def read(path: str) -> Deck:
     rows = []
     for _ in range(ne):
         lab_e, s_id, o_id = tk.int(), tk.int(), tk.int()
-        if kind.startswith("ELEMENT.BRICK8"):
-            inc = [idx[tk.int()] for _ in range(8)]
+        if kind.startswith("ELEMENT.BRICK"):
+            inc = [idx[tk.int()] for _ in range(20 if "BRICK20" in kind else 8)]
         else:
             inc = [tk.int() for _ in range(4)]  # patch, span r, s, t (1-based)
         rows.append((lab_e, s_id, o_id, inc))
@@ -217,9 +219,9 @@ def read(path: str) -> Deck:
         raise NotImplementedError("one integration order per deck")
     nip = nips.pop()
 
-    if kind.startswith("ELEMENT.BRICK8"):
+    if kind.startswith("ELEMENT.BRICK"):
         conn = np.asarray([r[3] for r in rows], dtype=np.int32)
-        mesh_kw = dict(shape="hex8")
+        mesh_kw = dict(shape="hex20" if "BRICK20" in kind else "hex8")
         ext = None
         deg = (1, 1, 1)
     else:
@@ -251,7 +253,7 @@ def read(path: str) -> Deck:
                     f[q - 1] += v[d]
     surfaces = []
     if "LOAD.CASE.AREA.UNIFORM" in sec:
-        if mesh.shape != "hex8":
+        if mesh.shape == "bsp":
             raise NotImplementedError("face loads on B-spline solids")
         surfaces.append(_face_group(mesh, sec["LOAD.CASE.AREA.UNIFORM"], idx))
     if "LOAD.CASE.AREA.IGA.UNIFORM" in sec:
@@ -322,7 +324,8 @@ _HANDLED = {"HEADER.ANALYSIS.ALGORITHM", "HEADER.ANALYSIS.MAXIMUM.STEPS",
             "HEADER.ANALYSIS.TOLERANCE",
             "NODE.COORD", "CONTROL.POINT.WEIGHT", "NODE.SUPPORT", "NODE.SOLVER.ORDER",
             "MATERIAL.ISOTROPIC", "SECTION.HOMOGENEOUS.ISOTROPIC.3D", "INTEGRATION.ORDER",
-            "ELEMENT.BRICK8", "ELEMENT.BRICK8TL", "ELEMENT.BSPLINE.SOLID",
+            "ELEMENT.BRICK8", "ELEMENT.BRICK8TL", "ELEMENT.BRICK20", "ELEMENT.BRICK20TL",
+            "ELEMENT.BSPLINE.SOLID",
             "ELEMENT.BSPLINE.SOLIDTL", "PATCH", "LOAD.CASE.AREA.UNIFORM",
             "LOAD.CASE.AREA.IGA.UNIFORM",
             "LOAD.CASE.BODY.FORCES", "LOAD.CASE.NODAL.FORCE", "LOAD.CASE.PRESCRIBED.DOF"}
@@ -441,23 +444,36 @@ def _iga_face_group(mesh, toks, elab, iga):
 def _face_group(mesh, toks, idx):
     """Uniform face loads on bricks (cFaceUnifLoad, load/load.cpp:1440-1498)
     as one surface group: the face is the first of the brick's faces in
-    table order that holds the three given corners (cShapeBrick8::GetFace),
-    its nodes n0 n1 n2 n3 given as n0 n1 n3 n2 (the bilinear surface, first
-    parameter fastest), on the element's Gauss order per direction
-    (cElemLoad::ExtForce, load.cpp:618-691)."""
+    table order whose corners hold the three given nodes (cShape::
+    GetShapeFace, shp/shape.cpp:461-489).  8-node bricks: its nodes n0 n1 n2
+    n3 given as n0 n1 n3 n2 (the bilinear surface, first parameter fastest);
+    20-node bricks: the face's eight nodes in cShapeBrick20::GetFace order
+    (the 8-node serendipity quadrilateral).  On the element's Gauss order per
+    direction (cElemLoad::ExtForce, load.cpp:618-691)."""
     tk = _Toks(toks, "LOAD.CASE.AREA.UNIFORM")
     nodes, local, q = [], [], []
+    hex20 = mesh.shape == "hex20"
     for _ in range(tk.int()):
         e = tk.int() - 1
         corners = {idx[tk.int()] for _ in range(3)}
         local.append(tk.int())
         q.append([tk.float(), tk.float(), tk.float()])
         cn = mesh.conn[e]
-        face = [fc for fc in _HEX8_FACES if corners <= {int(cn[a]) for a in fc}]
+        if hex20:
+            face = [fc for fc in HEX20_FACES
+                    if corners <= {int(cn[fc[k]]) for k in (0, 2, 4, 6)}]
+        else:
+            face = [fc for fc in _HEX8_FACES if corners <= {int(cn[a]) for a in fc}]
         if not face:
             raise ValueError("face of element %d not found" % (e + 1))
-        n0, n1, n2, n3 = (int(cn[a]) for a in face[0])
-        nodes.append([n0, n1, n3, n2])
+        if hex20:
+            nodes.append([int(cn[a]) for a in face[0]])
+        else:
+            n0, n1, n2, n3 = (int(cn[a]) for a in face[0])
+            nodes.append([n0, n1, n3, n2])
+    if hex20:
+        return SurfaceGroup(deg=(2, 2), nip=mesh.nip[0], nodes=np.asarray(nodes),
+                            q=np.asarray(q), local=np.asarray(local), shape="q8")
     return SurfaceGroup(deg=(1, 1), nip=mesh.nip[0], nodes=np.asarray(nodes), q=np.asarray(q),
                         local=np.asarray(local))
 

@@ -32,9 +32,11 @@ extern "C" {
 
 typedef struct fgpu_ctx fgpu_ctx;
 
-/* Element shape: cShapeBrick8 (shp/shpsolid.cpp:173-238) or cShapeBspSolid
- * (shp/shpbsp.cpp:873-1067, Bezier extraction per element). */
-enum { FGPU_HEX8 = 0, FGPU_BSP = 1 };
+/* Element shape: cShapeBrick8 (shp/shpsolid.cpp:173-238), cShapeBspSolid
+ * (shp/shpbsp.cpp:873-1067, Bezier extraction per element) or cShapeBrick20
+ * (shp/shpsolid.cpp:301-435, 20-node serendipity brick in the node order of
+ * its NodeNatCoord). */
+enum { FGPU_HEX8 = 0, FGPU_BSP = 1, FGPU_HEX20 = 2 };
 /* Formulation: cElmParam (elc/elmparam.cpp:449-597, linear), cElmParamTL
  * (elc/elmparam.cpp:1615-1807, Green-Lagrange + St.Venant-Kirchhoff), or
  * total-Lagrangian compressible neo-Hookean (no reference counterpart). */
@@ -45,13 +47,13 @@ enum { FGPU_LINEAR = 0, FGPU_TL_SVK = 1, FGPU_TL_NEOHOOKE = 2 };
  * (mat/material.cpp:846-851 cMatElastIso: E, nu; neo-Hookean: mu, lambda). */
 typedef struct {
   int nnode, nelem;
-  int shape;            /* FGPU_HEX8 | FGPU_BSP */
-  int nn;               /* nodes per element: 8, or prod(p_d+1) <= 512 */
+  int shape;            /* FGPU_HEX8 | FGPU_BSP | FGPU_HEX20 */
+  int nn;               /* nodes per element: 8, 20, or prod(p_d+1) <= 512 */
   int deg[3];           /* FGPU_BSP: degree per direction, 1..7; (3,3,3) and
                            (3,3,2) on 4x4x4 Gauss run the tuned FP32 MFMA
                            kernels, every other shape FP64 kernels */
   int nip[3];           /* Gauss points per direction (intpoint.cpp:579-617):
-                           hex8 isotropic 1..3; FGPU_BSP 1..16 each, at most
+                           hex8, hex20 isotropic 1..3; FGPU_BSP 1..16 each, at most
                            512 points per element */
   int form;             /* FGPU_LINEAR | FGPU_TL_SVK | FGPU_TL_NEOHOOKE */
   const double *coords; /* [nnode*3] */
@@ -343,16 +345,20 @@ int fgpu_gnn_newton_predictor(void *user, int step, const double *u_dev, double 
  * over its four corners with identity extraction (ext NULL): the nodes of
  * cShapeBrick8's face n0 n1 n2 n3 (shp/shpsolid.cpp:264-276) given as
  * n0 n1 n3 n2.  An IGA face is the patch's boundary surface over the
- * element's two spans. */
+ * element's two spans.  shape 1: faces of 20-node bricks, the 8-node
+ * serendipity quadrilateral of cShapeQ8 (shp/shpplane.cpp:428-486; deg
+ * (2, 2), ext NULL, the eight nodes of cShapeBrick20's face in its order,
+ * shp/shpsolid.cpp:459-470). */
 typedef struct {
   int nface;           /* faces of this group */
   int deg[2];          /* degree per surface parameter, 1..7 */
   int nip;             /* Gauss points per parameter, 1..16 */
-  const int *nodes;    /* [nface * (p1+1)(p2+1)] 0-based nodes */
+  const int *nodes;    /* [nface * (p1+1)(p2+1)] 0-based nodes (shape 1: 8) */
   const double *ext;   /* [nface * ((p1+1)^2 + (p2+1)^2)] row-major, or NULL
                           (identity: the Bernstein polynomials are the basis) */
   const int *local;    /* [nface] 1 = local axes, or NULL (all global) */
   const double *q;     /* [nface * 3] load per unit area */
+  int shape;           /* 0: tensor product (above); 1: 8-node serendipity */
 } fgpu_surface;
 /* Append a group of surface loads to the context (uploaded once; the
  * entries keep their order across groups); clear all of them. */

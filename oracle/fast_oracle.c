@@ -109,6 +109,80 @@ void orc_hex8_shape(double r, double s, double t, double *N, double *dN)
 }
 
 /* ------------------------------------------------------------------------
+ * 20-node serendipity brick (cShapeBrick20, shp/shpsolid.cpp:301-435): the
+ * node order of NodeNatCoord (t = +1 face first, counterclockwise from
+ * (-1,-1), corners and mid-edges alternating; then the four mid-height
+ * nodes; then the t = -1 face alike).  Functions from the serendipity
+ * definition: a corner node c = (cr, cs, ct) has
+ *   N = (1 + r cr)(1 + s cs)(1 + t ct)(r cr + s cs + t ct - 2) / 8,
+ * a mid-edge node with coordinate k = 0
+ *   N = (1 - x_k^2) prod_{j != k} (1 + x_j c_j) / 4.
+ * ---------------------------------------------------------------------- */
+static const double kHex20Nat[20][3] = {
+    {-1, -1, 1}, {0, -1, 1}, {1, -1, 1}, {1, 0, 1}, {1, 1, 1}, {0, 1, 1}, {-1, 1, 1},
+    {-1, 0, 1}, {-1, -1, 0}, {1, -1, 0}, {1, 1, 0}, {-1, 1, 0}, {-1, -1, -1},
+    {0, -1, -1}, {1, -1, -1}, {1, 0, -1}, {1, 1, -1}, {0, 1, -1}, {-1, 1, -1}, {-1, 0, -1}};
+
+void orc_hex20_shape(double r, double s, double t, double *N, double *dN)
+{
+  const double x[3] = {r, s, t};
+  for (int a = 0; a < 20; a++) {
+    const double *c = kHex20Nat[a];
+    int k = -1;
+    for (int j = 0; j < 3; j++)
+      if (c[j] == 0.0) k = j;
+    double f[3];
+    for (int j = 0; j < 3; j++) f[j] = 1.0 + x[j] * c[j];
+    if (k < 0) {  /* corner */
+      const double g = x[0] * c[0] + x[1] * c[1] + x[2] * c[2] - 2.0;
+      if (N) N[a] = 0.125 * f[0] * f[1] * f[2] * g;
+      if (dN)
+        for (int d = 0; d < 3; d++) {
+          /* d/dx_d of f_d g, times the other two factors */
+          const double fo = f[(d + 1) % 3] * f[(d + 2) % 3];
+          dN[3 * a + d] = 0.125 * c[d] * fo * (g + f[d]);
+        }
+    } else {  /* mid-edge, coordinate k zero */
+      const int j1 = (k + 1) % 3, j2 = (k + 2) % 3;
+      const double q = 1.0 - x[k] * x[k];
+      if (N) N[a] = 0.25 * q * f[j1] * f[j2];
+      if (dN) {
+        dN[3 * a + k] = -0.5 * x[k] * f[j1] * f[j2];
+        dN[3 * a + j1] = 0.25 * q * c[j1] * f[j2];
+        dN[3 * a + j2] = 0.25 * q * f[j1] * c[j2];
+      }
+    }
+  }
+}
+
+/* 8-node serendipity quadrilateral (cShapeQ8, shp/shpplane.cpp:428-486):
+ * nodes counterclockwise from (-1,-1), corners and mid-edges alternating;
+ * corner N = (1 + r cr)(1 + s cs)(r cr + s cs - 1) / 4, mid-edge
+ * N = (1 - x_k^2)(1 + x_j c_j) / 2.  dN: [8][2]. */
+void orc_q8_shape(double r, double s, double *N, double *dN)
+{
+  static const double c8[8][2] = {{-1, -1}, {0, -1}, {1, -1}, {1, 0},
+                                  {1, 1},   {0, 1},  {-1, 1}, {-1, 0}};
+  const double x[2] = {r, s};
+  for (int a = 0; a < 8; a++) {
+    const double *c = c8[a];
+    if (c[0] != 0.0 && c[1] != 0.0) {
+      const double f0 = 1.0 + r * c[0], f1 = 1.0 + s * c[1];
+      const double g = r * c[0] + s * c[1] - 1.0;
+      N[a] = 0.25 * f0 * f1 * g;
+      dN[2 * a] = 0.25 * c[0] * f1 * (g + f0);
+      dN[2 * a + 1] = 0.25 * c[1] * f0 * (g + f1);
+    } else {
+      const int k = c[0] == 0.0 ? 0 : 1, j = 1 - k;
+      const double q = 1.0 - x[k] * x[k], fj = 1.0 + x[j] * c[j];
+      N[a] = 0.5 * q * fj;
+      dN[2 * a + k] = -x[k] * fj;
+      dN[2 * a + j] = 0.5 * q * c[j];
+    }
+  }
+}
+
+/* ------------------------------------------------------------------------
  * Bernstein basis on [0,1].  Reference: geom/bernbasis.cpp:91-106
  * (CompPols, de Casteljau-style triangle) and 110-124 (CompDervPols:
  * n * (B_{i-1,n-1} - B_{i,n-1})).
@@ -223,6 +297,10 @@ static void natural_derivs(const orc_etype *et, const double *ext,
 {
   if (et->shape == ORC_HEX8) {
     orc_hex8_shape(rst[0], rst[1], rst[2], Nval, dN);
+    return;
+  }
+  if (et->shape == ORC_HEX20) {
+    orc_hex20_shape(rst[0], rst[1], rst[2], Nval, dN);
     return;
   }
   const int *p = et->deg;
@@ -663,7 +741,7 @@ int orc_elem_stress(const orc_etype *et, const double *X, const double *ext,
 }
 
 /* Natural coordinates of the element's nodes: cShapeBrick8::NodeNatCoord
- * (shp/shpsolid.cpp:173-185) and cShapeBspSolid::NodeNatCoord
+ * (shp/shpsolid.cpp:173-185), cShapeBrick20::NodeNatCoord (301-325) and cShapeBspSolid::NodeNatCoord
  * (shp/shpbsp.cpp:906-923: a uniform (p+1)^3 grid on [-1, 1]^3, r fastest). */
 static void node_nat_coords(const orc_etype *et, double *c)
 {
@@ -671,6 +749,10 @@ static void node_nat_coords(const orc_etype *et, double *c)
     static const double h[8][3] = {{-1, -1, 1}, {1, -1, 1}, {1, 1, 1}, {-1, 1, 1},
                                    {-1, -1, -1}, {1, -1, -1}, {1, 1, -1}, {-1, 1, -1}};
     memcpy(c, h, sizeof h);
+    return;
+  }
+  if (et->shape == ORC_HEX20) {
+    memcpy(c, kHex20Nat, sizeof kHex20Nat);
     return;
   }
   int nb[3] = {et->deg[0] + 1, et->deg[1] + 1, et->deg[2] + 1}, n = 0;
@@ -867,7 +949,9 @@ int orc_assemble_body(const orc_mesh *m, const double *b, int neq, double *f)
  * first fastest), as for the solid (shp/shpbsp.cpp:254-300, 985-1067). */
 int orc_surface_load(const orc_mesh *m, const orc_surface *s, double *f)
 {
-  const int p1 = s->deg[0], p2 = s->deg[1], n1 = p1 + 1, n2 = p2 + 1, nb = n1 * n2;
+  const int q8 = s->shape == 1;
+  if (q8 && (s->ext || s->deg[0] != 2 || s->deg[1] != 2)) return 0;
+  const int p1 = s->deg[0], p2 = s->deg[1], n1 = p1 + 1, n2 = p2 + 1, nb = q8 ? 8 : n1 * n2;
   const int q4 = !s->ext && p1 == 1 && p2 == 1;
   double r[16], wr[16];
   if (nb > 64 || s->nip < 1 || s->nip > 16 || !orc_gauss(s->nip, r, wr)) return 0;
@@ -891,7 +975,11 @@ int orc_surface_load(const orc_mesh *m, const orc_surface *s, double *f)
       for (int j = 0; j < s->nip; j++) {
         const double pr = r[j], ps = r[i], wgt = wr[j] * wr[i];
         double N[64], dr[64], ds[64];
-        if (q4) {
+        if (q8) {
+          double d8[16];
+          orc_q8_shape(pr, ps, N, d8);
+          for (int a = 0; a < 8; a++) { dr[a] = d8[2 * a]; ds[a] = d8[2 * a + 1]; }
+        } else if (q4) {
           N[0] = 0.25 * (1.0 - pr) * (1.0 - ps);
           N[1] = 0.25 * (1.0 + pr) * (1.0 - ps);
           N[2] = 0.25 * (1.0 + pr) * (1.0 + ps);

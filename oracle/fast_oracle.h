@@ -17,12 +17,12 @@
 extern "C" {
 #endif
 
-enum { ORC_HEX8 = 0, ORC_BSP = 1 };
+enum { ORC_HEX8 = 0, ORC_BSP = 1, ORC_HEX20 = 2 };
 enum { ORC_LINEAR = 0, ORC_TL_SVK = 1, ORC_TL_NEOHOOKE = 2 };
 
 /* Element family description (one per mesh). */
 typedef struct {
-  int shape;      /* ORC_HEX8 | ORC_BSP */
+  int shape;      /* ORC_HEX8 | ORC_BSP | ORC_HEX20 */
   int nn;         /* nodes / control points per element */
   int deg[3];     /* B-spline degree per parametric direction (ORC_BSP) */
   int nip[3];     /* Gauss points per direction */
@@ -48,6 +48,10 @@ int  orc_gauss(int n, double *r, double *w);
 int  orc_brick_points(const int nip[3], double *rst, double *w);
 
 void orc_hex8_shape(double r, double s, double t, double *N, double *dN);
+/* 20-node serendipity brick, node order of cShapeBrick20; dN [20][3] */
+void orc_hex20_shape(double r, double s, double t, double *N, double *dN);
+/* 8-node serendipity quadrilateral, node order of cShapeQ8; dN [8][2] */
+void orc_q8_shape(double r, double s, double *N, double *dN);
 void orc_bernstein(int p, double u, double *B);
 void orc_bernstein_deriv(int p, double u, double *dB);
 /* Bezier extraction operators for a knot vector (nspan*(p+1)^2). */
@@ -80,7 +84,9 @@ int  orc_assemble_fint(const orc_mesh *m, const double *u_nodal, int neq,
  * cFaceUnifLoadIGA faces): one group of faces of degree deg, nip x nip
  * Gauss points, nodes first surface parameter fastest, per-face extraction
  * rows of the two parameters (NULL: a brick face of degree (1, 1)), local
- * flags (NULL: global) and loads q [nface*3].  Adds into f [neq]. */
+ * flags (NULL: global) and loads q [nface*3].  shape 1: 8-node serendipity
+ * faces of 20-node bricks (deg (2, 2), ext NULL, nodes in cShapeQ8's order).
+ * Adds into f [neq]. */
 typedef struct {
   int nface;
   int deg[2];
@@ -89,6 +95,7 @@ typedef struct {
   const double *ext;
   const int *local;
   const double *q;
+  int shape;
 } orc_surface;
 int  orc_surface_load(const orc_mesh *m, const orc_surface *s, double *f);
 /* Stresses: integration-point values (Voigt xx yy zz xy xz yz, points in

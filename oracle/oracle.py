@@ -15,7 +15,7 @@ _HERE = os.path.dirname(os.path.abspath(__file__))
 _LIB = None
 
 FORMS = {"linear": 0, "tl_svk": 1, "tl_neohooke": 2}
-SHAPES = {"hex8": 0, "bsp": 1}
+SHAPES = {"hex8": 0, "bsp": 1, "hex20": 2}
 
 
 class EType(C.Structure):
@@ -45,6 +45,8 @@ def lib():
         L.orc_gauss.argtypes = [i, vp, vp]
         L.orc_brick_points.argtypes = [vp, vp, vp]
         L.orc_hex8_shape.argtypes = [d, d, d, vp, vp]
+        L.orc_hex20_shape.argtypes = [d, d, d, vp, vp]
+        L.orc_q8_shape.argtypes = [d, d, vp, vp]
         L.orc_bernstein.argtypes = [i, d, vp]
         L.orc_bernstein_deriv.argtypes = [i, d, vp]
         L.orc_bezier_extraction.argtypes = [i, i, vp, vp]
@@ -108,10 +110,22 @@ def hex8_shape(r, s, t):
     return N, dN
 
 
+def hex20_shape(r, s, t):
+    N, dN = np.zeros(20), np.zeros((20, 3))
+    lib().orc_hex20_shape(r, s, t, _p(N), _p(dN))
+    return N, dN
+
+
+def q8_shape(r, s):
+    N, dN = np.zeros(8), np.zeros((8, 2))
+    lib().orc_q8_shape(r, s, _p(N), _p(dN))
+    return N, dN
+
+
 class OSurface(C.Structure):
     _fields_ = [("nface", C.c_int), ("deg", C.c_int * 2), ("nip", C.c_int),
                 ("nodes", C.c_void_p), ("ext", C.c_void_p), ("local", C.c_void_p),
-                ("q", C.c_void_p)]
+                ("q", C.c_void_p), ("shape", C.c_int)]
 
 
 class OracleMesh:
@@ -198,8 +212,8 @@ class OracleMesh:
         """Equation-ordered load vector of uniform surface loads, groups in
         order (each with deg, nip, nodes [nface, nb] first parameter fastest,
         ext [nface, (p1+1)^2 + (p2+1)^2] or None, local [nface] or None and
-        q [nface, 3]): cElemLoad::ExtForce over each face, load/load.cpp
-        618-691."""
+        q [nface, 3]; shape "q8": 8-node faces of 20-node bricks):
+        cElemLoad::ExtForce over each face, load/load.cpp 618-691."""
         f = np.zeros(self.mesh.neq)
         for g in groups:
             nodes = np.ascontiguousarray(g.nodes, dtype=np.int32)
@@ -211,6 +225,7 @@ class OracleMesh:
             s.deg[0], s.deg[1] = int(g.deg[0]), int(g.deg[1])
             s.nip = int(g.nip)
             s.nodes, s.ext, s.local, s.q = _p(nodes), _p(ext), _p(loc), _p(q)
+            s.shape = 1 if getattr(g, "shape", "tensor") == "q8" else 0
             if not lib().orc_surface_load(C.byref(self.om), C.byref(s), _p(f)):
                 raise ValueError("surface load group rejected by the oracle")
         return f

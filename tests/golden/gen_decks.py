@@ -23,7 +23,11 @@ Decks in tests/golden/decks/:
   BeamB8Face.dat  the same beam under a uniform load on its distorted top
       faces in the faces' local axes (tangential and normal components);
   ScordelisLoDeg3TL.dat  the Scordelis-Lo deck with total-Lagrangian
-      B-spline solids under load control (2 steps).
+      B-spline solids under load control (2 steps);
+  BeamB20.dat  the reference's dat/continuum/BeamB20.dat: a 2x2x50
+      cantilever of 20-node bricks under uniform face loads on its tip faces;
+  BeamB20TL.dat  the same beam with total-Lagrangian 20-node bricks under
+      load control, 2 steps of 20 times the load.
 This script runs the reference program built from its sources
 (``make -C oracle ref`` -> oracle/_ref/fast) on a temporary copy of each deck
 with ``%HEADER.OUTPUT.PRECISION 16`` added, and stores what it prints for the
@@ -53,7 +57,8 @@ from fast_amd import neutral  # noqa: E402
 
 REF = os.path.join(ROOT, "oracle", "_ref", "fast")
 DECKS = ("TestB8", "ScordelisLoDeg3", "PinCylDeg4", "ThinCylinderDeg5", "CurvedBeamDeg4",
-         "BeamB8", "BeamB8TL", "BeamB8Face", "ScordelisLoDeg3TL", "PinSemiCylDeg4")
+         "BeamB8", "BeamB8TL", "BeamB8Face", "ScordelisLoDeg3TL", "PinSemiCylDeg4", "BeamB20",
+         "BeamB20TL")
 PRINT_STEP = {"PinSemiCylDeg4": 2}   # HEADER.ANALYSIS.PRINT.STEPS of the run
 RUN_STEPS = {"PinSemiCylDeg4": 40}   # HEADER.ANALYSIS.MAXIMUM.STEPS of the run
 SIG_ELEMS = {"PinSemiCylDeg4": 32}   # stresses kept for the first elements only
@@ -123,6 +128,11 @@ def deck_text(name: str) -> str:
         text = open(os.path.join(HERE, "decks", "ScordelisLoDeg3.dat")).read()
         text = text.replace("%ELEMENT.BSPLINE.SOLID\n", "%ELEMENT.BSPLINE.SOLIDTL\n", 1)
         return text.replace("%NODE\n", LCM.format(steps=2, factor=0.5, tol="1.0e-8") + "%NODE\n", 1)
+    if name == "BeamB20TL":
+        text = open(os.path.join(HERE, "decks", "BeamB20.dat")).read()
+        text = text.replace("%ELEMENT.BRICK20\n", "%ELEMENT.BRICK20TL\n", 1)
+        return text.replace("%NODE\n", LCM.format(steps=2, factor=20.0, tol="1.0e-8") + "%NODE\n",
+                            1)
     return open(os.path.join(HERE, "decks", name + ".dat")).read()
 
 
