@@ -124,15 +124,16 @@ int GpuInit(void)
   }
 
   // one element family: linear or total-Lagrangian parametric solids, all
-  // bricks or all B-spline solids of one degree and one integration rule
+  // 8-node bricks, all 20-node bricks or all B-spline solids of one degree
+  // and one integration rule
   cElement *e0 = cElement :: GetElm(0);
   eShpType st = e0->GetShape( )->GetType( );
-  if (st != SHAPE_BRICK8 && st != SHAPE_BSP_SOLID)
-    return Fail("GpuInit: the device path takes BRICK8 and B-spline solid elements");
+  if (st != SHAPE_BRICK8 && st != SHAPE_BRICK20 && st != SHAPE_BSP_SOLID)
+    return Fail("GpuInit: the device path takes BRICK8, BRICK20 and B-spline solid elements");
   fgpu_mesh m = fgpu_mesh( );
   m.nnode = nnode;
   m.nelem = nelm;
-  m.shape = st == SHAPE_BRICK8 ? FGPU_HEX8 : FGPU_BSP;
+  m.shape = st == SHAPE_BRICK8 ? FGPU_HEX8 : st == SHAPE_BRICK20 ? FGPU_HEX20 : FGPU_BSP;
   m.nn = e0->GetShape( )->GetNumNode( );
   m.form = e0->GetType( ) == PARAMETRICTL ? FGPU_TL_SVK : FGPU_LINEAR;
   PointsPerDirection(e0, m.nip);

@@ -28,7 +28,7 @@ pytestmark = pytest.mark.gpu
 EXE = os.path.join(ROOT, "oracle", "_ref", "ref_gpu")
 CASES = [("TestB8", "direct"), ("BeamB8", "pcg"), ("BeamB8Face", "direct"),
          ("BeamB8TL", "pcg"), ("PinCylDeg4", "direct"), ("ThinCylinderDeg5", "direct"),
-         ("CurvedBeamDeg4", "direct")]
+         ("CurvedBeamDeg4", "direct"), ("BeamB20", "direct"), ("BeamB20TL", "direct")]
 
 
 def read_steps(path):
