@@ -90,3 +90,36 @@ def test_inverted_element_is_reported(fg):
         g.stiffness()
     with pytest.raises(fg.FgpuError, match="inverted elements"):
         g.int_force(np.zeros((m.nnode, 3)))
+
+
+def test_pinched_semicylinder_matches_reference_run(fg, tmp_path):
+    """PinSemiCylDeg4 (256 total-Lagrangian NURBS solids of degree (4, 4, 2),
+    a point load, 'lcm' steps of 0.025; TARGET.ITERATIONS read and unused, as
+    the reference's load control does) for the 40 steps the fixture holds,
+    direct solver: at each of the reference program's 20 printed steps the
+    displacement field to 1e-6 of the largest and the loaded node's w to
+    1e-6 relative; the last step's stresses of the first 32 elements."""
+    from fast_amd import deck, neutral
+    text = open(os.path.join(DECKS, "PinSemiCylDeg4.dat")).read()
+    k0 = text.index("%HEADER.ANALYSIS.MAXIMUM.STEPS")
+    k1 = text.index("%", k0 + 1)
+    text = text[:k0] + "%HEADER.ANALYSIS.MAXIMUM.STEPS\n40\n\n" + text[k1:]
+    path = tmp_path / "PinSemiCylDeg4.dat"
+    path.write_text(text)
+    out = str(tmp_path / "psc.pos")
+    d, un, sig = deck.run(str(path), out=out, solver="direct")
+    ref = load_golden("deck_PinSemiCylDeg4.npz")
+    steps = neutral.read_pos(out)
+    assert len(steps) == 40
+    node = int(np.flatnonzero(d.labels == 400)[0])
+    kref = int(np.flatnonzero(ref["labels"] == 400)[0])
+    for i in range(20):
+        s = steps[2 * i + 1]
+        assert abs(s["factor"] - ref["factors"][i]) <= 1e-12
+        assert_same_displacements(d, s["u"], ref["u_steps"][i], "PinSemiCylDeg4", tol=1e-6)
+        w, w0 = s["u"][node, 1], ref["u_steps"][i][kref, 1]
+        assert abs(w - w0) <= 1e-6 * abs(w0), (i, w, w0)
+    ne = len(ref["sig_ip"])
+    scale = np.abs(ref["sig_ip"]).max()
+    assert np.abs(sig[0][:ne] - ref["sig_ip"]).max() <= 1e-5 * scale
+    assert np.abs(sig[1][:ne] - ref["sig_node"]).max() <= 1e-5 * scale

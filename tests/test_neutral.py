@@ -160,3 +160,51 @@ def assert_same_displacements(d, un, ref, name, tol=1e-10):
         assert np.allclose(un[[1, 2, 5, 6], 0], 100e6 * 5.0 / 200e9, rtol=1e-12)
     err = np.abs(diff).max() / np.abs(ref).max()
     assert err <= tol, err
+
+
+# the pinched semi-cylinder's header (dat/iga/PinSemiCylDeg4.dat:10-29): load
+# factor and the "FAST" column of the displacement w at the loading point
+PSC_HEADER = [(0.05, -5.39093), (0.10, -15.7282), (0.15, -27.1612), (0.20, -37.3189),
+              (0.25, -48.8163), (0.30, -66.7192), (0.35, -94.8713), (0.40, -113.724),
+              (0.45, -124.549), (0.50, -132.390), (0.55, -138.921), (0.60, -144.041),
+              (0.65, -148.793), (0.70, -152.909), (0.75, -156.634), (0.80, -160.468),
+              (0.85, -163.393), (0.90, -166.205), (0.95, -169.427), (1.00, -171.766)]
+
+
+def test_pinched_semicylinder_fixture_and_header():
+    """PinSemiCylDeg4 as the reference program built from its shipped
+    sources runs it (tests/golden/gen_decks.py: 40 load-control steps of
+    0.025, every second printed): the printed load factors are the header's,
+    and the displacement w (y) of the loaded node 400 follows the header's
+    FAST column to within 0.75% -- not to the digits printed: the shipped
+    program differs from its own header by 0.03% to 0.7%, so the header was
+    made with another version.  The device is held to the program's output
+    (tests/test_gpu_decks.py)."""
+    d = neutral.read(os.path.join(DECKS, "PinSemiCylDeg4.dat"))
+    assert d.mesh.shape == "bsp" and d.mesh.nelem == 256 and d.mesh.deg == (4, 4, 2)
+    assert d.analysis == "newton" and d.steps == 100 and d.factor == 0.025
+    ref = load_golden("deck_PinSemiCylDeg4.npz")
+    assert int(ref["nsteps"]) == 40 and len(ref["factors"]) == 20
+    k = int(np.flatnonzero(ref["labels"] == 400)[0])
+    dev = []
+    for i, (lf, w) in enumerate(PSC_HEADER):
+        assert abs(ref["factors"][i] - lf) <= 1e-12
+        dev.append(abs(ref["u_steps"][i][k, 1] - w) / abs(w))
+    print("reference program vs header: %.2e .. %.2e" % (min(dev), max(dev)))
+    assert max(dev) <= 7.5e-3 and min(dev) >= 1e-4
+
+
+def test_pinched_semicylinder_stresses(oracle_mod):
+    """The oracle's integration-point and element-nodal stresses of the
+    reference's last printed state, on the first 32 elements the fixture
+    keeps (degree (4, 4, 2) total-Lagrangian NURBS solids, 5x5x5 points)."""
+    d = neutral.read(os.path.join(DECKS, "PinSemiCylDeg4.dat"))
+    ref = load_golden("deck_PinSemiCylDeg4.npz")
+    om = oracle_mod.OracleMesh(d.mesh)
+    u = reference_state(d, ref)
+    si, sn, sa, ok = om.stresses(u)
+    assert ok
+    ne = len(ref["sig_ip"])
+    scale = np.abs(ref["sig_ip"]).max()
+    assert np.abs(si[:ne] - ref["sig_ip"]).max() <= 1e-11 * scale
+    assert np.abs(sn[:ne] - ref["sig_node"]).max() <= 1e-11 * scale
