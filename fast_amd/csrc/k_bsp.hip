@@ -809,11 +809,17 @@ __global__ __launch_bounds__(256, 3) void bsp_Ktm_kernel(
 // Phase 1, thread (q, part): geometry, then grad u at q from its 16 control
 // points (shuffle-reduced), then c times the stress that multiplies the
 // gradients: sigma (linear) or P = F S (total Lagrangian; the reference's
-// (B + Bl)^T S, elmparam.cpp:1615-1707).  Phase 2, thread (a, quarter):
-// g_a = sum_q c P_q grad N_a(q) over every fourth point (q = quarter mod 4:
-// the four quarters of a wave read neighbouring rows of the per-point LDS
-// arrays, in distinct banks), recomputing grad N_a from the tensor tables
-// and the stored J^-1, reduced over the 4 quarters.
+// (B + Bl)^T S, elmparam.cpp:1615-1707).
+//
+// Phase 2 by sum factorization.  With T = (c sigma or c P) J^-1 per point,
+// 1/W and dW the rational sums and d_a, n_a the tensor-product numerators
+// (w_a times dN_r N_s N_t, N_r dN_s N_t, N_r N_s dN_t and N_r N_s N_t):
+//   g_a[i] = sum_q sum_k T_ik dR_a,k = sum_q (sum_{k<3} Z_ik d_a,k + Z_i3 n_a),
+//   Z_ik = T_ik / W (k < 3),  Z_i3 = -(sum_k T_ik dW_k) / W^2,
+// and every term factors over the three point directions: the sums over
+// q_t, then q_s, then q_r are three small contractions with the
+// univariate tables (6.9 k multiply-adds per cubic element instead of
+// 64 x 64 gradients).
 template <int NB2, int FORM>
 struct FShared {
   Geo<NB2> g;
@@ -822,54 +828,87 @@ struct FShared {
   double rs[NQP][4];   // 1 / W, dW/dr, dW/ds, dW/dt
   // c * sigma (Voigt, linear) or c * P (row-major, total-Lagrangian)
   double sig[NQP][FORM == FGPU_LINEAR ? 6 : 9];
+  double Z[3][4][NQP];                 // [i][k][q]
+  double Y[3][3][NQ1 * NQ1 * NB];      // after the q_t sums: [i][term][(q_r, q_s, k_t)]
+  double V[3][2][NQ1 * NB * NB];       // after the q_s sums: [i][term][(q_r, j_s, k_t)]
 };
 
-// Second phase of the internal force, thread (a, quarter): g_a = sum over
-// every fourth point of (c sigma or c P) grad N_a, grad N_a recomputed from
-// the tensor tables, the point's J^-1 and rational sums; the quarters reduce
-// with shuffles.
 template <int NB2, int FORM>
-__device__ inline void fint_points(const FShared<NB2, FORM> &S, int e, double *__restrict__ ge) {
+__device__ inline void fint_points(FShared<NB2, FORM> &S, int e, double *__restrict__ ge) {
   constexpr int NBAS = Bsp<NB2>::NBAS;
   const int tid = threadIdx.x;
-  if (tid < 4 * NBAS) {  // whole waves: 4 NBAS is 256 or 192
-    const int a = tid / 4, quarter = tid % 4;
-    double g0 = 0.0, g1 = 0.0, g2 = 0.0;
-    #pragma unroll
-    for (int q = quarter; q < NQP; q += 4) {  // the quarters' points side by side in LDS
-      double n, d[3];
-      tensor(S.g, q, a, n, d);
-      const double is = S.rs[q][0];
-      const double R = n * is;
-      double dR[3], G[3];
-      for (int i = 0; i < 3; i++) dR[i] = (d[i] - R * S.rs[q][1 + i]) * is;
-      for (int i = 0; i < 3; i++)
-        G[i] = S.Ji[q][3 * i] * dR[0] + S.Ji[q][3 * i + 1] * dR[1] + S.Ji[q][3 * i + 2] * dR[2];
-      const double *s = S.sig[q];
-      if constexpr (FORM == FGPU_LINEAR) {
-        g0 += s[0] * G[0] + s[3] * G[1] + s[4] * G[2];
-        g1 += s[3] * G[0] + s[1] * G[1] + s[5] * G[2];
-        g2 += s[4] * G[0] + s[5] * G[1] + s[2] * G[2];
-      } else {
-        g0 += s[0] * G[0] + s[1] * G[1] + s[2] * G[2];
-        g1 += s[3] * G[0] + s[4] * G[1] + s[5] * G[2];
-        g2 += s[6] * G[0] + s[7] * G[1] + s[8] * G[2];
-      }
+  // Z per point and component: thread (q, i), 192 of them
+  if (tid < 3 * NQP) {
+    const int q = tid / 3, i = tid - 3 * q;
+    const double *s = S.sig[q];
+    double si[3];  // row i of c sigma / c P
+    if constexpr (FORM == FGPU_LINEAR) {
+      const int r0[3] = {0, 3, 4}, r1[3] = {3, 1, 5}, r2[3] = {4, 5, 2};
+      si[0] = s[r0[i]];
+      si[1] = s[r1[i]];
+      si[2] = s[r2[i]];
+    } else {
+      si[0] = s[3 * i];
+      si[1] = s[3 * i + 1];
+      si[2] = s[3 * i + 2];
     }
-    for (int m = 1; m < 4; m <<= 1) {
-      g0 += __shfl_xor(g0, m);
-      g1 += __shfl_xor(g1, m);
-      g2 += __shfl_xor(g2, m);
+    const double is = S.rs[q][0];
+    double t[3];
+    for (int k = 0; k < 3; k++)
+      t[k] = si[0] * S.Ji[q][k] + si[1] * S.Ji[q][3 + k] + si[2] * S.Ji[q][6 + k];
+    for (int k = 0; k < 3; k++) S.Z[i][k][q] = t[k] * is;
+    S.Z[i][3][q] = -(t[0] * S.rs[q][1] + t[1] * S.rs[q][2] + t[2] * S.rs[q][3]) * is * is;
+  }
+  __syncthreads();
+  // sums over q_t: thread (i, q_r, q_s, k_t)
+  constexpr int NY = NQ1 * NQ1 * NB2;
+  if (tid < 3 * NY) {
+    const int i = tid / NY, r = tid - NY * i;
+    const int kt = r / (NQ1 * NQ1), qrs = r - kt * NQ1 * NQ1;
+    double y0 = 0.0, y1 = 0.0, y23 = 0.0;
+    for (int qt = 0; qt < NQ1; qt++) {
+      const int q = qrs + NQ1 * NQ1 * qt;
+      const double nt = S.g.N[2][qt][kt], dnt = S.g.dN[2][qt][kt];
+      y0 += S.Z[i][0][q] * nt;
+      y1 += S.Z[i][1][q] * nt;
+      y23 += S.Z[i][2][q] * dnt + S.Z[i][3][q] * nt;
     }
-    if (quarter == 0) {
-      double *o = ge + ((size_t)e * NBAS + a) * 3;
-      o[0] = g0;
-      o[1] = g1;
-      o[2] = g2;
+    const int o = qrs + NQ1 * NQ1 * kt;
+    S.Y[i][0][o] = y0;
+    S.Y[i][1][o] = y1;
+    S.Y[i][2][o] = y23;
+  }
+  __syncthreads();
+  // sums over q_s: thread (i, q_r, j_s, k_t)
+  constexpr int NV = NQ1 * NB * NB2;
+  if (tid < 3 * NV) {
+    const int i = tid / NV, r = tid - NV * i;
+    const int kt = r / (NQ1 * NB), rj = r - kt * NQ1 * NB;
+    const int js = rj / NQ1, qr = rj - NQ1 * js;
+    double v0 = 0.0, v1 = 0.0;
+    for (int qs = 0; qs < NQ1; qs++) {
+      const int o = qr + NQ1 * qs + NQ1 * NQ1 * kt;
+      const double ns = S.g.N[1][qs][js], dns = S.g.dN[1][qs][js];
+      v0 += S.Y[i][0][o] * ns;
+      v1 += S.Y[i][1][o] * dns + S.Y[i][2][o] * ns;
     }
+    const int o = qr + NQ1 * js + NQ1 * NB * kt;
+    S.V[i][0][o] = v0;
+    S.V[i][1][o] = v1;
+  }
+  __syncthreads();
+  // sums over q_r: thread (a, i), consecutive threads writing consecutive entries
+  if (tid < 3 * NBAS) {
+    const int a = tid / 3, i = tid - 3 * a;
+    const int ir = a % 4, js = (a / 4) % 4, kt = a / 16;
+    double g = 0.0;
+    for (int qr = 0; qr < NQ1; qr++) {
+      const int o = qr + NQ1 * js + NQ1 * NB * kt;
+      g += S.V[i][0][o] * S.g.dN[0][qr][ir] + S.V[i][1][o] * S.g.N[0][qr][ir];
+    }
+    ge[((size_t)e * NBAS + a) * 3 + i] = S.g.W[a] * g;
   }
 }
-
 
 template <int FORM, int NB2>
 __global__ __launch_bounds__(256) void bsp_fint_kernel(
