@@ -335,7 +335,7 @@ void destroy(Ctx *c) {
                   c->d_contrib, c->d_nptr, c->d_nadj, c->d_kptr, c->d_rslot, c->d_u,
                   c->d_b, c->d_x,
                   c->d_r, c->d_p, c->d_w, c->d_z, c->d_dinv, c->d_part,
-                  c->d_scal, c->d_done, c->d_eqnode, c->d_TR, c->d_band, c->d_finfo,
+                  c->d_scal, c->d_done, c->d_eqnode, c->d_TR, c->d_band, c->d_finfo, c->d_bpart,
                   c->d_bval32, c->d_gsym, c->d_gG, c->d_gR, c->d_gH, c->d_gPQ,
                   c->d_gnodes};
   for (void *p : bufs)

@@ -252,6 +252,7 @@ struct Ctx {
   int band_bw = -1;            // half bandwidth (-1: not computed yet)
   double *d_band = nullptr;    // [neq * (bw + 1)]
   int *d_finfo = nullptr;      // [2] first pivot below the tolerance + 1, negative pivots
+  double *d_bpart = nullptr;   // partial sums of the wide backward solve [2][wg][32]
   bool factored = false;       // d_band holds the factors of the current matrix
   int neg_pivots = 0;
 

@@ -18,10 +18,16 @@
 // trailing band takes the rank-32 update A22 -= L21 D L21^T in 32 x 32 tiles
 // (one workgroup per tile, only tiles that meet the band).
 //
-// Solves: one workgroup sweeps the panels; the bw + NB live entries of the
-// right-hand side sit in an LDS ring, each panel's triangle is solved by one
-// wave through shuffles and the coupling to the bw rows below (forward) or
-// above (backward) is a 32-wide product over all threads.
+// Solves, narrow bands (bw < WIDE_BW): one workgroup sweeps the panels; the
+// bw + NB live entries of the right-hand side sit in an LDS ring, each
+// panel's triangle is solved by one wave through shuffles and the coupling
+// to the bw rows below (forward) or above (backward) is a 32-wide product
+// over all threads.  Wide bands: one launch per panel over as many
+// workgroups as the band has rows of 256 below the panel; every workgroup
+// solves the panel's triangle itself (the same operations, so the same
+// values; workgroup 0 stores them) and then its own 256 rows: forward, the
+// update y_i -= sum_j L_ij y_j; backward, its partial sums of L_ij x_i for
+// the next panel up, which the next launch adds in workgroup order.
 #include "fgpu_dev.h"
 
 namespace fgpu {
@@ -29,6 +35,8 @@ namespace {
 
 constexpr int NB = 32;
 constexpr int MAX_BW_SOLVE = 16384;  // LDS ring of the solve: (bw + NB) doubles
+constexpr int WIDE_BW = 256;         // from this half bandwidth on, the solves go wide
+constexpr int WROWS = 256;           // rows per workgroup of the wide solves
 
 struct Band {
   double *a;
@@ -37,6 +45,21 @@ struct Band {
   __device__ double &at(int i, int j) const { return a[(long long)j * ld + (i - j)]; }
   __device__ bool in(int i, int j) const { return i >= j && i - j <= bw && i < n; }
 };
+
+// A panel's unit triangle, loaded up front so that the loads are in flight
+// together: lane t < kb gets row k0 + t, Lr[j] = L(k0 + t, k0 + j) for j < t
+// (0 elsewhere), or column k0 + t, Lc[j] = L(k0 + j, k0 + t) for t < j < kb.
+__device__ inline void panel_row(const Band &B, int k0, int kb, int t, double *Lr) {
+#pragma unroll
+  for (int j = 0; j < NB; j++)
+    Lr[j] = (t < kb && j < t && t - j <= B.bw) ? B.at(k0 + t, k0 + j) : 0.0;
+}
+
+__device__ inline void panel_col(const Band &B, int k0, int kb, int t, double *Lc) {
+#pragma unroll
+  for (int j = 0; j < NB; j++)
+    Lc[j] = (t < kb && j > t && j < kb && j - t <= B.bw) ? B.at(k0 + j, k0 + t) : 0.0;
+}
 
 // lower triangle of the node-block matrix -> band (fixed directions dropped)
 __global__ __launch_bounds__(256) void band_fill_kernel(int nnode, const long long *__restrict__ bptr,
@@ -145,10 +168,13 @@ __global__ __launch_bounds__(1024) void band_forward_kernel(Band B, const double
     const int kb = min(NB, n - k0);
     __syncthreads();
     if (tid < 64) {  // the panel's triangle: one wave
+      double Lr[NB];
+      panel_row(B, k0, kb, tid, Lr);
       double v = tid < kb ? ring[(k0 + tid) % W] : 0.0;
-      for (int j = 0; j < kb; j++) {
+#pragma unroll
+      for (int j = 0; j < NB; j++) {
         const double vj = __shfl(v, j);
-        if (tid > j && tid < kb && tid - j <= B.bw) v -= B.at(k0 + tid, k0 + j) * vj;
+        v -= Lr[j] * vj;
       }
       if (tid < kb) ring[(k0 + tid) % W] = v;
     }
@@ -200,9 +226,12 @@ __global__ __launch_bounds__(1024) void band_backward_kernel(Band B, const doubl
         for (int ch = 0; ch < NB; ch++) s += red[tid][ch];
         v = ring[(k0 + tid) % W] - s;
       }
-      for (int j = kb - 1; j > 0; j--) {
+      double Lc[NB];
+      panel_col(B, k0, kb, tid, Lc);
+#pragma unroll
+      for (int j = NB - 1; j > 0; j--) {
         const double xj = __shfl(v, j);
-        if (tid < j && j - tid <= B.bw) v -= B.at(k0 + j, k0 + tid) * xj;
+        v -= Lc[j] * xj;
       }
       if (tid < kb) ring[(k0 + tid) % W] = v;
     }
@@ -213,6 +242,98 @@ __global__ __launch_bounds__(1024) void band_backward_kernel(Band B, const doubl
       const int e = k0 - NB + tid;  // the next panel's entries
       if (e >= 0) ring[e % W] = z[e];
     }
+  }
+}
+
+// Wide forward step for the panel at k0: every workgroup solves the panel's
+// triangle from y (final for these rows), workgroup 0 stores z = y / d for
+// them, and each workgroup updates its WROWS rows below the panel.  y of the
+// panel itself is left as it was (other workgroups read it).
+__global__ __launch_bounds__(WROWS) void band_fwd_wide_kernel(Band B, int k0,
+                                                              double *__restrict__ y,
+                                                              double *__restrict__ z) {
+  __shared__ double yp[NB];
+  const int n = B.n, tid = threadIdx.x;
+  const int kb = min(NB, n - k0);
+  if (tid < 64) {
+    double Lr[NB];
+    panel_row(B, k0, kb, tid, Lr);
+    double v = tid < kb ? y[k0 + tid] : 0.0;
+#pragma unroll
+    for (int j = 0; j < NB; j++) {
+      const double vj = __shfl(v, j);
+      v -= Lr[j] * vj;
+    }
+    if (tid < kb) yp[tid] = v;
+  }
+  __syncthreads();
+  if (blockIdx.x == 0 && tid < kb) z[k0 + tid] = yp[tid] / B.at(k0 + tid, k0 + tid);
+  const int m = min(B.bw, n - k0 - kb);
+  const int t = blockIdx.x * WROWS + tid;
+  if (t < m) {
+    const int i = k0 + kb + t;
+    double s = 0.0;
+    for (int j = 0; j < kb; j++)
+      if (i - (k0 + j) <= B.bw) s += B.at(i, k0 + j) * yp[j];
+    y[i] -= s;
+  }
+}
+
+// Wide backward step for the panel at k0 (panels from the last one up):
+// every workgroup adds the nin partial sums of the panel (written by the
+// previous step, in workgroup order), solves its triangle (L^T, unit
+// diagonal) from z, workgroup 0 stores x for it, and each workgroup writes
+// its partial sums of L(i, k0 - NB + j) x_i over its WROWS rows i >= k0 for
+// the panel above (pout[blockIdx.x][j]).
+__global__ __launch_bounds__(WROWS) void band_bwd_wide_kernel(Band B, int k0,
+                                                              const double *__restrict__ z,
+                                                              double *__restrict__ x,
+                                                              const double *__restrict__ pin,
+                                                              int nin, double *__restrict__ pout) {
+  __shared__ double xp[NB];
+  __shared__ double red[NB][9];
+  const int n = B.n, tid = threadIdx.x;
+  const int kb = min(NB, n - k0);
+  if (tid < 64) {
+    double v = 0.0;
+    if (tid < kb) {
+      double s = 0.0;
+      for (int w = 0; w < nin; w++) s += pin[w * NB + tid];
+      v = z[k0 + tid] - s;
+    }
+    double Lc[NB];
+    panel_col(B, k0, kb, tid, Lc);
+#pragma unroll
+    for (int j = NB - 1; j > 0; j--) {
+      const double xj = __shfl(v, j);
+      v -= Lc[j] * xj;
+    }
+    if (tid < kb) xp[tid] = v;
+  }
+  __syncthreads();
+  if (blockIdx.x == 0 && tid < kb) x[k0 + tid] = xp[tid];
+  if (k0 == 0) return;  // no panel above (uniform over the grid)
+  // partial sums for the panel above (columns k0 - NB + j, a full panel):
+  // thread (j, g) takes the workgroup's rows g, g + 8, ...
+  const int ka = k0 - NB;
+  const int m = min(B.bw, n - k0);  // rows below that panel
+  const int j = tid / 8, g = tid % 8;
+  const int r0 = blockIdx.x * WROWS;
+  double s = 0.0;
+  for (int r = g; r < WROWS; r += 8) {
+    const int t = r0 + r;
+    if (t >= m) break;
+    const int i = k0 + t;
+    if (i - (ka + j) > B.bw) continue;
+    const double xi = i < k0 + kb ? xp[i - k0] : x[i];
+    s += B.at(i, ka + j) * xi;
+  }
+  red[j][g] = s;
+  __syncthreads();
+  if (tid < NB) {
+    double t = 0.0;
+    for (int q = 0; q < 8; q++) t += red[tid][q];
+    pout[blockIdx.x * NB + tid] = t;
   }
 }
 
@@ -262,6 +383,8 @@ int band_setup(Ctx &c) {
     const size_t entries = (size_t)n * (bw + 1);
     FGPU_CHECK(hipMalloc(&c.d_band, entries * sizeof(double)));
     FGPU_CHECK(hipMalloc(&c.d_finfo, 2 * sizeof(int)));
+    if (bw >= WIDE_BW)
+      FGPU_CHECK(hipMalloc(&c.d_bpart, sizeof(double) * 2 * NB * ((bw + WROWS - 1) / WROWS)));
   }
   return 1;
 }
@@ -337,6 +460,31 @@ void enqueue_band_solve(Ctx &c, const double *b, double *x, double *z) {
   const int n = c.neq, bw = c.band_bw;
   if (n == 0) return;
   Band B{c.d_band, n, bw, (long long)bw + 1};
+  if (bw >= WIDE_BW) {
+    // forward in place on x (its entries are rewritten by the backward pass
+    // from the bottom up), z = D^-1 y; then backward from z into x
+    if (!c.d_bpart) throw Error("direct solve: wide-solve buffers not allocated");
+    launch_copy(c, b, x, n);
+    for (int k0 = 0; k0 < n; k0 += NB) {
+      const int m = std::min(bw, n - k0 - std::min(NB, n - k0));
+      const int grid = std::max(1, (m + WROWS - 1) / WROWS);
+      hipLaunchKernelGGL(band_fwd_wide_kernel, dim3(grid), dim3(WROWS), 0, c.stream, B, k0, x, z);
+    }
+    const int klast = (n - 1) / NB * NB;
+    const int half = NB * ((bw + WROWS - 1) / WROWS);
+    int nin = 0, par = 0;
+    for (int k0 = klast; k0 >= 0; k0 -= NB) {
+      const int m = k0 > 0 ? std::min(bw, n - k0) : 0;
+      const int grid = std::max(1, (m + WROWS - 1) / WROWS);
+      double *pin = c.d_bpart + (size_t)par * half, *pout = c.d_bpart + (size_t)(1 - par) * half;
+      hipLaunchKernelGGL(band_bwd_wide_kernel, dim3(grid), dim3(WROWS), 0, c.stream, B, k0, z, x,
+                         pin, nin, pout);
+      nin = grid;
+      par = 1 - par;
+    }
+    FGPU_CHECK(hipGetLastError());
+    return;
+  }
   const size_t lds = sizeof(double) * (size_t)(bw + NB);
   hipLaunchKernelGGL(band_forward_kernel, dim3(1), dim3(1024), lds, c.stream, B, b, z);
   hipLaunchKernelGGL(band_backward_kernel, dim3(1), dim3(1024), lds, c.stream, B, z, x);
