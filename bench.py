@@ -11,7 +11,7 @@ elements, 33.6 M equations), 2x2x2 Gauss, one launch per phase for the whole
 batch.  Beside the headline the same JSON line carries, each measured in this
 run: the cubic B-spline 32x32x8 patch step, the 64^3 neo-Hookean hex8 step
 and its 20-increment Newton solve, the learned warm start's solve-time ratios
-(plain / secant / network), and the relative errors of K.u, f_int and u
+(plain / secant / polynomial extrapolation of orders 2 and 4 / network), and the relative errors of K.u, f_int and u
 against the FP64 oracle for every configuration.
 
 ``--workload iga|iga_svk|hex8|batch`` runs one step workload alone (iga_svk:
@@ -494,7 +494,8 @@ def warmstart_record(nstates=2000, train_steps=4000, beta=30.0):
     training data (load levels U(0, 0.3) of a unit traction, seed 0, solved
     by continuation), device training of the graph network, then one load
     path (10 equal steps to 0.3) solved to |r| / max(1, |f|) <= 1e-8 plain,
-    with the secant predictor and with the network (second pass timed)."""
+    with the secant predictor, with the polynomial extrapolation through the
+    last 3 and 5 converged states, and with the network (second pass timed)."""
     import torch
     from fast_amd import mesh as M, fgpu
     n = 64
@@ -535,6 +536,7 @@ def warmstart_record(nstates=2000, train_steps=4000, beta=30.0):
     res = {}
     for rep in range(2):  # the first pass warms up; the second is timed
         for name, extra in (("plain", {}), ("secant", {"secant": True}),
+                            ("extrap2", {"extrap": 2}), ("extrap4", {"extrap": 4}),
                             ("network", {"predict": net})):
             torch.cuda.synchronize()
             t0 = time.perf_counter()
@@ -554,6 +556,8 @@ def warmstart_record(nstates=2000, train_steps=4000, beta=30.0):
         "network_over_plain": round(res["network"]["wall_s"] / res["plain"]["wall_s"], 4),
         "secant_over_plain": round(res["secant"]["wall_s"] / res["plain"]["wall_s"], 4),
         "network_over_secant": round(res["network"]["wall_s"] / res["secant"]["wall_s"], 4),
+        "extrap2_over_plain": round(res["extrap2"]["wall_s"] / res["plain"]["wall_s"], 4),
+        "extrap4_over_plain": round(res["extrap4"]["wall_s"] / res["plain"]["wall_s"], 4),
         "solves": res, "neq": int(m.neq), "train_states": nstates, "train_steps": train_steps,
         "loss_beta": beta, "datagen_s": round(t_data, 2), "train_s": round(t_train, 2),
         "train_ms_per_step": round(1e3 * t_train / train_steps, 3),
@@ -579,7 +583,7 @@ def run_warmstart(args):
 
 
 # ---------------------------------------------------------------- Newton
-def newton_record(variants=("plain_mat32", "secant_mat32")):
+def newton_record(variants=("plain_mat32", "secant_mat32", "extrap4_mat32")):
     """The 64^3 neo-Hookean cantilever under load control: 20 increments to
     the tip traction 0.2, Newton to |r| / max(1, |f|) <= 1e-8 with inexact
     PCG corrections (FP32 element tangents, FP64 f_int and PCG vectors)."""
@@ -597,6 +601,8 @@ def newton_record(variants=("plain_mat32", "secant_mat32")):
     res = {}
     for name in variants:
         extra = {"secant": True} if name.startswith("secant") else {}
+        if name.startswith("extrap"):
+            extra = {"extrap": int(name[6])}
         model.set_matrix_precision(32 if name.endswith("mat32") else 64)
         torch.cuda.synchronize()
         t0 = time.perf_counter()
@@ -616,7 +622,8 @@ def newton_record(variants=("plain_mat32", "secant_mat32")):
 
 
 def run_newton(args):
-    rec = newton_record(("plain", "secant", "plain_mat32", "secant_mat32"))
+    rec = newton_record(("plain", "secant", "extrap4", "plain_mat32", "secant_mat32",
+                         "extrap4_mat32"))
     res = rec["runs"]
     out = {
         "metric": "load-control Newton wall time, 20 increments to traction 0.2",

@@ -319,6 +319,26 @@ int fgpu_newton(fgpu_ctx *ctx, const double *q, const fgpu_nr_params *prm, doubl
     const bool verbose = venv && venv[0] == '1';
     double lam = prm->u0_dev ? prm->lam0 : 0.0, lam_last_step = lam;
     bool have_last = false;
+    // converged states for the extrapolation predictor: a ring of p + 1
+    // vectors, hl[] their load factors oldest first, hs[] their slots
+    const int xp = prm->predict ? 0 : std::max(0, std::min(prm->extrap, 4));
+    DevVec hist(xp ? (size_t)(xp + 1) * n : 1);
+    std::vector<double> hl;
+    std::vector<int> hs;
+    auto keep_state = [&](double l) {
+      if (!xp) return;
+      int slot = (int)hs.size();
+      if (slot == xp + 1) {  // full: reuse the oldest slot
+        slot = hs.front();
+        hs.erase(hs.begin());
+        hl.erase(hl.begin());
+      }
+      FGPU_CHECK(hipMemcpyAsync(hist.p + (size_t)slot * n, du.p, nb, hipMemcpyDeviceToDevice,
+                                c.stream));
+      hs.push_back(slot);
+      hl.push_back(l);
+    };
+    keep_state(lam);
     int done_steps = 0;
     for (int step = 1; step <= prm->steps; step++) {
       const double t_step = now_ms();
@@ -337,6 +357,26 @@ int fgpu_newton(fgpu_ctx *ctx, const double *q, const fgpu_nr_params *prm, doubl
       if (prm->predict) {
         predicted = prm->predict(prm->predict_user, step, du.p, lam_prev, lam,
                                  have_last ? dlast.p : nullptr, lam_last_step, dpred.p) != 0;
+      } else if (xp && hs.size() >= 2) {
+        // Lagrange weights of the kept states at lam (they sum to one)
+        const int m = (int)hs.size();
+        std::vector<double> w(m, 1.0);
+        bool distinct = true;
+        for (int j = 0; j < m; j++)
+          for (int i = 0; i < m; i++)
+            if (i != j) {
+              if (hl[j] == hl[i])
+                distinct = false;
+              else
+                w[j] *= (lam - hl[i]) / (hl[j] - hl[i]);
+            }
+        if (distinct) {
+          // d = sum_j w_j U_j - u; the newest kept state is u itself
+          launch_axpby(c, n, w[m - 1] - 1.0, du.p, w[0], hist.p + (size_t)hs[0] * n, dpred.p);
+          for (int j = 1; j < m - 1; j++)
+            launch_axpby(c, n, 1.0, dpred.p, w[j], hist.p + (size_t)hs[j] * n, dpred.p);
+          predicted = true;
+        }
       } else if (prm->secant && have_last && lam_prev != lam_last_step) {
         launch_axpby(c, n, (lam - lam_prev) / (lam_prev - lam_last_step), dinc.p, 0.0, nullptr,
                      dpred.p);
@@ -522,6 +562,7 @@ int fgpu_newton(fgpu_ctx *ctx, const double *q, const fgpu_nr_params *prm, doubl
       st.converged = conv;
       // converged increment of this step (for the secant predictor)
       launch_axpby(c, n, 1.0, du.p, -1.0, dlast.p, dinc.p);
+      if (conv) keep_state(lam);
       have_last = true;
       lam_last_step = lam_prev;
       if (prm->u_steps_dev)

@@ -51,7 +51,7 @@ class NrParams(C.Structure):
                 ("adaptive_pcg", C.c_int), ("predict", PREDICT_FN), ("predict_user", C.c_void_p),
                 ("lambdas", C.c_void_p), ("u0_dev", C.c_void_p), ("lam0", C.c_double),
                 ("u_steps_dev", C.c_void_p), ("secant", C.c_int), ("galerkin", C.c_int),
-                ("direct", C.c_int), ("graph", C.c_int)]
+                ("direct", C.c_int), ("graph", C.c_int), ("extrap", C.c_int)]
 
 
 class NrStat(C.Structure):
@@ -288,7 +288,7 @@ class GpuModel:
     def newton(self, q, steps, factor=0.0, maxit=20, tol=1e-5, pcg_tol=1e-5, pcg_maxit=0,
                adaptive_pcg=0, predict=None, lambdas=None, u0_dev=None, lam0=0.0,
                u_steps_dev=None, secant=False, galerkin=False, direct=False, keep_host=True,
-               graph=True):
+               graph=True, extrap=0):
         """cCtrlNR::Solver on the device (ctrl/ctrlnr.cpp:47-267).
 
         ``predict`` is a Python callable ``(step, u_dev_ptr, lam_prev, lam,
@@ -299,7 +299,8 @@ class GpuModel:
         converged displacement of every step; ``direct`` solves each
         correction with the device LDL^T instead of PCG; ``graph`` replays
         each Newton iteration from captured hipGraphs (bit-identical to the
-        eager launches)."""
+        eager launches); ``extrap`` = p starts each step from the polynomial
+        in lambda through the last p + 1 converged states (p <= 4)."""
         q = np.ascontiguousarray(q, dtype=np.float64)
         prm = NrParams()
         prm.steps, prm.factor, prm.maxit, prm.tol = steps, factor, maxit, tol
@@ -308,6 +309,7 @@ class GpuModel:
         prm.galerkin = int(galerkin)
         prm.direct = int(direct)
         prm.graph = int(graph)
+        prm.extrap = int(extrap)
         cb = None
         if isinstance(predict, Gnn):
             prm.predict = C.cast(lib().fgpu_gnn_newton_predictor, PREDICT_FN)

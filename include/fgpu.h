@@ -262,6 +262,12 @@ typedef struct {
                                residual around the PCG chunk graphs), one
                                host read per iteration; bit-identical to 0
                                (eager launches) */
+  int extrap;               /* p = 1..4, without a predictor callback: start
+                               each step from the polynomial in lambda through
+                               the last p + 1 converged states, extrapolated
+                               to the step's lambda (the start state is the
+                               first of them; fewer while fewer exist, none
+                               with one); 0: off (then secant applies) */
 } fgpu_nr_params;
 
 typedef struct {

@@ -190,6 +190,11 @@ def test_newton_driver_options(fg):
                               **base)
     assert eD == "" and nrel(uD[-1], uA[-1]) <= 1e-8
     assert abs(sD[-1]["lam"] - 0.2) < 1e-15
+    for p in (1, 2, 3, 4):  # polynomial extrapolation through the kept states
+        uX, sX, eX = model.newton(q, steps=4, factor=0.05, extrap=p, maxit=30, tol=1e-10,
+                                  pcg_tol=1.0, adaptive_pcg=2)
+        assert eX == "" and nrel(uX[-1], uA[-1]) <= 1e-7, (p, eX)
+        assert [s["predicted"] for s in sX] == [0, 1, 1, 1]
     ones = torch.ones(m.neq, dtype=torch.float64, device="cuda")
     torch.cuda.synchronize()
     L = fg.lib()
@@ -200,3 +205,25 @@ def test_newton_driver_options(fg):
     uE, sE, eE = model.newton(q, steps=4, factor=0.05, predict=bad, **base)
     assert eE == "" and nrel(uE[-1], uA[-1]) <= 1e-8
     assert all(s["predicted"] == -1 for s in sE)
+
+
+def test_extrapolation_predictors_save_work(fg):
+    """The load path of the warm-start benchmark on a 16^3 block: starting
+    each step from the polynomial through the last converged states reaches
+    the plain run's states with fewer Newton and PCG iterations; the order-4
+    polynomial beats the secant, which beats the plain start."""
+    m = cube(16)
+    model = fg.GpuModel(m)
+    q = m.load_vector()
+    kw = dict(steps=10, factor=0.03, maxit=30, tol=1e-8, pcg_tol=1.0, adaptive_pcg=2)
+    runs = {}
+    for name, extra in (("plain", {}), ("secant", {"secant": True}), ("extrap2", {"extrap": 2}),
+                        ("extrap4", {"extrap": 4})):
+        u, st, err = model.newton(q, **kw, **extra)
+        assert err == "", (name, err)
+        runs[name] = (u[-1], sum(s["iters"] for s in st), sum(s["pcg_iters"] for s in st))
+        print(name, runs[name][1:])
+    for name in ("secant", "extrap2", "extrap4"):
+        assert nrel(runs[name][0], runs["plain"][0]) <= 1e-6
+    assert runs["extrap4"][2] < runs["secant"][2] < runs["plain"][2]
+    assert runs["extrap4"][1] <= runs["secant"][1] <= runs["plain"][1]
