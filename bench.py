@@ -9,7 +9,8 @@ independent hex8 elastic unit cubes, edge n ~ U{4..32}, nu ~ U(0.2, 0.45),
 E = 1, one clamped face and pressure 1 on another, all from seed 0 (10.33 M
 elements, 33.6 M equations), 2x2x2 Gauss, one launch per phase for the whole
 batch.  Beside the headline the same JSON line carries, each measured in this
-run: the cubic B-spline 32x32x8 patch step, the 64^3 neo-Hookean hex8 step
+run: the 10^3 hex8 linear-static solve (configuration 1), the cubic B-spline
+32x32x8 patch step, the 64^3 neo-Hookean hex8 step
 and its 20-increment Newton solve, the learned warm start's solve-time ratios
 (plain / secant / polynomial extrapolation of orders 2 and 4 / network), and the relative errors of K.u, f_int and u
 against the FP64 oracle for every configuration.
@@ -534,21 +535,28 @@ def warmstart_record(nstates=2000, train_steps=4000, beta=30.0):
     torch.cuda.empty_cache()
     kw = dict(steps=10, factor=0.03, maxit=30, tol=1e-8, pcg_tol=1.0, adaptive_pcg=2)
     res = {}
+    last = {}
+    ust = torch.zeros((10, m.neq), dtype=torch.float64, device="cuda")
     for rep in range(2):  # the first pass warms up; the second is timed
         for name, extra in (("plain", {}), ("secant", {"secant": True}),
                             ("extrap2", {"extrap": 2}), ("extrap4", {"extrap": 4}),
                             ("network", {"predict": net})):
             torch.cuda.synchronize()
             t0 = time.perf_counter()
-            _, s, e = model.newton(q, keep_host=False, **kw, **extra)
+            _, s, e = model.newton(q, keep_host=False, u_steps_dev=ust.data_ptr(), **kw, **extra)
             wall = time.perf_counter() - t0
             if e:
                 raise RuntimeError("%s: %s" % (name, e))
             res[name] = {"wall_s": round(wall, 4), "newton_iters": sum(x["iters"] for x in s),
                          "pcg_iters": sum(x["pcg_iters"] for x in s),
                          "predictions_used": sum(x["predicted"] == 1 for x in s)}
+            last[name] = ust[-1].clone()
             if rep:
                 log("%s %s" % (name, res[name]))
+    for name in res:  # every start converges to the same state (Newton tolerance 1e-8)
+        res[name]["u_rel_vs_plain"] = float("%.3e" % float(
+            torch.linalg.norm(last[name] - last["plain"]) / torch.linalg.norm(last["plain"])))
+    del ust, last
     k = max(1, len(hist) // 10)
     net.close()
     model.close()
@@ -580,6 +588,40 @@ def run_warmstart(args):
         "warm_start": ws,
     }
     print(json.dumps(out))
+
+
+# ---------------------------------------------------------------- config 1
+def config1_record():
+    """Configuration 1: unit cube of 10^3 hex8 (E = 200e3, nu = 0.3), face x = 0
+    clamped, pressure 1.0 on face x = 1.  The device's linear-static solves
+    (Jacobi-PCG with FP64 residual refinement to 1e-10, and the band LDL^T with
+    refinement) against the oracle's FP64 skyline Crout solve of the same
+    system, timed on the host too."""
+    from fast_amd import fgpu, mesh as M
+    from oracle import oracle as O
+    m = M.hex8_cube(10, E=200e3, nu=0.3, clamp_face=0, load_face=1, pressure=1.0)
+    f = m.load_vector()
+    g = fgpu.GpuModel(m)
+    g.linear_static(f, tol=1e-10)  # first calls: tuning, band set-up
+    g.linear_static_direct(f)
+    t0 = time.perf_counter()
+    up, it, rr = g.linear_static(f, tol=1e-10)
+    tp = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    ud, k, rrd = g.linear_static_direct(f)
+    td = time.perf_counter() - t0
+    g.close()
+    t0 = time.perf_counter()
+    u0 = O.linear_static(O.OracleMesh(m), f, solver="skyline")
+    to = time.perf_counter() - t0
+    n0 = float(np.linalg.norm(u0))
+    return {"neq": int(m.neq), "elements": int(m.nelem),
+            "pcg": {"wall_s": round(tp, 4), "pcg_iters": int(it), "relres": float("%.3e" % rr),
+                    "u_rel_vs_fp64": float("%.3e" % (np.linalg.norm(up - u0) / n0))},
+            "direct": {"wall_s": round(td, 4), "refinements": int(k),
+                       "relres": float("%.3e" % rrd),
+                       "u_rel_vs_fp64": float("%.3e" % (np.linalg.norm(ud - u0) / n0))},
+            "oracle_skyline_s": round(to, 4)}
 
 
 # ---------------------------------------------------------------- Newton
@@ -733,6 +775,7 @@ def main():
                 "phase_ms": r["phase_ms"], "step_order": r["step_order"],
                 "roofline": r["roofline"], "f_int_roofline": r["f_int_roofline"],
                 "accuracy": r["accuracy"]}
+        extras["linear_static_hex8_10x10x10"] = config1_record()
         nr = newton_record()
         nr["u_rel_vs_fp64_newton_10cubed"] = accuracy_u_newton()
         extras["newton_hex8_neohooke_64x64x64"] = nr
