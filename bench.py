@@ -106,6 +106,11 @@ def bsp_executed_flops(m):
     return 10 * BSP_CHAINS[m.form] * 16 * 16 * 64 * 2
 
 
+def picked_mode(report):
+    mt = re.search(r"picked assembly mode (\d+)", report or "")
+    return int(mt.group(1)) if mt else -1
+
+
 def assembly_bytes(m, nbf, nblk):
     """Algorithmic HBM bytes of one assembly: every stored element block read
     once (9 FP32 + its 4-byte contribution code), per upper node block its
@@ -115,13 +120,18 @@ def assembly_bytes(m, nbf, nblk):
     return m.nelem * nsym * (36 + 4) + nblk * (8 + 16) + nbf * 72
 
 
-def measured_traffic(workload):
+def measured_traffic(workload, kernel):
     """HBM bytes per assembly launch from the committed rocprofv3 counter
-    summary of this workload (FETCH_SIZE + WRITE_SIZE, calibrated)."""
+    summary of this workload (FETCH_SIZE + WRITE_SIZE, calibrated), when that
+    summary measured the same assembly kernel."""
     path = os.path.join(ROOT, "profiles", "assembly_counters_%s.md" % workload)
     if not os.path.exists(path):
         return None
-    mt = re.search(r"traffic_bytes_per_launch:\s*([0-9.eE+]+)", open(path).read())
+    text = open(path).read()
+    mk = re.search(r"Dominant kernel of the step: (?:void )?(\w+)", text)
+    if not mk or mk.group(1) != kernel:
+        return None
+    mt = re.search(r"traffic_bytes_per_launch:\s*([0-9.eE+]+)", text)
     return float(mt.group(1)) if mt else None
 
 
@@ -147,10 +157,10 @@ def element_roofline(m, tk):
 def assembly_roofline(m, ta, cfg, counts, report=""):
     b = assembly_bytes(m, *counts)
     achieved = b / (ta * 1e-3) / 1e9
-    mt = re.search(r"picked assembly mode (\d+)", report)
-    kernel = "assemble_rows_kernel" if mt and int(mt.group(1)) == 8 else "assemble_kernel"
+    kernel = "assemble_rows_kernel" if picked_mode(report) == 8 else "assemble_kernel"
     return {"bound": "hbm", "achieved": round(achieved, 1), "peak": PEAK_HBM, "unit": "GB/s",
-            "frac": round(achieved / PEAK_HBM, 4), "traffic": measured_traffic(cfg["workload"]),
+            "frac": round(achieved / PEAK_HBM, 4),
+            "traffic": measured_traffic(cfg["workload"], kernel),
             "kernel": kernel, "algorithmic_bytes": int(b), "kernel_ms": round(float(ta), 4)}
 
 

@@ -465,7 +465,7 @@ void autotune(Ctx &c, const double *d_u) {
     }
     return best;
   };
-  char buf[96];
+  char buf[192];
   if (c.shape == FGPU_HEX8) {
     float best = 1e30f;
     int pick = 0;

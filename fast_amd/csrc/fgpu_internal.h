@@ -209,7 +209,7 @@ struct Ctx {
   int hex8_te = 0;       // elements per workgroup of the hex8 K kernel (0: largest)
   int asm_mode = -1;     // assembly: contribution-list loads in batches of 1, 4, 8,
                          // 16 (mode & 3), XCD-contiguous workgroups (mode & 4);
-                         // -1 default (2)
+                         // 8 one wave per node row; -1 default (2)
   bool tuned = false;
   int overlap = -1;      // f_int beside the assembly (1) or after K (0); -1 untimed
   std::string tune_report;

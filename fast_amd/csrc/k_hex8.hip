@@ -61,6 +61,37 @@ __device__ inline T inv3(const T J[3][3], T I[3][3]) {
   return det;
 }
 
+// Gradients g_a = J^-T dN_a of the 8 shape functions at (r, s, t) in FP32,
+// from the node coordinates X relative to node 0; returns det J
+__device__ inline float hex8_point_grad(const float *X, float r, float s, float t,
+                                        float g[8][3]) {
+  float dN[8][3];
+  float J[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
+  for (int a = 0; a < 8; a++) {
+    hex8_dN(r, s, t, a, dN[a][0], dN[a][1], dN[a][2]);
+    for (int i = 0; i < 3; i++)
+      for (int k = 0; k < 3; k++) J[i][k] += dN[a][i] * X[3 * a + k];
+  }
+  float Ji[3][3];
+  const float det = inv3(J, Ji);
+  for (int a = 0; a < 8; a++)
+    for (int i = 0; i < 3; i++)
+      g[a][i] = Ji[i][0] * dN[a][0] + Ji[i][1] * dN[a][1] + Ji[i][2] * dN[a][2];
+  return det;
+}
+
+// Linear isotropic block from M = sum_q s_a s_b^T:
+// K = lambda M + mu M^T + mu tr(M) I, every product and sum spelt out (fmaf)
+// so that each kernel forming blocks rounds them alike
+__device__ inline void lin_block(const float M[9], float lam, float mu, float K[9]) {
+  const float trs = (M[0] + M[4]) + M[8];
+  for (int i = 0; i < 3; i++)
+    for (int j = 0; j < 3; j++) {
+      const float v = fmaf(lam, M[3 * i + j], mu * M[3 * j + i]);
+      K[3 * i + j] = i == j ? fmaf(mu, trs, v) : v;
+    }
+}
+
 // Per element-point vectors staged in LDS (floats per point):
 //   LINEAR:   sqrt(c) g (24), c = w detJ: M_ab = sum_q (sqrt(c) g_a)(sqrt(c) g_b)^T
 //             costs 9 FMA per point and block, and a diagonal block comes out
@@ -124,12 +155,10 @@ __device__ inline void hex8_lin_rows(const float *S, int TE, int NQ, float lam, 
   for (int k = 0; k < 9; k++) {
     const int a = k < 8 - a1 ? a1 : a2;
     const int b = k < 8 - a1 ? a1 + k : a2 + (k - (8 - a1));
-    const float *Mk = M[k];
-    const float tr = mu * (Mk[0] + Mk[4] + Mk[8]);
+    float K[9];
+    lin_block(M[k], lam, mu, K);
     float *out = Ke + L.offset(e, sym_index(a, b, 8));
-    for (int i = 0; i < 3; i++)
-      for (int j = 0; j < 3; j++)
-        out[3 * i + j] = lam * Mk[3 * i + j] + mu * Mk[3 * j + i] + (i == j ? tr : 0.f);
+    for (int m = 0; m < 9; m++) out[m] = K[m];
   }
 }
 
@@ -186,20 +215,9 @@ __global__ __launch_bounds__(256) void hex8_K_kernel(
     const int ir = q % NIP, is = (q / NIP) % NIP, it = q / (NIP * NIP);
     const float r = gp[(0 * 16 + ir) * 2], s = gp[(1 * 16 + is) * 2], t = gp[(2 * 16 + it) * 2];
     const float w = gp[(0 * 16 + ir) * 2 + 1] * gp[(1 * 16 + is) * 2 + 1] * gp[(2 * 16 + it) * 2 + 1];
-    float dN[8][3];
-    float J[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
-    for (int a = 0; a < 8; a++) {
-      hex8_dN(r, s, t, a, dN[a][0], dN[a][1], dN[a][2]);
-      for (int i = 0; i < 3; i++)
-        for (int k = 0; k < 3; k++) J[i][k] += dN[a][i] * X[3 * a + k];
-    }
-    float Ji[3][3];
-    float det = inv3(J, Ji);
-    count_jacobian(status, det);
     float g[8][3];
-    for (int a = 0; a < 8; a++)
-      for (int i = 0; i < 3; i++)
-        g[a][i] = Ji[i][0] * dN[a][0] + Ji[i][1] * dN[a][1] + Ji[i][2] * dN[a][2];
+    const float det = hex8_point_grad(X, r, s, t, g);
+    count_jacobian(status, det);
     const float c = w * fabsf(det);  // |det J|: a left-handed mesh is accepted whole
     if constexpr (FORM == FGPU_LINEAR) {
       const float sc = sqrtf(c);
