@@ -135,7 +135,7 @@ def measured_traffic(workload, kernel):
     return float(mt.group(1)) if mt else None
 
 
-def element_roofline(m, tk):
+def element_roofline(m, tk, report=""):
     fl = kernel_flops(m)
     if fl:
         achieved = fl * m.nelem / (tk * 1e-3) / 1e12
@@ -149,8 +149,10 @@ def element_roofline(m, tk):
     # hex8: element kernel bound by its FP32 element-matrix stores
     bytes_per = 36 * 9 * 4
     achieved = bytes_per * m.nelem / (tk * 1e-3) / 1e9
+    lane = re.search(r"picked hex8_K elements/workgroup 1\b", report or "")
     return {"bound": "hbm", "achieved": round(achieved, 1), "peak": PEAK_HBM, "unit": "GB/s",
-            "frac": round(achieved / PEAK_HBM, 4), "traffic": None, "kernel": "hex8_K_kernel",
+            "frac": round(achieved / PEAK_HBM, 4), "traffic": None,
+            "kernel": "hex8_K_lane_kernel" if lane else "hex8_K_kernel",
             "bytes_per_elem": bytes_per, "kernel_ms": round(float(tk), 4)}
 
 
@@ -453,7 +455,7 @@ def run_step(name, steps, warmup, serial=False, with_accuracy=True):
     log(report.strip().replace("\n", " | "))
     tk, ta, tf = times.mean(axis=0)
     counts = model.block_counts()
-    rooflines = {"element_K": element_roofline(m, tk),
+    rooflines = {"element_K": element_roofline(m, tk, report),
                  "assembly": assembly_roofline(m, ta, cfg, counts, report)}
     dom = "assembly" if ta >= tk else "element_K"
     roof = dict(rooflines[dom])

@@ -477,6 +477,9 @@ void autotune(Ctx &c, const double *d_u) {
       if (ms < best) best = ms, pick = te;
     }
     c.hex8_te = pick;
+    std::snprintf(buf, sizeof buf, "picked hex8_K elements/workgroup %d%s\n", pick,
+                  pick == 1 ? " (one element per lane)" : "");
+    rep += buf;
     launch_hex8_K(c, d_u);
   } else {
     launch_elem_K(c, d_u);

@@ -61,6 +61,37 @@ __device__ inline T inv3(const T J[3][3], T I[3][3]) {
   return det;
 }
 
+// FP32 point geometry of the stiffness kernels, every product and sum spelt
+// out (fmaf) so that each kernel using it rounds alike, whether its shape
+// derivatives are run-time values or compile-time constants:
+//   J += dN_a (x) X_a over the nodes; J^-1 by cofactors; g = J^-1 dN
+__device__ inline void hex8_jac_add(float J[3][3], const float d[3], const float *Xa) {
+  for (int i = 0; i < 3; i++)
+    for (int k = 0; k < 3; k++) J[i][k] = fmaf(d[i], Xa[k], J[i][k]);
+}
+
+__device__ inline float inv3_fma(const float J[3][3], float I[3][3]) {
+  const float c00 = fmaf(J[1][1], J[2][2], -(J[1][2] * J[2][1]));
+  const float c01 = fmaf(J[1][0], J[2][2], -(J[1][2] * J[2][0]));
+  const float c02 = fmaf(J[1][0], J[2][1], -(J[1][1] * J[2][0]));
+  const float det = fmaf(J[0][0], c00, fmaf(-J[0][1], c01, J[0][2] * c02));
+  const float id = 1.f / det;
+  I[0][0] = c00 * id;
+  I[0][1] = -fmaf(J[0][1], J[2][2], -(J[0][2] * J[2][1])) * id;
+  I[0][2] = fmaf(J[0][1], J[1][2], -(J[0][2] * J[1][1])) * id;
+  I[1][0] = -c01 * id;
+  I[1][1] = fmaf(J[0][0], J[2][2], -(J[0][2] * J[2][0])) * id;
+  I[1][2] = -fmaf(J[0][0], J[1][2], -(J[0][2] * J[1][0])) * id;
+  I[2][0] = c02 * id;
+  I[2][1] = -fmaf(J[0][0], J[2][1], -(J[0][1] * J[2][0])) * id;
+  I[2][2] = fmaf(J[0][0], J[1][1], -(J[0][1] * J[1][0])) * id;
+  return det;
+}
+
+__device__ inline float hex8_grad_i(const float Ji[3][3], int i, const float d[3]) {
+  return fmaf(Ji[i][2], d[2], fmaf(Ji[i][1], d[1], Ji[i][0] * d[0]));
+}
+
 // Gradients g_a = J^-T dN_a of the 8 shape functions at (r, s, t) in FP32,
 // from the node coordinates X relative to node 0; returns det J
 __device__ inline float hex8_point_grad(const float *X, float r, float s, float t,
@@ -69,14 +100,12 @@ __device__ inline float hex8_point_grad(const float *X, float r, float s, float 
   float J[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
   for (int a = 0; a < 8; a++) {
     hex8_dN(r, s, t, a, dN[a][0], dN[a][1], dN[a][2]);
-    for (int i = 0; i < 3; i++)
-      for (int k = 0; k < 3; k++) J[i][k] += dN[a][i] * X[3 * a + k];
+    hex8_jac_add(J, dN[a], X + 3 * a);
   }
   float Ji[3][3];
-  const float det = inv3(J, Ji);
+  const float det = inv3_fma(J, Ji);
   for (int a = 0; a < 8; a++)
-    for (int i = 0; i < 3; i++)
-      g[a][i] = Ji[i][0] * dN[a][0] + Ji[i][1] * dN[a][1] + Ji[i][2] * dN[a][2];
+    for (int i = 0; i < 3; i++) g[a][i] = hex8_grad_i(Ji, i, dN[a]);
   return det;
 }
 
@@ -373,6 +402,109 @@ __global__ __launch_bounds__(256) void hex8_K_kernel(
   }
 }
 
+// ---------------------------------------------------------------- one element per lane
+// Linear bricks on 2x2x2 Gauss: a lane computes one element, its 8 x 24
+// scaled gradients held in registers, then the 36 upper blocks one at a
+// time, M = sum_q s_a s_b^T over the points in order and lin_block --
+// hex8_lin_rows' arithmetic, so the same blocks bit for bit.  The wave's 64
+// elements are one KeLayout tile: each block of the tile is one run of 576
+// floats, which the wave stages in LDS and writes as 9 contiguous rows of 64
+// floats (written straight from the registers, lane by lane 36 bytes apart,
+// the same kernel took 4.08 ms on 10.3 M elements instead of 2.86 ms).  Two
+// waves per SIMD (register budget 256).
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2, 2))) void hex8_K_lane_kernel(
+    int nelem, const double *__restrict__ coords, const int *__restrict__ conn,
+    const double *__restrict__ mat, float *__restrict__ Ke, KeLayout L,
+    int *__restrict__ status) {
+  __shared__ float sK[2][64 * 9];  // a block of the wave's 64 elements, [lane][9]
+  const int lane = threadIdx.x;
+  const int e0 = blockIdx.x * 64;
+  const int e = min(e0 + lane, nelem - 1);  // lanes past the end repeat the last element
+  const bool live = e0 + lane < nelem;
+  float X[24];
+  {
+    const int *cn = conn + (size_t)e * 8;
+    const size_t n0 = cn[0];
+#pragma unroll
+    for (int a = 0; a < 8; a++) {
+      const size_t n = cn[a];
+      for (int k = 0; k < 3; k++) X[3 * a + k] = (float)(coords[3 * n + k] - coords[3 * n0 + k]);
+    }
+  }
+  float lam, mu;
+  lame(mat[2 * (size_t)e], mat[2 * (size_t)e + 1], lam, mu);
+  // per point J^-1 and sqrt(c) first (80 floats; X dies), then the scaled
+  // gradients point by point as each J^-1 dies -- the values and operations
+  // of hex8_point_grad, in an order that keeps fewer registers live
+  // the 2-point rule's points and unit weights as constants (the values the
+  // context's FP32 table holds: -/+0.577350269189626 rounded to FP32, 1), so
+  // the shape-function derivatives fold at compile time
+  constexpr float G2 = 0.577350269189626f;
+  float Ji[8][3][3], sq[8];
+#pragma unroll
+  for (int q = 0; q < 8; q++) {
+    const int ir = q % 2, is = (q / 2) % 2, it = q / 4;
+    const float r = ir ? G2 : -G2, s = is ? G2 : -G2, t = it ? G2 : -G2;
+    const float w = 1.f;
+    float J[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
+    for (int a = 0; a < 8; a++) {
+      float d[3];
+      hex8_dN(r, s, t, a, d[0], d[1], d[2]);
+      hex8_jac_add(J, d, X + 3 * a);
+    }
+    const float det = inv3_fma(J, Ji[q]);
+    if (live) count_jacobian(status, det);
+    sq[q] = sqrtf(w * fabsf(det));
+  }
+  float S[8][24];
+#pragma unroll
+  for (int q = 0; q < 8; q++) {
+    const int ir = q % 2, is = (q / 2) % 2, it = q / 4;
+    const float r = ir ? G2 : -G2, s = is ? G2 : -G2, t = it ? G2 : -G2;
+    for (int a = 0; a < 8; a++) {
+      float d[3];
+      hex8_dN(r, s, t, a, d[0], d[1], d[2]);
+      for (int i = 0; i < 3; i++) S[q][3 * a + i] = sq[q] * hex8_grad_i(Ji[q], i, d);
+    }
+  }
+  // upper blocks in packed row-major order: (0,0) (0,1) .. (0,7) (1,1) ..
+#pragma unroll
+  for (int a = 0; a < 8; a++)
+#pragma unroll
+    for (int b = a; b < 8; b++) {
+      const int blk = a * 8 - a * (a - 1) / 2 + (b - a);
+      float M[9];
+#pragma unroll
+      for (int m = 0; m < 9; m++) M[m] = 0.f;
+#pragma unroll
+      for (int q = 0; q < 8; q++)
+#pragma unroll
+        for (int i = 0; i < 3; i++)
+#pragma unroll
+          for (int j = 0; j < 3; j++)
+            M[3 * i + j] = fmaf(S[q][3 * a + i], S[q][3 * b + j], M[3 * i + j]);
+      float K[9];
+      lin_block(M, lam, mu, K);
+      // the wave's 64 blocks are one run of 576 floats in the tile: staged
+      // in LDS ([lane][9], stride 9: no bank conflicts) and stored as 9
+      // contiguous rows of 64 floats
+      float *sk = sK[blk & 1];
+#pragma unroll
+      for (int m = 0; m < 9; m++) sk[9 * lane + m] = K[m];
+      __syncthreads();
+      float *out = Ke + L.offset(e0, blk);
+      if (e0 + 64 <= nelem) {
+#pragma unroll
+        for (int k = 0; k < 9; k++) out[64 * k + lane] = sk[64 * k + lane];
+      } else {
+        for (int k = 0; k < 9; k++) {
+          const int v = 64 * k + lane;
+          if (e0 + v / 9 < nelem) out[v] = sk[v];
+        }
+      }
+    }
+}
+
 // ---------------------------------------------------------------- f_int
 // One thread per element, FP64 throughout.  Linear: g = sum c B^T (C B u);
 // TL: g_a = sum c P grad N_a with P = F S (the B-bar^T S of the reference).
@@ -629,6 +761,14 @@ constexpr int te_max(int nq, int nf, int ne) {
 template <int FORM, int NIP>
 void launch_K_t(Ctx &c, const double *u) {
   constexpr int TMAX = te_max(NIP * NIP * NIP, Stage<FORM>::NF, elem_lds_floats<FORM>());
+  if constexpr (FORM == FGPU_LINEAR && NIP == 2) {
+    if (c.hex8_te == 1) {  // one element per lane, point data in registers
+      hipLaunchKernelGGL(hex8_K_lane_kernel, dim3((c.nelem + 63) / 64), dim3(64), 0, c.stream,
+                         c.nelem, c.d_coords, c.d_conn, c.d_mat, c.d_Ke, c.kel, c.d_status);
+      FGPU_CHECK(hipGetLastError());
+      return;
+    }
+  }
   const int te = c.hex8_te > 0 && c.hex8_te <= TMAX ? c.hex8_te : TMAX;
   switch (te) {
     case 4: launch_K_te<FORM, NIP, 4>(c, u); break;
@@ -735,6 +875,8 @@ std::vector<int> hex8_te_options(const Ctx &c) {
              : c.form == FGPU_TL_SVK ? te_limit<FGPU_TL_SVK>(c.nip[0])
                                      : te_limit<FGPU_TL_NEOHOOKE>(c.nip[0]);
   std::vector<int> out;
+  // 1: one element per lane with its point data in registers (linear, 2x2x2)
+  if (c.form == FGPU_LINEAR && c.nip[0] == 2 && c.nip[1] == 2 && c.nip[2] == 2) out.push_back(1);
   for (int te = 4; te <= tmax; te *= 2) out.push_back(te);
   return out;
 }

@@ -358,7 +358,7 @@ def test_stiffness_beside_internal_force_identical_bits(fg, kind, overlap):
 @pytest.mark.parametrize("kind", ["hex8_nh", "hex8_linear", "bsp"])
 def test_launch_configurations_identical_bits(fg, oracle_mod, kind):
     """Every launch configuration the autotuner may pick (hex8 elements per
-    workgroup; assembly loads in batches of 1, 4, 8 or 16, with and without the
+    workgroup, or one element per lane; assembly loads in batches of 1, 4, 8 or 16, with and without the
     XCD-contiguous workgroup order) gives the same global matrix, bit for bit,
     on neo-Hookean and linear bricks and on a cubic B-spline patch, and it
     matches the oracle to the FP32 element-matrix tolerance."""
@@ -380,7 +380,7 @@ def test_launch_configurations_identical_bits(fg, oracle_mod, kind):
     assert g.tuning_report()  # the autotuner ran on the first assembly
     ref = g.csr()[2].copy()
     seen = 0
-    for te in ([0, 4, 8, 16, 32, 64] if kind != "bsp" else [0]):
+    for te in ([0, 1, 4, 8, 16, 32, 64] if kind != "bsp" else [0]):
         for mode in range(9):
             try:
                 g.set_launch(te, mode)
