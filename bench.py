@@ -114,10 +114,10 @@ def picked_mode(report):
 def assembly_bytes(m, nbf, nblk):
     """Algorithmic HBM bytes of one assembly: every stored element block read
     once (9 FP32 + its 4-byte contribution code), per upper node block its
-    segment pointer (8 B) and four 4-byte indices (rows, positions), every
-    stored global block written once (9 FP64)."""
+    4-byte segment pointer, its 1-byte direction mask and its two 4-byte
+    positions, every stored global block written once (9 FP64)."""
     nsym = m.nn * (m.nn + 1) // 2
-    return m.nelem * nsym * (36 + 4) + nblk * (8 + 16) + nbf * 72
+    return m.nelem * nsym * (36 + 4) + nblk * (4 + 1 + 8) + nbf * 72
 
 
 def measured_traffic(workload, kernel):

@@ -172,9 +172,9 @@ void build_maps(Ctx &c, const int *conn) {
     throw Error("mesh too large for 32-bit assembly codes");
   (void)col_offset;
   std::vector<int> blkA(c.nblk), blkB(c.nblk), offAB(c.nblk), offBA(c.nblk);
-  std::vector<long long> cptr(c.nblk + 1);
+  std::vector<int> cptr(c.nblk + 1);  // contribution segments (fewer than 2^31 entries)
   std::vector<int> contrib(c.ncontrib);
-  cptr[c.nblk] = c.ncontrib;
+  cptr[c.nblk] = (int)c.ncontrib;
 
   parallel_for(nnode, [&](long Al) {
     int A = (int)Al;
@@ -203,7 +203,7 @@ void build_maps(Ctx &c, const int *conn) {
       blkB[bk] = B;
       offAB[bk] = (int)blk_pos(A, B);
       offBA[bk] = (int)blk_pos(B, A);
-      cptr[bk] = cc;
+      cptr[bk] = (int)cc;
       while (i < items.size() && items[i].first == B) contrib[cc++] = items[i++].second;
       bk++;
     }
@@ -220,6 +220,12 @@ void build_maps(Ctx &c, const int *conn) {
       if (c.eqmap[3 * n + d] >= 0) mk |= (unsigned char)(1 << d);
     fmask[n] = mk;
   }
+  // per upper block: the free directions of A (bits 0-2) and of B (bits
+  // 3-5), and A == B (bit 6) -- what the assembly needs of A and B
+  std::vector<unsigned char> bmask(c.nblk);
+  for (long long k = 0; k < c.nblk; k++)
+    bmask[k] = (unsigned char)(fmask[blkA[k]] | fmask[blkB[k]] << 3 |
+                               (blkA[k] == blkB[k] ? 1 << 6 : 0));
   hipStream_t s = c.stream;
   c.d_rowptr = dupload(rowptr.data(), rowptr.size(), s);  // columns: built on first export
   c.d_bptr = dupload(nbcnt.data(), nbcnt.size(), s);
@@ -228,7 +234,7 @@ void build_maps(Ctx &c, const int *conn) {
   c.d_bdiag = dupload(bdiag.data(), bdiag.size(), s);
   c.d_fmask = dupload(fmask.data(), fmask.size(), s);
   c.d_bval = dalloc<double>((size_t)c.nbf * 9);
-  c.d_blkA = dupload(blkA.data(), blkA.size(), s);
+  c.d_bmask = dupload(bmask.data(), bmask.size(), s);
   c.d_blkB = dupload(blkB.data(), blkB.size(), s);
   c.d_offAB = dupload(offAB.data(), offAB.size(), s);
   c.d_offBA = dupload(offBA.data(), offBA.size(), s);
@@ -331,7 +337,7 @@ void destroy(Ctx *c) {
                   c->d_epos,
                   c->d_status, c->d_rowptr, c->d_col, c->d_bptr, c->d_bcol,
                   c->d_bcoloff, c->d_bdiag, c->d_fmask, c->d_bval, c->d_bn,
-                  c->d_blkA, c->d_blkB, c->d_offAB, c->d_offBA, c->d_cptr,
+                  c->d_bmask, c->d_blkB, c->d_offAB, c->d_offBA, c->d_cptr,
                   c->d_contrib, c->d_nptr, c->d_nadj, c->d_kptr, c->d_rslot, c->d_u,
                   c->d_b, c->d_x,
                   c->d_r, c->d_p, c->d_w, c->d_z, c->d_dinv, c->d_part,

@@ -189,10 +189,11 @@ struct Ctx {
 
   // ---- assembly map: upper node blocks (A <= B)
   long long nblk = 0;
-  int *d_blkA = nullptr, *d_blkB = nullptr;       // [nblk]
+  unsigned char *d_bmask = nullptr;  // [nblk] free directions of A, of B, A == B
+  int *d_blkB = nullptr;             // [nblk] column node of each upper block
   int *d_offAB = nullptr;  // [nblk] position of block (A, B) among the node blocks
   int *d_offBA = nullptr;  // [nblk] position of block (B, A)
-  long long *d_cptr = nullptr;   // [nblk+1] contribution segments
+  int *d_cptr = nullptr;         // [nblk+1] contribution segments
   int *d_contrib = nullptr;      // [ncontrib] (e * nsym + sym) << 1 | transpose
   long long ncontrib = 0;
   // node -> (element, local node) adjacency for the f_int gather

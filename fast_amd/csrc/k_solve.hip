@@ -43,10 +43,9 @@ namespace {
 // kernels compute and keep FP64)
 template <int NB, bool XCD, class T = float>
 __global__ __launch_bounds__(256) void assemble_kernel(
-    long long nblk, const int *__restrict__ blkA, const int *__restrict__ blkB,
-    const int *__restrict__ posAB, const int *__restrict__ posBA,
-    const long long *__restrict__ cptr, const int *__restrict__ contrib,
-    const T *__restrict__ Ke, KeLayout L, const unsigned char *__restrict__ fmask,
+    long long nblk, const unsigned char *__restrict__ bmask, const int *__restrict__ posAB,
+    const int *__restrict__ posBA, const int *__restrict__ cptr,
+    const int *__restrict__ contrib, const T *__restrict__ Ke, KeLayout L,
     double *__restrict__ bval) {
   __shared__ double sv[256 * 9];  // the workgroup's summed blocks, masked
   __shared__ int sAB[256], sBA[256];
@@ -58,8 +57,8 @@ __global__ __launch_bounds__(256) void assemble_kernel(
   const int nk = (int)min((long long)blockDim.x, nblk - k0);  // blocks of this workgroup
   if (nk <= 0) return;  // the whole workgroup
   double s[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
-  const long long c0 = k < nblk ? cptr[k] : 0, c1 = k < nblk ? cptr[k + 1] : 0;
-  for (long long tb = c0; tb < c1; tb += NB) {
+  const int c0 = k < nblk ? cptr[k] : 0, c1 = k < nblk ? cptr[k + 1] : 0;
+  for (int tb = c0; tb < c1; tb += NB) {
     int code[NB];
 #pragma unroll
     for (int u = 0; u < NB; u++) code[u] = tb + u < c1 ? contrib[tb + u] : -1;
@@ -86,15 +85,15 @@ __global__ __launch_bounds__(256) void assemble_kernel(
     }
   }
   if (k < nblk) {
-    const int A = blkA[k], B = blkB[k];
-    const unsigned mA = fmask[A], mB = fmask[B];
+    const unsigned mk = bmask[k], mA = mk & 7u, mB = (mk >> 3) & 7u;
+    const bool diag = (mk >> 6) & 1u;
     for (int i = 0; i < 3; i++)
       for (int j = 0; j < 3; j++) {
         const bool live = ((mA >> i) & 1u) && ((mB >> j) & 1u);
-        sv[9 * tid + 3 * i + j] = live ? s[3 * i + j] : ((A == B && i == j) ? 1.0 : 0.0);
+        sv[9 * tid + 3 * i + j] = live ? s[3 * i + j] : ((diag && i == j) ? 1.0 : 0.0);
       }
     sAB[tid] = posAB[k];
-    sBA[tid] = A != B ? posBA[k] : -1;
+    sBA[tid] = diag ? -1 : posBA[k];
   }
   __syncthreads();
   // write-out value by value (consecutive lanes store consecutive doubles):
@@ -477,12 +476,12 @@ void launch_contrib(Ctx &c) {
   if (XCD) grid = (grid + 7) / 8 * 8;  // the remap is a bijection on multiples of 8
   if (c.d_Ke64)
     hipLaunchKernelGGL((assemble_kernel<NB, XCD, double>), dim3((unsigned)grid), dim3(256), 0,
-                       c.stream, c.nblk, c.d_blkA, c.d_blkB, c.d_offAB, c.d_offBA, c.d_cptr,
-                       c.d_contrib, c.d_Ke64, c.kel, c.d_fmask, c.d_bval);
+                       c.stream, c.nblk, c.d_bmask, c.d_offAB, c.d_offBA, c.d_cptr, c.d_contrib,
+                       c.d_Ke64, c.kel, c.d_bval);
   else
     hipLaunchKernelGGL((assemble_kernel<NB, XCD, float>), dim3((unsigned)grid), dim3(256), 0,
-                       c.stream, c.nblk, c.d_blkA, c.d_blkB, c.d_offAB, c.d_offBA, c.d_cptr,
-                       c.d_contrib, c.d_Ke, c.kel, c.d_fmask, c.d_bval);
+                       c.stream, c.nblk, c.d_bmask, c.d_offAB, c.d_offBA, c.d_cptr, c.d_contrib,
+                       c.d_Ke, c.kel, c.d_bval);
 }
 
 void launch_assemble_K(Ctx &c) {
