@@ -422,7 +422,8 @@ int fgpu_stresses_dev(fgpu_ctx *ctx, const double *u_nodal_dev, double *sig_ip_d
 
 /* Launch configuration.  On the first stiffness assembly of a context the
  * autotuner times every valid configuration of the element-stiffness kernel
- * (hex8: elements per workgroup 4..64) and of the assembly (contribution
+ * (hex8: elements per workgroup 4..64, or 1 = one element per lane with its
+ * point data in registers, linear bricks on 2x2x2 Gauss) and of the assembly (contribution
  * loads in batches of 1, 4, 8 or 16: mode & 3; mode & 4: workgroups ordered
  * so that each of the 8 XCDs sweeps a contiguous range of block rows; mode
  * 8, B-spline solids of at most 64 control points per element: one wave per
