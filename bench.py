@@ -9,7 +9,8 @@ independent hex8 elastic unit cubes, edge n ~ U{4..32}, nu ~ U(0.2, 0.45),
 E = 1, one clamped face and pressure 1 on another, all from seed 0 (10.33 M
 elements, 33.6 M equations), 2x2x2 Gauss, one launch per phase for the whole
 batch.  Beside the headline the same JSON line carries, each measured in this
-run: the 10^3 hex8 linear-static solve (configuration 1), the cubic B-spline
+run: the batch's per-mesh PCG solve to 1e-8, the 10^3 hex8 linear-static
+solve (configuration 1), the cubic B-spline
 32x32x8 patch step, the 64^3 neo-Hookean hex8 step
 and its 20-increment Newton solve, the learned warm start's solve-time ratios
 (plain / secant / polynomial extrapolation of orders 2 and 4 / network), and the relative errors of K.u, f_int and u
@@ -406,6 +407,39 @@ def accuracy_u_newton():
     return float(np.linalg.norm(us[-1] - uo[-1]) / np.linalg.norm(uo[-1]))
 
 
+# ---------------------------------------------------------------- batch solve
+def batch_solve_record(model, m, tol=1e-8):
+    """Configuration 4's solve: every mesh of the batch by its own Jacobi-PCG
+    to |r| / |b| <= tol inside shared launches (fgpu_solve_batch_dev, device
+    vectors), on the FP64 matrix and on its FP32 copy."""
+    import ctypes
+    import torch
+    from fast_amd import fgpu
+    L = fgpu.lib()
+    nb = L.fgpu_nbatch(model.h)
+    b = torch.from_numpy(m.load_vector()).cuda()
+    x = torch.zeros_like(b)
+    out = {"tol": tol, "meshes": int(nb)}
+    for bits in (64, 32):
+        model.set_matrix_precision(bits)
+        it = np.zeros(nb, dtype=np.int32)
+        rr = np.zeros(nb)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        n = L.fgpu_solve_batch_dev(model.h, b.data_ptr(), x.data_ptr(), m.neq, tol,
+                                   it.ctypes.data_as(ctypes.c_void_p),
+                                   rr.ctypes.data_as(ctypes.c_void_p))
+        wall = time.perf_counter() - t0
+        out["fp%d_matrix" % bits] = {
+            "wall_s": round(wall, 4), "meshes_converged": int(n),
+            "pcg_iters_max": int(it.max()), "pcg_iters_mean": round(float(it.mean()), 1),
+            "relres_max": float("%.3e" % rr.max())}
+        log("batch solve, FP%d matrix: %s" % (bits, out["fp%d_matrix" % bits]))
+    model.set_matrix_precision(64)
+    del b, x
+    return out
+
+
 # ---------------------------------------------------------------- step bench
 def run_step(name, steps, warmup, serial=False, with_accuracy=True):
     """K + f_int + assembly of one state, ``steps`` timed passes; returns the
@@ -490,6 +524,8 @@ def run_step(name, steps, warmup, serial=False, with_accuracy=True):
         rec["phase_ms"]["note"] = ("f_int: its span in the step, parts of it beside other "
                                    "kernels on the second stream; f_int_alone: its kernels "
                                    "with the GPU to themselves (the f_int roofline's time)")
+    if name == "batch":
+        rec["solve"] = batch_solve_record(model, m)
     if with_accuracy:
         acc = accuracy_Ku(model, m, u)
         gh = g.cpu().numpy()
