@@ -673,10 +673,12 @@ def config1_record():
 
 
 # ---------------------------------------------------------------- Newton
-def newton_record(variants=("plain_mat32", "secant_mat32", "extrap4_mat32")):
+def newton_record(variants=("plain_mat32", "secant_mat32", "extrap4_mat32", "extrap4_mat32_bj")):
     """The 64^3 neo-Hookean cantilever under load control: 20 increments to
     the tip traction 0.2, Newton to |r| / max(1, |f|) <= 1e-8 with inexact
-    PCG corrections (FP32 element tangents, FP64 f_int and PCG vectors)."""
+    PCG corrections (FP32 element tangents, FP64 f_int and PCG vectors).
+    Suffix _bj: node-block Jacobi PCG instead of the reference's point
+    Jacobi."""
     import torch
     from fast_amd import mesh as M, fgpu
     n = 64
@@ -693,7 +695,8 @@ def newton_record(variants=("plain_mat32", "secant_mat32", "extrap4_mat32")):
         extra = {"secant": True} if name.startswith("secant") else {}
         if name.startswith("extrap"):
             extra = {"extrap": int(name[6])}
-        model.set_matrix_precision(32 if name.endswith("mat32") else 64)
+        model.set_matrix_precision(32 if "mat32" in name else 64)
+        model.set_preconditioner(1 if name.endswith("_bj") else 0)
         torch.cuda.synchronize()
         t0 = time.perf_counter()
         _, st, err = model.newton(q, **kw, **extra)
@@ -713,7 +716,7 @@ def newton_record(variants=("plain_mat32", "secant_mat32", "extrap4_mat32")):
 
 def run_newton(args):
     rec = newton_record(("plain", "secant", "extrap4", "plain_mat32", "secant_mat32",
-                         "extrap4_mat32"))
+                         "extrap4_mat32", "extrap4_mat32_bj"))
     res = rec["runs"]
     out = {
         "metric": "load-control Newton wall time, 20 increments to traction 0.2",

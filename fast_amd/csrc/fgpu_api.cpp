@@ -341,7 +341,7 @@ void destroy(Ctx *c) {
                   c->d_contrib, c->d_nptr, c->d_nadj, c->d_kptr, c->d_rslot, c->d_u,
                   c->d_b, c->d_x,
                   c->d_r, c->d_p, c->d_w, c->d_z, c->d_dinv, c->d_part,
-                  c->d_scal, c->d_done, c->d_eqnode, c->d_TR, c->d_band, c->d_finfo, c->d_bpart,
+                  c->d_scal, c->d_done, c->d_eqnode, c->d_TR, c->d_band, c->d_finfo, c->d_bpart, c->d_binv,
                   c->d_bval32, c->d_gsym, c->d_gG, c->d_gR, c->d_gH, c->d_gPQ,
                   c->d_gnodes};
   for (void *p : bufs)
@@ -1150,6 +1150,15 @@ int fgpu_set_matrix_precision(fgpu_ctx *ctx, int bits) {
   return guarded(ctx, [&](Ctx &c) {
     if (bits != 32 && bits != 64) throw Error("matrix precision: 32 or 64 bits");
     c.mat_bits = bits;
+    return 1;
+  });
+}
+
+int fgpu_set_preconditioner(fgpu_ctx *ctx, int kind) {
+  return guarded(ctx, [&](Ctx &c) {
+    if (kind != 0 && kind != 1) throw Error("preconditioner: 0 (point Jacobi) or 1 (node blocks)");
+    if (kind == 1 && !c.d_binv) c.d_binv = dalloc<double>((size_t)c.nnode * 9);
+    c.precond = kind;
     return 1;
   });
 }

@@ -221,6 +221,8 @@ struct Ctx {
   double *d_b = nullptr, *d_x = nullptr, *d_r = nullptr, *d_p = nullptr,
          *d_w = nullptr, *d_z = nullptr, *d_dinv = nullptr, *d_bn = nullptr;
   double *d_part = nullptr;   // partial sums
+  int precond = 0;            // 0 point Jacobi (the reference's), 1 node-block Jacobi
+  double *d_binv = nullptr;   // [nnode*9] inverses of the diagonal node blocks (precond 1)
   double *d_scal = nullptr;   // device scalars
   int *d_done = nullptr;      // convergence iteration (0 = running)
   int *h_done = nullptr;      // pinned mirror

@@ -70,6 +70,101 @@ __global__ __launch_bounds__(256) void pcg_init_kernel(
   }
 }
 
+// Node-block Jacobi (fgpu_set_preconditioner 1): z_j = B_j^-1 r_j with B_j
+// node j's 3x3 diagonal block, binv = [nnode][9].  One thread per node (its
+// three entries; no thread reads an entry another writes), otherwise as
+// pcg_init_kernel and pcg_update_kernel.
+__device__ inline void bj_apply(const double *__restrict__ binv, long long j, const double r3[3],
+                                double z3[3]) {
+  const double *B = binv + 9 * j;
+  for (int d = 0; d < 3; d++) z3[d] = B[3 * d] * r3[0] + B[3 * d + 1] * r3[1] + B[3 * d + 2] * r3[2];
+}
+
+__global__ __launch_bounds__(256) void pcg_init_bj_kernel(
+    int nnode, const double *__restrict__ b, double *__restrict__ r,
+    const double *__restrict__ binv, double *__restrict__ z, double *__restrict__ p,
+    double *__restrict__ part) {
+  __shared__ double sh[16];
+  double zr = 0.0, bb = 0.0, rr = 0.0;
+  for (long long j = (long long)blockIdx.x * blockDim.x + threadIdx.x; j < nnode;
+       j += (long long)gridDim.x * blockDim.x) {
+    double r3[3], z3[3];
+    for (int d = 0; d < 3; d++) r3[d] = r[3 * j + d];
+    bj_apply(binv, j, r3, z3);
+    for (int d = 0; d < 3; d++) {
+      z[3 * j + d] = z3[d];
+      p[3 * j + d] = z3[d];
+      zr += z3[d] * r3[d];
+      bb += b[3 * j + d] * b[3 * j + d];
+      rr += r3[d] * r3[d];
+    }
+  }
+  double t0 = block_sum(zr, sh);
+  double t1 = block_sum(bb, sh);
+  double t2 = block_sum(rr, sh);
+  if (threadIdx.x == 0) {
+    part[blockIdx.x] = t0;
+    part[RED + blockIdx.x] = t1;
+    part[2 * RED + blockIdx.x] = t2;
+  }
+}
+
+__global__ __launch_bounds__(256) void pcg_update_bj_kernel(
+    int nnode, double *__restrict__ x, double *__restrict__ r, double *__restrict__ z,
+    const double *__restrict__ p, const double *__restrict__ w, const double *__restrict__ binv,
+    double *__restrict__ part, const double *__restrict__ scal, const int *__restrict__ done) {
+  __shared__ double sh[16];
+  if (done[0]) return;
+  const int it = done[2];
+  const double pw = reduce_partials(part, RED, sh);
+  const double alpha = scal[1 + ((it - 1) & 1)] / pw;
+  double zr = 0.0, rr = 0.0;
+  for (long long j = (long long)blockIdx.x * blockDim.x + threadIdx.x; j < nnode;
+       j += (long long)gridDim.x * blockDim.x) {
+    double r3[3], z3[3];
+    for (int d = 0; d < 3; d++) {
+      const long long i = 3 * j + d;
+      x[i] = x[i] + alpha * p[i];
+      r3[d] = r[i] + (-alpha) * w[i];
+      r[i] = r3[d];
+    }
+    bj_apply(binv, j, r3, z3);
+    for (int d = 0; d < 3; d++) {
+      z[3 * j + d] = z3[d];
+      zr += z3[d] * r3[d];
+      rr += r3[d] * r3[d];
+    }
+  }
+  double t0 = block_sum(zr, sh);
+  double t1 = block_sum(rr, sh);
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    part[RED + blockIdx.x] = t0;
+    part[2 * RED + blockIdx.x] = t1;
+  }
+}
+
+// inverse of each node's 3x3 diagonal block, FP64 cofactors
+__global__ void block_inv_kernel(int nnode, const long long *__restrict__ bdiag,
+                                 const double *__restrict__ bval, double *__restrict__ binv) {
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= nnode) return;
+  const double *A = bval + 9 * bdiag[j];
+  const double c00 = A[4] * A[8] - A[5] * A[7], c01 = A[5] * A[6] - A[3] * A[8],
+               c02 = A[3] * A[7] - A[4] * A[6];
+  const double id = 1.0 / (A[0] * c00 + A[1] * c01 + A[2] * c02);
+  double *I = binv + 9 * (size_t)j;
+  I[0] = c00 * id;
+  I[1] = (A[2] * A[7] - A[1] * A[8]) * id;
+  I[2] = (A[1] * A[5] - A[2] * A[4]) * id;
+  I[3] = c01 * id;
+  I[4] = (A[0] * A[8] - A[2] * A[6]) * id;
+  I[5] = (A[2] * A[3] - A[0] * A[5]) * id;
+  I[6] = c02 * id;
+  I[7] = (A[1] * A[6] - A[0] * A[7]) * id;
+  I[8] = (A[0] * A[4] - A[1] * A[3]) * id;
+}
+
 __global__ void pcg_start_kernel(const double *__restrict__ part, double *__restrict__ scal,
                                  int *__restrict__ done, double tol) {
   __shared__ double sh[16];
@@ -204,8 +299,12 @@ void enqueue_iteration(Ctx &c) {
     case 8: pcg_spmv_launch<8>(c); break;
     default: pcg_spmv_launch<4>(c); break;
   }
-  hipLaunchKernelGGL(pcg_update_kernel, dim3(RED), dim3(TPB), 0, c.stream, n, c.d_x, c.d_r,
-                     c.d_z, c.d_p, c.d_w, c.d_dinv, c.d_part, c.d_scal, c.d_done);
+  if (c.precond == 1)
+    hipLaunchKernelGGL(pcg_update_bj_kernel, dim3(RED), dim3(TPB), 0, c.stream, c.nnode, c.d_x,
+                       c.d_r, c.d_z, c.d_p, c.d_w, c.d_binv, c.d_part, c.d_scal, c.d_done);
+  else
+    hipLaunchKernelGGL(pcg_update_kernel, dim3(RED), dim3(TPB), 0, c.stream, n, c.d_x, c.d_r,
+                       c.d_z, c.d_p, c.d_w, c.d_dinv, c.d_part, c.d_scal, c.d_done);
   hipLaunchKernelGGL(pcg_dir_kernel, dim3(RED), dim3(TPB), 0, c.stream, n, c.d_p, c.d_z,
                      c.d_part, c.d_scal, c.d_done);
 }
@@ -407,12 +506,13 @@ void run_chunk(Ctx &c, bool batch) {
     return;
   }
   int &bits = batch ? c.g_batch_bits : c.g_single_bits;
-  if (ge && bits != c.mat_bits) {  // captured for the other matrix precision
+  const int key = c.mat_bits + (batch ? 0 : 1000 * c.precond);
+  if (ge && bits != key) {  // captured for the other matrix precision or preconditioner
     FGPU_CHECK(hipGraphExecDestroy(ge));
     ge = nullptr;
   }
   if (!ge) {
-    bits = c.mat_bits;
+    bits = key;
     hipGraph_t g;
     FGPU_CHECK(hipStreamBeginCapture(c.stream, hipStreamCaptureModeThreadLocal));
     for (int k = 0; k < CHUNK_IT; k++) batch ? enqueue_batch_iteration(c) : enqueue_iteration(c);
@@ -430,6 +530,9 @@ void prepare(Ctx &c, const double *b, const double *x0) {
   if (c.mat_bits == 32) ensure_bval32(c);
   hipLaunchKernelGGL(diag_inv_kernel, dim3(eb), dim3(256), 0, s, c.nnode, c.d_bdiag, c.d_bval,
                      c.d_dinv);
+  if (c.precond == 1)
+    hipLaunchKernelGGL(block_inv_kernel, dim3((c.nnode + 255) / 256), dim3(256), 0, s, c.nnode,
+                       c.d_bdiag, c.d_bval, c.d_binv);
   launch_eq_to_nodal(c, b, c.d_bn);
   if (x0) {
     launch_eq_to_nodal(c, x0, c.d_x);
@@ -440,6 +543,15 @@ void prepare(Ctx &c, const double *b, const double *x0) {
     FGPU_CHECK(hipMemcpyAsync(c.d_r, c.d_bn, sizeof(double) * n, hipMemcpyDeviceToDevice, s));
   }
   FGPU_CHECK(hipGetLastError());
+}
+
+void launch_pcg_init(Ctx &c) {
+  if (c.precond == 1)
+    hipLaunchKernelGGL(pcg_init_bj_kernel, dim3(RED), dim3(TPB), 0, c.stream, c.nnode, c.d_bn,
+                       c.d_r, c.d_binv, c.d_z, c.d_p, c.d_part);
+  else
+    hipLaunchKernelGGL(pcg_init_kernel, dim3(RED), dim3(TPB), 0, c.stream, 3 * c.nnode, c.d_bn,
+                       c.d_r, c.d_dinv, c.d_z, c.d_p, c.d_part);
 }
 
 }  // namespace
@@ -453,11 +565,13 @@ void enqueue_pcg_start(Ctx &c, const double *b, const double *d_tol, int abs_exi
   if (c.mat_bits == 32) ensure_bval32(c);
   hipLaunchKernelGGL(diag_inv_kernel, dim3(eb), dim3(256), 0, s, c.nnode, c.d_bdiag, c.d_bval,
                      c.d_dinv);
+  if (c.precond == 1)
+    hipLaunchKernelGGL(block_inv_kernel, dim3((c.nnode + 255) / 256), dim3(256), 0, s, c.nnode,
+                       c.d_bdiag, c.d_bval, c.d_binv);
   launch_eq_to_nodal(c, b, c.d_bn);
   launch_fill(c, c.d_x, n, 0.0);
   launch_copy(c, c.d_bn, c.d_r, n);
-  hipLaunchKernelGGL(pcg_init_kernel, dim3(RED), dim3(TPB), 0, s, n, c.d_bn, c.d_r, c.d_dinv,
-                     c.d_z, c.d_p, c.d_part);
+  launch_pcg_init(c);
   hipLaunchKernelGGL(pcg_start_dev_kernel, dim3(1), dim3(TPB), 0, s, c.d_part, c.d_scal,
                      c.d_done, d_tol, abs_exit);
   FGPU_CHECK(hipGetLastError());
@@ -504,8 +618,7 @@ int run_pcg(Ctx &c, const double *b, double *x, const double *x0, int maxit, dou
   if (relres) *relres = 0.0;
   if (c.neq == 0) return 1;
   prepare(c, b, x0);
-  hipLaunchKernelGGL(pcg_init_kernel, dim3(RED), dim3(TPB), 0, s, 3 * c.nnode, c.d_bn, c.d_r,
-                     c.d_dinv, c.d_z, c.d_p, c.d_part);
+  launch_pcg_init(c);
   hipLaunchKernelGGL(pcg_start_kernel, dim3(1), dim3(TPB), 0, s, c.d_part, c.d_scal, c.d_done,
                      rtol);
   FGPU_CHECK(hipGetLastError());

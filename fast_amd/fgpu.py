@@ -96,6 +96,7 @@ EXPORTS = {
     "fgpu_assemble_K_fint": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p]),
     "fgpu_assemble_K_fint_dev": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p]),
     "fgpu_set_matrix_precision": (C.c_int, [C.c_void_p, C.c_int]),
+    "fgpu_set_preconditioner": (C.c_int, [C.c_void_p, C.c_int]),
     "fgpu_factor": (C.c_int, [C.c_void_p]),
     "fgpu_linear_static_direct": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_double,
                                             C.c_void_p, C.c_void_p]),
@@ -364,6 +365,11 @@ class GpuModel:
         """32: the PCG multiplies by an FP32 copy of the matrix; 64: FP64."""
         self._check(lib().fgpu_set_matrix_precision(self.h, int(bits)),
                     "fgpu_set_matrix_precision")
+
+    def set_preconditioner(self, kind):
+        """PCG preconditioner: 0 point Jacobi (the reference's), 1 the inverse
+        of each node's 3x3 diagonal block."""
+        self._check(lib().fgpu_set_preconditioner(self.h, int(kind)), "fgpu_set_preconditioner")
 
     def direct_solve(self, b):
         """Solve K x = b with the device LDL^T of the current matrix (factorized

@@ -227,3 +227,23 @@ def test_extrapolation_predictors_save_work(fg):
         assert nrel(runs[name][0], runs["plain"][0]) <= 1e-6
     assert runs["extrap4"][2] < runs["secant"][2] < runs["plain"][2]
     assert runs["extrap4"][1] <= runs["secant"][1] <= runs["plain"][1]
+
+
+def test_block_jacobi_newton(fg):
+    """Node-block Jacobi PCG inside the load-control Newton driver: the same
+    converged states as point Jacobi (the reference's preconditioner) with
+    fewer PCG iterations on the 16^3 neo-Hookean block."""
+    m = cube(16)
+    model = fg.GpuModel(m)
+    q = m.load_vector()
+    kw = dict(steps=6, factor=0.03, maxit=30, tol=1e-8, pcg_tol=1.0, adaptive_pcg=2, extrap=4)
+    out = {}
+    for kind in (0, 1):
+        model.set_preconditioner(kind)
+        u, st, err = model.newton(q, **kw)
+        assert err == "", (kind, err)
+        out[kind] = (u[-1], sum(s["iters"] for s in st), sum(s["pcg_iters"] for s in st))
+        print("preconditioner", kind, out[kind][1:])
+    assert nrel(out[1][0], out[0][0]) <= 1e-6
+    assert out[1][2] < out[0][2]
+    assert out[1][1] <= out[0][1] + 2
