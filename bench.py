@@ -411,7 +411,8 @@ def accuracy_u_newton():
 def batch_solve_record(model, m, tol=1e-8):
     """Configuration 4's solve: every mesh of the batch by its own Jacobi-PCG
     to |r| / |b| <= tol inside shared launches (fgpu_solve_batch_dev, device
-    vectors), on the FP64 matrix and on its FP32 copy."""
+    vectors), on the FP64 matrix and on its FP32 copy, then on the FP32 copy
+    with node-block Jacobi."""
     import ctypes
     import torch
     from fast_amd import fgpu
@@ -420,8 +421,9 @@ def batch_solve_record(model, m, tol=1e-8):
     b = torch.from_numpy(m.load_vector()).cuda()
     x = torch.zeros_like(b)
     out = {"tol": tol, "meshes": int(nb)}
-    for bits in (64, 32):
+    for bits, pc in ((64, 0), (32, 0), (32, 1)):
         model.set_matrix_precision(bits)
+        model.set_preconditioner(pc)
         it = np.zeros(nb, dtype=np.int32)
         rr = np.zeros(nb)
         torch.cuda.synchronize()
@@ -430,12 +432,14 @@ def batch_solve_record(model, m, tol=1e-8):
                                    it.ctypes.data_as(ctypes.c_void_p),
                                    rr.ctypes.data_as(ctypes.c_void_p))
         wall = time.perf_counter() - t0
-        out["fp%d_matrix" % bits] = {
+        key = "fp%d_matrix" % bits + ("_node_blocks" if pc else "")
+        out[key] = {
             "wall_s": round(wall, 4), "meshes_converged": int(n),
             "pcg_iters_max": int(it.max()), "pcg_iters_mean": round(float(it.mean()), 1),
             "relres_max": float("%.3e" % rr.max())}
-        log("batch solve, FP%d matrix: %s" % (bits, out["fp%d_matrix" % bits]))
+        log("batch solve, %s: %s" % (key, out[key]))
     model.set_matrix_precision(64)
+    model.set_preconditioner(0)
     del b, x
     return out
 

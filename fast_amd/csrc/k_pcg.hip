@@ -392,6 +392,74 @@ __global__ void bpcg_count_kernel(int nmesh, const int *__restrict__ mdone, int 
   *ndone = k;
 }
 
+// Node-block Jacobi for the batch (fgpu_set_preconditioner 1): one thread
+// per node of the chunk, otherwise as bpcg_init_kernel / bpcg_update_kernel.
+__global__ __launch_bounds__(256) void bpcg_init_bj_kernel(
+    const long long *__restrict__ chn0, const long long *__restrict__ chn1,
+    const double *__restrict__ b, double *__restrict__ r, const double *__restrict__ binv,
+    double *__restrict__ z, double *__restrict__ p, double *__restrict__ part, int nch) {
+  __shared__ double sh[16];
+  const int ch = blockIdx.x;
+  double zr = 0.0, bb = 0.0, rr = 0.0;
+  for (long long j = chn0[ch] + threadIdx.x; j < chn1[ch]; j += blockDim.x) {
+    double r3[3], z3[3];
+    for (int d = 0; d < 3; d++) r3[d] = r[3 * j + d];
+    bj_apply(binv, j, r3, z3);
+    for (int d = 0; d < 3; d++) {
+      z[3 * j + d] = z3[d];
+      p[3 * j + d] = z3[d];
+      zr += z3[d] * r3[d];
+      bb += b[3 * j + d] * b[3 * j + d];
+      rr += r3[d] * r3[d];
+    }
+  }
+  double t0 = block_sum(zr, sh);
+  double t1 = block_sum(bb, sh);
+  double t2 = block_sum(rr, sh);
+  if (threadIdx.x == 0) {
+    part[ch] = t0;
+    part[nch + ch] = t1;
+    part[2 * nch + ch] = t2;
+  }
+}
+
+__global__ __launch_bounds__(256) void bpcg_update_bj_kernel(
+    const long long *__restrict__ chn0, const long long *__restrict__ chn1,
+    const int *__restrict__ chmesh, const int *__restrict__ meshch, double *__restrict__ x,
+    double *__restrict__ r, double *__restrict__ z, const double *__restrict__ p,
+    const double *__restrict__ w, const double *__restrict__ binv, double *__restrict__ part,
+    int nch, const double *__restrict__ mscal, const int *__restrict__ mdone) {
+  __shared__ double sh[16];
+  const int ch = blockIdx.x, m = chmesh[ch];
+  const int *dn = mdone + 4 * m;
+  if (dn[0]) return;
+  const int it = dn[2];
+  const double pw = mesh_sum(part, meshch[m], meshch[m + 1]);
+  const double alpha = mscal[8 * m + 1 + ((it - 1) & 1)] / pw;
+  double zr = 0.0, rr = 0.0;
+  for (long long j = chn0[ch] + threadIdx.x; j < chn1[ch]; j += blockDim.x) {
+    double r3[3], z3[3];
+    for (int d = 0; d < 3; d++) {
+      const long long i = 3 * j + d;
+      x[i] = x[i] + alpha * p[i];
+      r3[d] = r[i] + (-alpha) * w[i];
+      r[i] = r3[d];
+    }
+    bj_apply(binv, j, r3, z3);
+    for (int d = 0; d < 3; d++) {
+      z[3 * j + d] = z3[d];
+      zr += z3[d] * r3[d];
+      rr += r3[d] * r3[d];
+    }
+  }
+  double t0 = block_sum(zr, sh);
+  double t1 = block_sum(rr, sh);
+  if (threadIdx.x == 0) {
+    part[nch + ch] = t0;
+    part[2 * nch + ch] = t1;
+  }
+}
+
 __global__ __launch_bounds__(256) void bpcg_update_kernel(
     const long long *__restrict__ chn0, const long long *__restrict__ chn1,
     const int *__restrict__ chmesh, const int *__restrict__ meshch, double *__restrict__ x,
@@ -487,9 +555,14 @@ void enqueue_batch_iteration(Ctx &c) {
     case 8: bpcg_spmv_launch<8>(c); break;
     default: bpcg_spmv_launch<4>(c); break;
   }
-  hipLaunchKernelGGL(bpcg_update_kernel, dim3(c.nchunk), dim3(TPB), 0, c.stream, c.d_chn0,
-                     c.d_chn1, c.d_chmesh, c.d_meshch, c.d_x, c.d_r, c.d_z, c.d_p, c.d_w,
-                     c.d_dinv, c.d_part, c.nchunk, c.d_mscal, c.d_mdone);
+  if (c.precond == 1)
+    hipLaunchKernelGGL(bpcg_update_bj_kernel, dim3(c.nchunk), dim3(TPB), 0, c.stream, c.d_chn0,
+                       c.d_chn1, c.d_chmesh, c.d_meshch, c.d_x, c.d_r, c.d_z, c.d_p, c.d_w,
+                       c.d_binv, c.d_part, c.nchunk, c.d_mscal, c.d_mdone);
+  else
+    hipLaunchKernelGGL(bpcg_update_kernel, dim3(c.nchunk), dim3(TPB), 0, c.stream, c.d_chn0,
+                       c.d_chn1, c.d_chmesh, c.d_meshch, c.d_x, c.d_r, c.d_z, c.d_p, c.d_w,
+                       c.d_dinv, c.d_part, c.nchunk, c.d_mscal, c.d_mdone);
   hipLaunchKernelGGL(bpcg_dir_kernel, dim3(c.nchunk), dim3(TPB), 0, c.stream, c.d_chn0,
                      c.d_chn1, c.d_chmesh, c.d_meshch, c.d_p, c.d_z, c.d_part, c.nchunk,
                      c.d_mscal, c.d_mdone);
@@ -506,7 +579,7 @@ void run_chunk(Ctx &c, bool batch) {
     return;
   }
   int &bits = batch ? c.g_batch_bits : c.g_single_bits;
-  const int key = c.mat_bits + (batch ? 0 : 1000 * c.precond);
+  const int key = c.mat_bits + 1000 * c.precond;
   if (ge && bits != key) {  // captured for the other matrix precision or preconditioner
     FGPU_CHECK(hipGraphExecDestroy(ge));
     ge = nullptr;
@@ -657,8 +730,12 @@ int run_pcg_batch(Ctx &c, const double *b, double *x, int maxit, double rtol, in
   hipStream_t s = c.stream;
   if (c.nbatch <= 0) throw Error("not a batch context");
   prepare(c, b, nullptr);
-  hipLaunchKernelGGL(bpcg_init_kernel, dim3(c.nchunk), dim3(TPB), 0, s, c.d_chn0, c.d_chn1,
-                     c.d_bn, c.d_r, c.d_dinv, c.d_z, c.d_p, c.d_part, c.nchunk);
+  if (c.precond == 1)
+    hipLaunchKernelGGL(bpcg_init_bj_kernel, dim3(c.nchunk), dim3(TPB), 0, s, c.d_chn0, c.d_chn1,
+                       c.d_bn, c.d_r, c.d_binv, c.d_z, c.d_p, c.d_part, c.nchunk);
+  else
+    hipLaunchKernelGGL(bpcg_init_kernel, dim3(c.nchunk), dim3(TPB), 0, s, c.d_chn0, c.d_chn1,
+                       c.d_bn, c.d_r, c.d_dinv, c.d_z, c.d_p, c.d_part, c.nchunk);
   hipLaunchKernelGGL(bpcg_start_kernel, dim3((c.nbatch + 255) / 256), dim3(256), 0, s, c.nbatch,
                      c.d_meshch, c.d_part, c.nchunk, c.d_mscal, c.d_mdone, c.d_ndone, rtol);
   hipLaunchKernelGGL(bpcg_count_kernel, dim3(1), dim3(64), 0, s, c.nbatch, c.d_mdone, c.d_ndone);

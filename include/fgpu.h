@@ -145,14 +145,13 @@ int fgpu_solve_dev(fgpu_ctx *ctx, const double *b_dev, double *x_dev,
  * iteration's matrix-vector product.  No counterpart in the reference. */
 int fgpu_set_matrix_precision(fgpu_ctx *ctx, int bits);
 
-/* Preconditioner of every single-mesh PCG of the context (fgpu_solve, the
- * drivers): 0 (the default) point Jacobi, z_i = r_i / K_ii, the reference's
+/* Preconditioner of every PCG of the context (fgpu_solve, the drivers, the
+ * batch solve): 0 (the default) point Jacobi, z_i = r_i / K_ii, the reference's
  * (cSysMatrix PCG, mlib/sysmat.cpp); 1 the inverse of each node's 3x3
  * diagonal block, z_j = K_jj^-1 r_j (the three directions of a node
  * coupled; fixed directions keep their unit rows).  Same stopping rule; the
  * iterates, and so the iteration counts, differ from the reference's with 1.
- * The batch solve (fgpu_solve_batch) keeps point Jacobi.  No counterpart in
- * the reference for 1. */
+ * No counterpart in the reference for 1. */
 int fgpu_set_preconditioner(fgpu_ctx *ctx, int kind);
 
 /* Direct solve of the assembled global matrix, the reference's skyline

@@ -179,6 +179,33 @@ def test_pcg_same_iterations_as_reference_restatement(fg, oracle_mod):
     assert conv2 and it2 <= 1
 
 
+def test_block_jacobi_batch_solve(fg):
+    """The batch solve with node-block Jacobi: every mesh converges to the
+    point-Jacobi solution within the tolerance, in no more iterations in
+    total, and each mesh matches its own single-mesh block-Jacobi solve."""
+    m, prm = M.cube_batch(7, nmin=2, nmax=8, seed=4)
+    g = fg.GpuModel(m)
+    g.stiffness()
+    f = m.load_vector()
+    x0, it0, _, n0 = g.solve_batch(f, tol=1e-10)
+    g.set_preconditioner(1)
+    x1, it1, _, n1 = g.solve_batch(f, tol=1e-10)
+    print("batch PCG iterations: point", it0.sum(), "node blocks", it1.sum())
+    assert n0 == 7 and n1 == 7
+    assert it1.sum() < it0.sum()
+    assert nrel(x1, x0) <= 1e-8
+    bp = m.batch_nodeptr
+    for k in (0, 6):
+        mk = M.hex8_cube(int(prm["n"][k]), nu=float(prm["nu"][k]),
+                         clamp_face=int(prm["clamp_face"][k]), load_face=int(prm["load_face"][k]))
+        gk = fg.GpuModel(mk)
+        gk.stiffness()
+        gk.set_preconditioner(1)
+        xk, itk, _, conv = gk.solve(mk.load_vector(), tol=1e-10)
+        assert conv and abs(int(it1[k]) - itk) <= 1, (k, it1[k], itk)
+        assert nrel(m.scatter(x1)[bp[k]:bp[k + 1]], mk.scatter(xk)) <= 1e-9
+
+
 def test_block_jacobi_pcg(fg, oracle_mod):
     """Node-block Jacobi (fgpu_set_preconditioner 1) solves the oracle's
     system to the same tolerance as point Jacobi in fewer iterations, eager
